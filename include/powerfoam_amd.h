@@ -1,0 +1,183 @@
+/*
+ * powerfoam_amd.h -- C-ABI of the MI355X-native Power Foam renderer hot path.
+ *
+ * Plain C: raw pointers, sizes and an opaque stream handle. No HIP, torch or
+ * C++ types appear here, so a gcc-compiled client (ctypes, cgo, JNI) can bind
+ * it. Every device pointer is a HIP device address; `stream` is a hipStream_t
+ * passed as void* (NULL = the default stream). No entry point allocates or
+ * frees device memory: callers own every buffer, and scratch space is passed
+ * in explicitly with its size given by the matching *_workspace_size query.
+ *
+ * Every entry point returns PF_OK (0) or a negative pf_status; a readable
+ * message for the last failure on the calling thread is in pf_last_error().
+ *
+ * Conventions shared with the reference (theialab/powerfoam):
+ *   - screen tiles are 8x8 pixels, one tile = 64 rays (rasterize.py:10-12);
+ *   - tile index is column-major: tile = tile_i + tile_j * tiles_h, and the
+ *     pixel inside a tile is column-major too: i = lane % 8, j = lane / 8
+ *     (rasterize.py:919-938);
+ *   - tiles_h = 1 + (H-1)/8, tiles_w = 1 + (W-1)/8.
+ */
+#ifndef POWERFOAM_AMD_H
+#define POWERFOAM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PF_TILE_WIDTH 8
+#define PF_TILE_SIZE 64
+#define PF_MAX_TEXELS 8 /* texture.py:7 max_sites */
+
+typedef enum pf_status {
+  PF_OK = 0,
+  PF_ERR_ARG = -1,         /* a null/inconsistent argument */
+  PF_ERR_OVERFLOW = -2,    /* an output or workspace is too small */
+  PF_ERR_HIP = -3,         /* a HIP runtime call failed */
+  PF_ERR_UNSUPPORTED = -4  /* a mode the build does not provide */
+} pf_status;
+
+/* Camera: the reference's WarpCamera (camera.py:8-16). `ray_maps` is an
+ * [H,W,6] float array (origin xyz, direction xyz) for non-pinhole cameras, or
+ * NULL for a pinhole camera whose rays come from eye/right/up
+ * (camera.py:19-24). */
+typedef struct pf_camera {
+  float eye[3];
+  float right[3];
+  float up[3];
+  int32_t width;
+  int32_t height;
+  const float* ray_maps;
+  float fov_cos_cutoff;
+} pf_camera;
+
+/* Per-site scene records (rasterize.py:1619-1624 packing). */
+typedef struct pf_scene {
+  int32_t n_sites;                  /* N */
+  int32_t n_texels;                 /* T, 1..8 */
+  const float* spheres;             /* [N,4]  centre xyz, radius */
+  const float* nsigmas;             /* [N,4]  normal xyz, density */
+  const float* texel_sites;         /* [N,T,3] world-space texel sites */
+  const float* texel_rgbh;          /* [N,T,4] texel rgb, height */
+  const int32_t* adjacency;         /* [E]   neighbour site ids */
+  const int32_t* adjacency_offsets; /* [N+1] CSR offsets into adjacency */
+  const uint16_t* adjacency_diff;   /* [E,4] IEEE binary16 (c_adj - c, r_adj) */
+} pf_scene;
+
+/* Depth-sorted per-tile site lists produced by binning. */
+typedef struct pf_tile_lists {
+  int32_t n_tiles;
+  const int64_t* offsets; /* [n_tiles+1] */
+  const int32_t* sites;   /* [M] */
+} pf_tile_lists;
+
+typedef struct pf_raster_fwd_args {
+  pf_camera cam;
+  pf_scene scene;
+  pf_tile_lists tiles;
+  const float* depth_quantiles; /* [H,W,Q] transmittance levels, descending, or NULL */
+  int32_t n_quantiles;          /* Q (0 when depth_quantiles is NULL) */
+  const float* ray_gt;          /* [H,W,3] or NULL */
+  float transmittance_threshold;
+  int32_t return_point_err;
+  int32_t loss_accumulation;    /* render_objective == "surface" */
+  /* outputs (all written by the call; contrib/point_err must be zeroed and
+   * quantile_depths set to -1 by the caller, as the reference does) */
+  int32_t* tile_early_stop; /* [n_tiles] */
+  float* color;             /* [H,W,3] */
+  float* log_t;             /* [H,W]   */
+  float* normal_distance;   /* [H,W]   */
+  float* normal;            /* [H,W,3] */
+  float* quantile_depths;   /* [H,W,Q] or NULL */
+  float* err;               /* [H,W] or NULL (loss_accumulation only) */
+  float* contrib;           /* [N] accumulated */
+  float* point_err;         /* [N] accumulated, or NULL */
+} pf_raster_fwd_args;
+
+typedef struct pf_raster_bwd_args {
+  pf_camera cam;
+  pf_scene scene;
+  pf_tile_lists tiles;
+  const float* depth_quantiles; /* [H,W,Q] or NULL */
+  int32_t n_quantiles;
+  const int32_t* tile_early_stop;      /* [n_tiles] from the forward */
+  const float* log_t;                  /* [H,W] final log-transmittance */
+  const float* grad_color;             /* [H,W,3] */
+  const float* grad_log_t;             /* [H,W] = -grad_opacity * exp(log_t) */
+  const float* grad_normal_distance;   /* [H,W] */
+  const float* grad_normal;            /* [H,W,3] */
+  const float* grad_quantile_depths;   /* [H,W,Q] or NULL */
+  const float* grad_contrib;           /* [N] */
+  /* 0: the reference's backward, term for term (rasterize.py:1150-1514).
+   * 1: the exact derivative of the forward: no 1e-6 regularisers, no texel
+   *    weight cut-off, the far-end surface clip routed to the plane only, and
+   *    the normal_distance term differentiated as (n.d)^2. Used by the
+   *    finite-difference checks. */
+  int32_t exact_gradients;
+  /* outputs, accumulated (zero them first) */
+  float* grad_spheres;      /* [N,4] */
+  float* grad_nsigmas;      /* [N,4] */
+  float* grad_texel_sites;  /* [N,T,3] */
+  float* grad_texel_rgb;    /* [N,T,3] */
+  float* grad_texel_height; /* [N,T] */
+} pf_raster_bwd_args;
+
+/* ---- library ---------------------------------------------------------- */
+int pf_version(void);
+const char* pf_last_error(void);
+
+/* ---- binning (replaces count_visible_pinhole_kernel rasterize.py:140-200,
+ *      torch.cumsum + .item() rasterize.py:1713-1714,
+ *      write_visible_pinhole_kernel rasterize.py:293-363,
+ *      torch.argsort + gather rasterize.py:1761-1768) --------------------- */
+
+/* Zeroes tile_counts[0..n_tiles] and counts, per tile, the sites whose
+ * projected footprint passes the tile test; tile t's count lands in
+ * tile_counts[t+1] so an inclusive scan gives offsets directly. */
+int pf_bin_count(const pf_camera* cam, const float* spheres, int32_t n_sites,
+                 int32_t* tile_counts, void* stream);
+
+/* offsets[k] = sum(tile_counts[0..k]) as int64 (torch.cumsum of the counts). */
+int pf_bin_scan(const int32_t* tile_counts, int32_t n_tiles, int64_t* offsets,
+                void* stream);
+
+/* Writes one 64-bit record per (tile, site) pair into the tile's segment:
+ * (bits(|c-eye|^2 - r^2) << 32) | site. `cursor` is an int32 [n_tiles]
+ * scratch array the call zeroes. Sets visible[site] = 1 for every emitted
+ * site (the reference's prim_visible_mask, rasterize.py:1761-1765); the
+ * caller zeroes `visible` first. */
+int pf_bin_emit(const pf_camera* cam, const float* spheres, int32_t n_sites,
+                const int64_t* offsets, int32_t n_tiles, uint64_t* records,
+                int32_t* cursor, uint8_t* visible, void* stream);
+
+/* Bytes of scratch pf_bin_sort needs for these tile segments. */
+size_t pf_bin_sort_workspace_size(int64_t n_records, int32_t n_tiles);
+
+/* Sorts every tile segment ascending by record (depth key, then site id) and
+ * writes the site ids to `sites` [M]. Deterministic. */
+int pf_bin_sort(const int64_t* offsets, int32_t n_tiles, uint64_t* records,
+                int32_t* sites, void* workspace, size_t workspace_bytes,
+                void* stream);
+
+/* ---- adjacency encoding (prefetch_adjacency_kernel rasterize.py:580-606) */
+/* diff[e] = binary16(c_adj - c, r_adj) for every edge of every site. */
+int pf_adj_encode(const float* spheres, int32_t n_sites,
+                  const int32_t* adjacency, const int32_t* adjacency_offsets,
+                  uint16_t* diff, void* stream);
+
+/* ---- compositing (forward_kernel rasterize.py:892-1146,
+ *      backward_kernel rasterize.py:1150-1514) --------------------------- */
+int pf_raster_fwd(const pf_raster_fwd_args* args, void* stream);
+int pf_raster_bwd(const pf_raster_bwd_args* args, void* stream);
+
+/* Replaces non-finite entries with 0 (rasterize.py:1954-1958). */
+int pf_scrub_nonfinite(float* data, int64_t n, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* POWERFOAM_AMD_H */

@@ -1,0 +1,235 @@
+"""ORACLE -- test infrastructure only. ctypes wrapper of oracle/build/liborcl.so,
+the CPU restatement of the reference renderer hot path (see pf_oracle.c).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this
+module, and only as the checker; the product never imports it.
+"""
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from powerfoam_amd._abi import PfCamera, PfRasterBwdArgs, PfRasterFwdArgs, PfScene, PfTileLists, tiles_hw
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(_HERE, "build", "liborcl.so")
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        lib = ctypes.CDLL(LIB)
+        vp = ctypes.c_void_p
+        lib.orc_bin_count.argtypes = [vp, vp, ctypes.c_int32, vp]
+        lib.orc_bin_count.restype = ctypes.c_int64
+        lib.orc_bin_lists.argtypes = [vp, vp, ctypes.c_int32, vp, vp, vp]
+        lib.orc_bin_lists.restype = None
+        lib.orc_site_footprint_dbg.argtypes = [vp, vp, vp, vp]
+        lib.orc_site_footprint_dbg.restype = ctypes.c_int
+        lib.orc_tile_obb_test_dbg.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp]
+        lib.orc_tile_obb_test_dbg.restype = ctypes.c_int
+        lib.orc_adj_encode.argtypes = [vp, ctypes.c_int32, vp, vp, vp]
+        lib.orc_adj_encode.restype = None
+        lib.orc_f32_to_f16_array.argtypes = [vp, ctypes.c_int64, vp]
+        lib.orc_f16_to_f32_array.argtypes = [vp, ctypes.c_int64, vp]
+        for p in ("f32", "f64"):
+            getattr(lib, f"orc_raster_fwd_{p}").argtypes = [vp, vp]
+            getattr(lib, f"orc_raster_bwd_{p}").argtypes = [vp, vp]
+        _lib = lib
+    return _lib
+
+
+def _p(a):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+def _addr(a):
+    return None if a is None else a.ctypes.data
+
+
+def _f32(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float32)
+
+
+def camera(eye, right, up, width, height, ray_maps=None):
+    """PfCamera from plain sequences; ray_maps (float32 [H,W,6]) is kept alive
+    by the returned tuple (cam, keepalive)."""
+    c = PfCamera()
+    c.eye[:] = [float(v) for v in eye]
+    c.right[:] = [float(v) for v in right]
+    c.up[:] = [float(v) for v in up]
+    c.width = int(width)
+    c.height = int(height)
+    rm = None
+    if ray_maps is not None:
+        rm = np.ascontiguousarray(ray_maps, dtype=np.float32)
+        c.ray_maps = rm.ctypes.data
+    c.fov_cos_cutoff = 0.0
+    return c, rm
+
+
+def bin_lists(cam, spheres):
+    """(offsets int64 [tiles+1], sites int32 [M], keys uint32 [M])."""
+    lib = load()
+    sp = _f32(spheres)
+    th, tw = tiles_hw(cam.height, cam.width)
+    counts = np.zeros(th * tw + 1, np.int32)
+    lib.orc_bin_count(ctypes.byref(cam), _p(sp), sp.shape[0], _p(counts))
+    offsets = np.cumsum(counts.astype(np.int64))
+    m = int(offsets[-1])
+    sites = np.zeros(max(m, 1), np.int32)
+    keys = np.zeros(max(m, 1), np.uint32)
+    lib.orc_bin_lists(ctypes.byref(cam), _p(sp), sp.shape[0], _p(offsets), _p(sites), _p(keys))
+    return offsets, sites[:m], keys[:m]
+
+
+def site_footprint(cam, sphere):
+    lib = load()
+    sp = _f32(sphere)
+    rng = np.zeros(4, np.int32)
+    obb = np.zeros(6, np.float32)
+    ok = lib.orc_site_footprint_dbg(ctypes.byref(cam), _p(sp), _p(rng), _p(obb))
+    return (rng, obb) if ok else None
+
+
+def tile_obb_test(cam, tile_i, tile_j, obb):
+    lib = load()
+    o = _f32(obb)
+    return bool(lib.orc_tile_obb_test_dbg(int(tile_i), int(tile_j), ctypes.byref(cam), _p(o)))
+
+
+def adj_encode(spheres, adjacency, offsets):
+    lib = load()
+    sp = _f32(spheres)
+    adj = np.ascontiguousarray(adjacency, np.int32)
+    off = np.ascontiguousarray(offsets, np.int32)
+    out = np.zeros((max(adj.shape[0], 1), 4), np.uint16)
+    lib.orc_adj_encode(_p(sp), sp.shape[0], _p(adj), _p(off), _p(out))
+    return out[: adj.shape[0]]
+
+
+def f32_to_f16_bits(x):
+    lib = load()
+    x = _f32(x).ravel()
+    y = np.zeros(x.shape[0], np.uint16)
+    lib.orc_f32_to_f16_array(_p(x), x.shape[0], _p(y))
+    return y
+
+
+class Inputs:
+    """Contiguous float32/int32 copies of a scene, as the oracle reads them."""
+
+    def __init__(self, spheres, nsigmas, texel_sites, texel_rgbh, adjacency, adjacency_offsets,
+                 adjacency_diff):
+        self.spheres = _f32(spheres)
+        self.nsigmas = _f32(nsigmas)
+        self.texel_sites = _f32(texel_sites)
+        self.texel_rgbh = _f32(texel_rgbh)
+        self.adjacency = np.ascontiguousarray(adjacency, np.int32)
+        self.adjacency_offsets = np.ascontiguousarray(adjacency_offsets, np.int32)
+        self.adjacency_diff = None if adjacency_diff is None else np.ascontiguousarray(adjacency_diff, np.uint16)
+        self.n = self.spheres.shape[0]
+        self.T = self.texel_sites.shape[1]
+
+    def struct(self):
+        s = PfScene()
+        s.n_sites = self.n
+        s.n_texels = self.T
+        s.spheres = _addr(self.spheres)
+        s.nsigmas = _addr(self.nsigmas)
+        s.texel_sites = _addr(self.texel_sites)
+        s.texel_rgbh = _addr(self.texel_rgbh)
+        s.adjacency = _addr(self.adjacency) if self.adjacency.size else None
+        s.adjacency_offsets = _addr(self.adjacency_offsets)
+        s.adjacency_diff = _addr(self.adjacency_diff) if self.adjacency_diff is not None and self.adjacency_diff.size else None
+        return s
+
+
+class _FwdOut(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_void_p) for k in
+                ("tile_early_stop", "color", "log_t", "normal_distance", "normal", "quantile_depths",
+                 "err", "contrib", "point_err")]
+
+
+class _BwdOut(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_void_p) for k in
+                ("grad_spheres", "grad_nsigmas", "grad_texel_sites", "grad_texel_rgb", "grad_texel_height")]
+
+
+def raster_fwd(cam, inp, offsets, sites, depth_quantiles=None, ray_gt=None, threshold=1e-3,
+               return_point_err=False, loss_accumulation=False, precision="f64"):
+    lib = load()
+    dt = np.float64 if precision == "f64" else np.float32
+    H, W = cam.height, cam.width
+    n_tiles = offsets.shape[0] - 1
+    offsets = np.ascontiguousarray(offsets, np.int64)
+    sites = np.ascontiguousarray(sites, np.int32)
+    if sites.size == 0:
+        sites = np.zeros(1, np.int32)
+    dq = _f32(depth_quantiles)
+    gt = _f32(ray_gt)
+    a = PfRasterFwdArgs()
+    a.cam = cam
+    a.scene = inp.struct()
+    a.tiles = PfTileLists(n_tiles, _addr(offsets), _addr(sites))
+    a.depth_quantiles = _addr(dq)
+    a.n_quantiles = 0 if dq is None else dq.shape[-1]
+    a.ray_gt = _addr(gt)
+    a.transmittance_threshold = threshold
+    a.return_point_err = int(return_point_err)
+    a.loss_accumulation = int(loss_accumulation)
+    out = dict(
+        tile_early_stop=np.zeros(n_tiles, np.int32),
+        color=np.zeros((H, W, 3), dt), log_t=np.zeros((H, W), dt),
+        normal_distance=np.zeros((H, W), dt), normal=np.zeros((H, W, 3), dt),
+        quantile_depths=None if dq is None else -np.ones(dq.shape, dt),
+        err=np.zeros((H, W), dt) if loss_accumulation else None,
+        contrib=np.zeros(inp.n, dt),
+        point_err=np.zeros(inp.n, dt) if return_point_err else None,
+    )
+    o = _FwdOut(*[_addr(out[k]) for k, _ in _FwdOut._fields_])
+    getattr(lib, f"orc_raster_fwd_{precision}")(ctypes.byref(a), ctypes.byref(o))
+    return out
+
+
+def raster_bwd(cam, inp, offsets, sites, tile_early_stop, log_t, grad_color, grad_log_t,
+               grad_normal_distance, grad_normal, grad_contrib, depth_quantiles=None,
+               grad_quantile_depths=None, exact=False, precision="f64"):
+    lib = load()
+    dt = np.float64 if precision == "f64" else np.float32
+    n_tiles = offsets.shape[0] - 1
+    offsets = np.ascontiguousarray(offsets, np.int64)
+    sites = np.ascontiguousarray(sites, np.int32)
+    if sites.size == 0:
+        sites = np.zeros(1, np.int32)
+    es = np.ascontiguousarray(tile_early_stop, np.int32)
+    keep = [_f32(x) for x in (log_t, grad_color, grad_log_t, grad_normal_distance, grad_normal, grad_contrib,
+                              depth_quantiles, grad_quantile_depths)]
+    lt, gc, glt, gnd, gn, gcon, dq, gq = keep
+    a = PfRasterBwdArgs()
+    a.cam = cam
+    a.scene = inp.struct()
+    a.tiles = PfTileLists(n_tiles, _addr(offsets), _addr(sites))
+    a.depth_quantiles = _addr(dq)
+    a.n_quantiles = 0 if dq is None else dq.shape[-1]
+    a.tile_early_stop = _addr(es)
+    a.log_t, a.grad_color, a.grad_log_t = _addr(lt), _addr(gc), _addr(glt)
+    a.grad_normal_distance, a.grad_normal = _addr(gnd), _addr(gn)
+    a.grad_quantile_depths, a.grad_contrib = _addr(gq), _addr(gcon)
+    a.exact_gradients = int(exact)
+    N, T = inp.n, inp.T
+    out = dict(grad_spheres=np.zeros((N, 4), dt), grad_nsigmas=np.zeros((N, 4), dt),
+               grad_texel_sites=np.zeros((N, T, 3), dt), grad_texel_rgb=np.zeros((N, T, 3), dt),
+               grad_texel_height=np.zeros((N, T), dt))
+    o = _BwdOut(*[_addr(out[k]) for k, _ in _BwdOut._fields_])
+    getattr(lib, f"orc_raster_bwd_{precision}")(ctypes.byref(a), ctypes.byref(o))
+    return out

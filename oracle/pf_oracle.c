@@ -1,0 +1,138 @@
+/*
+ * ORACLE -- test infrastructure only: the CPU restatement of the reference
+ * (theialab/powerfoam) renderer hot path that tests/, the smoke check and the
+ * bench's CPU baseline use as the checker. The product never links or loads
+ * it. Built by oracle/Makefile into oracle/build/liborcl.so.
+ *
+ *   orc_bin_count / orc_bin_lists   rasterize.py:140-200, 293-363, 1713-1768
+ *   orc_adj_encode                  rasterize.py:580-606
+ *   orc_raster_fwd_{f32,f64}        rasterize.py:892-1146
+ *   orc_raster_bwd_{f32,f64}        rasterize.py:1150-1514
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "orc_bin.h"
+
+/* ---------------------------------------------------------------- binning */
+
+static int orc_tiles_h(const pf_camera* cam) { return 1 + (cam->height - 1) / PF_TILE_WIDTH; }
+static int orc_tiles_w(const pf_camera* cam) { return 1 + (cam->width - 1) / PF_TILE_WIDTH; }
+
+/* counts[t+1] = number of sites binned to tile t; returns the total. */
+int64_t orc_bin_count(const pf_camera* cam, const float* spheres, int32_t n, int32_t* counts) {
+  const int th = orc_tiles_h(cam), tw = orc_tiles_w(cam);
+  memset(counts, 0, sizeof(int32_t) * (size_t)(th * tw + 1));
+  int64_t total = 0;
+  for (int s = 0; s < n; s++) {
+    orc_footprint fp;
+    if (!orc_site_footprint(cam, spheres + 4 * (size_t)s, &fp)) continue;
+    for (int ti = fp.ti0; ti <= fp.ti1; ti++)
+      for (int tj = fp.tj0; tj <= fp.tj1; tj++) {
+        if (!orc_tile_obb_test(ti, tj, cam, &fp.obb)) continue;
+        counts[ti + tj * th + 1]++;
+        total++;
+      }
+  }
+  return total;
+}
+
+typedef struct { uint32_t key; int32_t site; } orc_rec;
+
+static int orc_rec_cmp(const void* a, const void* b) {
+  const orc_rec* x = (const orc_rec*)a;
+  const orc_rec* y = (const orc_rec*)b;
+  if (x->key != y->key) return x->key < y->key ? -1 : 1;
+  return (x->site > y->site) - (x->site < y->site);
+}
+
+/* Fills the per-tile lists given offsets (cumsum of counts), each tile's
+ * entries ordered by (depth key, site). keys_out may be NULL. */
+void orc_bin_lists(const pf_camera* cam, const float* spheres, int32_t n, const int64_t* offsets,
+                   int32_t* sites_out, uint32_t* keys_out) {
+  const int th = orc_tiles_h(cam), tw = orc_tiles_w(cam);
+  const int nt = th * tw;
+  int64_t m = offsets[nt];
+  orc_rec* recs = (orc_rec*)malloc(sizeof(orc_rec) * (size_t)(m > 0 ? m : 1));
+  int64_t* cur = (int64_t*)calloc((size_t)nt, sizeof(int64_t));
+  for (int s = 0; s < n; s++) {
+    orc_footprint fp;
+    if (!orc_site_footprint(cam, spheres + 4 * (size_t)s, &fp)) continue;
+    for (int ti = fp.ti0; ti <= fp.ti1; ti++)
+      for (int tj = fp.tj0; tj <= fp.tj1; tj++) {
+        if (!orc_tile_obb_test(ti, tj, cam, &fp.obb)) continue;
+        int t = ti + tj * th;
+        int64_t idx = offsets[t] + cur[t]++;
+        if (idx < offsets[t + 1]) {
+          recs[idx].key = fp.key;
+          recs[idx].site = s;
+        }
+      }
+  }
+  for (int t = 0; t < nt; t++) {
+    int64_t a = offsets[t], b = offsets[t + 1];
+    if (b - a > 1) qsort(recs + a, (size_t)(b - a), sizeof(orc_rec), orc_rec_cmp);
+  }
+  for (int64_t i = 0; i < m; i++) {
+    sites_out[i] = recs[i].site;
+    if (keys_out) keys_out[i] = recs[i].key;
+  }
+  free(cur);
+  free(recs);
+}
+
+/* Exposes the per-site footprint for the brute-force tests:
+ * out[0..3] = tile range (ti0, ti1, tj0, tj1), obb[0..5]; returns 0 if culled. */
+int orc_site_footprint_dbg(const pf_camera* cam, const float* sphere, int32_t* range, float* obb) {
+  orc_footprint fp;
+  if (!orc_site_footprint(cam, sphere, &fp)) return 0;
+  range[0] = fp.ti0; range[1] = fp.ti1; range[2] = fp.tj0; range[3] = fp.tj1;
+  obb[0] = fp.obb.uc; obb[1] = fp.obb.vc; obb[2] = fp.obb.r1;
+  obb[3] = fp.obb.r2; obb[4] = fp.obb.vx; obb[5] = fp.obb.vy;
+  return 1;
+}
+
+int orc_tile_obb_test_dbg(int tile_i, int tile_j, const pf_camera* cam, const float* obb) {
+  orc_obb o = {obb[0], obb[1], obb[2], obb[3], obb[4], obb[5]};
+  return orc_tile_obb_test(tile_i, tile_j, cam, &o);
+}
+
+/* ------------------------------------------------------- adjacency encode */
+
+void orc_adj_encode(const float* spheres, int32_t n, const int32_t* adj, const int32_t* adj_off,
+                    uint16_t* diff) {
+  for (int s = 0; s < n; s++) {
+    const float* sp = spheres + 4 * (size_t)s;
+    for (int e = adj_off[s]; e < adj_off[s + 1]; e++) {
+      const float* ap = spheres + 4 * (size_t)adj[e];
+      diff[4 * (size_t)e + 0] = orc_f32_to_f16(ap[0] - sp[0]);
+      diff[4 * (size_t)e + 1] = orc_f32_to_f16(ap[1] - sp[1]);
+      diff[4 * (size_t)e + 2] = orc_f32_to_f16(ap[2] - sp[2]);
+      diff[4 * (size_t)e + 3] = orc_f32_to_f16(ap[3]);
+    }
+  }
+}
+
+void orc_f32_to_f16_array(const float* x, int64_t n, uint16_t* y) {
+  for (int64_t i = 0; i < n; i++) y[i] = orc_f32_to_f16(x[i]);
+}
+
+void orc_f16_to_f32_array(const uint16_t* x, int64_t n, float* y) {
+  for (int64_t i = 0; i < n; i++) y[i] = orc_f16_to_f32(x[i]);
+}
+
+/* ------------------------------------------------------------- compositing */
+
+#define R float
+#define SFX _f32
+#include "orc_raster_impl.h"
+#undef R
+#undef SFX
+
+#define R double
+#define SFX _f64
+#include "orc_raster_impl.h"
+#undef R
+#undef SFX

@@ -1,0 +1,177 @@
+"""ctypes mirror of include/powerfoam_amd.h and the loader of libpfhip.so.
+
+This module replaces the reference's Warp launch shim (powerfoam/warp_interop.py:6-24):
+every kernel is reached through the C-ABI shared library built from
+powerfoam_amd/csrc, with raw device pointers and torch's current HIP stream.
+There is no fallback: if the library is missing, loading raises.
+"""
+
+import ctypes
+import os
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libpfhip.so")
+
+PF_OK = 0
+PF_ERR_ARG = -1
+PF_ERR_OVERFLOW = -2
+PF_ERR_HIP = -3
+PF_ERR_UNSUPPORTED = -4
+
+TILE_WIDTH = 8
+TILE_SIZE = 64
+MAX_TEXELS = 8
+
+c_float_p = ctypes.POINTER(ctypes.c_float)
+c_i32_p = ctypes.POINTER(ctypes.c_int32)
+c_i64_p = ctypes.POINTER(ctypes.c_int64)
+c_u16_p = ctypes.POINTER(ctypes.c_uint16)
+c_u64_p = ctypes.POINTER(ctypes.c_uint64)
+c_u8_p = ctypes.POINTER(ctypes.c_uint8)
+
+
+class PfCamera(ctypes.Structure):
+    _fields_ = [
+        ("eye", ctypes.c_float * 3),
+        ("right", ctypes.c_float * 3),
+        ("up", ctypes.c_float * 3),
+        ("width", ctypes.c_int32),
+        ("height", ctypes.c_int32),
+        ("ray_maps", ctypes.c_void_p),
+        ("fov_cos_cutoff", ctypes.c_float),
+    ]
+
+
+class PfScene(ctypes.Structure):
+    _fields_ = [
+        ("n_sites", ctypes.c_int32),
+        ("n_texels", ctypes.c_int32),
+        ("spheres", ctypes.c_void_p),
+        ("nsigmas", ctypes.c_void_p),
+        ("texel_sites", ctypes.c_void_p),
+        ("texel_rgbh", ctypes.c_void_p),
+        ("adjacency", ctypes.c_void_p),
+        ("adjacency_offsets", ctypes.c_void_p),
+        ("adjacency_diff", ctypes.c_void_p),
+    ]
+
+
+class PfTileLists(ctypes.Structure):
+    _fields_ = [
+        ("n_tiles", ctypes.c_int32),
+        ("offsets", ctypes.c_void_p),
+        ("sites", ctypes.c_void_p),
+    ]
+
+
+class PfRasterFwdArgs(ctypes.Structure):
+    _fields_ = [
+        ("cam", PfCamera),
+        ("scene", PfScene),
+        ("tiles", PfTileLists),
+        ("depth_quantiles", ctypes.c_void_p),
+        ("n_quantiles", ctypes.c_int32),
+        ("ray_gt", ctypes.c_void_p),
+        ("transmittance_threshold", ctypes.c_float),
+        ("return_point_err", ctypes.c_int32),
+        ("loss_accumulation", ctypes.c_int32),
+        ("tile_early_stop", ctypes.c_void_p),
+        ("color", ctypes.c_void_p),
+        ("log_t", ctypes.c_void_p),
+        ("normal_distance", ctypes.c_void_p),
+        ("normal", ctypes.c_void_p),
+        ("quantile_depths", ctypes.c_void_p),
+        ("err", ctypes.c_void_p),
+        ("contrib", ctypes.c_void_p),
+        ("point_err", ctypes.c_void_p),
+    ]
+
+
+class PfRasterBwdArgs(ctypes.Structure):
+    _fields_ = [
+        ("cam", PfCamera),
+        ("scene", PfScene),
+        ("tiles", PfTileLists),
+        ("depth_quantiles", ctypes.c_void_p),
+        ("n_quantiles", ctypes.c_int32),
+        ("tile_early_stop", ctypes.c_void_p),
+        ("log_t", ctypes.c_void_p),
+        ("grad_color", ctypes.c_void_p),
+        ("grad_log_t", ctypes.c_void_p),
+        ("grad_normal_distance", ctypes.c_void_p),
+        ("grad_normal", ctypes.c_void_p),
+        ("grad_quantile_depths", ctypes.c_void_p),
+        ("grad_contrib", ctypes.c_void_p),
+        ("exact_gradients", ctypes.c_int32),
+        ("grad_spheres", ctypes.c_void_p),
+        ("grad_nsigmas", ctypes.c_void_p),
+        ("grad_texel_sites", ctypes.c_void_p),
+        ("grad_texel_rgb", ctypes.c_void_p),
+        ("grad_texel_height", ctypes.c_void_p),
+    ]
+
+
+# Exported C symbols and their ctypes signatures (argtypes, restype).
+_VP = ctypes.c_void_p
+SIGNATURES = {
+    "pf_version": ([], ctypes.c_int),
+    "pf_last_error": ([], ctypes.c_char_p),
+    "pf_bin_count": ([_VP, _VP, ctypes.c_int32, _VP, _VP], ctypes.c_int),
+    "pf_bin_scan": ([_VP, ctypes.c_int32, _VP, _VP], ctypes.c_int),
+    "pf_bin_emit": ([_VP, _VP, ctypes.c_int32, _VP, ctypes.c_int32, _VP, _VP, _VP, _VP], ctypes.c_int),
+    "pf_bin_sort_workspace_size": ([ctypes.c_int64, ctypes.c_int32], ctypes.c_size_t),
+    "pf_bin_sort": ([_VP, ctypes.c_int32, _VP, _VP, _VP, ctypes.c_size_t, _VP], ctypes.c_int),
+    "pf_adj_encode": ([_VP, ctypes.c_int32, _VP, _VP, _VP, _VP], ctypes.c_int),
+    "pf_raster_fwd": ([_VP, _VP], ctypes.c_int),
+    "pf_raster_bwd": ([_VP, _VP], ctypes.c_int),
+    "pf_scrub_nonfinite": ([_VP, ctypes.c_int64, _VP], ctypes.c_int),
+}
+
+_lib = None
+
+
+class PfError(RuntimeError):
+    pass
+
+
+def load(path=None):
+    """Load libpfhip.so (once). Raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise PfError(
+            f"{p} is missing: build the HIP extension first "
+            "(python -c 'import __graft_entry__ as g; g.build()')"
+        )
+    lib = ctypes.CDLL(p)
+    for name, (argtypes, restype) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = argtypes
+        fn.restype = restype
+    _lib = lib
+    return lib
+
+
+def check(status):
+    if status != PF_OK:
+        msg = _lib.pf_last_error().decode() if _lib is not None else ""
+        raise PfError(f"powerfoam_amd call failed with status {status}: {msg}")
+
+
+def ptr(t):
+    """Raw address of a tensor (None -> NULL)."""
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def stream_handle():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def tiles_hw(height, width):
+    return 1 + (height - 1) // TILE_WIDTH, 1 + (width - 1) // TILE_WIDTH

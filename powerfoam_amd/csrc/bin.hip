@@ -1,0 +1,390 @@
+// Tile binning of power-cell screen footprints (pinhole cameras).
+//
+// Replaces, per frame:
+//   count_visible_pinhole_kernel  rasterize.py:140-200
+//   torch.cumsum + .item()        rasterize.py:1713-1714
+//   write_visible_pinhole_kernel  rasterize.py:293-363
+//   torch.argsort + gather        rasterize.py:1761-1768
+//
+// This file is compiled with -ffp-contract=off: tile membership and sort keys
+// are float-derived integers and must not depend on FMA contraction. Every
+// float operation below is a single correctly rounded IEEE operation, in the
+// order the reference writes it (camera.py:79-178, rasterize.py:155-190).
+//
+// Sorting: the count pass yields per-tile offsets, so the reference's global
+// 64-bit argsort becomes a sort inside each tile segment. Each segment holds
+// 64-bit records (depth key << 32 | site); one 256-thread workgroup sorts one
+// segment with an ascending-only bitonic network (virtual +inf padding to the
+// next power of two), staged in LDS up to 4096 records and merged through
+// global memory above that. Equal depth keys are ordered by site id, so the
+// lists are deterministic (the reference's argsort is not stable).
+#include <stdarg.h>
+#include <stdio.h>
+
+#include "pf_common.hpp"
+
+namespace pf {
+
+struct BinCam {
+  f3 eye, right, up;
+  int W, H, th;
+};
+
+static BinCam make_bincam(const pf_camera& c) {
+  BinCam b;
+  b.eye = f3{c.eye[0], c.eye[1], c.eye[2]};
+  b.right = f3{c.right[0], c.right[1], c.right[2]};
+  b.up = f3{c.up[0], c.up[1], c.up[2]};
+  b.W = c.width;
+  b.H = c.height;
+  b.th = tiles_h(c);
+  return b;
+}
+
+// float -> int32 as the hardware converts: truncate, saturate, NaN -> 0
+__device__ __forceinline__ int f2i(float x) {
+  if (x != x) return 0;
+  if (x >= 2147483648.0f) return 2147483647;
+  if (x <= -2147483648.0f) return (int)0x80000000;
+  return (int)x;
+}
+
+struct Obb {
+  float uc, vc, r1, r2, vx, vy;
+};
+
+// camera.py:79-128
+__device__ __forceinline__ Obb proj_sphere_to_obb(const BinCam& cam, f3 center, float radius) {
+  Obb o;
+  f3 X = cam.right, Y = cam.up;
+  f3 C = center - cam.eye;
+  float C2 = dot(C, C);
+  float R2 = radius * radius;
+  f3 zc = cross(Y, X);
+  float zl = sqrtf(dot(zc, zc));
+  f3 Z = zl > 0.0f ? f3{zc.x / zl, zc.y / zl, zc.z / zl} : f3{0.0f, 0.0f, 0.0f};
+  float cx = dot(C, X), cy = dot(C, Y), cz = dot(C, Z);
+  float k = C2 - R2;
+  float a = cx * cx - k * dot(X, X);
+  float b = cy * cy - k * dot(Y, Y);
+  float f = cz * cz - k * dot(Z, Z);
+  float c = 2.0f * cx * cy;
+  float d = 2.0f * cx * cz;
+  float e = 2.0f * cy * cz;
+  float denom = 4.0f * a * b - c * c;
+  if (fabsf(denom) < 1e-8f) {
+    o.uc = 0.0f; o.vc = 0.0f; o.r1 = 0.0f; o.r2 = 0.0f; o.vx = 1.0f; o.vy = 0.0f;
+    return o;
+  }
+  float uc = (c * e - 2.0f * b * d) / denom;
+  float vc = (c * d - 2.0f * a * e) / denom;
+  float F0 = a * uc * uc + b * vc * vc + c * uc * vc + d * uc + e * vc + f;
+  float ts = sqrtf((a - b) * (a - b) + c * c);
+  float l1 = (a + b + ts) / 2.0f;
+  float l2 = (a + b - ts) / 2.0f;
+  float r1 = sqrtf(-F0 / l1);
+  float r2 = sqrtf(-F0 / l2);
+  float vx = 0.5f * c;
+  float vy = l1 - a;
+  float norm = sqrtf(vx * vx + vy * vy);
+  if (norm < 1e-6f) {
+    vx = 1.0f; vy = 0.0f;
+  } else {
+    vx = vx / norm; vy = vy / norm;
+  }
+  o.uc = uc; o.vc = vc; o.r1 = r1; o.r2 = r2; o.vx = vx; o.vy = vy;
+  return o;
+}
+
+// camera.py:131-178 (two OBB axes)
+__device__ __forceinline__ bool tile_obb_test(int tile_i, int tile_j, const BinCam& cam, const Obb& o) {
+  float wm1 = (float)(cam.W - 1), hm1 = (float)(cam.H - 1);
+  float u_min = 2.0f * (float)(tile_j * PF_TILE_WIDTH) / wm1 - 1.0f;
+  float u_max = 2.0f * (float)((tile_j + 1) * PF_TILE_WIDTH) / wm1 - 1.0f;
+  float v_max = 1.0f - 2.0f * (float)(tile_i * PF_TILE_WIDTH) / hm1;
+  float v_min = 1.0f - 2.0f * (float)((tile_i + 1) * PF_TILE_WIDTH) / hm1;
+  float u_center = (u_min + u_max) * 0.5f;
+  float v_center = (v_min + v_max) * 0.5f;
+  float eu = (u_max - u_min) * 0.5f;
+  float ev = (v_max - v_min) * 0.5f;
+  float du = u_center - o.uc;
+  float dv = v_center - o.vc;
+  float p1 = du * o.vx + dv * o.vy;
+  float t1 = eu * fabsf(o.vx) + ev * fabsf(o.vy);
+  if (fabsf(p1) > o.r1 + t1) return false;
+  float p2 = du * -o.vy + dv * o.vx;
+  float t2 = eu * fabsf(-o.vy) + ev * fabsf(o.vx);
+  if (fabsf(p2) > o.r2 + t2) return false;
+  return true;
+}
+
+struct Footprint {
+  int ti0, ti1, tj0, tj1;
+  Obb obb;
+  uint32_t key;
+};
+
+// rasterize.py:150-190, 360-363
+__device__ __forceinline__ bool site_footprint(const BinCam& cam, float4 sph, Footprint& fp) {
+  f3 center{sph.x, sph.y, sph.z};
+  float radius = sph.w;
+  f3 v = center - cam.eye;
+  f3 fwd = cross(cam.up, cam.right);  // not normalised, as the reference
+  if (sqrtf(dot(v, v)) < 4.0f * radius || dot(v, fwd) < 0.1f) return false;
+  Obb o = proj_sphere_to_obb(cam, center, radius);
+  float ext_u = fabsf(o.vx * o.r1) + fabsf(o.vy * o.r2);
+  float ext_v = fabsf(o.vy * o.r1) + fabsf(o.vx * o.r2);
+  float u_min = o.uc - ext_u, u_max = o.uc + ext_u;
+  float v_min = o.vc - ext_v, v_max = o.vc + ext_v;
+  float wm1 = (float)(cam.W - 1), hm1 = (float)(cam.H - 1);
+  float j_min_f = (u_min + 1.0f) * 0.5f * wm1;
+  float j_max_f = (u_max + 1.0f) * 0.5f * wm1;
+  float i_min_f = (1.0f - v_max) * 0.5f * hm1;
+  float i_max_f = (1.0f - v_min) * 0.5f * hm1;
+  int i_min = max(f2i(floorf(i_min_f)), 0);
+  int i_max = min(f2i(ceilf(i_max_f)), cam.H - 1);
+  int j_min = max(f2i(floorf(j_min_f)), 0);
+  int j_max = min(f2i(ceilf(j_max_f)), cam.W - 1);
+  fp.ti0 = i_min / PF_TILE_WIDTH;
+  fp.ti1 = i_max / PF_TILE_WIDTH;
+  fp.tj0 = j_min / PF_TILE_WIDTH;
+  fp.tj1 = j_max / PF_TILE_WIDTH;
+  fp.obb = o;
+  float pow_dist = dot(v, v) - radius * radius;
+  fp.key = __builtin_bit_cast(uint32_t, pow_dist);
+  return true;
+}
+
+__global__ void __launch_bounds__(256) fill_i32_kernel(int32_t* p, int64_t n, int32_t v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void __launch_bounds__(256)
+    bin_count_kernel(BinCam cam, const float4* __restrict__ spheres, int n, int32_t* __restrict__ counts) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  Footprint fp;
+  if (!site_footprint(cam, spheres[s], fp)) return;
+  for (int ti = fp.ti0; ti <= fp.ti1; ti++)
+    for (int tj = fp.tj0; tj <= fp.tj1; tj++) {
+      if (!tile_obb_test(ti, tj, cam, fp.obb)) continue;
+      atomicAdd(&counts[ti + tj * cam.th + 1], 1);
+    }
+}
+
+__global__ void __launch_bounds__(256)
+    bin_emit_kernel(BinCam cam, const float4* __restrict__ spheres, int n, const int64_t* __restrict__ offsets,
+                    uint64_t* __restrict__ records, int32_t* __restrict__ cursor, uint8_t* __restrict__ visible) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  Footprint fp;
+  if (!site_footprint(cam, spheres[s], fp)) return;
+  const uint64_t rec = ((uint64_t)fp.key << 32) | (uint32_t)s;
+  bool any = false;
+  for (int ti = fp.ti0; ti <= fp.ti1; ti++)
+    for (int tj = fp.tj0; tj <= fp.tj1; tj++) {
+      if (!tile_obb_test(ti, tj, cam, fp.obb)) continue;
+      int t = ti + tj * cam.th;
+      int64_t idx = offsets[t] + (int64_t)atomicAdd(&cursor[t], 1);
+      if (idx < offsets[t + 1]) {
+        records[idx] = rec;
+        any = true;
+      }
+    }
+  if (any && visible) visible[s] = 1;
+}
+
+// Inclusive scan of counts[0..n] (int32) into offsets[0..n] (int64) by one
+// 1024-thread workgroup: each thread scans a contiguous run serially, then
+// the run totals are scanned across the workgroup.
+__global__ void __launch_bounds__(1024)
+    bin_scan_kernel(const int32_t* __restrict__ counts, int64_t n, int64_t* __restrict__ offsets) {
+  __shared__ int64_t part[1024];
+  const int t = threadIdx.x;
+  const int64_t per = (n + 1023) / 1024;
+  const int64_t a = t * per;
+  const int64_t b = min(a + per, n);
+  int64_t s = 0;
+  for (int64_t i = a; i < b; i++) s += counts[i];
+  part[t] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    int64_t v = t >= off ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int64_t run = t > 0 ? part[t - 1] : 0;
+  for (int64_t i = a; i < b; i++) {
+    run += counts[i];
+    offsets[i] = run;
+  }
+}
+
+constexpr int SORT_THREADS = 256;
+constexpr int SORT_CAP = 4096;  // records staged in LDS (32 KiB)
+
+// One step of the ascending-only bitonic network over n valid records in a
+// power-of-two frame: either the mirrored first step of a merge of size k
+// (mirror = true) or a half-cleaner with stride j. Pairs whose upper index
+// is >= n compare against virtual +inf and are skipped. k and j are powers
+// of two; lg_half = log2(k/2) for the mirror step, lg_j = log2(j) otherwise.
+template <typename Buf>
+__device__ __forceinline__ void bitonic_step(Buf* buf, int64_t n_pairs, int64_t n, int lg, bool mirror) {
+  const int64_t m = ((int64_t)1 << lg) - 1;
+  for (int64_t p = threadIdx.x; p < n_pairs; p += SORT_THREADS) {
+    const int64_t blk = p >> lg, l = p & m;
+    int64_t i, q;
+    if (mirror) {  // block of size k = 2^(lg+1): pair l with k-1-l
+      i = (blk << (lg + 1)) + l;
+      q = (blk << (lg + 1)) + (m << 1) + 1 - l;
+    } else {  // stride j = 2^lg
+      i = (blk << (lg + 1)) + l;
+      q = i + m + 1;
+    }
+    if (q >= n) continue;
+    uint64_t a = buf[i], b = buf[q];
+    if (a > b) {
+      buf[i] = b;
+      buf[q] = a;
+    }
+  }
+}
+
+__device__ __forceinline__ int ilog2(int64_t x) { return 63 - __builtin_clzll((unsigned long long)x); }
+
+__global__ void __launch_bounds__(SORT_THREADS)
+    bin_sort_kernel(const int64_t* __restrict__ offsets, int n_tiles, uint64_t* __restrict__ records,
+                    int32_t* __restrict__ sites) {
+  __shared__ uint64_t lds[SORT_CAP];
+  const int tile = blockIdx.x;
+  const int64_t a = offsets[tile];
+  const int64_t n = offsets[tile + 1] - a;
+  if (n <= 0) return;
+  uint64_t* g = records + a;
+  if (n == 1) {
+    if (threadIdx.x == 0) sites[a] = (int32_t)(uint32_t)g[0];
+    return;
+  }
+  int64_t P = 1;
+  while (P < n) P <<= 1;
+  const int lgP = ilog2(P);
+  if (n <= SORT_CAP) {
+    for (int64_t i = threadIdx.x; i < n; i += SORT_THREADS) lds[i] = g[i];
+    __syncthreads();
+    for (int lk = 1; lk <= lgP; lk++) {
+      bitonic_step(lds, P / 2, n, lk - 1, true);
+      __syncthreads();
+      for (int lj = lk - 2; lj >= 0; lj--) {
+        bitonic_step(lds, P / 2, n, lj, false);
+        __syncthreads();
+      }
+    }
+    for (int64_t i = threadIdx.x; i < n; i += SORT_THREADS) sites[a + i] = (int32_t)(uint32_t)lds[i];
+    return;
+  }
+  // Large segment: sort SORT_CAP chunks in LDS, then merge through global
+  // memory; strides that fit a chunk are finished in LDS.
+  const int lgC = ilog2(SORT_CAP);
+  const int64_t n_chunks = (n + SORT_CAP - 1) / SORT_CAP;
+  for (int lk = 1; lk <= lgP; lk++) {
+    if (lk > lgC) {
+      bitonic_step(g, P / 2, n, lk - 1, true);
+      __syncthreads();
+      for (int lj = lk - 2; lj >= lgC; lj--) {
+        bitonic_step(g, P / 2, n, lj, false);
+        __syncthreads();
+      }
+    }
+    if (lk > lgC || lk == lgC) {
+      // finish every step of merge sizes <= 2^lk whose pairs stay in a chunk
+      for (int64_t c = 0; c < n_chunks; c++) {
+        const int64_t base = c * SORT_CAP;
+        const int64_t cn = min((int64_t)SORT_CAP, n - base);
+        for (int64_t i = threadIdx.x; i < cn; i += SORT_THREADS) lds[i] = g[base + i];
+        __syncthreads();
+        if (lk == lgC) {
+          for (int l2 = 1; l2 <= lgC; l2++) {
+            bitonic_step(lds, SORT_CAP / 2, cn, l2 - 1, true);
+            __syncthreads();
+            for (int lj = l2 - 2; lj >= 0; lj--) {
+              bitonic_step(lds, SORT_CAP / 2, cn, lj, false);
+              __syncthreads();
+            }
+          }
+        } else {
+          for (int lj = lgC - 1; lj >= 0; lj--) {
+            bitonic_step(lds, SORT_CAP / 2, cn, lj, false);
+            __syncthreads();
+          }
+        }
+        for (int64_t i = threadIdx.x; i < cn; i += SORT_THREADS) g[base + i] = lds[i];
+        __syncthreads();
+      }
+    }
+  }
+  for (int64_t i = threadIdx.x; i < n; i += SORT_THREADS) sites[a + i] = (int32_t)(uint32_t)g[i];
+}
+
+}  // namespace pf
+
+using namespace pf;
+
+extern "C" {
+
+int pf_bin_count(const pf_camera* cam, const float* spheres, int32_t n_sites, int32_t* tile_counts, void* stream) {
+  if (!cam || !tile_counts || n_sites < 0 || (n_sites > 0 && !spheres))
+    return set_error(PF_ERR_ARG, "pf_bin_count: bad arguments");
+  if (cam->width < 2 || cam->height < 2) return set_error(PF_ERR_ARG, "pf_bin_count: image must be >= 2x2");
+  if (cam->ray_maps) return set_error(PF_ERR_UNSUPPORTED, "pf_bin_count: pinhole cameras only");
+  hipStream_t st = as_stream(stream);
+  const BinCam bc = make_bincam(*cam);
+  const int64_t nt = (int64_t)tiles_h(*cam) * tiles_w(*cam);
+  fill_i32_kernel<<<(unsigned)((nt + 1 + 255) / 256), 256, 0, st>>>(tile_counts, nt + 1, 0);
+  if (n_sites > 0)
+    bin_count_kernel<<<(n_sites + 255) / 256, 256, 0, st>>>(bc, reinterpret_cast<const float4*>(spheres), n_sites,
+                                                           tile_counts);
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+int pf_bin_scan(const int32_t* tile_counts, int32_t n_tiles, int64_t* offsets, void* stream) {
+  if (!tile_counts || !offsets || n_tiles < 1) return set_error(PF_ERR_ARG, "pf_bin_scan: bad arguments");
+  bin_scan_kernel<<<1, 1024, 0, as_stream(stream)>>>(tile_counts, (int64_t)n_tiles + 1, offsets);
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+int pf_bin_emit(const pf_camera* cam, const float* spheres, int32_t n_sites, const int64_t* offsets, int32_t n_tiles,
+                uint64_t* records, int32_t* cursor, uint8_t* visible, void* stream) {
+  if (!cam || !offsets || !cursor || n_sites < 0 || (n_sites > 0 && !spheres))
+    return set_error(PF_ERR_ARG, "pf_bin_emit: bad arguments");
+  if (cam->ray_maps) return set_error(PF_ERR_UNSUPPORTED, "pf_bin_emit: pinhole cameras only");
+  if (n_tiles != tiles_h(*cam) * tiles_w(*cam)) return set_error(PF_ERR_ARG, "pf_bin_emit: n_tiles mismatch");
+  hipStream_t st = as_stream(stream);
+  fill_i32_kernel<<<(unsigned)((n_tiles + 255) / 256), 256, 0, st>>>(cursor, n_tiles, 0);
+  if (n_sites > 0)
+    bin_emit_kernel<<<(n_sites + 255) / 256, 256, 0, st>>>(make_bincam(*cam), reinterpret_cast<const float4*>(spheres),
+                                                          n_sites, offsets, records, cursor, visible);
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+size_t pf_bin_sort_workspace_size(int64_t n_records, int32_t n_tiles) {
+  (void)n_records;
+  (void)n_tiles;
+  return 0;  // the segmented sort works in place
+}
+
+int pf_bin_sort(const int64_t* offsets, int32_t n_tiles, uint64_t* records, int32_t* sites, void* workspace,
+                size_t workspace_bytes, void* stream) {
+  (void)workspace;
+  (void)workspace_bytes;
+  if (!offsets || n_tiles < 1) return set_error(PF_ERR_ARG, "pf_bin_sort: bad arguments");
+  bin_sort_kernel<<<n_tiles, SORT_THREADS, 0, as_stream(stream)>>>(offsets, n_tiles, records, sites);
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+}  // extern "C"

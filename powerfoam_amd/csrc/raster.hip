@@ -1,0 +1,960 @@
+// Per-tile compositing, forward and backward, for gfx950.
+//
+//   forward_kernel   rasterize.py:892-1146
+//   backward_kernel  rasterize.py:1150-1514
+//   texel surface    texture.py:10-186
+//   intersections    rendering_math.py:4-147
+//
+// One wave64 per 8x8 tile (the reference's 64-thread tile block maps 1:1 onto
+// a CDNA wavefront); lane l shades pixel (tile_i*8 + l%8, tile_j*8 + l/8).
+// The wave walks its tile's depth-sorted site list in batches of 64: lanes
+// fetch 64 site records with one coalesced gather into LDS, and every
+// per-site quantity that does not depend on the pixel (face normals and
+// offsets, texel offsets, plane offset) is computed once per site by the wave
+// into LDS instead of once per lane. The tile-wide decisions of the reference
+// become ballots: stop when no lane has T >= threshold (the tile max of T is
+// below it), skip a site no lane hits. Intersections use cancellation-free
+// forms of the reference formulas (ray-sphere: disc = 4(r^2 - |oc x d|^2);
+// power face: t = ((c-o).n + (|n|^2 + r^2 - r_a^2)/2) / (d.n) with n the
+// decoded neighbour delta), so FP32 results stay within 1e-4 of the FP64
+// oracle where the reference's FP32 formulas drift by ~1e-3.
+//
+// The backward recomputes each segment back-to-front from the saved final
+// log-transmittance (rasterize.py:1219, 1372-1375) and reduces every per-site
+// gradient across the wave before touching global memory: neighbour-face
+// gradients are gathered in LDS per face slot, texel gradients are summed by
+// an LDS transpose, and one lane per value issues the atomic.
+#include "pf_common.hpp"
+
+namespace pf {
+
+constexpr int WAVE = 64;
+constexpr float LOG2E = 1.4426950408889634f;
+
+struct RasterCam {
+  f3 eye, right, up, fwd;  // fwd = normalize(up x right)
+  int W, H, th, tw;
+  const float* ray_maps;
+};
+
+static RasterCam make_rcam(const pf_camera& c) {
+  RasterCam r;
+  r.eye = f3{c.eye[0], c.eye[1], c.eye[2]};
+  r.right = f3{c.right[0], c.right[1], c.right[2]};
+  r.up = f3{c.up[0], c.up[1], c.up[2]};
+  // forward computed on the host in double, rounded once
+  double ux = c.up[0], uy = c.up[1], uz = c.up[2], rx = c.right[0], ry = c.right[1], rz = c.right[2];
+  double fx = uy * rz - uz * ry, fy = uz * rx - ux * rz, fz = ux * ry - uy * rx;
+  double fl = sqrt(fx * fx + fy * fy + fz * fz);
+  if (fl > 0) {
+    fx /= fl; fy /= fl; fz /= fl;
+  }
+  r.fwd = f3{(float)fx, (float)fy, (float)fz};
+  r.W = c.width;
+  r.H = c.height;
+  r.th = tiles_h(c);
+  r.tw = tiles_w(c);
+  r.ray_maps = c.ray_maps;
+  return r;
+}
+
+// Workgroups are dealt round-robin to the 8 XCDs; give each XCD a contiguous
+// run of tiles so neighbouring tiles (which share sites) share an L2.
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+  const int per = (n + 7) >> 3;
+  const int xcd = b & 7, k = b >> 3;
+  const int full = n - (per - 1) * 8;  // XCDs that own `per` tiles
+  int t;
+  if (xcd < full) t = xcd * per + k;
+  else t = full * per + (xcd - full) * (per - 1) + k;
+  return t;
+}
+__device__ __forceinline__ bool xcd_valid(int b, int n) {
+  const int per = (n + 7) >> 3;
+  const int xcd = b & 7, k = b >> 3;
+  const int full = n - (per - 1) * 8;
+  return xcd < full ? k < per : k < per - 1;
+}
+
+struct Ray {
+  f3 o, d;
+  int pi, pj;
+  bool oob;
+};
+
+template <bool PINHOLE>
+__device__ __forceinline__ Ray lane_ray(const RasterCam& cam, int tile, int lane) {
+  Ray r;
+  const int ti = tile % cam.th, tj = tile / cam.th;
+  int i = ti * PF_TILE_WIDTH + (lane & 7);
+  int j = tj * PF_TILE_WIDTH + (lane >> 3);
+  r.oob = i >= cam.H || j >= cam.W;
+  i = min(i, cam.H - 1);
+  j = min(j, cam.W - 1);
+  r.pi = i;
+  r.pj = j;
+  f3 d;
+  if (PINHOLE) {
+    const float x = 2.0f * (float)j / ((float)cam.W - 1.0f) - 1.0f;
+    const float y = 1.0f - 2.0f * (float)i / ((float)cam.H - 1.0f);
+    d = cam.right * x + cam.up * y + cam.fwd;
+    r.o = cam.eye;
+  } else {
+    const float* rm = cam.ray_maps + ((size_t)i * cam.W + j) * 6;
+    r.o = f3{rm[0], rm[1], rm[2]};
+    d = f3{rm[3], rm[4], rm[5]};
+  }
+  const float il = 1.0f / sqrtf(dot(d, d));
+  r.d = d * il;
+  return r;
+}
+
+// ray-sphere (rendering_math.py:4-23) with the perpendicular-distance
+// discriminant; oc = o - c
+__device__ __forceinline__ bool sphere_hit(f3 oc, f3 d, float r, float& tn, float& tf) {
+  const float b = dot(oc, d);
+  const f3 cr = cross(oc, d);
+  const float h2 = r * r - dot(cr, cr);
+  tn = 0.0f;
+  tf = 0.0f;
+  if (h2 < 0.0f) return false;
+  const float h = sqrtf(h2);
+  const float far = -b + h, near = -b - h;
+  if (near < 0.0f && far < 0.0f) return false;
+  tn = near < 0.0f ? 0.0f : near;
+  tf = far;
+  return true;
+}
+
+struct SiteRec {
+  float4 sph;  // centre, radius
+  float4 nsg;  // normal, sigma
+  int site, a0, a1, pad;
+};
+
+struct TexelRec {
+  float sx, sy, sz;  // texel site - centre
+  float r, g, b, h;
+  float pad;
+};
+
+struct FwdParams {
+  RasterCam cam;
+  int n_texels, n_tiles, Q;
+  const float4* spheres;
+  const float4* nsigmas;
+  const float* texel_sites;
+  const float4* texel_rgbh;
+  const int32_t* adjacency;
+  const int32_t* adj_off;
+  const uint16_t* adj_diff;
+  const int64_t* offsets;
+  const int32_t* sites;
+  const float* depth_q;
+  const float* ray_gt;
+  float thr, inv_hw;
+  int point_err_on, loss_acc;
+  int32_t* early;
+  float* color;
+  float* log_t;
+  float* ndist;
+  float* normal;
+  float* qdepth;
+  float* err;
+  float* contrib;
+  float* point_err;
+};
+
+// Face (n, k) of a site against neighbour e: n = delta, and for a ray from o
+// the crossing is t = (dot(c - o, n) + k) / dot(d, n) with
+// k = (|n|^2 + r^2 - r_a^2) / 2. With no encoded deltas the neighbour's exact
+// sphere is used.
+__device__ __forceinline__ float4 face_of(const int32_t* adjacency, const uint16_t* diff, const float4* spheres,
+                                          int64_t e, float4 sph) {
+  f3 n;
+  float ra;
+  if (diff) {
+    const uint2 raw = *reinterpret_cast<const uint2*>(diff + 4 * e);
+    n.x = half_bits_to_float((uint16_t)(raw.x & 0xffff));
+    n.y = half_bits_to_float((uint16_t)(raw.x >> 16));
+    n.z = half_bits_to_float((uint16_t)(raw.y & 0xffff));
+    ra = half_bits_to_float((uint16_t)(raw.y >> 16));
+  } else {
+    const float4 a = spheres[adjacency[e]];
+    n = f3{a.x - sph.x, a.y - sph.y, a.z - sph.z};
+    ra = a.w;
+  }
+  const float k = 0.5f * (dot(n, n) + sph.w * sph.w - ra * ra);
+  return make_float4(n.x, n.y, n.z, k);
+}
+
+template <bool PINHOLE>
+__global__ void __launch_bounds__(WAVE) raster_fwd_kernel(FwdParams p) {
+  __shared__ SiteRec s_site[WAVE];
+  __shared__ float4 s_face[WAVE];
+  __shared__ TexelRec s_tex[PF_MAX_TEXELS];
+
+  const int b = blockIdx.x;
+  if (!xcd_valid(b, p.n_tiles)) return;
+  const int tile = xcd_tile(b, p.n_tiles);
+  const int lane = threadIdx.x;
+  const Ray ray = lane_ray<PINHOLE>(p.cam, tile, lane);
+  const size_t px = (size_t)ray.pi * p.cam.W + ray.pj;
+  const int T = p.n_texels;
+
+  f3 gt{0.0f, 0.0f, 0.0f};
+  if (p.ray_gt) gt = ld3(p.ray_gt + 3 * px);
+  int qidx = 0;
+  float qcur = 0.0f;
+  if (p.Q > 0) qcur = p.depth_q[px * p.Q];
+
+  float log_t = 0.0f, nd = 0.0f, err = 0.0f;
+  f3 rgb{0.0f, 0.0f, 0.0f}, nacc{0.0f, 0.0f, 0.0f};
+
+  const int64_t start = p.offsets[tile];
+  const int total = (int)(p.offsets[tile + 1] - start);
+  int early = total;
+  bool stop = false;
+
+  for (int base = 0; base < total && !stop; base += WAVE) {
+    const int cnt = min(WAVE, total - base);
+    __syncthreads();
+    if (lane < cnt) {
+      const int s = p.sites[start + base + lane];
+      SiteRec r;
+      r.sph = p.spheres[s];
+      r.nsg = p.nsigmas[s];
+      r.site = s;
+      r.a0 = p.adj_off[s];
+      r.a1 = p.adj_off[s + 1];
+      r.pad = 0;
+      s_site[lane] = r;
+    }
+    __syncthreads();
+    for (int kk = 0; kk < cnt; kk++) {
+      const float trans = exp2f(log_t * LOG2E);
+      if (ballot(trans >= p.thr) == 0) {
+        early = base + kk;
+        stop = true;
+        break;
+      }
+      const SiteRec sr = s_site[kk];
+      const f3 c{sr.sph.x, sr.sph.y, sr.sph.z};
+      const float r = sr.sph.w;
+      const f3 pn{sr.nsg.x, sr.nsg.y, sr.nsg.z};
+      const float sigma = sr.nsg.w;
+      const f3 oc = ray.o - c;
+      float tn, tf;
+      const bool hit = sphere_hit(oc, ray.d, r, tn, tf);
+      if (ballot(hit) == 0) continue;
+
+      // power faces (rasterize.py:1038-1071), staged 64 at a time
+      const f3 co = -oc;
+      for (int e0 = sr.a0; e0 < sr.a1; e0 += WAVE) {
+        const int ec = min(WAVE, sr.a1 - e0);
+        __syncthreads();
+        if (lane < ec) s_face[lane] = face_of(p.adjacency, p.adj_diff, p.spheres, e0 + lane, sr.sph);
+        __syncthreads();
+        if (hit) {
+          for (int e = 0; e < ec; e++) {
+            const float4 f = s_face[e];
+            const f3 n{f.x, f.y, f.z};
+            const float dp = dot(ray.d, n);
+            const float t = (dot(co, n) + f.w) / dp;
+            tf = dp >= 0.0f ? fminf(t, tf) : tf;
+            tn = dp < 0.0f ? fmaxf(t, tn) : tn;
+          }
+        }
+      }
+      const bool shade = hit && !ray.oob && tn < tf;
+      if (ballot(shade) == 0) continue;
+
+      // texel records of this site (texture.py:23-24)
+      __syncthreads();
+      if (lane < T) {
+        const float* ts = p.texel_sites + ((size_t)sr.site * T + lane) * 3;
+        const float4 v = p.texel_rgbh[(size_t)sr.site * T + lane];
+        TexelRec tr;
+        tr.sx = ts[0] - c.x;
+        tr.sy = ts[1] - c.y;
+        tr.sz = ts[2] - c.z;
+        tr.r = v.x; tr.g = v.y; tr.b = v.z; tr.h = v.w;
+        tr.pad = 0.0f;
+        s_tex[lane] = tr;
+      }
+      __syncthreads();
+      float alpha_t = 0.0f, perr = 0.0f;
+      if (shade) {
+        // plane_intersection_fwd (texture.py:10-71)
+        const float dpn = dot(pn, ray.d);
+        const float cn = dot(co, pn);
+        const float kexp = -10.0f * LOG2E / (r * r);
+        const float ts0 = cn / dpn;
+        const float tq0 = dpn >= 0.0f ? tn : fmaxf(tn, ts0);
+        const f3 x0 = ray.d * tq0 - co;  // query point relative to the centre
+        float hsum = 0.0f, ws0 = 0.0f;
+        for (int i = 0; i < T; i++) {
+          const TexelRec tr = s_tex[i];
+          const f3 df{x0.x - tr.sx, x0.y - tr.sy, x0.z - tr.sz};
+          const float w = exp2f(dot(df, df) * kexp);
+          hsum += w * tr.h;
+          ws0 += w;
+        }
+        ws0 = fmaxf(ws0, 1e-20f);
+        const float height = hsum / ws0;
+        const float ts1 = (cn + height) / dpn;
+        const float tq = dpn >= 0.0f ? tn : fmaxf(tn, ts1);
+        const f3 x1 = ray.d * tq - co;
+        f3 rs{0.0f, 0.0f, 0.0f};
+        float ws = 0.0f;
+        for (int i = 0; i < T; i++) {
+          const TexelRec tr = s_tex[i];
+          const f3 df{x1.x - tr.sx, x1.y - tr.sy, x1.z - tr.sz};
+          const float w = exp2f(dot(df, df) * kexp);
+          rs = rs + f3{tr.r, tr.g, tr.b} * w;
+          ws += w;
+        }
+        ws = fmaxf(ws, 1e-20f);
+        const f3 col = rs * (1.0f / ws);
+        float tf2 = dpn >= 0.0f ? fminf(ts1, tf) : tf;
+        float tn2 = dpn < 0.0f ? fmaxf(ts1, tn) : tn;
+        const float dt = tf2 - tn2;
+        if (dt > 0.0f) {
+          const float dlt = -sigma * dt;
+          const float alpha = 1.0f - exp2f(dlt * LOG2E);
+          const float at = alpha * trans;
+          if (p.loss_acc) {
+            const f3 df = col - gt;
+            err += dot(df, df) * at;
+          }
+          rgb = rgb + col * at;
+          log_t += dlt;
+          alpha_t = at;
+          if (p.point_err_on) {
+            const f3 df = col - gt;
+            perr = at * (fabsf(df.x) + fabsf(df.y) + fabsf(df.z));
+          }
+          const float ndv = dot(pn, ray.d);
+          if (ndv > 0.0f) nd += (ndv * ndv) * at;
+          nacc = nacc + pn * at;
+          if (p.Q > 0) {
+            const float next_t = exp2f(log_t * LOG2E);
+            while (qidx < p.Q && next_t < qcur) {
+              p.qdepth[px * p.Q + qidx] = tn2 + logf(trans / qcur) / sigma;
+              qidx++;
+              if (qidx < p.Q) qcur = p.depth_q[px * p.Q + qidx];
+            }
+          }
+        }
+      }
+      // contrib / point_err: one atomic per site per wave (rasterize.py:1101-1110)
+      if (ballot(alpha_t > 0.0f) != 0) {
+        const float sa = wave_sum(alpha_t);
+        if (lane == 0) atomicAdd(&p.contrib[sr.site], sa * p.inv_hw);
+        if (p.point_err_on) {
+          const float se = wave_sum(perr);
+          if (lane == 0) atomicAdd(&p.point_err[sr.site], se * p.inv_hw);
+        }
+      }
+    }
+  }
+
+  if (lane == 0) p.early[tile] = early;
+  if (!ray.oob) {
+    p.color[3 * px + 0] = rgb.x;
+    p.color[3 * px + 1] = rgb.y;
+    p.color[3 * px + 2] = rgb.z;
+    p.log_t[px] = log_t;
+    p.ndist[px] = nd;
+    p.normal[3 * px + 0] = nacc.x;
+    p.normal[3 * px + 1] = nacc.y;
+    p.normal[3 * px + 2] = nacc.z;
+    if (p.loss_acc && p.err) p.err[px] = err + dot(gt, gt) * exp2f(log_t * LOG2E);
+  }
+}
+
+// ----------------------------------------------------------------- backward
+
+struct BwdParams {
+  RasterCam cam;
+  int n_texels, n_tiles, Q, exact;
+  const float4* spheres;
+  const float4* nsigmas;
+  const float* texel_sites;
+  const float4* texel_rgbh;
+  const int32_t* adjacency;
+  const int32_t* adj_off;
+  const uint16_t* adj_diff;
+  const int64_t* offsets;
+  const int32_t* sites;
+  const float* depth_q;
+  const int32_t* early;
+  const float* log_t;
+  const float* g_color;
+  const float* g_log_t;
+  const float* g_ndist;
+  const float* g_normal;
+  const float* g_qdepth;
+  const float* g_contrib;
+  float inv_hw;
+  float* g_spheres;
+  float* g_nsigmas;
+  float* g_tsites;
+  float* g_trgb;
+  float* g_theight;
+};
+
+constexpr int RED_STRIDE = 57;  // odd row stride: conflict-free LDS transpose
+
+// plane_intersection_fwd recomputed with its intermediates (texture.py:10-71)
+struct PlaneFwd {
+  float ts0, height, ws0, ts1, ws, dpn;
+  f3 col;
+};
+
+template <bool PINHOLE>
+__global__ void __launch_bounds__(WAVE) raster_bwd_kernel(BwdParams p) {
+  __shared__ SiteRec s_site[WAVE];
+  __shared__ float4 s_face[WAVE];
+  __shared__ TexelRec s_tex[PF_MAX_TEXELS];
+  __shared__ float s_fgrad[WAVE * 4];
+  __shared__ float s_red[WAVE * RED_STRIDE];
+
+  const int b = blockIdx.x;
+  if (!xcd_valid(b, p.n_tiles)) return;
+  const int tile = xcd_tile(b, p.n_tiles);
+  const int lane = threadIdx.x;
+  const Ray ray = lane_ray<PINHOLE>(p.cam, tile, lane);
+  const size_t px = (size_t)ray.pi * p.cam.W + ray.pj;
+  const int T = p.n_texels;
+  const bool exact = p.exact != 0;
+  const float pl_eps = exact ? 0.0f : 1e-6f;
+  const float cut = exact ? 0.0f : 1e-3f;
+  const float q_eps = exact ? 0.0f : 1e-6f;
+
+  float log_t = p.log_t[px];
+  float dLdlog_t = p.g_log_t[px];  // black background: no colour term
+  const f3 g_rgb = ld3(p.g_color + 3 * px);
+  const f3 g_n = ld3(p.g_normal + 3 * px);
+  const float g_nd = p.g_ndist[px];
+  int qidx = p.Q - 1;
+  float qcur = p.Q > 0 ? p.depth_q[px * p.Q + qidx] : 1.0f;
+
+  const int64_t start = p.offsets[tile];
+  const int early = p.early[tile];
+
+  for (int top = early; top > 0; top -= WAVE) {
+    const int cnt = min(WAVE, top);
+    const int base = top - cnt;  // batch covers [base, top)
+    __syncthreads();
+    if (lane < cnt) {
+      const int s = p.sites[start + base + lane];
+      SiteRec r;
+      r.sph = p.spheres[s];
+      r.nsg = p.nsigmas[s];
+      r.site = s;
+      r.a0 = p.adj_off[s];
+      r.a1 = p.adj_off[s + 1];
+      r.pad = 0;
+      s_site[lane] = r;
+    }
+    __syncthreads();
+    for (int kk = cnt - 1; kk >= 0; kk--) {
+      const SiteRec sr = s_site[kk];
+      const f3 c{sr.sph.x, sr.sph.y, sr.sph.z};
+      const float r = sr.sph.w;
+      const f3 pn{sr.nsg.x, sr.nsg.y, sr.nsg.z};
+      const float sigma = sr.nsg.w;
+      const f3 oc = ray.o - c;
+      const f3 co = -oc;
+      float tn, tf;
+      const bool hit = sphere_hit(oc, ray.d, r, tn, tf);
+      if (ballot(hit) == 0) continue;
+      const int n_adj = sr.a1 - sr.a0;
+      // id: -2 none, -1 sphere, 0..n-1 face, n plane (rasterize.py:1296)
+      int tn_id = tn < 1e-8f ? -2 : -1;
+      int tf_id = tf < 1e-8f ? -2 : -1;
+      for (int e0 = sr.a0; e0 < sr.a1; e0 += WAVE) {
+        const int ec = min(WAVE, sr.a1 - e0);
+        __syncthreads();
+        if (lane < ec) s_face[lane] = face_of(p.adjacency, p.adj_diff, p.spheres, e0 + lane, sr.sph);
+        __syncthreads();
+        if (hit) {
+          for (int e = 0; e < ec; e++) {
+            const float4 f = s_face[e];
+            const f3 n{f.x, f.y, f.z};
+            const float dp = dot(ray.d, n);
+            const float t = (dot(co, n) + f.w) / dp;
+            const int id = e0 - sr.a0 + e;
+            if (dp >= 0.0f && t < tf) tf_id = id;
+            tf = dp >= 0.0f ? fminf(t, tf) : tf;
+            if (dp < 0.0f && t > tn) tn_id = id;
+            tn = dp < 0.0f ? fmaxf(t, tn) : tn;
+          }
+        }
+      }
+      const bool shade = hit && !ray.oob && tn < tf;
+      if (ballot(shade) == 0) continue;
+      __syncthreads();
+      if (lane < T) {
+        const float* ts = p.texel_sites + ((size_t)sr.site * T + lane) * 3;
+        const float4 v = p.texel_rgbh[(size_t)sr.site * T + lane];
+        TexelRec tr;
+        tr.sx = ts[0] - c.x;
+        tr.sy = ts[1] - c.y;
+        tr.sz = ts[2] - c.z;
+        tr.r = v.x; tr.g = v.y; tr.b = v.z; tr.h = v.w;
+        tr.pad = 0.0f;
+        s_tex[lane] = tr;
+      }
+      const float kexp = -10.0f * LOG2E / (r * r);
+      const float tn0 = tn;
+      const int tn0_id = tn_id, tf0_id = tf_id;
+      PlaneFwd pf{};
+      float dt = -1.0f;
+      float cn = 0.0f;
+      __syncthreads();
+      if (shade) {
+        pf.dpn = dot(pn, ray.d);
+        cn = dot(co, pn);
+        pf.ts0 = cn / pf.dpn;
+        const float tq0 = pf.dpn >= 0.0f ? tn : fmaxf(tn, pf.ts0);
+        const f3 x0 = ray.d * tq0 - co;
+        float hsum = 0.0f, ws0 = 0.0f;
+        for (int i = 0; i < T; i++) {
+          const TexelRec tr = s_tex[i];
+          const f3 df{x0.x - tr.sx, x0.y - tr.sy, x0.z - tr.sz};
+          const float w = exp2f(dot(df, df) * kexp);
+          hsum += w * tr.h;
+          ws0 += w;
+        }
+        pf.ws0 = fmaxf(ws0, 1e-20f);
+        pf.height = hsum / pf.ws0;
+        pf.ts1 = (cn + pf.height) / pf.dpn;
+        const float tq = pf.dpn >= 0.0f ? tn : fmaxf(tn, pf.ts1);
+        const f3 x1 = ray.d * tq - co;
+        f3 rs{0.0f, 0.0f, 0.0f};
+        float ws = 0.0f;
+        for (int i = 0; i < T; i++) {
+          const TexelRec tr = s_tex[i];
+          const f3 df{x1.x - tr.sx, x1.y - tr.sy, x1.z - tr.sz};
+          const float w = exp2f(dot(df, df) * kexp);
+          rs = rs + f3{tr.r, tr.g, tr.b} * w;
+          ws += w;
+        }
+        pf.ws = fmaxf(ws, 1e-20f);
+        pf.col = rs * (1.0f / pf.ws);
+        if (pf.dpn >= 0.0f && pf.ts1 < tf) tf_id = n_adj;
+        tf = pf.dpn >= 0.0f ? fminf(pf.ts1, tf) : tf;
+        if (pf.dpn < 0.0f && pf.ts1 > tn) tn_id = n_adj;
+        tn = pf.dpn < 0.0f ? fmaxf(pf.ts1, tn) : tn;
+        dt = tf - tn;
+      }
+      const bool active = shade && dt > 0.0f;
+      if (ballot(active) == 0) continue;
+
+      // ---- compositing adjoint (rasterize.py:1366-1430)
+      float dLdtn = 0.0f, dLdtf = 0.0f, dLdr = 0.0f, dLdsig = 0.0f;
+      f3 dLdc{0.0f, 0.0f, 0.0f}, dLdnorm{0.0f, 0.0f, 0.0f}, dLdcol{0.0f, 0.0f, 0.0f};
+      float g_tex[PF_MAX_TEXELS][7];
+#pragma unroll
+      for (int i = 0; i < PF_MAX_TEXELS; i++)
+#pragma unroll
+        for (int q = 0; q < 7; q++) g_tex[i][q] = 0.0f;
+      int fid0 = -1, fid1 = -1;
+      float fg0[4] = {0, 0, 0, 0}, fg1[4] = {0, 0, 0, 0};
+      if (active) {
+        const float dlt = -sigma * dt;
+        log_t -= dlt;
+        const float e_dlt = exp2f(dlt * LOG2E);
+        const float alpha = 1.0f - e_dlt;
+        const float trans = exp2f(log_t * LOG2E);
+        const float ndv = dot(pn, ray.d);
+        float dLdlog_t_i = dLdlog_t;
+        float dLddlt = dLdlog_t;
+        const float gc = p.g_contrib[sr.site];
+        const float cdot = dot(g_rgb, pf.col);
+        const float ndt = exact ? ndv * ndv : ndv;
+        const float ndot = dot(g_n, pn);
+        float dLdalpha = cdot * trans;
+        if (ndv > 0.0f) dLdalpha += g_nd * ndt * trans;
+        dLdalpha += gc * trans * p.inv_hw;
+        dLdalpha += ndot * trans;
+        float dLdtrans = cdot * alpha;
+        if (ndv > 0.0f) dLdtrans += g_nd * ndt * alpha;
+        dLdtrans += gc * alpha * p.inv_hw;
+        dLdtrans += ndot * alpha;
+        dLdcol = g_rgb * (trans * alpha);
+        dLdlog_t_i += dLdtrans * trans;
+        dLddlt += -dLdalpha * e_dlt;
+        dLdsig += dLddlt * -dt;
+        dLdtn += dLddlt * sigma;
+        dLdtf += dLddlt * -sigma;
+        while (qidx >= 0 && trans > qcur) {
+          const float g = p.g_qdepth[px * p.Q + qidx];
+          dLdsig += -g * logf(trans / qcur) / (sigma * sigma + q_eps);
+          dLdtn += g;
+          dLdlog_t_i += g / (sigma + q_eps);
+          qidx--;
+          if (qidx >= 0) qcur = p.depth_q[px * p.Q + qidx];
+        }
+        dLdnorm = dLdnorm + g_n * (alpha * trans);
+        if (ndv > 0.0f) dLdnorm = dLdnorm + ray.d * (g_nd * alpha * trans * (exact ? 2.0f * ndv : 1.0f));
+        dLdlog_t = dLdlog_t_i;
+
+        // ---- plane_intersection_bwd (texture.py:74-186)
+        const float dLdts_in = (tn_id == n_adj) ? dLdtn : ((tf_id == n_adj) ? dLdtf : 0.0f);
+        const float r2 = r * r;
+        const float kw = 20.0f / r2;
+        float dLdheight = 0.0f;
+        f3 pdc{0.0f, 0.0f, 0.0f}, pdn{0.0f, 0.0f, 0.0f};
+        {
+          const float tq = pf.dpn >= 0.0f ? tn0 : fmaxf(tn0, pf.ts1);
+          const f3 x1 = ray.d * tq - co;
+          f3 dLdx{0.0f, 0.0f, 0.0f};
+          const float gdp_in = dot(dLdcol, pf.col);
+          const float inv_ws = 1.0f / pf.ws;
+#pragma unroll
+          for (int i = 0; i < PF_MAX_TEXELS; i++) {
+            if (i < T) {
+              const TexelRec tr = s_tex[i];
+              const f3 df{x1.x - tr.sx, x1.y - tr.sy, x1.z - tr.sz};
+              const float w = exp2f(dot(df, df) * kexp);
+              if (!(w * inv_ws < cut)) {
+                const float wn = w * inv_ws;
+                g_tex[i][3] += dLdcol.x * wn;
+                g_tex[i][4] += dLdcol.y * wn;
+                g_tex[i][5] += dLdcol.z * wn;
+                const float gdp = dLdcol.x * tr.r + dLdcol.y * tr.g + dLdcol.z * tr.b;
+                const float sgs = kw * w * (gdp - gdp_in) * inv_ws;
+                const f3 sg = df * sgs;
+                g_tex[i][0] += sg.x;
+                g_tex[i][1] += sg.y;
+                g_tex[i][2] += sg.z;
+                dLdx = dLdx - sg;
+              }
+            }
+          }
+          const float dLdtq = dot(dLdx, ray.d);
+          float dtn_q = (pf.dpn >= 0.0f || tn0 >= pf.ts1) ? dLdtq : 0.0f;
+          float dts = (pf.dpn < 0.0f && tn0 < pf.ts1) ? dLdtq : 0.0f;
+          dts += dLdts_in;
+          if (fabsf(dts) > 0.0f) {
+            // ray_plane_intersect_bwd with p = c, h = height (rendering_math.py:138-147)
+            const float den = pf.dpn + pl_eps;
+            pdc = pdc + pn * (dts / den);
+            const float num = cn + pf.height;
+            pdn = pdn + (co * (1.0f / den) - ray.d * (num / (pf.dpn * pf.dpn + pl_eps))) * dts;
+            dLdheight += dts / den;
+          }
+          // height stage
+          const float tq0 = pf.dpn >= 0.0f ? tn0 : fmaxf(tn0, pf.ts0);
+          const f3 x0 = ray.d * tq0 - co;
+          f3 dLdx0{0.0f, 0.0f, 0.0f};
+          const float gdp0_in = dLdheight * pf.height;
+          const float inv_ws0 = 1.0f / pf.ws0;
+#pragma unroll
+          for (int i = 0; i < PF_MAX_TEXELS; i++) {
+            if (i < T) {
+              const TexelRec tr = s_tex[i];
+              const f3 df{x0.x - tr.sx, x0.y - tr.sy, x0.z - tr.sz};
+              const float w = exp2f(dot(df, df) * kexp);
+              // the cut-off tests the colour-stage weight sum, as the reference
+              if (!(w * inv_ws < cut)) {
+                g_tex[i][6] += w * dLdheight * inv_ws0;
+                const float sgs = kw * w * (dLdheight * tr.h - gdp0_in) * inv_ws0;
+                const f3 sg = df * sgs;
+                g_tex[i][0] += sg.x;
+                g_tex[i][1] += sg.y;
+                g_tex[i][2] += sg.z;
+                dLdx0 = dLdx0 - sg;
+              }
+            }
+          }
+          const float dLdtq0 = dot(dLdx0, ray.d);
+          dtn_q += (pf.dpn >= 0.0f || tn0 >= pf.ts0) ? dLdtq0 : 0.0f;
+          const float dts0 = (pf.dpn < 0.0f && tn0 < pf.ts0) ? dLdtq0 : 0.0f;
+          if (fabsf(dts0) > 0.0f) {
+            const float den = pf.dpn + pl_eps;
+            pdc = pdc + pn * (dts0 / den);
+            pdn = pdn + (co * (1.0f / den) - ray.d * (cn / (pf.dpn * pf.dpn + pl_eps))) * dts0;
+          }
+          dLdc = dLdc + pdc;
+          dLdnorm = dLdnorm + pdn;
+          if (tn_id != n_adj) dtn_q += dLdtn;
+          dLdtn = dtn_q;
+        }
+        // ---- route dL/dt_near and dL/dt_far to sphere / faces (rasterize.py:1464-1505)
+#pragma unroll
+        for (int side = 0; side < 2; side++) {
+          const int tid = side == 0 ? tn0_id : tf0_id;
+          const float dLdt = side == 0 ? dLdtn : dLdtf;
+          if (exact && side == 1 && tf_id == n_adj) continue;
+          if (tid == -1) {
+            // ray_sphere_intersect_bwd (rendering_math.py:26-59), |d| = 1
+            const float qa = dot(ray.d, ray.d);
+            const float qb = 2.0f * dot(oc, ray.d);
+            const f3 cr = cross(oc, ray.d);
+            const float disc = 4.0f * (qa * r * r - dot(cr, cr));
+            if (disc >= 0.0f) {
+              const float sq = sqrtf(disc);
+              const float tfar = (-qb + sq) / (2.0f * qa);
+              const float tnear = (-qb - sq) / (2.0f * qa);
+              if (!(tnear < 0.0f && tfar < 0.0f)) {
+                const f3 w = (oc * (2.0f * qa) - ray.d * qb) * (1.0f / (qa * sq));
+                const f3 dq = ray.d * (1.0f / qa);
+                if (side == 0) {
+                  if (!(tnear < 0.0f)) {
+                    dLdc = dLdc + (dq - w) * dLdt;
+                    dLdr += dLdt * (-2.0f * r / sq);
+                  }
+                } else {
+                  dLdc = dLdc + (dq + w) * dLdt;
+                  dLdr += dLdt * (2.0f * r / sq);
+                }
+              }
+            }
+          } else if (tid >= 0 && tid < n_adj) {
+            // ray_pface_intersect_bwd (rendering_math.py:101-125) with the
+            // neighbour's full-precision sphere
+            const int adj = p.adjacency[sr.a0 + tid];
+            const float4 as = p.spheres[adj];
+            const f3 a{as.x, as.y, as.z};
+            const f3 n = a - c;
+            const float num = dot(co, n) + 0.5f * (dot(n, n) + r * r - as.w * as.w);
+            const float dp = dot(ray.d, n);
+            const float idp = 1.0f / dp;
+            const f3 qv = ray.d * (num * idp * idp);
+            dLdc = dLdc + (oc * idp + qv) * dLdt;
+            dLdr += dLdt * (r * idp);
+            const f3 ga = ((a - ray.o) * idp - qv) * dLdt;
+            const float gra = dLdt * (-as.w * idp);
+            if (side == 0) {
+              fid0 = tid; fg0[0] = ga.x; fg0[1] = ga.y; fg0[2] = ga.z; fg0[3] = gra;
+            } else {
+              fid1 = tid; fg1[0] = ga.x; fg1[1] = ga.y; fg1[2] = ga.z; fg1[3] = gra;
+            }
+          }
+        }
+      }
+
+      // ---- neighbour gradients: gather per face slot in LDS, then flush
+      if (n_adj <= WAVE) {
+        __syncthreads();
+        for (int q = lane; q < WAVE * 4; q += WAVE) s_fgrad[q] = 0.0f;
+        __syncthreads();
+        if (fid0 >= 0) {
+#pragma unroll
+          for (int q = 0; q < 4; q++) atomicAdd(&s_fgrad[fid0 * 4 + q], fg0[q]);
+        }
+        if (fid1 >= 0) {
+#pragma unroll
+          for (int q = 0; q < 4; q++) atomicAdd(&s_fgrad[fid1 * 4 + q], fg1[q]);
+        }
+        __syncthreads();
+        if (lane < n_adj) {
+          const float4 v = make_float4(s_fgrad[lane * 4 + 0], s_fgrad[lane * 4 + 1], s_fgrad[lane * 4 + 2],
+                                       s_fgrad[lane * 4 + 3]);
+          if (v.x != 0.0f || v.y != 0.0f || v.z != 0.0f || v.w != 0.0f) {
+            float* g = p.g_spheres + 4 * (size_t)p.adjacency[sr.a0 + lane];
+            atomicAdd(g + 0, v.x);
+            atomicAdd(g + 1, v.y);
+            atomicAdd(g + 2, v.z);
+            atomicAdd(g + 3, v.w);
+          }
+        }
+      } else {
+        if (fid0 >= 0) {
+          float* g = p.g_spheres + 4 * (size_t)p.adjacency[sr.a0 + fid0];
+#pragma unroll
+          for (int q = 0; q < 4; q++) atomicAdd(g + q, fg0[q]);
+        }
+        if (fid1 >= 0) {
+          float* g = p.g_spheres + 4 * (size_t)p.adjacency[sr.a0 + fid1];
+#pragma unroll
+          for (int q = 0; q < 4; q++) atomicAdd(g + q, fg1[q]);
+        }
+      }
+
+      // ---- site and texel gradients: LDS transpose, one lane per value
+      {
+        float* row = s_red + lane * RED_STRIDE;
+        row[0] = dLdc.x;
+        row[1] = dLdc.y;
+        row[2] = dLdc.z;
+        row[3] = dLdr;
+        row[4] = dLdnorm.x;
+        row[5] = dLdnorm.y;
+        row[6] = dLdnorm.z;
+        row[7] = dLdsig;
+#pragma unroll
+        for (int i = 0; i < PF_MAX_TEXELS; i++) {
+          if (i < T) {
+#pragma unroll
+            for (int q = 0; q < 7; q++) row[8 + i * 7 + q] = g_tex[i][q];
+          }
+        }
+        __syncthreads();
+        const int nv = 8 + 7 * T;
+        if (lane < nv) {
+          float s0 = 0.0f, s1 = 0.0f;
+          for (int l = 0; l < WAVE; l += 2) {
+            s0 += s_red[l * RED_STRIDE + lane];
+            s1 += s_red[(l + 1) * RED_STRIDE + lane];
+          }
+          const float v = s0 + s1;
+          if (v != 0.0f) {
+            float* dst;
+            if (lane < 4) dst = p.g_spheres + 4 * (size_t)sr.site + lane;
+            else if (lane < 8) dst = p.g_nsigmas + 4 * (size_t)sr.site + (lane - 4);
+            else {
+              const int i = (lane - 8) / 7, q = (lane - 8) % 7;
+              if (q < 3) dst = p.g_tsites + ((size_t)sr.site * T + i) * 3 + q;
+              else if (q < 6) dst = p.g_trgb + ((size_t)sr.site * T + i) * 3 + (q - 3);
+              else dst = p.g_theight + (size_t)sr.site * T + i;
+            }
+            atomicAdd(dst, v);
+          }
+        }
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) scrub_kernel(float* x, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && !isfinite(x[i])) x[i] = 0.0f;
+}
+
+}  // namespace pf
+
+using namespace pf;
+
+static int check_scene(const pf_scene& s, const char* who) {
+  if (s.n_sites < 0 || (s.n_sites > 0 && (!s.spheres || !s.nsigmas || !s.texel_sites || !s.texel_rgbh ||
+                                          !s.adjacency_offsets)))
+    return set_error(PF_ERR_ARG, "%s: missing scene arrays", who);
+  if (s.n_texels < 1 || s.n_texels > PF_MAX_TEXELS)
+    return set_error(PF_ERR_ARG, "%s: n_texels=%d outside 1..%d", who, s.n_texels, PF_MAX_TEXELS);
+  return PF_OK;
+}
+
+extern "C" {
+
+int pf_raster_fwd(const pf_raster_fwd_args* a, void* stream) {
+  if (!a) return set_error(PF_ERR_ARG, "pf_raster_fwd: null args");
+  int st = check_scene(a->scene, "pf_raster_fwd");
+  if (st) return st;
+  const pf_camera& c = a->cam;
+  if (c.width < 2 || c.height < 2) return set_error(PF_ERR_ARG, "pf_raster_fwd: image must be >= 2x2");
+  const int nt = tiles_h(c) * tiles_w(c);
+  if (a->tiles.n_tiles != nt || !a->tiles.offsets)
+    return set_error(PF_ERR_ARG, "pf_raster_fwd: tile lists do not match the camera (%d vs %d)", a->tiles.n_tiles, nt);
+  if (!a->tile_early_stop || !a->color || !a->log_t || !a->normal_distance || !a->normal || !a->contrib)
+    return set_error(PF_ERR_ARG, "pf_raster_fwd: missing outputs");
+  if ((a->return_point_err || a->loss_accumulation) && !a->ray_gt)
+    return set_error(PF_ERR_ARG, "pf_raster_fwd: ray_gt is required for point error / surface loss");
+  if (a->return_point_err && !a->point_err) return set_error(PF_ERR_ARG, "pf_raster_fwd: missing point_err");
+  if (a->depth_quantiles && (a->n_quantiles < 1 || !a->quantile_depths))
+    return set_error(PF_ERR_ARG, "pf_raster_fwd: quantiles need n_quantiles >= 1 and an output");
+  FwdParams p{};
+  p.cam = make_rcam(c);
+  p.n_texels = a->scene.n_texels;
+  p.n_tiles = nt;
+  p.Q = a->depth_quantiles ? a->n_quantiles : 0;
+  p.spheres = reinterpret_cast<const float4*>(a->scene.spheres);
+  p.nsigmas = reinterpret_cast<const float4*>(a->scene.nsigmas);
+  p.texel_sites = a->scene.texel_sites;
+  p.texel_rgbh = reinterpret_cast<const float4*>(a->scene.texel_rgbh);
+  p.adjacency = a->scene.adjacency;
+  p.adj_off = a->scene.adjacency_offsets;
+  p.adj_diff = a->scene.adjacency_diff;
+  p.offsets = a->tiles.offsets;
+  p.sites = a->tiles.sites;
+  p.depth_q = a->depth_quantiles;
+  p.ray_gt = a->ray_gt;
+  p.thr = a->transmittance_threshold;
+  p.inv_hw = 1.0f / (float)(c.height * c.width);
+  p.point_err_on = a->return_point_err;
+  p.loss_acc = a->loss_accumulation;
+  p.early = a->tile_early_stop;
+  p.color = a->color;
+  p.log_t = a->log_t;
+  p.ndist = a->normal_distance;
+  p.normal = a->normal;
+  p.qdepth = a->quantile_depths;
+  p.err = a->err;
+  p.contrib = a->contrib;
+  p.point_err = a->point_err;
+  const int grid = ((nt + 7) / 8) * 8;
+  if (c.ray_maps)
+    raster_fwd_kernel<false><<<grid, WAVE, 0, as_stream(stream)>>>(p);
+  else
+    raster_fwd_kernel<true><<<grid, WAVE, 0, as_stream(stream)>>>(p);
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+int pf_raster_bwd(const pf_raster_bwd_args* a, void* stream) {
+  if (!a) return set_error(PF_ERR_ARG, "pf_raster_bwd: null args");
+  int st = check_scene(a->scene, "pf_raster_bwd");
+  if (st) return st;
+  const pf_camera& c = a->cam;
+  const int nt = tiles_h(c) * tiles_w(c);
+  if (a->tiles.n_tiles != nt || !a->tiles.offsets || !a->tile_early_stop)
+    return set_error(PF_ERR_ARG, "pf_raster_bwd: tile lists do not match the camera");
+  if (!a->log_t || !a->grad_color || !a->grad_log_t || !a->grad_normal_distance || !a->grad_normal ||
+      !a->grad_contrib)
+    return set_error(PF_ERR_ARG, "pf_raster_bwd: missing incoming gradients");
+  if (!a->grad_spheres || !a->grad_nsigmas || !a->grad_texel_sites || !a->grad_texel_rgb || !a->grad_texel_height)
+    return set_error(PF_ERR_ARG, "pf_raster_bwd: missing gradient outputs");
+  if (a->depth_quantiles && (a->n_quantiles < 1 || !a->grad_quantile_depths))
+    return set_error(PF_ERR_ARG, "pf_raster_bwd: quantiles need their gradient");
+  BwdParams p{};
+  p.cam = make_rcam(c);
+  p.n_texels = a->scene.n_texels;
+  p.n_tiles = nt;
+  p.Q = a->depth_quantiles ? a->n_quantiles : 0;
+  p.exact = a->exact_gradients;
+  p.spheres = reinterpret_cast<const float4*>(a->scene.spheres);
+  p.nsigmas = reinterpret_cast<const float4*>(a->scene.nsigmas);
+  p.texel_sites = a->scene.texel_sites;
+  p.texel_rgbh = reinterpret_cast<const float4*>(a->scene.texel_rgbh);
+  p.adjacency = a->scene.adjacency;
+  p.adj_off = a->scene.adjacency_offsets;
+  p.adj_diff = a->scene.adjacency_diff;
+  p.offsets = a->tiles.offsets;
+  p.sites = a->tiles.sites;
+  p.depth_q = a->depth_quantiles;
+  p.early = a->tile_early_stop;
+  p.log_t = a->log_t;
+  p.g_color = a->grad_color;
+  p.g_log_t = a->grad_log_t;
+  p.g_ndist = a->grad_normal_distance;
+  p.g_normal = a->grad_normal;
+  p.g_qdepth = a->grad_quantile_depths;
+  p.g_contrib = a->grad_contrib;
+  p.inv_hw = 1.0f / (float)(c.height * c.width);
+  p.g_spheres = a->grad_spheres;
+  p.g_nsigmas = a->grad_nsigmas;
+  p.g_tsites = a->grad_texel_sites;
+  p.g_trgb = a->grad_texel_rgb;
+  p.g_theight = a->grad_texel_height;
+  const int grid = ((nt + 7) / 8) * 8;
+  if (c.ray_maps)
+    raster_bwd_kernel<false><<<grid, WAVE, 0, as_stream(stream)>>>(p);
+  else
+    raster_bwd_kernel<true><<<grid, WAVE, 0, as_stream(stream)>>>(p);
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+int pf_scrub_nonfinite(float* data, int64_t n, void* stream) {
+  if (n < 0 || (n > 0 && !data)) return set_error(PF_ERR_ARG, "pf_scrub_nonfinite: bad arguments");
+  if (n == 0) return PF_OK;
+  scrub_kernel<<<(unsigned)((n + 255) / 256), 256, 0, as_stream(stream)>>>(data, n);
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+}  // extern "C"

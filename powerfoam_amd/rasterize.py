@@ -1,0 +1,261 @@
+"""Drop-in replacement for the reference's powerfoam/rasterize.py (Rasterizer,
+RasterGradFn, VisOptions) on hand-written gfx950 kernels.
+
+Same class names, constructor and method signatures, outputs, gradient tuple
+and error behaviour as the reference (rasterize.py:15-65, 1587-1980); all
+launches go through the C-ABI library (powerfoam_amd/_abi.py) on torch's
+current stream. There is no CPU path: without the HIP library every call
+raises.
+"""
+
+from dataclasses import dataclass, field
+
+import torch
+
+from . import _abi
+from ._abi import check, ptr, stream_handle
+
+TILE_WIDTH = 8
+TILE_SIZE = TILE_WIDTH * TILE_WIDTH
+LEAF_SIZE = 2
+
+
+@dataclass
+class VisOptions:
+    """rasterize.py:15-20"""
+    transmittance_threshold: float = 1e-3
+    max_intersections: int = 1024
+    depth_quantile: float = 0.5
+    bkgd_color: tuple = field(default_factory=lambda: (0.0, 0.0, 0.0))
+
+
+class RasterGradFn(torch.autograd.Function):
+    """rasterize.py:23-34"""
+
+    @staticmethod
+    def forward(ctx, rasterizer, *args):
+        return rasterizer._forward(ctx, *args)
+
+    @staticmethod
+    def backward(ctx, *args):
+        return ctx.rasterizer._backward(ctx, *args)
+
+
+def _f32c(t):
+    return t.detach().to(torch.float32).contiguous()
+
+
+class Rasterizer:
+    def __init__(self, args, device, attr_dtype="float"):
+        """rasterize.py:38-65. `args` needs render_objective, num_texel_sites and
+        is_pinhole (the reference's Params fields); disable_coop_* are accepted
+        and ignored (loads are always staged through LDS)."""
+        if attr_dtype == "float":
+            self.tscalar = torch.float32
+        elif attr_dtype == "half":
+            self.tscalar = torch.float16
+        else:
+            raise ValueError(f"Unsupported attribute dtype: {attr_dtype}")
+        self.device = torch.device(device)
+        self.args = args
+        self.attr_dtype = attr_dtype
+        self.loss_accumulation = getattr(args, "render_objective", "volume") == "surface"
+        self.num_texel_sites = int(args.num_texel_sites)
+        if not 1 <= self.num_texel_sites <= _abi.MAX_TEXELS:
+            raise ValueError(f"num_texel_sites must be in 1..{_abi.MAX_TEXELS}")
+        self.lib = _abi.load()
+        # Not in the reference: derivative-check switches. exact_geometry feeds
+        # the faces full-precision neighbour spheres instead of the fp16
+        # encoding; exact_gradients differentiates the forward exactly (see
+        # pf_raster_bwd_args.exact_gradients).
+        self.exact_geometry = False
+        self.exact_gradients = False
+
+    # ------------------------------------------------------------------ binning
+    def bin(self, camera, all_spheres):
+        """count -> scan -> (one host sync for M) -> emit -> per-tile sort."""
+        lib = self.lib
+        if camera.ray_maps is not None:
+            raise NotImplementedError("non-pinhole binning is not built yet")
+        cam = camera.to_pf()
+        th, tw = _abi.tiles_hw(camera.height, camera.width)
+        nt = th * tw
+        n = all_spheres.shape[0]
+        dev = all_spheres.device
+        st = stream_handle()
+        counts = torch.empty(nt + 1, dtype=torch.int32, device=dev)
+        check(lib.pf_bin_count(_ref(cam), ptr(all_spheres), n, ptr(counts), st))
+        offsets = torch.empty(nt + 1, dtype=torch.int64, device=dev)
+        check(lib.pf_bin_scan(ptr(counts), nt, ptr(offsets), st))
+        m = int(offsets[-1].item())
+        records = torch.empty(max(m, 1), dtype=torch.int64, device=dev)
+        cursor = torch.empty(nt, dtype=torch.int32, device=dev)
+        visible = torch.zeros(n, dtype=torch.uint8, device=dev)
+        check(lib.pf_bin_emit(_ref(cam), ptr(all_spheres), n, ptr(offsets), nt, ptr(records), ptr(cursor),
+                              ptr(visible), st))
+        sites = torch.empty(max(m, 1), dtype=torch.int32, device=dev)
+        check(lib.pf_bin_sort(ptr(offsets), nt, ptr(records), ptr(sites), None, 0, st))
+        return offsets, sites[:m] if m > 0 else sites[:0], visible.bool(), nt
+
+    def encode_adjacency(self, all_spheres, adjacency, adjacency_offsets):
+        """prefetch_adjacency_kernel (rasterize.py:580-606): fp16 (c_adj - c, r_adj)."""
+        diff = torch.empty((max(adjacency.shape[0], 1), 4), dtype=torch.float16, device=all_spheres.device)
+        check(self.lib.pf_adj_encode(ptr(all_spheres), all_spheres.shape[0], ptr(adjacency),
+                                     ptr(adjacency_offsets), ptr(diff), stream_handle()))
+        return diff[: adjacency.shape[0]]
+
+    # ------------------------------------------------------------------ forward
+    def _forward(self, grad_ctx, camera, depth_quantiles, points, radii, density, normals, texel_sites,
+                 texel_rgb, texel_height, adjacency, adjacency_offsets, ray_gt, return_point_err,
+                 transmittance_threshold=1e-3):
+        if self.attr_dtype != "float":
+            raise ValueError("Forward pass only supports float precision")
+        if ray_gt is None and self.loss_accumulation:
+            raise ValueError("Ground truth rays must be provided for surface loss accumulation")
+        if ray_gt is None and return_point_err:
+            raise ValueError("Ground truth rays must be provided to return point error")
+        if texel_sites.shape[1] != self.num_texel_sites:
+            raise ValueError("texel_sites does not match args.num_texel_sites")
+        dev = points.device
+        H, W = camera.height, camera.width
+        N = points.shape[0]
+        all_spheres = _f32c(torch.cat([points, radii[:, None]], dim=-1))
+        all_nsigmas = _f32c(torch.cat([normals, density[:, None]], dim=-1))
+        all_texel_sites = _f32c(texel_sites)
+        all_texel_rgbh = _f32c(torch.cat([texel_rgb, texel_height[..., None]], dim=-1))
+        adjacency = adjacency.to(torch.int32).contiguous()
+        adjacency_offsets = adjacency_offsets.to(torch.int32).contiguous()
+        ray_gt_c = _f32c(ray_gt) if ray_gt is not None else None
+        dq = _f32c(depth_quantiles) if depth_quantiles is not None else None
+
+        offsets, tile_sites, prim_visible_mask, nt = self.bin(camera, all_spheres)
+        adjacency_diff = self.encode_adjacency(all_spheres, adjacency, adjacency_offsets)
+
+        f32 = dict(dtype=torch.float32, device=dev)
+        color_out = torch.zeros((H, W, 3), **f32)
+        log_t_out = torch.zeros((H, W), **f32)
+        normal_distance_out = torch.zeros((H, W), **f32)
+        normal_out = torch.zeros((H, W, 3), **f32)
+        contrib_out = torch.zeros(N, **f32)
+        quantile_depths_out = -torch.ones(dq.shape, **f32) if dq is not None else None
+        point_err_out = torch.zeros(N, **f32) if return_point_err else None
+        err_out = torch.zeros((H, W), **f32) if self.loss_accumulation else None
+        tile_early_stop = torch.zeros(nt, dtype=torch.int32, device=dev)
+
+        a = _abi.PfRasterFwdArgs()
+        a.cam = camera.to_pf()
+        a.scene = _scene_struct(all_spheres, all_nsigmas, all_texel_sites, all_texel_rgbh, adjacency,
+                                adjacency_offsets, None if self.exact_geometry else adjacency_diff)
+        a.tiles = _abi.PfTileLists(nt, offsets.data_ptr(), tile_sites.data_ptr() if tile_sites.numel() else None)
+        a.depth_quantiles = dq.data_ptr() if dq is not None else None
+        a.n_quantiles = dq.shape[-1] if dq is not None else 0
+        a.ray_gt = ray_gt_c.data_ptr() if ray_gt_c is not None else None
+        a.transmittance_threshold = float(transmittance_threshold)
+        a.return_point_err = int(bool(return_point_err))
+        a.loss_accumulation = int(self.loss_accumulation)
+        a.tile_early_stop = tile_early_stop.data_ptr()
+        a.color = color_out.data_ptr()
+        a.log_t = log_t_out.data_ptr()
+        a.normal_distance = normal_distance_out.data_ptr()
+        a.normal = normal_out.data_ptr()
+        a.quantile_depths = quantile_depths_out.data_ptr() if quantile_depths_out is not None else None
+        a.err = err_out.data_ptr() if err_out is not None else None
+        a.contrib = contrib_out.data_ptr()
+        a.point_err = point_err_out.data_ptr() if point_err_out is not None else None
+        check(self.lib.pf_raster_fwd(_ref(a), stream_handle()))
+
+        grad_ctx.rasterizer = self
+        grad_ctx.camera = camera
+        grad_ctx.save_for_backward(dq, all_spheres, all_nsigmas, all_texel_sites, all_texel_rgbh, adjacency,
+                                   adjacency_offsets, adjacency_diff, tile_sites, offsets, tile_early_stop,
+                                   ray_gt_c, color_out, log_t_out)
+        opacity_out = 1.0 - torch.exp(log_t_out)
+        return (color_out, opacity_out, normal_distance_out, normal_out, quantile_depths_out, err_out,
+                contrib_out, point_err_out, prim_visible_mask)
+
+    def forward(self, *args):
+        return RasterGradFn.apply(self, *args)
+
+    # ----------------------------------------------------------------- backward
+    def _backward(self, grad_ctx, grad_color_in, grad_opacity_in, grad_normal_distance_in, grad_normal_in,
+                  grad_quantile_depths_in, grad_err_in, grad_contrib_in, grad_point_err_in,
+                  grad_prim_visible_mask_in):
+        del grad_point_err_in, grad_prim_visible_mask_in  # unused, as the reference
+        if self.attr_dtype != "float":
+            raise ValueError("Backward pass only supports float precision")
+        if self.loss_accumulation:
+            raise NotImplementedError()  # rasterize.py:1232-1234
+        (dq, all_spheres, all_nsigmas, all_texel_sites, all_texel_rgbh, adjacency, adjacency_offsets,
+         adjacency_diff, tile_sites, offsets, tile_early_stop, ray_gt, color_out, log_t_out) = grad_ctx.saved_tensors
+        camera = grad_ctx.camera
+        H, W = camera.height, camera.width
+        N = all_spheres.shape[0]
+        T = all_texel_sites.shape[1]
+        dev = all_spheres.device
+        zf = lambda *s: torch.zeros(*s, dtype=torch.float32, device=dev)
+        g = lambda t, shape: _f32c(t) if t is not None else zf(shape)
+        grad_color = g(grad_color_in, (H, W, 3))
+        grad_opacity = g(grad_opacity_in, (H, W))
+        grad_log_t = (-grad_opacity * torch.exp(log_t_out)).contiguous()
+        grad_nd = g(grad_normal_distance_in, (H, W))
+        grad_n = g(grad_normal_in, (H, W, 3))
+        grad_contrib = g(grad_contrib_in, (N,))
+        grad_q = None
+        if dq is not None:
+            grad_q = g(grad_quantile_depths_in, tuple(dq.shape))
+
+        spheres_grad = zf((N, 4))
+        nsigmas_grad = zf((N, 4))
+        texel_sites_grad = zf((N, T, 3))
+        texel_rgb_grad = zf((N, T, 3))
+        texel_height_grad = zf((N, T))
+
+        a = _abi.PfRasterBwdArgs()
+        a.cam = camera.to_pf()
+        a.scene = _scene_struct(all_spheres, all_nsigmas, all_texel_sites, all_texel_rgbh, adjacency,
+                                adjacency_offsets, None if self.exact_geometry else adjacency_diff)
+        a.tiles = _abi.PfTileLists(offsets.shape[0] - 1, offsets.data_ptr(),
+                                   tile_sites.data_ptr() if tile_sites.numel() else None)
+        a.depth_quantiles = dq.data_ptr() if dq is not None else None
+        a.n_quantiles = dq.shape[-1] if dq is not None else 0
+        a.tile_early_stop = tile_early_stop.data_ptr()
+        a.log_t = log_t_out.data_ptr()
+        a.grad_color = grad_color.data_ptr()
+        a.grad_log_t = grad_log_t.data_ptr()
+        a.grad_normal_distance = grad_nd.data_ptr()
+        a.grad_normal = grad_n.data_ptr()
+        a.grad_quantile_depths = grad_q.data_ptr() if grad_q is not None else None
+        a.grad_contrib = grad_contrib.data_ptr()
+        a.exact_gradients = int(bool(self.exact_gradients))
+        a.grad_spheres = spheres_grad.data_ptr()
+        a.grad_nsigmas = nsigmas_grad.data_ptr()
+        a.grad_texel_sites = texel_sites_grad.data_ptr()
+        a.grad_texel_rgb = texel_rgb_grad.data_ptr()
+        a.grad_texel_height = texel_height_grad.data_ptr()
+        st = stream_handle()
+        check(self.lib.pf_raster_bwd(_ref(a), st))
+        for t in (spheres_grad, nsigmas_grad, texel_sites_grad, texel_rgb_grad, texel_height_grad):
+            check(self.lib.pf_scrub_nonfinite(ptr(t), t.numel(), st))
+        del grad_ctx.rasterizer, grad_ctx.camera
+        return (None, None, None, spheres_grad[:, :3], spheres_grad[:, 3], nsigmas_grad[:, 3],
+                nsigmas_grad[:, :3], texel_sites_grad, texel_rgb_grad, texel_height_grad, None, None, None, None)
+
+
+def _ref(struct):
+    import ctypes
+    return ctypes.byref(struct)
+
+
+def _scene_struct(all_spheres, all_nsigmas, all_texel_sites, all_texel_rgbh, adjacency, adjacency_offsets,
+                  adjacency_diff):
+    s = _abi.PfScene()
+    s.n_sites = all_spheres.shape[0]
+    s.n_texels = all_texel_sites.shape[1]
+    s.spheres = all_spheres.data_ptr()
+    s.nsigmas = all_nsigmas.data_ptr()
+    s.texel_sites = all_texel_sites.data_ptr()
+    s.texel_rgbh = all_texel_rgbh.data_ptr()
+    s.adjacency = adjacency.data_ptr() if adjacency.numel() else None
+    s.adjacency_offsets = adjacency_offsets.data_ptr()
+    s.adjacency_diff = adjacency_diff.data_ptr() if (adjacency_diff is not None and adjacency_diff.numel()) else None
+    return s
