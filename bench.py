@@ -1,0 +1,227 @@
+"""Benchmark of the Power Foam rasterizer hot path on one MI355X.
+
+Workload (one "step" = one frame): the 1M-site scene (seed 1; 90% from a
+64-component Gaussian mixture, 10% on a radius 6-10 shell; r log-uniform
+[0.002, 0.08]; density lognormal(0, 1.5); degree-3 SH colour), 32 pinhole
+cameras on a radius-4 circle at height 1, 1297x840, fov 70 deg. Per frame:
+SH colour head (texel_rgb for this camera) -> Rasterizer.forward (binning,
+one host sync for the list size, adjacency encoding, compositing) -> L2 loss
+against a uniform-random target -> loss.backward() (compositing adjoint, SH
+adjoint). All scene tensors are resident in HBM before the timed region.
+
+Prints one JSON line (rank 0). `value` is rays per second (Mrays/s) over the
+timed frames; `ms_per_step` is ms per frame. The roofline entry is for the
+kernel that takes the most time, with algorithmic bytes per launch from the
+formulas in DESIGN.md (measured workload counts of the frame) divided by that
+kernel's average duration measured with HIP events on its launch stream.
+The CPU baseline is the FP32 CPU restatement of the same algorithm (the
+oracle, OpenMP) on one frame of the same workload.
+"""
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "fwd+bwd ms/frame @1M sites 1297x840; Mrays/s; MLP vs SH head at PSNR≥45dB"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def build_workload(n_sites, n_views, width, height, dev):
+    import torch
+    from powerfoam_amd import synth
+
+    t0 = time.time()
+    s = synth.scene_gmm(1, n=n_sites)
+    log(f"scene: {n_sites} sites in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    adj, off, stats = synth.power_adjacency(s["points"], s["radii"], return_stats=True)
+    log(f"power/alpha adjacency: {stats} in {time.time() - t0:.1f}s")
+    cams = synth.circle_cameras(n_views=n_views, width=width, img_h=height)
+    tt = lambda a, dt=torch.float32: torch.tensor(np.ascontiguousarray(a), dtype=dt, device=dev)
+    rng = np.random.default_rng(1234)
+    target = tt(rng.uniform(0.0, 1.0, (height, width, 3)).astype(np.float32))
+    params = dict(points=tt(s["points"]), radii=tt(s["radii"]), density=tt(s["density"]),
+                  normals=tt(s["normals"]), texel_sites=tt(s["texel_sites"]), sh=tt(s["sh"]),
+                  texel_height=tt(s["texel_height"]))
+    for v in params.values():
+        v.requires_grad_(True)
+    return dict(scene=s, params=params, adjacency=tt(adj, torch.int32), adjacency_offsets=tt(off, torch.int32),
+                adj_np=adj, off_np=off, cams=cams, target=target, stats=stats)
+
+
+def frame(rast, head, w, cam):
+    """One training-style frame: colour head, rasterize, L2 loss, backward."""
+    import torch
+    p = w["params"]
+    texel_rgb = head.forward(p["texel_sites"], cam, p["sh"])
+    out = rast.forward(cam, None, p["points"], p["radii"], p["density"], p["normals"], p["texel_sites"],
+                       texel_rgb, p["texel_height"], w["adjacency"], w["adjacency_offsets"], w["target"], True)
+    loss = torch.mean((out[0] - w["target"]) ** 2)
+    loss.backward()
+    return out
+
+
+def workload_counts(rast, w, width, height):
+    """Per-frame counts for the algorithmic byte formulas, from the last frame."""
+    import torch
+    offsets, sites, early = rast.last_tiles
+    counts = (offsets[1:] - offsets[:-1]).to(torch.int64)
+    tile_of = torch.repeat_interleave(torch.arange(counts.numel(), device=counts.device), counts)
+    pos = torch.arange(sites.numel(), device=sites.device) - offsets[:-1][tile_of]
+    processed = sites[pos < early.to(torch.int64)[tile_of]].to(torch.int64)
+    off = w["adjacency_offsets"].to(torch.int64)
+    nadj = (off[processed + 1] - off[processed])
+    touched = torch.unique(processed)
+    deg = off[touched + 1] - off[touched]
+    nb = w["adjacency"][torch.repeat_interleave(off[touched], deg) +
+                        (torch.arange(int(deg.sum()), device=deg.device) -
+                         torch.repeat_interleave(torch.cumsum(deg, 0) - deg, deg))] if int(deg.sum()) else \
+        torch.zeros(0, dtype=torch.int32, device=deg.device)
+    return dict(M=int(sites.numel()), pairs=int(processed.numel()), sum_nadj=int(nadj.sum()),
+                sites_touched=int(touched.numel()), neighbours_touched=int(torch.unique(nb).numel()),
+                P=width * height)
+
+
+def algorithmic_bytes(c):
+    """Algorithmic HBM bytes per launch (formulas in DESIGN.md)."""
+    read = c["pairs"] * (4 + 32 + 224 + 8) + 12 * c["sum_nadj"]
+    fwd = read + c["P"] * 32 + 8 * c["sites_touched"]
+    bwd = read + c["P"] * (12 + 4 + 4 + 12 + 4) + c["pairs"] * 2 * (32 + 224) + 2 * 16 * c["neighbours_touched"]
+    return dict(raster_fwd=fwd, raster_bwd=bwd)
+
+
+def cpu_baseline(w, width, height):
+    """FP32 oracle fwd+bwd on frame 0 of the workload (same inputs)."""
+    from oracle import orc
+    s = w["scene"]
+    cam_t = w["cams"][0]
+    cam, _ = orc.camera(cam_t.eye.numpy(), cam_t.right.numpy(), cam_t.up.numpy(), width, height)
+    sp = np.concatenate([s["points"], s["radii"][:, None]], 1).astype(np.float32)
+    ns = np.concatenate([s["normals"], s["density"][:, None]], 1).astype(np.float32)
+    from powerfoam_amd.synth import sh_eval
+    ts = s["texel_sites"].astype(np.float64)
+    d = ts - cam_t.eye.numpy().astype(np.float64)[None, None]
+    d /= np.linalg.norm(d, axis=-1, keepdims=True)
+    rgb = sh_eval(s["sh"][:, None].astype(np.float64), d).astype(np.float32)
+    rgbh = np.concatenate([rgb, s["texel_height"][..., None]], -1)
+    target = w["target"].cpu().numpy()
+    t0 = time.time()
+    offsets, sites, _ = orc.bin_lists(cam, sp)
+    diff = orc.adj_encode(sp, w["adj_np"], w["off_np"])
+    inp = orc.Inputs(sp, ns, s["texel_sites"], rgbh, w["adj_np"], w["off_np"], diff)
+    o = orc.raster_fwd(cam, inp, offsets, sites, ray_gt=target, return_point_err=True, precision="f32")
+    g_color = (2.0 * (o["color"] - target) / o["color"].size).astype(np.float32)
+    z = np.zeros((height, width), np.float32)
+    orc.raster_bwd(cam, inp, offsets, sites, o["tile_early_stop"], o["log_t"], g_color, z, z,
+                   np.zeros((height, width, 3), np.float32), np.zeros(sp.shape[0], np.float32), precision="f32")
+    dt = time.time() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return dict(value=width * height / dt / 1e6, unit="Mrays/s", cores=cores, kind="port",
+                sample=f"1 frame (view 0) of the workload: binning + fwd + bwd, FP32 oracle, {dt:.1f}s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=64)
+    ap.add_argument("--warmup", type=int, default=8)
+    ap.add_argument("--sites", type=int, default=1_000_000)
+    ap.add_argument("--views", type=int, default=32)
+    ap.add_argument("--width", type=int, default=1297)
+    ap.add_argument("--height", type=int, default=840)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: the renderer hot path is built and timed on a single MI355X; "
+              "multi-GPU runs are not part of this build (see DESIGN.md, sharding).", file=sys.stderr)
+        sys.exit(2)
+    if "OMP_NUM_THREADS" not in os.environ:
+        os.environ["OMP_NUM_THREADS"] = str(min(16, os.cpu_count() or 1))
+
+    import torch
+    from types import SimpleNamespace
+    from powerfoam_amd.color_fn import SHColor
+    from powerfoam_amd.rasterize import Rasterizer
+
+    dev = torch.device("cuda:0")
+    w = build_workload(args.sites, args.views, args.width, args.height, dev)
+    rast = Rasterizer(SimpleNamespace(num_texel_sites=8, render_objective="volume", is_pinhole=True,
+                                      disable_coop_prim_load=False, disable_coop_adj_load=False), dev)
+    head = SHColor()
+    cams = w["cams"]
+
+    log(f"warmup {args.warmup} frames")
+    for i in range(args.warmup):
+        frame(rast, head, w, cams[i % len(cams)])
+    torch.cuda.synchronize()
+    counts = workload_counts(rast, w, args.width, args.height)
+    log(f"workload counts (last warmup frame): {counts}")
+
+    log(f"timing {args.steps} frames")
+    rast.event_log = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        frame(rast, head, w, cams[i % len(cams)])
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ev = rast.event_log
+    rast.event_log = None
+    kern = {}
+    for name, s, e in ev:
+        kern.setdefault(name, []).append(s.elapsed_time(e))
+    kern_ms = {k: float(np.mean(v)) for k, v in kern.items()}
+    ms = elapsed * 1e3 / args.steps
+    P = args.width * args.height
+    value = P * args.steps / elapsed / 1e6
+    log(f"{ms:.3f} ms/frame, {value:.1f} Mrays/s; kernels {kern_ms}")
+
+    # counts of every view for the roofline (kernel time averages over views)
+    per_view = []
+    for v in range(len(cams)):
+        frame(rast, head, w, cams[v])
+        per_view.append(workload_counts(rast, w, args.width, args.height))
+    torch.cuda.synchronize()
+    avg = {k: float(np.mean([c[k] for c in per_view])) for k in per_view[0]}
+    order = [cams[i % len(cams)] for i in range(args.steps)]
+    del order
+    abytes = algorithmic_bytes(avg)
+    dom = max(kern_ms, key=kern_ms.get)
+    achieved = abytes[dom] / (kern_ms[dom] * 1e-3) / 1e9
+    roofline = dict(bound="hbm", kernel=dom, achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=achieved / HBM_PEAK_GBS, traffic=None, algorithmic_bytes=abytes[dom])
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline (FP32 oracle, 1 frame)")
+        cpu = cpu_baseline(w, args.width, args.height)
+        log(f"cpu baseline: {cpu}")
+
+    line = dict(metric=METRIC, value=value, unit="Mrays/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
+                ms_per_step=ms, higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f32",
+                data="synthetic (seeded scene generator, random target image)",
+                config=dict(workload="rasterizer fwd+bwd with SH-3 colour head, 1M-site GMM scene, "
+                                     "32 pinhole views 1297x840",
+                            sites=args.sites, width=args.width, height=args.height, views=args.views,
+                            adjacency="power/alpha", adjacency_edges=int(w["stats"]["edges"]),
+                            pairs_per_frame=avg["M"], processed_pairs_per_frame=avg["pairs"]),
+                kernel_ms=kern_ms, roofline=roofline, cpu_baseline=cpu)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -173,6 +173,18 @@ int pf_adj_encode(const float* spheres, int32_t n_sites,
 int pf_raster_fwd(const pf_raster_fwd_args* args, void* stream);
 int pf_raster_bwd(const pf_raster_bwd_args* args, void* stream);
 
+/* ---- colour head: degree-3 SH per site, evaluated per texel direction ----
+ * No reference counterpart: fills the slot of the reference's per-texel
+ * colour pre-pass (SphericalVoronoi._forward / _backward, color_fn.py:188-315)
+ * and produces the rasterizer's texel_rgb input.
+ * rgb[n,t] = max(0, sh[n,0] + sum_k Y_k(normalize(texel_sites[n,t] - eye)) sh[n,k]).
+ * sh is [N,16,3]; texel_sites [N,T,3]; texel_rgb [N,T,3]. */
+int pf_sh_fwd(const float* sh, const float* texel_sites, int32_t n_sites, int32_t n_texels, const float* eye,
+              float* texel_rgb, void* stream);
+/* grad_sh [N,16,3] is overwritten (not accumulated). */
+int pf_sh_bwd(const float* sh, const float* texel_sites, const float* grad_texel_rgb, int32_t n_sites,
+              int32_t n_texels, const float* eye, float* grad_sh, void* stream);
+
 /* Replaces non-finite entries with 0 (rasterize.py:1954-1958). */
 int pf_scrub_nonfinite(float* data, int64_t n, void* stream);
 

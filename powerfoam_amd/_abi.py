@@ -127,6 +127,8 @@ SIGNATURES = {
     "pf_raster_fwd": ([_VP, _VP], ctypes.c_int),
     "pf_raster_bwd": ([_VP, _VP], ctypes.c_int),
     "pf_scrub_nonfinite": ([_VP, ctypes.c_int64, _VP], ctypes.c_int),
+    "pf_sh_fwd": ([_VP, _VP, ctypes.c_int32, ctypes.c_int32, _VP, _VP, _VP], ctypes.c_int),
+    "pf_sh_bwd": ([_VP, _VP, _VP, ctypes.c_int32, ctypes.c_int32, _VP, _VP, _VP], ctypes.c_int),
 }
 
 _lib = None

@@ -404,7 +404,7 @@ struct BwdParams {
   float* g_theight;
 };
 
-constexpr int RED_STRIDE = 57;  // odd row stride: conflict-free LDS transpose
+constexpr int RED_STRIDE = 65;  // >= 8 + 7*8 values, odd: conflict-free LDS transpose
 
 // plane_intersection_fwd recomputed with its intermediates (texture.py:10-71)
 struct PlaneFwd {

@@ -1,0 +1,154 @@
+// Degree-3 spherical-harmonic texel colour, forward and backward.
+//
+// Fills the slot of the reference's per-texel colour pre-pass
+// (SphericalVoronoi, color_fn.py:54-184): for texel t of site n the view
+// direction is d = normalize(texel_site - eye) (color_fn.py:72) and
+//   rgb = max(0, sh[n,0] + sum_{k=1..15} Y_k(d) sh[n,k]),
+// with the real-SH constants of sh_utils.py:4-49 and the DC coefficient taken
+// as the base colour. As the reference's colour head, no gradient flows to the
+// texel positions (color_fn.py:181-184).
+//
+// One thread per site: the 48 coefficients are read once and applied to the
+// site's T texels, so HBM traffic is 192 B (SH) + 12 T (texel sites) in and
+// 12 T (rgb) out per site; the backward reads the same plus 12 T of incoming
+// gradient and writes the 192 B coefficient gradient.
+#include "pf_common.hpp"
+
+namespace pf {
+
+constexpr float C1 = 0.4886025119029199f;
+constexpr float C2_0 = 1.0925484305920792f, C2_1 = -1.0925484305920792f, C2_2 = 0.31539156525252005f,
+                C2_3 = -1.0925484305920792f, C2_4 = 0.5462742152960396f;
+constexpr float C3_0 = -0.5900435899266435f, C3_1 = 2.890611442640554f, C3_2 = -0.4570457994644658f,
+                C3_3 = 0.3731763325901154f, C3_4 = -0.4570457994644658f, C3_5 = 1.445305721320277f,
+                C3_6 = -0.5900435899266435f;
+
+__device__ __forceinline__ void sh_basis(f3 d, float* y) {
+  const float x = d.x, yy_ = d.y, z = d.z;
+  const float xx = x * x, yy = yy_ * yy_, zz = z * z;
+  y[0] = -C1 * yy_;
+  y[1] = C1 * z;
+  y[2] = -C1 * x;
+  y[3] = C2_0 * x * yy_;
+  y[4] = C2_1 * yy_ * z;
+  y[5] = C2_2 * (2.0f * zz - xx - yy);
+  y[6] = C2_3 * x * z;
+  y[7] = C2_4 * (xx - yy);
+  y[8] = C3_0 * yy_ * (3.0f * xx - yy);
+  y[9] = C3_1 * x * yy_ * z;
+  y[10] = C3_2 * yy_ * (4.0f * zz - xx - yy);
+  y[11] = C3_3 * z * (2.0f * zz - 3.0f * xx - 3.0f * yy);
+  y[12] = C3_4 * x * (4.0f * zz - xx - yy);
+  y[13] = C3_5 * z * (xx - yy);
+  y[14] = C3_6 * x * (xx - 3.0f * yy);
+}
+
+__device__ __forceinline__ f3 texel_dir(const float* ts, f3 eye) {
+  f3 d = f3{ts[0] - eye.x, ts[1] - eye.y, ts[2] - eye.z};
+  const float l = sqrtf(dot(d, d));
+  return l > 0.0f ? d * (1.0f / l) : f3{0.0f, 0.0f, 0.0f};
+}
+
+__global__ void __launch_bounds__(256)
+    sh_fwd_kernel(const float* __restrict__ sh, const float* __restrict__ texel_sites, int n, int T, f3 eye,
+                  float* __restrict__ rgb) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  float c[48];
+  const float4* src = reinterpret_cast<const float4*>(sh + (size_t)s * 48);
+#pragma unroll
+  for (int i = 0; i < 12; i++) {
+    const float4 v = src[i];
+    c[4 * i] = v.x; c[4 * i + 1] = v.y; c[4 * i + 2] = v.z; c[4 * i + 3] = v.w;
+  }
+  for (int t = 0; t < T; t++) {
+    const f3 d = texel_dir(texel_sites + ((size_t)s * T + t) * 3, eye);
+    float y[15];
+    sh_basis(d, y);
+    float r = c[0], g = c[1], b = c[2];
+#pragma unroll
+    for (int k = 0; k < 15; k++) {
+      r += y[k] * c[3 * (k + 1)];
+      g += y[k] * c[3 * (k + 1) + 1];
+      b += y[k] * c[3 * (k + 1) + 2];
+    }
+    float* o = rgb + ((size_t)s * T + t) * 3;
+    o[0] = fmaxf(r, 0.0f);
+    o[1] = fmaxf(g, 0.0f);
+    o[2] = fmaxf(b, 0.0f);
+  }
+}
+
+__global__ void __launch_bounds__(256)
+    sh_bwd_kernel(const float* __restrict__ sh, const float* __restrict__ texel_sites,
+                  const float* __restrict__ grad_rgb, int n, int T, f3 eye, float* __restrict__ grad_sh) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  float c[48], g[48];
+  const float4* src = reinterpret_cast<const float4*>(sh + (size_t)s * 48);
+#pragma unroll
+  for (int i = 0; i < 12; i++) {
+    const float4 v = src[i];
+    c[4 * i] = v.x; c[4 * i + 1] = v.y; c[4 * i + 2] = v.z; c[4 * i + 3] = v.w;
+  }
+#pragma unroll
+  for (int i = 0; i < 48; i++) g[i] = 0.0f;
+  for (int t = 0; t < T; t++) {
+    const f3 d = texel_dir(texel_sites + ((size_t)s * T + t) * 3, eye);
+    float y[15];
+    sh_basis(d, y);
+    float r = c[0], gg = c[1], b = c[2];
+#pragma unroll
+    for (int k = 0; k < 15; k++) {
+      r += y[k] * c[3 * (k + 1)];
+      gg += y[k] * c[3 * (k + 1) + 1];
+      b += y[k] * c[3 * (k + 1) + 2];
+    }
+    const float* gi = grad_rgb + ((size_t)s * T + t) * 3;
+    const float gr = r > 0.0f ? gi[0] : 0.0f;
+    const float ggg = gg > 0.0f ? gi[1] : 0.0f;
+    const float gb = b > 0.0f ? gi[2] : 0.0f;
+    g[0] += gr; g[1] += ggg; g[2] += gb;
+#pragma unroll
+    for (int k = 0; k < 15; k++) {
+      g[3 * (k + 1)] += y[k] * gr;
+      g[3 * (k + 1) + 1] += y[k] * ggg;
+      g[3 * (k + 1) + 2] += y[k] * gb;
+    }
+  }
+  float4* dst = reinterpret_cast<float4*>(grad_sh + (size_t)s * 48);
+#pragma unroll
+  for (int i = 0; i < 12; i++) dst[i] = make_float4(g[4 * i], g[4 * i + 1], g[4 * i + 2], g[4 * i + 3]);
+}
+
+}  // namespace pf
+
+using namespace pf;
+
+extern "C" {
+
+int pf_sh_fwd(const float* sh, const float* texel_sites, int32_t n_sites, int32_t n_texels, const float* eye,
+              float* texel_rgb, void* stream) {
+  if (n_sites < 0 || n_texels < 1 || n_texels > PF_MAX_TEXELS || !eye)
+    return set_error(PF_ERR_ARG, "pf_sh_fwd: bad arguments");
+  if (n_sites == 0) return PF_OK;
+  if (!sh || !texel_sites || !texel_rgb) return set_error(PF_ERR_ARG, "pf_sh_fwd: null array");
+  sh_fwd_kernel<<<(n_sites + 255) / 256, 256, 0, as_stream(stream)>>>(sh, texel_sites, n_sites, n_texels,
+                                                                      f3{eye[0], eye[1], eye[2]}, texel_rgb);
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+int pf_sh_bwd(const float* sh, const float* texel_sites, const float* grad_texel_rgb, int32_t n_sites,
+              int32_t n_texels, const float* eye, float* grad_sh, void* stream) {
+  if (n_sites < 0 || n_texels < 1 || n_texels > PF_MAX_TEXELS || !eye)
+    return set_error(PF_ERR_ARG, "pf_sh_bwd: bad arguments");
+  if (n_sites == 0) return PF_OK;
+  if (!sh || !texel_sites || !grad_texel_rgb || !grad_sh) return set_error(PF_ERR_ARG, "pf_sh_bwd: null array");
+  sh_bwd_kernel<<<(n_sites + 255) / 256, 256, 0, as_stream(stream)>>>(sh, texel_sites, grad_texel_rgb, n_sites,
+                                                                      n_texels, f3{eye[0], eye[1], eye[2]}, grad_sh);
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+}  // extern "C"

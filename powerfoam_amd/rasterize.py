@@ -34,11 +34,16 @@ class RasterGradFn(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, rasterizer, *args):
+        ctx.n_args = len(args)
         return rasterizer._forward(ctx, *args)
 
     @staticmethod
     def backward(ctx, *args):
-        return ctx.rasterizer._backward(ctx, *args)
+        n_args = ctx.n_args
+        grads = ctx.rasterizer._backward(ctx, *args)
+        # one entry per forward input (the rasterizer itself included); an
+        # explicit transmittance_threshold gets None like the other non-tensors
+        return tuple(grads) + (None,) * (1 + n_args - len(grads))
 
 
 def _f32c(t):
@@ -70,6 +75,23 @@ class Rasterizer:
         # pf_raster_bwd_args.exact_gradients).
         self.exact_geometry = False
         self.exact_gradients = False
+        # Optional timing: when a list, (name, start, end) torch.cuda.Events are
+        # appended around each kernel launch, recorded on the launch stream.
+        self.event_log = None
+        # Tile lists and early-stop counters of the last forward, for workload
+        # accounting (the same tensors the backward keeps).
+        self.last_tiles = None
+
+    def _launch(self, name, fn, *a):
+        if self.event_log is None:
+            return check(fn(*a))
+        s = torch.cuda.Event(enable_timing=True)
+        e = torch.cuda.Event(enable_timing=True)
+        s.record()
+        st = fn(*a)
+        e.record()
+        self.event_log.append((name, s, e))
+        return check(st)
 
     # ------------------------------------------------------------------ binning
     def bin(self, camera, all_spheres):
@@ -162,7 +184,8 @@ class Rasterizer:
         a.err = err_out.data_ptr() if err_out is not None else None
         a.contrib = contrib_out.data_ptr()
         a.point_err = point_err_out.data_ptr() if point_err_out is not None else None
-        check(self.lib.pf_raster_fwd(_ref(a), stream_handle()))
+        self._launch("raster_fwd", self.lib.pf_raster_fwd, _ref(a), stream_handle())
+        self.last_tiles = (offsets, tile_sites, tile_early_stop)
 
         grad_ctx.rasterizer = self
         grad_ctx.camera = camera
@@ -233,7 +256,7 @@ class Rasterizer:
         a.grad_texel_rgb = texel_rgb_grad.data_ptr()
         a.grad_texel_height = texel_height_grad.data_ptr()
         st = stream_handle()
-        check(self.lib.pf_raster_bwd(_ref(a), st))
+        self._launch("raster_bwd", self.lib.pf_raster_bwd, _ref(a), st)
         for t in (spheres_grad, nsigmas_grad, texel_sites_grad, texel_rgb_grad, texel_height_grad):
             check(self.lib.pf_scrub_nonfinite(ptr(t), t.numel(), st))
         del grad_ctx.rasterizer, grad_ctx.camera

@@ -17,8 +17,8 @@ overlapping balls) for the small scene, and for the large scenes the power
 the alpha filter |ci-cj| < ri+rj (benchmark.py:25-83). Sites whose power cell
 is empty ("hidden") have no Delaunay edges; so that they render empty, as the
 Cech complex would make them, each gets the face of a ball that contains it,
-or, when none does, the 32 overlapping neighbours whose radical planes cut
-deepest into it. Neighbour ids are ascending within each site.
+or, when none does, those of its 32 power-nearest sites whose radical
+planes cut into it. Neighbour ids are ascending within each site.
 """
 
 import math
@@ -196,25 +196,17 @@ def power_adjacency(points, radii, alpha_complex=True, hidden_cap=32, return_sta
         ej.append(j[contained])
         rest = hidden[~contained]
         if rest.size:
-            tree = cKDTree(p)
-            lists = tree.query_ball_point(p[rest], r[rest] + float(r.max()), workers=-1)
-            for h, nb in zip(rest, lists):
-                nb = np.asarray(nb, np.int64)
-                nb = nb[nb != h]
-                if nb.size == 0:
-                    continue
-                dd = np.linalg.norm(p[nb] - p[h], axis=1)
-                ov = dd < r[nb] + r[h]
-                nb, dd = nb[ov], dd[ov]
-                if nb.size == 0:
-                    continue
-                delta = (dd * dd + r[h] ** 2 - r[nb] ** 2) / (2.0 * np.maximum(dd, 1e-30))
-                cut = delta < r[h]
-                nb, delta = nb[cut], delta[cut]
-                if nb.size > hidden_cap:
-                    nb = nb[np.argsort(delta, kind="stable")[:hidden_cap]]
-                ei.append(np.full(nb.size, h))
-                ej.append(nb)
+            # the power-nearest sites of the centre are the ones whose radical
+            # planes cut deepest; keep those that overlap the ball and cut it
+            k = min(hidden_cap + 1, n)
+            _, nn = tree4.query(np.concatenate([p[rest], np.zeros((rest.size, 1))], 1), k=k, workers=-1)
+            hh = np.repeat(rest, k)
+            nb = nn.reshape(-1).astype(np.int64)
+            dd = np.linalg.norm(p[nb] - p[hh], axis=1)
+            delta = (dd * dd + r[hh] ** 2 - r[nb] ** 2) / (2.0 * np.maximum(dd, 1e-30))
+            keep = (nb != hh) & (dd < r[nb] + r[hh]) & (delta < r[hh])
+            ei.append(hh[keep])
+            ej.append(nb[keep])
     # hidden-site faces are one-directional (a hidden site does not bound anyone)
     vis_i, vis_j = ei[0], ej[0]
     src = np.concatenate([vis_i, vis_j] + ei[1:]).astype(np.int64)
