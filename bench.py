@@ -143,6 +143,8 @@ def main():
     ap.add_argument("--width", type=int, default=1297)
     ap.add_argument("--height", type=int, default=840)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="also run 3 device copies of this many MiB (counter calibration)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -163,6 +165,16 @@ def main():
                                       disable_coop_prim_load=False, disable_coop_adj_load=False), dev)
     head = SHColor()
     cams = w["cams"]
+
+    if args.calib_copy_mb:
+        n = args.calib_copy_mb * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration: 3 copies of {args.calib_copy_mb} MiB")
+        del src, dst
 
     log(f"warmup {args.warmup} frames")
     for i in range(args.warmup):

@@ -136,3 +136,20 @@ void orc_f16_to_f32_array(const uint16_t* x, int64_t n, float* y) {
 #include "orc_raster_impl.h"
 #undef R
 #undef SFX
+
+/* ------------------------------------------------ ABI layout (for tests) */
+#include <stddef.h>
+/* sizeof and field offsets of the C-ABI structs as gcc lays them out. */
+int orc_abi_layout(int64_t* out, int cap) {
+  int64_t v[] = {
+      (int64_t)sizeof(pf_camera), (int64_t)offsetof(pf_camera, ray_maps), (int64_t)offsetof(pf_camera, fov_cos_cutoff),
+      (int64_t)sizeof(pf_scene), (int64_t)offsetof(pf_scene, adjacency_diff),
+      (int64_t)sizeof(pf_tile_lists), (int64_t)offsetof(pf_tile_lists, sites),
+      (int64_t)sizeof(pf_raster_fwd_args), (int64_t)offsetof(pf_raster_fwd_args, tiles),
+      (int64_t)offsetof(pf_raster_fwd_args, transmittance_threshold), (int64_t)offsetof(pf_raster_fwd_args, point_err),
+      (int64_t)sizeof(pf_raster_bwd_args), (int64_t)offsetof(pf_raster_bwd_args, exact_gradients),
+      (int64_t)offsetof(pf_raster_bwd_args, grad_texel_height)};
+  int n = (int)(sizeof(v) / sizeof(v[0]));
+  for (int i = 0; i < n && i < cap; i++) out[i] = v[i];
+  return n;
+}

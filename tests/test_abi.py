@@ -1,0 +1,67 @@
+"""The C-ABI library loads on a CPU-only host and exports every entry point
+include/powerfoam_amd.h declares; the ctypes struct mirror matches the C
+layout (checked against the gcc-built oracle, which includes the header)."""
+
+import ctypes
+import os
+import re
+
+import pytest
+
+from powerfoam_amd import _abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for f in os.listdir(os.path.join(ROOT, "include")):
+        if f.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", f)).read()
+            text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+            names |= set(re.findall(r"\b(pf_[a-z0-9_]+)\s*\(", text))
+    return names
+
+
+def test_library_exports_declared_symbols():
+    if not os.path.exists(_abi.LIB_PATH):
+        pytest.skip("libpfhip.so not built")
+    lib = ctypes.CDLL(_abi.LIB_PATH)
+    decl = declared_symbols()
+    assert {"pf_raster_fwd", "pf_raster_bwd", "pf_bin_count", "pf_bin_sort"} <= decl
+    for name in sorted(decl):
+        assert hasattr(lib, name), name
+    # every declared symbol has a ctypes signature and vice versa
+    assert decl == set(_abi.SIGNATURES)
+
+
+def test_load_sets_signatures_and_reports_errors():
+    if not os.path.exists(_abi.LIB_PATH):
+        pytest.skip("libpfhip.so not built")
+    lib = _abi.load()
+    assert lib.pf_version() >= 1
+    # argument validation runs on the host, before any device call
+    st = lib.pf_bin_scan(None, 0, None, None)
+    assert st == _abi.PF_ERR_ARG
+    assert b"pf_bin_scan" in lib.pf_last_error()
+    with pytest.raises(_abi.PfError):
+        _abi.check(st)
+
+
+def test_struct_layout_matches_c():
+    """ctypes mirror vs gcc's layout of the same header (the oracle library
+    includes include/powerfoam_amd.h and reports sizeof/offsetof)."""
+    import numpy as np
+    from oracle import orc
+    lib = orc.load()
+    lib.orc_abi_layout.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    out = np.zeros(32, np.int64)
+    n = lib.orc_abi_layout(out.ctypes.data, 32)
+    C, S, TL, F, B = _abi.PfCamera, _abi.PfScene, _abi.PfTileLists, _abi.PfRasterFwdArgs, _abi.PfRasterBwdArgs
+    expect = [ctypes.sizeof(C), C.ray_maps.offset, C.fov_cos_cutoff.offset,
+              ctypes.sizeof(S), S.adjacency_diff.offset,
+              ctypes.sizeof(TL), TL.sites.offset,
+              ctypes.sizeof(F), F.tiles.offset, F.transmittance_threshold.offset, F.point_err.offset,
+              ctypes.sizeof(B), B.exact_gradients.offset, B.grad_texel_height.offset]
+    assert n == len(expect)
+    assert out[:n].tolist() == expect
