@@ -162,7 +162,8 @@ class _FwdOut(ctypes.Structure):
 
 class _BwdOut(ctypes.Structure):
     _fields_ = [(k, ctypes.c_void_p) for k in
-                ("grad_spheres", "grad_nsigmas", "grad_texel_sites", "grad_texel_rgb", "grad_texel_height")]
+                ("grad_spheres", "grad_nsigmas", "grad_texel_sites", "grad_texel_rgb", "grad_texel_height",
+                 "fragile")]
 
 
 def raster_fwd(cam, inp, offsets, sites, depth_quantiles=None, ray_gt=None, threshold=1e-3,
@@ -229,7 +230,8 @@ def raster_bwd(cam, inp, offsets, sites, tile_early_stop, log_t, grad_color, gra
     N, T = inp.n, inp.T
     out = dict(grad_spheres=np.zeros((N, 4), dt), grad_nsigmas=np.zeros((N, 4), dt),
                grad_texel_sites=np.zeros((N, T, 3), dt), grad_texel_rgb=np.zeros((N, T, 3), dt),
-               grad_texel_height=np.zeros((N, T), dt))
+               grad_texel_height=np.zeros((N, T), dt), fragile=np.zeros(N, np.uint8))
     o = _BwdOut(*[_addr(out[k]) for k, _ in _BwdOut._fields_])
     getattr(lib, f"orc_raster_bwd_{precision}")(ctypes.byref(a), ctypes.byref(o))
+    out["fragile"] = out["fragile"].astype(bool)
     return out

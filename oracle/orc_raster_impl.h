@@ -454,6 +454,13 @@ typedef struct {
   R* grad_texel_sites;
   R* grad_texel_rgb;
   R* grad_texel_height;
+  /* optional [N]: 1 where a ray meets the site ill-conditioned -- tangent to
+   * its sphere (|r^2 - |oc x d|^2| < 1e-6 r^2) or with a clipped interval
+   * whose length is within 1e-5 (1 + t_far) of zero. There the gradient
+   * grows like 1/sqrt of that margin, so one-ulp differences between
+   * implementations flip whole contributions; the site and its neighbours
+   * are flagged. */
+  uint8_t* fragile;
 } FN(bwd_out);
 
 void FN(orc_raster_bwd)(const pf_raster_bwd_args* A, FN(bwd_out) * out) {
@@ -533,6 +540,16 @@ void FN(orc_raster_bwd)(const pf_raster_bwd_args* A, FN(bwd_out) * out) {
         tn_id = (pf.dp < (R)0 && pf.t_surf > tn) ? n_adj : tn_id;
         tn = pf.dp < (R)0 ? FN(rmax)(pf.t_surf, tn) : tn;
         R dt = tf - tn;
+        if (out->fragile && !oob[l]) {
+          V3 cr = FN(crs)(FN(sub)(ro[l], c), rd[l]);
+          R h2 = r * r - FN(dot)(cr, cr);
+          int frag = fabs((double)h2) < 1e-6 * (double)(r * r);
+          if (hs[l].hit && fabs((double)dt) < 1e-5 * (1.0 + fabs((double)tf))) frag = 1;
+          if (frag) {
+            out->fragile[site] = 1;
+            for (int e = a0; e < a1; e++) out->fragile[sc->adjacency[e]] = 1;
+          }
+        }
         R dLdtn = 0, dLdtf = 0, dLdr = 0, dLdsig = 0;
         V3 dLdc = FN(mk)(0, 0, 0), dLdnorm = FN(mk)(0, 0, 0), dLdcol = FN(mk)(0, 0, 0);
         const int active = !oob[l] && hs[l].hit && dt > (R)0;

@@ -7,29 +7,65 @@
 //
 // One wave64 per 8x8 tile (the reference's 64-thread tile block maps 1:1 onto
 // a CDNA wavefront); lane l shades pixel (tile_i*8 + l%8, tile_j*8 + l/8).
-// The wave walks its tile's depth-sorted site list in batches of 64: lanes
+// A 256-thread workgroup carries four independent tiles; each wave owns its
+// LDS slots and synchronises only with itself, so waves never wait on each
+// other's list lengths.
+//
+// A wave walks its tile's depth-sorted site list in batches of 64: lanes
 // fetch 64 site records with one coalesced gather into LDS, and every
 // per-site quantity that does not depend on the pixel (face normals and
-// offsets, texel offsets, plane offset) is computed once per site by the wave
-// into LDS instead of once per lane. The tile-wide decisions of the reference
-// become ballots: stop when no lane has T >= threshold (the tile max of T is
-// below it), skip a site no lane hits. Intersections use cancellation-free
-// forms of the reference formulas (ray-sphere: disc = 4(r^2 - |oc x d|^2);
-// power face: t = ((c-o).n + (|n|^2 + r^2 - r_a^2)/2) / (d.n) with n the
-// decoded neighbour delta), so FP32 results stay within 1e-4 of the FP64
-// oracle where the reference's FP32 formulas drift by ~1e-3.
+// offsets, texel offsets) is computed once per site by the wave into LDS
+// instead of once per lane. The tile-wide decisions of the reference become
+// ballots: stop when no lane has T >= threshold (the tile max of T is below
+// it), skip a site no lane hits. Intersections use cancellation-free forms of
+// the reference formulas (ray-sphere: h^2 = r^2 - |oc x d|^2; power face:
+// t = ((c-o).n + (|n|^2 + r^2 - r_a^2)/2) / (d.n) with n the decoded
+// neighbour delta), so FP32 results stay within 1e-4 of the FP64 oracle
+// where the reference's FP32 formulas drift by ~1e-3. Reciprocals and
+// exponentials use the hardware v_rcp_f32 / v_exp_f32 (1 ulp).
+//
+// Exact shortcuts: a site whose texel heights are all zero has identical
+// height-stage and colour-stage query points, so the height stage is skipped
+// (forward) or its weights reused (backward); a lane whose interval is empty
+// after the faces cannot contribute, so it does no texel work.
 //
 // The backward recomputes each segment back-to-front from the saved final
 // log-transmittance (rasterize.py:1219, 1372-1375) and reduces every per-site
 // gradient across the wave before touching global memory: neighbour-face
-// gradients are gathered in LDS per face slot, texel gradients are summed by
-// an LDS transpose, and one lane per value issues the atomic.
+// gradients are gathered in LDS per face slot, the site and texel gradients
+// are summed by an LDS transpose in four 16-value slices, and one lane per
+// value issues the atomic.
 #include "pf_common.hpp"
 
 namespace pf {
 
 constexpr int WAVE = 64;
+#ifndef PF_WPB
+#define PF_WPB 4
+#endif
+constexpr int WPB = PF_WPB;  // tiles (waves) per workgroup
 constexpr float LOG2E = 1.4426950408889634f;
+
+__device__ __forceinline__ float fast_exp(float x) { return __builtin_amdgcn_exp2f(x * LOG2E); }
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+// num / den from v_rcp_f32 plus one Newton step on the quotient: within about
+// half an ulp, so interval ends agree with a correctly rounded division except
+// at exact ties (the interval bounds decide which sites a ray shades).
+__device__ __forceinline__ float acc_div(float num, float den) {
+  const float r = __builtin_amdgcn_rcpf(den);
+  const float q = num * r;
+  return __builtin_fmaf(__builtin_fmaf(-den, q, num), r, q);
+}
+__device__ __forceinline__ float sel_min(float a, float b) { return a < b ? a : b; }
+__device__ __forceinline__ float sel_max(float a, float b) { return a > b ? a : b; }
+
+// Orders this wave's LDS traffic: LDS operations of one wave execute in
+// program order, so only the compiler has to be kept from moving them.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 struct RasterCam {
   f3 eye, right, up, fwd;  // fwd = normalize(up x right)
@@ -59,21 +95,14 @@ static RasterCam make_rcam(const pf_camera& c) {
 }
 
 // Workgroups are dealt round-robin to the 8 XCDs; give each XCD a contiguous
-// run of tiles so neighbouring tiles (which share sites) share an L2.
-__device__ __forceinline__ int xcd_tile(int b, int n) {
-  const int per = (n + 7) >> 3;
+// run of tile groups so neighbouring tiles (which share sites) share an L2.
+// Returns the first tile of this workgroup's group of WPB tiles.
+__device__ __forceinline__ int xcd_group(int b, int n_groups, bool& valid) {
+  const int per = (n_groups + 7) >> 3;
   const int xcd = b & 7, k = b >> 3;
-  const int full = n - (per - 1) * 8;  // XCDs that own `per` tiles
-  int t;
-  if (xcd < full) t = xcd * per + k;
-  else t = full * per + (xcd - full) * (per - 1) + k;
-  return t;
-}
-__device__ __forceinline__ bool xcd_valid(int b, int n) {
-  const int per = (n + 7) >> 3;
-  const int xcd = b & 7, k = b >> 3;
-  const int full = n - (per - 1) * 8;
-  return xcd < full ? k < per : k < per - 1;
+  const int full = n_groups - (per - 1) * 8;  // XCDs that own `per` groups
+  valid = xcd < full ? k < per : k < per - 1;
+  return xcd < full ? xcd * per + k : full * per + (xcd - full) * (per - 1) + k;
 }
 
 struct Ray {
@@ -104,13 +133,12 @@ __device__ __forceinline__ Ray lane_ray(const RasterCam& cam, int tile, int lane
     r.o = f3{rm[0], rm[1], rm[2]};
     d = f3{rm[3], rm[4], rm[5]};
   }
-  const float il = 1.0f / sqrtf(dot(d, d));
-  r.d = d * il;
+  r.d = d * __builtin_amdgcn_rsqf(dot(d, d));
   return r;
 }
 
 // ray-sphere (rendering_math.py:4-23) with the perpendicular-distance
-// discriminant; oc = o - c
+// discriminant; oc = o - c, |d| = 1
 __device__ __forceinline__ bool sphere_hit(f3 oc, f3 d, float r, float& tn, float& tf) {
   const float b = dot(oc, d);
   const f3 cr = cross(oc, d);
@@ -118,7 +146,7 @@ __device__ __forceinline__ bool sphere_hit(f3 oc, f3 d, float r, float& tn, floa
   tn = 0.0f;
   tf = 0.0f;
   if (h2 < 0.0f) return false;
-  const float h = sqrtf(h2);
+  const float h = __builtin_sqrtf(h2);
   const float far = -b + h, near = -b - h;
   if (near < 0.0f && far < 0.0f) return false;
   tn = near < 0.0f ? 0.0f : near;
@@ -132,11 +160,52 @@ struct SiteRec {
   int site, a0, a1, pad;
 };
 
-struct TexelRec {
-  float sx, sy, sz;  // texel site - centre
-  float r, g, b, h;
-  float pad;
+// Face (n, w) of a site against neighbour e: n = neighbour delta and
+// w = (|n|^2 + r^2 - r_a^2)/2, plus (c - eye).n for pinhole cameras; the ray
+// crosses it at t = (w + (c - o).n) / (d.n) -- for a pinhole the second term
+// is already folded in. With no encoded deltas the neighbour's
+// full-precision sphere is used.
+template <bool PINHOLE>
+__device__ __forceinline__ float4 face_of(const int32_t* adjacency, const uint16_t* diff, const float4* spheres,
+                                          int64_t e, float4 sph, f3 ce) {
+  f3 n;
+  float ra;
+  if (diff) {
+    const uint2 raw = *reinterpret_cast<const uint2*>(diff + 4 * e);
+    n.x = half_bits_to_float((uint16_t)(raw.x & 0xffff));
+    n.y = half_bits_to_float((uint16_t)(raw.x >> 16));
+    n.z = half_bits_to_float((uint16_t)(raw.y & 0xffff));
+    ra = half_bits_to_float((uint16_t)(raw.y >> 16));
+  } else {
+    const float4 a = spheres[adjacency[e]];
+    n = f3{a.x - sph.x, a.y - sph.y, a.z - sph.z};
+    ra = a.w;
+  }
+  float w = 0.5f * (dot(n, n) + sph.w * sph.w - ra * ra);
+  if (PINHOLE) w += dot(ce, n);
+  return make_float4(n.x, n.y, n.z, w);
+}
+
+// Texel records staged per site: (site - centre, height), (rgb, 0).
+struct TexelLds {
+  float4 pos_h;
+  float4 rgb;
 };
+
+// Loads the site's texels into LDS (lanes < T) and reports whether any
+// height is non-zero.
+__device__ __forceinline__ bool stage_texels(TexelLds* s_tex, const float* texel_sites, const float4* texel_rgbh,
+                                             int site, int T, f3 c, int lane) {
+  bool nz = false;
+  if (lane < T) {
+    const float* ts = texel_sites + ((size_t)site * T + lane) * 3;
+    const float4 v = texel_rgbh[(size_t)site * T + lane];
+    s_tex[lane].pos_h = make_float4(ts[0] - c.x, ts[1] - c.y, ts[2] - c.z, v.w);
+    s_tex[lane].rgb = make_float4(v.x, v.y, v.z, 0.0f);
+    nz = v.w != 0.0f;
+  }
+  return ballot(nz) != 0;
+}
 
 struct FwdParams {
   RasterCam cam;
@@ -165,39 +234,22 @@ struct FwdParams {
   float* point_err;
 };
 
-// Face (n, k) of a site against neighbour e: n = delta, and for a ray from o
-// the crossing is t = (dot(c - o, n) + k) / dot(d, n) with
-// k = (|n|^2 + r^2 - r_a^2) / 2. With no encoded deltas the neighbour's exact
-// sphere is used.
-__device__ __forceinline__ float4 face_of(const int32_t* adjacency, const uint16_t* diff, const float4* spheres,
-                                          int64_t e, float4 sph) {
-  f3 n;
-  float ra;
-  if (diff) {
-    const uint2 raw = *reinterpret_cast<const uint2*>(diff + 4 * e);
-    n.x = half_bits_to_float((uint16_t)(raw.x & 0xffff));
-    n.y = half_bits_to_float((uint16_t)(raw.x >> 16));
-    n.z = half_bits_to_float((uint16_t)(raw.y & 0xffff));
-    ra = half_bits_to_float((uint16_t)(raw.y >> 16));
-  } else {
-    const float4 a = spheres[adjacency[e]];
-    n = f3{a.x - sph.x, a.y - sph.y, a.z - sph.z};
-    ra = a.w;
-  }
-  const float k = 0.5f * (dot(n, n) + sph.w * sph.w - ra * ra);
-  return make_float4(n.x, n.y, n.z, k);
-}
-
 template <bool PINHOLE>
-__global__ void __launch_bounds__(WAVE) raster_fwd_kernel(FwdParams p) {
-  __shared__ SiteRec s_site[WAVE];
-  __shared__ float4 s_face[WAVE];
-  __shared__ TexelRec s_tex[PF_MAX_TEXELS];
+__global__ void __launch_bounds__(WAVE* WPB) raster_fwd_kernel(FwdParams p) {
+  __shared__ SiteRec s_site_all[WPB][WAVE];
+  __shared__ float4 s_face_all[WPB][WAVE];
+  __shared__ TexelLds s_tex_all[WPB][PF_MAX_TEXELS];
 
-  const int b = blockIdx.x;
-  if (!xcd_valid(b, p.n_tiles)) return;
-  const int tile = xcd_tile(b, p.n_tiles);
-  const int lane = threadIdx.x;
+  const int wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  bool valid;
+  const int group = xcd_group(blockIdx.x, (p.n_tiles + WPB - 1) / WPB, valid);
+  const int tile = group * WPB + wv;
+  if (!valid || tile >= p.n_tiles) return;
+  SiteRec* s_site = s_site_all[wv];
+  float4* s_face = s_face_all[wv];
+  TexelLds* s_tex = s_tex_all[wv];
+
   const Ray ray = lane_ray<PINHOLE>(p.cam, tile, lane);
   const size_t px = (size_t)ray.pi * p.cam.W + ray.pj;
   const int T = p.n_texels;
@@ -218,21 +270,18 @@ __global__ void __launch_bounds__(WAVE) raster_fwd_kernel(FwdParams p) {
 
   for (int base = 0; base < total && !stop; base += WAVE) {
     const int cnt = min(WAVE, total - base);
-    __syncthreads();
+    wave_sync();
     if (lane < cnt) {
       const int s = p.sites[start + base + lane];
-      SiteRec r;
-      r.sph = p.spheres[s];
-      r.nsg = p.nsigmas[s];
-      r.site = s;
-      r.a0 = p.adj_off[s];
-      r.a1 = p.adj_off[s + 1];
-      r.pad = 0;
-      s_site[lane] = r;
+      s_site[lane].sph = p.spheres[s];
+      s_site[lane].nsg = p.nsigmas[s];
+      s_site[lane].site = s;
+      s_site[lane].a0 = p.adj_off[s];
+      s_site[lane].a1 = p.adj_off[s + 1];
     }
-    __syncthreads();
+    wave_sync();
     for (int kk = 0; kk < cnt; kk++) {
-      const float trans = exp2f(log_t * LOG2E);
+      const float trans = fast_exp(log_t);
       if (ballot(trans >= p.thr) == 0) {
         early = base + kk;
         stop = true;
@@ -244,25 +293,27 @@ __global__ void __launch_bounds__(WAVE) raster_fwd_kernel(FwdParams p) {
       const f3 pn{sr.nsg.x, sr.nsg.y, sr.nsg.z};
       const float sigma = sr.nsg.w;
       const f3 oc = ray.o - c;
+      const f3 co = -oc;
       float tn, tf;
       const bool hit = sphere_hit(oc, ray.d, r, tn, tf);
       if (ballot(hit) == 0) continue;
 
       // power faces (rasterize.py:1038-1071), staged 64 at a time
-      const f3 co = -oc;
+      const f3 ce = c - p.cam.eye;
       for (int e0 = sr.a0; e0 < sr.a1; e0 += WAVE) {
         const int ec = min(WAVE, sr.a1 - e0);
-        __syncthreads();
-        if (lane < ec) s_face[lane] = face_of(p.adjacency, p.adj_diff, p.spheres, e0 + lane, sr.sph);
-        __syncthreads();
+        wave_sync();
+        if (lane < ec) s_face[lane] = face_of<PINHOLE>(p.adjacency, p.adj_diff, p.spheres, e0 + lane, sr.sph, ce);
+        wave_sync();
         if (hit) {
           for (int e = 0; e < ec; e++) {
             const float4 f = s_face[e];
             const f3 n{f.x, f.y, f.z};
             const float dp = dot(ray.d, n);
-            const float t = (dot(co, n) + f.w) / dp;
-            tf = dp >= 0.0f ? fminf(t, tf) : tf;
-            tn = dp < 0.0f ? fmaxf(t, tn) : tn;
+            const float num = PINHOLE ? f.w : f.w + dot(co, n);
+            const float t = acc_div(num, dp);
+            tf = (dp >= 0.0f && t < tf) ? t : tf;
+            tn = (dp < 0.0f && t > tn) ? t : tn;
           }
         }
       }
@@ -270,58 +321,56 @@ __global__ void __launch_bounds__(WAVE) raster_fwd_kernel(FwdParams p) {
       if (ballot(shade) == 0) continue;
 
       // texel records of this site (texture.py:23-24)
-      __syncthreads();
-      if (lane < T) {
-        const float* ts = p.texel_sites + ((size_t)sr.site * T + lane) * 3;
-        const float4 v = p.texel_rgbh[(size_t)sr.site * T + lane];
-        TexelRec tr;
-        tr.sx = ts[0] - c.x;
-        tr.sy = ts[1] - c.y;
-        tr.sz = ts[2] - c.z;
-        tr.r = v.x; tr.g = v.y; tr.b = v.z; tr.h = v.w;
-        tr.pad = 0.0f;
-        s_tex[lane] = tr;
-      }
-      __syncthreads();
+      wave_sync();
+      const bool heights = stage_texels(s_tex, p.texel_sites, p.texel_rgbh, sr.site, T, c, lane);
+      wave_sync();
       float alpha_t = 0.0f, perr = 0.0f;
       if (shade) {
         // plane_intersection_fwd (texture.py:10-71)
         const float dpn = dot(pn, ray.d);
         const float cn = dot(co, pn);
-        const float kexp = -10.0f * LOG2E / (r * r);
-        const float ts0 = cn / dpn;
-        const float tq0 = dpn >= 0.0f ? tn : fmaxf(tn, ts0);
-        const f3 x0 = ray.d * tq0 - co;  // query point relative to the centre
-        float hsum = 0.0f, ws0 = 0.0f;
-        for (int i = 0; i < T; i++) {
-          const TexelRec tr = s_tex[i];
-          const f3 df{x0.x - tr.sx, x0.y - tr.sy, x0.z - tr.sz};
-          const float w = exp2f(dot(df, df) * kexp);
-          hsum += w * tr.h;
-          ws0 += w;
+        const float kexp = -10.0f * fast_rcp(r * r);
+        const float ts0 = acc_div(cn, dpn);
+        float height = 0.0f;
+        if (heights) {
+          const float tq0 = dpn >= 0.0f ? tn : sel_max(tn, ts0);
+          const f3 x0 = ray.d * tq0 - co;  // query point relative to the centre
+          float hsum = 0.0f, ws0 = 0.0f;
+#pragma unroll
+          for (int i = 0; i < PF_MAX_TEXELS; i++) {
+            if (i < T) {
+              const float4 ph = s_tex[i].pos_h;
+              const f3 df{x0.x - ph.x, x0.y - ph.y, x0.z - ph.z};
+              const float w = fast_exp(dot(df, df) * kexp);
+              hsum += w * ph.w;
+              ws0 += w;
+            }
+          }
+          height = hsum * fast_rcp(sel_max(ws0, 1e-20f));
         }
-        ws0 = fmaxf(ws0, 1e-20f);
-        const float height = hsum / ws0;
-        const float ts1 = (cn + height) / dpn;
-        const float tq = dpn >= 0.0f ? tn : fmaxf(tn, ts1);
+        const float ts1 = heights ? acc_div(cn + height, dpn) : ts0;
+        const float tq = dpn >= 0.0f ? tn : sel_max(tn, ts1);
         const f3 x1 = ray.d * tq - co;
         f3 rs{0.0f, 0.0f, 0.0f};
         float ws = 0.0f;
-        for (int i = 0; i < T; i++) {
-          const TexelRec tr = s_tex[i];
-          const f3 df{x1.x - tr.sx, x1.y - tr.sy, x1.z - tr.sz};
-          const float w = exp2f(dot(df, df) * kexp);
-          rs = rs + f3{tr.r, tr.g, tr.b} * w;
-          ws += w;
+#pragma unroll
+        for (int i = 0; i < PF_MAX_TEXELS; i++) {
+          if (i < T) {
+            const float4 ph = s_tex[i].pos_h;
+            const float4 cl = s_tex[i].rgb;
+            const f3 df{x1.x - ph.x, x1.y - ph.y, x1.z - ph.z};
+            const float w = fast_exp(dot(df, df) * kexp);
+            rs = rs + f3{cl.x, cl.y, cl.z} * w;
+            ws += w;
+          }
         }
-        ws = fmaxf(ws, 1e-20f);
-        const f3 col = rs * (1.0f / ws);
-        float tf2 = dpn >= 0.0f ? fminf(ts1, tf) : tf;
-        float tn2 = dpn < 0.0f ? fmaxf(ts1, tn) : tn;
+        const f3 col = rs * fast_rcp(sel_max(ws, 1e-20f));
+        const float tf2 = (dpn >= 0.0f && ts1 < tf) ? ts1 : tf;
+        const float tn2 = (dpn < 0.0f && ts1 > tn) ? ts1 : tn;
         const float dt = tf2 - tn2;
         if (dt > 0.0f) {
           const float dlt = -sigma * dt;
-          const float alpha = 1.0f - exp2f(dlt * LOG2E);
+          const float alpha = 1.0f - fast_exp(dlt);
           const float at = alpha * trans;
           if (p.loss_acc) {
             const f3 df = col - gt;
@@ -334,13 +383,13 @@ __global__ void __launch_bounds__(WAVE) raster_fwd_kernel(FwdParams p) {
             const f3 df = col - gt;
             perr = at * (fabsf(df.x) + fabsf(df.y) + fabsf(df.z));
           }
-          const float ndv = dot(pn, ray.d);
+          const float ndv = dpn;
           if (ndv > 0.0f) nd += (ndv * ndv) * at;
           nacc = nacc + pn * at;
           if (p.Q > 0) {
-            const float next_t = exp2f(log_t * LOG2E);
+            const float next_t = fast_exp(log_t);
             while (qidx < p.Q && next_t < qcur) {
-              p.qdepth[px * p.Q + qidx] = tn2 + logf(trans / qcur) / sigma;
+              p.qdepth[px * p.Q + qidx] = tn2 + __logf(trans * fast_rcp(qcur)) * fast_rcp(sigma);
               qidx++;
               if (qidx < p.Q) qcur = p.depth_q[px * p.Q + qidx];
             }
@@ -369,7 +418,7 @@ __global__ void __launch_bounds__(WAVE) raster_fwd_kernel(FwdParams p) {
     p.normal[3 * px + 0] = nacc.x;
     p.normal[3 * px + 1] = nacc.y;
     p.normal[3 * px + 2] = nacc.z;
-    if (p.loss_acc && p.err) p.err[px] = err + dot(gt, gt) * exp2f(log_t * LOG2E);
+    if (p.loss_acc && p.err) p.err[px] = err + dot(gt, gt) * fast_exp(log_t);
   }
 }
 
@@ -404,29 +453,44 @@ struct BwdParams {
   float* g_theight;
 };
 
-constexpr int RED_STRIDE = 65;  // >= 8 + 7*8 values, odd: conflict-free LDS transpose
+constexpr int RED_SLICE = 16;      // values reduced per LDS pass
+constexpr int RED_W = RED_SLICE + 1;  // odd row stride: conflict-free
+constexpr int NVAL = 8 + 7 * PF_MAX_TEXELS;  // sphere 4, nsigma 4, 7 per texel
 
-// plane_intersection_fwd recomputed with its intermediates (texture.py:10-71)
-struct PlaneFwd {
-  float ts0, height, ws0, ts1, ws, dpn;
-  f3 col;
-};
+// Destination of reduced value v of a site (v < 8 + 7T).
+__device__ __forceinline__ float* grad_slot(const BwdParams& p, int site, int T, int v) {
+  if (v < 4) return p.g_spheres + 4 * (size_t)site + v;
+  if (v < 8) return p.g_nsigmas + 4 * (size_t)site + (v - 4);
+  const int i = (v - 8) / 7, q = (v - 8) % 7;
+  if (q < 3) return p.g_tsites + ((size_t)site * T + i) * 3 + q;
+  if (q < 6) return p.g_trgb + ((size_t)site * T + i) * 3 + (q - 3);
+  return p.g_theight + (size_t)site * T + i;
+}
 
 template <bool PINHOLE>
-__global__ void __launch_bounds__(WAVE) raster_bwd_kernel(BwdParams p) {
-  __shared__ SiteRec s_site[WAVE];
-  __shared__ float4 s_face[WAVE];
-  __shared__ TexelRec s_tex[PF_MAX_TEXELS];
-  __shared__ float s_fgrad[WAVE * 4];
-  __shared__ float s_red[WAVE * RED_STRIDE];
+__global__ void __launch_bounds__(WAVE* WPB) raster_bwd_kernel(BwdParams p) {
+  __shared__ SiteRec s_site_all[WPB][WAVE];
+  __shared__ float4 s_face_all[WPB][WAVE];
+  __shared__ TexelLds s_tex_all[WPB][PF_MAX_TEXELS];
+  __shared__ float s_fgrad_all[WPB][WAVE * 4];
+  __shared__ float s_red_all[WPB][WAVE * RED_W];
 
-  const int b = blockIdx.x;
-  if (!xcd_valid(b, p.n_tiles)) return;
-  const int tile = xcd_tile(b, p.n_tiles);
-  const int lane = threadIdx.x;
+  const int wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  bool valid;
+  const int group = xcd_group(blockIdx.x, (p.n_tiles + WPB - 1) / WPB, valid);
+  const int tile = group * WPB + wv;
+  if (!valid || tile >= p.n_tiles) return;
+  SiteRec* s_site = s_site_all[wv];
+  float4* s_face = s_face_all[wv];
+  TexelLds* s_tex = s_tex_all[wv];
+  float* s_fgrad = s_fgrad_all[wv];
+  float* s_red = s_red_all[wv];
+
   const Ray ray = lane_ray<PINHOLE>(p.cam, tile, lane);
   const size_t px = (size_t)ray.pi * p.cam.W + ray.pj;
   const int T = p.n_texels;
+  const int nval = 8 + 7 * T;
   const bool exact = p.exact != 0;
   const float pl_eps = exact ? 0.0f : 1e-6f;
   const float cut = exact ? 0.0f : 1e-3f;
@@ -446,19 +510,16 @@ __global__ void __launch_bounds__(WAVE) raster_bwd_kernel(BwdParams p) {
   for (int top = early; top > 0; top -= WAVE) {
     const int cnt = min(WAVE, top);
     const int base = top - cnt;  // batch covers [base, top)
-    __syncthreads();
+    wave_sync();
     if (lane < cnt) {
       const int s = p.sites[start + base + lane];
-      SiteRec r;
-      r.sph = p.spheres[s];
-      r.nsg = p.nsigmas[s];
-      r.site = s;
-      r.a0 = p.adj_off[s];
-      r.a1 = p.adj_off[s + 1];
-      r.pad = 0;
-      s_site[lane] = r;
+      s_site[lane].sph = p.spheres[s];
+      s_site[lane].nsg = p.nsigmas[s];
+      s_site[lane].site = s;
+      s_site[lane].a0 = p.adj_off[s];
+      s_site[lane].a1 = p.adj_off[s + 1];
     }
-    __syncthreads();
+    wave_sync();
     for (int kk = cnt - 1; kk >= 0; kk--) {
       const SiteRec sr = s_site[kk];
       const f3 c{sr.sph.x, sr.sph.y, sr.sph.z};
@@ -474,80 +535,99 @@ __global__ void __launch_bounds__(WAVE) raster_bwd_kernel(BwdParams p) {
       // id: -2 none, -1 sphere, 0..n-1 face, n plane (rasterize.py:1296)
       int tn_id = tn < 1e-8f ? -2 : -1;
       int tf_id = tf < 1e-8f ? -2 : -1;
+      const f3 ce = c - p.cam.eye;
       for (int e0 = sr.a0; e0 < sr.a1; e0 += WAVE) {
         const int ec = min(WAVE, sr.a1 - e0);
-        __syncthreads();
-        if (lane < ec) s_face[lane] = face_of(p.adjacency, p.adj_diff, p.spheres, e0 + lane, sr.sph);
-        __syncthreads();
+        wave_sync();
+        if (lane < ec) s_face[lane] = face_of<PINHOLE>(p.adjacency, p.adj_diff, p.spheres, e0 + lane, sr.sph, ce);
+        wave_sync();
         if (hit) {
           for (int e = 0; e < ec; e++) {
             const float4 f = s_face[e];
             const f3 n{f.x, f.y, f.z};
             const float dp = dot(ray.d, n);
-            const float t = (dot(co, n) + f.w) / dp;
+            const float num = PINHOLE ? f.w : f.w + dot(co, n);
+            const float t = acc_div(num, dp);
             const int id = e0 - sr.a0 + e;
-            if (dp >= 0.0f && t < tf) tf_id = id;
-            tf = dp >= 0.0f ? fminf(t, tf) : tf;
-            if (dp < 0.0f && t > tn) tn_id = id;
-            tn = dp < 0.0f ? fmaxf(t, tn) : tn;
+            if (dp >= 0.0f && t < tf) {
+              tf_id = id;
+              tf = t;
+            }
+            if (dp < 0.0f && t > tn) {
+              tn_id = id;
+              tn = t;
+            }
           }
         }
       }
       const bool shade = hit && !ray.oob && tn < tf;
       if (ballot(shade) == 0) continue;
-      __syncthreads();
-      if (lane < T) {
-        const float* ts = p.texel_sites + ((size_t)sr.site * T + lane) * 3;
-        const float4 v = p.texel_rgbh[(size_t)sr.site * T + lane];
-        TexelRec tr;
-        tr.sx = ts[0] - c.x;
-        tr.sy = ts[1] - c.y;
-        tr.sz = ts[2] - c.z;
-        tr.r = v.x; tr.g = v.y; tr.b = v.z; tr.h = v.w;
-        tr.pad = 0.0f;
-        s_tex[lane] = tr;
-      }
-      const float kexp = -10.0f * LOG2E / (r * r);
+      wave_sync();
+      const bool heights = stage_texels(s_tex, p.texel_sites, p.texel_rgbh, sr.site, T, c, lane);
+      wave_sync();
+
+      // ---- recompute plane_intersection_fwd (texture.py:10-71), keeping the
+      // colour-stage weights
       const float tn0 = tn;
       const int tn0_id = tn_id, tf0_id = tf_id;
-      PlaneFwd pf{};
+      const float kexp = -10.0f * fast_rcp(r * r);
+      float w1[PF_MAX_TEXELS];
+      float dpn = 0.0f, cn = 0.0f, ts0 = 0.0f, ts1 = 0.0f, height = 0.0f, ws0 = 1.0f, ws = 1.0f;
+      f3 col{0.0f, 0.0f, 0.0f};
       float dt = -1.0f;
-      float cn = 0.0f;
-      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < PF_MAX_TEXELS; i++) w1[i] = 0.0f;
       if (shade) {
-        pf.dpn = dot(pn, ray.d);
+        dpn = dot(pn, ray.d);
         cn = dot(co, pn);
-        pf.ts0 = cn / pf.dpn;
-        const float tq0 = pf.dpn >= 0.0f ? tn : fmaxf(tn, pf.ts0);
-        const f3 x0 = ray.d * tq0 - co;
-        float hsum = 0.0f, ws0 = 0.0f;
-        for (int i = 0; i < T; i++) {
-          const TexelRec tr = s_tex[i];
-          const f3 df{x0.x - tr.sx, x0.y - tr.sy, x0.z - tr.sz};
-          const float w = exp2f(dot(df, df) * kexp);
-          hsum += w * tr.h;
-          ws0 += w;
+        ts0 = acc_div(cn, dpn);
+        if (heights) {
+          const float tq0 = dpn >= 0.0f ? tn : sel_max(tn, ts0);
+          const f3 x0 = ray.d * tq0 - co;
+          float hsum = 0.0f, s0 = 0.0f;
+#pragma unroll
+          for (int i = 0; i < PF_MAX_TEXELS; i++) {
+            if (i < T) {
+              const float4 ph = s_tex[i].pos_h;
+              const f3 df{x0.x - ph.x, x0.y - ph.y, x0.z - ph.z};
+              const float w = fast_exp(dot(df, df) * kexp);
+              hsum += w * ph.w;
+              s0 += w;
+            }
+          }
+          ws0 = sel_max(s0, 1e-20f);
+          height = hsum * fast_rcp(ws0);
+          ts1 = acc_div(cn + height, dpn);
+        } else {
+          ts1 = ts0;
         }
-        pf.ws0 = fmaxf(ws0, 1e-20f);
-        pf.height = hsum / pf.ws0;
-        pf.ts1 = (cn + pf.height) / pf.dpn;
-        const float tq = pf.dpn >= 0.0f ? tn : fmaxf(tn, pf.ts1);
+        const float tq = dpn >= 0.0f ? tn : sel_max(tn, ts1);
         const f3 x1 = ray.d * tq - co;
         f3 rs{0.0f, 0.0f, 0.0f};
-        float ws = 0.0f;
-        for (int i = 0; i < T; i++) {
-          const TexelRec tr = s_tex[i];
-          const f3 df{x1.x - tr.sx, x1.y - tr.sy, x1.z - tr.sz};
-          const float w = exp2f(dot(df, df) * kexp);
-          rs = rs + f3{tr.r, tr.g, tr.b} * w;
-          ws += w;
+        float s1 = 0.0f;
+#pragma unroll
+        for (int i = 0; i < PF_MAX_TEXELS; i++) {
+          if (i < T) {
+            const float4 ph = s_tex[i].pos_h;
+            const float4 cl = s_tex[i].rgb;
+            const f3 df{x1.x - ph.x, x1.y - ph.y, x1.z - ph.z};
+            const float w = fast_exp(dot(df, df) * kexp);
+            w1[i] = w;
+            rs = rs + f3{cl.x, cl.y, cl.z} * w;
+            s1 += w;
+          }
         }
-        pf.ws = fmaxf(ws, 1e-20f);
-        pf.col = rs * (1.0f / pf.ws);
-        if (pf.dpn >= 0.0f && pf.ts1 < tf) tf_id = n_adj;
-        tf = pf.dpn >= 0.0f ? fminf(pf.ts1, tf) : tf;
-        if (pf.dpn < 0.0f && pf.ts1 > tn) tn_id = n_adj;
-        tn = pf.dpn < 0.0f ? fmaxf(pf.ts1, tn) : tn;
+        ws = sel_max(s1, 1e-20f);
+        if (!heights) ws0 = ws;
+        col = rs * fast_rcp(ws);
+        if (dpn >= 0.0f && ts1 < tf) {
+          tf_id = n_adj;
+          tf = ts1;
+        }
+        if (dpn < 0.0f && ts1 > tn) {
+          tn_id = n_adj;
+          tn = ts1;
+        }
         dt = tf - tn;
       }
       const bool active = shade && dt > 0.0f;
@@ -555,25 +635,25 @@ __global__ void __launch_bounds__(WAVE) raster_bwd_kernel(BwdParams p) {
 
       // ---- compositing adjoint (rasterize.py:1366-1430)
       float dLdtn = 0.0f, dLdtf = 0.0f, dLdr = 0.0f, dLdsig = 0.0f;
-      f3 dLdc{0.0f, 0.0f, 0.0f}, dLdnorm{0.0f, 0.0f, 0.0f}, dLdcol{0.0f, 0.0f, 0.0f};
+      f3 dLdc{0.0f, 0.0f, 0.0f}, dLdnorm{0.0f, 0.0f, 0.0f};
       float g_tex[PF_MAX_TEXELS][7];
 #pragma unroll
       for (int i = 0; i < PF_MAX_TEXELS; i++)
 #pragma unroll
         for (int q = 0; q < 7; q++) g_tex[i][q] = 0.0f;
       int fid0 = -1, fid1 = -1;
-      float fg0[4] = {0, 0, 0, 0}, fg1[4] = {0, 0, 0, 0};
+      float4 fg0 = make_float4(0, 0, 0, 0), fg1 = make_float4(0, 0, 0, 0);
       if (active) {
         const float dlt = -sigma * dt;
         log_t -= dlt;
-        const float e_dlt = exp2f(dlt * LOG2E);
+        const float e_dlt = fast_exp(dlt);
         const float alpha = 1.0f - e_dlt;
-        const float trans = exp2f(log_t * LOG2E);
-        const float ndv = dot(pn, ray.d);
+        const float trans = fast_exp(log_t);
+        const float ndv = dpn;
         float dLdlog_t_i = dLdlog_t;
         float dLddlt = dLdlog_t;
         const float gc = p.g_contrib[sr.site];
-        const float cdot = dot(g_rgb, pf.col);
+        const float cdot = dot(g_rgb, col);
         const float ndt = exact ? ndv * ndv : ndv;
         const float ndot = dot(g_n, pn);
         float dLdalpha = cdot * trans;
@@ -584,7 +664,7 @@ __global__ void __launch_bounds__(WAVE) raster_bwd_kernel(BwdParams p) {
         if (ndv > 0.0f) dLdtrans += g_nd * ndt * alpha;
         dLdtrans += gc * alpha * p.inv_hw;
         dLdtrans += ndot * alpha;
-        dLdcol = g_rgb * (trans * alpha);
+        const f3 dLdcol = g_rgb * (trans * alpha);
         dLdlog_t_i += dLdtrans * trans;
         dLddlt += -dLdalpha * e_dlt;
         dLdsig += dLddlt * -dt;
@@ -592,7 +672,7 @@ __global__ void __launch_bounds__(WAVE) raster_bwd_kernel(BwdParams p) {
         dLdtf += dLddlt * -sigma;
         while (qidx >= 0 && trans > qcur) {
           const float g = p.g_qdepth[px * p.Q + qidx];
-          dLdsig += -g * logf(trans / qcur) / (sigma * sigma + q_eps);
+          dLdsig += -g * __logf(trans * fast_rcp(qcur)) / (sigma * sigma + q_eps);
           dLdtn += g;
           dLdlog_t_i += g / (sigma + q_eps);
           qidx--;
@@ -604,86 +684,85 @@ __global__ void __launch_bounds__(WAVE) raster_bwd_kernel(BwdParams p) {
 
         // ---- plane_intersection_bwd (texture.py:74-186)
         const float dLdts_in = (tn_id == n_adj) ? dLdtn : ((tf_id == n_adj) ? dLdtf : 0.0f);
-        const float r2 = r * r;
-        const float kw = 20.0f / r2;
+        const float kw = 20.0f * fast_rcp(r * r);
+        const float inv_ws = fast_rcp(ws);
+        const float inv_ws0 = fast_rcp(ws0);
         float dLdheight = 0.0f;
         f3 pdc{0.0f, 0.0f, 0.0f}, pdn{0.0f, 0.0f, 0.0f};
-        {
-          const float tq = pf.dpn >= 0.0f ? tn0 : fmaxf(tn0, pf.ts1);
-          const f3 x1 = ray.d * tq - co;
-          f3 dLdx{0.0f, 0.0f, 0.0f};
-          const float gdp_in = dot(dLdcol, pf.col);
-          const float inv_ws = 1.0f / pf.ws;
+        // colour stage
+        const float tq = dpn >= 0.0f ? tn0 : sel_max(tn0, ts1);
+        const f3 x1 = ray.d * tq - co;
+        f3 dLdx{0.0f, 0.0f, 0.0f};
+        const float gdp_in = dot(dLdcol, col);
 #pragma unroll
-          for (int i = 0; i < PF_MAX_TEXELS; i++) {
-            if (i < T) {
-              const TexelRec tr = s_tex[i];
-              const f3 df{x1.x - tr.sx, x1.y - tr.sy, x1.z - tr.sz};
-              const float w = exp2f(dot(df, df) * kexp);
-              if (!(w * inv_ws < cut)) {
-                const float wn = w * inv_ws;
-                g_tex[i][3] += dLdcol.x * wn;
-                g_tex[i][4] += dLdcol.y * wn;
-                g_tex[i][5] += dLdcol.z * wn;
-                const float gdp = dLdcol.x * tr.r + dLdcol.y * tr.g + dLdcol.z * tr.b;
-                const float sgs = kw * w * (gdp - gdp_in) * inv_ws;
-                const f3 sg = df * sgs;
-                g_tex[i][0] += sg.x;
-                g_tex[i][1] += sg.y;
-                g_tex[i][2] += sg.z;
-                dLdx = dLdx - sg;
-              }
+        for (int i = 0; i < PF_MAX_TEXELS; i++) {
+          if (i < T) {
+            const float w = w1[i];
+            const float wn = w * inv_ws;
+            if (!(wn < cut)) {
+              const float4 ph = s_tex[i].pos_h;
+              const float4 cl = s_tex[i].rgb;
+              const f3 df{x1.x - ph.x, x1.y - ph.y, x1.z - ph.z};
+              g_tex[i][3] += dLdcol.x * wn;
+              g_tex[i][4] += dLdcol.y * wn;
+              g_tex[i][5] += dLdcol.z * wn;
+              const float gdp = dLdcol.x * cl.x + dLdcol.y * cl.y + dLdcol.z * cl.z;
+              const f3 sg = df * (kw * wn * (gdp - gdp_in));
+              g_tex[i][0] += sg.x;
+              g_tex[i][1] += sg.y;
+              g_tex[i][2] += sg.z;
+              dLdx = dLdx - sg;
             }
           }
-          const float dLdtq = dot(dLdx, ray.d);
-          float dtn_q = (pf.dpn >= 0.0f || tn0 >= pf.ts1) ? dLdtq : 0.0f;
-          float dts = (pf.dpn < 0.0f && tn0 < pf.ts1) ? dLdtq : 0.0f;
-          dts += dLdts_in;
-          if (fabsf(dts) > 0.0f) {
-            // ray_plane_intersect_bwd with p = c, h = height (rendering_math.py:138-147)
-            const float den = pf.dpn + pl_eps;
-            pdc = pdc + pn * (dts / den);
-            const float num = cn + pf.height;
-            pdn = pdn + (co * (1.0f / den) - ray.d * (num / (pf.dpn * pf.dpn + pl_eps))) * dts;
-            dLdheight += dts / den;
-          }
-          // height stage
-          const float tq0 = pf.dpn >= 0.0f ? tn0 : fmaxf(tn0, pf.ts0);
-          const f3 x0 = ray.d * tq0 - co;
-          f3 dLdx0{0.0f, 0.0f, 0.0f};
-          const float gdp0_in = dLdheight * pf.height;
-          const float inv_ws0 = 1.0f / pf.ws0;
-#pragma unroll
-          for (int i = 0; i < PF_MAX_TEXELS; i++) {
-            if (i < T) {
-              const TexelRec tr = s_tex[i];
-              const f3 df{x0.x - tr.sx, x0.y - tr.sy, x0.z - tr.sz};
-              const float w = exp2f(dot(df, df) * kexp);
-              // the cut-off tests the colour-stage weight sum, as the reference
-              if (!(w * inv_ws < cut)) {
-                g_tex[i][6] += w * dLdheight * inv_ws0;
-                const float sgs = kw * w * (dLdheight * tr.h - gdp0_in) * inv_ws0;
-                const f3 sg = df * sgs;
-                g_tex[i][0] += sg.x;
-                g_tex[i][1] += sg.y;
-                g_tex[i][2] += sg.z;
-                dLdx0 = dLdx0 - sg;
-              }
-            }
-          }
-          const float dLdtq0 = dot(dLdx0, ray.d);
-          dtn_q += (pf.dpn >= 0.0f || tn0 >= pf.ts0) ? dLdtq0 : 0.0f;
-          const float dts0 = (pf.dpn < 0.0f && tn0 < pf.ts0) ? dLdtq0 : 0.0f;
-          if (fabsf(dts0) > 0.0f) {
-            const float den = pf.dpn + pl_eps;
-            pdc = pdc + pn * (dts0 / den);
-            pdn = pdn + (co * (1.0f / den) - ray.d * (cn / (pf.dpn * pf.dpn + pl_eps))) * dts0;
-          }
-          dLdc = dLdc + pdc;
-          dLdnorm = dLdnorm + pdn;
-          if (tn_id != n_adj) dtn_q += dLdtn;
-          dLdtn = dtn_q;
         }
+        const float dLdtq = dot(dLdx, ray.d);
+        float dtn_q = (dpn >= 0.0f || tn0 >= ts1) ? dLdtq : 0.0f;
+        float dts = (dpn < 0.0f && tn0 < ts1) ? dLdtq : 0.0f;
+        dts += dLdts_in;
+        if (fabsf(dts) > 0.0f) {
+          // ray_plane_intersect_bwd with p = c, h = height (rendering_math.py:138-147)
+          const float iden = 1.0f / (dpn + pl_eps);
+          pdc = pdc + pn * (dts * iden);
+          pdn = pdn + (co * iden - ray.d * ((cn + height) / (dpn * dpn + pl_eps))) * dts;
+          dLdheight += dts * iden;
+        }
+        // height stage
+        const float tq0 = dpn >= 0.0f ? tn0 : sel_max(tn0, ts0);
+        const f3 x0 = ray.d * tq0 - co;
+        f3 dLdx0{0.0f, 0.0f, 0.0f};
+        const float gdp0_in = dLdheight * height;
+#pragma unroll
+        for (int i = 0; i < PF_MAX_TEXELS; i++) {
+          if (i < T) {
+            const float4 ph = s_tex[i].pos_h;
+            const f3 df{x0.x - ph.x, x0.y - ph.y, x0.z - ph.z};
+            // heights all zero: the height stage queries the colour-stage point
+            const float w = heights ? fast_exp(dot(df, df) * kexp) : w1[i];
+            // the cut-off tests the colour-stage weight sum, as the reference
+            if (!(w * inv_ws < cut)) {
+              const float wn0 = w * inv_ws0;
+              g_tex[i][6] += wn0 * dLdheight;
+              const f3 sg = df * (kw * wn0 * (dLdheight * ph.w - gdp0_in));
+              g_tex[i][0] += sg.x;
+              g_tex[i][1] += sg.y;
+              g_tex[i][2] += sg.z;
+              dLdx0 = dLdx0 - sg;
+            }
+          }
+        }
+        const float dLdtq0 = dot(dLdx0, ray.d);
+        dtn_q += (dpn >= 0.0f || tn0 >= ts0) ? dLdtq0 : 0.0f;
+        const float dts0 = (dpn < 0.0f && tn0 < ts0) ? dLdtq0 : 0.0f;
+        if (fabsf(dts0) > 0.0f) {
+          const float iden = 1.0f / (dpn + pl_eps);
+          pdc = pdc + pn * (dts0 * iden);
+          pdn = pdn + (co * iden - ray.d * (cn / (dpn * dpn + pl_eps))) * dts0;
+        }
+        dLdc = dLdc + pdc;
+        dLdnorm = dLdnorm + pdn;
+        if (tn_id != n_adj) dtn_q += dLdtn;
+        dLdtn = dtn_q;
+
         // ---- route dL/dt_near and dL/dt_far to sphere / faces (rasterize.py:1464-1505)
 #pragma unroll
         for (int side = 0; side < 2; side++) {
@@ -691,13 +770,13 @@ __global__ void __launch_bounds__(WAVE) raster_bwd_kernel(BwdParams p) {
           const float dLdt = side == 0 ? dLdtn : dLdtf;
           if (exact && side == 1 && tf_id == n_adj) continue;
           if (tid == -1) {
-            // ray_sphere_intersect_bwd (rendering_math.py:26-59), |d| = 1
+            // ray_sphere_intersect_bwd (rendering_math.py:26-59)
             const float qa = dot(ray.d, ray.d);
             const float qb = 2.0f * dot(oc, ray.d);
             const f3 cr = cross(oc, ray.d);
             const float disc = 4.0f * (qa * r * r - dot(cr, cr));
             if (disc >= 0.0f) {
-              const float sq = sqrtf(disc);
+              const float sq = __builtin_sqrtf(disc);
               const float tfar = (-qb + sq) / (2.0f * qa);
               const float tnear = (-qb - sq) / (2.0f * qa);
               if (!(tnear < 0.0f && tfar < 0.0f)) {
@@ -722,17 +801,18 @@ __global__ void __launch_bounds__(WAVE) raster_bwd_kernel(BwdParams p) {
             const f3 a{as.x, as.y, as.z};
             const f3 n = a - c;
             const float num = dot(co, n) + 0.5f * (dot(n, n) + r * r - as.w * as.w);
-            const float dp = dot(ray.d, n);
-            const float idp = 1.0f / dp;
+            const float idp = 1.0f / dot(ray.d, n);
             const f3 qv = ray.d * (num * idp * idp);
             dLdc = dLdc + (oc * idp + qv) * dLdt;
             dLdr += dLdt * (r * idp);
             const f3 ga = ((a - ray.o) * idp - qv) * dLdt;
-            const float gra = dLdt * (-as.w * idp);
+            const float4 g4 = make_float4(ga.x, ga.y, ga.z, dLdt * (-as.w * idp));
             if (side == 0) {
-              fid0 = tid; fg0[0] = ga.x; fg0[1] = ga.y; fg0[2] = ga.z; fg0[3] = gra;
+              fid0 = tid;
+              fg0 = g4;
             } else {
-              fid1 = tid; fg1[0] = ga.x; fg1[1] = ga.y; fg1[2] = ga.z; fg1[3] = gra;
+              fid1 = tid;
+              fg1 = g4;
             }
           }
         }
@@ -740,21 +820,25 @@ __global__ void __launch_bounds__(WAVE) raster_bwd_kernel(BwdParams p) {
 
       // ---- neighbour gradients: gather per face slot in LDS, then flush
       if (n_adj <= WAVE) {
-        __syncthreads();
-        for (int q = lane; q < WAVE * 4; q += WAVE) s_fgrad[q] = 0.0f;
-        __syncthreads();
-        if (fid0 >= 0) {
+        wave_sync();
 #pragma unroll
-          for (int q = 0; q < 4; q++) atomicAdd(&s_fgrad[fid0 * 4 + q], fg0[q]);
+        for (int q = 0; q < 4; q++) s_fgrad[q * WAVE + lane] = 0.0f;
+        wave_sync();
+        if (fid0 >= 0) {
+          atomicAdd(&s_fgrad[fid0 * 4 + 0], fg0.x);
+          atomicAdd(&s_fgrad[fid0 * 4 + 1], fg0.y);
+          atomicAdd(&s_fgrad[fid0 * 4 + 2], fg0.z);
+          atomicAdd(&s_fgrad[fid0 * 4 + 3], fg0.w);
         }
         if (fid1 >= 0) {
-#pragma unroll
-          for (int q = 0; q < 4; q++) atomicAdd(&s_fgrad[fid1 * 4 + q], fg1[q]);
+          atomicAdd(&s_fgrad[fid1 * 4 + 0], fg1.x);
+          atomicAdd(&s_fgrad[fid1 * 4 + 1], fg1.y);
+          atomicAdd(&s_fgrad[fid1 * 4 + 2], fg1.z);
+          atomicAdd(&s_fgrad[fid1 * 4 + 3], fg1.w);
         }
-        __syncthreads();
+        wave_sync();
         if (lane < n_adj) {
-          const float4 v = make_float4(s_fgrad[lane * 4 + 0], s_fgrad[lane * 4 + 1], s_fgrad[lane * 4 + 2],
-                                       s_fgrad[lane * 4 + 3]);
+          const float4 v = *reinterpret_cast<const float4*>(&s_fgrad[lane * 4]);
           if (v.x != 0.0f || v.y != 0.0f || v.z != 0.0f || v.w != 0.0f) {
             float* g = p.g_spheres + 4 * (size_t)p.adjacency[sr.a0 + lane];
             atomicAdd(g + 0, v.x);
@@ -766,55 +850,53 @@ __global__ void __launch_bounds__(WAVE) raster_bwd_kernel(BwdParams p) {
       } else {
         if (fid0 >= 0) {
           float* g = p.g_spheres + 4 * (size_t)p.adjacency[sr.a0 + fid0];
-#pragma unroll
-          for (int q = 0; q < 4; q++) atomicAdd(g + q, fg0[q]);
+          atomicAdd(g + 0, fg0.x);
+          atomicAdd(g + 1, fg0.y);
+          atomicAdd(g + 2, fg0.z);
+          atomicAdd(g + 3, fg0.w);
         }
         if (fid1 >= 0) {
           float* g = p.g_spheres + 4 * (size_t)p.adjacency[sr.a0 + fid1];
-#pragma unroll
-          for (int q = 0; q < 4; q++) atomicAdd(g + q, fg1[q]);
+          atomicAdd(g + 0, fg1.x);
+          atomicAdd(g + 1, fg1.y);
+          atomicAdd(g + 2, fg1.z);
+          atomicAdd(g + 3, fg1.w);
         }
       }
 
-      // ---- site and texel gradients: LDS transpose, one lane per value
-      {
-        float* row = s_red + lane * RED_STRIDE;
-        row[0] = dLdc.x;
-        row[1] = dLdc.y;
-        row[2] = dLdc.z;
-        row[3] = dLdr;
-        row[4] = dLdnorm.x;
-        row[5] = dLdnorm.y;
-        row[6] = dLdnorm.z;
-        row[7] = dLdsig;
-#pragma unroll
-        for (int i = 0; i < PF_MAX_TEXELS; i++) {
-          if (i < T) {
-#pragma unroll
-            for (int q = 0; q < 7; q++) row[8 + i * 7 + q] = g_tex[i][q];
-          }
+      // ---- site and texel gradients: LDS transpose in 16-value slices; lane
+      // L sums column L%16 over rows 16*(L/16)..+15, two shuffles finish the
+      // column, lanes 0..15 issue one atomic each
+      // value v of the site: 0-3 sphere, 4-7 nsigma, then 7 per texel
+      auto val = [&](int v) -> float {
+        switch (v) {
+          case 0: return dLdc.x;
+          case 1: return dLdc.y;
+          case 2: return dLdc.z;
+          case 3: return dLdr;
+          case 4: return dLdnorm.x;
+          case 5: return dLdnorm.y;
+          case 6: return dLdnorm.z;
+          case 7: return dLdsig;
+          default: return g_tex[(v - 8) / 7][(v - 8) % 7];
         }
-        __syncthreads();
-        const int nv = 8 + 7 * T;
-        if (lane < nv) {
-          float s0 = 0.0f, s1 = 0.0f;
-          for (int l = 0; l < WAVE; l += 2) {
-            s0 += s_red[l * RED_STRIDE + lane];
-            s1 += s_red[(l + 1) * RED_STRIDE + lane];
-          }
-          const float v = s0 + s1;
-          if (v != 0.0f) {
-            float* dst;
-            if (lane < 4) dst = p.g_spheres + 4 * (size_t)sr.site + lane;
-            else if (lane < 8) dst = p.g_nsigmas + 4 * (size_t)sr.site + (lane - 4);
-            else {
-              const int i = (lane - 8) / 7, q = (lane - 8) % 7;
-              if (q < 3) dst = p.g_tsites + ((size_t)sr.site * T + i) * 3 + q;
-              else if (q < 6) dst = p.g_trgb + ((size_t)sr.site * T + i) * 3 + (q - 3);
-              else dst = p.g_theight + (size_t)sr.site * T + i;
-            }
-            atomicAdd(dst, v);
-          }
+      };
+#pragma unroll
+      for (int sl = 0; sl < NVAL / RED_SLICE; sl++) {
+        if (sl * RED_SLICE < nval) {
+          wave_sync();
+#pragma unroll
+          for (int q = 0; q < RED_SLICE; q++) s_red[lane * RED_W + q] = val(sl * RED_SLICE + q);
+          wave_sync();
+          const int col_ = lane & (RED_SLICE - 1);
+          const int row0 = (lane >> 4) * 16;
+          float acc = 0.0f;
+#pragma unroll
+          for (int rr = 0; rr < 16; rr++) acc += s_red[(row0 + rr) * RED_W + col_];
+          acc += __shfl_xor(acc, 16, 64);
+          acc += __shfl_xor(acc, 32, 64);
+          const int v = sl * RED_SLICE + lane;
+          if (lane < RED_SLICE && v < nval && acc != 0.0f) atomicAdd(grad_slot(p, sr.site, T, v), acc);
         }
       }
     }
@@ -886,11 +968,12 @@ int pf_raster_fwd(const pf_raster_fwd_args* a, void* stream) {
   p.err = a->err;
   p.contrib = a->contrib;
   p.point_err = a->point_err;
-  const int grid = ((nt + 7) / 8) * 8;
+  const int groups = (nt + WPB - 1) / WPB;
+  const int grid = ((groups + 7) / 8) * 8;
   if (c.ray_maps)
-    raster_fwd_kernel<false><<<grid, WAVE, 0, as_stream(stream)>>>(p);
+    raster_fwd_kernel<false><<<grid, WAVE * WPB, 0, as_stream(stream)>>>(p);
   else
-    raster_fwd_kernel<true><<<grid, WAVE, 0, as_stream(stream)>>>(p);
+    raster_fwd_kernel<true><<<grid, WAVE * WPB, 0, as_stream(stream)>>>(p);
   PF_HIP(hipGetLastError());
   return PF_OK;
 }
@@ -940,11 +1023,12 @@ int pf_raster_bwd(const pf_raster_bwd_args* a, void* stream) {
   p.g_tsites = a->grad_texel_sites;
   p.g_trgb = a->grad_texel_rgb;
   p.g_theight = a->grad_texel_height;
-  const int grid = ((nt + 7) / 8) * 8;
+  const int groups = (nt + WPB - 1) / WPB;
+  const int grid = ((groups + 7) / 8) * 8;
   if (c.ray_maps)
-    raster_bwd_kernel<false><<<grid, WAVE, 0, as_stream(stream)>>>(p);
+    raster_bwd_kernel<false><<<grid, WAVE * WPB, 0, as_stream(stream)>>>(p);
   else
-    raster_bwd_kernel<true><<<grid, WAVE, 0, as_stream(stream)>>>(p);
+    raster_bwd_kernel<true><<<grid, WAVE * WPB, 0, as_stream(stream)>>>(p);
   PF_HIP(hipGetLastError());
   return PF_OK;
 }

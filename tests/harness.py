@@ -160,6 +160,17 @@ def run_oracle(case, grads_in=None, exact_gradients=False, exact_geometry=False,
     return out, grads
 
 
+GRAD_KEYS = ("grad_spheres", "grad_nsigmas", "grad_texel_sites", "grad_texel_rgb", "grad_texel_height")
+
+
+def well_conditioned(ref_grads, max_fraction=0.02):
+    """Rows (sites) the oracle did not flag as ill-conditioned; asserts that
+    only a small fraction is flagged."""
+    frag = ref_grads["fragile"]
+    assert frag.mean() <= max_fraction, f"{frag.mean():.3%} of sites flagged ill-conditioned"
+    return ~frag
+
+
 def rel_err(a, b):
     """||a - b|| / ||b|| over a whole gradient tensor."""
     a = np.asarray(a, np.float64)
