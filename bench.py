@@ -214,8 +214,13 @@ def main():
     abytes = algorithmic_bytes(avg)
     dom = max(kern_ms, key=kern_ms.get)
     achieved = abytes[dom] / (kern_ms[dom] * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "raster_c1_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f)["kernels"].get(dom, {}).get("bytes")
     roofline = dict(bound="hbm", kernel=dom, achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=achieved / HBM_PEAK_GBS, traffic=None, algorithmic_bytes=abytes[dom])
+                    frac=achieved / HBM_PEAK_GBS, traffic=traffic, algorithmic_bytes=abytes[dom])
 
     cpu = None
     if not args.no_cpu_baseline:
