@@ -125,6 +125,17 @@ typedef struct pf_raster_bwd_args {
   float* grad_texel_height; /* [N,T] */
 } pf_raster_bwd_args;
 
+/* Benchmark forward: colour only. scene.adjacency_diff holds the power-face
+ * encoding of benchmark.py:297-302, binary16 (c_adj - c, pm_adj - pm) with
+ * pm = (|c|^2 - r^2)/2. */
+typedef struct pf_raster_rgb_args {
+  pf_camera cam;
+  pf_scene scene;
+  pf_tile_lists tiles;
+  float transmittance_threshold;
+  float* color; /* [H,W,3], every in-image pixel written */
+} pf_raster_rgb_args;
+
 /* ---- library ---------------------------------------------------------- */
 int pf_version(void);
 const char* pf_last_error(void);
@@ -184,6 +195,9 @@ int pf_sh_fwd(const float* sh, const float* texel_sites, int32_t n_sites, int32_
 /* grad_sh [N,16,3] is overwritten (not accumulated). */
 int pf_sh_bwd(const float* sh, const float* texel_sites, const float* grad_texel_rgb, int32_t n_sites,
               int32_t n_texels, const float* eye, float* grad_sh, void* stream);
+
+/* benchmark_kernel rasterize.py:614-734 (Rasterizer.benchmark :1982-2131). */
+int pf_raster_rgb(const pf_raster_rgb_args* args, void* stream);
 
 /* Replaces non-finite entries with 0 (rasterize.py:1954-1958). */
 int pf_scrub_nonfinite(float* data, int64_t n, void* stream);

@@ -41,7 +41,10 @@ def load():
         lib.orc_adj_encode.restype = None
         lib.orc_f32_to_f16_array.argtypes = [vp, ctypes.c_int64, vp]
         lib.orc_f16_to_f32_array.argtypes = [vp, ctypes.c_int64, vp]
+        lib.orc_power_face_encode.argtypes = [vp, ctypes.c_int32, vp, vp, vp]
+        lib.orc_power_face_encode.restype = None
         for p in ("f32", "f64"):
+            getattr(lib, f"orc_raster_rgb_{p}").argtypes = [vp, vp, vp, vp, ctypes.c_int, ctypes.c_float, vp]
             getattr(lib, f"orc_raster_fwd_{p}").argtypes = [vp, vp]
             getattr(lib, f"orc_raster_bwd_{p}").argtypes = [vp, vp]
         _lib = lib
@@ -115,6 +118,32 @@ def adj_encode(spheres, adjacency, offsets):
     out = np.zeros((max(adj.shape[0], 1), 4), np.uint16)
     lib.orc_adj_encode(_p(sp), sp.shape[0], _p(adj), _p(off), _p(out))
     return out[: adj.shape[0]]
+
+
+def power_face_encode(spheres, adjacency, offsets):
+    """(c_adj - c, pm_adj - pm) in binary16 (benchmark.py:297-302)."""
+    lib = load()
+    sp = _f32(spheres)
+    adj = np.ascontiguousarray(adjacency, np.int32)
+    off = np.ascontiguousarray(offsets, np.int32)
+    out = np.zeros((max(adj.shape[0], 1), 4), np.uint16)
+    lib.orc_power_face_encode(_p(sp), sp.shape[0], _p(adj), _p(off), _p(out))
+    return out[: adj.shape[0]]
+
+
+def raster_rgb(cam, inp, offsets, sites, threshold=1e-3, precision="f64"):
+    """Benchmark forward (colour only); inp.adjacency_diff holds the power-face encoding."""
+    lib = load()
+    dt = np.float64 if precision == "f64" else np.float32
+    offsets = np.ascontiguousarray(offsets, np.int64)
+    sites = np.ascontiguousarray(sites, np.int32)
+    if sites.size == 0:
+        sites = np.zeros(1, np.int32)
+    color = np.zeros((cam.height, cam.width, 3), dt)
+    sc = inp.struct()
+    getattr(lib, f"orc_raster_rgb_{precision}")(ctypes.byref(cam), ctypes.byref(sc), _p(offsets), _p(sites),
+                                                offsets.shape[0] - 1, threshold, _p(color))
+    return color
 
 
 def f32_to_f16_bits(x):

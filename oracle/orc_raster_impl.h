@@ -657,6 +657,68 @@ void FN(orc_raster_bwd)(const pf_raster_bwd_args* A, FN(bwd_out) * out) {
   }
 }
 
+
+/* Benchmark forward (benchmark_kernel, rasterize.py:614-734): colour only,
+ * per-lane stop at T < threshold, faces from the (c_adj - c, pm_adj - pm)
+ * binary16 encoding of benchmark.py:297-302 via ray_pface_intersect_diff
+ * (rendering_math.py:85-98), a lane leaves the face loop once its interval
+ * is empty, plane_intersection_fwd_local (rasterize.py:67-138). */
+void FN(orc_raster_rgb)(const pf_camera* cam, const pf_scene* sc, const int64_t* offsets, const int32_t* sites,
+                        int n_tiles, float threshold, R* color) {
+  const int W = cam->width;
+  const int T = sc->n_texels;
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int tile = 0; tile < n_tiles; tile++) {
+    for (int l = 0; l < 64; l++) {
+      int pi, pj, oob;
+      V3 o, d;
+      FN(lane_ray)(cam, tile, l, &pi, &pj, &oob, &o, &d);
+      if (oob) continue;
+      V3 rgb = FN(mk)(0, 0, 0);
+      R log_t = 0;
+      for (int64_t k = offsets[tile]; k < offsets[tile + 1]; k++) {
+        R trans = FN(rexp)(log_t);
+        if (trans < (R)threshold) break;
+        const int site = sites[k];
+        const float* sp = sc->spheres + 4 * (size_t)site;
+        const float* ns = sc->nsigmas + 4 * (size_t)site;
+        V3 c = FN(ld3)(sp);
+        R r = (R)sp[3];
+        FN(sphere_hit) h = FN(ray_sphere)(o, d, c, r);
+        if (!h.hit) continue;
+        R tn = h.t_near, tf = h.t_far;
+        for (int e = sc->adjacency_offsets[site]; e < sc->adjacency_offsets[site + 1]; e++) {
+          const uint16_t* q = sc->adjacency_diff + 4 * (size_t)e;
+          V3 df = FN(mk)((R)orc_f16_to_f32(q[0]), (R)orc_f16_to_f32(q[1]), (R)orc_f16_to_f32(q[2]));
+          R pm = (R)orc_f16_to_f32(q[3]);
+          R dp = FN(dot)(d, df);
+          R t = (pm - FN(dot)(o, df)) / dp;
+          tf = dp >= (R)0 ? FN(rmin)(t, tf) : tf;
+          tn = dp < (R)0 ? FN(rmax)(t, tn) : tn;
+          if (tn > tf) break;
+        }
+        if (tn > tf) continue;
+        FN(plane_fwd_out) pf = FN(plane_fwd)(o, d, tn, c, FN(ld3)(ns), r, 1,
+                                             sc->texel_sites + (size_t)site * T * 3,
+                                             sc->texel_rgbh + (size_t)site * T * 4, T);
+        tf = pf.dp >= (R)0 ? FN(rmin)(pf.t_surf, tf) : tf;
+        tn = pf.dp < (R)0 ? FN(rmax)(pf.t_surf, tn) : tn;
+        R dt = tf - tn;
+        if (dt > (R)0) {
+          R dlt = -(R)ns[3] * dt;
+          R alpha = (R)1 - FN(rexp)(dlt);
+          rgb = FN(add)(rgb, FN(scl)(pf.rgb, alpha * trans));
+          log_t += dlt;
+        }
+      }
+      size_t px = (size_t)pi * W + pj;
+      color[3 * px + 0] = rgb.x;
+      color[3 * px + 1] = rgb.y;
+      color[3 * px + 2] = rgb.z;
+    }
+  }
+}
+
 #undef V3
 #undef FN
 #undef CAT

@@ -119,6 +119,24 @@ void orc_f32_to_f16_array(const float* x, int64_t n, uint16_t* y) {
   for (int64_t i = 0; i < n; i++) y[i] = orc_f32_to_f16(x[i]);
 }
 
+/* benchmark.py:297-302: (c_adj - c, pm_adj - pm), pm = (|c|^2 - r^2)/2, in
+ * binary16; the differences are formed in float32 as torch does. */
+void orc_power_face_encode(const float* spheres, int32_t n, const int32_t* adj, const int32_t* adj_off,
+                           uint16_t* diff) {
+  for (int s = 0; s < n; s++) {
+    const float* sp = spheres + 4 * (size_t)s;
+    float pm_s = 0.5f * ((sp[0] * sp[0] + sp[1] * sp[1] + sp[2] * sp[2]) - sp[3] * sp[3]);
+    for (int e = adj_off[s]; e < adj_off[s + 1]; e++) {
+      const float* ap = spheres + 4 * (size_t)adj[e];
+      float pm_a = 0.5f * ((ap[0] * ap[0] + ap[1] * ap[1] + ap[2] * ap[2]) - ap[3] * ap[3]);
+      diff[4 * (size_t)e + 0] = orc_f32_to_f16(ap[0] - sp[0]);
+      diff[4 * (size_t)e + 1] = orc_f32_to_f16(ap[1] - sp[1]);
+      diff[4 * (size_t)e + 2] = orc_f32_to_f16(ap[2] - sp[2]);
+      diff[4 * (size_t)e + 3] = orc_f32_to_f16(pm_a - pm_s);
+    }
+  }
+}
+
 void orc_f16_to_f32_array(const uint16_t* x, int64_t n, float* y) {
   for (int64_t i = 0; i < n; i++) y[i] = orc_f16_to_f32(x[i]);
 }

@@ -113,6 +113,16 @@ class PfRasterBwdArgs(ctypes.Structure):
     ]
 
 
+class PfRasterRgbArgs(ctypes.Structure):
+    _fields_ = [
+        ("cam", PfCamera),
+        ("scene", PfScene),
+        ("tiles", PfTileLists),
+        ("transmittance_threshold", ctypes.c_float),
+        ("color", ctypes.c_void_p),
+    ]
+
+
 # Exported C symbols and their ctypes signatures (argtypes, restype).
 _VP = ctypes.c_void_p
 SIGNATURES = {
@@ -126,6 +136,7 @@ SIGNATURES = {
     "pf_adj_encode": ([_VP, ctypes.c_int32, _VP, _VP, _VP, _VP], ctypes.c_int),
     "pf_raster_fwd": ([_VP, _VP], ctypes.c_int),
     "pf_raster_bwd": ([_VP, _VP], ctypes.c_int),
+    "pf_raster_rgb": ([_VP, _VP], ctypes.c_int),
     "pf_scrub_nonfinite": ([_VP, ctypes.c_int64, _VP], ctypes.c_int),
     "pf_sh_fwd": ([_VP, _VP, ctypes.c_int32, ctypes.c_int32, _VP, _VP, _VP], ctypes.c_int),
     "pf_sh_bwd": ([_VP, _VP, _VP, ctypes.c_int32, ctypes.c_int32, _VP, _VP, _VP], ctypes.c_int),

@@ -903,6 +903,167 @@ __global__ void __launch_bounds__(WAVE* WPB) raster_bwd_kernel(BwdParams p) {
   }
 }
 
+
+// ------------------------------------------------------ benchmark forward
+// benchmark_kernel (rasterize.py:614-734): colour only; a lane stops once its
+// T < threshold before a site, the wave once no lane is left; faces come from
+// the (c_adj - c, pm_adj - pm) encoding (ray_pface_intersect_diff,
+// rendering_math.py:85-98) and a lane's face loop ends with its interval.
+
+struct RgbParams {
+  RasterCam cam;
+  int n_texels, n_tiles;
+  const float4* spheres;
+  const float4* nsigmas;
+  const float* texel_sites;
+  const float4* texel_rgbh;
+  const int32_t* adj_off;
+  const uint16_t* adj_diff;
+  const int64_t* offsets;
+  const int32_t* sites;
+  float thr;
+  float* color;
+};
+
+template <bool PINHOLE>
+__global__ void __launch_bounds__(WAVE* WPB) raster_rgb_kernel(RgbParams p) {
+  __shared__ SiteRec s_site_all[WPB][WAVE];
+  __shared__ float4 s_face_all[WPB][WAVE];
+  __shared__ TexelLds s_tex_all[WPB][PF_MAX_TEXELS];
+
+  const int wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  bool valid;
+  const int group = xcd_group(blockIdx.x, (p.n_tiles + WPB - 1) / WPB, valid);
+  const int tile = group * WPB + wv;
+  if (!valid || tile >= p.n_tiles) return;
+  SiteRec* s_site = s_site_all[wv];
+  float4* s_face = s_face_all[wv];
+  TexelLds* s_tex = s_tex_all[wv];
+
+  const Ray ray = lane_ray<PINHOLE>(p.cam, tile, lane);
+  const int T = p.n_texels;
+  float log_t = 0.0f;
+  f3 rgb{0.0f, 0.0f, 0.0f};
+  bool alive = !ray.oob;  // out-of-image lanes are never written
+
+  const int64_t start = p.offsets[tile];
+  const int total = (int)(p.offsets[tile + 1] - start);
+  for (int base = 0; base < total && ballot(alive) != 0; base += WAVE) {
+    const int cnt = min(WAVE, total - base);
+    wave_sync();
+    if (lane < cnt) {
+      const int s = p.sites[start + base + lane];
+      s_site[lane].sph = p.spheres[s];
+      s_site[lane].nsg = p.nsigmas[s];
+      s_site[lane].site = s;
+      s_site[lane].a0 = p.adj_off[s];
+      s_site[lane].a1 = p.adj_off[s + 1];
+    }
+    wave_sync();
+    for (int kk = 0; kk < cnt; kk++) {
+      const float trans = fast_exp(log_t);
+      alive = alive && trans >= p.thr;
+      if (ballot(alive) == 0) break;
+      const SiteRec sr = s_site[kk];
+      const f3 c{sr.sph.x, sr.sph.y, sr.sph.z};
+      const float r = sr.sph.w;
+      const f3 pn{sr.nsg.x, sr.nsg.y, sr.nsg.z};
+      const float sigma = sr.nsg.w;
+      const f3 oc = ray.o - c;
+      const f3 co = -oc;
+      float tn, tf;
+      bool open = alive && sphere_hit(oc, ray.d, r, tn, tf);
+      if (ballot(open) == 0) continue;
+      for (int e0 = sr.a0; e0 < sr.a1 && ballot(open) != 0; e0 += WAVE) {
+        const int ec = min(WAVE, sr.a1 - e0);
+        wave_sync();
+        if (lane < ec) {
+          const uint2 raw = *reinterpret_cast<const uint2*>(p.adj_diff + 4 * (int64_t)(e0 + lane));
+          const f3 n{half_bits_to_float((uint16_t)(raw.x & 0xffff)), half_bits_to_float((uint16_t)(raw.x >> 16)),
+                     half_bits_to_float((uint16_t)(raw.y & 0xffff))};
+          float w = half_bits_to_float((uint16_t)(raw.y >> 16));
+          if (PINHOLE) w -= dot(p.cam.eye, n);
+          s_face[lane] = make_float4(n.x, n.y, n.z, w);
+        }
+        wave_sync();
+        if (open) {
+          for (int e = 0; e < ec; e++) {
+            const float4 f = s_face[e];
+            const f3 n{f.x, f.y, f.z};
+            const float dp = dot(ray.d, n);
+            const float num = PINHOLE ? f.w : f.w - dot(ray.o, n);
+            const float t = acc_div(num, dp);
+            tf = (dp >= 0.0f && t < tf) ? t : tf;
+            tn = (dp < 0.0f && t > tn) ? t : tn;
+            if (tn > tf) break;
+          }
+          open = !(tn > tf);
+        }
+      }
+      if (ballot(open) == 0) continue;
+      wave_sync();
+      const bool heights = stage_texels(s_tex, p.texel_sites, p.texel_rgbh, sr.site, T, c, lane);
+      wave_sync();
+      if (open) {
+        // plane_intersection_fwd_local (rasterize.py:67-138)
+        const float dpn = dot(pn, ray.d);
+        const float cn = dot(co, pn);
+        const float kexp = -10.0f * fast_rcp(r * r);
+        const float ts0 = acc_div(cn, dpn);
+        float height = 0.0f;
+        if (heights) {
+          const float tq0 = dpn >= 0.0f ? tn : sel_max(tn, ts0);
+          const f3 x0 = ray.d * tq0 - co;
+          float hsum = 0.0f, ws0 = 0.0f;
+#pragma unroll
+          for (int i = 0; i < PF_MAX_TEXELS; i++) {
+            if (i < T) {
+              const float4 ph = s_tex[i].pos_h;
+              const f3 df{x0.x - ph.x, x0.y - ph.y, x0.z - ph.z};
+              const float w = fast_exp(dot(df, df) * kexp);
+              hsum += w * ph.w;
+              ws0 += w;
+            }
+          }
+          height = hsum * fast_rcp(sel_max(ws0, 1e-20f));
+        }
+        const float ts1 = heights ? acc_div(cn + height, dpn) : ts0;
+        const float tq = dpn >= 0.0f ? tn : sel_max(tn, ts1);
+        const f3 x1 = ray.d * tq - co;
+        f3 rs{0.0f, 0.0f, 0.0f};
+        float ws = 0.0f;
+#pragma unroll
+        for (int i = 0; i < PF_MAX_TEXELS; i++) {
+          if (i < T) {
+            const float4 ph = s_tex[i].pos_h;
+            const float4 cl = s_tex[i].rgb;
+            const f3 df{x1.x - ph.x, x1.y - ph.y, x1.z - ph.z};
+            const float w = fast_exp(dot(df, df) * kexp);
+            rs = rs + f3{cl.x, cl.y, cl.z} * w;
+            ws += w;
+          }
+        }
+        const f3 col = rs * fast_rcp(sel_max(ws, 1e-20f));
+        const float tf2 = (dpn >= 0.0f && ts1 < tf) ? ts1 : tf;
+        const float tn2 = (dpn < 0.0f && ts1 > tn) ? ts1 : tn;
+        const float dt = tf2 - tn2;
+        if (dt > 0.0f) {
+          const float dlt = -sigma * dt;
+          rgb = rgb + col * ((1.0f - fast_exp(dlt)) * trans);
+          log_t += dlt;
+        }
+      }
+    }
+  }
+  if (!ray.oob) {
+    const size_t px = (size_t)ray.pi * p.cam.W + ray.pj;
+    p.color[3 * px + 0] = rgb.x;
+    p.color[3 * px + 1] = rgb.y;
+    p.color[3 * px + 2] = rgb.z;
+  }
+}
+
 __global__ void __launch_bounds__(256) scrub_kernel(float* x, int64_t n) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n && !isfinite(x[i])) x[i] = 0.0f;
@@ -1029,6 +1190,41 @@ int pf_raster_bwd(const pf_raster_bwd_args* a, void* stream) {
     raster_bwd_kernel<false><<<grid, WAVE * WPB, 0, as_stream(stream)>>>(p);
   else
     raster_bwd_kernel<true><<<grid, WAVE * WPB, 0, as_stream(stream)>>>(p);
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+int pf_raster_rgb(const pf_raster_rgb_args* a, void* stream) {
+  if (!a) return set_error(PF_ERR_ARG, "pf_raster_rgb: null args");
+  int st = check_scene(a->scene, "pf_raster_rgb");
+  if (st) return st;
+  const pf_camera& c = a->cam;
+  if (c.width < 2 || c.height < 2) return set_error(PF_ERR_ARG, "pf_raster_rgb: image must be >= 2x2");
+  const int nt = tiles_h(c) * tiles_w(c);
+  if (a->tiles.n_tiles != nt || !a->tiles.offsets) return set_error(PF_ERR_ARG, "pf_raster_rgb: tile lists do not match the camera");
+  if (!a->color) return set_error(PF_ERR_ARG, "pf_raster_rgb: missing colour output");
+  if (a->scene.n_sites > 0 && a->scene.adjacency_offsets && !a->scene.adjacency_diff)
+    return set_error(PF_ERR_ARG, "pf_raster_rgb: the power-face encoding is required");
+  RgbParams p{};
+  p.cam = make_rcam(c);
+  p.n_texels = a->scene.n_texels;
+  p.n_tiles = nt;
+  p.spheres = reinterpret_cast<const float4*>(a->scene.spheres);
+  p.nsigmas = reinterpret_cast<const float4*>(a->scene.nsigmas);
+  p.texel_sites = a->scene.texel_sites;
+  p.texel_rgbh = reinterpret_cast<const float4*>(a->scene.texel_rgbh);
+  p.adj_off = a->scene.adjacency_offsets;
+  p.adj_diff = a->scene.adjacency_diff;
+  p.offsets = a->tiles.offsets;
+  p.sites = a->tiles.sites;
+  p.thr = a->transmittance_threshold;
+  p.color = a->color;
+  const int groups = (nt + WPB - 1) / WPB;
+  const int grid = ((groups + 7) / 8) * 8;
+  if (c.ray_maps)
+    raster_rgb_kernel<false><<<grid, WAVE * WPB, 0, as_stream(stream)>>>(p);
+  else
+    raster_rgb_kernel<true><<<grid, WAVE * WPB, 0, as_stream(stream)>>>(p);
   PF_HIP(hipGetLastError());
   return PF_OK;
 }

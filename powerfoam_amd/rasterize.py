@@ -199,6 +199,35 @@ class Rasterizer:
     def forward(self, *args):
         return RasterGradFn.apply(self, *args)
 
+    # ---------------------------------------------------------------- benchmark
+    def benchmark(self, camera, points, radii, density, normals, texel_sites, texel_rgb, texel_height, adjacency,
+                  adjacency_offsets, adjacency_diff, transmittance_threshold=1e-3):
+        """rasterize.py:1982-2131: colour only, per-ray stop at T < threshold,
+        faces from the (c_adj - c, pm_adj - pm) fp16 encoding of benchmark.py
+        (see power_face_encoding). Attributes are rounded through fp16 when the
+        rasterizer was built with attr_dtype="half", as the reference stores them."""
+        with torch.no_grad():
+            H, W = camera.height, camera.width
+            q = (lambda t: t.to(self.tscalar).to(torch.float32).contiguous())
+            all_spheres = _f32c(torch.cat([points, radii[:, None]], dim=-1))
+            all_nsigmas = q(torch.cat([normals, density[:, None]], dim=-1))
+            all_texel_sites = q(texel_sites)
+            all_texel_rgbh = torch.cat([q(texel_rgb), q(texel_height)[..., None]], dim=-1).contiguous()
+            adjacency = adjacency.to(torch.int32).contiguous()
+            adjacency_offsets = adjacency_offsets.to(torch.int32).contiguous()
+            adjacency_diff = adjacency_diff.to(torch.float16).contiguous()
+            offsets, tile_sites, _, nt = self.bin(camera, all_spheres)
+            color_out = torch.zeros((H, W, 3), dtype=torch.float32, device=points.device)
+            a = _abi.PfRasterRgbArgs()
+            a.cam = camera.to_pf()
+            a.scene = _scene_struct(all_spheres, all_nsigmas, all_texel_sites, all_texel_rgbh, adjacency,
+                                    adjacency_offsets, adjacency_diff)
+            a.tiles = _abi.PfTileLists(nt, offsets.data_ptr(), tile_sites.data_ptr() if tile_sites.numel() else None)
+            a.transmittance_threshold = float(transmittance_threshold)
+            a.color = color_out.data_ptr()
+            self._launch("raster_rgb", self.lib.pf_raster_rgb, _ref(a), stream_handle())
+            return color_out
+
     # ----------------------------------------------------------------- backward
     def _backward(self, grad_ctx, grad_color_in, grad_opacity_in, grad_normal_distance_in, grad_normal_in,
                   grad_quantile_depths_in, grad_err_in, grad_contrib_in, grad_point_err_in,
@@ -262,6 +291,19 @@ class Rasterizer:
         del grad_ctx.rasterizer, grad_ctx.camera
         return (None, None, None, spheres_grad[:, :3], spheres_grad[:, 3], nsigmas_grad[:, 3],
                 nsigmas_grad[:, :3], texel_sites_grad, texel_rgb_grad, texel_height_grad, None, None, None, None)
+
+
+def power_face_encoding(points, radii, adjacency, adjacency_offsets):
+    """The benchmark's fp16 face encoding (benchmark.py:297-302): per edge
+    (c_adj - c, pm_adj - pm) with pm = (|c|^2 - r^2)/2, computed in torch as the
+    reference does."""
+    num_adjs = adjacency_offsets[1:] - adjacency_offsets[:-1]
+    pm = 0.5 * (points.norm(dim=-1) ** 2 - radii ** 2)
+    adj = adjacency.long()
+    self_points = points.repeat_interleave(num_adjs.long(), dim=0)
+    diff = points[adj] - self_points
+    pm_diff = pm[adj] - pm.repeat_interleave(num_adjs.long(), dim=0)
+    return torch.cat([diff, pm_diff[:, None]], dim=-1).to(torch.float16)
 
 
 def _ref(struct):

@@ -177,3 +177,31 @@ def rel_err(a, b):
     b = np.asarray(b, np.float64)
     nb = np.linalg.norm(b)
     return float(np.linalg.norm(a - b) / max(nb, 1e-30))
+
+
+def run_gpu_benchmark(case, dev, threshold=1e-2, attr_dtype="float"):
+    """Rasterizer.benchmark on the GPU; returns (colour, power-face encoding as uint16)."""
+    import torch
+    from powerfoam_amd.rasterize import Rasterizer, power_face_encoding
+
+    T = case.texel_sites.shape[1]
+    r = Rasterizer(_args(T), dev, attr_dtype)
+    tt = lambda a, dt=torch.float32: torch.tensor(np.ascontiguousarray(a), dtype=dt, device=dev)
+    pts, rad = tt(case.points), tt(case.radii)
+    adj, off = tt(case.adjacency, torch.int32), tt(case.adjacency_offsets, torch.int32)
+    diff = power_face_encoding(pts, rad, adj, off)
+    col = r.benchmark(case.cam, pts, rad, tt(case.density), tt(case.normals), tt(case.texel_sites),
+                      tt(case.texel_rgb), tt(case.texel_height), adj, off, diff, threshold)
+    torch.cuda.synchronize()
+    return col.cpu().numpy(), diff.cpu().numpy().view(np.uint16)
+
+
+def run_oracle_benchmark(case, diff, threshold=1e-2, attr_dtype="float"):
+    cam, _keep = case.pf_camera()
+    sp = case.spheres()
+    q = (lambda a: a.astype(np.float16).astype(np.float32)) if attr_dtype == "half" else (lambda a: a)
+    inp = orc.Inputs(sp, q(case.nsigmas()), q(case.texel_sites),
+                     np.concatenate([q(case.texel_rgb), q(case.texel_height)[..., None]], -1),
+                     case.adjacency, case.adjacency_offsets, diff)
+    offsets, sites, _ = orc.bin_lists(cam, sp)
+    return orc.raster_rgb(cam, inp, offsets, sites, threshold=threshold)

@@ -64,3 +64,16 @@ def test_backward_matches_fp64_oracle(dev, cfg):
     for k in harness.GRAD_KEYS:
         err = harness.rel_err(grads[k][keep], ref[k][keep])
         assert err < GRAD_REL_TOL, (k, err)
+
+
+@pytest.mark.parametrize("cfg,attr", [(CASES[0], "float"), (CASES[1], "float"), (CASES[2], "half")],
+                         ids=["case0", "case1", "case2-half"])
+def test_benchmark_matches_fp64_oracle(dev, cfg, attr):
+    """Rasterizer.benchmark (colour only, per-ray stop, fp16 power faces)."""
+    case = harness.small_case(**cfg)
+    print(f"benchmark case {cfg} {attr}", flush=True)
+    col, diff = harness.run_gpu_benchmark(case, dev, attr_dtype=attr)
+    ref = harness.run_oracle_benchmark(case, diff, attr_dtype=attr)
+    err = np.abs(col - ref).max()
+    assert err < ABS_TOL, err
+    assert ref.max() > 0.1

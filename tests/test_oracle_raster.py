@@ -256,3 +256,60 @@ def test_exact_gradients_match_central_differences(mode):
         tested += 1
         assert abs(ana - fds[0]) <= 2e-3 * abs(fds[0]) + 2e-3, (k, idx, ana, fds)
     assert tested >= 200, (tested, skipped)
+
+
+def grid_scene(seed, n=20, T=4):
+    """Sites on a 1/8 grid with radii in {1/4, 3/8, 1/2}: every centre delta
+    and every pm = (|c|^2 - r^2)/2 difference is exact in binary16, so the
+    benchmark's fp16 power-face encoding is exact too."""
+    rng = np.random.default_rng(seed)
+    cells = rng.choice(17 ** 3, n, replace=False)
+    g = np.stack(np.unravel_index(cells, (17, 17, 17)), 1).astype(np.float64)
+    pts = (g - 8) / 8.0
+    radii = rng.choice([0.25, 0.375, 0.5], n)
+    s = synth.scene_uniform(seed, n=n, n_texels=T, sigma_lognormal=(0.5, 0.5))
+    s["points"] = pts.astype(np.float32)
+    s["radii"] = radii.astype(np.float32)
+    s["texel_sites"] = (pts[:, None, :] + 0.02 * rng.normal(size=(n, T, 3))).astype(np.float32)
+    s["texel_height"] = np.zeros((n, T), np.float32)
+    s["texel_rgb"] = np.repeat(rng.uniform(0, 1, (n, 1, 3)), T, axis=1).astype(np.float32)
+    s["adj"], s["off"] = synth.cech_adjacency(s["points"], s["radii"])
+    return s
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_benchmark_forward_matches_exact_renderer(seed):
+    """With exact encodings and no early stop, the benchmark forward equals
+    the independent exact volume renderer."""
+    s = grid_scene(seed)
+    t, cam = camera(40, 30, (0.4, 0.3, 3.2), 50.0)
+    sp, inp = inputs(s, exact_geometry=True)
+    diff = orc.power_face_encode(sp, s["adj"], s["off"])
+    # the encoding is exact: decoding reproduces the float64 differences
+    src = np.repeat(np.arange(sp.shape[0]), np.diff(s["off"]))
+    pm = 0.5 * ((sp[:, :3].astype(np.float64) ** 2).sum(1) - sp[:, 3].astype(np.float64) ** 2)
+    dec = diff.view(np.float16).astype(np.float64)
+    np.testing.assert_array_equal(dec[:, 3], pm[s["adj"]] - pm[src])
+    inp.adjacency_diff = diff
+    offsets, sites, _ = orc.bin_lists(cam, sp)
+    col = orc.raster_rgb(cam, inp, offsets, sites, threshold=0.0)
+    color, opac, _ = exact_render(s, t)
+    assert opac.max() > 0.5
+    np.testing.assert_allclose(col, color, atol=1e-8)
+
+
+def test_benchmark_per_ray_stop():
+    """A ray stops before the first site it reaches with T < threshold: the
+    colour drops by at most the remaining transmittance (colours <= 1) and
+    never rises."""
+    s = grid_scene(3, n=40)
+    t, cam = camera(40, 30, (0.4, 0.3, 3.2), 50.0)
+    sp, inp = inputs(s, exact_geometry=True)
+    inp.adjacency_diff = orc.power_face_encode(sp, s["adj"], s["off"])
+    offsets, sites, _ = orc.bin_lists(cam, sp)
+    full = orc.raster_rgb(cam, inp, offsets, sites, threshold=0.0)
+    thr = 0.3
+    cut = orc.raster_rgb(cam, inp, offsets, sites, threshold=thr)
+    assert np.all(cut <= full + 1e-12)
+    assert np.all(full - cut <= thr + 1e-12)
+    assert np.any(full - cut > 1e-3)  # the stop happens somewhere
