@@ -136,6 +136,50 @@ typedef struct pf_raster_rgb_args {
   float* color; /* [H,W,3], every in-image pixel written */
 } pf_raster_rgb_args;
 
+/* Ray tracer (raytrace.py:115-241). scene.adjacency must be the full power
+ * (weighted Delaunay) adjacency; scene.adjacency_diff holds the power-face
+ * encoding (as pf_raster_rgb_args) or is NULL for exact geometry computed from
+ * the spheres. Rays start in start_sites[pixel] when given, else start_site. */
+typedef struct pf_trace_args {
+  pf_camera cam;
+  pf_scene scene;
+  int32_t start_site;
+  const int32_t* start_sites;   /* [H*W] or NULL */
+  float transmittance_threshold;
+  float depth_quantile;         /* depth output: where T first drops below this */
+  int32_t max_steps;            /* walk length cap per ray (<= 0: 65536) */
+  /* outputs */
+  float* color;    /* [H,W,3] */
+  float* log_t;    /* [H,W]   or NULL */
+  float* normal;   /* [H,W,3] or NULL: sum of alpha T n */
+  float* depth;    /* [H,W]   or NULL: -1 where T never drops below depth_quantile */
+  int32_t* q_step; /* [H,W]   or NULL: walk step of the depth crossing (backward input) */
+  float* q_sigma;  /* [H,W]   or NULL: density of that cell (backward input) */
+  int32_t* cells;  /* [H,W,max_cells] or NULL: visited cells in order */
+  int32_t max_cells;
+  int32_t* n_cells;/* [H,W] or NULL: number of cells visited */
+} pf_trace_args;
+
+/* Exact adjoint of pf_trace_fwd (no reference counterpart: the reference ray
+ * tracer has no backward). Re-walks each ray front to back. */
+typedef struct pf_trace_bwd_args {
+  pf_trace_args fwd;           /* same inputs as the forward; its output pointers are ignored */
+  const float* color;          /* [H,W,3] saved forward outputs */
+  const float* normal;         /* [H,W,3] */
+  const float* log_t;          /* [H,W] */
+  const int32_t* q_step;       /* [H,W] */
+  const float* q_sigma;        /* [H,W] */
+  const float* grad_color;     /* [H,W,3] */
+  const float* grad_normal;    /* [H,W,3] */
+  const float* grad_depth;     /* [H,W] or NULL */
+  const float* grad_opacity;   /* [H,W] */
+  float* grad_spheres;         /* [N,4] accumulated */
+  float* grad_nsigmas;         /* [N,4] */
+  float* grad_texel_sites;     /* [N,T,3] */
+  float* grad_texel_rgb;       /* [N,T,3] */
+  float* grad_texel_height;    /* [N,T] */
+} pf_trace_bwd_args;
+
 /* ---- library ---------------------------------------------------------- */
 int pf_version(void);
 const char* pf_last_error(void);
@@ -198,6 +242,10 @@ int pf_sh_bwd(const float* sh, const float* texel_sites, const float* grad_texel
 
 /* benchmark_kernel rasterize.py:614-734 (Rasterizer.benchmark :1982-2131). */
 int pf_raster_rgb(const pf_raster_rgb_args* args, void* stream);
+
+/* ---- ray tracer (RayTracer.benchmark_kernel raytrace.py:115-241) ------ */
+int pf_trace_fwd(const pf_trace_args* args, void* stream);
+int pf_trace_bwd(const pf_trace_bwd_args* args, void* stream);
 
 /* Replaces non-finite entries with 0 (rasterize.py:1954-1958). */
 int pf_scrub_nonfinite(float* data, int64_t n, void* stream);

@@ -41,6 +41,11 @@ def load():
         lib.orc_adj_encode.restype = None
         lib.orc_f32_to_f16_array.argtypes = [vp, ctypes.c_int64, vp]
         lib.orc_f16_to_f32_array.argtypes = [vp, ctypes.c_int64, vp]
+        for p in ("f32", "f64"):
+            getattr(lib, f"orc_trace_fwd_{p}").argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_float,
+                                                           ctypes.c_float, ctypes.c_int, vp]
+            getattr(lib, f"orc_trace_bwd_{p}").argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_float,
+                                                           ctypes.c_float, ctypes.c_int, vp, vp]
         lib.orc_power_face_encode.argtypes = [vp, ctypes.c_int32, vp, vp, vp]
         lib.orc_power_face_encode.restype = None
         for p in ("f32", "f64"):
@@ -263,4 +268,62 @@ def raster_bwd(cam, inp, offsets, sites, tile_early_stop, log_t, grad_color, gra
     o = _BwdOut(*[_addr(out[k]) for k, _ in _BwdOut._fields_])
     getattr(lib, f"orc_raster_bwd_{precision}")(ctypes.byref(a), ctypes.byref(o))
     out["fragile"] = out["fragile"].astype(bool)
+    return out
+
+
+class _TraceOut(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_void_p) for k in
+                ("color", "log_t", "normal", "depth", "q_step", "q_sigma", "cells", "n_cells", "fragile")] + \
+               [("K", ctypes.c_int)]
+
+
+class _TraceBwdIn(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_void_p) for k in
+                ("color", "normal", "log_t", "q_step", "q_sigma", "g_color", "g_normal", "g_depth", "g_opacity")]
+
+
+def trace_fwd(cam, inp, start_site, start_sites=None, threshold=1e-3, depth_q=0.5, max_steps=1 << 16,
+              record_cells=0, precision="f64"):
+    """Ray-tracer forward; inp.adjacency_diff is the power-face encoding or None (exact).
+    `fragile` [H,W] marks rays that cross an ill-conditioned cell (orc_trace_fwd)."""
+    lib = load()
+    dt = np.float64 if precision == "f64" else np.float32
+    H, W = cam.height, cam.width
+    P = H * W
+    ss = None if start_sites is None else np.ascontiguousarray(start_sites, np.int32)
+    out = dict(color=np.zeros((H, W, 3), dt), log_t=np.zeros((H, W), dt), normal=np.zeros((H, W, 3), dt),
+               depth=np.zeros((H, W), dt), q_step=np.zeros((H, W), np.int32), q_sigma=np.zeros((H, W), dt),
+               cells=np.full((H, W, max(record_cells, 1)), -1, np.int32) if record_cells else None,
+               n_cells=np.zeros((H, W), np.int32), fragile=np.zeros((H, W), np.uint8))
+    o = _TraceOut(*[_addr(out[k]) for k, _ in _TraceOut._fields_[:-1]], int(record_cells))
+    sc = inp.struct()
+    getattr(lib, f"orc_trace_fwd_{precision}")(ctypes.byref(cam), ctypes.byref(sc), _p(ss), int(start_site),
+                                               threshold, depth_q, max_steps, ctypes.byref(o))
+    out["fragile"] = out["fragile"].astype(bool)
+    return out
+
+
+def trace_bwd(cam, inp, start_site, fwd, g_color, g_normal, g_depth, g_opacity, start_sites=None, threshold=1e-3,
+              depth_q=0.5, max_steps=1 << 16, precision="f64"):
+    """Exact adjoint of trace_fwd (front to back); `fwd` must come from trace_fwd
+    at the same precision."""
+    lib = load()
+    dt = np.float64 if precision == "f64" else np.float32
+    ss = None if start_sites is None else np.ascontiguousarray(start_sites, np.int32)
+    keep = dict(color=np.ascontiguousarray(fwd["color"], dt),
+                normal=np.ascontiguousarray(fwd["normal"], dt),
+                log_t=np.ascontiguousarray(fwd["log_t"], dt),
+                q_step=np.ascontiguousarray(fwd["q_step"], np.int32),
+                q_sigma=np.ascontiguousarray(fwd["q_sigma"], dt),
+                g_color=_f32(g_color), g_normal=_f32(g_normal), g_depth=_f32(g_depth), g_opacity=_f32(g_opacity))
+    bi = _TraceBwdIn(*[_addr(keep[k]) for k, _ in _TraceBwdIn._fields_])
+    N, T = inp.n, inp.T
+    out = dict(grad_spheres=np.zeros((N, 4), dt), grad_nsigmas=np.zeros((N, 4), dt),
+               grad_texel_sites=np.zeros((N, T, 3), dt), grad_texel_rgb=np.zeros((N, T, 3), dt),
+               grad_texel_height=np.zeros((N, T), dt), fragile=None)
+    o = _BwdOut(*[_addr(out[k]) for k, _ in _BwdOut._fields_])
+    sc = inp.struct()
+    getattr(lib, f"orc_trace_bwd_{precision}")(ctypes.byref(cam), ctypes.byref(sc), _p(ss), int(start_site),
+                                               threshold, depth_q, max_steps, ctypes.byref(bi), ctypes.byref(o))
+    del out["fragile"]
     return out

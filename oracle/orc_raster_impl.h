@@ -719,6 +719,356 @@ void FN(orc_raster_rgb)(const pf_camera* cam, const pf_scene* sc, const int64_t*
   }
 }
 
+
+/* ------------------------------------------------------------ ray tracer
+ * RayTracer.benchmark_kernel (raytrace.py:115-241): per pixel, walk the power
+ * cells from a start cell; the next cell is the neighbour whose face the ray
+ * leaves through first (argmin t over faces with d.n >= 0, first in list order
+ * on ties); a cell shades when the ray meets its ball (not within 4 r of the
+ * centre), its interval is non-empty and sigma >= 1e-3. Faces come from the
+ * (c_adj - c, pm_adj - pm) binary16 encoding, or from the spheres when no
+ * encoding is given (exact geometry). Added outputs: log T, the alpha-weighted
+ * normal, the depth where T first drops below depth_q (-1 if never), the
+ * visited cells (first K) and their count. */
+typedef struct {
+  R* color;
+  R* log_t;
+  R* normal;
+  R* depth;
+  int32_t* q_step;  /* walk step of the depth crossing, -1 if none */
+  R* q_sigma;
+  int32_t* cells;   /* [P, K] or NULL */
+  int32_t* n_cells; /* [P] */
+  uint8_t* fragile; /* [P] or NULL: the ray crosses an ill-conditioned cell */
+  int K;
+} FN(trace_out);
+
+static inline void FN(trace_face)(const pf_scene* sc, int64_t e, V3 c, R r, V3* n, R* pm) {
+  if (sc->adjacency_diff) {
+    const uint16_t* q = sc->adjacency_diff + 4 * e;
+    *n = FN(mk)((R)orc_f16_to_f32(q[0]), (R)orc_f16_to_f32(q[1]), (R)orc_f16_to_f32(q[2]));
+    *pm = (R)orc_f16_to_f32(q[3]);
+  } else {
+    const float* a = sc->spheres + 4 * (size_t)sc->adjacency[e];
+    V3 ac = FN(ld3)(a);
+    R ra = (R)a[3];
+    *n = FN(sub)(ac, c);
+    *pm = (R)0.5 * (FN(dot)(ac, ac) - ra * ra) - (R)0.5 * (FN(dot)(c, c) - r * r);
+  }
+}
+
+/* The ray of pixel (i, j) (camera.py:19-24 / ray_maps). */
+static inline void FN(pixel_ray)(const pf_camera* cam, int i, int j, V3* o, V3* d) {
+  if (cam->ray_maps == NULL) {
+    V3 right = FN(ld3)(cam->right), up = FN(ld3)(cam->up);
+    V3 fwd = FN(nrm)(FN(crs)(up, right));
+    R x = (R)2 * (R)j / ((R)cam->width - (R)1) - (R)1;
+    R y = (R)1 - (R)2 * (R)i / ((R)cam->height - (R)1);
+    V3 rd = FN(add)(FN(add)(FN(scl)(right, x), FN(scl)(up, y)), fwd);
+    *d = FN(dvs)(rd, FN(len)(rd));
+    *o = FN(ld3)(cam->eye);
+  } else {
+    const float* rm = cam->ray_maps + ((size_t)i * cam->width + j) * 6;
+    *o = FN(ld3)(rm);
+    V3 rd = FN(ld3)(rm + 3);
+    *d = FN(dvs)(rd, FN(len)(rd));
+  }
+}
+
+/* A cell of the walk is ill-conditioned when a float32 rounding of its inputs
+ * can change which bound binds, whether the ball is met, or which cell comes
+ * next, or is amplified past 1e-4: a grazing ball (h^2 = r^2 - |oc x d|^2 is
+ * computed with an absolute error near 1e-7 |oc| r, and the interval ends move
+ * like 1/h, so h^2 < 1e-3 |oc| r counts), a centre within a hair of the 4 r
+ * exclusion, a face nearly parallel to the ray, or two candidate bounds
+ * (faces and the ball) within a hair of each other on either end of the
+ * interval. Rays through such cells are left out of gradient comparisons
+ * between precisions. */
+static inline int FN(trace_fragile)(const pf_scene* sc, V3 o, V3 d, V3 c, R r, FN(sphere_hit) h, int a0, int a1,
+                                    R tn, R tf, R pt_far) {
+  const V3 cr = FN(crs)(FN(sub)(o, c), d);
+  const double h2 = (double)r * (double)r - (double)FN(dot)(cr, cr);
+  const double dist = (double)FN(len)(FN(sub)(c, o));
+  const double tol = 1e-5 * (1.0 + fabs((double)tf));
+  if (fabs(h2) < 1e-3 * dist * (double)r || fabs(dist - 4.0 * (double)r) < 1e-5 * (double)r) return 1;
+  if (h.hit && fabs((double)(tf - tn)) < tol) return 1;
+  int near_ties = h.hit && fabs((double)(h.t_near - tn)) < tol;
+  int far_ties = h.hit && fabs((double)(h.t_far - tf)) < tol;
+  int next_ties = 0;
+  for (int e = a0; e < a1; e++) {
+    V3 n;
+    R pm;
+    FN(trace_face)(sc, e, c, r, &n, &pm);
+    const R dp = FN(dot)(d, n);
+    if (fabs((double)dp) < 1e-6 * (double)FN(len)(n)) return 1;
+    const R t = (pm - FN(dot)(o, n)) / dp;
+    if (dp >= (R)0) {
+      far_ties += fabs((double)(t - tf)) < tol;
+      next_ties += fabs((double)(t - pt_far)) < tol;
+    } else {
+      near_ties += fabs((double)(t - tn)) < tol;
+    }
+  }
+  return near_ties > 1 || far_ties > 1 || next_ties > 1;
+}
+
+void FN(orc_trace_fwd)(const pf_camera* cam, const pf_scene* sc, const int32_t* start_sites, int start_site,
+                       float threshold, float depth_q, int max_steps, FN(trace_out) * out) {
+  const int H = cam->height, W = cam->width, T = sc->n_texels;
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t px = 0; px < (int64_t)H * W; px++) {
+    const int i = (int)(px / W), j = (int)(px % W);
+    V3 o, d;
+    FN(pixel_ray)(cam, i, j, &o, &d);
+    int prim = start_sites ? start_sites[px] : start_site;
+    R log_t = 0, depth = -1, qsig = 0;
+    int qstep = -1, step = 0, frag = 0;
+    V3 rgb = FN(mk)(0, 0, 0), nacc = FN(mk)(0, 0, 0);
+    while (1) {
+      R trans = FN(rexp)(log_t);
+      if (trans < (R)threshold || prim == 0x7FFFFFFF || prim < 0 || step >= max_steps) break;
+      if (out->cells && step < out->K) out->cells[px * out->K + step] = prim;
+      const float* sp = sc->spheres + 4 * (size_t)prim;
+      const float* ns = sc->nsigmas + 4 * (size_t)prim;
+      V3 c = FN(ld3)(sp);
+      R r = (R)sp[3];
+      FN(sphere_hit) h = FN(ray_sphere)(o, d, c, r);
+      int hit = h.hit;
+      if (FN(len)(FN(sub)(c, o)) < (R)4 * r) hit = 0;
+      R tn = h.t_near, tf = h.t_far;
+      int next = 0x7FFFFFFF;
+      R pt_far = (R)1e10;
+      for (int e = sc->adjacency_offsets[prim]; e < sc->adjacency_offsets[prim + 1]; e++) {
+        V3 n;
+        R pm;
+        FN(trace_face)(sc, e, c, r, &n, &pm);
+        R dp = FN(dot)(d, n);
+        R t = (pm - FN(dot)(o, n)) / dp;
+        if (dp >= (R)0 && t < pt_far) {
+          next = sc->adjacency[e];
+          pt_far = t;
+        }
+        tf = dp >= (R)0 ? FN(rmin)(t, tf) : tf;
+        tn = dp < (R)0 ? FN(rmax)(t, tn) : tn;
+      }
+      R sigma = (R)ns[3];
+      if (out->fragile && FN(trace_fragile)(sc, o, d, c, r, h, sc->adjacency_offsets[prim],
+                                            sc->adjacency_offsets[prim + 1], tn, tf, pt_far))
+        frag = 1;
+      if (!hit || tn > tf || sigma < (R)1e-3) {
+        prim = next;
+        step++;
+        continue;
+      }
+      FN(plane_fwd_out) pf = FN(plane_fwd)(o, d, tn, c, FN(ld3)(ns), r, 1, sc->texel_sites + (size_t)prim * T * 3,
+                                           sc->texel_rgbh + (size_t)prim * T * 4, T);
+      const double tsel = pf.dp >= (R)0 ? (double)tf : (double)tn;
+      tf = pf.dp >= (R)0 ? FN(rmin)(pf.t_surf, tf) : tf;
+      tn = pf.dp < (R)0 ? FN(rmax)(pf.t_surf, tn) : tn;
+      R dt = tf - tn;
+      if (out->fragile) {
+        /* surface nearly parallel to the ray, its bound nearly tied with the
+         * other bound on that end, a near-empty interval, or T after this
+         * cell within a hair of the depth quantile or the early-stop level */
+        const double tol = 1e-5 * (1.0 + fabs((double)tf));
+        frag |= fabs((double)pf.dp) < 1e-6 || fabs((double)pf.t_surf - tsel) < tol || fabs((double)dt) < tol;
+        if (dt > (R)0) {
+          const double t_end = exp((double)log_t - (double)sigma * (double)dt);
+          frag |= fabs(t_end - (double)depth_q) < 1e-5 || fabs(t_end - (double)threshold) < 1e-5 * threshold;
+        }
+      }
+      if (dt > (R)0) {
+        R dlt = -sigma * dt;
+        R alpha = (R)1 - FN(rexp)(dlt);
+        rgb = FN(add)(rgb, FN(scl)(pf.rgb, alpha * trans));
+        nacc = FN(add)(nacc, FN(scl)(FN(ld3)(ns), alpha * trans));
+        log_t += dlt;
+        if (qstep < 0 && FN(rexp)(log_t) < (R)depth_q) {
+          depth = tn + FN(rlog)(trans / (R)depth_q) / sigma;
+          qstep = step;
+          qsig = sigma;
+        }
+      }
+      prim = next;
+      step++;
+    }
+    out->color[3 * px + 0] = rgb.x;
+    out->color[3 * px + 1] = rgb.y;
+    out->color[3 * px + 2] = rgb.z;
+    if (out->log_t) out->log_t[px] = log_t;
+    if (out->normal) {
+      out->normal[3 * px + 0] = nacc.x;
+      out->normal[3 * px + 1] = nacc.y;
+      out->normal[3 * px + 2] = nacc.z;
+    }
+    if (out->depth) out->depth[px] = depth;
+    if (out->q_step) out->q_step[px] = qstep;
+    if (out->q_sigma) out->q_sigma[px] = qsig;
+    if (out->n_cells) out->n_cells[px] = step;
+    if (out->fragile) out->fragile[px] = (uint8_t)frag;
+  }
+}
+
+/* Exact adjoint of orc_trace_fwd, front to back: the suffix sums a cell's
+ * optical depth u = sigma dt acts on are the saved final colour/normal minus
+ * the running prefix, so no cell stack is kept:
+ *   dL/du_k = g_C.(c_k T_{k+1} - S^C_k) + g_N.(n_k T_{k+1} - S^N_k) + g_O T_end
+ *             - [k before the depth cell] g_D / sigma*,
+ * plus the depth cell's own terms. Face gradients use the full-precision
+ * spheres; texel gradients follow plane_intersection_bwd without cut-offs. */
+typedef struct {
+  const R* color;
+  const R* normal;
+  const R* log_t;
+  const int32_t* q_step;
+  const R* q_sigma;
+  const float* g_color;
+  const float* g_normal;
+  const float* g_depth;
+  const float* g_opacity;
+} FN(trace_bwd_in);
+
+void FN(orc_trace_bwd)(const pf_camera* cam, const pf_scene* sc, const int32_t* start_sites, int start_site,
+                       float threshold, float depth_q, int max_steps, const FN(trace_bwd_in) * in,
+                       FN(bwd_out) * out) {
+  const int H = cam->height, W = cam->width, T = sc->n_texels;
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t px = 0; px < (int64_t)H * W; px++) {
+    const int i = (int)(px / W), j = (int)(px % W);
+    V3 o, d;
+    FN(pixel_ray)(cam, i, j, &o, &d);
+    const V3 gC = FN(ld3)(in->g_color + 3 * px), gN = FN(ld3)(in->g_normal + 3 * px);
+    const R gD = (R)in->g_depth[px], gO = (R)in->g_opacity[px];
+    const V3 Cf = FN(mk)(in->color[3 * px], in->color[3 * px + 1], in->color[3 * px + 2]);
+    const V3 Nf = FN(mk)(in->normal[3 * px], in->normal[3 * px + 1], in->normal[3 * px + 2]);
+    const R Tend = FN(rexp)(in->log_t[px]);
+    const int qstep = in->q_step[px];
+    const R qsig = in->q_sigma[px];
+    int prim = start_sites ? start_sites[px] : start_site;
+    R log_t = 0;
+    int step = 0;
+    V3 Cp = FN(mk)(0, 0, 0), Np = FN(mk)(0, 0, 0);
+    while (1) {
+      R trans = FN(rexp)(log_t);
+      if (trans < (R)threshold || prim == 0x7FFFFFFF || prim < 0 || step >= max_steps) break;
+      const float* sp = sc->spheres + 4 * (size_t)prim;
+      const float* ns = sc->nsigmas + 4 * (size_t)prim;
+      V3 c = FN(ld3)(sp);
+      R r = (R)sp[3];
+      FN(sphere_hit) h = FN(ray_sphere)(o, d, c, r);
+      int hit = h.hit;
+      if (FN(len)(FN(sub)(c, o)) < (R)4 * r) hit = 0;
+      R tn = h.t_near, tf = h.t_far;
+      int tn_id = tn < (R)1e-8 ? -2 : -1, tf_id = tf < (R)1e-8 ? -2 : -1;
+      const int a0 = sc->adjacency_offsets[prim], a1 = sc->adjacency_offsets[prim + 1];
+      const int n_adj = a1 - a0;
+      int next = 0x7FFFFFFF;
+      R pt_far = (R)1e10;
+      for (int e = a0; e < a1; e++) {
+        V3 n;
+        R pm;
+        FN(trace_face)(sc, e, c, r, &n, &pm);
+        R dp = FN(dot)(d, n);
+        R t = (pm - FN(dot)(o, n)) / dp;
+        if (dp >= (R)0 && t < pt_far) {
+          next = sc->adjacency[e];
+          pt_far = t;
+        }
+        if (dp >= (R)0 && t < tf) tf_id = e - a0;
+        tf = dp >= (R)0 ? FN(rmin)(t, tf) : tf;
+        if (dp < (R)0 && t > tn) tn_id = e - a0;
+        tn = dp < (R)0 ? FN(rmax)(t, tn) : tn;
+      }
+      R sigma = (R)ns[3];
+      if (!hit || tn > tf || sigma < (R)1e-3) {
+        prim = next;
+        step++;
+        continue;
+      }
+      const R tn0 = tn;
+      const int tn0_id = tn_id, tf0_id = tf_id;
+      const float* tsites = sc->texel_sites + (size_t)prim * T * 3;
+      const float* trgbh = sc->texel_rgbh + (size_t)prim * T * 4;
+      V3 pn = FN(ld3)(ns);
+      FN(plane_fwd_out) pf = FN(plane_fwd)(o, d, tn, c, pn, r, 1, tsites, trgbh, T);
+      if (pf.dp >= (R)0 && pf.t_surf < tf) tf_id = n_adj;
+      tf = pf.dp >= (R)0 ? FN(rmin)(pf.t_surf, tf) : tf;
+      if (pf.dp < (R)0 && pf.t_surf > tn) tn_id = n_adj;
+      tn = pf.dp < (R)0 ? FN(rmax)(pf.t_surf, tn) : tn;
+      R dt = tf - tn;
+      if (dt > (R)0) {
+        R u = sigma * dt;
+        R e_u = FN(rexp)(-u);
+        R alpha = (R)1 - e_u;
+        R Tn = trans * e_u;
+        V3 Ck = FN(scl)(pf.rgb, alpha * trans), Nk = FN(scl)(pn, alpha * trans);
+        Cp = FN(add)(Cp, Ck);
+        Np = FN(add)(Np, Nk);
+        V3 SC = FN(sub)(Cf, Cp), SN = FN(sub)(Nf, Np);
+        R dLdu = FN(dot)(gC, FN(sub)(FN(scl)(pf.rgb, Tn), SC)) + FN(dot)(gN, FN(sub)(FN(scl)(pn, Tn), SN)) +
+                 gO * Tend;
+        R dLdsig = 0, dLdtn = 0, dLdtf = 0;
+        if (qstep >= 0 && step < qstep) dLdu += -gD / qsig;
+        if (step == qstep) {
+          dLdtn += gD;
+          dLdsig += -gD * FN(rlog)(trans / (R)depth_q) / (sigma * sigma);
+        }
+        dLdsig += dLdu * dt;
+        dLdtf += dLdu * sigma;
+        dLdtn -= dLdu * sigma;
+        V3 dLdcol = FN(scl)(gC, alpha * trans);
+        V3 dLdnorm = FN(scl)(gN, alpha * trans);
+        R dLdts = (tn_id == n_adj) ? dLdtn : ((tf_id == n_adj) ? dLdtf : (R)0);
+        V3 pdc, pdn;
+        R dtq;
+        FN(plane_bwd)(o, d, tn0, c, pn, r, 1, tsites, trgbh, &pf, dLdcol, dLdts, T, 1,
+                      out->grad_texel_sites + (size_t)prim * T * 3, out->grad_texel_rgb + (size_t)prim * T * 3,
+                      out->grad_texel_height + (size_t)prim * T, &pdc, &pdn, &dtq);
+        V3 dLdc = pdc;
+        dLdnorm = FN(add)(dLdnorm, pdn);
+        R dLdr = 0;
+        if (tn_id != n_adj) dtq += dLdtn;
+        for (int side = 0; side < 2; side++) {
+          int tid = side == 0 ? tn0_id : tf0_id;
+          R g = side == 0 ? dtq : dLdtf;
+          if (side == 1 && tf_id == n_adj) continue;
+          if (tid == -1) {
+            FN(sphere_grad) sg = FN(ray_sphere_bwd)(o, d, c, r);
+            dLdc = FN(add)(dLdc, FN(scl)(side == 0 ? sg.dtn_dc : sg.dtf_dc, g));
+            dLdr += g * (side == 0 ? sg.dtn_dr : sg.dtf_dr);
+          } else if (tid >= 0 && tid < n_adj) {
+            int adj = sc->adjacency[a0 + tid];
+            const float* asp = sc->spheres + 4 * (size_t)adj;
+            V3 dtdc, dtda;
+            R dtdr, dtdra;
+            FN(ray_pface_bwd)(o, d, c, r, FN(ld3)(asp), (R)asp[3], &dtdc, &dtdr, &dtda, &dtdra);
+            dLdc = FN(add)(dLdc, FN(scl)(dtdc, g));
+            dLdr += g * dtdr;
+            R* ga = out->grad_spheres + 4 * (size_t)adj;
+            FN(atomic_addr)(&ga[0], g * dtda.x);
+            FN(atomic_addr)(&ga[1], g * dtda.y);
+            FN(atomic_addr)(&ga[2], g * dtda.z);
+            FN(atomic_addr)(&ga[3], g * dtdra);
+          }
+        }
+        R* gs = out->grad_spheres + 4 * (size_t)prim;
+        FN(atomic_addr)(&gs[0], dLdc.x);
+        FN(atomic_addr)(&gs[1], dLdc.y);
+        FN(atomic_addr)(&gs[2], dLdc.z);
+        FN(atomic_addr)(&gs[3], dLdr);
+        R* gq = out->grad_nsigmas + 4 * (size_t)prim;
+        FN(atomic_addr)(&gq[0], dLdnorm.x);
+        FN(atomic_addr)(&gq[1], dLdnorm.y);
+        FN(atomic_addr)(&gq[2], dLdnorm.z);
+        FN(atomic_addr)(&gq[3], dLdsig);
+        log_t -= u;
+      }
+      prim = next;
+      step++;
+    }
+  }
+}
+
 #undef V3
 #undef FN
 #undef CAT

@@ -123,6 +123,34 @@ class PfRasterRgbArgs(ctypes.Structure):
     ]
 
 
+class PfTraceArgs(ctypes.Structure):
+    _fields_ = [
+        ("cam", PfCamera),
+        ("scene", PfScene),
+        ("start_site", ctypes.c_int32),
+        ("start_sites", ctypes.c_void_p),
+        ("transmittance_threshold", ctypes.c_float),
+        ("depth_quantile", ctypes.c_float),
+        ("max_steps", ctypes.c_int32),
+        ("color", ctypes.c_void_p),
+        ("log_t", ctypes.c_void_p),
+        ("normal", ctypes.c_void_p),
+        ("depth", ctypes.c_void_p),
+        ("q_step", ctypes.c_void_p),
+        ("q_sigma", ctypes.c_void_p),
+        ("cells", ctypes.c_void_p),
+        ("max_cells", ctypes.c_int32),
+        ("n_cells", ctypes.c_void_p),
+    ]
+
+
+class PfTraceBwdArgs(ctypes.Structure):
+    _fields_ = [("fwd", PfTraceArgs)] + [(k, ctypes.c_void_p) for k in (
+        "color", "normal", "log_t", "q_step", "q_sigma", "grad_color", "grad_normal", "grad_depth",
+        "grad_opacity", "grad_spheres", "grad_nsigmas", "grad_texel_sites", "grad_texel_rgb",
+        "grad_texel_height")]
+
+
 # Exported C symbols and their ctypes signatures (argtypes, restype).
 _VP = ctypes.c_void_p
 SIGNATURES = {
@@ -137,6 +165,8 @@ SIGNATURES = {
     "pf_raster_fwd": ([_VP, _VP], ctypes.c_int),
     "pf_raster_bwd": ([_VP, _VP], ctypes.c_int),
     "pf_raster_rgb": ([_VP, _VP], ctypes.c_int),
+    "pf_trace_fwd": ([_VP, _VP], ctypes.c_int),
+    "pf_trace_bwd": ([_VP, _VP], ctypes.c_int),
     "pf_scrub_nonfinite": ([_VP, ctypes.c_int64, _VP], ctypes.c_int),
     "pf_sh_fwd": ([_VP, _VP, ctypes.c_int32, ctypes.c_int32, _VP, _VP, _VP], ctypes.c_int),
     "pf_sh_bwd": ([_VP, _VP, _VP, ctypes.c_int32, ctypes.c_int32, _VP, _VP, _VP], ctypes.c_int),

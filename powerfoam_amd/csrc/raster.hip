@@ -44,21 +44,6 @@ constexpr int WAVE = 64;
 #define PF_WPB 4
 #endif
 constexpr int WPB = PF_WPB;  // tiles (waves) per workgroup
-constexpr float LOG2E = 1.4426950408889634f;
-
-__device__ __forceinline__ float fast_exp(float x) { return __builtin_amdgcn_exp2f(x * LOG2E); }
-__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
-// num / den from v_rcp_f32 plus one Newton step on the quotient: within about
-// half an ulp, so interval ends agree with a correctly rounded division except
-// at exact ties (the interval bounds decide which sites a ray shades).
-__device__ __forceinline__ float acc_div(float num, float den) {
-  const float r = __builtin_amdgcn_rcpf(den);
-  const float q = num * r;
-  return __builtin_fmaf(__builtin_fmaf(-den, q, num), r, q);
-}
-__device__ __forceinline__ float sel_min(float a, float b) { return a < b ? a : b; }
-__device__ __forceinline__ float sel_max(float a, float b) { return a > b ? a : b; }
-
 // Orders this wave's LDS traffic: LDS operations of one wave execute in
 // program order, so only the compiler has to be kept from moving them.
 __device__ __forceinline__ void wave_sync() {
@@ -135,23 +120,6 @@ __device__ __forceinline__ Ray lane_ray(const RasterCam& cam, int tile, int lane
   }
   r.d = d * __builtin_amdgcn_rsqf(dot(d, d));
   return r;
-}
-
-// ray-sphere (rendering_math.py:4-23) with the perpendicular-distance
-// discriminant; oc = o - c, |d| = 1
-__device__ __forceinline__ bool sphere_hit(f3 oc, f3 d, float r, float& tn, float& tf) {
-  const float b = dot(oc, d);
-  const f3 cr = cross(oc, d);
-  const float h2 = r * r - dot(cr, cr);
-  tn = 0.0f;
-  tf = 0.0f;
-  if (h2 < 0.0f) return false;
-  const float h = __builtin_sqrtf(h2);
-  const float far = -b + h, near = -b - h;
-  if (near < 0.0f && far < 0.0f) return false;
-  tn = near < 0.0f ? 0.0f : near;
-  tf = far;
-  return true;
 }
 
 struct SiteRec {

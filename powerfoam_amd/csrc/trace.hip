@@ -1,0 +1,582 @@
+// Ray tracer: per-pixel walk through the power cells, forward and backward.
+//
+//   RayTracer.benchmark_kernel  raytrace.py:115-241
+//   RayTracer.benchmark         raytrace.py:247-309
+//
+// Each lane walks one pixel's ray from a start cell; the next cell is the
+// neighbour whose face the ray leaves through first (argmin t over faces with
+// d.n >= 0, the first in list order on ties). The walk -- ray direction, face
+// crossings and the choice of the next cell -- is evaluated without FMA
+// contraction and with IEEE division and square root, in the operation order
+// of the FP32 oracle, so visited cell sequences are bit-exact; shading uses the
+// same fast paths as the rasterizer. One wave covers an 8x8 pixel tile so that
+// neighbouring rays walk neighbouring cells, and lanes whose rays have ended
+// drop out of the loop (the wave runs while any lane walks).
+//
+// Outputs beyond the reference (which returns colour only): log T, the
+// alpha-weighted normal, the depth where T first drops below depth_q, and the
+// visited cells. The backward is the exact adjoint computed front to back: a
+// cell's optical depth u = sigma dt acts on the suffix sums of colour and
+// normal, which are the saved final values minus the running prefix, so the
+// backward re-walks the ray instead of storing it.
+#include "pf_common.hpp"
+
+namespace pf {
+
+constexpr int TWAVE = 64;
+constexpr int TWPB = 4;
+constexpr int SENTINEL = 0x7FFFFFFF;
+
+struct TraceParams {
+  int W, H, tiles_h, n_tiles;
+  f3 eye, right, up;
+  const float* ray_maps;
+  int n_texels;
+  const float4* spheres;
+  const float4* nsigmas;
+  const float* texel_sites;
+  const float4* texel_rgbh;
+  const int32_t* adjacency;
+  const int32_t* adj_off;
+  const uint16_t* adj_diff;
+  int start_site;
+  const int32_t* start_sites;
+  float thr, depth_q;
+  int max_steps;
+  // forward outputs (optional except colour)
+  float* color;
+  float* log_t;
+  float* normal;
+  float* depth;
+  int32_t* q_step;
+  float* q_sigma;
+  int32_t* cells;
+  int K;
+  int32_t* n_cells;
+  // backward inputs / outputs
+  const float* s_color;
+  const float* s_normal;
+  const float* s_log_t;
+  const int32_t* s_q_step;
+  const float* s_q_sigma;
+  const float* g_color;
+  const float* g_normal;
+  const float* g_depth;
+  const float* g_opacity;
+  float* g_spheres;
+  float* g_nsigmas;
+  float* g_tsites;
+  float* g_trgb;
+  float* g_theight;
+};
+
+// camera.py:19-24 / ray_maps, in the FP32 oracle's operation order
+__device__ __forceinline__ void walk_ray(const TraceParams& p, int i, int j, f3& o, f3& d) {
+#pragma clang fp contract(off)
+  f3 rd;
+  if (!p.ray_maps) {
+    const f3 cr{p.up.y * p.right.z - p.up.z * p.right.y, p.up.z * p.right.x - p.up.x * p.right.z,
+                p.up.x * p.right.y - p.up.y * p.right.x};
+    const float l = sqrtf(cr.x * cr.x + cr.y * cr.y + cr.z * cr.z);
+    const f3 fw = l > 0.0f ? f3{cr.x / l, cr.y / l, cr.z / l} : f3{0.0f, 0.0f, 0.0f};
+    const float x = 2.0f * (float)j / ((float)p.W - 1.0f) - 1.0f;
+    const float y = 1.0f - 2.0f * (float)i / ((float)p.H - 1.0f);
+    rd = f3{p.right.x * x + p.up.x * y + fw.x, p.right.y * x + p.up.y * y + fw.y, p.right.z * x + p.up.z * y + fw.z};
+    o = p.eye;
+  } else {
+    const float* rm = p.ray_maps + ((size_t)i * p.W + j) * 6;
+    o = f3{rm[0], rm[1], rm[2]};
+    rd = f3{rm[3], rm[4], rm[5]};
+  }
+  const float len = sqrtf(rd.x * rd.x + rd.y * rd.y + rd.z * rd.z);
+  d = f3{rd.x / len, rd.y / len, rd.z / len};
+}
+
+// Face e of a cell (centre c, radius r): normal n and offset pm, from the
+// binary16 (c_adj - c, pm_adj - pm) encoding or the exact spheres.
+__device__ __forceinline__ void walk_face(const TraceParams& p, int e, f3 c, float r, f3& n, float& pm) {
+#pragma clang fp contract(off)
+  if (p.adj_diff) {
+    const uint2 raw = *reinterpret_cast<const uint2*>(p.adj_diff + 4 * (int64_t)e);
+    n = f3{half_bits_to_float((uint16_t)(raw.x & 0xffff)), half_bits_to_float((uint16_t)(raw.x >> 16)),
+           half_bits_to_float((uint16_t)(raw.y & 0xffff))};
+    pm = half_bits_to_float((uint16_t)(raw.y >> 16));
+  } else {
+    const float4 a = p.spheres[p.adjacency[e]];
+    n = f3{a.x - c.x, a.y - c.y, a.z - c.z};
+    pm = 0.5f * ((a.x * a.x + a.y * a.y + a.z * a.z) - a.w * a.w) - 0.5f * ((c.x * c.x + c.y * c.y + c.z * c.z) - r * r);
+  }
+}
+
+// Crossing of face (n, pm): t = (pm - o.n) / (d.n), exactly as the oracle.
+__device__ __forceinline__ float walk_cross(f3 o, f3 d, f3 n, float pm, float& dp) {
+#pragma clang fp contract(off)
+  dp = d.x * n.x + d.y * n.y + d.z * n.z;
+  const float on = o.x * n.x + o.y * n.y + o.z * n.z;
+  return (pm - on) / dp;
+}
+
+// Texel colour at the cell entry (plane_intersection_fwd_local,
+// raytrace.py:42-113), with the colour-stage weights returned.
+struct TexelEval {
+  float ts0, ts1, height, ws0, ws, dpn, cn;
+  f3 col;
+  bool heights;
+};
+
+__device__ __forceinline__ TexelEval texel_eval(const TraceParams& p, int site, f3 c, float r, f3 pn, f3 o, f3 d,
+                                                float tn, float* w1) {
+  TexelEval te;
+  const int T = p.n_texels;
+  const f3 co = c - o;
+  te.dpn = dot(pn, d);
+  te.cn = dot(co, pn);
+  te.ts0 = acc_div(te.cn, te.dpn);
+  const float kexp = -10.0f * fast_rcp(r * r);
+  float4 ph[PF_MAX_TEXELS];
+  bool nz = false;
+#pragma unroll
+  for (int i = 0; i < PF_MAX_TEXELS; i++) {
+    if (i < T) {
+      const float* ts = p.texel_sites + ((size_t)site * T + i) * 3;
+      const float h = p.texel_rgbh[(size_t)site * T + i].w;
+      ph[i] = make_float4(ts[0] - c.x, ts[1] - c.y, ts[2] - c.z, h);
+      nz = nz || h != 0.0f;
+    }
+  }
+  te.heights = nz;
+  te.height = 0.0f;
+  te.ws0 = 1.0f;
+  if (nz) {
+    const float tq0 = te.dpn >= 0.0f ? tn : sel_max(tn, te.ts0);
+    const f3 x0 = d * tq0 - co;
+    float hsum = 0.0f, s0 = 0.0f;
+#pragma unroll
+    for (int i = 0; i < PF_MAX_TEXELS; i++) {
+      if (i < T) {
+        const f3 df{x0.x - ph[i].x, x0.y - ph[i].y, x0.z - ph[i].z};
+        const float w = fast_exp(dot(df, df) * kexp);
+        hsum += w * ph[i].w;
+        s0 += w;
+      }
+    }
+    te.ws0 = sel_max(s0, 1e-20f);
+    te.height = hsum * fast_rcp(te.ws0);
+    te.ts1 = acc_div(te.cn + te.height, te.dpn);
+  } else {
+    te.ts1 = te.ts0;
+  }
+  const float tq = te.dpn >= 0.0f ? tn : sel_max(tn, te.ts1);
+  const f3 x1 = d * tq - co;
+  f3 rs{0.0f, 0.0f, 0.0f};
+  float s1 = 0.0f;
+#pragma unroll
+  for (int i = 0; i < PF_MAX_TEXELS; i++) {
+    if (i < T) {
+      const float4 cl = p.texel_rgbh[(size_t)site * T + i];
+      const f3 df{x1.x - ph[i].x, x1.y - ph[i].y, x1.z - ph[i].z};
+      const float w = fast_exp(dot(df, df) * kexp);
+      if (w1) w1[i] = w;
+      rs = rs + f3{cl.x, cl.y, cl.z} * w;
+      s1 += w;
+    }
+  }
+  te.ws = sel_max(s1, 1e-20f);
+  if (!nz) te.ws0 = te.ws;
+  te.col = rs * fast_rcp(te.ws);
+  return te;
+}
+
+__device__ __forceinline__ bool trace_lane(const TraceParams& p, int tile, int lane, int& i, int& j) {
+  const int ti = tile % p.tiles_h, tj = tile / p.tiles_h;
+  i = ti * 8 + (lane & 7);
+  j = tj * 8 + (lane >> 3);
+  return i < p.H && j < p.W;
+}
+
+__global__ void __launch_bounds__(TWAVE* TWPB) trace_fwd_kernel(TraceParams p) {
+  const int tile = blockIdx.x * TWPB + (threadIdx.x >> 6);
+  if (tile >= p.n_tiles) return;
+  int i, j;
+  if (!trace_lane(p, tile, threadIdx.x & 63, i, j)) return;
+  const size_t px = (size_t)i * p.W + j;
+  f3 o, d;
+  walk_ray(p, i, j, o, d);
+  int prim = p.start_sites ? p.start_sites[px] : p.start_site;
+  float log_t = 0.0f, depth = -1.0f, qsig = 0.0f;
+  int qstep = -1, step = 0;
+  f3 rgb{0.0f, 0.0f, 0.0f}, nacc{0.0f, 0.0f, 0.0f};
+  while (true) {
+    const float trans = fast_exp(log_t);
+    if (trans < p.thr || prim == SENTINEL || prim < 0 || step >= p.max_steps) break;
+    if (p.cells && step < p.K) p.cells[px * p.K + step] = prim;
+    const float4 sph = p.spheres[prim];
+    const f3 c{sph.x, sph.y, sph.z};
+    const float r = sph.w;
+    float tn, tf;
+    const f3 oc = o - c;
+    bool hit = sphere_hit(oc, d, r, tn, tf);
+    if (__builtin_sqrtf(dot(oc, oc)) < 4.0f * r) hit = false;
+    int next = SENTINEL;
+    float pt_far = 1e10f;
+    const int a0 = p.adj_off[prim], a1 = p.adj_off[prim + 1];
+    for (int e = a0; e < a1; e++) {
+      f3 n;
+      float pm, dp;
+      walk_face(p, e, c, r, n, pm);
+      const float t = walk_cross(o, d, n, pm, dp);
+      if (dp >= 0.0f && t < pt_far) {
+        next = p.adjacency[e];
+        pt_far = t;
+      }
+      tf = (dp >= 0.0f && t < tf) ? t : tf;
+      tn = (dp < 0.0f && t > tn) ? t : tn;
+    }
+    const float4 ns = p.nsigmas[prim];
+    const float sigma = ns.w;
+    if (hit && !(tn > tf) && !(sigma < 1e-3f)) {
+      const f3 pn{ns.x, ns.y, ns.z};
+      const TexelEval te = texel_eval(p, prim, c, r, pn, o, d, tn, nullptr);
+      const float tf2 = (te.dpn >= 0.0f && te.ts1 < tf) ? te.ts1 : tf;
+      const float tn2 = (te.dpn < 0.0f && te.ts1 > tn) ? te.ts1 : tn;
+      const float dt = tf2 - tn2;
+      if (dt > 0.0f) {
+        const float dlt = -sigma * dt;
+        const float at = (1.0f - fast_exp(dlt)) * trans;
+        rgb = rgb + te.col * at;
+        nacc = nacc + pn * at;
+        log_t += dlt;
+        if (qstep < 0 && fast_exp(log_t) < p.depth_q) {
+          depth = tn2 + __logf(trans * fast_rcp(p.depth_q)) * fast_rcp(sigma);
+          qstep = step;
+          qsig = sigma;
+        }
+      }
+    }
+    prim = next;
+    step++;
+  }
+  p.color[3 * px + 0] = rgb.x;
+  p.color[3 * px + 1] = rgb.y;
+  p.color[3 * px + 2] = rgb.z;
+  if (p.log_t) p.log_t[px] = log_t;
+  if (p.normal) {
+    p.normal[3 * px + 0] = nacc.x;
+    p.normal[3 * px + 1] = nacc.y;
+    p.normal[3 * px + 2] = nacc.z;
+  }
+  if (p.depth) p.depth[px] = depth;
+  if (p.q_step) p.q_step[px] = qstep;
+  if (p.q_sigma) p.q_sigma[px] = qsig;
+  if (p.n_cells) p.n_cells[px] = step;
+}
+
+__global__ void __launch_bounds__(TWAVE* TWPB) trace_bwd_kernel(TraceParams p) {
+  const int tile = blockIdx.x * TWPB + (threadIdx.x >> 6);
+  if (tile >= p.n_tiles) return;
+  int i, j;
+  if (!trace_lane(p, tile, threadIdx.x & 63, i, j)) return;
+  const size_t px = (size_t)i * p.W + j;
+  f3 o, d;
+  walk_ray(p, i, j, o, d);
+  const int T = p.n_texels;
+  const f3 gC = ld3(p.g_color + 3 * px), gN = ld3(p.g_normal + 3 * px);
+  const float gD = p.g_depth ? p.g_depth[px] : 0.0f;
+  const float gO = p.g_opacity[px];
+  const f3 Cf = ld3(p.s_color + 3 * px), Nf = ld3(p.s_normal + 3 * px);
+  const float Tend = fast_exp(p.s_log_t[px]);
+  const int qstep = p.s_q_step[px];
+  const float qsig = p.s_q_sigma[px];
+  int prim = p.start_sites ? p.start_sites[px] : p.start_site;
+  float log_t = 0.0f;
+  int step = 0;
+  f3 Cp{0.0f, 0.0f, 0.0f}, Np{0.0f, 0.0f, 0.0f};
+  while (true) {
+    const float trans = fast_exp(log_t);
+    if (trans < p.thr || prim == SENTINEL || prim < 0 || step >= p.max_steps) break;
+    const float4 sph = p.spheres[prim];
+    const f3 c{sph.x, sph.y, sph.z};
+    const float r = sph.w;
+    float tn, tf;
+    const f3 oc = o - c;
+    bool hit = sphere_hit(oc, d, r, tn, tf);
+    if (__builtin_sqrtf(dot(oc, oc)) < 4.0f * r) hit = false;
+    int tn_id = tn < 1e-8f ? -2 : -1, tf_id = tf < 1e-8f ? -2 : -1;
+    int next = SENTINEL;
+    float pt_far = 1e10f;
+    const int a0 = p.adj_off[prim], a1 = p.adj_off[prim + 1];
+    const int n_adj = a1 - a0;
+    for (int e = a0; e < a1; e++) {
+      f3 n;
+      float pm, dp;
+      walk_face(p, e, c, r, n, pm);
+      const float t = walk_cross(o, d, n, pm, dp);
+      if (dp >= 0.0f && t < pt_far) {
+        next = p.adjacency[e];
+        pt_far = t;
+      }
+      if (dp >= 0.0f && t < tf) {
+        tf_id = e - a0;
+        tf = t;
+      }
+      if (dp < 0.0f && t > tn) {
+        tn_id = e - a0;
+        tn = t;
+      }
+    }
+    const float4 ns = p.nsigmas[prim];
+    const float sigma = ns.w;
+    if (hit && !(tn > tf) && !(sigma < 1e-3f)) {
+      const f3 pn{ns.x, ns.y, ns.z};
+      const f3 co = -oc;
+      float w1[PF_MAX_TEXELS];
+      const float tn0 = tn;
+      const int tn0_id = tn_id, tf0_id = tf_id;
+      const TexelEval te = texel_eval(p, prim, c, r, pn, o, d, tn, w1);
+      if (te.dpn >= 0.0f && te.ts1 < tf) {
+        tf_id = n_adj;
+        tf = te.ts1;
+      }
+      if (te.dpn < 0.0f && te.ts1 > tn) {
+        tn_id = n_adj;
+        tn = te.ts1;
+      }
+      const float dt = tf - tn;
+      if (dt > 0.0f) {
+        const float u = sigma * dt;
+        const float e_u = fast_exp(-u);
+        const float alpha = 1.0f - e_u;
+        const float Tn = trans * e_u;
+        Cp = Cp + te.col * (alpha * trans);
+        Np = Np + pn * (alpha * trans);
+        const f3 SC = Cf - Cp, SN = Nf - Np;
+        float dLdu = dot(gC, te.col * Tn - SC) + dot(gN, pn * Tn - SN) + gO * Tend;
+        float dLdsig = 0.0f, dLdtn = 0.0f, dLdtf = 0.0f;
+        if (qstep >= 0 && step < qstep) dLdu += -gD / qsig;
+        if (step == qstep) {
+          dLdtn += gD;
+          dLdsig += -gD * __logf(trans * fast_rcp(p.depth_q)) / (sigma * sigma);
+        }
+        dLdsig += dLdu * dt;
+        dLdtf += dLdu * sigma;
+        dLdtn -= dLdu * sigma;
+        const f3 dLdcol = gC * (alpha * trans);
+        f3 dLdnorm = gN * (alpha * trans);
+        const float dLdts_in = (tn_id == n_adj) ? dLdtn : ((tf_id == n_adj) ? dLdtf : 0.0f);
+        // plane_intersection_bwd (texture.py:74-186), exact form
+        const float kw = 20.0f * fast_rcp(r * r);
+        const float kexp = -10.0f * fast_rcp(r * r);
+        const float inv_ws = fast_rcp(te.ws), inv_ws0 = fast_rcp(te.ws0);
+        float* gts = p.g_tsites + (size_t)prim * T * 3;
+        float* gtr = p.g_trgb + (size_t)prim * T * 3;
+        float* gth = p.g_theight + (size_t)prim * T;
+        float dLdheight = 0.0f;
+        f3 pdc{0.0f, 0.0f, 0.0f}, pdn{0.0f, 0.0f, 0.0f};
+        const float tq = te.dpn >= 0.0f ? tn0 : sel_max(tn0, te.ts1);
+        const f3 x1 = d * tq - co;
+        f3 dLdx{0.0f, 0.0f, 0.0f};
+        const float gdp_in = dot(dLdcol, te.col);
+        for (int k = 0; k < T; k++) {
+          const float* ts = p.texel_sites + ((size_t)prim * T + k) * 3;
+          const float4 cl = p.texel_rgbh[(size_t)prim * T + k];
+          const f3 df{x1.x - (ts[0] - c.x), x1.y - (ts[1] - c.y), x1.z - (ts[2] - c.z)};
+          const float wn = w1[k] * inv_ws;
+          atomicAdd(gtr + 3 * k + 0, dLdcol.x * wn);
+          atomicAdd(gtr + 3 * k + 1, dLdcol.y * wn);
+          atomicAdd(gtr + 3 * k + 2, dLdcol.z * wn);
+          const float gdp = dLdcol.x * cl.x + dLdcol.y * cl.y + dLdcol.z * cl.z;
+          const f3 sg = df * (kw * wn * (gdp - gdp_in));
+          atomicAdd(gts + 3 * k + 0, sg.x);
+          atomicAdd(gts + 3 * k + 1, sg.y);
+          atomicAdd(gts + 3 * k + 2, sg.z);
+          dLdx = dLdx - sg;
+        }
+        const float dLdtq = dot(dLdx, d);
+        float dtn_q = (te.dpn >= 0.0f || tn0 >= te.ts1) ? dLdtq : 0.0f;
+        float dts = (te.dpn < 0.0f && tn0 < te.ts1) ? dLdtq : 0.0f;
+        dts += dLdts_in;
+        if (dts != 0.0f) {
+          const float iden = 1.0f / te.dpn;
+          pdc = pdc + pn * (dts * iden);
+          pdn = pdn + (co * iden - d * ((te.cn + te.height) / (te.dpn * te.dpn))) * dts;
+          dLdheight += dts * iden;
+        }
+        const float tq0 = te.dpn >= 0.0f ? tn0 : sel_max(tn0, te.ts0);
+        const f3 x0 = d * tq0 - co;
+        f3 dLdx0{0.0f, 0.0f, 0.0f};
+        const float gdp0_in = dLdheight * te.height;
+        for (int k = 0; k < T; k++) {
+          const float* ts = p.texel_sites + ((size_t)prim * T + k) * 3;
+          const float hk = p.texel_rgbh[(size_t)prim * T + k].w;
+          const f3 df{x0.x - (ts[0] - c.x), x0.y - (ts[1] - c.y), x0.z - (ts[2] - c.z)};
+          const float w = te.heights ? fast_exp(dot(df, df) * kexp) : w1[k];
+          const float wn0 = w * inv_ws0;
+          atomicAdd(gth + k, wn0 * dLdheight);
+          const f3 sg = df * (kw * wn0 * (dLdheight * hk - gdp0_in));
+          atomicAdd(gts + 3 * k + 0, sg.x);
+          atomicAdd(gts + 3 * k + 1, sg.y);
+          atomicAdd(gts + 3 * k + 2, sg.z);
+          dLdx0 = dLdx0 - sg;
+        }
+        const float dLdtq0 = dot(dLdx0, d);
+        dtn_q += (te.dpn >= 0.0f || tn0 >= te.ts0) ? dLdtq0 : 0.0f;
+        const float dts0 = (te.dpn < 0.0f && tn0 < te.ts0) ? dLdtq0 : 0.0f;
+        if (dts0 != 0.0f) {
+          const float iden = 1.0f / te.dpn;
+          pdc = pdc + pn * (dts0 * iden);
+          pdn = pdn + (co * iden - d * (te.cn / (te.dpn * te.dpn))) * dts0;
+        }
+        f3 dLdc = pdc;
+        dLdnorm = dLdnorm + pdn;
+        float dLdr = 0.0f;
+        if (tn_id != n_adj) dtn_q += dLdtn;
+        for (int side = 0; side < 2; side++) {
+          const int tid = side == 0 ? tn0_id : tf0_id;
+          const float g = side == 0 ? dtn_q : dLdtf;
+          if (side == 1 && tf_id == n_adj) continue;
+          if (tid == -1) {
+            const f3 cr = cross(oc, d);
+            const float sq = 2.0f * __builtin_sqrtf(fmaxf(r * r - dot(cr, cr), 0.0f));
+            const float qb = 2.0f * dot(oc, d);
+            const float tnear = (-qb - sq) * 0.5f;
+            const f3 w = (oc * 2.0f - d * qb) * (1.0f / sq);
+            if (side == 0) {
+              if (!(tnear < 0.0f)) {
+                dLdc = dLdc + (d - w) * g;
+                dLdr += g * (-2.0f * r / sq);
+              }
+            } else {
+              dLdc = dLdc + (d + w) * g;
+              dLdr += g * (2.0f * r / sq);
+            }
+          } else if (tid >= 0 && tid < n_adj) {
+            const int adj = p.adjacency[a0 + tid];
+            const float4 as = p.spheres[adj];
+            const f3 a{as.x, as.y, as.z};
+            const f3 n = a - c;
+            const float num = dot(co, n) + 0.5f * (dot(n, n) + r * r - as.w * as.w);
+            const float idp = 1.0f / dot(d, n);
+            const f3 qv = d * (num * idp * idp);
+            dLdc = dLdc + (oc * idp + qv) * g;
+            dLdr += g * (r * idp);
+            const f3 ga = ((a - o) * idp - qv) * g;
+            float* gadj = p.g_spheres + 4 * (size_t)adj;
+            atomicAdd(gadj + 0, ga.x);
+            atomicAdd(gadj + 1, ga.y);
+            atomicAdd(gadj + 2, ga.z);
+            atomicAdd(gadj + 3, g * (-as.w * idp));
+          }
+        }
+        float* gs = p.g_spheres + 4 * (size_t)prim;
+        atomicAdd(gs + 0, dLdc.x);
+        atomicAdd(gs + 1, dLdc.y);
+        atomicAdd(gs + 2, dLdc.z);
+        atomicAdd(gs + 3, dLdr);
+        float* gq = p.g_nsigmas + 4 * (size_t)prim;
+        atomicAdd(gq + 0, dLdnorm.x);
+        atomicAdd(gq + 1, dLdnorm.y);
+        atomicAdd(gq + 2, dLdnorm.z);
+        atomicAdd(gq + 3, dLdsig);
+        log_t -= u;
+      }
+    }
+    prim = next;
+    step++;
+  }
+}
+
+static TraceParams make_params(const pf_trace_args* a) {
+  TraceParams p{};
+  const pf_camera& c = a->cam;
+  p.W = c.width;
+  p.H = c.height;
+  p.tiles_h = tiles_h(c);
+  p.n_tiles = tiles_h(c) * tiles_w(c);
+  p.eye = f3{c.eye[0], c.eye[1], c.eye[2]};
+  p.right = f3{c.right[0], c.right[1], c.right[2]};
+  p.up = f3{c.up[0], c.up[1], c.up[2]};
+  p.ray_maps = c.ray_maps;
+  p.n_texels = a->scene.n_texels;
+  p.spheres = reinterpret_cast<const float4*>(a->scene.spheres);
+  p.nsigmas = reinterpret_cast<const float4*>(a->scene.nsigmas);
+  p.texel_sites = a->scene.texel_sites;
+  p.texel_rgbh = reinterpret_cast<const float4*>(a->scene.texel_rgbh);
+  p.adjacency = a->scene.adjacency;
+  p.adj_off = a->scene.adjacency_offsets;
+  p.adj_diff = a->scene.adjacency_diff;
+  p.start_site = a->start_site;
+  p.start_sites = a->start_sites;
+  p.thr = a->transmittance_threshold;
+  p.depth_q = a->depth_quantile;
+  p.max_steps = a->max_steps > 0 ? a->max_steps : (1 << 16);
+  return p;
+}
+
+static int check_trace(const pf_trace_args* a, const char* who) {
+  if (!a) return set_error(PF_ERR_ARG, "%s: null args", who);
+  const pf_scene& s = a->scene;
+  if (s.n_sites < 1 || !s.spheres || !s.nsigmas || !s.texel_sites || !s.texel_rgbh || !s.adjacency_offsets)
+    return set_error(PF_ERR_ARG, "%s: missing scene arrays", who);
+  if (s.n_texels < 1 || s.n_texels > PF_MAX_TEXELS) return set_error(PF_ERR_ARG, "%s: n_texels out of range", who);
+  if (a->cam.width < 2 || a->cam.height < 2) return set_error(PF_ERR_ARG, "%s: image must be >= 2x2", who);
+  if (!a->start_sites && (a->start_site < 0 || a->start_site >= s.n_sites))
+    return set_error(PF_ERR_ARG, "%s: start_site %d outside 0..%d", who, a->start_site, s.n_sites - 1);
+  if (a->cells && a->max_cells < 1) return set_error(PF_ERR_ARG, "%s: cells needs max_cells >= 1", who);
+  return PF_OK;
+}
+
+}  // namespace pf
+
+using namespace pf;
+
+extern "C" {
+
+int pf_trace_fwd(const pf_trace_args* a, void* stream) {
+  int st = check_trace(a, "pf_trace_fwd");
+  if (st) return st;
+  if (!a->color) return set_error(PF_ERR_ARG, "pf_trace_fwd: missing colour output");
+  TraceParams p = make_params(a);
+  p.color = a->color;
+  p.log_t = a->log_t;
+  p.normal = a->normal;
+  p.depth = a->depth;
+  p.q_step = a->q_step;
+  p.q_sigma = a->q_sigma;
+  p.cells = a->cells;
+  p.K = a->max_cells;
+  p.n_cells = a->n_cells;
+  trace_fwd_kernel<<<(p.n_tiles + TWPB - 1) / TWPB, TWAVE * TWPB, 0, as_stream(stream)>>>(p);
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+int pf_trace_bwd(const pf_trace_bwd_args* b, void* stream) {
+  if (!b) return set_error(PF_ERR_ARG, "pf_trace_bwd: null args");
+  int st = check_trace(&b->fwd, "pf_trace_bwd");
+  if (st) return st;
+  if (!b->color || !b->normal || !b->log_t || !b->q_step || !b->q_sigma || !b->grad_color || !b->grad_normal ||
+      !b->grad_opacity)
+    return set_error(PF_ERR_ARG, "pf_trace_bwd: missing saved outputs or incoming gradients");
+  if (!b->grad_spheres || !b->grad_nsigmas || !b->grad_texel_sites || !b->grad_texel_rgb || !b->grad_texel_height)
+    return set_error(PF_ERR_ARG, "pf_trace_bwd: missing gradient outputs");
+  TraceParams p = make_params(&b->fwd);
+  p.s_color = b->color;
+  p.s_normal = b->normal;
+  p.s_log_t = b->log_t;
+  p.s_q_step = b->q_step;
+  p.s_q_sigma = b->q_sigma;
+  p.g_color = b->grad_color;
+  p.g_normal = b->grad_normal;
+  p.g_depth = b->grad_depth;
+  p.g_opacity = b->grad_opacity;
+  p.g_spheres = b->grad_spheres;
+  p.g_nsigmas = b->grad_nsigmas;
+  p.g_tsites = b->grad_texel_sites;
+  p.g_trgb = b->grad_texel_rgb;
+  p.g_theight = b->grad_texel_height;
+  trace_bwd_kernel<<<(p.n_tiles + TWPB - 1) / TWPB, TWAVE * TWPB, 0, as_stream(stream)>>>(p);
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+}  // extern "C"

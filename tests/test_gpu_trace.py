@@ -1,0 +1,162 @@
+"""GPU ray tracer (pf_trace_fwd / pf_trace_bwd) against the oracle:
+visited cell sequences bit-exact against the FP32 oracle walk (pinhole and an
+equidistant fisheye given as ray maps), colour / opacity / depth / normals
+within 1e-4 of the FP64 oracle, gradients within 1e-3 of the FP64 adjoint and
+against FP64 central differences."""
+
+from types import SimpleNamespace
+
+import numpy as np
+import pytest
+
+from oracle import orc
+from powerfoam_amd import synth
+from powerfoam_amd.camera import TorchCamera, fisheye_ray_maps, look_at
+from tests import harness
+from tests.test_oracle_raster import PARAMS, make_scene
+from tests.test_oracle_trace import _trace_objective, start_cell, trace_scene
+
+pytestmark = pytest.mark.gpu
+
+ABS_TOL = 1e-4
+GRAD_REL_TOL = 1e-3
+
+
+def big_scene(seed, n=3000, T=8):
+    s = synth.scene_uniform(seed, n=n, n_texels=T, r_range=(0.03, 0.12), height_std=0.05)
+    rng = np.random.default_rng(seed)
+    s["texel_rgb"] = rng.uniform(0, 1, (n, T, 3)).astype(np.float32)
+    s["adj"], s["off"] = synth.power_adjacency(s["points"], s["radii"], alpha_complex=False)
+    return s
+
+
+def cameras(kind):
+    eye = (0.1, 0.2, 2.2)
+    if kind == "pinhole":
+        return look_at(eye, (0.0, 0.0, 0.0), 72, 56, 60.0)
+    # fisheye from inside the scene: rays leave the start cell in every direction
+    eye = (0.1, 0.2, 0.4)
+    rm = fisheye_ray_maps(eye, (0.0, 0.0, -1.0), 64)
+    import torch
+    t = look_at(eye, (0.0, 0.0, -1.0), 64, 64, 60.0)
+    return TorchCamera(t.eye, t.right, t.up, 64, 64, torch.tensor(rm))
+
+
+def _tensors(s, dev):
+    import torch
+    tt = lambda a, dt=torch.float32: torch.tensor(np.ascontiguousarray(a), dtype=dt, device=dev)
+    return dict(points=tt(s["points"]), radii=tt(s["radii"]), density=tt(s["density"]), normals=tt(s["normals"]),
+                texel_sites=tt(s["texel_sites"]), texel_rgb=tt(s["texel_rgb"]),
+                texel_height=tt(s["texel_height"]), adj=tt(s["adj"], torch.int32), off=tt(s["off"], torch.int32))
+
+
+def _tracer(T):
+    from powerfoam_amd.raytrace import RayTracer
+    return RayTracer(SimpleNamespace(num_texel_sites=T, is_pinhole=True), "cuda")
+
+
+def _gpu_cam(cam):
+    return TorchCamera(cam.eye, cam.right, cam.up, cam.width, cam.height,
+                       None if cam.ray_maps is None else cam.ray_maps.cuda().contiguous())
+
+
+def _oracle_cam(cam):
+    rm = None if cam.ray_maps is None else cam.ray_maps.numpy()
+    return orc.camera(cam.eye.numpy(), cam.right.numpy(), cam.up.numpy(), cam.width, cam.height, rm)
+
+
+@pytest.mark.parametrize("kind", ["pinhole", "fisheye"])
+@pytest.mark.parametrize("encoded", [True, False], ids=["fp16faces", "exact"])
+def test_cell_sequences_bitexact(kind, encoded):
+    import torch
+    from powerfoam_amd.rasterize import power_face_encoding
+    s = big_scene(1)
+    cam = cameras(kind)
+    x = _tensors(s, "cuda")
+    diff = power_face_encoding(x["points"], x["radii"], x["adj"], x["off"]) if encoded else None
+    st = start_cell(s, cam.eye.numpy().astype(np.float64))
+    print(f"cells {kind} encoded={encoded} start={st}", flush=True)
+    cells, n = _tracer(8).trace_cells(_gpu_cam(cam), st, x["points"], x["radii"], x["density"], x["normals"],
+                                      x["texel_sites"], x["texel_rgb"], x["texel_height"], x["adj"], x["off"], diff,
+                                      0.0, max_cells=128)
+    ocam, _k = _oracle_cam(cam)
+    sp = np.concatenate([s["points"], s["radii"][:, None]], 1)
+    ns = np.concatenate([s["normals"], s["density"][:, None]], 1)
+    rgbh = np.concatenate([s["texel_rgb"], s["texel_height"][..., None]], -1)
+    inp = orc.Inputs(sp, ns, s["texel_sites"], rgbh, s["adj"], s["off"],
+                     None if diff is None else diff.cpu().numpy().view(np.uint16))
+    ref = orc.trace_fwd(ocam, inp, st, threshold=0.0, record_cells=128, precision="f32")
+    assert np.array_equal(n.cpu().numpy(), ref["n_cells"])
+    assert np.array_equal(cells.cpu().numpy(), ref["cells"])
+    assert ref["n_cells"].mean() > 5
+
+
+@pytest.mark.parametrize("kind", ["pinhole", "fisheye"])
+def test_trace_forward_matches_fp64_oracle(kind):
+    from powerfoam_amd.rasterize import power_face_encoding
+    s = big_scene(2)
+    cam = cameras(kind)
+    x = _tensors(s, "cuda")
+    diff = power_face_encoding(x["points"], x["radii"], x["adj"], x["off"])
+    st = start_cell(s, cam.eye.numpy().astype(np.float64))
+    tr = _tracer(8)
+    col, opac, depth, normal = tr.forward(_gpu_cam(cam), st, x["points"], x["radii"], x["density"], x["normals"],
+                                          x["texel_sites"], x["texel_rgb"], x["texel_height"], x["adj"], x["off"],
+                                          diff, 1e-3, 0.5)
+    bench = tr.benchmark(_gpu_cam(cam), st, x["points"], x["radii"], x["density"], x["normals"], x["texel_sites"],
+                         x["texel_rgb"], x["texel_height"], x["adj"], x["off"], diff, 1e-3)
+    ocam, _k = _oracle_cam(cam)
+    sp = np.concatenate([s["points"], s["radii"][:, None]], 1)
+    ns = np.concatenate([s["normals"], s["density"][:, None]], 1)
+    rgbh = np.concatenate([s["texel_rgb"], s["texel_height"][..., None]], -1)
+    inp = orc.Inputs(sp, ns, s["texel_sites"], rgbh, s["adj"], s["off"], diff.cpu().numpy().view(np.uint16))
+    ref = orc.trace_fwd(ocam, inp, st, threshold=1e-3, depth_q=0.5)
+    assert np.array_equal(bench.cpu().numpy(), col.detach().cpu().numpy())
+    assert np.abs(col.detach().cpu().numpy() - ref["color"]).max() < ABS_TOL
+    assert np.abs(opac.detach().cpu().numpy() - (1 - np.exp(ref["log_t"]))).max() < ABS_TOL
+    assert np.abs(normal.detach().cpu().numpy() - ref["normal"]).max() < ABS_TOL
+    d = depth.detach().cpu().numpy()
+    assert np.array_equal(d == -1, ref["depth"] == -1)
+    assert np.abs(d - ref["depth"]).max() < ABS_TOL
+    assert (ref["depth"] != -1).mean() > 0.1
+
+
+@pytest.mark.parametrize("kind", ["pinhole", "fisheye"])
+def test_trace_backward_matches_fp64_adjoint(kind):
+    import torch
+    s = big_scene(3)
+    cam = cameras(kind)
+    x = _tensors(s, "cuda")
+    leaves = [x[k].requires_grad_(True) for k in ("points", "radii", "density", "normals", "texel_sites",
+                                                 "texel_rgb", "texel_height")]
+    st = start_cell(s, cam.eye.numpy().astype(np.float64))
+    col, opac, depth, normal = _tracer(8).forward(_gpu_cam(cam), st, *leaves, x["adj"], x["off"], None, 1e-3, 0.5)
+    H, W = cam.height, cam.width
+    ocam, _k = _oracle_cam(cam)
+    sp = np.concatenate([s["points"], s["radii"][:, None]], 1)
+    ns = np.concatenate([s["normals"], s["density"][:, None]], 1)
+    rgbh = np.concatenate([s["texel_rgb"], s["texel_height"][..., None]], -1)
+    inp = orc.Inputs(sp, ns, s["texel_sites"], rgbh, s["adj"], s["off"], None)
+    ref = orc.trace_fwd(ocam, inp, st, threshold=1e-3, depth_q=0.5)
+    # rays through ill-conditioned cells (grazing balls, tied bounds) carry no
+    # loss weight on either side
+    keep = ~ref["fragile"]
+    assert keep.mean() > 0.5
+    rng = np.random.default_rng(7)
+    w = dict(color=rng.normal(size=(H, W, 3)) * keep[..., None], normal=rng.normal(size=(H, W, 3)) * keep[..., None],
+             opacity=rng.normal(size=(H, W)) * keep, depth=rng.normal(size=(H, W)) * keep)
+    valid = (depth.detach() != -1)
+    tw = {k: torch.tensor(v, dtype=torch.float32, device="cuda") for k, v in w.items()}
+    L = (col * tw["color"]).sum() + (normal * tw["normal"]).sum() + (opac * tw["opacity"]).sum() + \
+        (torch.where(valid, depth, torch.zeros_like(depth)) * tw["depth"]).sum()
+    L.backward()
+    wd = np.where(ref["depth"] != -1, w["depth"], 0.0)
+    g = orc.trace_bwd(ocam, inp, st, ref, w["color"], w["normal"], wd, w["opacity"], threshold=1e-3, depth_q=0.5)
+    got = dict(grad_spheres=np.concatenate([leaves[0].grad.cpu().numpy(), leaves[1].grad.cpu().numpy()[:, None]], 1),
+               grad_nsigmas=np.concatenate([leaves[3].grad.cpu().numpy(), leaves[2].grad.cpu().numpy()[:, None]], 1),
+               grad_texel_sites=leaves[4].grad.cpu().numpy(), grad_texel_rgb=leaves[5].grad.cpu().numpy(),
+               grad_texel_height=leaves[6].grad.cpu().numpy())
+    for k in harness.GRAD_KEYS:
+        err = harness.rel_err(got[k], g[k])
+        print(k, err, flush=True)
+        assert err < GRAD_REL_TOL, (k, err)
