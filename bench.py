@@ -11,9 +11,10 @@ adjoint). All scene tensors are resident in HBM before the timed region.
 
 Prints one JSON line (rank 0). `value` is rays per second (Mrays/s) over the
 timed frames; `ms_per_step` is ms per frame. The roofline entry is for the
-kernel that takes the most time, with algorithmic bytes per launch from the
-formulas in DESIGN.md (measured workload counts of the frame) divided by that
-kernel's average duration measured with HIP events on its launch stream.
+kernel that takes the most time, with algorithmic bytes per launch from
+`algorithmic_bytes` (per-pair, per-pixel and per-site byte counts applied to
+the frame's measured workload counts) divided by that kernel's average
+duration measured with HIP events on its launch stream.
 The CPU baseline is the FP32 CPU restatement of the same algorithm (the
 oracle, OpenMP) on one frame of the same workload.
 """
@@ -96,7 +97,13 @@ def workload_counts(rast, w, width, height):
 
 
 def algorithmic_bytes(c):
-    """Algorithmic HBM bytes per launch (formulas in DESIGN.md)."""
+    """Algorithmic HBM bytes per launch. Forward: per listed (tile, site) pair
+    the list entry (4 B), the site record (sphere + normal/sigma, 32 B) and its
+    8 texels (8 x 28 B), per face 8 B of fp16 encoding + 4 B index, per pixel
+    32 B of outputs, per distinct site 8 B of contrib/point_err. Backward: the
+    forward's reads plus 36 B of saved/incoming per-pixel data and, per pair,
+    a read-modify-write of the site and texel gradients, and of the sphere
+    gradients of touched neighbours."""
     read = c["pairs"] * (4 + 32 + 224 + 8) + 12 * c["sum_nadj"]
     fwd = read + c["P"] * 32 + 8 * c["sites_touched"]
     bwd = read + c["P"] * (12 + 4 + 4 + 12 + 4) + c["pairs"] * 2 * (32 + 224) + 2 * 16 * c["neighbours_touched"]
@@ -149,7 +156,8 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: the renderer hot path is built and timed on a single MI355X; "
-              "multi-GPU runs are not part of this build (see DESIGN.md, sharding).", file=sys.stderr)
+              "multi-GPU runs are not measured here; the per-view sharding is data parallel over cameras.",
+              file=sys.stderr)
         sys.exit(2)
     if "OMP_NUM_THREADS" not in os.environ:
         os.environ["OMP_NUM_THREADS"] = str(min(16, os.cpu_count() or 1))

@@ -208,6 +208,33 @@ int pf_bin_emit(const pf_camera* cam, const float* spheres, int32_t n_sites,
                 const int64_t* offsets, int32_t n_tiles, uint64_t* records,
                 int32_t* cursor, uint8_t* visible, void* stream);
 
+/* Ray-map cameras (cam->ray_maps set; rasterize.py:1681-1760 when
+ * args.is_pinhole is false) bin through a quadtree of ray cones instead of
+ * the projected ellipse.
+ *
+ * Number of cone nodes (float4 each) for a width x height image, or -1 when
+ * the size is outside 5..8192 pixels a side. Replaces the node count of
+ * _precompute_cones (rasterize.py:1518-1550). Host-only, no GPU work. */
+int32_t pf_cone_tree_nodes(int32_t width, int32_t height);
+
+/* Builds the cone tree of cam->ray_maps into cones [nodes, 4] (axis, cos of
+ * the half-angle; cos > 1 = no valid ray). Replaces
+ * compute_leaf_cones_kernel + merge_cones_kernel launches
+ * (rasterize.py:468-569, 1552-1583). */
+int pf_bin_cones(const pf_camera* cam, float* cones, void* stream);
+
+/* pf_bin_count for ray-map cameras: count_visible_generic_kernel
+ * (rasterize.py:202-291). */
+int pf_bin_count_cones(const pf_camera* cam, const float* cones, const float* spheres, int32_t n_sites,
+                       int32_t* tile_counts, void* stream);
+
+/* pf_bin_emit for ray-map cameras: write_visible_generic_kernel
+ * (rasterize.py:365-466). Records and their order after pf_bin_sort are as
+ * for pf_bin_emit. */
+int pf_bin_emit_cones(const pf_camera* cam, const float* cones, const float* spheres, int32_t n_sites,
+                      const int64_t* offsets, int32_t n_tiles, uint64_t* records, int32_t* cursor,
+                      uint8_t* visible, void* stream);
+
 /* Bytes of scratch pf_bin_sort needs for these tile segments. */
 size_t pf_bin_sort_workspace_size(int64_t n_records, int32_t n_tiles);
 

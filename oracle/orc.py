@@ -37,6 +37,10 @@ def load():
         lib.orc_site_footprint_dbg.restype = ctypes.c_int
         lib.orc_tile_obb_test_dbg.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp]
         lib.orc_tile_obb_test_dbg.restype = ctypes.c_int
+        lib.orc_cone_tree_info.argtypes = [ctypes.c_int, ctypes.c_int, vp]
+        lib.orc_cones.argtypes = [vp, vp]
+        lib.orc_site_cone_tiles_dbg.argtypes = [vp, vp, vp]
+        lib.orc_site_cone_tiles_dbg.restype = ctypes.c_int
         lib.orc_adj_encode.argtypes = [vp, ctypes.c_int32, vp, vp, vp]
         lib.orc_adj_encode.restype = None
         lib.orc_f32_to_f16_array.argtypes = [vp, ctypes.c_int64, vp]
@@ -98,6 +102,29 @@ def bin_lists(cam, spheres):
     keys = np.zeros(max(m, 1), np.uint32)
     lib.orc_bin_lists(ctypes.byref(cam), _p(sp), sp.shape[0], _p(offsets), _p(sites), _p(keys))
     return offsets, sites[:m], keys[:m]
+
+
+def cone_tree_info(width, height):
+    """(nodes, levels, tile_level) of the cone tree of a width x height image."""
+    out = np.zeros(3, np.int32)
+    load().orc_cone_tree_info(int(width), int(height), _p(out))
+    return int(out[0]), int(out[1]), int(out[2])
+
+
+def cones(cam):
+    """Cone tree [nodes, 4] of a camera with ray maps (rasterize.py:468-569)."""
+    nodes, _, _ = cone_tree_info(cam.width, cam.height)
+    out = np.zeros((nodes, 4), np.float32)
+    load().orc_cones(ctypes.byref(cam), _p(out))
+    return out
+
+
+def site_cone_tiles(cam, sphere):
+    """Tiles the cone-tree descent bins one site to, or None if culled."""
+    th, tw = tiles_hw(cam.height, cam.width)
+    out = np.zeros(th * tw, np.int32)
+    n = load().orc_site_cone_tiles_dbg(ctypes.byref(cam), _p(_f32(sphere)), _p(out))
+    return None if n < 0 else out[:n]
 
 
 def site_footprint(cam, sphere):

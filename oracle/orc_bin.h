@@ -9,9 +9,9 @@
  * ellipse/rectangle overlap test (tests/test_oracle_binning.py).
  * Parity unpinned: the reference runs under Warp/NVRTC, whose FMA
  * contraction cannot be reproduced; bit-exactness is defined against this
- * mirror. Two Warp semantics are fixed here by choice and recorded in
- * wp.cast(float, uint32) is read as a bit reinterpretation, and
- * integer // truncates toward zero (C division).
+ * mirror. Two Warp semantics are fixed here by choice: wp.cast(float,
+ * uint32) is read as a bit reinterpretation, and integer // truncates toward
+ * zero (C division).
  */
 #ifndef ORC_BIN_H
 #define ORC_BIN_H
@@ -198,6 +198,180 @@ static inline float orc_f16_to_f32(uint16_t h) {
   }
   float f; memcpy(&f, &x, 4);
   return f;
+}
+
+/* ------------------------------------------------ non-pinhole cone tree
+ * _precompute_cones (rasterize.py:1518-1585): level 0 holds one cone per 2x2
+ * pixel block (LEAF_SIZE, rasterize.py:12), each further level one per 2x2
+ * block of the level below (dims rounded up) until 1x1; nodes are stored
+ * level after level, row-major inside a level. Level 2 has exactly the 8x8
+ * tiles (tile_level, :1546-1550). A cone is (axis, cos half-angle), cos > 1
+ * marking a node without valid rays. */
+#define ORC_LEAF_SIZE 2
+#define ORC_MAX_LEVELS 32
+
+typedef struct {
+  int n_levels, tile_level, total;
+  int off[ORC_MAX_LEVELS], dh[ORC_MAX_LEVELS], dw[ORC_MAX_LEVELS];
+} orc_cone_tree;
+
+static inline void orc_cone_tree_dims(int width, int height, orc_cone_tree* t) {
+  int dh = (height + ORC_LEAF_SIZE - 1) / ORC_LEAF_SIZE, dw = (width + ORC_LEAF_SIZE - 1) / ORC_LEAF_SIZE;
+  t->n_levels = 0;
+  t->total = 0;
+  while (t->n_levels < ORC_MAX_LEVELS) {
+    t->off[t->n_levels] = t->total;
+    t->dh[t->n_levels] = dh;
+    t->dw[t->n_levels] = dw;
+    t->total += dh * dw;
+    t->n_levels++;
+    if (dh == 1 && dw == 1) break;
+    dh = (dh + 1) / 2;
+    dw = (dw + 1) / 2;
+  }
+  t->tile_level = 0;
+  for (int sz = ORC_LEAF_SIZE; sz < PF_TILE_WIDTH; sz *= 2) t->tile_level++;
+}
+
+/* compute_leaf_cones_kernel (rasterize.py:468-512) and merge_cones_kernel
+ * (:514-569), level by level; cones is [total, 4]. */
+static inline void orc_build_cones(const pf_camera* cam, const orc_cone_tree* t, float* cones) {
+  const int W = cam->width, H = cam->height;
+  for (int a = 0; a < t->dh[0]; a++)
+    for (int b = 0; b < t->dw[0]; b++) {
+      float* out = cones + 4 * (size_t)(a * t->dw[0] + b);
+      v3f rays[4];
+      v3f sum = v3f_make(0.0f, 0.0f, 0.0f);
+      int nv = 0;
+      for (int di = 0; di < ORC_LEAF_SIZE; di++)
+        for (int dj = 0; dj < ORC_LEAF_SIZE; dj++) {
+          int pi = a * ORC_LEAF_SIZE + di, pj = b * ORC_LEAF_SIZE + dj;
+          if (pi > H - 1) pi = H - 1;
+          if (pj > W - 1) pj = W - 1;
+          const float* rm = cam->ray_maps + ((size_t)pi * W + pj) * 6;
+          v3f r = v3f_make(rm[3], rm[4], rm[5]);
+          float len = v3f_len(r);
+          if (len > 1.0e-6f) {
+            v3f rn = v3f_make(r.x / len, r.y / len, r.z / len);
+            rays[nv++] = rn;
+            sum = v3f_make(sum.x + rn.x, sum.y + rn.y, sum.z + rn.z);
+          }
+        }
+      if (nv == 0) {
+        out[0] = out[1] = out[2] = 0.0f;
+        out[3] = 2.0f;
+        continue;
+      }
+      v3f axis = v3f_normalize(sum);
+      float ch = 1.0f;
+      for (int k = 0; k < nv; k++) {
+        float d = v3f_dot(axis, rays[k]);
+        ch = d < ch ? d : ch;
+      }
+      out[0] = axis.x; out[1] = axis.y; out[2] = axis.z; out[3] = ch;
+    }
+  for (int l = 1; l < t->n_levels; l++) {
+    const int pdw = t->dw[l], cdh = t->dh[l - 1], cdw = t->dw[l - 1];
+    for (int a = 0; a < t->dh[l]; a++)
+      for (int b = 0; b < pdw; b++) {
+        float* out = cones + 4 * (size_t)(t->off[l] + a * pdw + b);
+        v3f ax[4];
+        float cc[4];
+        v3f sum = v3f_make(0.0f, 0.0f, 0.0f);
+        int nv = 0;
+        for (int di = 0; di < 2; di++)
+          for (int dj = 0; dj < 2; dj++) {
+            int ca = 2 * a + di, cb = 2 * b + dj;
+            if (ca < cdh && cb < cdw) {
+              const float* c = cones + 4 * (size_t)(t->off[l - 1] + ca * cdw + cb);
+              if (c[3] <= 1.0f) {
+                ax[nv] = v3f_make(c[0], c[1], c[2]);
+                cc[nv] = c[3];
+                sum = v3f_make(sum.x + c[0], sum.y + c[1], sum.z + c[2]);
+                nv++;
+              }
+            }
+          }
+        if (nv == 0) {
+          out[0] = out[1] = out[2] = 0.0f;
+          out[3] = 2.0f;
+          continue;
+        }
+        v3f axis = v3f_normalize(sum);
+        float ch = 1.0f;
+        for (int k = 0; k < nv; k++) {
+          float d = v3f_dot(axis, ax[k]);
+          float sa = 1.0f - d * d;
+          float sc = 1.0f - cc[k] * cc[k];
+          float sin_angle = sqrtf(sa > 0.0f ? sa : 0.0f);
+          float sin_child = sqrtf(sc > 0.0f ? sc : 0.0f);
+          float ext = d * cc[k] - sin_angle * sin_child;
+          ch = ext < ch ? ext : ch;
+        }
+        out[0] = axis.x; out[1] = axis.y; out[2] = axis.z; out[3] = ch;
+      }
+  }
+}
+
+/* camera.py:27-44 sphere_cone_intersect */
+static inline int orc_sphere_cone(v3f center, float radius, v3f eye, const float* cone) {
+  v3f v = v3f_sub(center, eye);
+  float dist = v3f_len(v);
+  float cos_alpha = v3f_dot(v, v3f_make(cone[0], cone[1], cone[2])) / dist;
+  float sin_half = sqrtf(1.0f - cone[3] * cone[3]);
+  float sin_beta = radius / dist;
+  float cos_beta = sqrtf(1.0f - sin_beta * sin_beta);
+  float threshold = cone[3] * cos_beta - sin_half * sin_beta;
+  return cos_alpha > threshold;
+}
+
+/* count_visible_generic_kernel / write_visible_generic_kernel
+ * (rasterize.py:202-291, 365-466): the cull of the pinhole path, then a
+ * depth-first descent (32-entry stack, children pushed in (0,0), (0,1),
+ * (1,0), (1,1) order) through the nodes whose cone meets the sphere; tiles
+ * reached at tile_level are written to tiles_out in visiting order. Returns
+ * the tile count, or -1 when the site is culled; *key gets the depth key. */
+static inline int orc_site_cone_tiles(const pf_camera* cam, const orc_cone_tree* t, const float* cones,
+                                      const float* sphere, int32_t* tiles_out, uint32_t* key) {
+  v3f center = v3f_make(sphere[0], sphere[1], sphere[2]);
+  float radius = sphere[3];
+  v3f eye = v3f_make(cam->eye[0], cam->eye[1], cam->eye[2]);
+  v3f up = v3f_make(cam->up[0], cam->up[1], cam->up[2]);
+  v3f right = v3f_make(cam->right[0], cam->right[1], cam->right[2]);
+  v3f v = v3f_sub(center, eye);
+  v3f fwd = v3f_cross(up, right);
+  if (v3f_len(v) < 4.0f * radius || v3f_dot(v, fwd) < 0.1f) return -1;
+  float pow_dist = v3f_dot(v, v) - radius * radius;
+  memcpy(key, &pow_dist, 4);
+  const int th = 1 + (cam->height - 1) / PF_TILE_WIDTH;
+  int sl[32], sa[32], sb[32];
+  int n = 0, sz = 1;
+  sl[0] = t->n_levels - 1;
+  sa[0] = 0;
+  sb[0] = 0;
+  while (sz > 0) {
+    sz--;
+    const int level = sl[sz], a = sa[sz], b = sb[sz];
+    const float* cone = cones + 4 * (size_t)(t->off[level] + a * t->dw[level] + b);
+    if (cone[3] > 1.0f) continue;
+    if (!orc_sphere_cone(center, radius, eye, cone)) continue;
+    if (level == t->tile_level) {
+      tiles_out[n++] = a + b * th;
+      continue;
+    }
+    const int cl = level - 1;
+    for (int di = 0; di < 2; di++)
+      for (int dj = 0; dj < 2; dj++) {
+        int ca = 2 * a + di, cb = 2 * b + dj;
+        if (ca < t->dh[cl] && cb < t->dw[cl]) {
+          sl[sz] = cl;
+          sa[sz] = ca;
+          sb[sz] = cb;
+          sz++;
+        }
+      }
+  }
+  return n;
 }
 
 #endif /* ORC_BIN_H */

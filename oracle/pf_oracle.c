@@ -4,7 +4,8 @@
  * bench's CPU baseline use as the checker. The product never links or loads
  * it. Built by oracle/Makefile into oracle/build/liborcl.so.
  *
- *   orc_bin_count / orc_bin_lists   rasterize.py:140-200, 293-363, 1713-1768
+ *   orc_bin_count / orc_bin_lists   rasterize.py:140-200, 293-363, 1713-1768;
+ *                                   non-pinhole: 202-291, 365-569, 1518-1585
  *   orc_adj_encode                  rasterize.py:580-606
  *   orc_raster_fwd_{f32,f64}        rasterize.py:892-1146
  *   orc_raster_bwd_{f32,f64}        rasterize.py:1150-1514
@@ -21,11 +22,68 @@
 static int orc_tiles_h(const pf_camera* cam) { return 1 + (cam->height - 1) / PF_TILE_WIDTH; }
 static int orc_tiles_w(const pf_camera* cam) { return 1 + (cam->width - 1) / PF_TILE_WIDTH; }
 
+/* Non-pinhole cameras (cam->ray_maps set) bin through the cone tree. */
+typedef struct {
+  orc_cone_tree t;
+  float* cones;
+  int32_t* tiles;
+} orc_cone_ctx;
+
+static void orc_cone_ctx_init(const pf_camera* cam, orc_cone_ctx* c) {
+  orc_cone_tree_dims(cam->width, cam->height, &c->t);
+  c->cones = (float*)malloc(sizeof(float) * 4 * (size_t)c->t.total);
+  c->tiles = (int32_t*)malloc(sizeof(int32_t) * (size_t)(orc_tiles_h(cam) * orc_tiles_w(cam)));
+  orc_build_cones(cam, &c->t, c->cones);
+}
+
+static void orc_cone_ctx_free(orc_cone_ctx* c) {
+  free(c->cones);
+  free(c->tiles);
+}
+
+/* Node count, level count and tile level of the cone tree of a W x H image. */
+void orc_cone_tree_info(int width, int height, int32_t* out3) {
+  orc_cone_tree t;
+  orc_cone_tree_dims(width, height, &t);
+  out3[0] = t.total;
+  out3[1] = t.n_levels;
+  out3[2] = t.tile_level;
+}
+
+/* The cone tree of cam (ray_maps required), [nodes, 4]. */
+void orc_cones(const pf_camera* cam, float* cones) {
+  orc_cone_tree t;
+  orc_cone_tree_dims(cam->width, cam->height, &t);
+  orc_build_cones(cam, &t, cones);
+}
+
+/* Tiles one site is binned to through the cone tree (visiting order); -1 if culled. */
+int orc_site_cone_tiles_dbg(const pf_camera* cam, const float* sphere, int32_t* tiles_out) {
+  orc_cone_ctx c;
+  orc_cone_ctx_init(cam, &c);
+  uint32_t key;
+  int n = orc_site_cone_tiles(cam, &c.t, c.cones, sphere, tiles_out, &key);
+  orc_cone_ctx_free(&c);
+  return n;
+}
+
 /* counts[t+1] = number of sites binned to tile t; returns the total. */
 int64_t orc_bin_count(const pf_camera* cam, const float* spheres, int32_t n, int32_t* counts) {
   const int th = orc_tiles_h(cam), tw = orc_tiles_w(cam);
   memset(counts, 0, sizeof(int32_t) * (size_t)(th * tw + 1));
   int64_t total = 0;
+  if (cam->ray_maps) {
+    orc_cone_ctx c;
+    orc_cone_ctx_init(cam, &c);
+    for (int s = 0; s < n; s++) {
+      uint32_t key;
+      int k = orc_site_cone_tiles(cam, &c.t, c.cones, spheres + 4 * (size_t)s, c.tiles, &key);
+      for (int i = 0; i < k; i++) counts[c.tiles[i] + 1]++;
+      total += k > 0 ? k : 0;
+    }
+    orc_cone_ctx_free(&c);
+    return total;
+  }
   for (int s = 0; s < n; s++) {
     orc_footprint fp;
     if (!orc_site_footprint(cam, spheres + 4 * (size_t)s, &fp)) continue;
@@ -57,7 +115,24 @@ void orc_bin_lists(const pf_camera* cam, const float* spheres, int32_t n, const 
   int64_t m = offsets[nt];
   orc_rec* recs = (orc_rec*)malloc(sizeof(orc_rec) * (size_t)(m > 0 ? m : 1));
   int64_t* cur = (int64_t*)calloc((size_t)nt, sizeof(int64_t));
-  for (int s = 0; s < n; s++) {
+  if (cam->ray_maps) {
+    orc_cone_ctx c;
+    orc_cone_ctx_init(cam, &c);
+    for (int s = 0; s < n; s++) {
+      uint32_t key;
+      int k = orc_site_cone_tiles(cam, &c.t, c.cones, spheres + 4 * (size_t)s, c.tiles, &key);
+      for (int i = 0; i < k; i++) {
+        int t = c.tiles[i];
+        int64_t idx = offsets[t] + cur[t]++;
+        if (idx < offsets[t + 1]) {
+          recs[idx].key = key;
+          recs[idx].site = s;
+        }
+      }
+    }
+    orc_cone_ctx_free(&c);
+  }
+  for (int s = 0; s < n && !cam->ray_maps; s++) {
     orc_footprint fp;
     if (!orc_site_footprint(cam, spheres + 4 * (size_t)s, &fp)) continue;
     for (int ti = fp.ti0; ti <= fp.ti1; ti++)

@@ -1,10 +1,14 @@
-// Tile binning of power-cell screen footprints (pinhole cameras).
+// Tile binning of power-cell screen footprints.
 //
 // Replaces, per frame:
 //   count_visible_pinhole_kernel  rasterize.py:140-200
 //   torch.cumsum + .item()        rasterize.py:1713-1714
 //   write_visible_pinhole_kernel  rasterize.py:293-363
 //   torch.argsort + gather        rasterize.py:1761-1768
+// and for ray-map cameras
+//   _precompute_cones, compute_leaf_cones_kernel, merge_cones_kernel
+//                                 rasterize.py:1518-1585, 468-569
+//   count/write_visible_generic_kernel  rasterize.py:202-291, 365-466
 //
 // This file is compiled with -ffp-contract=off: tile membership and sort keys
 // are float-derived integers and must not depend on FMA contraction. Every
@@ -195,6 +199,241 @@ __global__ void __launch_bounds__(256)
   if (any && visible) visible[s] = 1;
 }
 
+// ------------------------------------------------ non-pinhole cameras
+// Ray-map cameras bin through a quadtree of ray cones (rasterize.py:1518-1585):
+// level 0 holds one cone per 2x2 pixel block, each further level one per 2x2
+// block of the level below (dims rounded up) until 1x1, level after level,
+// row-major inside a level; level 2 is exactly the 8x8 tiles. A cone is
+// (unit axis, cos of the half-angle); cos > 1 marks a node without rays.
+//
+// A site is binned to every tile whose cone, and every ancestor's cone, meets
+// its ball (sphere_cone_intersect, camera.py:27-44). The reference descends
+// with a 32-entry stack per thread; here the descent is stackless: a quadtree
+// over a grid has implicit parent/child/sibling links, so the walk needs only
+// (level, row, column) in registers. Visiting order differs from the
+// reference's; it only decides atomic slot order, which the per-tile sort
+// removes.
+constexpr int CONE_LEAF = 2;
+constexpr int CONE_MAX_LEVELS = 14;  // images up to 8192 pixels a side
+
+struct ConeTree {
+  int n_levels, tile_level, total;
+  int off[CONE_MAX_LEVELS], dh[CONE_MAX_LEVELS], dw[CONE_MAX_LEVELS];
+};
+
+static bool make_cone_tree(int W, int H, ConeTree& t) {
+  t = ConeTree{};
+  int dh = (H + CONE_LEAF - 1) / CONE_LEAF, dw = (W + CONE_LEAF - 1) / CONE_LEAF;
+  t.n_levels = 0;
+  t.total = 0;
+  for (;;) {
+    if (t.n_levels == CONE_MAX_LEVELS) return false;
+    t.off[t.n_levels] = t.total;
+    t.dh[t.n_levels] = dh;
+    t.dw[t.n_levels] = dw;
+    t.total += dh * dw;
+    t.n_levels++;
+    if (dh == 1 && dw == 1) break;
+    dh = (dh + 1) / 2;
+    dw = (dw + 1) / 2;
+  }
+  t.tile_level = 0;
+  for (int sz = CONE_LEAF; sz < PF_TILE_WIDTH; sz *= 2) t.tile_level++;
+  return t.n_levels > t.tile_level;
+}
+
+// compute_leaf_cones_kernel (rasterize.py:468-512): one thread per leaf.
+__global__ void __launch_bounds__(256)
+    cone_leaf_kernel(const float* __restrict__ ray_maps, int W, int H, int dh, int dw, float4* __restrict__ cones) {
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= dh * dw) return;
+  const int a = id / dw, b = id % dw;
+  f3 rays[4];
+  f3 sum{0.0f, 0.0f, 0.0f};
+  int nv = 0;
+#pragma unroll
+  for (int di = 0; di < CONE_LEAF; di++)
+#pragma unroll
+    for (int dj = 0; dj < CONE_LEAF; dj++) {
+      const int pi = min(a * CONE_LEAF + di, H - 1), pj = min(b * CONE_LEAF + dj, W - 1);
+      const float* rm = ray_maps + ((size_t)pi * W + pj) * 6;
+      const f3 r{rm[3], rm[4], rm[5]};
+      const float len = sqrtf(dot(r, r));
+      if (len > 1.0e-6f) {
+        const f3 rn{r.x / len, r.y / len, r.z / len};
+        rays[nv++] = rn;
+        sum = sum + rn;
+      }
+    }
+  if (nv == 0) {
+    cones[id] = make_float4(0.0f, 0.0f, 0.0f, 2.0f);
+    return;
+  }
+  const float sl = sqrtf(dot(sum, sum));
+  const f3 axis = sl > 0.0f ? f3{sum.x / sl, sum.y / sl, sum.z / sl} : f3{0.0f, 0.0f, 0.0f};
+  float ch = 1.0f;
+  for (int k = 0; k < nv; k++) {
+    const float d = dot(axis, rays[k]);
+    ch = d < ch ? d : ch;
+  }
+  cones[id] = make_float4(axis.x, axis.y, axis.z, ch);
+}
+
+// merge_cones_kernel (rasterize.py:514-569): one thread per parent node.
+__global__ void __launch_bounds__(256) cone_merge_kernel(float4* __restrict__ cones, int poff, int pdh, int pdw,
+                                                         int coff, int cdh, int cdw) {
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= pdh * pdw) return;
+  const int a = id / pdw, b = id % pdw;
+  // children in (0,0) (0,1) (1,0) (1,1) order; fixed slots keep them in registers
+  float4 c[4];
+  bool ok[4];
+  f3 sum{0.0f, 0.0f, 0.0f};
+  bool any = false;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int ca = 2 * a + (k >> 1), cb = 2 * b + (k & 1);
+    ok[k] = false;
+    c[k] = make_float4(0.0f, 0.0f, 0.0f, 2.0f);
+    if (ca < cdh && cb < cdw) {
+      c[k] = cones[coff + ca * cdw + cb];
+      ok[k] = c[k].w <= 1.0f;
+    }
+    if (ok[k]) {
+      sum = sum + f3{c[k].x, c[k].y, c[k].z};
+      any = true;
+    }
+  }
+  if (!any) {
+    cones[poff + id] = make_float4(0.0f, 0.0f, 0.0f, 2.0f);
+    return;
+  }
+  const float sl = sqrtf(dot(sum, sum));
+  const f3 axis = sl > 0.0f ? f3{sum.x / sl, sum.y / sl, sum.z / sl} : f3{0.0f, 0.0f, 0.0f};
+  float ch = 1.0f;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    if (!ok[k]) continue;
+    const float d = dot(axis, f3{c[k].x, c[k].y, c[k].z});
+    const float sa = 1.0f - d * d, sc = 1.0f - c[k].w * c[k].w;
+    const float ext = d * c[k].w - sqrtf(sa > 0.0f ? sa : 0.0f) * sqrtf(sc > 0.0f ? sc : 0.0f);
+    ch = ext < ch ? ext : ch;
+  }
+  cones[poff + id] = make_float4(axis.x, axis.y, axis.z, ch);
+}
+
+// Per-site part of sphere_cone_intersect (camera.py:27-44) and the cull of
+// rasterize.py:226-229.
+struct ConeSite {
+  f3 v;
+  float dist, sin_beta, cos_beta;
+  uint32_t key;
+};
+
+__device__ __forceinline__ bool cone_site(const BinCam& cam, float4 sph, ConeSite& cs) {
+  const f3 center{sph.x, sph.y, sph.z};
+  const float radius = sph.w;
+  cs.v = center - cam.eye;
+  const f3 fwd = cross(cam.up, cam.right);
+  cs.dist = sqrtf(dot(cs.v, cs.v));
+  if (cs.dist < 4.0f * radius || dot(cs.v, fwd) < 0.1f) return false;
+  cs.sin_beta = radius / cs.dist;
+  cs.cos_beta = sqrtf(1.0f - cs.sin_beta * cs.sin_beta);
+  const float pow_dist = dot(cs.v, cs.v) - radius * radius;
+  cs.key = __builtin_bit_cast(uint32_t, pow_dist);
+  return true;
+}
+
+__device__ __forceinline__ bool cone_hit(const ConeSite& cs, float4 cone) {
+  const float cos_alpha = dot(cs.v, f3{cone.x, cone.y, cone.z}) / cs.dist;
+  const float sin_half = sqrtf(1.0f - cone.w * cone.w);
+  const float threshold = cone.w * cs.cos_beta - sin_half * cs.sin_beta;
+  return cos_alpha > threshold;
+}
+
+// Stackless depth-first walk; visit(tile) for every tile reached. The level
+// tables come from LDS (the level index differs per lane).
+template <typename Visit>
+__device__ __forceinline__ void cone_walk(const ConeSite& cs, const float4* __restrict__ cones, const int* s_off,
+                                          const int* s_dh, const int* s_dw, int top, int tile_level, int th,
+                                          Visit visit) {
+  int level = top, a = 0, b = 0;
+  for (;;) {
+    const float4 cone = cones[s_off[level] + a * s_dw[level] + b];
+    if (cone.w <= 1.0f && cone_hit(cs, cone)) {
+      if (level == tile_level) {
+        visit(a + b * th);
+      } else {
+        level--;
+        a *= 2;
+        b *= 2;
+        continue;  // the (0,0) child always exists
+      }
+    }
+    // next sibling in (0,0) (0,1) (1,0) (1,1) order, else climb
+    for (;;) {
+      if (level == top) return;
+      if (!(b & 1) && b + 1 < s_dw[level]) {
+        b++;
+        break;
+      }
+      if (!(a & 1) && a + 1 < s_dh[level]) {
+        a++;
+        b &= ~1;
+        break;
+      }
+      level++;
+      a >>= 1;
+      b >>= 1;
+    }
+  }
+}
+
+#define CONE_LDS_TABLES(t)                                                            \
+  __shared__ int s_off[CONE_MAX_LEVELS], s_dh[CONE_MAX_LEVELS], s_dw[CONE_MAX_LEVELS]; \
+  if (threadIdx.x < CONE_MAX_LEVELS) {                                                \
+    s_off[threadIdx.x] = t.off[threadIdx.x];                                          \
+    s_dh[threadIdx.x] = t.dh[threadIdx.x];                                            \
+    s_dw[threadIdx.x] = t.dw[threadIdx.x];                                            \
+  }                                                                                   \
+  __syncthreads()
+
+// count_visible_generic_kernel (rasterize.py:202-291)
+__global__ void __launch_bounds__(256)
+    bin_count_cones_kernel(BinCam cam, ConeTree t, const float4* __restrict__ cones,
+                           const float4* __restrict__ spheres, int n, int32_t* __restrict__ counts) {
+  CONE_LDS_TABLES(t);
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  ConeSite cs;
+  if (!cone_site(cam, spheres[s], cs)) return;
+  cone_walk(cs, cones, s_off, s_dh, s_dw, t.n_levels - 1, t.tile_level, cam.th,
+            [&](int tile) { atomicAdd(&counts[tile + 1], 1); });
+}
+
+// write_visible_generic_kernel (rasterize.py:365-466)
+__global__ void __launch_bounds__(256)
+    bin_emit_cones_kernel(BinCam cam, ConeTree t, const float4* __restrict__ cones,
+                          const float4* __restrict__ spheres, int n, const int64_t* __restrict__ offsets,
+                          uint64_t* __restrict__ records, int32_t* __restrict__ cursor,
+                          uint8_t* __restrict__ visible) {
+  CONE_LDS_TABLES(t);
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  ConeSite cs;
+  if (!cone_site(cam, spheres[s], cs)) return;
+  const uint64_t rec = ((uint64_t)cs.key << 32) | (uint32_t)s;
+  bool any = false;
+  cone_walk(cs, cones, s_off, s_dh, s_dw, t.n_levels - 1, t.tile_level, cam.th, [&](int tile) {
+    const int64_t idx = offsets[tile] + (int64_t)atomicAdd(&cursor[tile], 1);
+    if (idx < offsets[tile + 1]) {
+      records[idx] = rec;
+      any = true;
+    }
+  });
+  if (any && visible) visible[s] = 1;
+}
+
 // Inclusive scan of counts[0..n] (int32) into offsets[0..n] (int64) by one
 // 1024-thread workgroup: each thread scans a contiguous run serially, then
 // the run totals are scanned across the workgroup.
@@ -367,6 +606,67 @@ int pf_bin_emit(const pf_camera* cam, const float* spheres, int32_t n_sites, con
   if (n_sites > 0)
     bin_emit_kernel<<<(n_sites + 255) / 256, 256, 0, st>>>(make_bincam(*cam), reinterpret_cast<const float4*>(spheres),
                                                           n_sites, offsets, records, cursor, visible);
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+int32_t pf_cone_tree_nodes(int32_t width, int32_t height) {
+  ConeTree t;
+  if (width < 2 || height < 2 || !make_cone_tree(width, height, t)) return -1;
+  return t.total;
+}
+
+int pf_bin_cones(const pf_camera* cam, float* cones, void* stream) {
+  if (!cam || !cones || !cam->ray_maps) return set_error(PF_ERR_ARG, "pf_bin_cones: needs a camera with ray_maps");
+  ConeTree t;
+  if (cam->width < 2 || cam->height < 2 || !make_cone_tree(cam->width, cam->height, t))
+    return set_error(PF_ERR_ARG, "pf_bin_cones: image must be 5..8192 pixels a side");
+  hipStream_t st = as_stream(stream);
+  float4* c = reinterpret_cast<float4*>(cones);
+  const int nl = t.dh[0] * t.dw[0];
+  cone_leaf_kernel<<<(nl + 255) / 256, 256, 0, st>>>(cam->ray_maps, cam->width, cam->height, t.dh[0], t.dw[0], c);
+  for (int l = 1; l < t.n_levels; l++) {
+    const int np = t.dh[l] * t.dw[l];
+    cone_merge_kernel<<<(np + 255) / 256, 256, 0, st>>>(c, t.off[l], t.dh[l], t.dw[l], t.off[l - 1], t.dh[l - 1],
+                                                          t.dw[l - 1]);
+  }
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+int pf_bin_count_cones(const pf_camera* cam, const float* cones, const float* spheres, int32_t n_sites,
+                       int32_t* tile_counts, void* stream) {
+  if (!cam || !cones || !tile_counts || n_sites < 0 || (n_sites > 0 && !spheres))
+    return set_error(PF_ERR_ARG, "pf_bin_count_cones: bad arguments");
+  ConeTree t;
+  if (cam->width < 2 || cam->height < 2 || !make_cone_tree(cam->width, cam->height, t))
+    return set_error(PF_ERR_ARG, "pf_bin_count_cones: image must be 5..8192 pixels a side");
+  hipStream_t st = as_stream(stream);
+  const int64_t nt = (int64_t)tiles_h(*cam) * tiles_w(*cam);
+  fill_i32_kernel<<<(unsigned)((nt + 1 + 255) / 256), 256, 0, st>>>(tile_counts, nt + 1, 0);
+  if (n_sites > 0)
+    bin_count_cones_kernel<<<(n_sites + 255) / 256, 256, 0, st>>>(
+        make_bincam(*cam), t, reinterpret_cast<const float4*>(cones), reinterpret_cast<const float4*>(spheres),
+        n_sites, tile_counts);
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+int pf_bin_emit_cones(const pf_camera* cam, const float* cones, const float* spheres, int32_t n_sites,
+                      const int64_t* offsets, int32_t n_tiles, uint64_t* records, int32_t* cursor, uint8_t* visible,
+                      void* stream) {
+  if (!cam || !cones || !offsets || !cursor || n_sites < 0 || (n_sites > 0 && !spheres))
+    return set_error(PF_ERR_ARG, "pf_bin_emit_cones: bad arguments");
+  ConeTree t;
+  if (cam->width < 2 || cam->height < 2 || !make_cone_tree(cam->width, cam->height, t))
+    return set_error(PF_ERR_ARG, "pf_bin_emit_cones: image must be 5..8192 pixels a side");
+  if (n_tiles != tiles_h(*cam) * tiles_w(*cam)) return set_error(PF_ERR_ARG, "pf_bin_emit_cones: n_tiles mismatch");
+  hipStream_t st = as_stream(stream);
+  fill_i32_kernel<<<(unsigned)((n_tiles + 255) / 256), 256, 0, st>>>(cursor, n_tiles, 0);
+  if (n_sites > 0)
+    bin_emit_cones_kernel<<<(n_sites + 255) / 256, 256, 0, st>>>(
+        make_bincam(*cam), t, reinterpret_cast<const float4*>(cones), reinterpret_cast<const float4*>(spheres),
+        n_sites, offsets, records, cursor, visible);
   PF_HIP(hipGetLastError());
   return PF_OK;
 }

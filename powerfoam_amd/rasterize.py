@@ -14,6 +14,7 @@ import torch
 
 from . import _abi
 from ._abi import check, ptr, stream_handle
+from .camera import TorchCamera
 
 TILE_WIDTH = 8
 TILE_SIZE = TILE_WIDTH * TILE_WIDTH
@@ -94,27 +95,55 @@ class Rasterizer:
         return check(st)
 
     # ------------------------------------------------------------------ binning
+    def _camera(self, camera, ray_maps=None):
+        """(C-ABI camera, tensor to keep alive). With args.is_pinhole the rays
+        are generated in the kernels and camera.ray_maps is not read, as in the
+        reference's pinhole kernels (rasterize.py:940-944); otherwise they come
+        from camera.ray_maps, or from pinhole ray maps built as TorchCamera.to_warp
+        does (camera.py:197-215), and binning goes through the cone tree."""
+        if getattr(self.args, "is_pinhole", True):
+            return TorchCamera(camera.eye, camera.right, camera.up, camera.width, camera.height).to_pf(), None
+        rm = ray_maps if ray_maps is not None else camera.ray_maps
+        if rm is None:
+            rm = TorchCamera(camera.eye, camera.right, camera.up, camera.width, camera.height) \
+                ._build_pinhole_ray_maps()
+        rm = rm.detach().to(device=self.device, dtype=torch.float32).contiguous()
+        return TorchCamera(camera.eye, camera.right, camera.up, camera.width, camera.height, rm).to_pf(), rm
+
     def bin(self, camera, all_spheres):
-        """count -> scan -> (one host sync for M) -> emit -> per-tile sort."""
+        """count -> scan -> (one host sync for M) -> emit -> per-tile sort; ray-map
+        cameras count and emit through the cone tree (rasterize.py:1518-1760)."""
         lib = self.lib
-        if camera.ray_maps is not None:
-            raise NotImplementedError("non-pinhole binning is not built yet")
-        cam = camera.to_pf()
+        cam, keep = self._camera(camera)
         th, tw = _abi.tiles_hw(camera.height, camera.width)
         nt = th * tw
         n = all_spheres.shape[0]
         dev = all_spheres.device
         st = stream_handle()
+        cones = None
+        if keep is not None:
+            nodes = lib.pf_cone_tree_nodes(camera.width, camera.height)
+            if nodes < 0:
+                raise ValueError("ray-map cameras need 5..8192 pixels a side")
+            cones = torch.empty((nodes, 4), dtype=torch.float32, device=dev)
+            check(lib.pf_bin_cones(_ref(cam), ptr(cones), st))
         counts = torch.empty(nt + 1, dtype=torch.int32, device=dev)
-        check(lib.pf_bin_count(_ref(cam), ptr(all_spheres), n, ptr(counts), st))
+        if cones is None:
+            check(lib.pf_bin_count(_ref(cam), ptr(all_spheres), n, ptr(counts), st))
+        else:
+            check(lib.pf_bin_count_cones(_ref(cam), ptr(cones), ptr(all_spheres), n, ptr(counts), st))
         offsets = torch.empty(nt + 1, dtype=torch.int64, device=dev)
         check(lib.pf_bin_scan(ptr(counts), nt, ptr(offsets), st))
         m = int(offsets[-1].item())
         records = torch.empty(max(m, 1), dtype=torch.int64, device=dev)
         cursor = torch.empty(nt, dtype=torch.int32, device=dev)
         visible = torch.zeros(n, dtype=torch.uint8, device=dev)
-        check(lib.pf_bin_emit(_ref(cam), ptr(all_spheres), n, ptr(offsets), nt, ptr(records), ptr(cursor),
-                              ptr(visible), st))
+        if cones is None:
+            check(lib.pf_bin_emit(_ref(cam), ptr(all_spheres), n, ptr(offsets), nt, ptr(records), ptr(cursor),
+                                  ptr(visible), st))
+        else:
+            check(lib.pf_bin_emit_cones(_ref(cam), ptr(cones), ptr(all_spheres), n, ptr(offsets), nt, ptr(records),
+                                        ptr(cursor), ptr(visible), st))
         sites = torch.empty(max(m, 1), dtype=torch.int32, device=dev)
         check(lib.pf_bin_sort(ptr(offsets), nt, ptr(records), ptr(sites), None, 0, st))
         return offsets, sites[:m] if m > 0 else sites[:0], visible.bool(), nt
@@ -165,7 +194,7 @@ class Rasterizer:
         tile_early_stop = torch.zeros(nt, dtype=torch.int32, device=dev)
 
         a = _abi.PfRasterFwdArgs()
-        a.cam = camera.to_pf()
+        a.cam, ray_maps = self._camera(camera)
         a.scene = _scene_struct(all_spheres, all_nsigmas, all_texel_sites, all_texel_rgbh, adjacency,
                                 adjacency_offsets, None if self.exact_geometry else adjacency_diff)
         a.tiles = _abi.PfTileLists(nt, offsets.data_ptr(), tile_sites.data_ptr() if tile_sites.numel() else None)
@@ -189,6 +218,7 @@ class Rasterizer:
 
         grad_ctx.rasterizer = self
         grad_ctx.camera = camera
+        grad_ctx.ray_maps = ray_maps
         grad_ctx.save_for_backward(dq, all_spheres, all_nsigmas, all_texel_sites, all_texel_rgbh, adjacency,
                                    adjacency_offsets, adjacency_diff, tile_sites, offsets, tile_early_stop,
                                    ray_gt_c, color_out, log_t_out)
@@ -219,7 +249,7 @@ class Rasterizer:
             offsets, tile_sites, _, nt = self.bin(camera, all_spheres)
             color_out = torch.zeros((H, W, 3), dtype=torch.float32, device=points.device)
             a = _abi.PfRasterRgbArgs()
-            a.cam = camera.to_pf()
+            a.cam, ray_maps = self._camera(camera)
             a.scene = _scene_struct(all_spheres, all_nsigmas, all_texel_sites, all_texel_rgbh, adjacency,
                                     adjacency_offsets, adjacency_diff)
             a.tiles = _abi.PfTileLists(nt, offsets.data_ptr(), tile_sites.data_ptr() if tile_sites.numel() else None)
@@ -263,7 +293,7 @@ class Rasterizer:
         texel_height_grad = zf((N, T))
 
         a = _abi.PfRasterBwdArgs()
-        a.cam = camera.to_pf()
+        a.cam, ray_maps = self._camera(camera, grad_ctx.ray_maps)
         a.scene = _scene_struct(all_spheres, all_nsigmas, all_texel_sites, all_texel_rgbh, adjacency,
                                 adjacency_offsets, None if self.exact_geometry else adjacency_diff)
         a.tiles = _abi.PfTileLists(offsets.shape[0] - 1, offsets.data_ptr(),
@@ -288,7 +318,7 @@ class Rasterizer:
         self._launch("raster_bwd", self.lib.pf_raster_bwd, _ref(a), st)
         for t in (spheres_grad, nsigmas_grad, texel_sites_grad, texel_rgb_grad, texel_height_grad):
             check(self.lib.pf_scrub_nonfinite(ptr(t), t.numel(), st))
-        del grad_ctx.rasterizer, grad_ctx.camera
+        del grad_ctx.rasterizer, grad_ctx.camera, grad_ctx.ray_maps
         return (None, None, None, spheres_grad[:, :3], spheres_grad[:, 3], nsigmas_grad[:, 3],
                 nsigmas_grad[:, :3], texel_sites_grad, texel_rgb_grad, texel_height_grad, None, None, None, None)
 

@@ -56,14 +56,21 @@ class Case:
 
 
 def small_case(seed=0, n=500, width=64, height=48, T=8, height_std=0.0, fov=60.0, eye=(0.0, 0.0, 3.0),
-               r_range=(0.02, 0.12), quantiles=None, extent=1.0):
+               r_range=(0.02, 0.12), quantiles=None, extent=1.0, fisheye_fov=None):
     """Uniform scene with radii large enough that tiles hold many sites and
-    faces (Cech adjacency), random per-texel colours, a random target."""
+    faces (Cech adjacency), random per-texel colours, a random target.
+    fisheye_fov: an equidistant fisheye of that field of view given as ray
+    maps (width == height), binned through the cone tree."""
     s = synth.scene_uniform(seed, n=n, n_texels=T, height_std=height_std, r_range=r_range, extent=extent)
     rng = np.random.default_rng(seed + 100)
     rgb = rng.uniform(0.0, 1.0, (n, T, 3)).astype(np.float32)
     adj, off = synth.cech_adjacency(s["points"], s["radii"])
     cam = look_at(eye, (0.0, 0.0, 0.0), width, height, fov)
+    if fisheye_fov is not None:
+        import torch
+        from powerfoam_amd.camera import fisheye_ray_maps
+        assert width == height
+        cam.ray_maps = torch.tensor(fisheye_ray_maps(eye, (0.0, 0.0, 0.0), width, fisheye_fov))
     gt = rng.uniform(0.0, 1.0, (height, width, 3)).astype(np.float32)
     dq = None
     if quantiles:
@@ -83,8 +90,8 @@ def random_grads(case, seed=0, quantiles=False):
     return {k: v.astype(np.float32) for k, v in g.items()}
 
 
-def _args(T):
-    return SimpleNamespace(render_objective="volume", num_texel_sites=T, is_pinhole=True,
+def _args(T, is_pinhole=True):
+    return SimpleNamespace(render_objective="volume", num_texel_sites=T, is_pinhole=is_pinhole,
                            disable_coop_prim_load=False, disable_coop_adj_load=False)
 
 
@@ -96,7 +103,7 @@ def run_gpu(case, dev, grads_in=None, exact_gradients=False, exact_geometry=Fals
     from powerfoam_amd.rasterize import Rasterizer
 
     T = case.texel_sites.shape[1]
-    r = Rasterizer(_args(T), dev)
+    r = Rasterizer(_args(T, case.cam.ray_maps is None), dev)
     r.exact_gradients = exact_gradients
     r.exact_geometry = exact_geometry
     tt = lambda a, dt=torch.float32: torch.tensor(np.ascontiguousarray(a), dtype=dt, device=dev)
@@ -185,12 +192,14 @@ def run_gpu_benchmark(case, dev, threshold=1e-2, attr_dtype="float"):
     from powerfoam_amd.rasterize import Rasterizer, power_face_encoding
 
     T = case.texel_sites.shape[1]
-    r = Rasterizer(_args(T), dev, attr_dtype)
+    r = Rasterizer(_args(T, case.cam.ray_maps is None), dev, attr_dtype)
     tt = lambda a, dt=torch.float32: torch.tensor(np.ascontiguousarray(a), dtype=dt, device=dev)
     pts, rad = tt(case.points), tt(case.radii)
     adj, off = tt(case.adjacency, torch.int32), tt(case.adjacency_offsets, torch.int32)
     diff = power_face_encoding(pts, rad, adj, off)
-    col = r.benchmark(case.cam, pts, rad, tt(case.density), tt(case.normals), tt(case.texel_sites),
+    cam = TorchCamera(case.cam.eye, case.cam.right, case.cam.up, case.width, case.height,
+                      None if case.cam.ray_maps is None else case.cam.ray_maps.to(dev))
+    col = r.benchmark(cam, pts, rad, tt(case.density), tt(case.normals), tt(case.texel_sites),
                       tt(case.texel_rgb), tt(case.texel_height), adj, off, diff, threshold)
     torch.cuda.synchronize()
     return col.cpu().numpy(), diff.cpu().numpy().view(np.uint16)

@@ -14,15 +14,20 @@ pytestmark = pytest.mark.gpu
 def _bin_gpu(cam_t, spheres):
     import torch
     from types import SimpleNamespace
+    from powerfoam_amd.camera import TorchCamera
     from powerfoam_amd.rasterize import Rasterizer
-    r = Rasterizer(SimpleNamespace(num_texel_sites=8, render_objective="volume", is_pinhole=True), "cuda")
+    pin = cam_t.ray_maps is None
+    r = Rasterizer(SimpleNamespace(num_texel_sites=8, render_objective="volume", is_pinhole=pin), "cuda")
     sp = torch.tensor(spheres, device="cuda")
-    offsets, sites, visible, nt = r.bin(cam_t, sp)
+    cam = TorchCamera(cam_t.eye, cam_t.right, cam_t.up, cam_t.width, cam_t.height,
+                      None if pin else cam_t.ray_maps.cuda())
+    offsets, sites, visible, nt = r.bin(cam, sp)
     return offsets.cpu().numpy(), sites.cpu().numpy(), visible.cpu().numpy()
 
 
 def _check(cam_t, spheres):
-    cam, _ = orc.camera(cam_t.eye.numpy(), cam_t.right.numpy(), cam_t.up.numpy(), cam_t.width, cam_t.height)
+    rm = None if cam_t.ray_maps is None else cam_t.ray_maps.numpy()
+    cam, _ = orc.camera(cam_t.eye.numpy(), cam_t.right.numpy(), cam_t.up.numpy(), cam_t.width, cam_t.height, rm)
     off_o, sites_o, _ = orc.bin_lists(cam, spheres)
     off_g, sites_g, vis_g = _bin_gpu(cam_t, spheres)
     assert np.array_equal(off_g, off_o)
@@ -55,3 +60,45 @@ def test_binning_empty_and_offscreen():
     cam = look_at((0.0, 0.0, 3.0), (0.0, 0.0, 0.0), 64, 48, 60.0)
     sp = np.array([[0.0, 0.0, 5.0, 0.1], [0.0, 0.0, 2.9, 0.5], [50.0, 0.0, 0.0, 0.1]], np.float32)
     assert _check(cam, sp) == 0
+
+
+def _fisheye_cam(eye, size, fov):
+    import torch
+    from powerfoam_amd.camera import fisheye_ray_maps
+    cam = look_at(eye, (0.0, 0.0, 0.0), size, size, 60.0)
+    cam.ray_maps = torch.tensor(fisheye_ray_maps(eye, (0.0, 0.0, 0.0), size, fov))
+    return cam
+
+
+@pytest.mark.parametrize("size,fov", [(37, 180.0), (256, 120.0), (1024, 180.0)])
+def test_cone_tree_bitexact(size, fov):
+    import torch
+    from powerfoam_amd import _abi
+    cam_t = _fisheye_cam((0.3, 0.2, 2.5), size, fov)
+    rm = cam_t.ray_maps.numpy()
+    cam, _k = orc.camera(cam_t.eye.numpy(), cam_t.right.numpy(), cam_t.up.numpy(), size, size, rm)
+    ref = orc.cones(cam)
+    lib = _abi.load()
+    assert lib.pf_cone_tree_nodes(size, size) == ref.shape[0]
+    rmd = cam_t.ray_maps.cuda().contiguous()
+    from powerfoam_amd.camera import TorchCamera
+    pc = TorchCamera(cam_t.eye, cam_t.right, cam_t.up, size, size, rmd).to_pf()
+    out = torch.empty(ref.shape, dtype=torch.float32, device="cuda")
+    _abi.check(lib.pf_bin_cones(_abi.ctypes.byref(pc), out.data_ptr(), _abi.stream_handle()))
+    torch.cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy().view(np.uint32), ref.view(np.uint32))
+
+
+@pytest.mark.parametrize("seed,size,fov", [(0, 96, 180.0), (1, 333, 120.0)])
+def test_cone_binning_bitexact(seed, size, fov):
+    s = synth.scene_uniform(seed, n=20000, r_range=(0.005, 0.2), extent=3.0)
+    sp = np.concatenate([s["points"], s["radii"][:, None]], 1)
+    m = _check(_fisheye_cam((0.3, 0.2, 3.0), size, fov), sp)
+    assert m > 1000
+
+
+def test_cone_binning_bitexact_gmm_fisheye():
+    s = synth.scene_gmm(1, n=100_000)
+    sp = np.concatenate([s["points"], s["radii"][:, None]], 1)
+    m = _check(_fisheye_cam((4.0, 1.0, 0.0), 1024, 180.0), sp)
+    assert m > 100_000

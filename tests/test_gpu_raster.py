@@ -29,6 +29,9 @@ CASES = [
     dict(seed=5, n=1500, width=97, height=61, fov=70.0),   # ragged: out-of-bounds lanes
     dict(seed=7, n=800, width=64, height=48, height_std=0.05, T=5),
     dict(seed=9, n=600, width=40, height=33, quantiles=[0.9, 0.5, 0.1]),
+    # ray-map cameras (equidistant fisheye), binned through the cone tree
+    dict(seed=11, n=1500, width=56, height=56, eye=(0.0, 0.0, 1.3), fisheye_fov=150.0),
+    dict(seed=12, n=1500, width=64, height=64, eye=(0.2, 0.1, 0.6), fisheye_fov=180.0, height_std=0.05),
 ]
 
 
@@ -66,8 +69,9 @@ def test_backward_matches_fp64_oracle(dev, cfg):
         assert err < GRAD_REL_TOL, (k, err)
 
 
-@pytest.mark.parametrize("cfg,attr", [(CASES[0], "float"), (CASES[1], "float"), (CASES[2], "half")],
-                         ids=["case0", "case1", "case2-half"])
+@pytest.mark.parametrize("cfg,attr", [(CASES[0], "float"), (CASES[1], "float"), (CASES[2], "half"),
+                                      (CASES[4], "float")],
+                         ids=["case0", "case1", "case2-half", "case4-fisheye"])
 def test_benchmark_matches_fp64_oracle(dev, cfg, attr):
     """Rasterizer.benchmark (colour only, per-ray stop, fp16 power faces)."""
     case = harness.small_case(**cfg)
