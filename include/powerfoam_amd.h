@@ -136,6 +136,25 @@ typedef struct pf_raster_rgb_args {
   float* color; /* [H,W,3], every in-image pixel written */
 } pf_raster_rgb_args;
 
+/* Visualisation pass (visualization_kernel, rasterize.py:739-890; VisOptions
+ * :15-20): like pf_raster_rgb, but faces from the training encoding
+ * (c_adj - c, r_adj) of pf_adj_encode, a cap on the list entries each ray
+ * looks at, and extra outputs. */
+typedef struct pf_raster_vis_args {
+  pf_camera cam;
+  pf_scene scene;
+  pf_tile_lists tiles;
+  float transmittance_threshold;
+  int32_t max_intersections;
+  float depth_quantile;
+  float bkgd_color[3];
+  float* color;           /* [H,W,3]: composited colour + bkgd * T */
+  float* depth;           /* [H,W]: where T first drops below depth_quantile, 0 if never */
+  float* normal;          /* [H,W,3]: sum of alpha T n */
+  float* alpha;           /* [H,W]: 1 - T */
+  int32_t* intersections; /* [H,W]: list entries looked at (the stopping one included) */
+} pf_raster_vis_args;
+
 /* Ray tracer (raytrace.py:115-241). scene.adjacency must be the full power
  * (weighted Delaunay) adjacency; scene.adjacency_diff holds the power-face
  * encoding (as pf_raster_rgb_args) or is NULL for exact geometry computed from
@@ -269,6 +288,9 @@ int pf_sh_bwd(const float* sh, const float* texel_sites, const float* grad_texel
 
 /* benchmark_kernel rasterize.py:614-734 (Rasterizer.benchmark :1982-2131). */
 int pf_raster_rgb(const pf_raster_rgb_args* args, void* stream);
+
+/* visualization_kernel rasterize.py:739-890 (Rasterizer.visualize :2133-2311). */
+int pf_raster_vis(const pf_raster_vis_args* args, void* stream);
 
 /* ---- ray tracer (RayTracer.benchmark_kernel raytrace.py:115-241) ------ */
 int pf_trace_fwd(const pf_trace_args* args, void* stream);

@@ -53,6 +53,8 @@ def load():
         lib.orc_power_face_encode.argtypes = [vp, ctypes.c_int32, vp, vp, vp]
         lib.orc_power_face_encode.restype = None
         for p in ("f32", "f64"):
+            getattr(lib, f"orc_raster_vis_{p}").argtypes = [vp, vp, vp, vp, ctypes.c_int, ctypes.c_float, ctypes.c_int,
+                                                            ctypes.c_float, vp, vp]
             getattr(lib, f"orc_raster_rgb_{p}").argtypes = [vp, vp, vp, vp, ctypes.c_int, ctypes.c_float, vp]
             getattr(lib, f"orc_raster_fwd_{p}").argtypes = [vp, vp]
             getattr(lib, f"orc_raster_bwd_{p}").argtypes = [vp, vp]
@@ -184,6 +186,32 @@ def f32_to_f16_bits(x):
     y = np.zeros(x.shape[0], np.uint16)
     lib.orc_f32_to_f16_array(_p(x), x.shape[0], _p(y))
     return y
+
+
+class _VisOut(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_void_p) for k in ("color", "depth", "normal", "alpha", "inter")]
+
+
+def raster_vis(cam, inp, offsets, sites, threshold=1e-3, max_intersections=1024, depth_quantile=0.0,
+               bkgd_color=(0.0, 0.0, 0.0), precision="f64"):
+    """visualization_kernel (rasterize.py:739-890): dict of color, depth,
+    normal, alpha, intersections."""
+    lib = load()
+    dt = np.float64 if precision == "f64" else np.float32
+    H, W = cam.height, cam.width
+    offsets = np.ascontiguousarray(offsets, np.int64)
+    sites = np.ascontiguousarray(sites, np.int32)
+    if sites.size == 0:
+        sites = np.zeros(1, np.int32)
+    out = dict(color=np.zeros((H, W, 3), dt), depth=np.zeros((H, W), dt), normal=np.zeros((H, W, 3), dt),
+               alpha=np.zeros((H, W), dt), intersections=np.zeros((H, W), np.int32))
+    o = _VisOut(*[_addr(out[k]) for k in ("color", "depth", "normal", "alpha", "intersections")])
+    bk = np.ascontiguousarray(bkgd_color, np.float32)
+    sc = inp.struct()
+    getattr(lib, f"orc_raster_vis_{precision}")(ctypes.byref(cam), ctypes.byref(sc), _p(offsets), _p(sites),
+                                                offsets.shape[0] - 1, threshold, int(max_intersections),
+                                                depth_quantile, _p(bk), ctypes.byref(o))
+    return out
 
 
 class Inputs:

@@ -720,6 +720,95 @@ void FN(orc_raster_rgb)(const pf_camera* cam, const pf_scene* sc, const int64_t*
 }
 
 
+/* Visualisation (visualization_kernel, rasterize.py:739-890): colour over a
+ * background, the first depth where T falls below depth_q (0 if never),
+ * alpha-weighted normals, alpha and the number of list entries each ray
+ * looked at. Per ray: stop after max_inter entries or at T < threshold
+ * (the entry that stops it is counted); faces from the training encoding
+ * (c_adj - c, r_adj) (prefetch_adjacency_kernel), leaving the face loop once
+ * the interval is empty. */
+typedef struct {
+  R* color;
+  R* depth;
+  R* normal;
+  R* alpha;
+  int32_t* inter;
+} FN(vis_out);
+
+void FN(orc_raster_vis)(const pf_camera* cam, const pf_scene* sc, const int64_t* offsets, const int32_t* sites,
+                        int n_tiles, float threshold, int max_inter, float depth_q, const float* bkgd,
+                        FN(vis_out) * out) {
+  const int W = cam->width;
+  const int T = sc->n_texels;
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int tile = 0; tile < n_tiles; tile++) {
+    for (int l = 0; l < 64; l++) {
+      int pi, pj, oob;
+      V3 o, d;
+      FN(lane_ray)(cam, tile, l, &pi, &pj, &oob, &o, &d);
+      if (oob) continue;
+      V3 rgb = FN(mk)(0, 0, 0), nacc = FN(mk)(0, 0, 0);
+      R log_t = 0, depth = 0;
+      int found = 0, nhit = 0;
+      const int total = (int)(offsets[tile + 1] - offsets[tile]);
+      for (int k = 0; k < total; k++) {
+        nhit++;
+        if (k >= max_inter) break;
+        R trans = FN(rexp)(log_t);
+        if (trans < (R)threshold) break;
+        const int site = sites[offsets[tile] + k];
+        const float* sp = sc->spheres + 4 * (size_t)site;
+        const float* ns = sc->nsigmas + 4 * (size_t)site;
+        V3 c = FN(ld3)(sp);
+        R r = (R)sp[3];
+        FN(sphere_hit) h = FN(ray_sphere)(o, d, c, r);
+        if (!h.hit) continue;
+        R tn = h.t_near, tf = h.t_far;
+        for (int e = sc->adjacency_offsets[site]; e < sc->adjacency_offsets[site + 1]; e++) {
+          V3 a;
+          R ra, t, dp;
+          FN(face_geom)(sc, e, c, &a, &ra);
+          FN(ray_pface)(o, d, c, r, a, ra, &t, &dp);
+          tf = dp >= (R)0 ? FN(rmin)(t, tf) : tf;
+          tn = dp < (R)0 ? FN(rmax)(t, tn) : tn;
+          if (tn > tf) break;
+        }
+        if (tn > tf) continue;
+        V3 pn = FN(ld3)(ns);
+        R sigma = (R)ns[3];
+        FN(plane_fwd_out) pf = FN(plane_fwd)(o, d, tn, c, pn, r, 1, sc->texel_sites + (size_t)site * T * 3,
+                                             sc->texel_rgbh + (size_t)site * T * 4, T);
+        tf = pf.dp >= (R)0 ? FN(rmin)(pf.t_surf, tf) : tf;
+        tn = pf.dp < (R)0 ? FN(rmax)(pf.t_surf, tn) : tn;
+        R dt = tf - tn;
+        if (dt > (R)0) {
+          R dlt = -sigma * dt;
+          R alpha = (R)1 - FN(rexp)(dlt);
+          rgb = FN(add)(rgb, FN(scl)(pf.rgb, alpha * trans));
+          nacc = FN(add)(nacc, FN(scl)(pn, alpha * trans));
+          log_t += dlt;
+          if (!found && FN(rexp)(log_t) < (R)depth_q) {
+            depth = tn + FN(rlog)(trans / (R)depth_q) / sigma;
+            found = 1;
+          }
+        }
+      }
+      const size_t px = (size_t)pi * W + pj;
+      const R tr = FN(rexp)(log_t);
+      out->color[3 * px + 0] = rgb.x + (R)bkgd[0] * tr;
+      out->color[3 * px + 1] = rgb.y + (R)bkgd[1] * tr;
+      out->color[3 * px + 2] = rgb.z + (R)bkgd[2] * tr;
+      out->depth[px] = depth;
+      out->normal[3 * px + 0] = nacc.x;
+      out->normal[3 * px + 1] = nacc.y;
+      out->normal[3 * px + 2] = nacc.z;
+      out->alpha[px] = (R)1 - tr;
+      out->inter[px] = nhit;
+    }
+  }
+}
+
+
 /* ------------------------------------------------------------ ray tracer
  * RayTracer.benchmark_kernel (raytrace.py:115-241): per pixel, walk the power
  * cells from a start cell; the next cell is the neighbour whose face the ray

@@ -241,7 +241,14 @@ int orc_abi_layout(int64_t* out, int cap) {
       (int64_t)sizeof(pf_raster_fwd_args), (int64_t)offsetof(pf_raster_fwd_args, tiles),
       (int64_t)offsetof(pf_raster_fwd_args, transmittance_threshold), (int64_t)offsetof(pf_raster_fwd_args, point_err),
       (int64_t)sizeof(pf_raster_bwd_args), (int64_t)offsetof(pf_raster_bwd_args, exact_gradients),
-      (int64_t)offsetof(pf_raster_bwd_args, grad_texel_height)};
+      (int64_t)offsetof(pf_raster_bwd_args, grad_texel_height),
+      (int64_t)sizeof(pf_raster_rgb_args), (int64_t)offsetof(pf_raster_rgb_args, color),
+      (int64_t)sizeof(pf_raster_vis_args), (int64_t)offsetof(pf_raster_vis_args, bkgd_color),
+      (int64_t)offsetof(pf_raster_vis_args, intersections),
+      (int64_t)sizeof(pf_trace_args), (int64_t)offsetof(pf_trace_args, max_steps),
+      (int64_t)offsetof(pf_trace_args, n_cells),
+      (int64_t)sizeof(pf_trace_bwd_args), (int64_t)offsetof(pf_trace_bwd_args, grad_opacity),
+      (int64_t)offsetof(pf_trace_bwd_args, grad_texel_height)};
   int n = (int)(sizeof(v) / sizeof(v[0]));
   for (int i = 0; i < n && i < cap; i++) out[i] = v[i];
   return n;

@@ -123,6 +123,23 @@ class PfRasterRgbArgs(ctypes.Structure):
     ]
 
 
+class PfRasterVisArgs(ctypes.Structure):
+    _fields_ = [
+        ("cam", PfCamera),
+        ("scene", PfScene),
+        ("tiles", PfTileLists),
+        ("transmittance_threshold", ctypes.c_float),
+        ("max_intersections", ctypes.c_int32),
+        ("depth_quantile", ctypes.c_float),
+        ("bkgd_color", ctypes.c_float * 3),
+        ("color", ctypes.c_void_p),
+        ("depth", ctypes.c_void_p),
+        ("normal", ctypes.c_void_p),
+        ("alpha", ctypes.c_void_p),
+        ("intersections", ctypes.c_void_p),
+    ]
+
+
 class PfTraceArgs(ctypes.Structure):
     _fields_ = [
         ("cam", PfCamera),
@@ -169,6 +186,7 @@ SIGNATURES = {
     "pf_raster_fwd": ([_VP, _VP], ctypes.c_int),
     "pf_raster_bwd": ([_VP, _VP], ctypes.c_int),
     "pf_raster_rgb": ([_VP, _VP], ctypes.c_int),
+    "pf_raster_vis": ([_VP, _VP], ctypes.c_int),
     "pf_trace_fwd": ([_VP, _VP], ctypes.c_int),
     "pf_trace_bwd": ([_VP, _VP], ctypes.c_int),
     "pf_scrub_nonfinite": ([_VP, ctypes.c_int64, _VP], ctypes.c_int),

@@ -885,15 +885,29 @@ struct RgbParams {
   const float4* nsigmas;
   const float* texel_sites;
   const float4* texel_rgbh;
+  const int32_t* adjacency;
   const int32_t* adj_off;
   const uint16_t* adj_diff;
   const int64_t* offsets;
   const int32_t* sites;
   float thr;
   float* color;
+  // visualisation only
+  int max_inter;
+  float depth_q;
+  f3 bkgd;
+  float* depth;
+  float* normal;
+  float* alpha;
+  int32_t* inter;
 };
 
-template <bool PINHOLE>
+// Colour-only compositing with a per-ray stop. VIS = false: the benchmark
+// kernel (rasterize.py:614-734), faces from the (c_adj - c, pm_adj - pm)
+// encoding. VIS = true: the visualisation kernel (rasterize.py:739-890), faces
+// from the training encoding (c_adj - c, r_adj), plus background, depth at
+// T = depth_q, normals, alpha and per-ray entry counts.
+template <bool PINHOLE, bool VIS>
 __global__ void __launch_bounds__(WAVE* WPB) raster_rgb_kernel(RgbParams p) {
   __shared__ SiteRec s_site_all[WPB][WAVE];
   __shared__ float4 s_face_all[WPB][WAVE];
@@ -913,6 +927,10 @@ __global__ void __launch_bounds__(WAVE* WPB) raster_rgb_kernel(RgbParams p) {
   const int T = p.n_texels;
   float log_t = 0.0f;
   f3 rgb{0.0f, 0.0f, 0.0f};
+  f3 nacc{0.0f, 0.0f, 0.0f};
+  float depth = 0.0f;
+  bool found = false;
+  int nhit = 0;
   bool alive = !ray.oob;  // out-of-image lanes are never written
 
   const int64_t start = p.offsets[tile];
@@ -931,6 +949,11 @@ __global__ void __launch_bounds__(WAVE* WPB) raster_rgb_kernel(RgbParams p) {
     wave_sync();
     for (int kk = 0; kk < cnt; kk++) {
       const float trans = fast_exp(log_t);
+      if (VIS) {
+        // the entry that stops a ray is counted (rasterize.py:790-797)
+        nhit += alive ? 1 : 0;
+        alive = alive && base + kk < p.max_inter;
+      }
       alive = alive && trans >= p.thr;
       if (ballot(alive) == 0) break;
       const SiteRec sr = s_site[kk];
@@ -947,12 +970,16 @@ __global__ void __launch_bounds__(WAVE* WPB) raster_rgb_kernel(RgbParams p) {
         const int ec = min(WAVE, sr.a1 - e0);
         wave_sync();
         if (lane < ec) {
-          const uint2 raw = *reinterpret_cast<const uint2*>(p.adj_diff + 4 * (int64_t)(e0 + lane));
-          const f3 n{half_bits_to_float((uint16_t)(raw.x & 0xffff)), half_bits_to_float((uint16_t)(raw.x >> 16)),
-                     half_bits_to_float((uint16_t)(raw.y & 0xffff))};
-          float w = half_bits_to_float((uint16_t)(raw.y >> 16));
-          if (PINHOLE) w -= dot(p.cam.eye, n);
-          s_face[lane] = make_float4(n.x, n.y, n.z, w);
+          if (VIS) {
+            s_face[lane] = face_of<PINHOLE>(p.adjacency, p.adj_diff, p.spheres, e0 + lane, sr.sph, c - p.cam.eye);
+          } else {
+            const uint2 raw = *reinterpret_cast<const uint2*>(p.adj_diff + 4 * (int64_t)(e0 + lane));
+            const f3 n{half_bits_to_float((uint16_t)(raw.x & 0xffff)), half_bits_to_float((uint16_t)(raw.x >> 16)),
+                       half_bits_to_float((uint16_t)(raw.y & 0xffff))};
+            float w = half_bits_to_float((uint16_t)(raw.y >> 16));
+            if (PINHOLE) w -= dot(p.cam.eye, n);
+            s_face[lane] = make_float4(n.x, n.y, n.z, w);
+          }
         }
         wave_sync();
         if (open) {
@@ -960,7 +987,7 @@ __global__ void __launch_bounds__(WAVE* WPB) raster_rgb_kernel(RgbParams p) {
             const float4 f = s_face[e];
             const f3 n{f.x, f.y, f.z};
             const float dp = dot(ray.d, n);
-            const float num = PINHOLE ? f.w : f.w - dot(ray.o, n);
+            const float num = PINHOLE ? f.w : (VIS ? f.w + dot(co, n) : f.w - dot(ray.o, n));
             const float t = acc_div(num, dp);
             tf = (dp >= 0.0f && t < tf) ? t : tf;
             tn = (dp < 0.0f && t > tn) ? t : tn;
@@ -1018,14 +1045,32 @@ __global__ void __launch_bounds__(WAVE* WPB) raster_rgb_kernel(RgbParams p) {
         const float dt = tf2 - tn2;
         if (dt > 0.0f) {
           const float dlt = -sigma * dt;
-          rgb = rgb + col * ((1.0f - fast_exp(dlt)) * trans);
+          const float at = (1.0f - fast_exp(dlt)) * trans;
+          rgb = rgb + col * at;
           log_t += dlt;
+          if (VIS) {
+            nacc = nacc + pn * at;
+            if (!found && fast_exp(log_t) < p.depth_q) {
+              depth = tn2 + logf(trans / p.depth_q) / sigma;
+              found = true;
+            }
+          }
         }
       }
     }
   }
   if (!ray.oob) {
     const size_t px = (size_t)ray.pi * p.cam.W + ray.pj;
+    if (VIS) {
+      const float tr = fast_exp(log_t);
+      rgb = rgb + p.bkgd * tr;
+      p.depth[px] = depth;
+      p.normal[3 * px + 0] = nacc.x;
+      p.normal[3 * px + 1] = nacc.y;
+      p.normal[3 * px + 2] = nacc.z;
+      p.alpha[px] = 1.0f - tr;
+      p.inter[px] = nhit;
+    }
     p.color[3 * px + 0] = rgb.x;
     p.color[3 * px + 1] = rgb.y;
     p.color[3 * px + 2] = rgb.z;
@@ -1190,9 +1235,54 @@ int pf_raster_rgb(const pf_raster_rgb_args* a, void* stream) {
   const int groups = (nt + WPB - 1) / WPB;
   const int grid = ((groups + 7) / 8) * 8;
   if (c.ray_maps)
-    raster_rgb_kernel<false><<<grid, WAVE * WPB, 0, as_stream(stream)>>>(p);
+    raster_rgb_kernel<false, false><<<grid, WAVE * WPB, 0, as_stream(stream)>>>(p);
   else
-    raster_rgb_kernel<true><<<grid, WAVE * WPB, 0, as_stream(stream)>>>(p);
+    raster_rgb_kernel<true, false><<<grid, WAVE * WPB, 0, as_stream(stream)>>>(p);
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+int pf_raster_vis(const pf_raster_vis_args* a, void* stream) {
+  if (!a) return set_error(PF_ERR_ARG, "pf_raster_vis: null args");
+  int st = check_scene(a->scene, "pf_raster_vis");
+  if (st) return st;
+  const pf_camera& c = a->cam;
+  if (c.width < 2 || c.height < 2) return set_error(PF_ERR_ARG, "pf_raster_vis: image must be >= 2x2");
+  const int nt = tiles_h(c) * tiles_w(c);
+  if (a->tiles.n_tiles != nt || !a->tiles.offsets)
+    return set_error(PF_ERR_ARG, "pf_raster_vis: tile lists do not match the camera");
+  if (!a->color || !a->depth || !a->normal || !a->alpha || !a->intersections)
+    return set_error(PF_ERR_ARG, "pf_raster_vis: missing outputs");
+  if (a->scene.n_sites > 0 && a->scene.adjacency_offsets && !a->scene.adjacency_diff && !a->scene.adjacency)
+    return set_error(PF_ERR_ARG, "pf_raster_vis: needs the adjacency encoding or the adjacency");
+  RgbParams p{};
+  p.cam = make_rcam(c);
+  p.n_texels = a->scene.n_texels;
+  p.n_tiles = nt;
+  p.spheres = reinterpret_cast<const float4*>(a->scene.spheres);
+  p.nsigmas = reinterpret_cast<const float4*>(a->scene.nsigmas);
+  p.texel_sites = a->scene.texel_sites;
+  p.texel_rgbh = reinterpret_cast<const float4*>(a->scene.texel_rgbh);
+  p.adjacency = a->scene.adjacency;
+  p.adj_off = a->scene.adjacency_offsets;
+  p.adj_diff = a->scene.adjacency_diff;
+  p.offsets = a->tiles.offsets;
+  p.sites = a->tiles.sites;
+  p.thr = a->transmittance_threshold;
+  p.color = a->color;
+  p.max_inter = a->max_intersections;
+  p.depth_q = a->depth_quantile;
+  p.bkgd = f3{a->bkgd_color[0], a->bkgd_color[1], a->bkgd_color[2]};
+  p.depth = a->depth;
+  p.normal = a->normal;
+  p.alpha = a->alpha;
+  p.inter = a->intersections;
+  const int groups = (nt + WPB - 1) / WPB;
+  const int grid = ((groups + 7) / 8) * 8;
+  if (c.ray_maps)
+    raster_rgb_kernel<false, true><<<grid, WAVE * WPB, 0, as_stream(stream)>>>(p);
+  else
+    raster_rgb_kernel<true, true><<<grid, WAVE * WPB, 0, as_stream(stream)>>>(p);
   PF_HIP(hipGetLastError());
   return PF_OK;
 }

@@ -23,10 +23,12 @@ LEAF_SIZE = 2
 
 @dataclass
 class VisOptions:
-    """rasterize.py:15-20"""
-    transmittance_threshold: float = 1e-3
-    max_intersections: int = 1024
-    depth_quantile: float = 0.5
+    """rasterize.py:15-20. Fields start at zero like the reference's wp.struct;
+    Rasterizer.visualize(vis_options=None) uses threshold 1e-3, 1024
+    intersections, a black background and depth quantile 0 (:2146-2150)."""
+    transmittance_threshold: float = 0.0
+    max_intersections: int = 0
+    depth_quantile: float = 0.0
     bkgd_color: tuple = field(default_factory=lambda: (0.0, 0.0, 0.0))
 
 
@@ -257,6 +259,47 @@ class Rasterizer:
             a.color = color_out.data_ptr()
             self._launch("raster_rgb", self.lib.pf_raster_rgb, _ref(a), stream_handle())
             return color_out
+
+    # ------------------------------------------------------------- visualize
+    def visualize(self, camera, points, radii, density, normals, texel_sites, texel_rgb, texel_height, adjacency,
+                  adjacency_offsets, vis_options=None):
+        """rasterize.py:2133-2311: (color [H,W,3] over vis_options.bkgd_color,
+        depth [H,W] where T first drops below depth_quantile (0 if never),
+        normal [H,W,3], alpha [H,W], intersections [H,W] int32). Attributes go
+        through fp16 when attr_dtype="half", as the reference stores them."""
+        if vis_options is None:
+            vis_options = VisOptions(transmittance_threshold=1e-3, max_intersections=1024)
+        with torch.no_grad():
+            H, W = camera.height, camera.width
+            dev = points.device
+            q = (lambda t: t.to(self.tscalar).to(torch.float32).contiguous())
+            all_spheres = _f32c(torch.cat([points, radii[:, None]], dim=-1))
+            all_nsigmas = q(torch.cat([normals, density[:, None]], dim=-1))
+            all_texel_sites = q(texel_sites)
+            all_texel_rgbh = torch.cat([q(texel_rgb), q(texel_height)[..., None]], dim=-1).contiguous()
+            adjacency = adjacency.to(torch.int32).contiguous()
+            adjacency_offsets = adjacency_offsets.to(torch.int32).contiguous()
+            offsets, tile_sites, _, nt = self.bin(camera, all_spheres)
+            adjacency_diff = self.encode_adjacency(all_spheres, adjacency, adjacency_offsets)
+            f32 = dict(dtype=torch.float32, device=dev)
+            color = torch.zeros((H, W, 3), **f32)
+            depth = torch.zeros((H, W), **f32)
+            normal = torch.zeros((H, W, 3), **f32)
+            alpha = torch.zeros((H, W), **f32)
+            inter = torch.zeros((H, W), dtype=torch.int32, device=dev)
+            a = _abi.PfRasterVisArgs()
+            a.cam, ray_maps = self._camera(camera)
+            a.scene = _scene_struct(all_spheres, all_nsigmas, all_texel_sites, all_texel_rgbh, adjacency,
+                                    adjacency_offsets, adjacency_diff)
+            a.tiles = _abi.PfTileLists(nt, offsets.data_ptr(), tile_sites.data_ptr() if tile_sites.numel() else None)
+            a.transmittance_threshold = float(vis_options.transmittance_threshold)
+            a.max_intersections = int(vis_options.max_intersections)
+            a.depth_quantile = float(vis_options.depth_quantile)
+            a.bkgd_color[:] = [float(v) for v in vis_options.bkgd_color]
+            a.color, a.depth, a.normal = color.data_ptr(), depth.data_ptr(), normal.data_ptr()
+            a.alpha, a.intersections = alpha.data_ptr(), inter.data_ptr()
+            self._launch("raster_vis", self.lib.pf_raster_vis, _ref(a), stream_handle())
+            return color, depth, normal, alpha, inter
 
     # ----------------------------------------------------------------- backward
     def _backward(self, grad_ctx, grad_color_in, grad_opacity_in, grad_normal_distance_in, grad_normal_in,

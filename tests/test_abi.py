@@ -55,13 +55,18 @@ def test_struct_layout_matches_c():
     from oracle import orc
     lib = orc.load()
     lib.orc_abi_layout.argtypes = [ctypes.c_void_p, ctypes.c_int]
-    out = np.zeros(32, np.int64)
-    n = lib.orc_abi_layout(out.ctypes.data, 32)
+    out = np.zeros(64, np.int64)
+    n = lib.orc_abi_layout(out.ctypes.data, 64)
     C, S, TL, F, B = _abi.PfCamera, _abi.PfScene, _abi.PfTileLists, _abi.PfRasterFwdArgs, _abi.PfRasterBwdArgs
+    RG, V, TA, TB = _abi.PfRasterRgbArgs, _abi.PfRasterVisArgs, _abi.PfTraceArgs, _abi.PfTraceBwdArgs
     expect = [ctypes.sizeof(C), C.ray_maps.offset, C.fov_cos_cutoff.offset,
               ctypes.sizeof(S), S.adjacency_diff.offset,
               ctypes.sizeof(TL), TL.sites.offset,
               ctypes.sizeof(F), F.tiles.offset, F.transmittance_threshold.offset, F.point_err.offset,
-              ctypes.sizeof(B), B.exact_gradients.offset, B.grad_texel_height.offset]
+              ctypes.sizeof(B), B.exact_gradients.offset, B.grad_texel_height.offset,
+              ctypes.sizeof(RG), RG.color.offset,
+              ctypes.sizeof(V), V.bkgd_color.offset, V.intersections.offset,
+              ctypes.sizeof(TA), TA.max_steps.offset, TA.n_cells.offset,
+              ctypes.sizeof(TB), TB.grad_opacity.offset, TB.grad_texel_height.offset]
     assert n == len(expect)
     assert out[:n].tolist() == expect

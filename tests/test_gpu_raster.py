@@ -81,3 +81,40 @@ def test_benchmark_matches_fp64_oracle(dev, cfg, attr):
     err = np.abs(col - ref).max()
     assert err < ABS_TOL, err
     assert ref.max() > 0.1
+
+
+@pytest.mark.parametrize("cfg,opts", [
+    (CASES[0], dict(transmittance_threshold=1e-3, max_intersections=1024, depth_quantile=0.5,
+                    bkgd_color=(0.1, 0.7, 0.3))),
+    (CASES[1], dict(transmittance_threshold=1e-2, max_intersections=7, depth_quantile=0.8, bkgd_color=(1, 1, 1))),
+    (CASES[5], dict(transmittance_threshold=1e-3, max_intersections=1024, depth_quantile=0.9,
+                    bkgd_color=(0, 0, 0))),
+], ids=["case0", "case1-capped", "case5-fisheye"])
+def test_visualize_matches_fp64_oracle(dev, cfg, opts):
+    """Rasterizer.visualize (visualization_kernel): colour over background,
+    depth, normals, alpha within 1e-4; entry counts exact."""
+    import torch
+    from powerfoam_amd.camera import TorchCamera
+    from powerfoam_amd.rasterize import Rasterizer, VisOptions
+    from oracle import orc
+    case = harness.small_case(**cfg)
+    print(f"visualize case {cfg} {opts}", flush=True)
+    T = case.texel_sites.shape[1]
+    r = Rasterizer(harness._args(T, case.cam.ray_maps is None), dev)
+    tt = lambda a, dt=torch.float32: torch.tensor(np.ascontiguousarray(a), dtype=dt, device=dev)
+    cam = TorchCamera(case.cam.eye, case.cam.right, case.cam.up, case.width, case.height,
+                      None if case.cam.ray_maps is None else case.cam.ray_maps.to(dev))
+    out = r.visualize(cam, tt(case.points), tt(case.radii), tt(case.density), tt(case.normals),
+                      tt(case.texel_sites), tt(case.texel_rgb), tt(case.texel_height),
+                      tt(case.adjacency, torch.int32), tt(case.adjacency_offsets, torch.int32), VisOptions(**opts))
+    color, depth, normal, alpha, inter = [t.cpu().numpy() for t in out]
+    ocam, _k = case.pf_camera()
+    offsets, sites, _ = orc.bin_lists(ocam, case.spheres())
+    ref = orc.raster_vis(ocam, harness.oracle_inputs(case), offsets, sites, threshold=opts["transmittance_threshold"],
+                         max_intersections=opts["max_intersections"], depth_quantile=opts["depth_quantile"],
+                         bkgd_color=opts["bkgd_color"])
+    for k, v in (("color", color), ("depth", depth), ("normal", normal), ("alpha", alpha)):
+        err = np.abs(v - ref[k]).max()
+        assert err < ABS_TOL, (k, err)
+    assert np.array_equal(inter, ref["intersections"])
+    assert (ref["depth"] != 0).mean() > 0.01

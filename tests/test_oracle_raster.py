@@ -313,3 +313,36 @@ def test_benchmark_per_ray_stop():
     assert np.all(cut <= full + 1e-12)
     assert np.all(full - cut <= thr + 1e-12)
     assert np.any(full - cut > 1e-3)  # the stop happens somewhere
+
+
+def test_visualize_reduces_to_the_pinned_forward():
+    """orc_raster_vis (rasterize.py:739-890) with no early stop equals the
+    training forward (pinned above by the exact renderer): colour, normals,
+    opacity and the depth quantile; the background adds bkgd * T; the
+    intersection count is the tile list length, or max + 1 when capped."""
+    from tests import harness
+    case = harness.small_case(seed=4, n=600, width=40, height=33, quantiles=[0.5], height_std=0.05)
+    cam, _k = case.pf_camera()
+    sp = case.spheres()
+    offsets, sites, _ = orc.bin_lists(cam, sp)
+    inp = harness.oracle_inputs(case)
+    fwd = orc.raster_fwd(cam, inp, offsets, sites, depth_quantiles=case.depth_quantiles, ray_gt=case.gt,
+                         threshold=0.0)
+    vis = orc.raster_vis(cam, inp, offsets, sites, threshold=0.0, max_intersections=1 << 30, depth_quantile=0.5)
+    np.testing.assert_allclose(vis["color"], fwd["color"], atol=1e-12)
+    np.testing.assert_allclose(vis["normal"], fwd["normal"], atol=1e-12)
+    np.testing.assert_allclose(vis["alpha"], 1 - np.exp(fwd["log_t"]), atol=1e-12)
+    q = fwd["quantile_depths"][..., 0]
+    np.testing.assert_allclose(vis["depth"], np.where(q == -1, 0.0, q), atol=1e-12)
+    assert (q != -1).mean() > 0.01
+    H, W = case.height, case.width
+    th = 1 + (H - 1) // 8
+    ii, jj = np.meshgrid(np.arange(H), np.arange(W), indexing="ij")
+    lens = np.diff(offsets)[(ii // 8) + (jj // 8) * th]
+    assert np.array_equal(vis["intersections"], lens)
+    bk = (0.2, 0.5, 0.9)
+    vb = orc.raster_vis(cam, inp, offsets, sites, threshold=0.0, max_intersections=1 << 30, bkgd_color=bk)
+    np.testing.assert_allclose(vb["color"] - vis["color"], np.asarray(bk) * (1 - vis["alpha"])[..., None],
+                               atol=1e-7)
+    cap = orc.raster_vis(cam, inp, offsets, sites, threshold=0.0, max_intersections=3)
+    assert np.array_equal(cap["intersections"], np.minimum(lens, 4))
