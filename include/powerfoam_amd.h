@@ -296,6 +296,56 @@ int pf_raster_vis(const pf_raster_vis_args* args, void* stream);
 int pf_trace_fwd(const pf_trace_args* args, void* stream);
 int pf_trace_bwd(const pf_trace_bwd_args* args, void* stream);
 
+/* ---- learned colour head (no reference counterpart) -------------------
+ * A shared MLP fills the texel-colour slot that the SH head (pf_sh_fwd) and
+ * the reference's SphericalVoronoi pre-pass (color_fn.py:54-184) fill: for a
+ * row of site s seen along unit direction d,
+ *   x = [z_s (16) ; 1, Y_1(d) .. Y_15(d)]   (Y: real SH basis, sh_utils.py:4-49)
+ *   rgb = max(0, W3 relu(W2 relu(W1 x + b1) + b2) + b3).
+ * W1 [64,32], W2 [64,64], W3 [3,64] and the latents z [N,16] are bf16
+ * (row-major, 16-byte aligned); biases fp32. x, h1, h2 are rounded to bf16
+ * before each product; products accumulate in fp32. */
+typedef struct pf_mlp_weights {
+  const uint16_t* w1;
+  const float* b1;
+  const uint16_t* w2;
+  const float* b2;
+  const uint16_t* w3;
+  const float* b3;
+} pf_mlp_weights;
+
+/* Rows: texel mode (pair_sites NULL): row r is texel r of site r / n_texels,
+ * d = normalize(texel_sites[r] - eye) (color_fn.py:72). Pair mode: row r is
+ * site pair_sites[r] along pair_dirs[r] (training pairs). */
+typedef struct pf_mlp_rows {
+  int32_t n_rows;
+  int32_t n_sites;   /* latents rows; pair rows naming other sites are skipped */
+  int32_t n_texels;
+  const float* texel_sites; /* [n_rows,3] */
+  float eye[3];
+  const int32_t* pair_sites; /* [n_rows] or NULL */
+  const float* pair_dirs;    /* [n_rows,3] */
+  const uint16_t* latents;   /* [N,16] bf16 */
+} pf_mlp_rows;
+
+/* fp32 gradient buffers, accumulated into (zero them first). */
+typedef struct pf_mlp_grads {
+  float* latents; /* [N,16] */
+  float* w1;
+  float* b1;
+  float* w2;
+  float* b2;
+  float* w3;
+  float* b3;
+} pf_mlp_grads;
+
+/* rgb [n_rows,3]. */
+int pf_mlp_fwd(const pf_mlp_weights* w, const pf_mlp_rows* rows, float* rgb, void* stream);
+/* Adjoint of pf_mlp_fwd for dL/d rgb = grad_rgb [n_rows,3]; no gradient to
+ * directions or texel positions (as color_fn.py:181-184). */
+int pf_mlp_bwd(const pf_mlp_weights* w, const pf_mlp_rows* rows, const float* grad_rgb, pf_mlp_grads* g,
+               void* stream);
+
 /* Replaces non-finite entries with 0 (rasterize.py:1954-1958). */
 int pf_scrub_nonfinite(float* data, int64_t n, void* stream);
 

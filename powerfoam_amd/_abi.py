@@ -140,6 +140,27 @@ class PfRasterVisArgs(ctypes.Structure):
     ]
 
 
+class PfMlpWeights(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_void_p) for k in ("w1", "b1", "w2", "b2", "w3", "b3")]
+
+
+class PfMlpRows(ctypes.Structure):
+    _fields_ = [
+        ("n_rows", ctypes.c_int32),
+        ("n_sites", ctypes.c_int32),
+        ("n_texels", ctypes.c_int32),
+        ("texel_sites", ctypes.c_void_p),
+        ("eye", ctypes.c_float * 3),
+        ("pair_sites", ctypes.c_void_p),
+        ("pair_dirs", ctypes.c_void_p),
+        ("latents", ctypes.c_void_p),
+    ]
+
+
+class PfMlpGrads(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_void_p) for k in ("latents", "w1", "b1", "w2", "b2", "w3", "b3")]
+
+
 class PfTraceArgs(ctypes.Structure):
     _fields_ = [
         ("cam", PfCamera),
@@ -187,6 +208,8 @@ SIGNATURES = {
     "pf_raster_bwd": ([_VP, _VP], ctypes.c_int),
     "pf_raster_rgb": ([_VP, _VP], ctypes.c_int),
     "pf_raster_vis": ([_VP, _VP], ctypes.c_int),
+    "pf_mlp_fwd": ([_VP, _VP, _VP, _VP], ctypes.c_int),
+    "pf_mlp_bwd": ([_VP, _VP, _VP, _VP, _VP], ctypes.c_int),
     "pf_trace_fwd": ([_VP, _VP], ctypes.c_int),
     "pf_trace_bwd": ([_VP, _VP], ctypes.c_int),
     "pf_scrub_nonfinite": ([_VP, ctypes.c_int64, _VP], ctypes.c_int),

@@ -46,6 +46,16 @@ __device__ __forceinline__ float half_bits_to_float(uint16_t h) {
   return static_cast<float>(__builtin_bit_cast(_Float16, h));
 }
 
+constexpr int WAVE = 64;
+
+// Orders this wave's LDS traffic: LDS operations of one wave execute in
+// program order, so only the compiler has to be kept from moving them.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ------------------------------------------------------- wave64 reductions --
 // Full-wave sum, every lane must be active; the result is uniform.
 __device__ __forceinline__ float wave_sum(float v) {

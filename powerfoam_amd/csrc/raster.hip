@@ -39,18 +39,10 @@
 
 namespace pf {
 
-constexpr int WAVE = 64;
 #ifndef PF_WPB
 #define PF_WPB 4
 #endif
 constexpr int WPB = PF_WPB;  // tiles (waves) per workgroup
-// Orders this wave's LDS traffic: LDS operations of one wave execute in
-// program order, so only the compiler has to be kept from moving them.
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 struct RasterCam {
   f3 eye, right, up, fwd;  // fwd = normalize(up x right)

@@ -1,0 +1,118 @@
+"""GPU parity of the learned colour head (pf_mlp_fwd / pf_mlp_bwd) against the
+oracle with the kernels' bf16 rounding points, and what training buys.
+
+Tolerances: the kernels and the oracle round the same values to bf16, but
+accumulate in FP32 vs FP64, so an activation within an FP32 rounding error of
+a bf16 tie can round the other way (one bf16 ulp = 2^-8 relative). Outputs:
+max abs 5e-3, mean abs 5e-5. Gradients: ||gpu - oracle|| / ||oracle|| < 1e-3
+per tensor.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import mlp as om
+from powerfoam_amd import synth
+from tests import harness
+
+pytestmark = pytest.mark.gpu
+
+
+def _head(n, seed):
+    import torch
+    from powerfoam_amd.mlp_head import MLPColor
+    rng = np.random.default_rng(seed)
+    head = MLPColor(n, "cuda", seed=seed, latents=rng.normal(0, 0.5, (n, 16)))
+    with torch.no_grad():
+        head.params["b1"].copy_(torch.tensor(rng.normal(0, 0.1, 64)))
+        head.params["b2"].copy_(torch.tensor(rng.normal(0, 0.1, 64)))
+        head.params["b3"].copy_(torch.tensor(rng.normal(0.4, 0.1, 3)))
+    return head
+
+
+def _np_params(head):
+    return {k: v.detach().cpu().numpy().astype(np.float64) for k, v in head.params.items()}
+
+
+def test_texel_forward_and_backward_match_oracle():
+    import torch
+    from powerfoam_amd.camera import look_at
+    n, T = 3001, 8  # ragged: the last 32-row step is partial
+    s = synth.scene_uniform(11, n=n, n_texels=T)
+    head = _head(n, 1)
+    cam = look_at((0.3, 0.2, 2.5), (0.0, 0.0, 0.0), 64, 48, 60.0)
+    ts = torch.tensor(s["texel_sites"], device="cuda")
+    print("mlp texel fwd/bwd n=3001 T=8", flush=True)
+    rgb = head.forward(ts, cam)
+    sites, dirs = om.rows_texels(s["texel_sites"], cam.eye.numpy(), T)
+    p = _np_params(head)
+    ref, _ = om.forward(p, sites, dirs)
+    got = rgb.detach().cpu().numpy().reshape(-1, 3)
+    err = np.abs(got - ref)
+    print("fwd max", err.max(), "mean", err.mean(), flush=True)
+    assert err.max() < 5e-3 and err.mean() < 5e-5
+    assert (ref > 0).mean() > 0.5
+    g = np.random.default_rng(2).normal(size=got.shape).astype(np.float32)
+    (rgb.view(-1, 3) * torch.tensor(g, device="cuda")).sum().backward()
+    want = om.backward(p, sites, dirs, g)
+    errs = {k: harness.rel_err(v.grad.cpu().numpy(), want[k]) for k, v in head.params.items()}
+    print(errs, flush=True)
+    assert max(errs.values()) < 1e-3, errs
+
+
+def test_pair_forward_and_backward_match_oracle():
+    import torch
+    n, B = 500, 5000
+    head = _head(n, 3)
+    rng = np.random.default_rng(4)
+    sites = rng.integers(0, n, B).astype(np.int32)
+    d = rng.normal(size=(B, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    print("mlp pair fwd/bwd", flush=True)
+    rgb = head.forward_pairs(torch.tensor(sites, device="cuda"), torch.tensor(d, dtype=torch.float32, device="cuda"))
+    p = _np_params(head)
+    ref, _ = om.forward(p, sites, d.astype(np.float32).astype(np.float64))
+    err = np.abs(rgb.detach().cpu().numpy() - ref)
+    assert err.max() < 5e-3 and err.mean() < 5e-5, (err.max(), err.mean())
+    g = rng.normal(size=ref.shape).astype(np.float32)
+    (rgb * torch.tensor(g, device="cuda")).sum().backward()
+    want = om.backward(p, sites, d.astype(np.float32).astype(np.float64), g)
+    errs = {k: harness.rel_err(v.grad.cpu().numpy(), want[k]) for k, v in head.params.items()}
+    print(errs, flush=True)
+    assert max(errs.values()) < 1e-3, errs
+
+
+def test_training_beats_the_view_independent_colour():
+    """Trained on SH-labelled pairs with the project's kernels, the head's
+    colours on held-out directions are closer to the SH head's than the
+    method without it (each site's view-independent DC colour) and than the
+    untrained head.
+
+    What a 16-dim latent can reach here: the synthetic SH has 45 i.i.d.
+    N(0, 0.05^2) higher-band coefficients per site, each adding 0.0025/(4 pi)
+    to the per-channel MSE when unknown, so the DC colour alone gives about
+    25.3 dB and a latent carrying the DC plus 13 coefficient combinations at
+    best (45 - 13)/45 of that MSE, about 26.7 dB."""
+    import torch
+    from powerfoam_amd.mlp_head import MLPColor, fit, latents_from_sh, psnr, sh_pair_colors
+    n = 20000
+    s = synth.scene_uniform(5, n=n)
+    sh = torch.tensor(s["sh"], device="cuda")
+    head = MLPColor(n, "cuda", seed=0, latents=latents_from_sh(sh))
+    g = torch.Generator(device="cuda").manual_seed(123)
+    sites = torch.randint(0, n, (200000,), device="cuda", generator=g).to(torch.int32)
+    d = torch.randn((200000, 3), device="cuda", generator=g)
+    d = d / d.norm(dim=1, keepdim=True)
+    target = sh_pair_colors(sh, sites, d)
+    with torch.no_grad():
+        before = psnr(head.forward_pairs(sites, d), target)
+    dc = torch.clamp(sh[sites.long(), 0, :], min=0.0)
+    p_dc = psnr(dc, target)
+    print("training 1500 steps", flush=True)
+    hist = fit(head, sh, steps=1500, batch=32768, log_every=250)
+    with torch.no_grad():
+        after = psnr(head.forward_pairs(sites, d), target)
+    print("psnr untrained", before, "dc-only", p_dc, "trained", after, "loss", hist, flush=True)
+    assert hist[-1][1] < 0.2 * hist[0][1]
+    assert after > p_dc + 0.5
+    assert after > before + 10.0
