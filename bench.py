@@ -40,12 +40,12 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def build_workload(n_sites, n_views, width, height, dev):
+def build_workload(n_sites, n_views, width, height, dev, seed=1):
     import torch
     from powerfoam_amd import synth
 
     t0 = time.time()
-    s = synth.scene_gmm(1, n=n_sites)
+    s = synth.scene_gmm(seed, n=n_sites)
     log(f"scene: {n_sites} sites in {time.time() - t0:.1f}s")
     t0 = time.time()
     adj, off, stats = synth.power_adjacency(s["points"], s["radii"], return_stats=True)
@@ -73,6 +73,104 @@ def frame(rast, head, w, cam):
     loss = torch.mean((out[0] - w["target"]) ** 2)
     loss.backward()
     return out
+
+
+class MlpSlot:
+    """The learned head in the colour-head slot of `frame` (same call shape as SHColor)."""
+
+    def __init__(self, mlp):
+        self.mlp = mlp
+
+    def forward(self, texel_sites, camera, sh):
+        return self.mlp.forward(texel_sites, camera)
+
+
+def time_frames(rast, head, w, steps, heads_log=None):
+    """Wall time of `steps` frames (synchronised on both sides) and the mean
+    kernel times the rasterizer (and an MLP head) recorded with HIP events."""
+    import torch
+    cams = w["cams"]
+    rast.event_log = []
+    if heads_log is not None:
+        heads_log.event_log = rast.event_log
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        frame(rast, head, w, cams[i % len(cams)])
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ev = rast.event_log
+    rast.event_log = None
+    if heads_log is not None:
+        heads_log.event_log = None
+    kern = {}
+    for name, s, e in ev:
+        kern.setdefault(name, []).append(s.elapsed_time(e))
+    return elapsed, {k: float(np.mean(v)) for k, v in kern.items()}
+
+
+def render_psnr(rast, w, head_a, head_b):
+    """Mean over views of the PSNR of head_a's render against head_b's (forward only)."""
+    import torch
+    from powerfoam_amd.mlp_head import psnr
+    p = w["params"]
+    vals = []
+    with torch.no_grad():
+        for cam in w["cams"]:
+            imgs = []
+            for h in (head_a, head_b):
+                rgb = h.forward(p["texel_sites"], cam, p["sh"]).detach()
+                out = rast.forward(cam, None, p["points"], p["radii"], p["density"], p["normals"], p["texel_sites"],
+                                   rgb, p["texel_height"], w["adjacency"], w["adjacency_offsets"], w["target"], False)
+                imgs.append(out[0])
+            vals.append(psnr(imgs[0], imgs[1]))
+    return float(np.mean(vals))
+
+
+def run_mlp_head(args, rast, w, dev):
+    """Learned colour head workload: train on SH-labelled pairs, then time
+    fwd+bwd frames with the MLP head against the SH head on the same scene and
+    report the MLP render's PSNR against the SH render."""
+    import torch
+    from powerfoam_amd.color_fn import SHColor
+    from powerfoam_amd.mlp_head import MLPColor, fit, latents_from_sh
+
+    sh = w["params"]["sh"].detach()
+    mlp = MLPColor(sh.shape[0], dev, seed=0, latents=latents_from_sh(sh))
+    t0 = time.time()
+    hist = fit(mlp, sh, steps=args.mlp_steps, batch=args.mlp_batch, log_every=max(1, args.mlp_steps // 8))
+    torch.cuda.synchronize()
+    train_s = time.time() - t0
+    log(f"trained the MLP head: {args.mlp_steps} steps x {args.mlp_batch} pairs in {train_s:.1f}s, loss {hist}")
+    sh_head, mlp_head = SHColor(), MlpSlot(mlp)
+    psnr_mlp = render_psnr(rast, w, mlp_head, sh_head)
+
+    class _DC:
+        def forward(self, texel_sites, camera, sh_):
+            n, T = texel_sites.shape[:2]
+            return torch.clamp(sh_[:, 0, :].detach(), min=0.0)[:, None, :].expand(n, T, 3).contiguous()
+
+    psnr_dc = render_psnr(rast, w, _DC(), sh_head)
+    log(f"render PSNR vs the SH render: MLP {psnr_mlp:.2f} dB, view-independent DC {psnr_dc:.2f} dB")
+    for i in range(args.warmup):
+        frame(rast, sh_head, w, w["cams"][i % len(w["cams"])])
+        frame(rast, mlp_head, w, w["cams"][i % len(w["cams"])])
+    sh_s, sh_k = time_frames(rast, sh_head, w, args.steps)
+    mlp_s, mlp_k = time_frames(rast, mlp_head, w, args.steps, heads_log=mlp)
+    P = args.width * args.height
+    ms = mlp_s * 1e3 / args.steps
+    line = dict(metric=METRIC, value=P * args.steps / mlp_s / 1e6, unit="Mrays/s", n_gpus=1, steps=args.steps,
+                warmup=args.warmup, ms_per_step=ms, higher_is_better=True, scaling="weak", vs_baseline=None,
+                dtype="bf16 head / f32 geometry", data="synthetic (seeded scene generator, random target image)",
+                config=dict(workload=f"rasterizer fwd+bwd with the learned MLP colour head vs the SH-3 head, "
+                                     f"{args.sites}-site GMM scene (seed 2), {args.views} pinhole views "
+                                     f"{args.width}x{args.height}",
+                            sites=args.sites, width=args.width, height=args.height, views=args.views,
+                            training_pairs=args.mlp_steps * args.mlp_batch),
+                sh_head_ms_per_step=sh_s * 1e3 / args.steps, speedup_vs_sh_head=sh_s / mlp_s,
+                psnr_vs_sh_render_db=psnr_mlp, psnr_dc_only_vs_sh_render_db=psnr_dc,
+                kernel_ms_mlp=mlp_k, kernel_ms_sh=sh_k, train_seconds=train_s, train_loss=hist)
+    print(json.dumps(line), flush=True)
 
 
 def workload_counts(rast, w, width, height):
@@ -152,7 +250,13 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run 3 device copies of this many MiB (counter calibration)")
+    ap.add_argument("--head", choices=("sh", "mlp"), default="sh",
+                    help="mlp: the learned-head workload (3M-site scene, MLP vs SH head, PSNR)")
+    ap.add_argument("--mlp-steps", type=int, default=2000)
+    ap.add_argument("--mlp-batch", type=int, default=65536)
     args = ap.parse_args()
+    if args.head == "mlp" and args.sites == 1_000_000:
+        args.sites = 3_000_000
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: the renderer hot path is built and timed on a single MI355X; "
@@ -168,9 +272,11 @@ def main():
     from powerfoam_amd.rasterize import Rasterizer
 
     dev = torch.device("cuda:0")
-    w = build_workload(args.sites, args.views, args.width, args.height, dev)
+    w = build_workload(args.sites, args.views, args.width, args.height, dev, seed=2 if args.head == "mlp" else 1)
     rast = Rasterizer(SimpleNamespace(num_texel_sites=8, render_objective="volume", is_pinhole=True,
                                       disable_coop_prim_load=False, disable_coop_adj_load=False), dev)
+    if args.head == "mlp":
+        return run_mlp_head(args, rast, w, dev)
     head = SHColor()
     cams = w["cams"]
 

@@ -415,7 +415,6 @@ struct BwdParams {
 
 constexpr int RED_SLICE = 16;      // values reduced per LDS pass
 constexpr int RED_W = RED_SLICE + 1;  // odd row stride: conflict-free
-constexpr int NVAL = 8 + 7 * PF_MAX_TEXELS;  // sphere 4, nsigma 4, 7 per texel
 
 // Destination of reduced value v of a site (v < 8 + 7T).
 __device__ __forceinline__ float* grad_slot(const BwdParams& p, int site, int T, int v) {
@@ -428,7 +427,12 @@ __device__ __forceinline__ float* grad_slot(const BwdParams& p, int site, int T,
 }
 
 template <bool PINHOLE>
-__global__ void __launch_bounds__(WAVE* WPB) raster_bwd_kernel(BwdParams p) {
+#ifdef PF_BWD_WAVES
+#define PF_BWD_OCC __attribute__((amdgpu_waves_per_eu(PF_BWD_WAVES)))
+#else
+#define PF_BWD_OCC
+#endif
+__global__ void __launch_bounds__(WAVE* WPB) PF_BWD_OCC raster_bwd_kernel(BwdParams p) {
   __shared__ SiteRec s_site_all[WPB][WAVE];
   __shared__ float4 s_face_all[WPB][WAVE];
   __shared__ TexelLds s_tex_all[WPB][PF_MAX_TEXELS];
@@ -450,7 +454,6 @@ __global__ void __launch_bounds__(WAVE* WPB) raster_bwd_kernel(BwdParams p) {
   const Ray ray = lane_ray<PINHOLE>(p.cam, tile, lane);
   const size_t px = (size_t)ray.pi * p.cam.W + ray.pj;
   const int T = p.n_texels;
-  const int nval = 8 + 7 * T;
   const bool exact = p.exact != 0;
   const float pl_eps = exact ? 0.0f : 1e-6f;
   const float cut = exact ? 0.0f : 1e-3f;
@@ -596,11 +599,10 @@ __global__ void __launch_bounds__(WAVE* WPB) raster_bwd_kernel(BwdParams p) {
       // ---- compositing adjoint (rasterize.py:1366-1430)
       float dLdtn = 0.0f, dLdtf = 0.0f, dLdr = 0.0f, dLdsig = 0.0f;
       f3 dLdc{0.0f, 0.0f, 0.0f}, dLdnorm{0.0f, 0.0f, 0.0f};
-      float g_tex[PF_MAX_TEXELS][7];
-#pragma unroll
-      for (int i = 0; i < PF_MAX_TEXELS; i++)
-#pragma unroll
-        for (int q = 0; q < 7; q++) g_tex[i][q] = 0.0f;
+      // what the per-texel gradient pass needs (zero on inactive lanes)
+      f3 dLdcol{0.0f, 0.0f, 0.0f}, x1{0.0f, 0.0f, 0.0f}, x0{0.0f, 0.0f, 0.0f};
+      float gdp_in = 0.0f, gdp0_in = 0.0f, dLdheight = 0.0f, inv_ws = 0.0f, inv_ws0 = 0.0f;
+      const float kw = 20.0f * fast_rcp(r * r);
       int fid0 = -1, fid1 = -1;
       float4 fg0 = make_float4(0, 0, 0, 0), fg1 = make_float4(0, 0, 0, 0);
       if (active) {
@@ -624,7 +626,7 @@ __global__ void __launch_bounds__(WAVE* WPB) raster_bwd_kernel(BwdParams p) {
         if (ndv > 0.0f) dLdtrans += g_nd * ndt * alpha;
         dLdtrans += gc * alpha * p.inv_hw;
         dLdtrans += ndot * alpha;
-        const f3 dLdcol = g_rgb * (trans * alpha);
+        dLdcol = g_rgb * (trans * alpha);
         dLdlog_t_i += dLdtrans * trans;
         dLddlt += -dLdalpha * e_dlt;
         dLdsig += dLddlt * -dt;
@@ -644,34 +646,25 @@ __global__ void __launch_bounds__(WAVE* WPB) raster_bwd_kernel(BwdParams p) {
 
         // ---- plane_intersection_bwd (texture.py:74-186)
         const float dLdts_in = (tn_id == n_adj) ? dLdtn : ((tf_id == n_adj) ? dLdtf : 0.0f);
-        const float kw = 20.0f * fast_rcp(r * r);
-        const float inv_ws = fast_rcp(ws);
-        const float inv_ws0 = fast_rcp(ws0);
-        float dLdheight = 0.0f;
+        inv_ws = fast_rcp(ws);
+        inv_ws0 = fast_rcp(ws0);
         f3 pdc{0.0f, 0.0f, 0.0f}, pdn{0.0f, 0.0f, 0.0f};
-        // colour stage
+        // colour stage: the query-point gradient (the per-texel values are
+        // recomputed in the texel pass below)
         const float tq = dpn >= 0.0f ? tn0 : sel_max(tn0, ts1);
-        const f3 x1 = ray.d * tq - co;
+        x1 = ray.d * tq - co;
         f3 dLdx{0.0f, 0.0f, 0.0f};
-        const float gdp_in = dot(dLdcol, col);
+        gdp_in = dot(dLdcol, col);
 #pragma unroll
         for (int i = 0; i < PF_MAX_TEXELS; i++) {
           if (i < T) {
-            const float w = w1[i];
-            const float wn = w * inv_ws;
+            const float wn = w1[i] * inv_ws;
             if (!(wn < cut)) {
               const float4 ph = s_tex[i].pos_h;
               const float4 cl = s_tex[i].rgb;
               const f3 df{x1.x - ph.x, x1.y - ph.y, x1.z - ph.z};
-              g_tex[i][3] += dLdcol.x * wn;
-              g_tex[i][4] += dLdcol.y * wn;
-              g_tex[i][5] += dLdcol.z * wn;
               const float gdp = dLdcol.x * cl.x + dLdcol.y * cl.y + dLdcol.z * cl.z;
-              const f3 sg = df * (kw * wn * (gdp - gdp_in));
-              g_tex[i][0] += sg.x;
-              g_tex[i][1] += sg.y;
-              g_tex[i][2] += sg.z;
-              dLdx = dLdx - sg;
+              dLdx = dLdx - df * (kw * wn * (gdp - gdp_in));
             }
           }
         }
@@ -688,9 +681,9 @@ __global__ void __launch_bounds__(WAVE* WPB) raster_bwd_kernel(BwdParams p) {
         }
         // height stage
         const float tq0 = dpn >= 0.0f ? tn0 : sel_max(tn0, ts0);
-        const f3 x0 = ray.d * tq0 - co;
+        x0 = ray.d * tq0 - co;
         f3 dLdx0{0.0f, 0.0f, 0.0f};
-        const float gdp0_in = dLdheight * height;
+        gdp0_in = dLdheight * height;
 #pragma unroll
         for (int i = 0; i < PF_MAX_TEXELS; i++) {
           if (i < T) {
@@ -699,15 +692,7 @@ __global__ void __launch_bounds__(WAVE* WPB) raster_bwd_kernel(BwdParams p) {
             // heights all zero: the height stage queries the colour-stage point
             const float w = heights ? fast_exp(dot(df, df) * kexp) : w1[i];
             // the cut-off tests the colour-stage weight sum, as the reference
-            if (!(w * inv_ws < cut)) {
-              const float wn0 = w * inv_ws0;
-              g_tex[i][6] += wn0 * dLdheight;
-              const f3 sg = df * (kw * wn0 * (dLdheight * ph.w - gdp0_in));
-              g_tex[i][0] += sg.x;
-              g_tex[i][1] += sg.y;
-              g_tex[i][2] += sg.z;
-              dLdx0 = dLdx0 - sg;
-            }
+            if (!(w * inv_ws < cut)) dLdx0 = dLdx0 - df * (kw * (w * inv_ws0) * (dLdheight * ph.w - gdp0_in));
           }
         }
         const float dLdtq0 = dot(dLdx0, ray.d);
@@ -824,40 +809,71 @@ __global__ void __launch_bounds__(WAVE* WPB) raster_bwd_kernel(BwdParams p) {
         }
       }
 
-      // ---- site and texel gradients: LDS transpose in 16-value slices; lane
-      // L sums column L%16 over rows 16*(L/16)..+15, two shuffles finish the
-      // column, lanes 0..15 issue one atomic each
-      // value v of the site: 0-3 sphere, 4-7 nsigma, then 7 per texel
-      auto val = [&](int v) -> float {
-        switch (v) {
-          case 0: return dLdc.x;
-          case 1: return dLdc.y;
-          case 2: return dLdc.z;
-          case 3: return dLdr;
-          case 4: return dLdnorm.x;
-          case 5: return dLdnorm.y;
-          case 6: return dLdnorm.z;
-          case 7: return dLdsig;
-          default: return g_tex[(v - 8) / 7][(v - 8) % 7];
-        }
+      // ---- texel and site gradients: summed over the wave by an LDS
+      // transpose in 16-value slices (lane L sums column L%16 over rows
+      // 16*(L/16)..+15, two shuffles finish the column, one atomic per value).
+      // Texel values are produced two texels (14 values) per slice, so no
+      // per-texel accumulators stay live across the adjoint.
+      auto flush_slice = [&](const float* v, int count, int v0) {
+        wave_sync();
+#pragma unroll
+        for (int q = 0; q < RED_SLICE; q++) s_red[lane * RED_W + q] = q < count ? v[q] : 0.0f;
+        wave_sync();
+        const int col_ = lane & (RED_SLICE - 1);
+        const int row0 = (lane >> 4) * 16;
+        float acc = 0.0f;
+#pragma unroll
+        for (int rr = 0; rr < 16; rr++) acc += s_red[(row0 + rr) * RED_W + col_];
+        acc += __shfl_xor(acc, 16, 64);
+        acc += __shfl_xor(acc, 32, 64);
+        if (lane < count && acc != 0.0f) atomicAdd(grad_slot(p, sr.site, T, v0 + lane), acc);
       };
 #pragma unroll
-      for (int sl = 0; sl < NVAL / RED_SLICE; sl++) {
-        if (sl * RED_SLICE < nval) {
-          wave_sync();
+      for (int i0 = 0; i0 < PF_MAX_TEXELS; i0 += 2) {
+        if (i0 < T) {
+          float v[14];
 #pragma unroll
-          for (int q = 0; q < RED_SLICE; q++) s_red[lane * RED_W + q] = val(sl * RED_SLICE + q);
-          wave_sync();
-          const int col_ = lane & (RED_SLICE - 1);
-          const int row0 = (lane >> 4) * 16;
-          float acc = 0.0f;
+          for (int q = 0; q < 14; q++) v[q] = 0.0f;
+          if (active) {
 #pragma unroll
-          for (int rr = 0; rr < 16; rr++) acc += s_red[(row0 + rr) * RED_W + col_];
-          acc += __shfl_xor(acc, 16, 64);
-          acc += __shfl_xor(acc, 32, 64);
-          const int v = sl * RED_SLICE + lane;
-          if (lane < RED_SLICE && v < nval && acc != 0.0f) atomicAdd(grad_slot(p, sr.site, T, v), acc);
+            for (int k = 0; k < 2; k++) {
+              const int i = i0 + k;
+              if (i < T) {
+                const float4 ph = s_tex[i].pos_h;
+                const float4 cl = s_tex[i].rgb;
+                // colour stage (texture.py:158-171)
+                const float wn = w1[i] * inv_ws;
+                if (!(wn < cut)) {
+                  const f3 df{x1.x - ph.x, x1.y - ph.y, x1.z - ph.z};
+                  v[7 * k + 3] = dLdcol.x * wn;
+                  v[7 * k + 4] = dLdcol.y * wn;
+                  v[7 * k + 5] = dLdcol.z * wn;
+                  const float gdp = dLdcol.x * cl.x + dLdcol.y * cl.y + dLdcol.z * cl.z;
+                  const f3 sg = df * (kw * wn * (gdp - gdp_in));
+                  v[7 * k + 0] = sg.x;
+                  v[7 * k + 1] = sg.y;
+                  v[7 * k + 2] = sg.z;
+                }
+                // height stage (texture.py:118-131)
+                const f3 df{x0.x - ph.x, x0.y - ph.y, x0.z - ph.z};
+                const float w = heights ? fast_exp(dot(df, df) * kexp) : w1[i];
+                if (!(w * inv_ws < cut)) {
+                  const float wn0 = w * inv_ws0;
+                  v[7 * k + 6] = wn0 * dLdheight;
+                  const f3 sg = df * (kw * wn0 * (dLdheight * ph.w - gdp0_in));
+                  v[7 * k + 0] += sg.x;
+                  v[7 * k + 1] += sg.y;
+                  v[7 * k + 2] += sg.z;
+                }
+              }
+            }
+          }
+          flush_slice(v, i0 + 1 < T ? 14 : 7, 8 + 7 * i0);
         }
+      }
+      {
+        const float v[8] = {dLdc.x, dLdc.y, dLdc.z, dLdr, dLdnorm.x, dLdnorm.y, dLdnorm.z, dLdsig};
+        flush_slice(v, 8, 0);
       }
     }
   }
