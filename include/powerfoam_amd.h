@@ -72,6 +72,10 @@ typedef struct pf_tile_lists {
   int32_t n_tiles;
   const int64_t* offsets; /* [n_tiles+1] */
   const int32_t* sites;   /* [M] */
+  /* [n_tiles] or NULL: the order in which tiles are started (a permutation).
+   * Results do not depend on it; starting the longest lists first shortens
+   * the kernels' tail. NULL: natural order, dealt to the XCDs in runs. */
+  const int32_t* order;
 } pf_tile_lists;
 
 typedef struct pf_raster_fwd_args {

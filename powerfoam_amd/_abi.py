@@ -63,6 +63,7 @@ class PfTileLists(ctypes.Structure):
         ("n_tiles", ctypes.c_int32),
         ("offsets", ctypes.c_void_p),
         ("sites", ctypes.c_void_p),
+        ("order", ctypes.c_void_p),
     ]
 
 

@@ -7,9 +7,11 @@
 //
 // One wave64 per 8x8 tile (the reference's 64-thread tile block maps 1:1 onto
 // a CDNA wavefront); lane l shades pixel (tile_i*8 + l%8, tile_j*8 + l/8).
-// A 256-thread workgroup carries four independent tiles; each wave owns its
-// LDS slots and synchronises only with itself, so waves never wait on each
-// other's list lengths.
+// Each workgroup is one tile, so its LDS and slot free up as soon as that
+// tile is done. List lengths are very uneven (on a 1M-site view the longest
+// list is ~12x the mean), so the host passes a longest-first tile order: the
+// longest tiles start first and the short ones fill in around them, instead
+// of one long tile starting last and setting the kernel's length.
 //
 // A wave walks its tile's depth-sorted site list in batches of 64: lanes
 // fetch 64 site records with one coalesced gather into LDS, and every
@@ -33,14 +35,17 @@
 // log-transmittance (rasterize.py:1219, 1372-1375) and reduces every per-site
 // gradient across the wave before touching global memory: neighbour-face
 // gradients are gathered in LDS per face slot, the site and texel gradients
-// are summed by an LDS transpose in four 16-value slices, and one lane per
-// value issues the atomic.
+// are summed by an LDS transpose in 16-value slices (two texels per slice,
+// recomputed in a final pass so no per-texel accumulators stay live), and
+// one lane per value issues the atomic.
 #include "pf_common.hpp"
 
 namespace pf {
 
+// One tile (wave) per workgroup: a workgroup's LDS and slot are released as
+// soon as its tile is done, so long and short tiles do not wait on each other.
 #ifndef PF_WPB
-#define PF_WPB 4
+#define PF_WPB 1
 #endif
 constexpr int WPB = PF_WPB;  // tiles (waves) per workgroup
 
@@ -80,6 +85,19 @@ __device__ __forceinline__ int xcd_group(int b, int n_groups, bool& valid) {
   const int full = n_groups - (per - 1) * 8;  // XCDs that own `per` groups
   valid = xcd < full ? k < per : k < per - 1;
   return xcd < full ? xcd * per + k : full * per + (xcd - full) * (per - 1) + k;
+}
+
+// Tile of wave `wv` of workgroup `b`: with an order array, slot b*WPB + wv
+// takes order[slot] (workgroups start in slot order as CUs free up);
+// otherwise the XCD-aware natural order above. -1: no tile.
+__device__ __forceinline__ int wave_tile(const int32_t* order, int n_tiles, int b, int wv) {
+  if (order) {
+    const int slot = b * WPB + wv;
+    return slot < n_tiles ? order[slot] : -1;
+  }
+  bool valid;
+  const int t = xcd_group(b, (n_tiles + WPB - 1) / WPB, valid) * WPB + wv;
+  return valid && t < n_tiles ? t : -1;
 }
 
 struct Ray {
@@ -179,6 +197,7 @@ struct FwdParams {
   const uint16_t* adj_diff;
   const int64_t* offsets;
   const int32_t* sites;
+  const int32_t* order;
   const float* depth_q;
   const float* ray_gt;
   float thr, inv_hw;
@@ -202,10 +221,8 @@ __global__ void __launch_bounds__(WAVE* WPB) raster_fwd_kernel(FwdParams p) {
 
   const int wv = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
-  bool valid;
-  const int group = xcd_group(blockIdx.x, (p.n_tiles + WPB - 1) / WPB, valid);
-  const int tile = group * WPB + wv;
-  if (!valid || tile >= p.n_tiles) return;
+  const int tile = wave_tile(p.order, p.n_tiles, blockIdx.x, wv);
+  if (tile < 0) return;
   SiteRec* s_site = s_site_all[wv];
   float4* s_face = s_face_all[wv];
   TexelLds* s_tex = s_tex_all[wv];
@@ -396,6 +413,7 @@ struct BwdParams {
   const uint16_t* adj_diff;
   const int64_t* offsets;
   const int32_t* sites;
+  const int32_t* order;
   const float* depth_q;
   const int32_t* early;
   const float* log_t;
@@ -441,10 +459,8 @@ __global__ void __launch_bounds__(WAVE* WPB) PF_BWD_OCC raster_bwd_kernel(BwdPar
 
   const int wv = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
-  bool valid;
-  const int group = xcd_group(blockIdx.x, (p.n_tiles + WPB - 1) / WPB, valid);
-  const int tile = group * WPB + wv;
-  if (!valid || tile >= p.n_tiles) return;
+  const int tile = wave_tile(p.order, p.n_tiles, blockIdx.x, wv);
+  if (tile < 0) return;
   SiteRec* s_site = s_site_all[wv];
   float4* s_face = s_face_all[wv];
   TexelLds* s_tex = s_tex_all[wv];
@@ -898,6 +914,7 @@ struct RgbParams {
   const uint16_t* adj_diff;
   const int64_t* offsets;
   const int32_t* sites;
+  const int32_t* order;
   float thr;
   float* color;
   // visualisation only
@@ -923,10 +940,8 @@ __global__ void __launch_bounds__(WAVE* WPB) raster_rgb_kernel(RgbParams p) {
 
   const int wv = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
-  bool valid;
-  const int group = xcd_group(blockIdx.x, (p.n_tiles + WPB - 1) / WPB, valid);
-  const int tile = group * WPB + wv;
-  if (!valid || tile >= p.n_tiles) return;
+  const int tile = wave_tile(p.order, p.n_tiles, blockIdx.x, wv);
+  if (tile < 0) return;
   SiteRec* s_site = s_site_all[wv];
   float4* s_face = s_face_all[wv];
   TexelLds* s_tex = s_tex_all[wv];
@@ -1135,6 +1150,7 @@ int pf_raster_fwd(const pf_raster_fwd_args* a, void* stream) {
   p.adj_diff = a->scene.adjacency_diff;
   p.offsets = a->tiles.offsets;
   p.sites = a->tiles.sites;
+  p.order = a->tiles.order;
   p.depth_q = a->depth_quantiles;
   p.ray_gt = a->ray_gt;
   p.thr = a->transmittance_threshold;
@@ -1190,6 +1206,7 @@ int pf_raster_bwd(const pf_raster_bwd_args* a, void* stream) {
   p.adj_diff = a->scene.adjacency_diff;
   p.offsets = a->tiles.offsets;
   p.sites = a->tiles.sites;
+  p.order = a->tiles.order;
   p.depth_q = a->depth_quantiles;
   p.early = a->tile_early_stop;
   p.log_t = a->log_t;
@@ -1238,6 +1255,7 @@ int pf_raster_rgb(const pf_raster_rgb_args* a, void* stream) {
   p.adj_diff = a->scene.adjacency_diff;
   p.offsets = a->tiles.offsets;
   p.sites = a->tiles.sites;
+  p.order = a->tiles.order;
   p.thr = a->transmittance_threshold;
   p.color = a->color;
   const int groups = (nt + WPB - 1) / WPB;
@@ -1276,6 +1294,7 @@ int pf_raster_vis(const pf_raster_vis_args* a, void* stream) {
   p.adj_diff = a->scene.adjacency_diff;
   p.offsets = a->tiles.offsets;
   p.sites = a->tiles.sites;
+  p.order = a->tiles.order;
   p.thr = a->transmittance_threshold;
   p.color = a->color;
   p.max_inter = a->max_intersections;

@@ -84,6 +84,9 @@ class Rasterizer:
         # Tile lists and early-stop counters of the last forward, for workload
         # accounting (the same tensors the backward keeps).
         self.last_tiles = None
+        # Start the tiles with the longest site lists first (the results do not
+        # depend on the order; the kernels' tail gets shorter).
+        self.longest_first = True
 
     def _launch(self, name, fn, *a):
         if self.event_log is None:
@@ -150,6 +153,16 @@ class Rasterizer:
         check(lib.pf_bin_sort(ptr(offsets), nt, ptr(records), ptr(sites), None, 0, st))
         return offsets, sites[:m] if m > 0 else sites[:0], visible.bool(), nt
 
+    def tile_order(self, offsets):
+        """Tiles by descending list length (int32), or None for the natural order."""
+        if not self.longest_first:
+            return None
+        return torch.argsort(offsets[1:] - offsets[:-1], descending=True).to(torch.int32)
+
+    def _tiles(self, nt, offsets, tile_sites, order):
+        return _abi.PfTileLists(nt, offsets.data_ptr(), tile_sites.data_ptr() if tile_sites.numel() else None,
+                                order.data_ptr() if order is not None else None)
+
     def encode_adjacency(self, all_spheres, adjacency, adjacency_offsets):
         """prefetch_adjacency_kernel (rasterize.py:580-606): fp16 (c_adj - c, r_adj)."""
         diff = torch.empty((max(adjacency.shape[0], 1), 4), dtype=torch.float16, device=all_spheres.device)
@@ -182,6 +195,7 @@ class Rasterizer:
         dq = _f32c(depth_quantiles) if depth_quantiles is not None else None
 
         offsets, tile_sites, prim_visible_mask, nt = self.bin(camera, all_spheres)
+        order = self.tile_order(offsets)
         adjacency_diff = self.encode_adjacency(all_spheres, adjacency, adjacency_offsets)
 
         f32 = dict(dtype=torch.float32, device=dev)
@@ -199,7 +213,7 @@ class Rasterizer:
         a.cam, ray_maps = self._camera(camera)
         a.scene = _scene_struct(all_spheres, all_nsigmas, all_texel_sites, all_texel_rgbh, adjacency,
                                 adjacency_offsets, None if self.exact_geometry else adjacency_diff)
-        a.tiles = _abi.PfTileLists(nt, offsets.data_ptr(), tile_sites.data_ptr() if tile_sites.numel() else None)
+        a.tiles = self._tiles(nt, offsets, tile_sites, order)
         a.depth_quantiles = dq.data_ptr() if dq is not None else None
         a.n_quantiles = dq.shape[-1] if dq is not None else 0
         a.ray_gt = ray_gt_c.data_ptr() if ray_gt_c is not None else None
@@ -221,6 +235,7 @@ class Rasterizer:
         grad_ctx.rasterizer = self
         grad_ctx.camera = camera
         grad_ctx.ray_maps = ray_maps
+        grad_ctx.order = order
         grad_ctx.save_for_backward(dq, all_spheres, all_nsigmas, all_texel_sites, all_texel_rgbh, adjacency,
                                    adjacency_offsets, adjacency_diff, tile_sites, offsets, tile_early_stop,
                                    ray_gt_c, color_out, log_t_out)
@@ -249,12 +264,13 @@ class Rasterizer:
             adjacency_offsets = adjacency_offsets.to(torch.int32).contiguous()
             adjacency_diff = adjacency_diff.to(torch.float16).contiguous()
             offsets, tile_sites, _, nt = self.bin(camera, all_spheres)
+            order = self.tile_order(offsets)
             color_out = torch.zeros((H, W, 3), dtype=torch.float32, device=points.device)
             a = _abi.PfRasterRgbArgs()
             a.cam, ray_maps = self._camera(camera)
             a.scene = _scene_struct(all_spheres, all_nsigmas, all_texel_sites, all_texel_rgbh, adjacency,
                                     adjacency_offsets, adjacency_diff)
-            a.tiles = _abi.PfTileLists(nt, offsets.data_ptr(), tile_sites.data_ptr() if tile_sites.numel() else None)
+            a.tiles = self._tiles(nt, offsets, tile_sites, order)
             a.transmittance_threshold = float(transmittance_threshold)
             a.color = color_out.data_ptr()
             self._launch("raster_rgb", self.lib.pf_raster_rgb, _ref(a), stream_handle())
@@ -280,6 +296,7 @@ class Rasterizer:
             adjacency = adjacency.to(torch.int32).contiguous()
             adjacency_offsets = adjacency_offsets.to(torch.int32).contiguous()
             offsets, tile_sites, _, nt = self.bin(camera, all_spheres)
+            order = self.tile_order(offsets)
             adjacency_diff = self.encode_adjacency(all_spheres, adjacency, adjacency_offsets)
             f32 = dict(dtype=torch.float32, device=dev)
             color = torch.zeros((H, W, 3), **f32)
@@ -291,7 +308,7 @@ class Rasterizer:
             a.cam, ray_maps = self._camera(camera)
             a.scene = _scene_struct(all_spheres, all_nsigmas, all_texel_sites, all_texel_rgbh, adjacency,
                                     adjacency_offsets, adjacency_diff)
-            a.tiles = _abi.PfTileLists(nt, offsets.data_ptr(), tile_sites.data_ptr() if tile_sites.numel() else None)
+            a.tiles = self._tiles(nt, offsets, tile_sites, order)
             a.transmittance_threshold = float(vis_options.transmittance_threshold)
             a.max_intersections = int(vis_options.max_intersections)
             a.depth_quantile = float(vis_options.depth_quantile)
@@ -339,8 +356,7 @@ class Rasterizer:
         a.cam, ray_maps = self._camera(camera, grad_ctx.ray_maps)
         a.scene = _scene_struct(all_spheres, all_nsigmas, all_texel_sites, all_texel_rgbh, adjacency,
                                 adjacency_offsets, None if self.exact_geometry else adjacency_diff)
-        a.tiles = _abi.PfTileLists(offsets.shape[0] - 1, offsets.data_ptr(),
-                                   tile_sites.data_ptr() if tile_sites.numel() else None)
+        a.tiles = self._tiles(offsets.shape[0] - 1, offsets, tile_sites, grad_ctx.order)
         a.depth_quantiles = dq.data_ptr() if dq is not None else None
         a.n_quantiles = dq.shape[-1] if dq is not None else 0
         a.tile_early_stop = tile_early_stop.data_ptr()
@@ -361,7 +377,7 @@ class Rasterizer:
         self._launch("raster_bwd", self.lib.pf_raster_bwd, _ref(a), st)
         for t in (spheres_grad, nsigmas_grad, texel_sites_grad, texel_rgb_grad, texel_height_grad):
             check(self.lib.pf_scrub_nonfinite(ptr(t), t.numel(), st))
-        del grad_ctx.rasterizer, grad_ctx.camera, grad_ctx.ray_maps
+        del grad_ctx.rasterizer, grad_ctx.camera, grad_ctx.ray_maps, grad_ctx.order
         return (None, None, None, spheres_grad[:, :3], spheres_grad[:, 3], nsigmas_grad[:, 3],
                 nsigmas_grad[:, :3], texel_sites_grad, texel_rgb_grad, texel_height_grad, None, None, None, None)
 
