@@ -78,18 +78,39 @@ typedef struct pf_tile_lists {
   const int32_t* order;
 } pf_tile_lists;
 
+/* Optional splitting of long tile lists (no reference counterpart; the
+ * results are those of the unsplit walk up to float summation order). A list
+ * of `count` sites becomes max(1, ceil(count / seg_len)) segments that run in
+ * parallel: the forward first sums -sigma dt over every segment but a tile's
+ * last (log_t_sums), each segment then starts from the exact transmittance
+ * the earlier ones leave; the backward starts each segment from the adjoint
+ * state the later ones leave. n_segs = 0 keeps one wave per whole list.
+ * The forward and the backward of a frame must use the same table. */
+typedef struct pf_segments {
+  int32_t n_segs;
+  int32_t seg_len;            /* sites per segment, a positive multiple of 64 */
+  const int32_t* tile_first;  /* [n_tiles+1] first segment id of each tile */
+  const int32_t* seg_tile;    /* [n_segs] tile of each segment id */
+  const int32_t* order;       /* [n_segs] segment ids in start order (longest first) */
+  float* log_t_sums;          /* [n_segs,64] written by the forward, read by the backward */
+  uint8_t* lane_active;       /* [n_segs,64] written by the forward, read by the backward */
+  float* scratch;             /* [n_segs,64] per-call scratch */
+} pf_segments;
+
 typedef struct pf_raster_fwd_args {
   pf_camera cam;
   pf_scene scene;
   pf_tile_lists tiles;
+  pf_segments segs;
   const float* depth_quantiles; /* [H,W,Q] transmittance levels, descending, or NULL */
   int32_t n_quantiles;          /* Q (0 when depth_quantiles is NULL) */
   const float* ray_gt;          /* [H,W,3] or NULL */
   float transmittance_threshold;
   int32_t return_point_err;
   int32_t loss_accumulation;    /* render_objective == "surface" */
-  /* outputs (all written by the call; contrib/point_err must be zeroed and
-   * quantile_depths set to -1 by the caller, as the reference does) */
+  /* outputs (color, normal_distance, normal, err, contrib and point_err
+   * must be zeroed and quantile_depths set to -1 by the caller, as the
+   * reference allocates them) */
   int32_t* tile_early_stop; /* [n_tiles] */
   float* color;             /* [H,W,3] */
   float* log_t;             /* [H,W]   */
@@ -105,6 +126,7 @@ typedef struct pf_raster_bwd_args {
   pf_camera cam;
   pf_scene scene;
   pf_tile_lists tiles;
+  pf_segments segs;
   const float* depth_quantiles; /* [H,W,Q] or NULL */
   int32_t n_quantiles;
   const int32_t* tile_early_stop;      /* [n_tiles] from the forward */

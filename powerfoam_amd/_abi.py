@@ -58,6 +58,19 @@ class PfScene(ctypes.Structure):
     ]
 
 
+class PfSegments(ctypes.Structure):
+    _fields_ = [
+        ("n_segs", ctypes.c_int32),
+        ("seg_len", ctypes.c_int32),
+        ("tile_first", ctypes.c_void_p),
+        ("seg_tile", ctypes.c_void_p),
+        ("order", ctypes.c_void_p),
+        ("log_t_sums", ctypes.c_void_p),
+        ("lane_active", ctypes.c_void_p),
+        ("scratch", ctypes.c_void_p),
+    ]
+
+
 class PfTileLists(ctypes.Structure):
     _fields_ = [
         ("n_tiles", ctypes.c_int32),
@@ -72,6 +85,7 @@ class PfRasterFwdArgs(ctypes.Structure):
         ("cam", PfCamera),
         ("scene", PfScene),
         ("tiles", PfTileLists),
+        ("segs", PfSegments),
         ("depth_quantiles", ctypes.c_void_p),
         ("n_quantiles", ctypes.c_int32),
         ("ray_gt", ctypes.c_void_p),
@@ -95,6 +109,7 @@ class PfRasterBwdArgs(ctypes.Structure):
         ("cam", PfCamera),
         ("scene", PfScene),
         ("tiles", PfTileLists),
+        ("segs", PfSegments),
         ("depth_quantiles", ctypes.c_void_p),
         ("n_quantiles", ctypes.c_int32),
         ("tile_early_stop", ctypes.c_void_p),

@@ -211,9 +211,51 @@ struct FwdParams {
   float* err;
   float* contrib;
   float* point_err;
+  // segments (n_segs = 0: one wave per whole list)
+  int n_segs, seg_len;
+  const int32_t* seg_first;
+  const int32_t* seg_tile;
+  const int32_t* seg_order;
+  float* seg_logt;   // [n_segs, 64] sum of -sigma dt over the segment
+  float* seg_final;  // [n_segs, 64] log T where the segment's wave stopped
+  uint8_t* seg_active;  // [n_segs, 64] the lane shaded a site of its walk
 };
 
-template <bool PINHOLE>
+// The work item of a wave: a whole tile list, or segment k of nk of a tile
+// (local sites [k*seg_len, min((k+1)*seg_len, total))). Returns false when
+// the slot has no work.
+struct WorkItem {
+  int tile, seg, k, nk, f0;
+};
+
+__device__ __forceinline__ bool work_item(int n_segs, const int32_t* seg_order, const int32_t* seg_tile,
+                                          const int32_t* seg_first, const int32_t* order, int n_tiles, int b, int wv,
+                                          WorkItem& w) {
+  if (n_segs > 0) {
+    const int slot = b * WPB + wv;
+    if (slot >= n_segs) return false;
+    w.seg = seg_order[slot];
+    w.tile = seg_tile[w.seg];
+    w.f0 = seg_first[w.tile];
+    w.nk = seg_first[w.tile + 1] - w.f0;
+    w.k = w.seg - w.f0;
+    return true;
+  }
+  w.tile = wave_tile(order, n_tiles, b, wv);
+  w.seg = -1;
+  w.k = 0;
+  w.nk = 1;
+  w.f0 = 0;
+  return w.tile >= 0;
+}
+
+// LOGT = true: the segment prepass of a split list. It walks the segment's
+// sites without the early stop and without colour, and stores the per-lane
+// sum of -sigma dt, from which every later segment of the tile starts. A
+// stop inside an earlier segment leaves every lane below the threshold for
+// all later sites (T only decreases), so later segments stop at their first
+// site and the split reproduces the serial walk.
+template <bool PINHOLE, bool LOGT>
 __global__ void __launch_bounds__(WAVE* WPB) raster_fwd_kernel(FwdParams p) {
   __shared__ SiteRec s_site_all[WPB][WAVE];
   __shared__ float4 s_face_all[WPB][WAVE];
@@ -221,8 +263,10 @@ __global__ void __launch_bounds__(WAVE* WPB) raster_fwd_kernel(FwdParams p) {
 
   const int wv = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
-  const int tile = wave_tile(p.order, p.n_tiles, blockIdx.x, wv);
-  if (tile < 0) return;
+  WorkItem wi;
+  if (!work_item(p.n_segs, p.seg_order, p.seg_tile, p.seg_first, p.order, p.n_tiles, blockIdx.x, wv, wi)) return;
+  if (LOGT && wi.k == wi.nk - 1) return;  // no later segment needs it
+  const int tile = wi.tile;
   SiteRec* s_site = s_site_all[wv];
   float4* s_face = s_face_all[wv];
   TexelLds* s_tex = s_tex_all[wv];
@@ -233,20 +277,35 @@ __global__ void __launch_bounds__(WAVE* WPB) raster_fwd_kernel(FwdParams p) {
 
   f3 gt{0.0f, 0.0f, 0.0f};
   if (p.ray_gt) gt = ld3(p.ray_gt + 3 * px);
-  int qidx = 0;
-  float qcur = 0.0f;
-  if (p.Q > 0) qcur = p.depth_q[px * p.Q];
 
   float log_t = 0.0f, nd = 0.0f, err = 0.0f;
   f3 rgb{0.0f, 0.0f, 0.0f}, nacc{0.0f, 0.0f, 0.0f};
+  if (!LOGT)
+    for (int j = 0; j < wi.k; j++) log_t += p.seg_logt[(size_t)(wi.f0 + j) * WAVE + lane];
+  // quantiles whose depth an earlier segment wrote: T already below them
+  int qidx = 0;
+  float qcur = 0.0f;
+  if (!LOGT && p.Q > 0) {
+    qcur = p.depth_q[px * p.Q];
+    if (wi.k > 0) {
+      const float t0 = fast_exp(log_t);
+      while (qidx < p.Q && t0 < qcur) {
+        qidx++;
+        if (qidx < p.Q) qcur = p.depth_q[px * p.Q + qidx];
+      }
+    }
+  }
 
   const int64_t start = p.offsets[tile];
   const int total = (int)(p.offsets[tile + 1] - start);
+  const int lo = wi.nk > 1 ? wi.k * p.seg_len : 0;
+  const int hi = wi.nk > 1 ? min(lo + p.seg_len, total) : total;
   int early = total;
   bool stop = false;
+  bool any_active = false;
 
-  for (int base = 0; base < total && !stop; base += WAVE) {
-    const int cnt = min(WAVE, total - base);
+  for (int base = lo; base < hi && !stop; base += WAVE) {
+    const int cnt = min(WAVE, hi - base);
     wave_sync();
     if (lane < cnt) {
       const int s = p.sites[start + base + lane];
@@ -258,8 +317,8 @@ __global__ void __launch_bounds__(WAVE* WPB) raster_fwd_kernel(FwdParams p) {
     }
     wave_sync();
     for (int kk = 0; kk < cnt; kk++) {
-      const float trans = fast_exp(log_t);
-      if (ballot(trans >= p.thr) == 0) {
+      const float trans = LOGT ? 1.0f : fast_exp(log_t);
+      if (!LOGT && ballot(trans >= p.thr) == 0) {
         early = base + kk;
         stop = true;
         break;
@@ -326,6 +385,12 @@ __global__ void __launch_bounds__(WAVE* WPB) raster_fwd_kernel(FwdParams p) {
           height = hsum * fast_rcp(sel_max(ws0, 1e-20f));
         }
         const float ts1 = heights ? acc_div(cn + height, dpn) : ts0;
+        if (LOGT) {
+          const float tf2 = (dpn >= 0.0f && ts1 < tf) ? ts1 : tf;
+          const float tn2 = (dpn < 0.0f && ts1 > tn) ? ts1 : tn;
+          if (tf2 - tn2 > 0.0f) log_t += -sigma * (tf2 - tn2);
+          continue;
+        }
         const float tq = dpn >= 0.0f ? tn : sel_max(tn, ts1);
         const f3 x1 = ray.d * tq - co;
         f3 rs{0.0f, 0.0f, 0.0f};
@@ -346,6 +411,7 @@ __global__ void __launch_bounds__(WAVE* WPB) raster_fwd_kernel(FwdParams p) {
         const float tn2 = (dpn < 0.0f && ts1 > tn) ? ts1 : tn;
         const float dt = tf2 - tn2;
         if (dt > 0.0f) {
+          any_active = true;
           const float dlt = -sigma * dt;
           const float alpha = 1.0f - fast_exp(dlt);
           const float at = alpha * trans;
@@ -374,7 +440,7 @@ __global__ void __launch_bounds__(WAVE* WPB) raster_fwd_kernel(FwdParams p) {
         }
       }
       // contrib / point_err: one atomic per site per wave (rasterize.py:1101-1110)
-      if (ballot(alpha_t > 0.0f) != 0) {
+      if (!LOGT && ballot(alpha_t > 0.0f) != 0) {
         const float sa = wave_sum(alpha_t);
         if (lane == 0) atomicAdd(&p.contrib[sr.site], sa * p.inv_hw);
         if (p.point_err_on) {
@@ -385,18 +451,64 @@ __global__ void __launch_bounds__(WAVE* WPB) raster_fwd_kernel(FwdParams p) {
     }
   }
 
-  if (lane == 0) p.early[tile] = early;
-  if (!ray.oob) {
-    p.color[3 * px + 0] = rgb.x;
-    p.color[3 * px + 1] = rgb.y;
-    p.color[3 * px + 2] = rgb.z;
-    p.log_t[px] = log_t;
-    p.ndist[px] = nd;
-    p.normal[3 * px + 0] = nacc.x;
-    p.normal[3 * px + 1] = nacc.y;
-    p.normal[3 * px + 2] = nacc.z;
-    if (p.loss_acc && p.err) p.err[px] = err + dot(gt, gt) * fast_exp(log_t);
+  if (LOGT) {
+    p.seg_logt[(size_t)wi.seg * WAVE + lane] = log_t;
+    return;
   }
+  if (wi.seg >= 0) p.seg_active[(size_t)wi.seg * WAVE + lane] = any_active ? 1 : 0;
+  if (wi.nk == 1) {
+    if (lane == 0) p.early[tile] = early;
+    if (!ray.oob) {
+      p.color[3 * px + 0] = rgb.x;
+      p.color[3 * px + 1] = rgb.y;
+      p.color[3 * px + 2] = rgb.z;
+      p.log_t[px] = log_t;
+      p.ndist[px] = nd;
+      p.normal[3 * px + 0] = nacc.x;
+      p.normal[3 * px + 1] = nacc.y;
+      p.normal[3 * px + 2] = nacc.z;
+      if (p.loss_acc && p.err) p.err[px] = err + dot(gt, gt) * fast_exp(log_t);
+    }
+    return;
+  }
+  // a segment of a split list: partial sums into the zeroed outputs; the
+  // final log T and the early-stop index are settled by the finishing kernel
+  if (lane == 0 && (stop || wi.k == wi.nk - 1)) atomicMin(&p.early[tile], early);
+  p.seg_final[(size_t)wi.seg * WAVE + lane] = log_t;
+  if (!ray.oob) {
+    atomicAdd(&p.color[3 * px + 0], rgb.x);
+    atomicAdd(&p.color[3 * px + 1], rgb.y);
+    atomicAdd(&p.color[3 * px + 2], rgb.z);
+    atomicAdd(&p.ndist[px], nd);
+    atomicAdd(&p.normal[3 * px + 0], nacc.x);
+    atomicAdd(&p.normal[3 * px + 1], nacc.y);
+    atomicAdd(&p.normal[3 * px + 2], nacc.z);
+    if (p.loss_acc && p.err) atomicAdd(&p.err[px], err);
+  }
+}
+
+// Split lists: log T of a pixel is where its tile's walk stopped -- in the
+// segment holding the early-stop index, or the last one.
+template <bool PINHOLE>
+__global__ void __launch_bounds__(WAVE) raster_fwd_finish_kernel(FwdParams p) {
+  const int tile = blockIdx.x, lane = threadIdx.x;
+  const int f0 = p.seg_first[tile], nk = p.seg_first[tile + 1] - f0;
+  if (nk <= 1) return;
+  const int ks = min(p.early[tile] / p.seg_len, nk - 1);
+  const float lt = p.seg_final[(size_t)(f0 + ks) * WAVE + lane];
+  const Ray ray = lane_ray<PINHOLE>(p.cam, tile, lane);
+  if (ray.oob) return;
+  const size_t px = (size_t)ray.pi * p.cam.W + ray.pj;
+  p.log_t[px] = lt;
+  if (p.loss_acc && p.err) {
+    const f3 gt = ld3(p.ray_gt + 3 * px);
+    atomicAdd(&p.err[px], dot(gt, gt) * fast_exp(lt));
+  }
+}
+
+__global__ void __launch_bounds__(256) fill_int_kernel(int32_t* x, int n, int32_t v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = v;
 }
 
 // ----------------------------------------------------------------- backward
@@ -429,6 +541,14 @@ struct BwdParams {
   float* g_tsites;
   float* g_trgb;
   float* g_theight;
+  // segments (n_segs = 0: one wave per whole list)
+  int n_segs, seg_len;
+  const int32_t* seg_first;
+  const int32_t* seg_tile;
+  const int32_t* seg_order;
+  const float* seg_logt;  // [n_segs, 64] from the forward
+  const uint8_t* seg_active;  // [n_segs, 64] from the forward
+  float* seg_dsum;        // [n_segs, 64] per-segment sum of dL/dlog T increments
 };
 
 constexpr int RED_SLICE = 16;      // values reduced per LDS pass
@@ -444,12 +564,18 @@ __device__ __forceinline__ float* grad_slot(const BwdParams& p, int site, int T,
   return p.g_theight + (size_t)site * T + i;
 }
 
-template <bool PINHOLE>
 #ifdef PF_BWD_WAVES
 #define PF_BWD_OCC __attribute__((amdgpu_waves_per_eu(PF_BWD_WAVES)))
 #else
 #define PF_BWD_OCC
 #endif
+// DSUM = true: the segment prepass of a split list. dL/dlog T, the adjoint
+// that carries information from the back of a list to its front, gains an
+// additive increment at every site; this pass sums the increments of one
+// segment so that every earlier segment can start from the value the later
+// ones leave. The geometry, colour and quantile state are those of the main
+// pass; no parameter gradients are formed.
+template <bool PINHOLE, bool DSUM>
 __global__ void __launch_bounds__(WAVE* WPB) PF_BWD_OCC raster_bwd_kernel(BwdParams p) {
   __shared__ SiteRec s_site_all[WPB][WAVE];
   __shared__ float4 s_face_all[WPB][WAVE];
@@ -459,8 +585,16 @@ __global__ void __launch_bounds__(WAVE* WPB) PF_BWD_OCC raster_bwd_kernel(BwdPar
 
   const int wv = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
-  const int tile = wave_tile(p.order, p.n_tiles, blockIdx.x, wv);
-  if (tile < 0) return;
+  WorkItem wi;
+  if (!work_item(p.n_segs, p.seg_order, p.seg_tile, p.seg_first, p.order, p.n_tiles, blockIdx.x, wv, wi)) return;
+  const int tile = wi.tile;
+  const int early = p.early[tile];
+  // this segment's share of the walked sites [0, early)
+  const int lo = wi.nk > 1 ? wi.k * p.seg_len : 0;
+  const int hi = wi.nk > 1 ? min(lo + p.seg_len, early) : early;
+  if (hi <= lo) return;
+  const int k_last = wi.nk > 1 ? (early - 1) / p.seg_len : 0;  // segment of the last walked site
+  if (DSUM && wi.k == 0) return;  // no earlier segment needs it
   SiteRec* s_site = s_site_all[wv];
   float4* s_face = s_face_all[wv];
   TexelLds* s_tex = s_tex_all[wv];
@@ -475,19 +609,40 @@ __global__ void __launch_bounds__(WAVE* WPB) PF_BWD_OCC raster_bwd_kernel(BwdPar
   const float cut = exact ? 0.0f : 1e-3f;
   const float q_eps = exact ? 0.0f : 1e-6f;
 
-  float log_t = p.log_t[px];
-  float dLdlog_t = p.g_log_t[px];  // black background: no colour term
+  // state after the segment's last site: log T and dL/dlog T (black
+  // background: no colour term), and the quantiles later segments handle
+  float log_t, dLdlog_t;
+  int qidx = p.Q - 1;
+  if (wi.k == k_last) {
+    log_t = p.log_t[px];
+    dLdlog_t = DSUM ? 0.0f : p.g_log_t[px];
+  } else {
+    log_t = 0.0f;
+    for (int j = 0; j <= wi.k; j++) log_t += p.seg_logt[(size_t)(wi.f0 + j) * WAVE + lane];
+    dLdlog_t = 0.0f;
+    if (!DSUM) {
+      dLdlog_t = p.g_log_t[px];
+      for (int j = wi.k + 1; j <= k_last; j++) dLdlog_t += p.seg_dsum[(size_t)(wi.f0 + j) * WAVE + lane];
+    }
+    // later segments handle the quantiles below T at this segment's end --
+    // provided this lane shades a site there (a quantile is handled at an
+    // active site only)
+    bool later = false;
+    for (int j = wi.k + 1; j <= k_last; j++) later = later || p.seg_active[(size_t)(wi.f0 + j) * WAVE + lane] != 0;
+    if (later) {
+      const float t_end = fast_exp(log_t);
+      while (qidx >= 0 && t_end > p.depth_q[px * p.Q + qidx]) qidx--;
+    }
+  }
   const f3 g_rgb = ld3(p.g_color + 3 * px);
   const f3 g_n = ld3(p.g_normal + 3 * px);
   const float g_nd = p.g_ndist[px];
-  int qidx = p.Q - 1;
-  float qcur = p.Q > 0 ? p.depth_q[px * p.Q + qidx] : 1.0f;
+  float qcur = qidx >= 0 ? p.depth_q[px * p.Q + qidx] : 1.0f;
 
   const int64_t start = p.offsets[tile];
-  const int early = p.early[tile];
 
-  for (int top = early; top > 0; top -= WAVE) {
-    const int cnt = min(WAVE, top);
+  for (int top = hi; top > lo; top -= WAVE) {
+    const int cnt = min(WAVE, top - lo);
     const int base = top - cnt;  // batch covers [base, top)
     wave_sync();
     if (lane < cnt) {
@@ -611,6 +766,28 @@ __global__ void __launch_bounds__(WAVE* WPB) PF_BWD_OCC raster_bwd_kernel(BwdPar
       }
       const bool active = shade && dt > 0.0f;
       if (ballot(active) == 0) continue;
+      if (DSUM) {
+        // the dL/dlog T increments of the adjoint below, nothing else
+        if (active) {
+          const float dlt = -sigma * dt;
+          log_t -= dlt;
+          const float alpha = 1.0f - fast_exp(dlt);
+          const float trans = fast_exp(log_t);
+          const float ndv = dpn;
+          const float ndt = exact ? ndv * ndv : ndv;
+          float dLdtrans = dot(g_rgb, col) * alpha;
+          if (ndv > 0.0f) dLdtrans += g_nd * ndt * alpha;
+          dLdtrans += p.g_contrib[sr.site] * alpha * p.inv_hw;
+          dLdtrans += dot(g_n, pn) * alpha;
+          dLdlog_t += dLdtrans * trans;
+          while (qidx >= 0 && trans > qcur) {
+            dLdlog_t += p.g_qdepth[px * p.Q + qidx] / (sigma + q_eps);
+            qidx--;
+            if (qidx >= 0) qcur = p.depth_q[px * p.Q + qidx];
+          }
+        }
+        continue;
+      }
 
       // ---- compositing adjoint (rasterize.py:1366-1430)
       float dLdtn = 0.0f, dLdtf = 0.0f, dLdr = 0.0f, dLdsig = 0.0f;
@@ -893,6 +1070,7 @@ __global__ void __launch_bounds__(WAVE* WPB) PF_BWD_OCC raster_bwd_kernel(BwdPar
       }
     }
   }
+  if (DSUM) p.seg_dsum[(size_t)wi.seg * WAVE + lane] = dLdlog_t;
 }
 
 
@@ -1166,12 +1344,40 @@ int pf_raster_fwd(const pf_raster_fwd_args* a, void* stream) {
   p.err = a->err;
   p.contrib = a->contrib;
   p.point_err = a->point_err;
+  const pf_segments& sg = a->segs;
+  hipStream_t hs = as_stream(stream);
+  if (sg.n_segs > 0) {
+    if (sg.seg_len < WAVE || sg.seg_len % WAVE || !sg.tile_first || !sg.seg_tile || !sg.order || !sg.log_t_sums ||
+        !sg.lane_active || !sg.scratch || sg.n_segs < nt)
+      return set_error(PF_ERR_ARG, "pf_raster_fwd: bad segment table");
+    p.n_segs = sg.n_segs;
+    p.seg_len = sg.seg_len;
+    p.seg_first = sg.tile_first;
+    p.seg_tile = sg.seg_tile;
+    p.seg_order = sg.order;
+    p.seg_logt = sg.log_t_sums;
+    p.seg_final = sg.scratch;
+    p.seg_active = sg.lane_active;
+    fill_int_kernel<<<(nt + 255) / 256, 256, 0, hs>>>(a->tile_early_stop, nt, 0x7fffffff);
+    const int grid = (sg.n_segs + WPB - 1) / WPB;
+    if (c.ray_maps) {
+      raster_fwd_kernel<false, true><<<grid, WAVE * WPB, 0, hs>>>(p);
+      raster_fwd_kernel<false, false><<<grid, WAVE * WPB, 0, hs>>>(p);
+      raster_fwd_finish_kernel<false><<<nt, WAVE, 0, hs>>>(p);
+    } else {
+      raster_fwd_kernel<true, true><<<grid, WAVE * WPB, 0, hs>>>(p);
+      raster_fwd_kernel<true, false><<<grid, WAVE * WPB, 0, hs>>>(p);
+      raster_fwd_finish_kernel<true><<<nt, WAVE, 0, hs>>>(p);
+    }
+    PF_HIP(hipGetLastError());
+    return PF_OK;
+  }
   const int groups = (nt + WPB - 1) / WPB;
   const int grid = ((groups + 7) / 8) * 8;
   if (c.ray_maps)
-    raster_fwd_kernel<false><<<grid, WAVE * WPB, 0, as_stream(stream)>>>(p);
+    raster_fwd_kernel<false, false><<<grid, WAVE * WPB, 0, hs>>>(p);
   else
-    raster_fwd_kernel<true><<<grid, WAVE * WPB, 0, as_stream(stream)>>>(p);
+    raster_fwd_kernel<true, false><<<grid, WAVE * WPB, 0, hs>>>(p);
   PF_HIP(hipGetLastError());
   return PF_OK;
 }
@@ -1222,12 +1428,37 @@ int pf_raster_bwd(const pf_raster_bwd_args* a, void* stream) {
   p.g_tsites = a->grad_texel_sites;
   p.g_trgb = a->grad_texel_rgb;
   p.g_theight = a->grad_texel_height;
+  const pf_segments& sg = a->segs;
+  hipStream_t hs = as_stream(stream);
+  if (sg.n_segs > 0) {
+    if (sg.seg_len < WAVE || sg.seg_len % WAVE || !sg.tile_first || !sg.seg_tile || !sg.order || !sg.log_t_sums ||
+        !sg.lane_active || !sg.scratch || sg.n_segs < nt)
+      return set_error(PF_ERR_ARG, "pf_raster_bwd: bad segment table");
+    p.n_segs = sg.n_segs;
+    p.seg_len = sg.seg_len;
+    p.seg_first = sg.tile_first;
+    p.seg_tile = sg.seg_tile;
+    p.seg_order = sg.order;
+    p.seg_logt = sg.log_t_sums;
+    p.seg_active = sg.lane_active;
+    p.seg_dsum = sg.scratch;
+    const int grid = (sg.n_segs + WPB - 1) / WPB;
+    if (c.ray_maps) {
+      raster_bwd_kernel<false, true><<<grid, WAVE * WPB, 0, hs>>>(p);
+      raster_bwd_kernel<false, false><<<grid, WAVE * WPB, 0, hs>>>(p);
+    } else {
+      raster_bwd_kernel<true, true><<<grid, WAVE * WPB, 0, hs>>>(p);
+      raster_bwd_kernel<true, false><<<grid, WAVE * WPB, 0, hs>>>(p);
+    }
+    PF_HIP(hipGetLastError());
+    return PF_OK;
+  }
   const int groups = (nt + WPB - 1) / WPB;
   const int grid = ((groups + 7) / 8) * 8;
   if (c.ray_maps)
-    raster_bwd_kernel<false><<<grid, WAVE * WPB, 0, as_stream(stream)>>>(p);
+    raster_bwd_kernel<false, false><<<grid, WAVE * WPB, 0, hs>>>(p);
   else
-    raster_bwd_kernel<true><<<grid, WAVE * WPB, 0, as_stream(stream)>>>(p);
+    raster_bwd_kernel<true, false><<<grid, WAVE * WPB, 0, hs>>>(p);
   PF_HIP(hipGetLastError());
   return PF_OK;
 }

@@ -87,6 +87,12 @@ class Rasterizer:
         # Start the tiles with the longest site lists first (the results do not
         # depend on the order; the kernels' tail gets shorter).
         self.longest_first = True
+        # Split site lists longer than this many sites into segments that run
+        # in parallel (None: one wave per list). A multiple of 64. The forward
+        # is bound by its longest lists and gains; the backward is bound by
+        # its total work, so by default it walks whole lists (split_backward).
+        self.seg_len = 512
+        self.split_backward = False
 
     def _launch(self, name, fn, *a):
         if self.event_log is None:
@@ -159,6 +165,38 @@ class Rasterizer:
             return None
         return torch.argsort(offsets[1:] - offsets[:-1], descending=True).to(torch.int32)
 
+    def segments(self, offsets):
+        """Segment table of the compositing kernels (pf_segments), or None."""
+        if not self.seg_len:
+            return None
+        L = int(self.seg_len)
+        nt = offsets.shape[0] - 1
+        dev = offsets.device
+        cnt = offsets[1:] - offsets[:-1]
+        nseg = torch.clamp((cnt + L - 1) // L, min=1)
+        first = torch.zeros(nt + 1, dtype=torch.int64, device=dev)
+        first[1:] = torch.cumsum(nseg, 0)
+        n_segs = int(first[-1].item())
+        seg_tile = torch.repeat_interleave(torch.arange(nt, device=dev), nseg, output_size=n_segs)
+        k = torch.arange(n_segs, device=dev) - first[seg_tile]
+        size = torch.clamp(cnt[seg_tile] - k * L, max=L)
+        order = torch.argsort(size, descending=True)
+        return dict(n_segs=n_segs, seg_len=L, tile_first=first.to(torch.int32), seg_tile=seg_tile.to(torch.int32),
+                    order=order.to(torch.int32),
+                    log_t_sums=torch.empty(n_segs * 64, dtype=torch.float32, device=dev),
+                    lane_active=torch.empty(n_segs * 64, dtype=torch.uint8, device=dev))
+
+    def _segs(self, sg, scratch=None):
+        out = _abi.PfSegments()
+        if sg is None:
+            return out
+        out.n_segs, out.seg_len = sg["n_segs"], sg["seg_len"]
+        out.tile_first, out.seg_tile = sg["tile_first"].data_ptr(), sg["seg_tile"].data_ptr()
+        out.order, out.log_t_sums = sg["order"].data_ptr(), sg["log_t_sums"].data_ptr()
+        out.lane_active = sg["lane_active"].data_ptr()
+        out.scratch = scratch.data_ptr() if scratch is not None else None
+        return out
+
     def _tiles(self, nt, offsets, tile_sites, order):
         return _abi.PfTileLists(nt, offsets.data_ptr(), tile_sites.data_ptr() if tile_sites.numel() else None,
                                 order.data_ptr() if order is not None else None)
@@ -195,7 +233,8 @@ class Rasterizer:
         dq = _f32c(depth_quantiles) if depth_quantiles is not None else None
 
         offsets, tile_sites, prim_visible_mask, nt = self.bin(camera, all_spheres)
-        order = self.tile_order(offsets)
+        sg = self.segments(offsets)
+        order = self.tile_order(offsets) if sg is None else None
         adjacency_diff = self.encode_adjacency(all_spheres, adjacency, adjacency_offsets)
 
         f32 = dict(dtype=torch.float32, device=dev)
@@ -214,6 +253,8 @@ class Rasterizer:
         a.scene = _scene_struct(all_spheres, all_nsigmas, all_texel_sites, all_texel_rgbh, adjacency,
                                 adjacency_offsets, None if self.exact_geometry else adjacency_diff)
         a.tiles = self._tiles(nt, offsets, tile_sites, order)
+        scratch = torch.empty(sg["n_segs"] * 64, **f32) if sg is not None else None
+        a.segs = self._segs(sg, scratch)
         a.depth_quantiles = dq.data_ptr() if dq is not None else None
         a.n_quantiles = dq.shape[-1] if dq is not None else 0
         a.ray_gt = ray_gt_c.data_ptr() if ray_gt_c is not None else None
@@ -236,6 +277,7 @@ class Rasterizer:
         grad_ctx.camera = camera
         grad_ctx.ray_maps = ray_maps
         grad_ctx.order = order
+        grad_ctx.segs = sg
         grad_ctx.save_for_backward(dq, all_spheres, all_nsigmas, all_texel_sites, all_texel_rgbh, adjacency,
                                    adjacency_offsets, adjacency_diff, tile_sites, offsets, tile_early_stop,
                                    ray_gt_c, color_out, log_t_out)
@@ -356,7 +398,13 @@ class Rasterizer:
         a.cam, ray_maps = self._camera(camera, grad_ctx.ray_maps)
         a.scene = _scene_struct(all_spheres, all_nsigmas, all_texel_sites, all_texel_rgbh, adjacency,
                                 adjacency_offsets, None if self.exact_geometry else adjacency_diff)
-        a.tiles = self._tiles(offsets.shape[0] - 1, offsets, tile_sites, grad_ctx.order)
+        sg = grad_ctx.segs if self.split_backward else None
+        order = grad_ctx.order
+        if sg is None and order is None:
+            order = self.tile_order(offsets)
+        a.tiles = self._tiles(offsets.shape[0] - 1, offsets, tile_sites, order)
+        a.segs = self._segs(sg, torch.empty(sg["n_segs"] * 64, dtype=torch.float32, device=dev)
+                            if sg is not None else None)
         a.depth_quantiles = dq.data_ptr() if dq is not None else None
         a.n_quantiles = dq.shape[-1] if dq is not None else 0
         a.tile_early_stop = tile_early_stop.data_ptr()
@@ -377,7 +425,7 @@ class Rasterizer:
         self._launch("raster_bwd", self.lib.pf_raster_bwd, _ref(a), st)
         for t in (spheres_grad, nsigmas_grad, texel_sites_grad, texel_rgb_grad, texel_height_grad):
             check(self.lib.pf_scrub_nonfinite(ptr(t), t.numel(), st))
-        del grad_ctx.rasterizer, grad_ctx.camera, grad_ctx.ray_maps, grad_ctx.order
+        del grad_ctx.rasterizer, grad_ctx.camera, grad_ctx.ray_maps, grad_ctx.order, grad_ctx.segs
         return (None, None, None, spheres_grad[:, :3], spheres_grad[:, 3], nsigmas_grad[:, 3],
                 nsigmas_grad[:, :3], texel_sites_grad, texel_rgb_grad, texel_height_grad, None, None, None, None)
 

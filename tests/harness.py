@@ -56,12 +56,13 @@ class Case:
 
 
 def small_case(seed=0, n=500, width=64, height=48, T=8, height_std=0.0, fov=60.0, eye=(0.0, 0.0, 3.0),
-               r_range=(0.02, 0.12), quantiles=None, extent=1.0, fisheye_fov=None):
+               r_range=(0.02, 0.12), quantiles=None, extent=1.0, fisheye_fov=None, density_scale=1.0):
     """Uniform scene with radii large enough that tiles hold many sites and
     faces (Cech adjacency), random per-texel colours, a random target.
     fisheye_fov: an equidistant fisheye of that field of view given as ray
     maps (width == height), binned through the cone tree."""
     s = synth.scene_uniform(seed, n=n, n_texels=T, height_std=height_std, r_range=r_range, extent=extent)
+    s["density"] = (s["density"] * density_scale).astype(np.float32)
     rng = np.random.default_rng(seed + 100)
     rgb = rng.uniform(0.0, 1.0, (n, T, 3)).astype(np.float32)
     adj, off = synth.cech_adjacency(s["points"], s["radii"])
@@ -96,14 +97,18 @@ def _args(T, is_pinhole=True):
 
 
 def run_gpu(case, dev, grads_in=None, exact_gradients=False, exact_geometry=False, return_point_err=True,
-            threshold=1e-3):
+            threshold=1e-3, seg_len="default", split_backward=False):
     """Forward (+ backward with the given output gradients) through the drop-in
-    Rasterizer. Returns numpy dicts."""
+    Rasterizer. Returns numpy dicts. seg_len: the Rasterizer's list-splitting
+    length (None: no splitting)."""
     import torch
     from powerfoam_amd.rasterize import Rasterizer
 
     T = case.texel_sites.shape[1]
     r = Rasterizer(_args(T, case.cam.ray_maps is None), dev)
+    if seg_len != "default":
+        r.seg_len = seg_len
+    r.split_backward = split_backward
     r.exact_gradients = exact_gradients
     r.exact_geometry = exact_geometry
     tt = lambda a, dt=torch.float32: torch.tensor(np.ascontiguousarray(a), dtype=dt, device=dev)

@@ -118,3 +118,40 @@ def test_visualize_matches_fp64_oracle(dev, cfg, opts):
         assert err < ABS_TOL, (k, err)
     assert np.array_equal(inter, ref["intersections"])
     assert (ref["depth"] != 0).mean() > 0.01
+
+
+SPLIT_CASES = [
+    dict(seed=3, n=400, width=48, height=40),
+    # dense, zoomed in: lists of 100-180 sites, tiles stop inside later segments
+    dict(seed=21, n=3000, width=40, height=32, r_range=(0.05, 0.2), fov=25.0, density_scale=5.0),
+    dict(seed=22, n=4000, width=48, height=40, r_range=(0.04, 0.15), quantiles=[0.9, 0.5, 0.1], fov=25.0,
+         density_scale=5.0, height_std=0.05),
+    dict(seed=12, n=1500, width=64, height=64, eye=(0.2, 0.1, 0.6), fisheye_fov=180.0, height_std=0.05),
+]
+
+
+@pytest.mark.parametrize("cfg", SPLIT_CASES, ids=[f"split{i}" for i in range(len(SPLIT_CASES))])
+@pytest.mark.parametrize("threshold", [1e-3, 0.3])
+@pytest.mark.parametrize("split_backward", [False, True], ids=["bwd-whole", "bwd-split"])
+def test_split_lists_match_unsplit_and_oracle(dev, cfg, threshold, split_backward):
+    """Lists split into 64-site segments: the forward, the early-stop indices
+    and the gradients equal the unsplit walk up to summation order and the
+    FP64 oracle within the parity bars; threshold 0.3 stops tiles inside
+    segments."""
+    case = harness.small_case(**cfg)
+    print(f"split case {cfg} thr={threshold}", flush=True)
+    gin = harness.random_grads(case, seed=cfg["seed"])
+    out_s, g_s = harness.run_gpu(case, dev, gin, threshold=threshold, seg_len=64, split_backward=split_backward)
+    out_u, g_u = harness.run_gpu(case, dev, gin, threshold=threshold, seg_len=None)
+    ref, gref = harness.run_oracle(case, gin, threshold=threshold)
+    for k in ("color", "opacity", "normal_distance", "normal"):
+        assert np.abs(out_s[k] - out_u[k]).max() < 2e-6, k
+        assert np.abs(out_s[k] - ref[k]).max() < ABS_TOL, k
+    if case.depth_quantiles is not None:
+        assert np.array_equal(out_s["quantile_depths"] == -1, ref["quantile_depths"] == -1)
+        assert np.abs(out_s["quantile_depths"] - ref["quantile_depths"]).max() < ABS_TOL
+    assert np.abs(out_s["contrib"] - out_u["contrib"]).max() < 1e-6
+    keep = harness.well_conditioned(gref, max_fraction=0.05)  # dense views graze more sites
+    for k in harness.GRAD_KEYS:
+        assert harness.rel_err(g_s[k], g_u[k]) < 1e-5, k
+        assert harness.rel_err(g_s[k][keep], gref[k][keep]) < GRAD_REL_TOL, k
