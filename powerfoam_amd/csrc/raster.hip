@@ -877,15 +877,17 @@ __global__ void __launch_bounds__(WAVE* WPB) PF_BWD_OCC raster_bwd_kernel(BwdPar
         x0 = ray.d * tq0 - co;
         f3 dLdx0{0.0f, 0.0f, 0.0f};
         gdp0_in = dLdheight * height;
+        // heights all zero: every term below is zero (ph.w = 0, height = 0)
+        if (heights) {
 #pragma unroll
-        for (int i = 0; i < PF_MAX_TEXELS; i++) {
-          if (i < T) {
-            const float4 ph = s_tex[i].pos_h;
-            const f3 df{x0.x - ph.x, x0.y - ph.y, x0.z - ph.z};
-            // heights all zero: the height stage queries the colour-stage point
-            const float w = heights ? fast_exp(dot(df, df) * kexp) : w1[i];
-            // the cut-off tests the colour-stage weight sum, as the reference
-            if (!(w * inv_ws < cut)) dLdx0 = dLdx0 - df * (kw * (w * inv_ws0) * (dLdheight * ph.w - gdp0_in));
+          for (int i = 0; i < PF_MAX_TEXELS; i++) {
+            if (i < T) {
+              const float4 ph = s_tex[i].pos_h;
+              const f3 df{x0.x - ph.x, x0.y - ph.y, x0.z - ph.z};
+              const float w = fast_exp(dot(df, df) * kexp);
+              // the cut-off tests the colour-stage weight sum, as the reference
+              if (!(w * inv_ws < cut)) dLdx0 = dLdx0 - df * (kw * (w * inv_ws0) * (dLdheight * ph.w - gdp0_in));
+            }
           }
         }
         const float dLdtq0 = dot(dLdx0, ray.d);
@@ -1047,16 +1049,21 @@ __global__ void __launch_bounds__(WAVE* WPB) PF_BWD_OCC raster_bwd_kernel(BwdPar
                   v[7 * k + 1] = sg.y;
                   v[7 * k + 2] = sg.z;
                 }
-                // height stage (texture.py:118-131)
-                const f3 df{x0.x - ph.x, x0.y - ph.y, x0.z - ph.z};
-                const float w = heights ? fast_exp(dot(df, df) * kexp) : w1[i];
-                if (!(w * inv_ws < cut)) {
-                  const float wn0 = w * inv_ws0;
-                  v[7 * k + 6] = wn0 * dLdheight;
-                  const f3 sg = df * (kw * wn0 * (dLdheight * ph.w - gdp0_in));
-                  v[7 * k + 0] += sg.x;
-                  v[7 * k + 1] += sg.y;
-                  v[7 * k + 2] += sg.z;
+                // height stage (texture.py:118-131); with all heights zero
+                // it queries the colour-stage point and its site terms vanish
+                if (heights) {
+                  const f3 df{x0.x - ph.x, x0.y - ph.y, x0.z - ph.z};
+                  const float w = fast_exp(dot(df, df) * kexp);
+                  if (!(w * inv_ws < cut)) {
+                    const float wn0 = w * inv_ws0;
+                    v[7 * k + 6] = wn0 * dLdheight;
+                    const f3 sg = df * (kw * wn0 * (dLdheight * ph.w - gdp0_in));
+                    v[7 * k + 0] += sg.x;
+                    v[7 * k + 1] += sg.y;
+                    v[7 * k + 2] += sg.z;
+                  }
+                } else if (!(w1[i] * inv_ws < cut)) {
+                  v[7 * k + 6] = (w1[i] * inv_ws0) * dLdheight;
                 }
               }
             }
