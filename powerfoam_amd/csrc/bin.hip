@@ -164,39 +164,92 @@ __global__ void __launch_bounds__(256) fill_i32_kernel(int32_t* p, int64_t n, in
   if (i < n) p[i] = v;
 }
 
+// Candidate tiles of a wave's sites, dealt out to its lanes. One lane per site
+// leaves a wave as slow as its largest footprint (a site near the camera spans
+// thousands of tiles, the median a handful), so each lane computes its site's
+// tile rectangle, the wave scans the rectangle sizes, and the lanes then take
+// the wave's candidates 64 at a time; a lane finds the site owning candidate k
+// by binary search over the scanned starts in LDS. Which lane tests which
+// (site, tile) pair changes only the order of the atomics below, and the
+// per-tile sort removes that order.
+struct WaveSite {
+  float uc, vc, r1, r2, vx, vy;
+  int ti0, tj0, nj, start;
+  uint32_t key;
+  int site;
+};
+
+template <class F>
+__device__ __forceinline__ void for_each_overlap(const BinCam& cam, WaveSite* ws, int lane, bool valid,
+                                                 const Footprint& fp, int site, F&& f) {
+  const int ni = valid ? max(fp.ti1 - fp.ti0 + 1, 0) : 0;
+  const int nj = valid ? max(fp.tj1 - fp.tj0 + 1, 0) : 0;
+  const int nc = ni * nj;
+  int incl = nc;
+#pragma unroll
+  for (int m = 1; m < WAVE; m <<= 1) {
+    const int v = __shfl_up(incl, m, WAVE);
+    if (lane >= m) incl += v;
+  }
+  const int total = __shfl(incl, WAVE - 1, WAVE);
+  wave_sync();
+  WaveSite& me = ws[lane];
+  me.uc = fp.obb.uc;
+  me.vc = fp.obb.vc;
+  me.r1 = fp.obb.r1;
+  me.r2 = fp.obb.r2;
+  me.vx = fp.obb.vx;
+  me.vy = fp.obb.vy;
+  me.ti0 = fp.ti0;
+  me.tj0 = fp.tj0;
+  me.nj = nj;
+  me.start = incl - nc;
+  me.key = fp.key;
+  me.site = site;
+  wave_sync();
+  for (int base = 0; base < total; base += WAVE) {
+    const int k = base + lane;
+    if (k < total) {
+      // largest l with start[l] <= k; it owns k because start[l + 1] > k
+      int l = 0;
+#pragma unroll
+      for (int step = WAVE / 2; step >= 1; step >>= 1)
+        if (ws[l + step].start <= k) l += step;
+      const WaveSite& o = ws[l];
+      const int local = k - o.start;
+      const int ti = o.ti0 + local / o.nj, tj = o.tj0 + local % o.nj;
+      const Obb obb{o.uc, o.vc, o.r1, o.r2, o.vx, o.vy};
+      if (tile_obb_test(ti, tj, cam, obb)) f(ti + tj * cam.th, o);
+    }
+  }
+}
+
 __global__ void __launch_bounds__(256)
     bin_count_kernel(BinCam cam, const float4* __restrict__ spheres, int n, int32_t* __restrict__ counts) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  Footprint fp;
-  if (!site_footprint(cam, spheres[s], fp)) return;
-  for (int ti = fp.ti0; ti <= fp.ti1; ti++)
-    for (int tj = fp.tj0; tj <= fp.tj1; tj++) {
-      if (!tile_obb_test(ti, tj, cam, fp.obb)) continue;
-      atomicAdd(&counts[ti + tj * cam.th + 1], 1);
-    }
+  __shared__ WaveSite s_ws[256];
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & (WAVE - 1);
+  Footprint fp{};
+  const bool valid = s < n && site_footprint(cam, spheres[s], fp);
+  for_each_overlap(cam, s_ws + (threadIdx.x & ~(WAVE - 1)), lane, valid, fp, s,
+                   [&](int t, const WaveSite&) { atomicAdd(&counts[t + 1], 1); });
 }
 
 __global__ void __launch_bounds__(256)
     bin_emit_kernel(BinCam cam, const float4* __restrict__ spheres, int n, const int64_t* __restrict__ offsets,
                     uint64_t* __restrict__ records, int32_t* __restrict__ cursor, uint8_t* __restrict__ visible) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  Footprint fp;
-  if (!site_footprint(cam, spheres[s], fp)) return;
-  const uint64_t rec = ((uint64_t)fp.key << 32) | (uint32_t)s;
-  bool any = false;
-  for (int ti = fp.ti0; ti <= fp.ti1; ti++)
-    for (int tj = fp.tj0; tj <= fp.tj1; tj++) {
-      if (!tile_obb_test(ti, tj, cam, fp.obb)) continue;
-      int t = ti + tj * cam.th;
-      int64_t idx = offsets[t] + (int64_t)atomicAdd(&cursor[t], 1);
-      if (idx < offsets[t + 1]) {
-        records[idx] = rec;
-        any = true;
-      }
+  __shared__ WaveSite s_ws[256];
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & (WAVE - 1);
+  Footprint fp{};
+  const bool valid = s < n && site_footprint(cam, spheres[s], fp);
+  for_each_overlap(cam, s_ws + (threadIdx.x & ~(WAVE - 1)), lane, valid, fp, s, [&](int t, const WaveSite& o) {
+    const int64_t idx = offsets[t] + (int64_t)atomicAdd(&cursor[t], 1);
+    if (idx < offsets[t + 1]) {
+      records[idx] = ((uint64_t)o.key << 32) | (uint32_t)o.site;
+      if (visible) visible[o.site] = 1;
     }
-  if (any && visible) visible[s] = 1;
+  });
 }
 
 // ------------------------------------------------ non-pinhole cameras

@@ -13,15 +13,12 @@
 // accumulate in FP32 (v_mfma_f32_16x16x32_bf16); biases are FP32.
 //
 // MFMA use. The layers are computed transposed, H^T = W X^T, so each weight
-// matrix is the A operand (16 output units x 32 inputs, a 16-byte row slice
-// per lane from LDS) and the activations are the B operand (32 inputs x 16
-// rows). For the 16x16x32 instruction lane l holds A[l%16][8(l/16)+j],
-// B[8(l/16)+j][l%16] and D[4(l/16)+j][l%16], j = 0..7 (A, B) or 0..3 (D): a D
-// tile gives each lane 4 consecutive units of one row. The forward chains
-// layers through a row-major [row][unit] LDS tile (one 8-byte write, one
-// 16-byte read per lane and layer). The backward keeps [unit][row] tiles
-// instead, because its weight gradients contract over rows (K = 32 rows per
-// step) and need 8 consecutive rows per lane.
+// matrix is the A operand (16 output units x 32 inputs) and the activations
+// are the B operand (32 inputs x 16 rows). For the 16x16x32 instruction lane
+// l = 16g + c16 holds A[c16][8g+j], B[8g+j][c16] and D[4g+j][c16], j = 0..7
+// (A, B) or 0..3 (D): a D tile gives each lane 4 consecutive units of one
+// row. (The 16x16x16 instruction, used for transposes, holds A[c16][4g+j],
+// B[4g+j][c16], D[4g+j][c16], j = 0..3.)
 #include "pf_common.hpp"
 #include "sh_basis.hpp"
 
@@ -31,10 +28,7 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int ML_LAT = 16, ML_IN = 32, ML_HID = 64;
-constexpr int ML_WPB = 4;           // waves per workgroup
-constexpr int FW_RS = ML_HID + 8;   // forward [row][unit] stride (bf16): 144 B rows
-constexpr int BW_ROWS = 32;         // backward rows per step
-constexpr int BW_RS = BW_ROWS + 8;  // backward [unit][row] stride (bf16): 80 B rows
+constexpr int ML_WPB = 4;  // forward waves per workgroup
 
 struct MlpParams {
   int n_rows, n_sites, T;
@@ -78,295 +72,394 @@ __device__ __forceinline__ bool row_ok(const MlpParams& p, int row) {
   return site >= 0 && site < p.n_sites;
 }
 
-// Input slice x[row][8g .. 8g+7]: g = 0, 1 latent halves; g = 2, 3 encoding halves.
-__device__ __forceinline__ bf16x8 input_slice(const MlpParams& p, int row, int g) {
-  bf16x8 v;
-  const int site = row >= p.n_rows ? -1 : (p.pair_sites ? p.pair_sites[row] : row / p.T);
-  if (site < 0 || site >= p.n_sites) {
-#pragma unroll
-    for (int j = 0; j < 8; j++) v[j] = static_cast<__bf16>(0.0f);
-    return v;
-  }
-  if (g < 2) return ld8(p.latents + (size_t)site * ML_LAT + 8 * g);
-  f3 d;
-  if (p.pair_sites) {
-    d = ld3(p.pair_dirs + 3 * (size_t)row);
-    const float l = sqrtf(dot(d, d));
-    d = l > 0.0f ? d * (1.0f / l) : f3{0.0f, 0.0f, 0.0f};
-  } else {
-    d = texel_dir(p.texel_sites + 3 * (size_t)row, p.eye);
-  }
-  float y[15];
-  sh_basis(d, y);
-  float e[16];
-  e[0] = 1.0f;
-#pragma unroll
-  for (int k = 0; k < 15; k++) e[k + 1] = y[k];
-#pragma unroll
-  for (int j = 0; j < 8; j++) v[j] = static_cast<__bf16>(g == 2 ? e[j] : e[8 + j]);
-  return v;
-}
-
-// Weights (and, for the backward, their transposes) staged once per workgroup.
-struct alignas(16) WeightsLds {
-  uint16_t w1[ML_HID * ML_IN];   // [h1][in]
-  uint16_t w2[ML_HID * ML_HID];  // [h2][h1]
-  uint16_t w3[16 * ML_HID];      // [out][h2], out rows 3..15 zero
-  float b1[ML_HID], b2[ML_HID], b3[4];
-};
-
-__device__ void stage_weights(WeightsLds& s, const MlpParams& p) {
-  for (int i = threadIdx.x; i < ML_HID * ML_IN; i += blockDim.x) s.w1[i] = p.w1[i];
-  for (int i = threadIdx.x; i < ML_HID * ML_HID; i += blockDim.x) s.w2[i] = p.w2[i];
-  for (int i = threadIdx.x; i < 16 * ML_HID; i += blockDim.x) s.w3[i] = i < 3 * ML_HID ? p.w3[i] : (uint16_t)0;
-  for (int i = threadIdx.x; i < ML_HID; i += blockDim.x) {
-    s.b1[i] = p.b1[i];
-    s.b2[i] = p.b2[i];
-  }
-  if (threadIdx.x < 4) s.b3[threadIdx.x] = threadIdx.x < 3 ? p.b3[threadIdx.x] : 0.0f;
-}
-
 // ---------------------------------------------------------------- forward
+// The forward chains the three products in registers. A D tile holds, per
+// lane, units 16t + 4g + j (j = 0..3) of row c16; two such tiles (t = 2q,
+// 2q + 1) packed to bf16 are exactly a B operand over the K-slot order
+//   slot 8g + j  <->  unit 32q + 4g + j         (j < 4)
+//                     unit 32q + 16 + 4g + j-4  (j >= 4),
+// so the next layer's A operand (its weights) is loaded with its columns in
+// that order and no activation goes through LDS. Weights stay in registers
+// for the whole persistent launch; biases ride in as the MFMA accumulator.
+// The inputs of 64 rows are built one row per lane (latent copy, SH encoding)
+// and staged in LDS, from where each 16-row tile reads its B operand.
+constexpr int FW_ROWS = 64;  // rows per wave step
+constexpr int XS = 40;       // staged input row stride (bf16): 80 B, conflict-free 16-byte reads
+
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint32_t pk_bf(float a, float b) {
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){a, b}, bf16x2));
+}
+
+// relu of two packed bf16: a bf16 is negative exactly when its bits are as int16
+__device__ __forceinline__ uint32_t relu_pk(uint32_t v) {
+  return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(s16x2, v), (s16x2){0, 0}));
+}
+
+// bf16(relu(z)) of two D tiles as one B operand (slot order above)
+__device__ __forceinline__ bf16x8 act_op(f32x4 lo, f32x4 hi) {
+  const uint4 u = {relu_pk(pk_bf(lo[0], lo[1])), relu_pk(pk_bf(lo[2], lo[3])), relu_pk(pk_bf(hi[0], hi[1])),
+                   relu_pk(pk_bf(hi[2], hi[3]))};
+  return __builtin_bit_cast(bf16x8, u);
+}
+
+// A operand of a 64-column weight row in slot order: columns 32q + 4g .. +3
+// and 32q + 16 + 4g .. +3 of row `row`.
+__device__ __forceinline__ bf16x8 ld_wrow(const uint16_t* w, int row, int q, int g) {
+  const uint2 a = *reinterpret_cast<const uint2*>(w + row * ML_HID + 32 * q + 4 * g);
+  const uint2 b = *reinterpret_cast<const uint2*>(w + row * ML_HID + 32 * q + 16 + 4 * g);
+  return __builtin_bit_cast(bf16x8, make_uint4(a.x, a.y, b.x, b.y));
+}
+
+// Input row x = [latent ; e(d)] of `row` as 32 bf16 at dst (16-byte aligned);
+// zero for rows past the end or naming no valid site.
+__device__ __forceinline__ void stage_row(const MlpParams& p, int row, uint16_t* dst) {
+  uint4 q0 = {0u, 0u, 0u, 0u}, q1 = q0, q2 = q0, q3 = q0;
+  const int site = row >= p.n_rows ? -1 : (p.pair_sites ? p.pair_sites[row] : row / p.T);
+  if (site >= 0 && site < p.n_sites) {
+    const uint4* lp = reinterpret_cast<const uint4*>(p.latents + (size_t)site * ML_LAT);
+    q0 = lp[0];
+    q1 = lp[1];
+    f3 d;
+    if (p.pair_sites) {
+      d = ld3(p.pair_dirs + 3 * (size_t)row);
+      const float l = sqrtf(dot(d, d));
+      d = l > 0.0f ? d * (1.0f / l) : f3{0.0f, 0.0f, 0.0f};
+    } else {
+      d = texel_dir(p.texel_sites + 3 * (size_t)row, p.eye);
+    }
+    float y[15];
+    sh_basis(d, y);
+    q2 = make_uint4(pk_bf(1.0f, y[0]), pk_bf(y[1], y[2]), pk_bf(y[3], y[4]), pk_bf(y[5], y[6]));
+    q3 = make_uint4(pk_bf(y[7], y[8]), pk_bf(y[9], y[10]), pk_bf(y[11], y[12]), pk_bf(y[13], y[14]));
+  }
+  uint4* o = reinterpret_cast<uint4*>(dst);
+  o[0] = q0;
+  o[1] = q1;
+  o[2] = q2;
+  o[3] = q3;
+}
+
 __global__ void __launch_bounds__(WAVE* ML_WPB) mlp_fwd_kernel(MlpParams p) {
-  __shared__ WeightsLds sw;
-  __shared__ alignas(16) uint16_t s_h_all[ML_WPB][2][16 * FW_RS];
-  stage_weights(sw, p);
-  __syncthreads();
+  __shared__ alignas(16) uint16_t s_x_all[ML_WPB][FW_ROWS * XS];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int g = lane >> 4, c16 = lane & 15;
-  uint16_t* sh1 = s_h_all[wv][0];
-  uint16_t* sh2 = s_h_all[wv][1];
-  const int n_tiles = (p.n_rows + 15) / 16;
-  for (int tile = blockIdx.x * ML_WPB + wv; tile < n_tiles; tile += gridDim.x * ML_WPB) {
-    const int row = tile * 16 + c16;
-    const bf16x8 x = input_slice(p, row, g);
-    // layer 1: H1^T tile t = W1[16t.., :] x^T
-    wave_sync();
+  uint16_t* sx = s_x_all[wv];
+  bf16x8 w1[4], w2[4][2], w3[2];
+  f32x4 b1[4], b2[4], b3;
 #pragma unroll
-    for (int t = 0; t < 4; t++) {
-      const f32x4 a = mfma(ld8(sw.w1 + (16 * t + c16) * ML_IN + 8 * g), x, zero4());
-      uint16_t o[4];
+  for (int t = 0; t < 4; t++) {
+    w1[t] = ld8(p.w1 + (16 * t + c16) * ML_IN + 8 * g);
 #pragma unroll
-      for (int j = 0; j < 4; j++) o[j] = to_bf(fmaxf(a[j] + sw.b1[16 * t + 4 * g + j], 0.0f));
-      *reinterpret_cast<uint2*>(sh1 + c16 * FW_RS + 16 * t + 4 * g) =
-          make_uint2(o[0] | ((uint32_t)o[1] << 16), o[2] | ((uint32_t)o[3] << 16));
+    for (int q = 0; q < 2; q++) w2[t][q] = ld_wrow(p.w2, 16 * t + c16, q, g);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      b1[t][j] = p.b1[16 * t + 4 * g + j];
+      b2[t][j] = p.b2[16 * t + 4 * g + j];
     }
-    wave_sync();
-    // layer 2
-    const bf16x8 h1a = ld8(sh1 + c16 * FW_RS + 8 * g), h1b = ld8(sh1 + c16 * FW_RS + 32 + 8 * g);
+  }
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
-      f32x4 a = mfma(ld8(sw.w2 + (16 * u + c16) * ML_HID + 8 * g), h1a, zero4());
-      a = mfma(ld8(sw.w2 + (16 * u + c16) * ML_HID + 32 + 8 * g), h1b, a);
-      uint16_t o[4];
+  for (int q = 0; q < 2; q++) w3[q] = c16 < 3 ? ld_wrow(p.w3, c16, q, g) : __builtin_bit_cast(bf16x8, make_uint4(0u, 0u, 0u, 0u));
 #pragma unroll
-      for (int j = 0; j < 4; j++) o[j] = to_bf(fmaxf(a[j] + sw.b2[16 * u + 4 * g + j], 0.0f));
-      *reinterpret_cast<uint2*>(sh2 + c16 * FW_RS + 16 * u + 4 * g) =
-          make_uint2(o[0] | ((uint32_t)o[1] << 16), o[2] | ((uint32_t)o[3] << 16));
-    }
+  for (int j = 0; j < 4; j++) b3[j] = (g == 0 && j < 3) ? p.b3[j] : 0.0f;
+
+  const int n_steps = (p.n_rows + FW_ROWS - 1) / FW_ROWS;
+  for (int step = blockIdx.x * ML_WPB + wv; step < n_steps; step += gridDim.x * ML_WPB) {
+    const int base = step * FW_ROWS;
     wave_sync();
-    // layer 3: lanes of group 0 hold the 3 outputs of their row
-    f32x4 a = mfma(ld8(sw.w3 + c16 * ML_HID + 8 * g), ld8(sh2 + c16 * FW_RS + 8 * g), zero4());
-    a = mfma(ld8(sw.w3 + c16 * ML_HID + 32 + 8 * g), ld8(sh2 + c16 * FW_RS + 32 + 8 * g), a);
-    if (g == 0 && row < p.n_rows) {
-      // rows naming no valid site get the biases' colour of a zero input
-      float* o = p.rgb + 3 * (size_t)row;
-      o[0] = fmaxf(a[0] + sw.b3[0], 0.0f);
-      o[1] = fmaxf(a[1] + sw.b3[1], 0.0f);
-      o[2] = fmaxf(a[2] + sw.b3[2], 0.0f);
+    stage_row(p, base + lane, sx + lane * XS);
+    wave_sync();
+#pragma unroll
+    for (int tau = 0; tau < FW_ROWS / 16; tau++) {
+      const int r = 16 * tau + c16;
+      const bf16x8 x = ld8(sx + r * XS + 8 * g);
+      f32x4 z1[4], z2[4];
+#pragma unroll
+      for (int t = 0; t < 4; t++) z1[t] = mfma(w1[t], x, b1[t]);
+      const bf16x8 h1[2] = {act_op(z1[0], z1[1]), act_op(z1[2], z1[3])};
+#pragma unroll
+      for (int u = 0; u < 4; u++) z2[u] = mfma(w2[u][1], h1[1], mfma(w2[u][0], h1[0], b2[u]));
+      const bf16x8 h2[2] = {act_op(z2[0], z2[1]), act_op(z2[2], z2[3])};
+      const f32x4 o = mfma(w3[1], h2[1], mfma(w3[0], h2[0], b3));
+      const int row = base + r;
+      // lanes of group 0 hold outputs 0..2 of their row; rows naming no valid
+      // site get the biases' colour of a zero input
+      if (g == 0 && row < p.n_rows) {
+        float* out = p.rgb + 3 * (size_t)row;
+        out[0] = fmaxf(o[0], 0.0f);
+        out[1] = fmaxf(o[1], 0.0f);
+        out[2] = fmaxf(o[2], 0.0f);
+      }
     }
   }
 }
 
 // --------------------------------------------------------------- backward
-struct alignas(16) WeightsTLds {
-  uint16_t w3t[ML_HID * 32];     // [h2][out], out 3..31 zero (K padded to 32)
-  uint16_t w2t[ML_HID * ML_HID];  // [h1][h2]
-  uint16_t w1t[ML_LAT * ML_HID];  // [in < 16][h1]
-};
+// One wave step is 32 rows, two 16-row halves. Per half, in registers:
+//   forward again (as above), keeping z1, z2 for the relu masks;
+//   dOut  = [o > 0] grad_rgb                    (lanes of group 0)
+//   dH2^T = W3^T dOut^T,  dZ2 = [z2 > 0] dH2
+//   dH1^T = W2^T dZ2^T,   dZ1 = [z1 > 0] dH1
+//   dX^T  = W1[:, :16]^T dZ1^T                  (latent inputs only)
+// all in the transposed form, with the same K-slot order as the forward.
+// The weight gradients dW1 = dZ1^T X, dW2 = dZ2^T H1, dW3 = dOut^T H2
+// contract over rows, so each operand needs 8 rows of one unit per lane,
+// while a D tile holds 4 units of one row. An MFMA against a 0/1 selection
+// matrix transposes a tile exactly (bf16 values times one, summed with
+// zeros), giving 4 rows of one unit per lane; the two halves' tiles together
+// are the 8 rows, K slot 8g + j being row 4g + j of half 0 for j < 4 and
+// row 16 + 4g + j - 4 of half 1 for j >= 4. The weight gradients accumulate
+// in registers over the whole persistent launch; at the end the workgroup's
+// waves sum them in LDS and issue one atomic per weight.
+// All A operands (weights, their transposes) sit in LDS as per-lane images:
+// one conflict-free 16-byte read each.
+constexpr int BW_WPB = 4;  // waves per workgroup (one per SIMD: the backward needs ~350 registers)
+constexpr int BW_ROWS = 32;
+enum { IM_W1 = 0, IM_W2 = 4, IM_W3 = 12, IM_W3T = 14, IM_W2T = 18, IM_W1T = 26, IM_N = 28 };
 
-// One wave's [unit][row] tiles for 32 rows.
-struct alignas(16) BwdTiles {
-  uint16_t x[ML_IN * BW_RS];
-  uint16_t h1[ML_HID * BW_RS];
-  uint16_t h2[ML_HID * BW_RS];
-  uint16_t dout[16 * BW_RS];
-  uint16_t dh1[ML_HID * BW_RS];
-  uint16_t dh2[ML_HID * BW_RS];
-  float dlat[BW_ROWS * ML_LAT];
-};
+typedef short s16x4 __attribute__((ext_vector_type(4)));
 
-// B operand slice for row `r` (tile-local) from a [unit][row] tile: units k0..k0+7.
-__device__ __forceinline__ bf16x8 col8(const uint16_t* t, int k0, int r) {
-  bf16x8 v;
-#pragma unroll
-  for (int j = 0; j < 8; j++) v[j] = __builtin_bit_cast(__bf16, t[(k0 + j) * BW_RS + r]);
-  return v;
+// K slot 8g + j of chunk q <-> unit 32q + 16(j / 4) + 4g + j % 4 (forward order)
+__device__ __forceinline__ int slot_unit(int q, int g, int j) { return 32 * q + 16 * (j >> 2) + 4 * g + (j & 3); }
+
+// entry i of the A-operand images: tile i / 512, lane (i / 8) % 64, element i % 8
+__device__ __forceinline__ uint16_t image_entry(const MlpParams& p, int i) {
+  const int tile = i >> 9, lane = (i >> 3) & 63, j = i & 7;
+  const int g = lane >> 4, c = lane & 15;
+  if (tile < IM_W2) return p.w1[(16 * tile + c) * ML_IN + 8 * g + j];
+  if (tile < IM_W3) {
+    const int u = (tile - IM_W2) >> 1, q = (tile - IM_W2) & 1;
+    return p.w2[(16 * u + c) * ML_HID + slot_unit(q, g, j)];
+  }
+  if (tile < IM_W3T) return c < 3 ? p.w3[c * ML_HID + slot_unit(tile - IM_W3, g, j)] : (uint16_t)0;
+  if (tile < IM_W2T) return (g == 0 && j < 3) ? p.w3[j * ML_HID + 16 * (tile - IM_W3T) + c] : (uint16_t)0;
+  if (tile < IM_W1T) {
+    const int v = (tile - IM_W2T) >> 1, q = (tile - IM_W2T) & 1;
+    return p.w2[slot_unit(q, g, j) * ML_HID + 16 * v + c];
+  }
+  return p.w1[slot_unit(tile - IM_W1T, g, j) * ML_IN + c];
 }
 
-// D tile (4 consecutive units u0..u0+3 of row r) into a [unit][row] tile.
-__device__ __forceinline__ void put4(uint16_t* t, int u0, int r, const float* v) {
-#pragma unroll
-  for (int j = 0; j < 4; j++) t[(u0 + j) * BW_RS + r] = to_bf(v[j]);
+struct alignas(16) BwdShared {
+  union {
+    uint16_t im[IM_N * 512];             // A-operand images
+    float red[ML_HID * ML_IN + ML_HID * ML_HID + 3 * ML_HID + 2 * ML_HID + 4];  // end-of-launch sums
+  };
+  float b1[ML_HID], b2[ML_HID], b3[4];
+  uint16_t x[BW_WPB][BW_ROWS * XS];     // staged input rows
+  float dlat[BW_WPB][BW_ROWS * ML_LAT];  // latent gradient rows
+};
+
+__device__ __forceinline__ bf16x8 im_ld(const BwdShared& sm, int tile, int lane) { return ld8(sm.im + tile * 512 + lane * 8); }
+
+__device__ __forceinline__ f32x4 ld_f4(const float* p) { return *reinterpret_cast<const f32x4*>(p); }
+
+// [h > 0] d for the D tile t of an activation operand pair (tiles 2q, 2q + 1).
+// The relu masks are read from the bf16 activations, which are zero exactly
+// where z <= 0 (and for positive z below the smallest bf16 normal, where the
+// activation itself is zero).
+__device__ __forceinline__ f32x4 mask_tile(bf16x8 hop, int t, f32x4 d) {
+  const uint4 u = __builtin_bit_cast(uint4, hop);
+  const uint32_t a = (t & 1) ? u.z : u.x, b = (t & 1) ? u.w : u.y;
+  f32x4 r;
+  r[0] = (a & 0xFFFFu) ? d[0] : 0.0f;
+  r[1] = (a >> 16) ? d[1] : 0.0f;
+  r[2] = (b & 0xFFFFu) ? d[2] : 0.0f;
+  r[3] = (b >> 16) ? d[3] : 0.0f;
+  return r;
 }
 
-__global__ void __launch_bounds__(WAVE* ML_WPB) mlp_bwd_kernel(MlpParams p) {
-  __shared__ WeightsLds sw;
-  __shared__ WeightsTLds st;
-  __shared__ BwdTiles s_all[ML_WPB];
-  stage_weights(sw, p);
-  for (int i = threadIdx.x; i < ML_HID * 32; i += blockDim.x) {
-    const int h2 = i / 32, o = i % 32;
-    st.w3t[i] = o < 3 ? p.w3[o * ML_HID + h2] : (uint16_t)0;
+__device__ __forceinline__ bf16x8 pack_op(f32x4 lo, f32x4 hi) {
+  return __builtin_bit_cast(bf16x8, make_uint4(pk_bf(lo[0], lo[1]), pk_bf(lo[2], lo[3]), pk_bf(hi[0], hi[1]), pk_bf(hi[2], hi[3])));
+}
+
+// Transpose of a 16x16 bf16 tile given as the D-layout quadruple of this lane
+// (2 packed dwords): returns, packed, the 4 rows 4g .. 4g+3 of column c16.
+__device__ __forceinline__ uint2 tr16(uint32_t lo, uint32_t hi, s16x4 sel) {
+  const s16x4 a = __builtin_bit_cast(s16x4, make_uint2(lo, hi));
+  const f32x4 d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, sel, zero4(), 0, 0, 0);
+  return make_uint2(pk_bf(d[0], d[1]), pk_bf(d[2], d[3]));
+}
+
+// B operand selecting input columns 16b .. 16b + 15: slot 8g + j is 1 where 8g + j = 16b + c16
+__device__ __forceinline__ bf16x8 sel_x(int g, int c16, int b) {
+  const int j = 16 * b + c16 - 8 * g;  // the slot of this lane that is one, if 0 <= j < 8
+  const uint32_t one = 0x3F80u;
+  uint4 u = {0u, 0u, 0u, 0u};
+  const uint32_t v = (j & 1) ? (one << 16) : one;
+  if (j >= 0 && j < 8) {
+    if ((j >> 1) == 0) u.x = v;
+    else if ((j >> 1) == 1) u.y = v;
+    else if ((j >> 1) == 2) u.z = v;
+    else u.w = v;
   }
-  for (int i = threadIdx.x; i < ML_HID * ML_HID; i += blockDim.x) {
-    const int h1 = i / ML_HID, h2 = i % ML_HID;
-    st.w2t[i] = p.w2[h2 * ML_HID + h1];
+  return __builtin_bit_cast(bf16x8, u);
+}
+
+__device__ __forceinline__ uint4 dw(bf16x8 v) { return __builtin_bit_cast(uint4, v); }
+
+__device__ __forceinline__ bf16x8 unit_op(uint2 h0, uint2 h1) { return __builtin_bit_cast(bf16x8, make_uint4(h0.x, h0.y, h1.x, h1.y)); }
+
+// One half's transposed operands (4 rows of one unit per lane, packed)
+struct HalfT {
+  uint2 h1[4], h2[4], z2[4], z1[4], x[2], o;
+};
+
+__global__ void __launch_bounds__(WAVE* BW_WPB) mlp_bwd_kernel(MlpParams p) {
+  __shared__ BwdShared sm;
+  for (int i = threadIdx.x; i < IM_N * 512; i += blockDim.x) sm.im[i] = image_entry(p, i);
+  for (int i = threadIdx.x; i < ML_HID; i += blockDim.x) {
+    sm.b1[i] = p.b1[i];
+    sm.b2[i] = p.b2[i];
   }
-  for (int i = threadIdx.x; i < ML_LAT * ML_HID; i += blockDim.x) {
-    const int in = i / ML_HID, h1 = i % ML_HID;
-    st.w1t[i] = p.w1[h1 * ML_IN + in];
-  }
-  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int g = lane >> 4, c16 = lane & 15;
-  BwdTiles& s = s_all[wv];
-  // output rows 3..15 of dout stay zero
-  for (int i = lane; i < 16 * BW_RS; i += WAVE) s.dout[i] = 0;
+  if (threadIdx.x < 4) sm.b3[threadIdx.x] = threadIdx.x < 3 ? p.b3[threadIdx.x] : 0.0f;
   __syncthreads();
 
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int g = lane >> 4, c16 = lane & 15;
+  uint16_t* sx = sm.x[wv];
+  float* sdl = sm.dlat[wv];
+  // selection operands: transpose (16x16x16), and input columns 16b.. (16x16x32)
+  s16x4 sel_t;
+#pragma unroll
+  for (int j = 0; j < 4; j++) sel_t[j] = (c16 - 4 * g == j) ? (short)0x3F80 : (short)0;
+
   f32x4 gw1[4][2], gw2[4][4], gw3[4];
+  f32x4 gb1[4], gb2[4];
+  float gb3[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
   for (int a = 0; a < 4; a++) {
-    gw3[a] = zero4();
+    gw3[a] = gb1[a] = gb2[a] = zero4();
 #pragma unroll
     for (int b = 0; b < 2; b++) gw1[a][b] = zero4();
 #pragma unroll
     for (int b = 0; b < 4; b++) gw2[a][b] = zero4();
   }
-  float gb1[4][4], gb2[4][4], gb3[4];
-#pragma unroll
-  for (int a = 0; a < 4; a++) {
-    gb3[a] = 0.0f;
-#pragma unroll
-    for (int j = 0; j < 4; j++) gb1[a][j] = gb2[a][j] = 0.0f;
-  }
 
   const int n_steps = (p.n_rows + BW_ROWS - 1) / BW_ROWS;
-  for (int step = blockIdx.x * ML_WPB + wv; step < n_steps; step += gridDim.x * ML_WPB) {
+  for (int step = blockIdx.x * BW_WPB + wv; step < n_steps; step += gridDim.x * BW_WPB) {
     const int base = step * BW_ROWS;
-#pragma unroll 1
-    for (int half = 0; half < 2; half++) {
-      const int r = 16 * half + c16;  // tile-local row of this lane's column
+    wave_sync();
+    if (lane < BW_ROWS) stage_row(p, base + lane, sx + lane * XS);
+    wave_sync();
+    HalfT T0;  // half 0's transposed operands, used with half 1's
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int r = 16 * h + c16;
       const int row = base + r;
-      const bf16x8 x = input_slice(p, row, g);
-      wave_sync();
-      // single-element reads of the bf16 vector returned element 0 for every
-      // index with this compiler, so the raw bits are split instead
-      const uint4 xw = __builtin_bit_cast(uint4, x);
-      const uint32_t xr[4] = {xw.x, xw.y, xw.z, xw.w};
+      HalfT T;
+      const bf16x8 x = ld8(sx + r * XS + 8 * g);
 #pragma unroll
-      for (int j = 0; j < 8; j++) s.x[(8 * g + j) * BW_RS + r] = (uint16_t)(xr[j >> 1] >> (16 * (j & 1)));
-      // recompute layer 1, keeping pre-activations (D layout: units 16t + 4g + j of row r)
-      float z1[4][4], z2[4][4];
-#pragma unroll
-      for (int t = 0; t < 4; t++) {
-        const f32x4 a = mfma(ld8(sw.w1 + (16 * t + c16) * ML_IN + 8 * g), x, zero4());
-        float h[4];
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-          z1[t][j] = a[j] + sw.b1[16 * t + 4 * g + j];
-          h[j] = fmaxf(z1[t][j], 0.0f);
-        }
-        put4(s.h1, 16 * t + 4 * g, r, h);
+      for (int b = 0; b < 2; b++) {
+        const f32x4 d = mfma(x, sel_x(g, c16, b), zero4());
+        T.x[b] = make_uint2(pk_bf(d[0], d[1]), pk_bf(d[2], d[3]));
       }
-      wave_sync();
-      const bf16x8 h1a = col8(s.h1, 8 * g, r), h1b = col8(s.h1, 32 + 8 * g, r);
+      bf16x8 h1[2], h2[2];
+      {
+        f32x4 z1[4];
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
-        f32x4 a = mfma(ld8(sw.w2 + (16 * u + c16) * ML_HID + 8 * g), h1a, zero4());
-        a = mfma(ld8(sw.w2 + (16 * u + c16) * ML_HID + 32 + 8 * g), h1b, a);
-        float h[4];
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-          z2[u][j] = a[j] + sw.b2[16 * u + 4 * g + j];
-          h[j] = fmaxf(z2[u][j], 0.0f);
-        }
-        put4(s.h2, 16 * u + 4 * g, r, h);
+        for (int t = 0; t < 4; t++) z1[t] = mfma(im_ld(sm, IM_W1 + t, lane), x, ld_f4(sm.b1 + 16 * t + 4 * g));
+        h1[0] = act_op(z1[0], z1[1]);
+        h1[1] = act_op(z1[2], z1[3]);
       }
-      wave_sync();
-      f32x4 o = mfma(ld8(sw.w3 + c16 * ML_HID + 8 * g), col8(s.h2, 8 * g, r), zero4());
-      o = mfma(ld8(sw.w3 + c16 * ML_HID + 32 + 8 * g), col8(s.h2, 32 + 8 * g, r), o);
-      // dL/d(pre-clamp output), lanes of group 0 (outputs 0..2 of row r)
+      {
+        f32x4 z2[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+          z2[u] = mfma(im_ld(sm, IM_W2 + 2 * u + 1, lane), h1[1],
+                       mfma(im_ld(sm, IM_W2 + 2 * u, lane), h1[0], ld_f4(sm.b2 + 16 * u + 4 * g)));
+        h2[0] = act_op(z2[0], z2[1]);
+        h2[1] = act_op(z2[2], z2[3]);
+      }
+      const f32x4 o = mfma(im_ld(sm, IM_W3 + 1, lane), h2[1], mfma(im_ld(sm, IM_W3, lane), h2[0], ld_f4(sm.b3)));
+      // dL/d(pre-clamp output): lanes of group 0, outputs 0..2 of row r
       float d_out[3] = {0.0f, 0.0f, 0.0f};
       if (g == 0 && row_ok(p, row)) {
 #pragma unroll
-        for (int j = 0; j < 3; j++)
-          d_out[j] = (o[j] + sw.b3[j] > 0.0f) ? p.grad_rgb[3 * (size_t)row + j] : 0.0f;
+        for (int j = 0; j < 3; j++) d_out[j] = o[j] > 0.0f ? p.grad_rgb[3 * (size_t)row + j] : 0.0f;
       }
-      bf16x8 dob;
 #pragma unroll
-      for (int j = 0; j < 8; j++) dob[j] = static_cast<__bf16>(j < 3 ? d_out[j] : 0.0f);
-      if (g == 0) {
+      for (int j = 0; j < 3; j++) gb3[j] += d_out[j];
+      const uint32_t do01 = pk_bf(d_out[0], d_out[1]), do23 = pk_bf(d_out[2], 0.0f);
+      const bf16x8 dob = __builtin_bit_cast(bf16x8, make_uint4(do01, do23, 0u, 0u));
+      T.o = tr16(do01, do23, sel_t);
 #pragma unroll
-        for (int j = 0; j < 3; j++) {
-          s.dout[j * BW_RS + r] = to_bf(d_out[j]);
-          gb3[j] += d_out[j];
+      for (int t = 0; t < 4; t++) {
+        const uint4 a1 = dw(h1[t >> 1]), a2 = dw(h2[t >> 1]);
+        T.h1[t] = (t & 1) ? tr16(a1.z, a1.w, sel_t) : tr16(a1.x, a1.y, sel_t);
+        T.h2[t] = (t & 1) ? tr16(a2.z, a2.w, sel_t) : tr16(a2.x, a2.y, sel_t);
+      }
+      if (h == 1) {
+#pragma unroll
+        for (int a = 0; a < 4; a++) gw3[a] = mfma(unit_op(T0.o, T.o), unit_op(T0.h2[a], T.h2[a]), gw3[a]);
+      }
+      // dZ2 = [z2 > 0] W3^T dOut
+      bf16x8 dz2op[2];
+      {
+        f32x4 dz2[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          dz2[u] = mask_tile(h2[u >> 1], u, mfma(im_ld(sm, IM_W3T + u, lane), dob, zero4()));
+          gb2[u] += dz2[u];
         }
+        dz2op[0] = pack_op(dz2[0], dz2[1]);
+        dz2op[1] = pack_op(dz2[2], dz2[3]);
       }
-      // dH2^T = W3^T dOut^T (K = outputs, padded to 32: only group 0 carries data)
-      float dz2[4][4];
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
-        const f32x4 a = mfma(ld8(st.w3t + (16 * u + c16) * 32 + 8 * g), dob, zero4());
+      for (int t = 0; t < 4; t++) {
+        const uint4 a3 = dw(dz2op[t >> 1]);
+        T.z2[t] = (t & 1) ? tr16(a3.z, a3.w, sel_t) : tr16(a3.x, a3.y, sel_t);
+      }
+      if (h == 1) {
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-          dz2[u][j] = z2[u][j] > 0.0f ? a[j] : 0.0f;
-          gb2[u][j] += dz2[u][j];
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+          for (int b = 0; b < 4; b++)
+            gw2[a][b] = mfma(unit_op(T0.z2[a], T.z2[a]), unit_op(T0.h1[b], T.h1[b]), gw2[a][b]);
+      }
+      // dZ1 = [z1 > 0] W2^T dZ2
+      bf16x8 dz1op[2];
+      {
+        f32x4 dz1[4];
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          dz1[v] = mask_tile(h1[v >> 1], v,
+                             mfma(im_ld(sm, IM_W2T + 2 * v + 1, lane), dz2op[1],
+                                  mfma(im_ld(sm, IM_W2T + 2 * v, lane), dz2op[0], zero4())));
+          gb1[v] += dz1[v];
         }
-        put4(s.dh2, 16 * u + 4 * g, r, dz2[u]);
+        dz1op[0] = pack_op(dz1[0], dz1[1]);
+        dz1op[1] = pack_op(dz1[2], dz1[3]);
       }
-      wave_sync();
-      // dH1^T = W2^T dH2^T
-      const bf16x8 d2a = col8(s.dh2, 8 * g, r), d2b = col8(s.dh2, 32 + 8 * g, r);
-      float dz1[4][4];
+      // dX^T (latent inputs 4g + j of row r)
+      const f32x4 dx = mfma(im_ld(sm, IM_W1T + 1, lane), dz1op[1], mfma(im_ld(sm, IM_W1T, lane), dz1op[0], zero4()));
+      *reinterpret_cast<f32x4*>(sdl + r * ML_LAT + 4 * g) = dx;
 #pragma unroll
-      for (int v = 0; v < 4; v++) {
-        f32x4 a = mfma(ld8(st.w2t + (16 * v + c16) * ML_HID + 8 * g), d2a, zero4());
-        a = mfma(ld8(st.w2t + (16 * v + c16) * ML_HID + 32 + 8 * g), d2b, a);
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-          dz1[v][j] = z1[v][j] > 0.0f ? a[j] : 0.0f;
-          gb1[v][j] += dz1[v][j];
-        }
-        put4(s.dh1, 16 * v + 4 * g, r, dz1[v]);
+      for (int t = 0; t < 4; t++) {
+        const uint4 a4 = dw(dz1op[t >> 1]);
+        T.z1[t] = (t & 1) ? tr16(a4.z, a4.w, sel_t) : tr16(a4.x, a4.y, sel_t);
       }
-      wave_sync();
-      // dX^T (latent inputs only) = W1[:, :16]^T dH1^T
-      f32x4 dx = mfma(ld8(st.w1t + c16 * ML_HID + 8 * g), col8(s.dh1, 8 * g, r), zero4());
-      dx = mfma(ld8(st.w1t + c16 * ML_HID + 32 + 8 * g), col8(s.dh1, 32 + 8 * g, r), dx);
+      if (h == 1) {
 #pragma unroll
-      for (int j = 0; j < 4; j++) s.dlat[r * ML_LAT + 4 * g + j] = dx[j];
-    }
-    wave_sync();
-    // weight gradients over these 32 rows (K = rows): lane l supplies rows 8g..8g+7
-    const int k0 = 8 * g;
+        for (int a = 0; a < 4; a++)
 #pragma unroll
-    for (int a = 0; a < 4; a++) {
-      const bf16x8 ad1 = ld8(s.dh1 + (16 * a + c16) * BW_RS + k0);
-      const bf16x8 ad2 = ld8(s.dh2 + (16 * a + c16) * BW_RS + k0);
-#pragma unroll
-      for (int b = 0; b < 2; b++) gw1[a][b] = mfma(ad1, ld8(s.x + (16 * b + c16) * BW_RS + k0), gw1[a][b]);
-#pragma unroll
-      for (int b = 0; b < 4; b++) gw2[a][b] = mfma(ad2, ld8(s.h1 + (16 * b + c16) * BW_RS + k0), gw2[a][b]);
-      gw3[a] = mfma(ld8(s.dout + c16 * BW_RS + k0), ld8(s.h2 + (16 * a + c16) * BW_RS + k0), gw3[a]);
+          for (int b = 0; b < 2; b++)
+            gw1[a][b] = mfma(unit_op(T0.z1[a], T.z1[a]), unit_op(T0.x[b], T.x[b]), gw1[a][b]);
+      } else {
+        T0 = T;
+      }
     }
     // latent gradients: sum the rows of each site, one atomic per (site, unit)
+    wave_sync();
     const int row_end = min(base + BW_ROWS, p.n_rows);
     if (p.pair_sites) {
       for (int i = lane; i < (row_end - base) * ML_LAT; i += WAVE) {
         const int rr = i / ML_LAT, u = i % ML_LAT;
         const int site = p.pair_sites[base + rr];
-        if (site >= 0 && site < p.n_sites) atomicAdd(p.g_lat + (size_t)site * ML_LAT + u, s.dlat[rr * ML_LAT + u]);
+        if (site >= 0 && site < p.n_sites) atomicAdd(p.g_lat + (size_t)site * ML_LAT + u, sdl[rr * ML_LAT + u]);
       }
     } else {
       const int s0 = base / p.T, s1 = (row_end - 1) / p.T;
@@ -374,30 +467,41 @@ __global__ void __launch_bounds__(WAVE* ML_WPB) mlp_bwd_kernel(MlpParams p) {
         const int site = s0 + i / ML_LAT, u = i % ML_LAT;
         const int ra = max(site * p.T, base), rb = min((site + 1) * p.T, row_end);
         float acc = 0.0f;
-        for (int rr = ra; rr < rb; rr++) acc += s.dlat[(rr - base) * ML_LAT + u];
-        atomicAdd(p.g_lat + (size_t)site * ML_LAT + u, acc);
+        for (int rr = ra; rr < rb; rr++) acc += sdl[(rr - base) * ML_LAT + u];
+        if (site < p.n_sites) atomicAdd(p.g_lat + (size_t)site * ML_LAT + u, acc);
       }
     }
   }
 
-  // flush: D layout puts unit (rows of the product) 16a + 4g + j, column 16b + c16
+  // end of launch: the workgroup's waves sum into LDS, then one atomic per value
+  float* red_w1 = sm.red;
+  float* red_w2 = red_w1 + ML_HID * ML_IN;
+  float* red_w3 = red_w2 + ML_HID * ML_HID;
+  float* red_b1 = red_w3 + 3 * ML_HID;
+  float* red_b2 = red_b1 + ML_HID;
+  float* red_b3 = red_b2 + ML_HID;
+  constexpr int N_RED = ML_HID * ML_IN + ML_HID * ML_HID + 3 * ML_HID + 2 * ML_HID + 3;
+  __syncthreads();
+  for (int i = threadIdx.x; i < N_RED; i += blockDim.x) sm.red[i] = 0.0f;
+  __syncthreads();
+  // D layout: product row 16a + 4g + j, column 16b + c16
 #pragma unroll
   for (int a = 0; a < 4; a++) {
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       const int m = 16 * a + 4 * g + j;
 #pragma unroll
-      for (int b = 0; b < 2; b++) atomicAdd(p.g_w1 + m * ML_IN + 16 * b + c16, gw1[a][b][j]);
+      for (int b = 0; b < 2; b++) atomicAdd(red_w1 + m * ML_IN + 16 * b + c16, gw1[a][b][j]);
 #pragma unroll
-      for (int b = 0; b < 4; b++) atomicAdd(p.g_w2 + m * ML_HID + 16 * b + c16, gw2[a][b][j]);
+      for (int b = 0; b < 4; b++) atomicAdd(red_w2 + m * ML_HID + 16 * b + c16, gw2[a][b][j]);
     }
-    // gw3: product rows = outputs 4g + j (only 0..2), columns h2 = 16a + c16
+    // gw3: product rows = outputs 4g + j (0..2 used), columns h2 = 16a + c16
     if (g == 0) {
 #pragma unroll
-      for (int j = 0; j < 3; j++) atomicAdd(p.g_w3 + j * ML_HID + 16 * a + c16, gw3[a][j]);
+      for (int j = 0; j < 3; j++) atomicAdd(red_w3 + j * ML_HID + 16 * a + c16, gw3[a][j]);
     }
   }
-  // biases: partial sums per lane over its rows; reduce over the 16 row lanes
+  // biases: per-lane sums over its rows; reduce over the 16 row lanes
 #pragma unroll
   for (int a = 0; a < 4; a++) {
 #pragma unroll
@@ -409,17 +513,28 @@ __global__ void __launch_bounds__(WAVE* ML_WPB) mlp_bwd_kernel(MlpParams p) {
         v2 += __shfl_xor(v2, m, 64);
       }
       if (c16 == 0) {
-        atomicAdd(p.g_b1 + 16 * a + 4 * g + j, v1);
-        atomicAdd(p.g_b2 + 16 * a + 4 * g + j, v2);
+        atomicAdd(red_b1 + 16 * a + 4 * g + j, v1);
+        atomicAdd(red_b2 + 16 * a + 4 * g + j, v2);
       }
     }
   }
 #pragma unroll
   for (int j = 0; j < 3; j++) {
-    float v = gb3[j];
-#pragma unroll
-    for (int m = 1; m < 64; m <<= 1) v += __shfl_xor(v, m, 64);
-    if (lane == 0) atomicAdd(p.g_b3 + j, v);
+    const float v = wave_sum(gb3[j]);
+    if (lane == 0) atomicAdd(red_b3 + j, v);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < N_RED; i += blockDim.x) {
+    const float v = sm.red[i];
+    float* dst;
+    int k = i;
+    if (k < ML_HID * ML_IN) dst = p.g_w1 + k;
+    else if ((k -= ML_HID * ML_IN) < ML_HID * ML_HID) dst = p.g_w2 + k;
+    else if ((k -= ML_HID * ML_HID) < 3 * ML_HID) dst = p.g_w3 + k;
+    else if ((k -= 3 * ML_HID) < ML_HID) dst = p.g_b1 + k;
+    else if ((k -= ML_HID) < ML_HID) dst = p.g_b2 + k;
+    else dst = p.g_b3 + (k - ML_HID);
+    atomicAdd(dst, v);
   }
 }
 
@@ -432,7 +547,8 @@ static int make_params(const pf_mlp_weights* w, const pf_mlp_rows* rows, MlpPara
   } else if (rows->n_rows > 0 && (!rows->texel_sites || rows->n_texels < 1)) {
     return set_error(PF_ERR_ARG, "%s: texel mode needs texel_sites and n_texels >= 1", who);
   }
-  if ((reinterpret_cast<uintptr_t>(rows->latents) & 15) || (reinterpret_cast<uintptr_t>(w->w1) & 15))
+  if ((reinterpret_cast<uintptr_t>(rows->latents) & 15) || (reinterpret_cast<uintptr_t>(w->w1) & 15) ||
+      (reinterpret_cast<uintptr_t>(w->w2) & 15) || (reinterpret_cast<uintptr_t>(w->w3) & 15))
     return set_error(PF_ERR_ARG, "%s: latents and weights must be 16-byte aligned", who);
   if (rows->n_sites < 0 || (!rows->pair_sites && (int64_t)rows->n_sites * rows->n_texels < rows->n_rows))
     return set_error(PF_ERR_ARG, "%s: texel rows exceed n_sites * n_texels", who);
@@ -454,14 +570,20 @@ static int make_params(const pf_mlp_weights* w, const pf_mlp_rows* rows, MlpPara
   return PF_OK;
 }
 
-static int persistent_grid(int n_units) {
-  int dev = 0, cus = 256;
+// Workgroups for a persistent launch: as many as are resident at once (from
+// the kernel's occupancy), at most one per wave-step of every wave.
+template <class K>
+static int persistent_grid(K kernel, int threads, int n_steps) {
+  int dev = 0, cus = 256, per_cu = 1;
   if (hipGetDevice(&dev) == hipSuccess) {
     int v = 0;
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0) cus = v;
   }
-  const int groups = (n_units + ML_WPB - 1) / ML_WPB;
-  return max(1, min(groups, 2 * cus));
+  int occ = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, threads, 0) == hipSuccess && occ > 0) per_cu = occ;
+  const int waves = threads / WAVE;
+  const int groups = (n_steps + waves - 1) / waves;
+  return max(1, min(groups, per_cu * cus));
 }
 
 }  // namespace pf
@@ -477,7 +599,7 @@ int pf_mlp_fwd(const pf_mlp_weights* w, const pf_mlp_rows* rows, float* rgb, voi
   if (p.n_rows == 0) return PF_OK;
   if (!rgb) return set_error(PF_ERR_ARG, "pf_mlp_fwd: missing output");
   p.rgb = rgb;
-  mlp_fwd_kernel<<<persistent_grid((p.n_rows + 15) / 16), WAVE * ML_WPB, 0, as_stream(stream)>>>(p);
+  mlp_fwd_kernel<<<persistent_grid(mlp_fwd_kernel, WAVE * ML_WPB, (p.n_rows + FW_ROWS - 1) / FW_ROWS), WAVE * ML_WPB, 0, as_stream(stream)>>>(p);
   PF_HIP(hipGetLastError());
   return PF_OK;
 }
@@ -498,7 +620,7 @@ int pf_mlp_bwd(const pf_mlp_weights* w, const pf_mlp_rows* rows, const float* gr
   p.g_b2 = g->b2;
   p.g_w3 = g->w3;
   p.g_b3 = g->b3;
-  mlp_bwd_kernel<<<persistent_grid((p.n_rows + BW_ROWS - 1) / BW_ROWS), WAVE * ML_WPB, 0, as_stream(stream)>>>(p);
+  mlp_bwd_kernel<<<persistent_grid(mlp_bwd_kernel, WAVE * BW_WPB, (p.n_rows + BW_ROWS - 1) / BW_ROWS), WAVE * BW_WPB, 0, as_stream(stream)>>>(p);
   PF_HIP(hipGetLastError());
   return PF_OK;
 }
