@@ -354,7 +354,10 @@ typedef struct pf_mlp_rows {
   const uint16_t* latents;   /* [N,16] bf16 */
 } pf_mlp_rows;
 
-/* fp32 gradient buffers, accumulated into (zero them first). */
+/* fp32 gradient buffers; the caller zeroes them first (weights and biases
+ * are accumulated into; a texel-mode site whose rows one wave step covers has
+ * its latent gradient stored). Bias gradients sum the bf16-rounded output
+ * gradients of their layer, as the weight gradients use them. */
 typedef struct pf_mlp_grads {
   float* latents; /* [N,16] */
   float* w1;

@@ -7,8 +7,10 @@ SH basis (sh_utils.py:4-49).
 
 numpy, FP64 arithmetic. With rounding=True it rounds to bf16 (nearest-even)
 at the points the kernels do (inputs, weights, h1, h2; in the backward the
-output gradient and dH2, dH1 before their products); with rounding=False it
-is the plain FP64 network whose derivatives the finite-difference tests pin.
+output gradient and dZ2, dZ1 before their products and before the bias
+gradients' sums, as a bf16 linear layer's backward sums its bf16 output
+gradient); with rounding=False it is the plain FP64 network whose
+derivatives the finite-difference tests pin.
 """
 
 import numpy as np
@@ -72,5 +74,6 @@ def backward(params, sites, dirs, grad_rgb, rounding=True):
     n = np.asarray(params["latents"]).shape[0]
     g_lat = np.zeros((n, 16))
     np.add.at(g_lat, np.asarray(sites), dx[:, :16])
-    return dict(latents=g_lat, w1=dz1b.T @ s["x"], b1=dz1.sum(0), w2=dz2b.T @ s["h1"], b2=dz2.sum(0),
-                w3=d_ob.T @ s["h2"], b3=d_o.sum(0))
+    # bias gradients sum the same (rounded) dZ / dOut the weight gradients use
+    return dict(latents=g_lat, w1=dz1b.T @ s["x"], b1=dz1b.sum(0), w2=dz2b.T @ s["h1"], b2=dz2b.sum(0),
+                w3=d_ob.T @ s["h2"], b3=d_ob.sum(0))

@@ -117,30 +117,29 @@ __device__ __forceinline__ bf16x8 ld_wrow(const uint16_t* w, int row, int q, int
 // Input row x = [latent ; e(d)] of `row` as 32 bf16 at dst (16-byte aligned);
 // zero for rows past the end or naming no valid site.
 __device__ __forceinline__ void stage_row(const MlpParams& p, int row, uint16_t* dst) {
-  uint4 q0 = {0u, 0u, 0u, 0u}, q1 = q0, q2 = q0, q3 = q0;
-  const int site = row >= p.n_rows ? -1 : (p.pair_sites ? p.pair_sites[row] : row / p.T);
-  if (site >= 0 && site < p.n_sites) {
-    const uint4* lp = reinterpret_cast<const uint4*>(p.latents + (size_t)site * ML_LAT);
-    q0 = lp[0];
-    q1 = lp[1];
-    f3 d;
-    if (p.pair_sites) {
-      d = ld3(p.pair_dirs + 3 * (size_t)row);
-      const float l = sqrtf(dot(d, d));
-      d = l > 0.0f ? d * (1.0f / l) : f3{0.0f, 0.0f, 0.0f};
-    } else {
-      d = texel_dir(p.texel_sites + 3 * (size_t)row, p.eye);
-    }
-    float y[15];
-    sh_basis(d, y);
-    q2 = make_uint4(pk_bf(1.0f, y[0]), pk_bf(y[1], y[2]), pk_bf(y[3], y[4]), pk_bf(y[5], y[6]));
-    q3 = make_uint4(pk_bf(y[7], y[8]), pk_bf(y[9], y[10]), pk_bf(y[11], y[12]), pk_bf(y[13], y[14]));
+  const bool in = row < p.n_rows;
+  const int rc = in ? row : 0;
+  int site = p.pair_sites ? p.pair_sites[rc] : rc / p.T;
+  const bool ok = in && site >= 0 && site < p.n_sites;
+  site = ok ? site : 0;
+  const uint4* lp = reinterpret_cast<const uint4*>(p.latents + (size_t)site * ML_LAT);
+  f3 d;
+  if (p.pair_sites) {
+    d = ld3(p.pair_dirs + 3 * (size_t)rc);
+    const float l = sqrtf(dot(d, d));
+    d = l > 0.0f ? d * (1.0f / l) : f3{0.0f, 0.0f, 0.0f};
+  } else {
+    d = texel_dir(p.texel_sites + 3 * (size_t)rc, p.eye);
   }
+  float y[15];
+  sh_basis(d, y);
+  const uint32_t m = ok ? 0xFFFFFFFFu : 0u;
+  const uint4 q0 = lp[0], q1 = lp[1];
   uint4* o = reinterpret_cast<uint4*>(dst);
-  o[0] = q0;
-  o[1] = q1;
-  o[2] = q2;
-  o[3] = q3;
+  o[0] = make_uint4(q0.x & m, q0.y & m, q0.z & m, q0.w & m);
+  o[1] = make_uint4(q1.x & m, q1.y & m, q1.z & m, q1.w & m);
+  o[2] = make_uint4(pk_bf(1.0f, y[0]) & m, pk_bf(y[1], y[2]) & m, pk_bf(y[3], y[4]) & m, pk_bf(y[5], y[6]) & m);
+  o[3] = make_uint4(pk_bf(y[7], y[8]) & m, pk_bf(y[9], y[10]) & m, pk_bf(y[11], y[12]) & m, pk_bf(y[13], y[14]) & m);
 }
 
 __global__ void __launch_bounds__(WAVE* ML_WPB) mlp_fwd_kernel(MlpParams p) {
@@ -211,12 +210,16 @@ __global__ void __launch_bounds__(WAVE* ML_WPB) mlp_fwd_kernel(MlpParams p) {
 // matrix transposes a tile exactly (bf16 values times one, summed with
 // zeros), giving 4 rows of one unit per lane; the two halves' tiles together
 // are the 8 rows, K slot 8g + j being row 4g + j of half 0 for j < 4 and
-// row 16 + 4g + j - 4 of half 1 for j >= 4. The weight gradients accumulate
-// in registers over the whole persistent launch; at the end the workgroup's
-// waves sum them in LDS and issue one atomic per weight.
+// row 16 + 4g + j - 4 of half 1 for j >= 4 (half 0's transposed tiles wait
+// in LDS). The weight gradients accumulate in registers over the whole
+// persistent launch; at the end the workgroup's waves sum them in LDS and
+// issue one atomic per weight. Bias gradients sum the same bf16 dZ the
+// weight gradients use: layer 1's is dW1's column of the constant input
+// e_0 = 1, layers 2 and 3 sum the transposed dZ2 and dOut (v_dot2 with ones).
+// The relu masks of the backward are read from the bf16 activations.
 // All A operands (weights, their transposes) sit in LDS as per-lane images:
 // one conflict-free 16-byte read each.
-constexpr int BW_WPB = 4;  // waves per workgroup (one per SIMD: the backward needs ~350 registers)
+constexpr int BW_WPB = 8;  // waves per workgroup
 constexpr int BW_ROWS = 32;
 enum { IM_W1 = 0, IM_W2 = 4, IM_W3 = 12, IM_W3T = 14, IM_W2T = 18, IM_W1T = 26, IM_N = 28 };
 
@@ -243,6 +246,9 @@ __device__ __forceinline__ uint16_t image_entry(const MlpParams& p, int i) {
   return p.w1[slot_unit(tile - IM_W1T, g, j) * ML_IN + c];
 }
 
+// Half 0's transposed operands wait in LDS for half 1's: [item][lane] pairs.
+enum { TI_H1 = 0, TI_H2 = 4, TI_Z2 = 8, TI_Z1 = 12, TI_X = 16, TI_O = 18, TI_N = 19 };
+
 struct alignas(16) BwdShared {
   union {
     uint16_t im[IM_N * 512];             // A-operand images
@@ -251,29 +257,27 @@ struct alignas(16) BwdShared {
   float b1[ML_HID], b2[ML_HID], b3[4];
   uint16_t x[BW_WPB][BW_ROWS * XS];     // staged input rows
   float dlat[BW_WPB][BW_ROWS * ML_LAT];  // latent gradient rows
+  uint2 t0[BW_WPB][TI_N][WAVE];          // half 0's transposed operands
 };
 
 __device__ __forceinline__ bf16x8 im_ld(const BwdShared& sm, int tile, int lane) { return ld8(sm.im + tile * 512 + lane * 8); }
 
 __device__ __forceinline__ f32x4 ld_f4(const float* p) { return *reinterpret_cast<const f32x4*>(p); }
 
-// [h > 0] d for the D tile t of an activation operand pair (tiles 2q, 2q + 1).
-// The relu masks are read from the bf16 activations, which are zero exactly
+// bf16 [h > 0] d of two D tiles as one B operand: the relu masks are read
+// from the bf16 activation operand h of the same slots, which is zero exactly
 // where z <= 0 (and for positive z below the smallest bf16 normal, where the
-// activation itself is zero).
-__device__ __forceinline__ f32x4 mask_tile(bf16x8 hop, int t, f32x4 d) {
-  const uint4 u = __builtin_bit_cast(uint4, hop);
-  const uint32_t a = (t & 1) ? u.z : u.x, b = (t & 1) ? u.w : u.y;
-  f32x4 r;
-  r[0] = (a & 0xFFFFu) ? d[0] : 0.0f;
-  r[1] = (a >> 16) ? d[1] : 0.0f;
-  r[2] = (b & 0xFFFFu) ? d[2] : 0.0f;
-  r[3] = (b >> 16) ? d[3] : 0.0f;
-  return r;
+// activation itself is zero). Relu'd bf16 halves lie in [0, 0x7FFF], so
+// adding 0x7FFF to each (no carry between halves) sets a half's bit 15 exactly
+// when it is nonzero; an arithmetic shift spreads that bit over the half.
+__device__ __forceinline__ uint32_t nz_mask(uint32_t h) {
+  return __builtin_bit_cast(uint32_t, __builtin_bit_cast(s16x2, h + 0x7FFF7FFFu) >> (s16x2){15, 15});
 }
 
-__device__ __forceinline__ bf16x8 pack_op(f32x4 lo, f32x4 hi) {
-  return __builtin_bit_cast(bf16x8, make_uint4(pk_bf(lo[0], lo[1]), pk_bf(lo[2], lo[3]), pk_bf(hi[0], hi[1]), pk_bf(hi[2], hi[3])));
+__device__ __forceinline__ bf16x8 pack_masked(f32x4 lo, f32x4 hi, bf16x8 hop) {
+  const uint4 m = __builtin_bit_cast(uint4, hop);
+  return __builtin_bit_cast(bf16x8, make_uint4(pk_bf(lo[0], lo[1]) & nz_mask(m.x), pk_bf(lo[2], lo[3]) & nz_mask(m.y),
+                                               pk_bf(hi[0], hi[1]) & nz_mask(m.z), pk_bf(hi[2], hi[3]) & nz_mask(m.w)));
 }
 
 // Transpose of a 16x16 bf16 tile given as the D-layout quadruple of this lane
@@ -284,19 +288,23 @@ __device__ __forceinline__ uint2 tr16(uint32_t lo, uint32_t hi, s16x4 sel) {
   return make_uint2(pk_bf(d[0], d[1]), pk_bf(d[2], d[3]));
 }
 
-// B operand selecting input columns 16b .. 16b + 15: slot 8g + j is 1 where 8g + j = 16b + c16
+// B operand selecting input columns 16b .. 16b + 15: slot 8g + j is one
+// where 8g + j = 16b + c16
 __device__ __forceinline__ bf16x8 sel_x(int g, int c16, int b) {
   const int j = 16 * b + c16 - 8 * g;  // the slot of this lane that is one, if 0 <= j < 8
-  const uint32_t one = 0x3F80u;
-  uint4 u = {0u, 0u, 0u, 0u};
-  const uint32_t v = (j & 1) ? (one << 16) : one;
-  if (j >= 0 && j < 8) {
-    if ((j >> 1) == 0) u.x = v;
-    else if ((j >> 1) == 1) u.y = v;
-    else if ((j >> 1) == 2) u.z = v;
-    else u.w = v;
-  }
-  return __builtin_bit_cast(bf16x8, u);
+  const uint32_t v = (j & 1) ? (0x3F80u << 16) : 0x3F80u;
+  return __builtin_bit_cast(bf16x8, make_uint4(j >> 1 == 0 ? v : 0u, j >> 1 == 1 ? v : 0u, j >> 1 == 2 ? v : 0u,
+                                               j >> 1 == 3 ? v : 0u));
+}
+
+// acc + the sum of 8 bf16 values (exact products with one, FP32 sums)
+__device__ __forceinline__ float sum8(bf16x8 v, float acc) {
+  const bf16x2 one = {static_cast<__bf16>(1.0f), static_cast<__bf16>(1.0f)};
+  const uint4 u = __builtin_bit_cast(uint4, v);
+  acc = __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, u.x), one, acc, false);
+  acc = __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, u.y), one, acc, false);
+  acc = __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, u.z), one, acc, false);
+  return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, u.w), one, acc, false);
 }
 
 __device__ __forceinline__ uint4 dw(bf16x8 v) { return __builtin_bit_cast(uint4, v); }
@@ -326,13 +334,16 @@ __global__ void __launch_bounds__(WAVE* BW_WPB) mlp_bwd_kernel(MlpParams p) {
   s16x4 sel_t;
 #pragma unroll
   for (int j = 0; j < 4; j++) sel_t[j] = (c16 - 4 * g == j) ? (short)0x3F80 : (short)0;
+  const bf16x8 selx[2] = {sel_x(g, c16, 0), sel_x(g, c16, 1)};
 
   f32x4 gw1[4][2], gw2[4][4], gw3[4];
-  f32x4 gb1[4], gb2[4];
-  float gb3[3] = {0.0f, 0.0f, 0.0f};
+  // bias gradients of layers 2 and 3 (layer 1's is dW1's column of the
+  // constant input e_0 = 1): sums of the transposed bf16 dZ2 / dOut, per lane
+  // for unit 16a + c16 (output c16)
+  float gbn2[4] = {0.0f, 0.0f, 0.0f, 0.0f}, gbn3 = 0.0f;
 #pragma unroll
   for (int a = 0; a < 4; a++) {
-    gw3[a] = gb1[a] = gb2[a] = zero4();
+    gw3[a] = zero4();
 #pragma unroll
     for (int b = 0; b < 2; b++) gw1[a][b] = zero4();
 #pragma unroll
@@ -345,7 +356,7 @@ __global__ void __launch_bounds__(WAVE* BW_WPB) mlp_bwd_kernel(MlpParams p) {
     wave_sync();
     if (lane < BW_ROWS) stage_row(p, base + lane, sx + lane * XS);
     wave_sync();
-    HalfT T0;  // half 0's transposed operands, used with half 1's
+    uint2(*t0)[WAVE] = sm.t0[wv];
 #pragma unroll
     for (int h = 0; h < 2; h++) {
       const int r = 16 * h + c16;
@@ -354,8 +365,9 @@ __global__ void __launch_bounds__(WAVE* BW_WPB) mlp_bwd_kernel(MlpParams p) {
       const bf16x8 x = ld8(sx + r * XS + 8 * g);
 #pragma unroll
       for (int b = 0; b < 2; b++) {
-        const f32x4 d = mfma(x, sel_x(g, c16, b), zero4());
+        const f32x4 d = mfma(x, selx[b], zero4());
         T.x[b] = make_uint2(pk_bf(d[0], d[1]), pk_bf(d[2], d[3]));
+        if (h == 0) t0[TI_X + b][lane] = T.x[b];
       }
       bf16x8 h1[2], h2[2];
       {
@@ -381,8 +393,6 @@ __global__ void __launch_bounds__(WAVE* BW_WPB) mlp_bwd_kernel(MlpParams p) {
 #pragma unroll
         for (int j = 0; j < 3; j++) d_out[j] = o[j] > 0.0f ? p.grad_rgb[3 * (size_t)row + j] : 0.0f;
       }
-#pragma unroll
-      for (int j = 0; j < 3; j++) gb3[j] += d_out[j];
       const uint32_t do01 = pk_bf(d_out[0], d_out[1]), do23 = pk_bf(d_out[2], 0.0f);
       const bf16x8 dob = __builtin_bit_cast(bf16x8, make_uint4(do01, do23, 0u, 0u));
       T.o = tr16(do01, do23, sel_t);
@@ -392,47 +402,56 @@ __global__ void __launch_bounds__(WAVE* BW_WPB) mlp_bwd_kernel(MlpParams p) {
         T.h1[t] = (t & 1) ? tr16(a1.z, a1.w, sel_t) : tr16(a1.x, a1.y, sel_t);
         T.h2[t] = (t & 1) ? tr16(a2.z, a2.w, sel_t) : tr16(a2.x, a2.y, sel_t);
       }
-      if (h == 1) {
+      if (h == 0) {
+        t0[TI_O][lane] = T.o;
 #pragma unroll
-        for (int a = 0; a < 4; a++) gw3[a] = mfma(unit_op(T0.o, T.o), unit_op(T0.h2[a], T.h2[a]), gw3[a]);
+        for (int t = 0; t < 4; t++) {
+          t0[TI_H1 + t][lane] = T.h1[t];
+          t0[TI_H2 + t][lane] = T.h2[t];
+        }
+      } else {
+        const bf16x8 uo = unit_op(t0[TI_O][lane], T.o);
+        gbn3 = sum8(uo, gbn3);
+#pragma unroll
+        for (int a = 0; a < 4; a++) gw3[a] = mfma(uo, unit_op(t0[TI_H2 + a][lane], T.h2[a]), gw3[a]);
       }
       // dZ2 = [z2 > 0] W3^T dOut
       bf16x8 dz2op[2];
       {
-        f32x4 dz2[4];
+        f32x4 dh2[4];
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
-          dz2[u] = mask_tile(h2[u >> 1], u, mfma(im_ld(sm, IM_W3T + u, lane), dob, zero4()));
-          gb2[u] += dz2[u];
-        }
-        dz2op[0] = pack_op(dz2[0], dz2[1]);
-        dz2op[1] = pack_op(dz2[2], dz2[3]);
+        for (int u = 0; u < 4; u++) dh2[u] = mfma(im_ld(sm, IM_W3T + u, lane), dob, zero4());
+        dz2op[0] = pack_masked(dh2[0], dh2[1], h2[0]);
+        dz2op[1] = pack_masked(dh2[2], dh2[3], h2[1]);
       }
 #pragma unroll
       for (int t = 0; t < 4; t++) {
         const uint4 a3 = dw(dz2op[t >> 1]);
         T.z2[t] = (t & 1) ? tr16(a3.z, a3.w, sel_t) : tr16(a3.x, a3.y, sel_t);
+        if (h == 0) t0[TI_Z2 + t][lane] = T.z2[t];
       }
       if (h == 1) {
+        bf16x8 ub[4];
 #pragma unroll
-        for (int a = 0; a < 4; a++)
+        for (int b = 0; b < 4; b++) ub[b] = unit_op(t0[TI_H1 + b][lane], T.h1[b]);
 #pragma unroll
-          for (int b = 0; b < 4; b++)
-            gw2[a][b] = mfma(unit_op(T0.z2[a], T.z2[a]), unit_op(T0.h1[b], T.h1[b]), gw2[a][b]);
+        for (int a = 0; a < 4; a++) {
+          const bf16x8 ua = unit_op(t0[TI_Z2 + a][lane], T.z2[a]);
+          gbn2[a] = sum8(ua, gbn2[a]);
+#pragma unroll
+          for (int b = 0; b < 4; b++) gw2[a][b] = mfma(ua, ub[b], gw2[a][b]);
+        }
       }
       // dZ1 = [z1 > 0] W2^T dZ2
       bf16x8 dz1op[2];
       {
-        f32x4 dz1[4];
+        f32x4 dh1[4];
 #pragma unroll
-        for (int v = 0; v < 4; v++) {
-          dz1[v] = mask_tile(h1[v >> 1], v,
-                             mfma(im_ld(sm, IM_W2T + 2 * v + 1, lane), dz2op[1],
-                                  mfma(im_ld(sm, IM_W2T + 2 * v, lane), dz2op[0], zero4())));
-          gb1[v] += dz1[v];
-        }
-        dz1op[0] = pack_op(dz1[0], dz1[1]);
-        dz1op[1] = pack_op(dz1[2], dz1[3]);
+        for (int v = 0; v < 4; v++)
+          dh1[v] = mfma(im_ld(sm, IM_W2T + 2 * v + 1, lane), dz2op[1],
+                        mfma(im_ld(sm, IM_W2T + 2 * v, lane), dz2op[0], zero4()));
+        dz1op[0] = pack_masked(dh1[0], dh1[1], h1[0]);
+        dz1op[1] = pack_masked(dh1[2], dh1[3], h1[1]);
       }
       // dX^T (latent inputs 4g + j of row r)
       const f32x4 dx = mfma(im_ld(sm, IM_W1T + 1, lane), dz1op[1], mfma(im_ld(sm, IM_W1T, lane), dz1op[0], zero4()));
@@ -441,15 +460,16 @@ __global__ void __launch_bounds__(WAVE* BW_WPB) mlp_bwd_kernel(MlpParams p) {
       for (int t = 0; t < 4; t++) {
         const uint4 a4 = dw(dz1op[t >> 1]);
         T.z1[t] = (t & 1) ? tr16(a4.z, a4.w, sel_t) : tr16(a4.x, a4.y, sel_t);
+        if (h == 0) t0[TI_Z1 + t][lane] = T.z1[t];
       }
       if (h == 1) {
+        const bf16x8 ux[2] = {unit_op(t0[TI_X][lane], T.x[0]), unit_op(t0[TI_X + 1][lane], T.x[1])};
 #pragma unroll
-        for (int a = 0; a < 4; a++)
+        for (int a = 0; a < 4; a++) {
+          const bf16x8 ua = unit_op(t0[TI_Z1 + a][lane], T.z1[a]);
 #pragma unroll
-          for (int b = 0; b < 2; b++)
-            gw1[a][b] = mfma(unit_op(T0.z1[a], T.z1[a]), unit_op(T0.x[b], T.x[b]), gw1[a][b]);
-      } else {
-        T0 = T;
+          for (int b = 0; b < 2; b++) gw1[a][b] = mfma(ua, ux[b], gw1[a][b]);
+        }
       }
     }
     // latent gradients: sum the rows of each site, one atomic per (site, unit)
@@ -468,7 +488,14 @@ __global__ void __launch_bounds__(WAVE* BW_WPB) mlp_bwd_kernel(MlpParams p) {
         const int ra = max(site * p.T, base), rb = min((site + 1) * p.T, row_end);
         float acc = 0.0f;
         for (int rr = ra; rr < rb; rr++) acc += sdl[(rr - base) * ML_LAT + u];
-        if (site < p.n_sites) atomicAdd(p.g_lat + (size_t)site * ML_LAT + u, acc);
+        // a site whose rows all fall in this step has no other writer
+        const bool whole = site * p.T >= base && ((site + 1) * p.T <= row_end || row_end == p.n_rows);
+        if (site < p.n_sites) {
+          if (whole)
+            p.g_lat[(size_t)site * ML_LAT + u] = acc;
+          else
+            atomicAdd(p.g_lat + (size_t)site * ML_LAT + u, acc);
+        }
       }
     }
   }
@@ -501,28 +528,22 @@ __global__ void __launch_bounds__(WAVE* BW_WPB) mlp_bwd_kernel(MlpParams p) {
       for (int j = 0; j < 3; j++) atomicAdd(red_w3 + j * ML_HID + 16 * a + c16, gw3[a][j]);
     }
   }
-  // biases: per-lane sums over its rows; reduce over the 16 row lanes
+  // biases: per-lane sums over its rows; reduce over the 4 lane groups
 #pragma unroll
   for (int a = 0; a < 4; a++) {
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      float v1 = gb1[a][j], v2 = gb2[a][j];
-#pragma unroll
-      for (int m = 1; m < 16; m <<= 1) {
-        v1 += __shfl_xor(v1, m, 64);
-        v2 += __shfl_xor(v2, m, 64);
-      }
-      if (c16 == 0) {
-        atomicAdd(red_b1 + 16 * a + 4 * g + j, v1);
-        atomicAdd(red_b2 + 16 * a + 4 * g + j, v2);
-      }
-    }
+    float v = gbn2[a];
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
+    if (g == 0) atomicAdd(red_b2 + 16 * a + c16, v);
   }
-#pragma unroll
-  for (int j = 0; j < 3; j++) {
-    const float v = wave_sum(gb3[j]);
-    if (lane == 0) atomicAdd(red_b3 + j, v);
+  {
+    float v = gbn3;
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
+    if (g == 0 && c16 < 3) atomicAdd(red_b3 + c16, v);
   }
+  __syncthreads();
+  if (threadIdx.x < ML_HID) red_b1[threadIdx.x] = red_w1[threadIdx.x * ML_IN + ML_LAT];
   __syncthreads();
   for (int i = threadIdx.x; i < N_RED; i += blockDim.x) {
     const float v = sm.red[i];
