@@ -87,7 +87,8 @@ typedef struct pf_tile_lists {
  * state the later ones leave. n_segs = 0 keeps one wave per whole list.
  * The forward and the backward of a frame must use the same table. */
 typedef struct pf_segments {
-  int32_t n_segs;
+  int32_t n_segs;             /* table capacity (>= n_tiles) */
+  const int32_t* count;       /* device int: segments in use (<= n_segs), or NULL: all n_segs */
   int32_t seg_len;            /* sites per segment, a positive multiple of 64 */
   const int32_t* tile_first;  /* [n_tiles+1] first segment id of each tile */
   const int32_t* seg_tile;    /* [n_segs] tile of each segment id */
@@ -288,6 +289,22 @@ size_t pf_bin_sort_workspace_size(int64_t n_records, int32_t n_tiles);
 int pf_bin_sort(const int64_t* offsets, int32_t n_tiles, uint64_t* records,
                 int32_t* sites, void* workspace, size_t workspace_bytes,
                 void* stream);
+
+/* Dispatch order of the compositing kernels, built on the device from the
+ * tile offsets (no reference counterpart: the reference starts tiles in index
+ * order; this replaces host-side argsorts and a host sync). Writes
+ *   tile_order [n_tiles]: tiles by descending list length (pf_tile_lists.order);
+ *   tile_first [n_tiles+1], seg_tile, seg_order [max_segs], *count: the
+ *   segment table of pf_segments for segments of seg_len sites, segments by
+ *   descending size.
+ * Orders are by length bucket: exact below 512 sites, in steps of 32 sites
+ * up to 16,864, one bucket above. n_records = offsets[n_tiles]; the table
+ * needs max_segs >= n_tiles + n_records / seg_len; seg_len is a positive
+ * multiple of 64 up to 1024. tile_order, or the four segment outputs, may be
+ * NULL to skip that part. */
+int pf_tile_schedule(const int64_t* offsets, int32_t n_tiles, int64_t n_records, int32_t seg_len,
+                     int32_t max_segs, int32_t* tile_order, int32_t* tile_first, int32_t* seg_tile,
+                     int32_t* seg_order, int32_t* count, void* stream);
 
 /* ---- adjacency encoding (prefetch_adjacency_kernel rasterize.py:580-606) */
 /* diff[e] = binary16(c_adj - c, r_adj) for every edge of every site. */

@@ -61,6 +61,7 @@ class PfScene(ctypes.Structure):
 class PfSegments(ctypes.Structure):
     _fields_ = [
         ("n_segs", ctypes.c_int32),
+        ("count", ctypes.c_void_p),
         ("seg_len", ctypes.c_int32),
         ("tile_first", ctypes.c_void_p),
         ("seg_tile", ctypes.c_void_p),
@@ -215,6 +216,8 @@ SIGNATURES = {
     "pf_bin_emit": ([_VP, _VP, ctypes.c_int32, _VP, ctypes.c_int32, _VP, _VP, _VP, _VP], ctypes.c_int),
     "pf_bin_sort_workspace_size": ([ctypes.c_int64, ctypes.c_int32], ctypes.c_size_t),
     "pf_bin_sort": ([_VP, ctypes.c_int32, _VP, _VP, _VP, ctypes.c_size_t, _VP], ctypes.c_int),
+    "pf_tile_schedule": ([_VP, ctypes.c_int32, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, _VP, _VP, _VP, _VP, _VP,
+                          _VP], ctypes.c_int),
     "pf_cone_tree_nodes": ([ctypes.c_int32, ctypes.c_int32], ctypes.c_int32),
     "pf_bin_cones": ([_VP, _VP, _VP], ctypes.c_int),
     "pf_bin_count_cones": ([_VP, _VP, _VP, ctypes.c_int32, _VP, _VP], ctypes.c_int),

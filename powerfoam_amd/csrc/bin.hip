@@ -619,6 +619,96 @@ __global__ void __launch_bounds__(SORT_THREADS)
   for (int64_t i = threadIdx.x; i < n; i += SORT_THREADS) sites[a + i] = (int32_t)(uint32_t)g[i];
 }
 
+
+// ------------------------------------------------ dispatch schedule
+// One workgroup turns the tile offsets into the compositing kernels' dispatch
+// order: a block-wide exclusive scan of the per-tile segment counts gives the
+// segment table, and counting sorts over length buckets give the
+// longest-first orders (members of one bucket land in atomic order, which
+// only the schedule sees, never a result).
+constexpr int SCHED_THREADS = 1024, SCHED_BUCKETS = 1024;
+
+__device__ __forceinline__ int len_bucket(int64_t len) {
+  if (len < 512) return (int)len;
+  return (int)min((int64_t)(512 + ((len - 512) >> 5)), (int64_t)(SCHED_BUCKETS - 1));
+}
+
+// Exclusive scan of one int per thread over the workgroup; *total gets the sum.
+__device__ int block_excl_scan(int v, int* wsum, int* total) {
+  const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+  int x = v;
+#pragma unroll
+  for (int m = 1; m < WAVE; m <<= 1) {
+    const int y = __shfl_up(x, m, WAVE);
+    if (lane >= m) x += y;
+  }
+  if (lane == WAVE - 1) wsum[w] = x;
+  __syncthreads();
+  int base = 0, tot = 0;
+  for (int k = 0; k < SCHED_THREADS / WAVE; k++) {
+    const int s = wsum[k];
+    if (k < w) base += s;
+    tot += s;
+  }
+  __syncthreads();
+  *total = tot;
+  return base + x - v;
+}
+
+// Writes the n items in descending bucket order to out.
+template <class B>
+__device__ void bucket_sort_desc(int n, B bucket, int32_t* out, int* hist, int* wsum) {
+  for (int i = threadIdx.x; i < SCHED_BUCKETS; i += SCHED_THREADS) hist[i] = 0;
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += SCHED_THREADS) atomicAdd(&hist[bucket(i)], 1);
+  __syncthreads();
+  // thread i owns bucket SCHED_BUCKETS - 1 - i: buckets in descending order
+  const int b = SCHED_BUCKETS - 1 - threadIdx.x;
+  int total;
+  const int start = block_excl_scan(hist[b], wsum, &total);
+  hist[b] = start;
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += SCHED_THREADS) out[atomicAdd(&hist[bucket(i)], 1)] = i;
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(SCHED_THREADS)
+    tile_schedule_kernel(const int64_t* __restrict__ offsets, int nt, int L, int32_t* tile_order,
+                         int32_t* tile_first, int32_t* seg_tile, int32_t* seg_order, int32_t* count) {
+  __shared__ int hist[SCHED_BUCKETS];
+  __shared__ int wsum[SCHED_THREADS / WAVE];
+  auto cnt = [&](int t) { return offsets[t + 1] - offsets[t]; };
+  if (tile_first) {
+    // tile t has max(1, ceil(cnt / L)) segments; each thread scans a run of tiles
+    const int per = (nt + SCHED_THREADS - 1) / SCHED_THREADS;
+    const int t0 = min((int)threadIdx.x * per, nt), t1 = min(t0 + per, nt);
+    int local = 0;
+    for (int t = t0; t < t1; t++) local += max(1, (int)((cnt(t) + L - 1) / L));
+    int total;
+    int first = block_excl_scan(local, wsum, &total);
+    for (int t = t0; t < t1; t++) {
+      const int ns = max(1, (int)((cnt(t) + L - 1) / L));
+      tile_first[t] = first;
+      for (int k = 0; k < ns; k++) seg_tile[first + k] = t;
+      first += ns;
+    }
+    if (threadIdx.x == 0) {
+      tile_first[nt] = total;
+      *count = total;
+    }
+    __syncthreads();
+    bucket_sort_desc(
+        total,
+        [&](int s) {
+          const int t = seg_tile[s];
+          const int64_t size = cnt(t) - (int64_t)(s - tile_first[t]) * L;
+          return (int)min(size, (int64_t)(SCHED_BUCKETS - 1));
+        },
+        seg_order, hist, wsum);
+  }
+  if (tile_order) bucket_sort_desc(nt, [&](int t) { return len_bucket(cnt(t)); }, tile_order, hist, wsum);
+}
+
 }  // namespace pf
 
 using namespace pf;
@@ -736,6 +826,27 @@ int pf_bin_sort(const int64_t* offsets, int32_t n_tiles, uint64_t* records, int3
   (void)workspace_bytes;
   if (!offsets || n_tiles < 1) return set_error(PF_ERR_ARG, "pf_bin_sort: bad arguments");
   bin_sort_kernel<<<n_tiles, SORT_THREADS, 0, as_stream(stream)>>>(offsets, n_tiles, records, sites);
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+int pf_tile_schedule(const int64_t* offsets, int32_t n_tiles, int64_t n_records, int32_t seg_len,
+                     int32_t max_segs, int32_t* tile_order, int32_t* tile_first, int32_t* seg_tile,
+                     int32_t* seg_order, int32_t* count, void* stream) {
+  if (!offsets || n_tiles < 1 || n_records < 0) return set_error(PF_ERR_ARG, "pf_tile_schedule: bad arguments");
+  const bool segs = tile_first || seg_tile || seg_order || count;
+  if (segs) {
+    if (!tile_first || !seg_tile || !seg_order || !count)
+      return set_error(PF_ERR_ARG, "pf_tile_schedule: the segment outputs go together");
+    if (seg_len < WAVE || seg_len % WAVE || seg_len > SCHED_BUCKETS)
+      return set_error(PF_ERR_ARG, "pf_tile_schedule: seg_len must be a multiple of 64 up to 1024");
+    if ((int64_t)max_segs < (int64_t)n_tiles + n_records / seg_len)
+      return set_error(PF_ERR_ARG, "pf_tile_schedule: max_segs below n_tiles + n_records / seg_len");
+  }
+  if (!segs && !tile_order) return PF_OK;
+  tile_schedule_kernel<<<1, SCHED_THREADS, 0, as_stream(stream)>>>(offsets, n_tiles, seg_len, tile_order,
+                                                                    segs ? tile_first : nullptr, seg_tile, seg_order,
+                                                                    count);
   PF_HIP(hipGetLastError());
   return PF_OK;
 }

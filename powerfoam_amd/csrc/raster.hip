@@ -211,8 +211,9 @@ struct FwdParams {
   float* err;
   float* contrib;
   float* point_err;
-  // segments (n_segs = 0: one wave per whole list)
+  // segments (n_segs = 0: one wave per whole list; seg_count: segments in use)
   int n_segs, seg_len;
+  const int32_t* seg_count;
   const int32_t* seg_first;
   const int32_t* seg_tile;
   const int32_t* seg_order;
@@ -228,12 +229,12 @@ struct WorkItem {
   int tile, seg, k, nk, f0;
 };
 
-__device__ __forceinline__ bool work_item(int n_segs, const int32_t* seg_order, const int32_t* seg_tile,
-                                          const int32_t* seg_first, const int32_t* order, int n_tiles, int b, int wv,
-                                          WorkItem& w) {
+__device__ __forceinline__ bool work_item(int n_segs, const int32_t* seg_count, const int32_t* seg_order,
+                                          const int32_t* seg_tile, const int32_t* seg_first, const int32_t* order,
+                                          int n_tiles, int b, int wv, WorkItem& w) {
   if (n_segs > 0) {
     const int slot = b * WPB + wv;
-    if (slot >= n_segs) return false;
+    if (slot >= n_segs || (seg_count && slot >= *seg_count)) return false;
     w.seg = seg_order[slot];
     w.tile = seg_tile[w.seg];
     w.f0 = seg_first[w.tile];
@@ -264,7 +265,7 @@ __global__ void __launch_bounds__(WAVE* WPB) raster_fwd_kernel(FwdParams p) {
   const int wv = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   WorkItem wi;
-  if (!work_item(p.n_segs, p.seg_order, p.seg_tile, p.seg_first, p.order, p.n_tiles, blockIdx.x, wv, wi)) return;
+  if (!work_item(p.n_segs, p.seg_count, p.seg_order, p.seg_tile, p.seg_first, p.order, p.n_tiles, blockIdx.x, wv, wi)) return;
   if (LOGT && wi.k == wi.nk - 1) return;  // no later segment needs it
   const int tile = wi.tile;
   SiteRec* s_site = s_site_all[wv];
@@ -541,8 +542,9 @@ struct BwdParams {
   float* g_tsites;
   float* g_trgb;
   float* g_theight;
-  // segments (n_segs = 0: one wave per whole list)
+  // segments (n_segs = 0: one wave per whole list; seg_count: segments in use)
   int n_segs, seg_len;
+  const int32_t* seg_count;
   const int32_t* seg_first;
   const int32_t* seg_tile;
   const int32_t* seg_order;
@@ -586,7 +588,7 @@ __global__ void __launch_bounds__(WAVE* WPB) PF_BWD_OCC raster_bwd_kernel(BwdPar
   const int wv = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   WorkItem wi;
-  if (!work_item(p.n_segs, p.seg_order, p.seg_tile, p.seg_first, p.order, p.n_tiles, blockIdx.x, wv, wi)) return;
+  if (!work_item(p.n_segs, p.seg_count, p.seg_order, p.seg_tile, p.seg_first, p.order, p.n_tiles, blockIdx.x, wv, wi)) return;
   const int tile = wi.tile;
   const int early = p.early[tile];
   // this segment's share of the walked sites [0, early)
@@ -1358,6 +1360,7 @@ int pf_raster_fwd(const pf_raster_fwd_args* a, void* stream) {
         !sg.lane_active || !sg.scratch || sg.n_segs < nt)
       return set_error(PF_ERR_ARG, "pf_raster_fwd: bad segment table");
     p.n_segs = sg.n_segs;
+    p.seg_count = sg.count;
     p.seg_len = sg.seg_len;
     p.seg_first = sg.tile_first;
     p.seg_tile = sg.seg_tile;
@@ -1442,6 +1445,7 @@ int pf_raster_bwd(const pf_raster_bwd_args* a, void* stream) {
         !sg.lane_active || !sg.scratch || sg.n_segs < nt)
       return set_error(PF_ERR_ARG, "pf_raster_bwd: bad segment table");
     p.n_segs = sg.n_segs;
+    p.seg_count = sg.count;
     p.seg_len = sg.seg_len;
     p.seg_first = sg.tile_first;
     p.seg_tile = sg.seg_tile;

@@ -160,37 +160,44 @@ class Rasterizer:
         return offsets, sites[:m] if m > 0 else sites[:0], visible.bool(), nt
 
     def tile_order(self, offsets):
-        """Tiles by descending list length (int32), or None for the natural order."""
+        """Tiles by descending list length (int32, pf_tile_schedule), or None for
+        the natural order."""
         if not self.longest_first:
             return None
-        return torch.argsort(offsets[1:] - offsets[:-1], descending=True).to(torch.int32)
+        nt = offsets.shape[0] - 1
+        order = torch.empty(nt, dtype=torch.int32, device=offsets.device)
+        check(self.lib.pf_tile_schedule(ptr(offsets), nt, 0, 0, 0, ptr(order), None, None, None, None,
+                                        stream_handle()))
+        return order
 
-    def segments(self, offsets):
-        """Segment table of the compositing kernels (pf_segments), or None."""
+    def segments(self, offsets, n_records):
+        """Segment table of the compositing kernels (pf_segments) for lists of
+        n_records sites in all, with the longest-first tile order under
+        "tile_order", built on the device in one launch; or None."""
         if not self.seg_len:
             return None
         L = int(self.seg_len)
         nt = offsets.shape[0] - 1
         dev = offsets.device
-        cnt = offsets[1:] - offsets[:-1]
-        nseg = torch.clamp((cnt + L - 1) // L, min=1)
-        first = torch.zeros(nt + 1, dtype=torch.int64, device=dev)
-        first[1:] = torch.cumsum(nseg, 0)
-        n_segs = int(first[-1].item())
-        seg_tile = torch.repeat_interleave(torch.arange(nt, device=dev), nseg, output_size=n_segs)
-        k = torch.arange(n_segs, device=dev) - first[seg_tile]
-        size = torch.clamp(cnt[seg_tile] - k * L, max=L)
-        order = torch.argsort(size, descending=True)
-        return dict(n_segs=n_segs, seg_len=L, tile_first=first.to(torch.int32), seg_tile=seg_tile.to(torch.int32),
-                    order=order.to(torch.int32),
-                    log_t_sums=torch.empty(n_segs * 64, dtype=torch.float32, device=dev),
-                    lane_active=torch.empty(n_segs * 64, dtype=torch.uint8, device=dev))
+        cap = nt + int(n_records) // L
+        i32 = dict(dtype=torch.int32, device=dev)
+        sg = dict(n_segs=cap, seg_len=L, count=torch.empty(1, **i32), tile_first=torch.empty(nt + 1, **i32),
+                  seg_tile=torch.empty(cap, **i32), order=torch.empty(cap, **i32),
+                  tile_order=torch.empty(nt, **i32) if self.longest_first else None,
+                  log_t_sums=torch.empty(cap * 64, dtype=torch.float32, device=dev),
+                  lane_active=torch.empty(cap * 64, dtype=torch.uint8, device=dev))
+        to = sg["tile_order"]
+        check(self.lib.pf_tile_schedule(ptr(offsets), nt, int(n_records), L, cap, ptr(to) if to is not None else None,
+                                        ptr(sg["tile_first"]), ptr(sg["seg_tile"]), ptr(sg["order"]),
+                                        ptr(sg["count"]), stream_handle()))
+        return sg
 
     def _segs(self, sg, scratch=None):
         out = _abi.PfSegments()
         if sg is None:
             return out
         out.n_segs, out.seg_len = sg["n_segs"], sg["seg_len"]
+        out.count = sg["count"].data_ptr()
         out.tile_first, out.seg_tile = sg["tile_first"].data_ptr(), sg["seg_tile"].data_ptr()
         out.order, out.log_t_sums = sg["order"].data_ptr(), sg["log_t_sums"].data_ptr()
         out.lane_active = sg["lane_active"].data_ptr()
@@ -233,7 +240,7 @@ class Rasterizer:
         dq = _f32c(depth_quantiles) if depth_quantiles is not None else None
 
         offsets, tile_sites, prim_visible_mask, nt = self.bin(camera, all_spheres)
-        sg = self.segments(offsets)
+        sg = self.segments(offsets, tile_sites.shape[0])
         order = self.tile_order(offsets) if sg is None else None
         adjacency_diff = self.encode_adjacency(all_spheres, adjacency, adjacency_offsets)
 
@@ -276,7 +283,7 @@ class Rasterizer:
         grad_ctx.rasterizer = self
         grad_ctx.camera = camera
         grad_ctx.ray_maps = ray_maps
-        grad_ctx.order = order
+        grad_ctx.order = sg["tile_order"] if sg is not None else order
         grad_ctx.segs = sg
         grad_ctx.save_for_backward(dq, all_spheres, all_nsigmas, all_texel_sites, all_texel_rgbh, adjacency,
                                    adjacency_offsets, adjacency_diff, tile_sites, offsets, tile_early_stop,

@@ -59,7 +59,7 @@ def test_struct_layout_matches_c():
     n = lib.orc_abi_layout(out.ctypes.data, 64)
     C, S, TL, F, B = _abi.PfCamera, _abi.PfScene, _abi.PfTileLists, _abi.PfRasterFwdArgs, _abi.PfRasterBwdArgs
     RG, V, TA, TB = _abi.PfRasterRgbArgs, _abi.PfRasterVisArgs, _abi.PfTraceArgs, _abi.PfTraceBwdArgs
-    MR = _abi.PfMlpRows
+    MR, SG = _abi.PfMlpRows, _abi.PfSegments
     expect = [ctypes.sizeof(C), C.ray_maps.offset, C.fov_cos_cutoff.offset,
               ctypes.sizeof(S), S.adjacency_diff.offset,
               ctypes.sizeof(TL), TL.sites.offset,
@@ -69,6 +69,7 @@ def test_struct_layout_matches_c():
               ctypes.sizeof(V), V.bkgd_color.offset, V.intersections.offset,
               ctypes.sizeof(TA), TA.max_steps.offset, TA.n_cells.offset,
               ctypes.sizeof(TB), TB.grad_opacity.offset, TB.grad_texel_height.offset,
-              ctypes.sizeof(MR), MR.eye.offset, MR.latents.offset, ctypes.sizeof(_abi.PfMlpGrads)]
+              ctypes.sizeof(MR), MR.eye.offset, MR.latents.offset, ctypes.sizeof(_abi.PfMlpGrads),
+              ctypes.sizeof(SG), SG.count.offset, SG.seg_len.offset, SG.scratch.offset]
     assert n == len(expect)
     assert out[:n].tolist() == expect

@@ -102,3 +102,52 @@ def test_cone_binning_bitexact_gmm_fisheye():
     sp = np.concatenate([s["points"], s["radii"][:, None]], 1)
     m = _check(_fisheye_cam((4.0, 1.0, 0.0), 1024, 180.0), sp)
     assert m > 100_000
+
+
+@pytest.mark.parametrize("seg_len", [64, 512, 1024])
+def test_tile_schedule_orders_and_segment_table(seg_len):
+    """pf_tile_schedule: tile order and segment order are permutations sorted by
+    descending length bucket (exact below 512, steps of 32 up to 16,864, one
+    bucket above); the segment table is the exclusive scan of
+    max(1, ceil(count / seg_len)) with each segment naming its tile."""
+    import torch
+    from powerfoam_amd import _abi
+    from powerfoam_amd._abi import check, ptr
+    lib = _abi.load()
+    rng = np.random.default_rng(seg_len)
+    nt = 17115
+    cnt = np.minimum(rng.lognormal(4.0, 1.6, nt).astype(np.int64), 40000)
+    cnt[rng.random(nt) < 0.05] = 0
+    off = np.zeros(nt + 1, np.int64)
+    off[1:] = np.cumsum(cnt)
+    m = int(off[-1])
+    cap = nt + m // seg_len
+    dev = torch.device("cuda")
+    i32 = dict(dtype=torch.int32, device=dev)
+    offsets = torch.tensor(off, device=dev)
+    order, first = torch.empty(nt, **i32), torch.empty(nt + 1, **i32)
+    seg_tile, seg_order, count = torch.empty(cap, **i32), torch.empty(cap, **i32), torch.empty(1, **i32)
+    print(f"tile schedule nt={nt} m={m} seg_len={seg_len}", flush=True)
+    check(lib.pf_tile_schedule(ptr(offsets), nt, m, seg_len, cap, ptr(order), ptr(first), ptr(seg_tile),
+                               ptr(seg_order), ptr(count), None))
+    torch.cuda.synchronize()
+
+    def bucket(n):
+        n = np.asarray(n, np.int64)
+        return np.where(n < 512, n, np.minimum(512 + ((n - 512) >> 5), 1023))
+
+    o = order.cpu().numpy()
+    assert np.array_equal(np.sort(o), np.arange(nt))
+    assert np.all(np.diff(bucket(cnt[o])) <= 0)
+    ns = np.maximum(1, (cnt + seg_len - 1) // seg_len)
+    want_first = np.concatenate([[0], np.cumsum(ns)])
+    assert np.array_equal(first.cpu().numpy(), want_first)
+    total = int(want_first[-1])
+    assert int(count.item()) == total <= cap
+    st = seg_tile.cpu().numpy()[:total]
+    assert np.array_equal(st, np.repeat(np.arange(nt), ns))
+    so = seg_order.cpu().numpy()[:total]
+    assert np.array_equal(np.sort(so), np.arange(total))
+    size = cnt[st] - (np.arange(total) - want_first[st]) * seg_len
+    size = np.minimum(size, seg_len)
+    assert np.all(np.diff(np.minimum(size[so], 1023)) <= 0)
