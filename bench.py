@@ -173,6 +173,121 @@ def run_mlp_head(args, rast, w, dev):
     print(json.dumps(line), flush=True)
 
 
+def run_trace(args, dev):
+    """Ray tracer workload: the 1M-site GMM scene (seed 1) with its full power
+    adjacency (the tracer walks power cells; alpha filtering would cut cells
+    open), one equidistant 180-degree fisheye view of 1024x1024 rays per camera
+    of the circle, starting in the cell of the eye (benchmark.py:320-324), with
+    exact neighbour geometry (the fp16 face encoding of RayTracer.benchmark
+    rounds the faces around a cell edge independently, so they no longer meet
+    in one line, and on this scene a few rays circle such an edge until the
+    walk cap); per view the SH head, the tracer forward (colour, opacity,
+    depth, normals) and the backward of an L2 colour loss. Reports rays/s over
+    the timed views and the mean cells visited per ray. Algorithmic bytes
+    count, per distinct cell a wave of 64 rays visits (the kernels walk an 8x8
+    block of rays together): site record 32 B + CSR offsets 8 B + 20 B per face
+    (neighbour id and sphere) + 8 texels 224 B, and 40 B of outputs per ray
+    (forward); the backward reads as much and read-modify-writes the 256 B of
+    site and texel gradients per distinct cell, with 64 B of saved and
+    incoming data per ray."""
+    import torch
+    from types import SimpleNamespace
+    from powerfoam_amd import synth
+    from powerfoam_amd.camera import TorchCamera, fisheye_ray_maps, look_at
+    from powerfoam_amd.color_fn import SHColor
+    from powerfoam_amd.raytrace import RayTracer, start_cell
+
+    size = 1024
+    t0 = time.time()
+    s = synth.scene_gmm(1, n=args.sites)
+    adj, off, stats = synth.power_adjacency(s["points"], s["radii"], alpha_complex=False, return_stats=True)
+    log(f"scene + full power adjacency {stats} in {time.time() - t0:.1f}s")
+    tt = lambda a, dt=torch.float32: torch.tensor(np.ascontiguousarray(a), dtype=dt, device=dev)
+    p = dict(points=tt(s["points"]), radii=tt(s["radii"]), density=tt(s["density"]), normals=tt(s["normals"]),
+             texel_sites=tt(s["texel_sites"]), sh=tt(s["sh"]), texel_height=tt(s["texel_height"]))
+    for v in p.values():
+        v.requires_grad_(True)
+    adjacency, offsets = tt(adj, torch.int32), tt(off, torch.int32)
+    cams, starts = [], []
+    for c in synth.circle_cameras(n_views=args.views):
+        eye = c.eye.numpy().astype(np.float64)
+        rm = torch.tensor(fisheye_ray_maps(eye, (0.0, 0.0, 0.0), size), device=dev)
+        t = look_at(eye, (0.0, 0.0, 0.0), size, size, 90.0)
+        cams.append(TorchCamera(t.eye, t.right, t.up, size, size, rm))
+        starts.append(start_cell(p["points"].detach(), p["radii"].detach(), eye))
+    tracer = RayTracer(SimpleNamespace(num_texel_sites=8, is_pinhole=True), dev)
+    head = SHColor()
+    target = tt(np.random.default_rng(1234).uniform(0.0, 1.0, (size, size, 3)).astype(np.float32))
+
+    def view(i):
+        cam = cams[i % len(cams)]
+        rgb = head.forward(p["texel_sites"], cam, p["sh"])
+        col, opac, depth, normal = tracer.forward(cam, starts[i % len(cams)], p["points"], p["radii"], p["density"],
+                                                  p["normals"], p["texel_sites"], rgb, p["texel_height"], adjacency,
+                                                  offsets, None, 1e-3, 0.5)
+        torch.mean((col - target) ** 2).backward()
+
+    for i in range(args.warmup):
+        view(i)
+    torch.cuda.synchronize()
+    tracer.event_log = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        view(i)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    kern = {}
+    for name, a, b in tracer.event_log:
+        kern.setdefault(name, []).append(a.elapsed_time(b))
+    tracer.event_log = None
+    kern_ms = {k: float(np.mean(v)) for k, v in kern.items()}
+    # visited cells per ray, and the cells each wave (an 8x8 block of rays,
+    # which the kernels run together) visits at least once: the bytes a wave
+    # must fetch once, over the timed views
+    deg = (offsets[1:] - offsets[:-1]).to(torch.int64)
+    visits, uniq, uniq_faces = [], [], []
+    K = 256
+    with torch.no_grad():
+        for i in range(min(args.steps, len(cams))):
+            cam = cams[i]
+            rgb = head.forward(p["texel_sites"], cam, p["sh"])
+            cells, n_cells = tracer.trace_cells(cam, starts[i], p["points"], p["radii"], p["density"], p["normals"],
+                                                p["texel_sites"], rgb, p["texel_height"], adjacency, offsets, None,
+                                                1e-3, max_cells=K)
+            visits.append(float(n_cells.to(torch.float64).sum()))
+            w = cells.view(size // 8, 8, size // 8, 8, K).permute(2, 0, 1, 3, 4).reshape(-1, 64 * K)
+            w = torch.sort(w, dim=1).values
+            first = torch.ones_like(w, dtype=torch.bool)
+            first[:, 1:] = w[:, 1:] != w[:, :-1]
+            first &= w >= 0
+            u = w[first].to(torch.int64)
+            uniq.append(float(u.numel()))
+            uniq_faces.append(float(deg[u].sum()))
+            del cells, w, first
+    rays = size * size
+    v_mean, u_mean, uf_mean = float(np.mean(visits)), float(np.mean(uniq)), float(np.mean(uniq_faces))
+    fwd_bytes = u_mean * (32 + 8 + 224) + 20 * uf_mean + rays * 40
+    bwd_bytes = u_mean * (32 + 8 + 224 + 2 * 256) + 20 * uf_mean + rays * 64
+    abytes = dict(trace_fwd=fwd_bytes, trace_bwd=bwd_bytes)
+    dom = max(kern_ms, key=kern_ms.get)
+    achieved = abytes[dom] / (kern_ms[dom] * 1e-3) / 1e9
+    ms = elapsed * 1e3 / args.steps
+    line = dict(metric="ray tracer fwd+bwd, Mrays/s", value=rays * args.steps / elapsed / 1e6, unit="Mrays/s",
+                n_gpus=1, steps=args.steps, warmup=args.warmup, ms_per_step=ms, higher_is_better=True,
+                scaling="weak", vs_baseline=None, dtype="f32", data="synthetic (seeded scene generator)",
+                config=dict(workload=f"ray tracer fwd+bwd with SH-3 colour head, {args.sites}-site GMM scene, "
+                                     f"{args.views} equidistant 180-degree fisheye views {size}x{size}",
+                            sites=args.sites, rays=rays, views=args.views, adjacency="full power",
+                            adjacency_edges=int(stats["edges"]), cells_per_ray=v_mean / rays,
+                            distinct_cells_per_wave=u_mean / (rays / 64),
+                            faces_per_distinct_cell=uf_mean / max(u_mean, 1.0)),
+                kernel_ms=kern_ms,
+                roofline=dict(bound="hbm", kernel=dom, achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
+                              frac=achieved / HBM_PEAK_GBS, traffic=None, algorithmic_bytes=abytes[dom]))
+    log(f"{ms:.3f} ms/view, {line['value']:.1f} Mrays/s; kernels {kern_ms}; cells/ray {v_mean / rays:.1f}")
+    print(json.dumps(line), flush=True)
+
+
 def workload_counts(rast, w, width, height):
     """Per-frame counts for the algorithmic byte formulas, from the last frame."""
     import torch
@@ -254,6 +369,8 @@ def main():
                     help="mlp: the learned-head workload (3M-site scene, MLP vs SH head, PSNR)")
     ap.add_argument("--mlp-steps", type=int, default=2000)
     ap.add_argument("--mlp-batch", type=int, default=65536)
+    ap.add_argument("--workload", choices=("raster", "trace"), default="raster",
+                    help="raster: the rasterizer frame (default); trace: the ray tracer on fisheye views")
     args = ap.parse_args()
     if args.head == "mlp" and args.sites == 1_000_000:
         args.sites = 3_000_000
@@ -272,6 +389,8 @@ def main():
     from powerfoam_amd.rasterize import Rasterizer
 
     dev = torch.device("cuda:0")
+    if args.workload == "trace":
+        return run_trace(args, dev)
     w = build_workload(args.sites, args.views, args.width, args.height, dev, seed=2 if args.head == "mlp" else 1)
     rast = Rasterizer(SimpleNamespace(num_texel_sites=8, render_objective="volume", is_pinhole=True,
                                       disable_coop_prim_load=False, disable_coop_adj_load=False), dev)

@@ -412,11 +412,14 @@ __global__ void __launch_bounds__(TWAVE* TWPB) trace_bwd_kernel(TraceParams p) {
           const float w = te.heights ? fast_exp(dot(df, df) * kexp) : w1[k];
           const float wn0 = w * inv_ws0;
           atomicAdd(gth + k, wn0 * dLdheight);
-          const f3 sg = df * (kw * wn0 * (dLdheight * hk - gdp0_in));
-          atomicAdd(gts + 3 * k + 0, sg.x);
-          atomicAdd(gts + 3 * k + 1, sg.y);
-          atomicAdd(gts + 3 * k + 2, sg.z);
-          dLdx0 = dLdx0 - sg;
+          // all heights zero: hk = 0 and gdp0_in = 0, so sg is zero
+          if (te.heights) {
+            const f3 sg = df * (kw * wn0 * (dLdheight * hk - gdp0_in));
+            atomicAdd(gts + 3 * k + 0, sg.x);
+            atomicAdd(gts + 3 * k + 1, sg.y);
+            atomicAdd(gts + 3 * k + 2, sg.z);
+            dLdx0 = dLdx0 - sg;
+          }
         }
         const float dLdtq0 = dot(dLdx0, d);
         dtn_q += (te.dpn >= 0.0f || tn0 >= te.ts0) ? dLdtq0 : 0.0f;
