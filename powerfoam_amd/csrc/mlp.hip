@@ -354,13 +354,27 @@ __global__ void __launch_bounds__(WAVE* BW_WPB) mlp_bwd_kernel(MlpParams p) {
   for (int step = blockIdx.x * BW_WPB + wv; step < n_steps; step += gridDim.x * BW_WPB) {
     const int base = step * BW_ROWS;
     wave_sync();
+    // incoming colour gradients of both halves (lanes of group 0), loaded
+    // before the forward recompute so that it hides their latency
+    float gr[2][3] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
+    bool okr[2] = {false, false};
+    if (g == 0) {
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const int row = base + 16 * h + c16;
+        okr[h] = row_ok(p, row);
+        if (okr[h]) {
+#pragma unroll
+          for (int j = 0; j < 3; j++) gr[h][j] = p.grad_rgb[3 * (size_t)row + j];
+        }
+      }
+    }
     if (lane < BW_ROWS) stage_row(p, base + lane, sx + lane * XS);
     wave_sync();
     uint2(*t0)[WAVE] = sm.t0[wv];
 #pragma unroll
     for (int h = 0; h < 2; h++) {
       const int r = 16 * h + c16;
-      const int row = base + r;
       HalfT T;
       const bf16x8 x = ld8(sx + r * XS + 8 * g);
 #pragma unroll
@@ -389,9 +403,9 @@ __global__ void __launch_bounds__(WAVE* BW_WPB) mlp_bwd_kernel(MlpParams p) {
       const f32x4 o = mfma(im_ld(sm, IM_W3 + 1, lane), h2[1], mfma(im_ld(sm, IM_W3, lane), h2[0], ld_f4(sm.b3)));
       // dL/d(pre-clamp output): lanes of group 0, outputs 0..2 of row r
       float d_out[3] = {0.0f, 0.0f, 0.0f};
-      if (g == 0 && row_ok(p, row)) {
+      if (okr[h]) {
 #pragma unroll
-        for (int j = 0; j < 3; j++) d_out[j] = o[j] > 0.0f ? p.grad_rgb[3 * (size_t)row + j] : 0.0f;
+        for (int j = 0; j < 3; j++) d_out[j] = o[j] > 0.0f ? gr[h][j] : 0.0f;
       }
       const uint32_t do01 = pk_bf(d_out[0], d_out[1]), do23 = pk_bf(d_out[2], 0.0f);
       const bf16x8 dob = __builtin_bit_cast(bf16x8, make_uint4(do01, do23, 0u, 0u));

@@ -271,40 +271,87 @@ __global__ void __launch_bounds__(TWAVE* TWPB) trace_fwd_kernel(TraceParams p) {
   if (p.n_cells) p.n_cells[px] = step;
 }
 
+// Gradient cache of a wave. The 64 rays of a wave walk mostly the same
+// cells (on a 1M-site fisheye view a wave visits ~110 distinct cells for
+// 64 x 50 visits), so a visit adds its site's gradients (texel colours and
+// positions, heights, sphere, normal and density: 7T + 8 values) into an LDS
+// slot of that site, and a slot goes out with one atomic per value when
+// another site claims it or the wave ends. Slot = site mod GC_SLOTS; a lane
+// whose slot another lane's site won this step adds to global memory
+// directly. Only the summation order of the gradients changes.
+constexpr int GC_SLOTS = 32;
+constexpr int GC_VALS = 7 * PF_MAX_TEXELS + 8;
+
+struct GradCache {
+  int tag[GC_SLOTS];
+  float v[GC_SLOTS][GC_VALS];
+};
+
+// address of gradient value k (< 7T + 8) of site s
+__device__ __forceinline__ float* grad_addr(const TraceParams& p, int s, int k, int T) {
+  if (k < 3 * T) return p.g_trgb + (size_t)s * 3 * T + k;
+  k -= 3 * T;
+  if (k < 3 * T) return p.g_tsites + (size_t)s * 3 * T + k;
+  k -= 3 * T;
+  if (k < T) return p.g_theight + (size_t)s * T + k;
+  k -= T;
+  if (k < 4) return p.g_spheres + 4 * (size_t)s + k;
+  return p.g_nsigmas + 4 * (size_t)s + (k - 4);
+}
+
+// all 64 lanes: slot q's values to global memory (7T + 8 <= 64 values)
+__device__ __forceinline__ void gc_flush(const TraceParams& p, const GradCache& gc, int q, int lane, int T) {
+  const int s = gc.tag[q];
+  if (s >= 0 && lane < 7 * T + 8) {
+    const float v = gc.v[q][lane];
+    if (v != 0.0f) atomicAdd(grad_addr(p, s, lane, T), v);
+  }
+}
+
 __global__ void __launch_bounds__(TWAVE* TWPB) trace_bwd_kernel(TraceParams p) {
+  __shared__ GradCache s_gc[TWPB];
   const int tile = blockIdx.x * TWPB + (threadIdx.x >> 6);
   if (tile >= p.n_tiles) return;
+  const int lane = threadIdx.x & 63;
+  GradCache& gc = s_gc[threadIdx.x >> 6];
+  if (lane < GC_SLOTS) gc.tag[lane] = -1;
   int i, j;
-  if (!trace_lane(p, tile, threadIdx.x & 63, i, j)) return;
-  const size_t px = (size_t)i * p.W + j;
-  f3 o, d;
-  walk_ray(p, i, j, o, d);
+  // lanes past the image edge take part in the wave's cache bookkeeping only
+  bool live = trace_lane(p, tile, lane, i, j);
+  const size_t px = live ? (size_t)i * p.W + j : 0;
+  f3 o{0.0f, 0.0f, 0.0f}, d{0.0f, 0.0f, 1.0f};
+  if (live) walk_ray(p, i, j, o, d);
   const int T = p.n_texels;
-  const f3 gC = ld3(p.g_color + 3 * px), gN = ld3(p.g_normal + 3 * px);
-  const float gD = p.g_depth ? p.g_depth[px] : 0.0f;
-  const float gO = p.g_opacity[px];
-  const f3 Cf = ld3(p.s_color + 3 * px), Nf = ld3(p.s_normal + 3 * px);
-  const float Tend = fast_exp(p.s_log_t[px]);
-  const int qstep = p.s_q_step[px];
-  const float qsig = p.s_q_sigma[px];
-  int prim = p.start_sites ? p.start_sites[px] : p.start_site;
+  const f3 gC = live ? ld3(p.g_color + 3 * px) : f3{0.0f, 0.0f, 0.0f};
+  const f3 gN = live ? ld3(p.g_normal + 3 * px) : f3{0.0f, 0.0f, 0.0f};
+  const float gD = (live && p.g_depth) ? p.g_depth[px] : 0.0f;
+  const float gO = live ? p.g_opacity[px] : 0.0f;
+  const f3 Cf = live ? ld3(p.s_color + 3 * px) : f3{0.0f, 0.0f, 0.0f};
+  const f3 Nf = live ? ld3(p.s_normal + 3 * px) : f3{0.0f, 0.0f, 0.0f};
+  const float Tend = live ? fast_exp(p.s_log_t[px]) : 0.0f;
+  const int qstep = live ? p.s_q_step[px] : -1;
+  const float qsig = live ? p.s_q_sigma[px] : 1.0f;
+  int prim = live ? (p.start_sites ? p.start_sites[px] : p.start_site) : SENTINEL;
   float log_t = 0.0f;
   int step = 0;
   f3 Cp{0.0f, 0.0f, 0.0f}, Np{0.0f, 0.0f, 0.0f};
+  wave_sync();
   while (true) {
     const float trans = fast_exp(log_t);
-    if (trans < p.thr || prim == SENTINEL || prim < 0 || step >= p.max_steps) break;
-    const float4 sph = p.spheres[prim];
+    if (live && (trans < p.thr || prim == SENTINEL || prim < 0 || step >= p.max_steps)) live = false;
+    if (ballot(live) == 0) break;
+    float4 sph = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    if (live) sph = p.spheres[prim];
     const f3 c{sph.x, sph.y, sph.z};
     const float r = sph.w;
-    float tn, tf;
+    float tn = 0.0f, tf = 0.0f;
     const f3 oc = o - c;
-    bool hit = sphere_hit(oc, d, r, tn, tf);
+    bool hit = live && sphere_hit(oc, d, r, tn, tf);
     if (__builtin_sqrtf(dot(oc, oc)) < 4.0f * r) hit = false;
     int tn_id = tn < 1e-8f ? -2 : -1, tf_id = tf < 1e-8f ? -2 : -1;
     int next = SENTINEL;
     float pt_far = 1e10f;
-    const int a0 = p.adj_off[prim], a1 = p.adj_off[prim + 1];
+    const int a0 = live ? p.adj_off[prim] : 0, a1 = live ? p.adj_off[prim + 1] : 0;
     const int n_adj = a1 - a0;
     for (int e = a0; e < a1; e++) {
       f3 n;
@@ -324,15 +371,18 @@ __global__ void __launch_bounds__(TWAVE* TWPB) trace_bwd_kernel(TraceParams p) {
         tn = t;
       }
     }
-    const float4 ns = p.nsigmas[prim];
+    float4 ns = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (live) ns = p.nsigmas[prim];
     const float sigma = ns.w;
-    if (hit && !(tn > tf) && !(sigma < 1e-3f)) {
-      const f3 pn{ns.x, ns.y, ns.z};
-      const f3 co = -oc;
-      float w1[PF_MAX_TEXELS];
-      const float tn0 = tn;
-      const int tn0_id = tn_id, tf0_id = tf_id;
-      const TexelEval te = texel_eval(p, prim, c, r, pn, o, d, tn, w1);
+    const f3 pn{ns.x, ns.y, ns.z};
+    const f3 co = -oc;
+    float w1[PF_MAX_TEXELS];
+    const float tn0 = tn;
+    const int tn0_id = tn_id, tf0_id = tf_id;
+    TexelEval te{};
+    float dt = -1.0f;
+    if (live && hit && !(tn > tf) && !(sigma < 1e-3f)) {
+      te = texel_eval(p, prim, c, r, pn, o, d, tn, w1);
       if (te.dpn >= 0.0f && te.ts1 < tf) {
         tf_id = n_adj;
         tf = te.ts1;
@@ -341,8 +391,47 @@ __global__ void __launch_bounds__(TWAVE* TWPB) trace_bwd_kernel(TraceParams p) {
         tn_id = n_adj;
         tn = te.ts1;
       }
-      const float dt = tf - tn;
-      if (dt > 0.0f) {
+      dt = tf - tn;
+    }
+    const bool act = dt > 0.0f;
+    // the cache slot of this visit's site
+    const int slot = prim & (GC_SLOTS - 1);
+    bool in_lds = false;
+    wave_sync();  // the last step's cache adds land before any flush reads them
+    {
+      bool pending = act;
+      if (pending && gc.tag[slot] == prim) {
+        in_lds = true;
+        pending = false;
+      }
+      uint64_t pm;
+      while ((pm = ballot(pending)) != 0) {
+        const int leader = __builtin_ctzll(pm);
+        const int ls = readlane(slot, leader), lp = readlane(prim, leader);
+        // a slot that holds a site visited this step stays; its rivals go direct
+        const bool taken = ballot(in_lds && slot == ls) != 0;
+        if (!taken) {
+          gc_flush(p, gc, ls, lane, T);
+          wave_sync();
+          if (lane < GC_VALS) gc.v[ls][lane] = 0.0f;
+          if (lane == 0) gc.tag[ls] = lp;
+          wave_sync();
+        }
+        if (pending && slot == ls) {
+          in_lds = !taken && prim == lp;
+          pending = false;
+        }
+      }
+    }
+    float* cache = gc.v[slot];
+    // value k of this visit's site gradient: into the cache, or to global memory
+    auto acc = [&](int k, float v) {
+      if (in_lds)
+        atomicAdd(cache + k, v);
+      else
+        atomicAdd(grad_addr(p, prim, k, T), v);
+    };
+    if (act) {
         const float u = sigma * dt;
         const float e_u = fast_exp(-u);
         const float alpha = 1.0f - e_u;
@@ -367,9 +456,6 @@ __global__ void __launch_bounds__(TWAVE* TWPB) trace_bwd_kernel(TraceParams p) {
         const float kw = 20.0f * fast_rcp(r * r);
         const float kexp = -10.0f * fast_rcp(r * r);
         const float inv_ws = fast_rcp(te.ws), inv_ws0 = fast_rcp(te.ws0);
-        float* gts = p.g_tsites + (size_t)prim * T * 3;
-        float* gtr = p.g_trgb + (size_t)prim * T * 3;
-        float* gth = p.g_theight + (size_t)prim * T;
         float dLdheight = 0.0f;
         f3 pdc{0.0f, 0.0f, 0.0f}, pdn{0.0f, 0.0f, 0.0f};
         const float tq = te.dpn >= 0.0f ? tn0 : sel_max(tn0, te.ts1);
@@ -381,14 +467,14 @@ __global__ void __launch_bounds__(TWAVE* TWPB) trace_bwd_kernel(TraceParams p) {
           const float4 cl = p.texel_rgbh[(size_t)prim * T + k];
           const f3 df{x1.x - (ts[0] - c.x), x1.y - (ts[1] - c.y), x1.z - (ts[2] - c.z)};
           const float wn = w1[k] * inv_ws;
-          atomicAdd(gtr + 3 * k + 0, dLdcol.x * wn);
-          atomicAdd(gtr + 3 * k + 1, dLdcol.y * wn);
-          atomicAdd(gtr + 3 * k + 2, dLdcol.z * wn);
+          acc(3 * k + 0, dLdcol.x * wn);
+          acc(3 * k + 1, dLdcol.y * wn);
+          acc(3 * k + 2, dLdcol.z * wn);
           const float gdp = dLdcol.x * cl.x + dLdcol.y * cl.y + dLdcol.z * cl.z;
           const f3 sg = df * (kw * wn * (gdp - gdp_in));
-          atomicAdd(gts + 3 * k + 0, sg.x);
-          atomicAdd(gts + 3 * k + 1, sg.y);
-          atomicAdd(gts + 3 * k + 2, sg.z);
+          acc(3 * T + 3 * k + 0, sg.x);
+          acc(3 * T + 3 * k + 1, sg.y);
+          acc(3 * T + 3 * k + 2, sg.z);
           dLdx = dLdx - sg;
         }
         const float dLdtq = dot(dLdx, d);
@@ -411,13 +497,13 @@ __global__ void __launch_bounds__(TWAVE* TWPB) trace_bwd_kernel(TraceParams p) {
           const f3 df{x0.x - (ts[0] - c.x), x0.y - (ts[1] - c.y), x0.z - (ts[2] - c.z)};
           const float w = te.heights ? fast_exp(dot(df, df) * kexp) : w1[k];
           const float wn0 = w * inv_ws0;
-          atomicAdd(gth + k, wn0 * dLdheight);
+          acc(6 * T + k, wn0 * dLdheight);
           // all heights zero: hk = 0 and gdp0_in = 0, so sg is zero
           if (te.heights) {
             const f3 sg = df * (kw * wn0 * (dLdheight * hk - gdp0_in));
-            atomicAdd(gts + 3 * k + 0, sg.x);
-            atomicAdd(gts + 3 * k + 1, sg.y);
-            atomicAdd(gts + 3 * k + 2, sg.z);
+            acc(3 * T + 3 * k + 0, sg.x);
+            acc(3 * T + 3 * k + 1, sg.y);
+            acc(3 * T + 3 * k + 2, sg.z);
             dLdx0 = dLdx0 - sg;
           }
         }
@@ -470,22 +556,23 @@ __global__ void __launch_bounds__(TWAVE* TWPB) trace_bwd_kernel(TraceParams p) {
             atomicAdd(gadj + 3, g * (-as.w * idp));
           }
         }
-        float* gs = p.g_spheres + 4 * (size_t)prim;
-        atomicAdd(gs + 0, dLdc.x);
-        atomicAdd(gs + 1, dLdc.y);
-        atomicAdd(gs + 2, dLdc.z);
-        atomicAdd(gs + 3, dLdr);
-        float* gq = p.g_nsigmas + 4 * (size_t)prim;
-        atomicAdd(gq + 0, dLdnorm.x);
-        atomicAdd(gq + 1, dLdnorm.y);
-        atomicAdd(gq + 2, dLdnorm.z);
-        atomicAdd(gq + 3, dLdsig);
+        acc(7 * T + 0, dLdc.x);
+        acc(7 * T + 1, dLdc.y);
+        acc(7 * T + 2, dLdc.z);
+        acc(7 * T + 3, dLdr);
+        acc(7 * T + 4, dLdnorm.x);
+        acc(7 * T + 5, dLdnorm.y);
+        acc(7 * T + 6, dLdnorm.z);
+        acc(7 * T + 7, dLdsig);
         log_t -= u;
-      }
     }
-    prim = next;
-    step++;
+    if (live) {
+      prim = next;
+      step++;
+    }
   }
+  wave_sync();
+  for (int q = 0; q < GC_SLOTS; q++) gc_flush(p, gc, q, lane, T);
 }
 
 static TraceParams make_params(const pf_trace_args* a) {
