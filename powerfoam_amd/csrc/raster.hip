@@ -251,12 +251,13 @@ __device__ __forceinline__ bool work_item(int n_segs, const int32_t* seg_count, 
 }
 
 // LOGT = true: the segment prepass of a split list. It walks the segment's
-// sites without colour, from T = 1, and stores the per-lane sum of
-// -sigma dt, from which every later segment of the tile starts; it ends early
-// once every lane's own T is below the threshold (the true T is lower still).
-// A stop inside an earlier segment leaves every lane below the threshold for
+// sites without the early stop and without colour, and stores the per-lane
+// sum of -sigma dt, from which every later segment of the tile starts. A
+// stop inside an earlier segment leaves every lane below the threshold for
 // all later sites (T only decreases), so later segments stop at their first
-// site and the split reproduces the serial walk.
+// site and the split reproduces the serial walk. (Ending the prepass once
+// every lane's own T is below the threshold was measured slower: long lists
+// on these views rarely get that opaque within one segment.)
 template <bool PINHOLE, bool LOGT>
 __global__ void __launch_bounds__(WAVE* WPB) raster_fwd_kernel(FwdParams p) {
   __shared__ SiteRec s_site_all[WPB][WAVE];
@@ -319,12 +320,8 @@ __global__ void __launch_bounds__(WAVE* WPB) raster_fwd_kernel(FwdParams p) {
     }
     wave_sync();
     for (int kk = 0; kk < cnt; kk++) {
-      const float trans = fast_exp(log_t);
-      // The prepass sums from T = 1, so its T bounds the true one from above:
-      // once every lane is below the threshold the tile stops in this segment
-      // or earlier, later segments only need to see T below it (they stop at
-      // their first site), and no backward reads this segment's sum.
-      if (ballot(trans >= p.thr) == 0) {
+      const float trans = LOGT ? 1.0f : fast_exp(log_t);
+      if (!LOGT && ballot(trans >= p.thr) == 0) {
         early = base + kk;
         stop = true;
         break;
