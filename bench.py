@@ -64,9 +64,15 @@ def build_workload(n_sites, n_views, width, height, dev, seed=1):
 
 
 def frame(rast, head, w, cam):
-    """One training-style frame: colour head, rasterize, L2 loss, backward."""
+    """One training-style frame: colour head, rasterize, L2 loss, backward.
+    Gradients start from None, as after an optimiser step with
+    zero_grad(set_to_none=True)."""
     import torch
     p = w["params"]
+    for v in p.values():
+        v.grad = None
+    if hasattr(head, "zero_grad"):
+        head.zero_grad()
     texel_rgb = head.forward(p["texel_sites"], cam, p["sh"])
     out = rast.forward(cam, None, p["points"], p["radii"], p["density"], p["normals"], p["texel_sites"],
                        texel_rgb, p["texel_height"], w["adjacency"], w["adjacency_offsets"], w["target"], True)
@@ -83,6 +89,10 @@ class MlpSlot:
 
     def forward(self, texel_sites, camera, sh):
         return self.mlp.forward(texel_sites, camera)
+
+    def zero_grad(self):
+        for q in self.mlp.parameters():
+            q.grad = None
 
 
 def time_frames(rast, head, w, steps, heads_log=None):

@@ -49,3 +49,32 @@ def test_sh_head_matches_fp64(n, T):
     (rgb * torch.tensor(g, dtype=torch.float32, device="cuda")).sum().backward()
     want_g = np.einsum("ntk,ntc->nkc", y, g * (lin > 0.0))
     np.testing.assert_allclose(sh_t.grad.cpu().numpy(), want_g, rtol=0, atol=1e-5)
+
+
+def test_sh_head_unaligned_arrays_match_aligned():
+    """Arrays not 16-byte aligned take the kernels' 4-byte copy path: same
+    colours and coefficient gradients as aligned copies of the same data."""
+    import torch
+    from powerfoam_amd.camera import look_at
+    from powerfoam_amd.color_fn import SHColor
+    rng = np.random.default_rng(7)
+    n, T = 300, 8
+    sh = rng.normal(0.0, 0.3, (n, 16, 3)).astype(np.float32)
+    ts = rng.normal(0.0, 1.0, (n, T, 3)).astype(np.float32)
+    cam = look_at((0.3, -0.2, 4.0), (0.0, 0.0, 0.0), 64, 48, 60.0)
+    g = torch.tensor(rng.normal(size=(n, T, 3)).astype(np.float32), device="cuda")
+    outs = []
+    for off in (0, 1):
+        flat_sh = torch.zeros(n * 48 + off, device="cuda")
+        flat_sh[off:] = torch.tensor(sh.ravel(), device="cuda")
+        flat_ts = torch.zeros(n * T * 3 + off, device="cuda")
+        flat_ts[off:] = torch.tensor(ts.ravel(), device="cuda")
+        sh_t = flat_sh[off:].view(n, 16, 3).detach().requires_grad_(True)
+        ts_t = flat_ts[off:].view(n, T, 3)
+        assert (ts_t.data_ptr() % 16 != 0) == (off == 1)
+        print(f"sh head offset {off}", flush=True)
+        rgb = SHColor().forward(ts_t, cam, sh_t)
+        (rgb * g).sum().backward()
+        outs.append((rgb.detach().cpu().numpy(), sh_t.grad.cpu().numpy()))
+    np.testing.assert_array_equal(outs[0][0], outs[1][0])
+    np.testing.assert_array_equal(outs[0][1], outs[1][1])
