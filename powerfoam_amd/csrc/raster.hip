@@ -138,6 +138,24 @@ struct SiteRec {
   int site, a0, a1, pad;
 };
 
+__device__ __forceinline__ float uniform_f(float x) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, x)));
+}
+
+// A site record read from LDS by every lane is the same in each: saying so
+// puts it in scalar registers, so the per-site loops (faces, texels) count
+// and branch in scalar code.
+__device__ __forceinline__ SiteRec uniform_site(const SiteRec& r) {
+  SiteRec u;
+  u.sph = make_float4(uniform_f(r.sph.x), uniform_f(r.sph.y), uniform_f(r.sph.z), uniform_f(r.sph.w));
+  u.nsg = make_float4(uniform_f(r.nsg.x), uniform_f(r.nsg.y), uniform_f(r.nsg.z), uniform_f(r.nsg.w));
+  u.site = __builtin_amdgcn_readfirstlane(r.site);
+  u.a0 = __builtin_amdgcn_readfirstlane(r.a0);
+  u.a1 = __builtin_amdgcn_readfirstlane(r.a1);
+  u.pad = 0;
+  return u;
+}
+
 // Face (n, w) of a site against neighbour e: n = neighbour delta and
 // w = (|n|^2 + r^2 - r_a^2)/2, plus (c - eye).n for pinhole cameras; the ray
 // crosses it at t = (w + (c - o).n) / (d.n) -- for a pinhole the second term
@@ -326,7 +344,7 @@ __global__ void __launch_bounds__(WAVE* WPB) raster_fwd_kernel(FwdParams p) {
         stop = true;
         break;
       }
-      const SiteRec sr = s_site[kk];
+      const SiteRec sr = uniform_site(s_site[kk]);
       const f3 c{sr.sph.x, sr.sph.y, sr.sph.z};
       const float r = sr.sph.w;
       const f3 pn{sr.nsg.x, sr.nsg.y, sr.nsg.z};
@@ -340,20 +358,20 @@ __global__ void __launch_bounds__(WAVE* WPB) raster_fwd_kernel(FwdParams p) {
       // power faces (rasterize.py:1038-1071), staged 64 at a time
       const f3 ce = c - p.cam.eye;
       for (int e0 = sr.a0; e0 < sr.a1; e0 += WAVE) {
-        const int ec = min(WAVE, sr.a1 - e0);
+        const int ec = __builtin_amdgcn_readfirstlane(min(WAVE, sr.a1 - e0));
         wave_sync();
         if (lane < ec) s_face[lane] = face_of<PINHOLE>(p.adjacency, p.adj_diff, p.spheres, e0 + lane, sr.sph, ce);
         wave_sync();
-        if (hit) {
-          for (int e = 0; e < ec; e++) {
-            const float4 f = s_face[e];
-            const f3 n{f.x, f.y, f.z};
-            const float dp = dot(ray.d, n);
-            const float num = PINHOLE ? f.w : f.w + dot(co, n);
-            const float t = acc_div(num, dp);
-            tf = (dp >= 0.0f && t < tf) ? t : tf;
-            tn = (dp < 0.0f && t > tn) ? t : tn;
-          }
+        // every lane runs the faces (a lane that misses the sphere only
+        // updates values nothing reads): the loop stays in scalar control
+        for (int e = 0; e < ec; e++) {
+          const float4 f = s_face[e];
+          const f3 n{f.x, f.y, f.z};
+          const float dp = dot(ray.d, n);
+          const float num = PINHOLE ? f.w : f.w + dot(co, n);
+          const float t = acc_div(num, dp);
+          tf = (dp >= 0.0f && t < tf) ? t : tf;
+          tn = (dp < 0.0f && t > tn) ? t : tn;
         }
       }
       const bool shade = hit && !ray.oob && tn < tf;
@@ -659,7 +677,7 @@ __global__ void __launch_bounds__(WAVE* WPB) PF_BWD_OCC raster_bwd_kernel(BwdPar
     }
     wave_sync();
     for (int kk = cnt - 1; kk >= 0; kk--) {
-      const SiteRec sr = s_site[kk];
+      const SiteRec sr = uniform_site(s_site[kk]);
       const f3 c{sr.sph.x, sr.sph.y, sr.sph.z};
       const float r = sr.sph.w;
       const f3 pn{sr.nsg.x, sr.nsg.y, sr.nsg.z};
@@ -675,26 +693,26 @@ __global__ void __launch_bounds__(WAVE* WPB) PF_BWD_OCC raster_bwd_kernel(BwdPar
       int tf_id = tf < 1e-8f ? -2 : -1;
       const f3 ce = c - p.cam.eye;
       for (int e0 = sr.a0; e0 < sr.a1; e0 += WAVE) {
-        const int ec = min(WAVE, sr.a1 - e0);
+        const int ec = __builtin_amdgcn_readfirstlane(min(WAVE, sr.a1 - e0));
         wave_sync();
         if (lane < ec) s_face[lane] = face_of<PINHOLE>(p.adjacency, p.adj_diff, p.spheres, e0 + lane, sr.sph, ce);
         wave_sync();
-        if (hit) {
-          for (int e = 0; e < ec; e++) {
-            const float4 f = s_face[e];
-            const f3 n{f.x, f.y, f.z};
-            const float dp = dot(ray.d, n);
-            const float num = PINHOLE ? f.w : f.w + dot(co, n);
-            const float t = acc_div(num, dp);
-            const int id = e0 - sr.a0 + e;
-            if (dp >= 0.0f && t < tf) {
-              tf_id = id;
-              tf = t;
-            }
-            if (dp < 0.0f && t > tn) {
-              tn_id = id;
-              tn = t;
-            }
+        // every lane runs the faces (a lane that misses the sphere only
+        // updates values nothing reads): the loop stays in scalar control
+        for (int e = 0; e < ec; e++) {
+          const float4 f = s_face[e];
+          const f3 n{f.x, f.y, f.z};
+          const float dp = dot(ray.d, n);
+          const float num = PINHOLE ? f.w : f.w + dot(co, n);
+          const float t = acc_div(num, dp);
+          const int id = e0 - sr.a0 + e;
+          if (dp >= 0.0f && t < tf) {
+            tf_id = id;
+            tf = t;
+          }
+          if (dp < 0.0f && t > tn) {
+            tn_id = id;
+            tn = t;
           }
         }
       }
@@ -1168,7 +1186,7 @@ __global__ void __launch_bounds__(WAVE* WPB) raster_rgb_kernel(RgbParams p) {
       }
       alive = alive && trans >= p.thr;
       if (ballot(alive) == 0) break;
-      const SiteRec sr = s_site[kk];
+      const SiteRec sr = uniform_site(s_site[kk]);
       const f3 c{sr.sph.x, sr.sph.y, sr.sph.z};
       const float r = sr.sph.w;
       const f3 pn{sr.nsg.x, sr.nsg.y, sr.nsg.z};
@@ -1179,7 +1197,7 @@ __global__ void __launch_bounds__(WAVE* WPB) raster_rgb_kernel(RgbParams p) {
       bool open = alive && sphere_hit(oc, ray.d, r, tn, tf);
       if (ballot(open) == 0) continue;
       for (int e0 = sr.a0; e0 < sr.a1 && ballot(open) != 0; e0 += WAVE) {
-        const int ec = min(WAVE, sr.a1 - e0);
+        const int ec = __builtin_amdgcn_readfirstlane(min(WAVE, sr.a1 - e0));
         wave_sync();
         if (lane < ec) {
           if (VIS) {
