@@ -1045,50 +1045,46 @@ __global__ void __launch_bounds__(WAVE* WPB) PF_BWD_OCC raster_bwd_kernel(BwdPar
         acc += __shfl_xor(acc, 32, 64);
         if (lane < count && acc != 0.0f) atomicAdd(grad_slot(p, sr.site, T, v0 + lane), acc);
       };
+      // Branch-free: a skipped texel (w / sum w < cut, or an inactive lane)
+      // selects zero, so no per-texel control flow forms in this pass.
 #pragma unroll
       for (int i0 = 0; i0 < PF_MAX_TEXELS; i0 += 2) {
         if (i0 < T) {
           float v[14];
 #pragma unroll
-          for (int q = 0; q < 14; q++) v[q] = 0.0f;
-          if (active) {
-#pragma unroll
-            for (int k = 0; k < 2; k++) {
-              const int i = i0 + k;
-              if (i < T) {
-                const float4 ph = s_tex[i].pos_h;
-                const float4 cl = s_tex[i].rgb;
-                // colour stage (texture.py:158-171)
-                const float wn = w1[i] * inv_ws;
-                if (!(wn < cut)) {
-                  const f3 df{x1.x - ph.x, x1.y - ph.y, x1.z - ph.z};
-                  v[7 * k + 3] = dLdcol.x * wn;
-                  v[7 * k + 4] = dLdcol.y * wn;
-                  v[7 * k + 5] = dLdcol.z * wn;
-                  const float gdp = dLdcol.x * cl.x + dLdcol.y * cl.y + dLdcol.z * cl.z;
-                  const f3 sg = df * (kw * wn * (gdp - gdp_in));
-                  v[7 * k + 0] = sg.x;
-                  v[7 * k + 1] = sg.y;
-                  v[7 * k + 2] = sg.z;
-                }
-                // height stage (texture.py:118-131); with all heights zero
-                // it queries the colour-stage point and its site terms vanish
-                if (heights) {
-                  const f3 df{x0.x - ph.x, x0.y - ph.y, x0.z - ph.z};
-                  const float w = fast_exp(dot(df, df) * kexp);
-                  if (!(w * inv_ws < cut)) {
-                    const float wn0 = w * inv_ws0;
-                    v[7 * k + 6] = wn0 * dLdheight;
-                    const f3 sg = df * (kw * wn0 * (dLdheight * ph.w - gdp0_in));
-                    v[7 * k + 0] += sg.x;
-                    v[7 * k + 1] += sg.y;
-                    v[7 * k + 2] += sg.z;
-                  }
-                } else if (!(w1[i] * inv_ws < cut)) {
-                  v[7 * k + 6] = (w1[i] * inv_ws0) * dLdheight;
-                }
-              }
+          for (int k = 0; k < 2; k++) {
+            // i <= 7 stays inside s_tex; values past T are never flushed
+            const int i = i0 + k;
+            const float4 ph = s_tex[i].pos_h;
+            const float4 cl = s_tex[i].rgb;
+            // colour stage (texture.py:158-171)
+            const float wn = w1[i] * inv_ws;
+            const bool use1 = active && !(wn < cut);
+            const f3 df{x1.x - ph.x, x1.y - ph.y, x1.z - ph.z};
+            const float gdp = dLdcol.x * cl.x + dLdcol.y * cl.y + dLdcol.z * cl.z;
+            const float s1 = kw * wn * (gdp - gdp_in);
+            float sx = use1 ? df.x * s1 : 0.0f, sy = use1 ? df.y * s1 : 0.0f, sz = use1 ? df.z * s1 : 0.0f;
+            v[7 * k + 3] = use1 ? dLdcol.x * wn : 0.0f;
+            v[7 * k + 4] = use1 ? dLdcol.y * wn : 0.0f;
+            v[7 * k + 5] = use1 ? dLdcol.z * wn : 0.0f;
+            // height stage (texture.py:118-131); with all heights zero it
+            // queries the colour-stage point and its site terms vanish
+            if (heights) {
+              const f3 d0{x0.x - ph.x, x0.y - ph.y, x0.z - ph.z};
+              const float w = fast_exp(dot(d0, d0) * kexp);
+              const bool use0 = active && !(w * inv_ws < cut);
+              const float wn0 = w * inv_ws0;
+              const float s0 = kw * wn0 * (dLdheight * ph.w - gdp0_in);
+              v[7 * k + 6] = use0 ? wn0 * dLdheight : 0.0f;
+              sx += use0 ? d0.x * s0 : 0.0f;
+              sy += use0 ? d0.y * s0 : 0.0f;
+              sz += use0 ? d0.z * s0 : 0.0f;
+            } else {
+              v[7 * k + 6] = use1 ? (w1[i] * inv_ws0) * dLdheight : 0.0f;
             }
+            v[7 * k + 0] = sx;
+            v[7 * k + 1] = sy;
+            v[7 * k + 2] = sz;
           }
           flush_slice(v, i0 + 1 < T ? 14 : 7, 8 + 7 * i0);
         }
