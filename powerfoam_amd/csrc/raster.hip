@@ -370,8 +370,11 @@ __global__ void __launch_bounds__(WAVE* WPB) raster_fwd_kernel(FwdParams p) {
           const float dp = dot(ray.d, n);
           const float num = PINHOLE ? f.w : f.w + dot(co, n);
           const float t = acc_div(num, dp);
-          tf = (dp >= 0.0f && t < tf) ? t : tf;
-          tn = (dp < 0.0f && t > tn) ? t : tn;
+          // dp < 0 as the complement of dp >= 0: one compare (a NaN dp
+          // gives a NaN t, which no comparison below accepts)
+          const bool pos = dp >= 0.0f;
+          tf = (pos && t < tf) ? t : tf;
+          tn = (!pos && t > tn) ? t : tn;
         }
       }
       const bool shade = hit && !ray.oob && tn < tf;
@@ -706,11 +709,12 @@ __global__ void __launch_bounds__(WAVE* WPB) PF_BWD_OCC raster_bwd_kernel(BwdPar
           const float num = PINHOLE ? f.w : f.w + dot(co, n);
           const float t = acc_div(num, dp);
           const int id = e0 - sr.a0 + e;
-          if (dp >= 0.0f && t < tf) {
+          const bool pos = dp >= 0.0f;  // one compare, as in the forward
+          if (pos && t < tf) {
             tf_id = id;
             tf = t;
           }
-          if (dp < 0.0f && t > tn) {
+          if (!pos && t > tn) {
             tn_id = id;
             tn = t;
           }
