@@ -281,11 +281,14 @@ int pf_bin_emit_cones(const pf_camera* cam, const float* cones, const float* sph
                       const int64_t* offsets, int32_t n_tiles, uint64_t* records, int32_t* cursor,
                       uint8_t* visible, void* stream);
 
-/* Bytes of scratch pf_bin_sort needs for these tile segments. */
+/* Bytes of scratch pf_bin_sort can use for these tile segments. */
 size_t pf_bin_sort_workspace_size(int64_t n_records, int32_t n_tiles);
 
 /* Sorts every tile segment ascending by record (depth key, then site id) and
- * writes the site ids to `sites` [M]. Deterministic. */
+ * writes the site ids to `sites` [M]. Deterministic. The workspace is
+ * optional: with it, segments longer than 4,096 records are listed and sorted
+ * by 1,024 threads each with 16,384 records in LDS; without it they merge
+ * through global memory in the one-workgroup-per-tile kernel. */
 int pf_bin_sort(const int64_t* offsets, int32_t n_tiles, uint64_t* records,
                 int32_t* sites, void* workspace, size_t workspace_bytes,
                 void* stream);

@@ -522,10 +522,11 @@ constexpr int SORT_CAP = 4096;  // records staged in LDS (32 KiB)
 // (mirror = true) or a half-cleaner with stride j. Pairs whose upper index
 // is >= n compare against virtual +inf and are skipped. k and j are powers
 // of two; lg_half = log2(k/2) for the mirror step, lg_j = log2(j) otherwise.
-template <typename Buf>
-__device__ __forceinline__ void bitonic_step(Buf* buf, int64_t n_pairs, int64_t n, int lg, bool mirror) {
+template <int THREADS, typename Buf>
+__device__ __forceinline__ void bitonic_step(Buf* buf, int64_t n_pairs, int64_t n, int lg, bool mirror,
+                                             int tid = -1) {
   const int64_t m = ((int64_t)1 << lg) - 1;
-  for (int64_t p = threadIdx.x; p < n_pairs; p += SORT_THREADS) {
+  for (int64_t p = tid >= 0 ? tid : (int)threadIdx.x; p < n_pairs; p += THREADS) {
     const int64_t blk = p >> lg, l = p & m;
     int64_t i, q;
     if (mirror) {  // block of size k = 2^(lg+1): pair l with k-1-l
@@ -546,11 +547,11 @@ __device__ __forceinline__ void bitonic_step(Buf* buf, int64_t n_pairs, int64_t 
 
 __device__ __forceinline__ int ilog2(int64_t x) { return 63 - __builtin_clzll((unsigned long long)x); }
 
-__global__ void __launch_bounds__(SORT_THREADS)
-    bin_sort_kernel(const int64_t* __restrict__ offsets, int n_tiles, uint64_t* __restrict__ records,
-                    int32_t* __restrict__ sites) {
-  __shared__ uint64_t lds[SORT_CAP];
-  const int tile = blockIdx.x;
+// Sorts tile `tile`'s segment with CAP records staged in LDS by THREADS
+// threads; longer segments merge through global memory.
+template <int CAP, int THREADS>
+__device__ void sort_tile(const int64_t* __restrict__ offsets, int tile, uint64_t* __restrict__ records,
+                          int32_t* __restrict__ sites, uint64_t* lds) {
   const int64_t a = offsets[tile];
   const int64_t n = offsets[tile + 1] - a;
   if (n <= 0) return;
@@ -562,61 +563,141 @@ __global__ void __launch_bounds__(SORT_THREADS)
   int64_t P = 1;
   while (P < n) P <<= 1;
   const int lgP = ilog2(P);
-  if (n <= SORT_CAP) {
-    for (int64_t i = threadIdx.x; i < n; i += SORT_THREADS) lds[i] = g[i];
+  if (n <= CAP) {
+    for (int64_t i = threadIdx.x; i < n; i += THREADS) lds[i] = g[i];
     __syncthreads();
     for (int lk = 1; lk <= lgP; lk++) {
-      bitonic_step(lds, P / 2, n, lk - 1, true);
+      bitonic_step<THREADS>(lds, P / 2, n, lk - 1, true);
       __syncthreads();
       for (int lj = lk - 2; lj >= 0; lj--) {
-        bitonic_step(lds, P / 2, n, lj, false);
+        bitonic_step<THREADS>(lds, P / 2, n, lj, false);
         __syncthreads();
       }
     }
-    for (int64_t i = threadIdx.x; i < n; i += SORT_THREADS) sites[a + i] = (int32_t)(uint32_t)lds[i];
+    for (int64_t i = threadIdx.x; i < n; i += THREADS) sites[a + i] = (int32_t)(uint32_t)lds[i];
     return;
   }
-  // Large segment: sort SORT_CAP chunks in LDS, then merge through global
+  // Large segment: sort CAP chunks in LDS, then merge through global
   // memory; strides that fit a chunk are finished in LDS.
-  const int lgC = ilog2(SORT_CAP);
-  const int64_t n_chunks = (n + SORT_CAP - 1) / SORT_CAP;
+  const int lgC = ilog2(CAP);
+  const int64_t n_chunks = (n + CAP - 1) / CAP;
   for (int lk = 1; lk <= lgP; lk++) {
     if (lk > lgC) {
-      bitonic_step(g, P / 2, n, lk - 1, true);
+      bitonic_step<THREADS>(g, P / 2, n, lk - 1, true);
       __syncthreads();
       for (int lj = lk - 2; lj >= lgC; lj--) {
-        bitonic_step(g, P / 2, n, lj, false);
+        bitonic_step<THREADS>(g, P / 2, n, lj, false);
         __syncthreads();
       }
     }
     if (lk > lgC || lk == lgC) {
       // finish every step of merge sizes <= 2^lk whose pairs stay in a chunk
       for (int64_t c = 0; c < n_chunks; c++) {
-        const int64_t base = c * SORT_CAP;
-        const int64_t cn = min((int64_t)SORT_CAP, n - base);
-        for (int64_t i = threadIdx.x; i < cn; i += SORT_THREADS) lds[i] = g[base + i];
+        const int64_t base = c * CAP;
+        const int64_t cn = min((int64_t)CAP, n - base);
+        for (int64_t i = threadIdx.x; i < cn; i += THREADS) lds[i] = g[base + i];
         __syncthreads();
         if (lk == lgC) {
           for (int l2 = 1; l2 <= lgC; l2++) {
-            bitonic_step(lds, SORT_CAP / 2, cn, l2 - 1, true);
+            bitonic_step<THREADS>(lds, CAP / 2, cn, l2 - 1, true);
             __syncthreads();
             for (int lj = l2 - 2; lj >= 0; lj--) {
-              bitonic_step(lds, SORT_CAP / 2, cn, lj, false);
+              bitonic_step<THREADS>(lds, CAP / 2, cn, lj, false);
               __syncthreads();
             }
           }
         } else {
           for (int lj = lgC - 1; lj >= 0; lj--) {
-            bitonic_step(lds, SORT_CAP / 2, cn, lj, false);
+            bitonic_step<THREADS>(lds, CAP / 2, cn, lj, false);
             __syncthreads();
           }
         }
-        for (int64_t i = threadIdx.x; i < cn; i += SORT_THREADS) g[base + i] = lds[i];
+        for (int64_t i = threadIdx.x; i < cn; i += THREADS) g[base + i] = lds[i];
         __syncthreads();
       }
     }
   }
-  for (int64_t i = threadIdx.x; i < n; i += SORT_THREADS) sites[a + i] = (int32_t)(uint32_t)g[i];
+  for (int64_t i = threadIdx.x; i < n; i += THREADS) sites[a + i] = (int32_t)(uint32_t)g[i];
+}
+
+// One workgroup per tile. With `skip_big`, tiles above SORT_CAP are left to
+// bin_sort_big_kernel; with `skip_small`, tiles of at most SMALL_CAP records
+// to bin_sort_small_kernel.
+constexpr int SMALL_CAP = 512;
+
+__global__ void __launch_bounds__(SORT_THREADS)
+    bin_sort_kernel(const int64_t* __restrict__ offsets, int n_tiles, uint64_t* __restrict__ records,
+                    int32_t* __restrict__ sites, int skip_big, int skip_small) {
+  __shared__ uint64_t lds[SORT_CAP];
+  const int tile = blockIdx.x;
+  const int64_t n = offsets[tile + 1] - offsets[tile];
+  if ((skip_big && n > SORT_CAP) || (skip_small && n <= SMALL_CAP)) return;
+  sort_tile<SORT_CAP, SORT_THREADS>(offsets, tile, records, sites, lds);
+}
+
+// Most tiles hold a few hundred records: one wave sorts one of them in its
+// own LDS slice, with no workgroup barriers (a wave's LDS operations run in
+// order, so wave_sync orders the network's steps).
+__global__ void __launch_bounds__(SORT_THREADS)
+    bin_sort_small_kernel(const int64_t* __restrict__ offsets, int n_tiles, uint64_t* __restrict__ records,
+                          int32_t* __restrict__ sites) {
+  __shared__ uint64_t lds_all[SORT_THREADS / WAVE][SMALL_CAP];
+  const int wv = threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
+  const int tile = blockIdx.x * (SORT_THREADS / WAVE) + wv;
+  if (tile >= n_tiles) return;
+  const int64_t a = offsets[tile];
+  const int n = (int)(offsets[tile + 1] - a);
+  if (n <= 0 || n > SMALL_CAP) return;
+  uint64_t* lds = lds_all[wv];
+  const uint64_t* g = records + a;
+  if (n == 1) {
+    if (lane == 0) sites[a] = (int32_t)(uint32_t)g[0];
+    return;
+  }
+  int P = 1;
+  while (P < n) P <<= 1;
+  const int lgP = ilog2(P);
+  for (int i = lane; i < n; i += WAVE) lds[i] = g[i];
+  wave_sync();
+  for (int lk = 1; lk <= lgP; lk++) {
+    bitonic_step<WAVE>(lds, P / 2, n, lk - 1, true, lane);
+    wave_sync();
+    for (int lj = lk - 2; lj >= 0; lj--) {
+      bitonic_step<WAVE>(lds, P / 2, n, lj, false, lane);
+      wave_sync();
+    }
+  }
+  for (int i = lane; i < n; i += WAVE) sites[a + i] = (int32_t)(uint32_t)lds[i];
+}
+
+// The tiles longer than SORT_CAP (a handful per view, up to ~20k records)
+// bound the sort when one 256-thread workgroup merges them through global
+// memory: list them, then sort each with 1024 threads and 16k records in
+// LDS (128 KiB).
+constexpr int BIG_THREADS = 1024;
+constexpr int BIG_CAP = 16384;
+constexpr int BIG_GRID = 256;
+
+__global__ void __launch_bounds__(1024) big_tiles_kernel(const int64_t* __restrict__ offsets, int n_tiles,
+                                                        int32_t* __restrict__ list) {
+  __shared__ int n_big;
+  if (threadIdx.x == 0) n_big = 0;
+  __syncthreads();
+  for (int t = threadIdx.x; t < n_tiles; t += blockDim.x)
+    if (offsets[t + 1] - offsets[t] > SORT_CAP) list[1 + atomicAdd(&n_big, 1)] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) list[0] = n_big;
+}
+
+__global__ void __launch_bounds__(BIG_THREADS)
+    bin_sort_big_kernel(const int64_t* __restrict__ offsets, const int32_t* __restrict__ list,
+                        uint64_t* __restrict__ records, int32_t* __restrict__ sites) {
+  __shared__ uint64_t lds[BIG_CAP];
+  const int n_big = list[0];
+  for (int k = blockIdx.x; k < n_big; k += gridDim.x) {
+    sort_tile<BIG_CAP, BIG_THREADS>(offsets, list[1 + k], records, sites, lds);
+    __syncthreads();
+  }
 }
 
 
@@ -625,7 +706,9 @@ __global__ void __launch_bounds__(SORT_THREADS)
 // order: a block-wide exclusive scan of the per-tile segment counts gives the
 // segment table, and counting sorts over length buckets give the
 // longest-first orders (members of one bucket land in atomic order, which
-// only the schedule sees, never a result).
+// only the schedule sees, never a result). Each thread walks a run of tiles
+// three times (count, table and histograms, scatter), so no pass reads
+// another thread's output from global memory.
 constexpr int SCHED_THREADS = 1024, SCHED_BUCKETS = 1024;
 
 __device__ __forceinline__ int len_bucket(int64_t len) {
@@ -655,58 +738,70 @@ __device__ int block_excl_scan(int v, int* wsum, int* total) {
   return base + x - v;
 }
 
-// Writes the n items in descending bucket order to out.
-template <class B>
-__device__ void bucket_sort_desc(int n, B bucket, int32_t* out, int* hist, int* wsum) {
-  for (int i = threadIdx.x; i < SCHED_BUCKETS; i += SCHED_THREADS) hist[i] = 0;
-  __syncthreads();
-  for (int i = threadIdx.x; i < n; i += SCHED_THREADS) atomicAdd(&hist[bucket(i)], 1);
-  __syncthreads();
-  // thread i owns bucket SCHED_BUCKETS - 1 - i: buckets in descending order
-  const int b = SCHED_BUCKETS - 1 - threadIdx.x;
-  int total;
-  const int start = block_excl_scan(hist[b], wsum, &total);
-  hist[b] = start;
-  __syncthreads();
-  for (int i = threadIdx.x; i < n; i += SCHED_THREADS) out[atomicAdd(&hist[bucket(i)], 1)] = i;
-  __syncthreads();
-}
-
 __global__ void __launch_bounds__(SCHED_THREADS)
     tile_schedule_kernel(const int64_t* __restrict__ offsets, int nt, int L, int32_t* tile_order,
                          int32_t* tile_first, int32_t* seg_tile, int32_t* seg_order, int32_t* count) {
-  __shared__ int hist[SCHED_BUCKETS];
+  // hs: segments by size bucket; ht: tiles by length bucket
+  __shared__ int hs[SCHED_BUCKETS], ht[SCHED_BUCKETS];
   __shared__ int wsum[SCHED_THREADS / WAVE];
+  const bool segs = tile_first != nullptr;
+  for (int i = threadIdx.x; i < SCHED_BUCKETS; i += SCHED_THREADS) hs[i] = ht[i] = 0;
+  // each thread owns a run of tiles; everything below walks only its own run
+  const int per = (nt + SCHED_THREADS - 1) / SCHED_THREADS;
+  const int t0 = min((int)threadIdx.x * per, nt), t1 = min(t0 + per, nt);
   auto cnt = [&](int t) { return offsets[t + 1] - offsets[t]; };
-  if (tile_first) {
-    // tile t has max(1, ceil(cnt / L)) segments; each thread scans a run of tiles
-    const int per = (nt + SCHED_THREADS - 1) / SCHED_THREADS;
-    const int t0 = min((int)threadIdx.x * per, nt), t1 = min(t0 + per, nt);
-    int local = 0;
-    for (int t = t0; t < t1; t++) local += max(1, (int)((cnt(t) + L - 1) / L));
-    int total;
-    int first = block_excl_scan(local, wsum, &total);
-    for (int t = t0; t < t1; t++) {
-      const int ns = max(1, (int)((cnt(t) + L - 1) / L));
+  auto nseg = [&](int64_t c) { return max(1, (int)((c + L - 1) / L)); };
+  // size bucket of segment k of a list of c sites (k = ns - 1 holds the rest)
+  auto seg_bucket = [&](int64_t c, int k) {
+    return (int)min(min(c - (int64_t)k * L, (int64_t)L), (int64_t)(SCHED_BUCKETS - 1));
+  };
+  int local = 0;
+  if (segs)
+    for (int t = t0; t < t1; t++) local += nseg(cnt(t));
+  __syncthreads();
+  int total = 0;
+  const int run_first = segs ? block_excl_scan(local, wsum, &total) : 0;
+  int first = run_first;
+  // table, and the histograms of both orders
+  for (int t = t0; t < t1; t++) {
+    const int64_t c = cnt(t);
+    if (tile_order) atomicAdd(&ht[len_bucket(c)], 1);
+    if (segs) {
+      const int ns = nseg(c);
       tile_first[t] = first;
-      for (int k = 0; k < ns; k++) seg_tile[first + k] = t;
+      for (int k = 0; k < ns; k++) {
+        seg_tile[first + k] = t;
+        atomicAdd(&hs[seg_bucket(c, k)], 1);
+      }
       first += ns;
     }
-    if (threadIdx.x == 0) {
-      tile_first[nt] = total;
-      *count = total;
-    }
-    __syncthreads();
-    bucket_sort_desc(
-        total,
-        [&](int s) {
-          const int t = seg_tile[s];
-          const int64_t size = cnt(t) - (int64_t)(s - tile_first[t]) * L;
-          return (int)min(size, (int64_t)(SCHED_BUCKETS - 1));
-        },
-        seg_order, hist, wsum);
   }
-  if (tile_order) bucket_sort_desc(nt, [&](int t) { return len_bucket(cnt(t)); }, tile_order, hist, wsum);
+  if (segs && threadIdx.x == 0) {
+    tile_first[nt] = total;
+    *count = total;
+  }
+  __syncthreads();
+  // bucket starts in descending bucket order: thread i owns bucket 1023 - i
+  {
+    const int bk = SCHED_BUCKETS - 1 - threadIdx.x;
+    int tot;
+    const int s_start = block_excl_scan(hs[bk], wsum, &tot);
+    const int t_start = block_excl_scan(ht[bk], wsum, &tot);
+    hs[bk] = s_start;
+    ht[bk] = t_start;
+  }
+  __syncthreads();
+  // scatter (members of one bucket land in atomic order)
+  first = run_first;
+  for (int t = t0; t < t1; t++) {
+    const int64_t c = cnt(t);
+    if (tile_order) tile_order[atomicAdd(&ht[len_bucket(c)], 1)] = t;
+    if (segs) {
+      const int ns = nseg(c);
+      for (int k = 0; k < ns; k++) seg_order[atomicAdd(&hs[seg_bucket(c, k)], 1)] = first + k;
+      first += ns;
+    }
+  }
 }
 
 }  // namespace pf
@@ -816,16 +911,22 @@ int pf_bin_emit_cones(const pf_camera* cam, const float* cones, const float* sph
 
 size_t pf_bin_sort_workspace_size(int64_t n_records, int32_t n_tiles) {
   (void)n_records;
-  (void)n_tiles;
-  return 0;  // the segmented sort works in place
+  return n_tiles > 0 ? sizeof(int32_t) * ((size_t)n_tiles + 1) : 0;  // the list of long tiles
 }
 
 int pf_bin_sort(const int64_t* offsets, int32_t n_tiles, uint64_t* records, int32_t* sites, void* workspace,
                 size_t workspace_bytes, void* stream) {
-  (void)workspace;
-  (void)workspace_bytes;
   if (!offsets || n_tiles < 1) return set_error(PF_ERR_ARG, "pf_bin_sort: bad arguments");
-  bin_sort_kernel<<<n_tiles, SORT_THREADS, 0, as_stream(stream)>>>(offsets, n_tiles, records, sites);
+  hipStream_t st = as_stream(stream);
+  const bool big = workspace && workspace_bytes >= pf_bin_sort_workspace_size(0, n_tiles);
+  const int per = SORT_THREADS / WAVE;
+  bin_sort_small_kernel<<<(n_tiles + per - 1) / per, SORT_THREADS, 0, st>>>(offsets, n_tiles, records, sites);
+  bin_sort_kernel<<<n_tiles, SORT_THREADS, 0, st>>>(offsets, n_tiles, records, sites, big ? 1 : 0, 1);
+  if (big) {
+    int32_t* list = static_cast<int32_t*>(workspace);
+    big_tiles_kernel<<<1, 1024, 0, st>>>(offsets, n_tiles, list);
+    bin_sort_big_kernel<<<BIG_GRID, BIG_THREADS, 0, st>>>(offsets, list, records, sites);
+  }
   PF_HIP(hipGetLastError());
   return PF_OK;
 }

@@ -156,7 +156,9 @@ class Rasterizer:
             check(lib.pf_bin_emit_cones(_ref(cam), ptr(cones), ptr(all_spheres), n, ptr(offsets), nt, ptr(records),
                                         ptr(cursor), ptr(visible), st))
         sites = torch.empty(max(m, 1), dtype=torch.int32, device=dev)
-        check(lib.pf_bin_sort(ptr(offsets), nt, ptr(records), ptr(sites), None, 0, st))
+        ws_bytes = int(lib.pf_bin_sort_workspace_size(m, nt))
+        ws = torch.empty(max(ws_bytes // 4, 1), dtype=torch.int32, device=dev)
+        check(lib.pf_bin_sort(ptr(offsets), nt, ptr(records), ptr(sites), ptr(ws), ws_bytes, st))
         return offsets, sites[:m] if m > 0 else sites[:0], visible.bool(), nt
 
     def tile_order(self, offsets):

@@ -151,3 +151,36 @@ def test_tile_schedule_orders_and_segment_table(seg_len):
     size = cnt[st] - (np.arange(total) - want_first[st]) * seg_len
     size = np.minimum(size, seg_len)
     assert np.all(np.diff(np.minimum(size[so], 1023)) <= 0)
+
+
+def test_sort_long_tiles_with_and_without_workspace():
+    """pf_bin_sort on segments from 1 to 40,000 records (the 16k-record LDS path
+    and its global-merge tail above that): with the workspace (long tiles in
+    the second kernel) and without it, the sites equal numpy's sort of the
+    64-bit records, duplicate keys ordered by site id."""
+    import torch
+    from powerfoam_amd import _abi
+    from powerfoam_amd._abi import check, ptr
+    lib = _abi.load()
+    rng = np.random.default_rng(11)
+    cnt = np.array([0, 1, 2, 63, 64, 65, 4095, 4096, 4097, 5000, 9000, 16384, 16385, 23000, 40000, 7, 300],
+                   np.int64)
+    off = np.zeros(cnt.size + 1, np.int64)
+    off[1:] = np.cumsum(cnt)
+    m = int(off[-1])
+    keys = rng.integers(0, 2000, m).astype(np.uint64)  # many ties
+    site = rng.permutation(m).astype(np.uint64)
+    rec = (keys << np.uint64(32)) | site
+    want = np.concatenate([np.sort(rec[off[t]:off[t + 1]]) for t in range(cnt.size)]) & np.uint64(0xFFFFFFFF)
+    nt = cnt.size
+    for use_ws in (True, False):
+        print(f"long-tile sort workspace={use_ws}", flush=True)
+        offsets = torch.tensor(off, device="cuda")
+        records = torch.tensor(rec.view(np.int64), device="cuda")
+        sites = torch.empty(m, dtype=torch.int32, device="cuda")
+        wsb = int(lib.pf_bin_sort_workspace_size(m, nt)) if use_ws else 0
+        ws = torch.empty(max(wsb // 4, 1), dtype=torch.int32, device="cuda")
+        check(lib.pf_bin_sort(ptr(offsets), nt, ptr(records), ptr(sites), ptr(ws) if use_ws else None, wsb, None))
+        torch.cuda.synchronize()
+        got = sites.cpu().numpy().astype(np.int64)
+        assert np.array_equal(got, want.astype(np.int64))
