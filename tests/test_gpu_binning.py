@@ -184,3 +184,34 @@ def test_sort_long_tiles_with_and_without_workspace():
         torch.cuda.synchronize()
         got = sites.cpu().numpy().astype(np.int64)
         assert np.array_equal(got, want.astype(np.int64))
+
+
+@pytest.mark.parametrize("n", [1, 300, 5000])
+def test_adjacency_encoding_bit_exact(n):
+    """pf_adj_encode (prefetch_adjacency_kernel, rasterize.py:580-606) against the
+    oracle's binary16 encoding, bit for bit: degrees from 0 to 150 (edges of one
+    site spread over several passes of a wave), site counts not a multiple of
+    the 256 sites a workgroup takes."""
+    import torch
+    from powerfoam_amd import _abi
+    from powerfoam_amd._abi import check, ptr
+    lib = _abi.load()
+    rng = np.random.default_rng(n)
+    sp = np.concatenate([rng.normal(0, 1, (n, 3)), rng.uniform(0.01, 0.2, (n, 1))], 1).astype(np.float32)
+    deg = rng.integers(0, 12, n)
+    deg[rng.random(n) < 0.05] = 0
+    if n > 10:
+        deg[rng.integers(0, n, 3)] = 150
+    off = np.zeros(n + 1, np.int64)
+    off[1:] = np.cumsum(deg)
+    adj = rng.integers(0, n, int(off[-1])).astype(np.int32)
+    want = orc.adj_encode(sp, adj, off.astype(np.int32))
+    print(f"adjacency encode n={n} edges={adj.size}", flush=True)
+    d_sp = torch.tensor(sp, device="cuda")
+    d_adj = torch.tensor(adj, device="cuda")
+    d_off = torch.tensor(off.astype(np.int32), device="cuda")
+    out = torch.zeros((max(adj.size, 1), 4), dtype=torch.int16, device="cuda")
+    check(lib.pf_adj_encode(ptr(d_sp), n, ptr(d_adj), ptr(d_off), ptr(out), None))
+    torch.cuda.synchronize()
+    got = out.cpu().numpy().view(np.uint16)[: adj.size]
+    assert np.array_equal(got, want)
