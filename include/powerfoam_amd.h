@@ -237,9 +237,11 @@ const char* pf_last_error(void);
 
 /* Zeroes tile_counts[0..n_tiles] and counts, per tile, the sites whose
  * projected footprint passes the tile test; tile t's count lands in
- * tile_counts[t+1] so an inclusive scan gives offsets directly. */
-int pf_bin_count(const pf_camera* cam, const float* spheres, int32_t n_sites,
-                 int32_t* tile_counts, void* stream);
+ * tile_counts[t+1] so an inclusive scan gives offsets directly. `live`
+ * (uint8 [n_sites], from pf_live_sites) leaves out the sites marked 0; NULL
+ * counts every site, as the reference does. */
+int pf_bin_count(const pf_camera* cam, const float* spheres, const uint8_t* live,
+                 int32_t n_sites, int32_t* tile_counts, void* stream);
 
 /* offsets[k] = sum(tile_counts[0..k]) as int64 (torch.cumsum of the counts). */
 int pf_bin_scan(const int32_t* tile_counts, int32_t n_tiles, int64_t* offsets,
@@ -247,12 +249,33 @@ int pf_bin_scan(const int32_t* tile_counts, int32_t n_tiles, int64_t* offsets,
 
 /* Writes one 64-bit record per (tile, site) pair into the tile's segment:
  * (bits(|c-eye|^2 - r^2) << 32) | site. `cursor` is an int32 [n_tiles]
- * scratch array the call zeroes. Sets visible[site] = 1 for every emitted
- * site (the reference's prim_visible_mask, rasterize.py:1761-1765); the
- * caller zeroes `visible` first. */
-int pf_bin_emit(const pf_camera* cam, const float* spheres, int32_t n_sites,
-                const int64_t* offsets, int32_t n_tiles, uint64_t* records,
-                int32_t* cursor, uint8_t* visible, void* stream);
+ * scratch array the call zeroes. Sets visible[site] = 1 for every site with
+ * a tile (the reference's prim_visible_mask, rasterize.py:1761-1765), also
+ * for sites that `live` leaves out of the lists; the caller zeroes `visible`
+ * first (or passes NULL). `live` as for pf_bin_count, and the same mask. */
+int pf_bin_emit(const pf_camera* cam, const float* spheres, const uint8_t* live,
+                int32_t n_sites, const int64_t* offsets, int32_t n_tiles,
+                uint64_t* records, int32_t* cursor, uint8_t* visible, void* stream);
+
+/* Face encodings pf_live_sites reads. */
+enum {
+  PF_FACES_RADIUS = 0, /* (c_adj - c, r_adj) from pf_adj_encode */
+  PF_FACES_POWER = 1   /* (c_adj - c, pm_adj - pm), pm = (|c|^2 - r^2)/2 (benchmark.py:297-302) */
+};
+
+/* live[s] = 0 for a site whose sphere lies wholly beyond the plane of one of
+ * its power faces, by more than the rounding margin of the compositing
+ * kernels for rays from cam->eye; 1 otherwise. Such a site has an empty
+ * clipped interval on every ray, so the compositing results (colour,
+ * transmittance, normals, depths, contributions, every gradient) are the
+ * same whether or not it is binned. The faces are read as the compositing
+ * kernels read them: `diff` in the form `diff_kind` names, or, with diff
+ * NULL, the full-precision spheres of `adjacency`. Pinhole cameras only.
+ * Not in the reference: the reference bins every site in the footprint
+ * (count_visible_pinhole_kernel rasterize.py:140-200 has no such test). */
+int pf_live_sites(const pf_camera* cam, const float* spheres, int32_t n_sites,
+                  const int32_t* adjacency, const int32_t* adjacency_offsets,
+                  const uint16_t* diff, int32_t diff_kind, uint8_t* live, void* stream);
 
 /* Ray-map cameras (cam->ray_maps set; rasterize.py:1681-1760 when
  * args.is_pinhole is false) bin through a quadtree of ray cones instead of

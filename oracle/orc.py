@@ -165,6 +165,61 @@ def power_face_encode(spheres, adjacency, offsets):
     return out[: adj.shape[0]]
 
 
+def live_sites(eye, spheres, adjacency, offsets, diff=None, kind=0, return_margin=False):
+    """Sites that a ray from `eye` can shade (uint8, pf_live_sites): 0 where the
+    sphere lies beyond the plane of one of its faces by more than the
+    compositing kernels' rounding margin. Not a reference function: the
+    reference bins every site; this predicate only says which of them the
+    compositing can skip. The faces are read as the kernels read them:
+    `diff` (uint16 binary16 bits [E,4]) in the radius form (kind 0,
+    adj_encode) or the power-weight form (kind 1, power_face_encode), or the
+    full-precision neighbour spheres when diff is None. float32 throughout.
+    With return_margin, also the per-edge decision value (beyond - margin,
+    in units of |n|; > 0 culls) for borderline checks."""
+    sp = np.asarray(spheres, np.float32)
+    adj = np.asarray(adjacency, np.int64)
+    off = np.asarray(offsets, np.int64)
+    n = sp.shape[0]
+    deg = np.diff(off)
+    src = np.repeat(np.arange(n), deg)
+    c = sp[src, :3]
+    r = sp[src, 3]
+    eye = np.asarray(eye, np.float32).reshape(3)
+    if diff is not None:
+        d = np.asarray(diff, np.uint16).view(np.float16).astype(np.float32).reshape(-1, 4)
+        nv, q = d[:, :3], d[:, 3]
+        if kind == 1:
+            h = q - (c * nv).sum(1, dtype=np.float32)
+        else:
+            h = np.float32(0.5) * ((nv * nv).sum(1, dtype=np.float32) + r * r - q * q)
+    else:
+        a = sp[adj]
+        nv = a[:, :3] - c
+        h = np.float32(0.5) * ((nv * nv).sum(1, dtype=np.float32) + r * r - a[:, 3] * a[:, 3])
+    nl = np.sqrt((nv * nv).sum(1, dtype=np.float32))
+    dist = np.sqrt(((c - eye) ** 2).sum(1, dtype=np.float32)) + r
+    m = (np.float32(8e-3) * np.sqrt(r * dist)
+         + np.float32(1e-5) * (dist + np.sqrt((c * c).sum(1, dtype=np.float32)) + np.sqrt((eye * eye).sum())))
+    val = (-h - r * nl) - (m * nl + np.float32(1e-5) * np.abs(h))
+    dead_e = (nl > 0) & (val > 0)
+    live = np.ones(n, np.uint8)
+    live[src[dead_e]] = 0
+    if return_margin:
+        return live, val / np.maximum(nl, np.float32(1e-30)), src
+    return live
+
+
+def filter_lists(offsets, sites, live):
+    """Tile lists with the sites live marks 0 left out (order kept)."""
+    off = np.asarray(offsets, np.int64)
+    keep = np.asarray(live)[sites] != 0
+    tile = np.repeat(np.arange(off.size - 1), np.diff(off))
+    cnt = np.bincount(tile[keep], minlength=off.size - 1)
+    out = np.zeros_like(off)
+    out[1:] = np.cumsum(cnt)
+    return out, np.asarray(sites)[keep]
+
+
 def raster_rgb(cam, inp, offsets, sites, threshold=1e-3, precision="f64"):
     """Benchmark forward (colour only); inp.adjacency_diff holds the power-face encoding."""
     lib = load()

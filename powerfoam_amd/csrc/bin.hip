@@ -225,31 +225,103 @@ __device__ __forceinline__ void for_each_overlap(const BinCam& cam, WaveSite* ws
 }
 
 __global__ void __launch_bounds__(256)
-    bin_count_kernel(BinCam cam, const float4* __restrict__ spheres, int n, int32_t* __restrict__ counts) {
+    bin_count_kernel(BinCam cam, const float4* __restrict__ spheres, const uint8_t* __restrict__ live, int n,
+                     int32_t* __restrict__ counts) {
   __shared__ WaveSite s_ws[256];
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & (WAVE - 1);
   Footprint fp{};
-  const bool valid = s < n && site_footprint(cam, spheres[s], fp);
+  const bool valid = s < n && (!live || live[s]) && site_footprint(cam, spheres[s], fp);
   for_each_overlap(cam, s_ws + (threadIdx.x & ~(WAVE - 1)), lane, valid, fp, s,
                    [&](int t, const WaveSite&) { atomicAdd(&counts[t + 1], 1); });
 }
 
 __global__ void __launch_bounds__(256)
-    bin_emit_kernel(BinCam cam, const float4* __restrict__ spheres, int n, const int64_t* __restrict__ offsets,
-                    uint64_t* __restrict__ records, int32_t* __restrict__ cursor, uint8_t* __restrict__ visible) {
+    bin_emit_kernel(BinCam cam, const float4* __restrict__ spheres, const uint8_t* __restrict__ live, int n,
+                    const int64_t* __restrict__ offsets, uint64_t* __restrict__ records, int32_t* __restrict__ cursor,
+                    uint8_t* __restrict__ visible) {
   __shared__ WaveSite s_ws[256];
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & (WAVE - 1);
   Footprint fp{};
-  const bool valid = s < n && site_footprint(cam, spheres[s], fp);
+  // a site left out of the lists still walks its tiles when the visibility
+  // mask is asked for: the mask is the footprint's, as without the mask
+  const bool valid = s < n && (visible || !live || live[s]) && site_footprint(cam, spheres[s], fp);
   for_each_overlap(cam, s_ws + (threadIdx.x & ~(WAVE - 1)), lane, valid, fp, s, [&](int t, const WaveSite& o) {
-    const int64_t idx = offsets[t] + (int64_t)atomicAdd(&cursor[t], 1);
-    if (idx < offsets[t + 1]) {
-      records[idx] = ((uint64_t)o.key << 32) | (uint32_t)o.site;
-      if (visible) visible[o.site] = 1;
+    if (!live || live[o.site]) {
+      const int64_t idx = offsets[t] + (int64_t)atomicAdd(&cursor[t], 1);
+      if (idx < offsets[t + 1]) records[idx] = ((uint64_t)o.key << 32) | (uint32_t)o.site;
     }
+    if (visible) visible[o.site] = 1;
   });
+}
+
+// Sites that cannot shade a ray from the camera. A site's shaded interval is
+// its ray-sphere interval clipped by the planes of its power faces; when the
+// whole sphere lies beyond one of those planes, every ray's clipped interval
+// is empty and the compositing kernels give the site no contribution and no
+// gradient, so the tile lists can leave it out. The plane is the one those
+// kernels evaluate: the decoded fp16 deltas (radius form of pf_adj_encode,
+// power-weight form of the benchmark) or full-precision spheres. The sphere
+// has to clear the plane by more than the rounding of the kernels' interval
+// ends: a tangent ray's ends move by about 5e-4 sqrt(r |c - eye|) (the square
+// root of a rounded r^2 - |oc x d|^2), a plane crossing by a few ulp of the
+// coordinates involved; the margin is 16 and about 170 times these.
+// A wave takes 64 consecutive sites and deals their edges to its lanes, as
+// the adjacency encoder does.
+__global__ void __launch_bounds__(256)
+    live_sites_kernel(const float4* __restrict__ spheres, int n, const int32_t* __restrict__ adj,
+                      const int32_t* __restrict__ off, const uint2* __restrict__ diff, int power, f3 eye,
+                      uint8_t* __restrict__ live) {
+  __shared__ int s_off[4][WAVE + 1];
+  __shared__ float4 s_sph[4][WAVE];
+  __shared__ int s_dead[4][WAVE];
+  const int wv = threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
+  const int s0 = (blockIdx.x * 4 + wv) * WAVE;
+  if (s0 >= n) return;
+  const int ns = min(WAVE, n - s0);
+  int* so = s_off[wv];
+  float4* sp = s_sph[wv];
+  int* sd = s_dead[wv];
+  if (lane < ns) {
+    sp[lane] = spheres[s0 + lane];
+    so[lane] = off[s0 + lane];
+  }
+  if (lane == 0) so[ns] = off[s0 + ns];  // ns may be 64: one past the last lane
+  sd[lane] = 0;
+  wave_sync();
+  const float eye_len = __builtin_sqrtf(eye.x * eye.x + eye.y * eye.y + eye.z * eye.z);
+  const int e1 = so[ns];
+  for (int e = so[0] + lane; e < e1; e += WAVE) {
+    int l = 0;
+#pragma unroll
+    for (int step = WAVE / 2; step >= 1; step >>= 1)
+      if (l + step < ns && so[l + step] <= e) l += step;
+    const float4 c = sp[l];
+    f3 nv;
+    float h;
+    if (diff) {
+      const uint2 raw = diff[e];
+      nv = f3{half_bits_to_float((uint16_t)(raw.x & 0xffff)), half_bits_to_float((uint16_t)(raw.x >> 16)),
+              half_bits_to_float((uint16_t)(raw.y & 0xffff))};
+      const float q = half_bits_to_float((uint16_t)(raw.y >> 16));
+      h = power ? q - (c.x * nv.x + c.y * nv.y + c.z * nv.z) : 0.5f * (dot(nv, nv) + c.w * c.w - q * q);
+    } else {
+      const float4 a = spheres[adj[e]];
+      nv = f3{a.x - c.x, a.y - c.y, a.z - c.z};
+      h = 0.5f * (dot(nv, nv) + c.w * c.w - a.w * a.w);
+    }
+    // the cell is {y : y.n <= h} around the centre; the sphere reaches
+    // y.n >= -r |n| only, so it lies beyond by G = -h / |n| - r
+    const float nl = __builtin_sqrtf(dot(nv, nv));
+    const f3 ce{c.x - eye.x, c.y - eye.y, c.z - eye.z};
+    const float dist = __builtin_sqrtf(dot(ce, ce)) + c.w;
+    const float c_len = __builtin_sqrtf(c.x * c.x + c.y * c.y + c.z * c.z);
+    const float m = 8e-3f * __builtin_sqrtf(c.w * dist) + 1e-5f * (dist + c_len + eye_len);
+    if (nl > 0.0f && -h - c.w * nl > m * nl + 1e-5f * fabsf(h)) sd[l] = 1;
+  }
+  wave_sync();
+  if (lane < ns) live[s0 + lane] = sd[lane] ? 0 : 1;
 }
 
 // ------------------------------------------------ non-pinhole cameras
@@ -810,7 +882,8 @@ using namespace pf;
 
 extern "C" {
 
-int pf_bin_count(const pf_camera* cam, const float* spheres, int32_t n_sites, int32_t* tile_counts, void* stream) {
+int pf_bin_count(const pf_camera* cam, const float* spheres, const uint8_t* live, int32_t n_sites,
+                 int32_t* tile_counts, void* stream) {
   if (!cam || !tile_counts || n_sites < 0 || (n_sites > 0 && !spheres))
     return set_error(PF_ERR_ARG, "pf_bin_count: bad arguments");
   if (cam->width < 2 || cam->height < 2) return set_error(PF_ERR_ARG, "pf_bin_count: image must be >= 2x2");
@@ -820,8 +893,8 @@ int pf_bin_count(const pf_camera* cam, const float* spheres, int32_t n_sites, in
   const int64_t nt = (int64_t)tiles_h(*cam) * tiles_w(*cam);
   fill_i32_kernel<<<(unsigned)((nt + 1 + 255) / 256), 256, 0, st>>>(tile_counts, nt + 1, 0);
   if (n_sites > 0)
-    bin_count_kernel<<<(n_sites + 255) / 256, 256, 0, st>>>(bc, reinterpret_cast<const float4*>(spheres), n_sites,
-                                                           tile_counts);
+    bin_count_kernel<<<(n_sites + 255) / 256, 256, 0, st>>>(bc, reinterpret_cast<const float4*>(spheres), live,
+                                                           n_sites, tile_counts);
   PF_HIP(hipGetLastError());
   return PF_OK;
 }
@@ -833,8 +906,9 @@ int pf_bin_scan(const int32_t* tile_counts, int32_t n_tiles, int64_t* offsets, v
   return PF_OK;
 }
 
-int pf_bin_emit(const pf_camera* cam, const float* spheres, int32_t n_sites, const int64_t* offsets, int32_t n_tiles,
-                uint64_t* records, int32_t* cursor, uint8_t* visible, void* stream) {
+int pf_bin_emit(const pf_camera* cam, const float* spheres, const uint8_t* live, int32_t n_sites,
+                const int64_t* offsets, int32_t n_tiles, uint64_t* records, int32_t* cursor, uint8_t* visible,
+                void* stream) {
   if (!cam || !offsets || !cursor || n_sites < 0 || (n_sites > 0 && !spheres))
     return set_error(PF_ERR_ARG, "pf_bin_emit: bad arguments");
   if (cam->ray_maps) return set_error(PF_ERR_UNSUPPORTED, "pf_bin_emit: pinhole cameras only");
@@ -843,7 +917,25 @@ int pf_bin_emit(const pf_camera* cam, const float* spheres, int32_t n_sites, con
   fill_i32_kernel<<<(unsigned)((n_tiles + 255) / 256), 256, 0, st>>>(cursor, n_tiles, 0);
   if (n_sites > 0)
     bin_emit_kernel<<<(n_sites + 255) / 256, 256, 0, st>>>(make_bincam(*cam), reinterpret_cast<const float4*>(spheres),
-                                                          n_sites, offsets, records, cursor, visible);
+                                                          live, n_sites, offsets, records, cursor, visible);
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+int pf_live_sites(const pf_camera* cam, const float* spheres, int32_t n_sites, const int32_t* adjacency,
+                  const int32_t* adjacency_offsets, const uint16_t* diff, int32_t diff_kind, uint8_t* live,
+                  void* stream) {
+  if (!cam || n_sites < 0 || (n_sites > 0 && (!spheres || !adjacency_offsets || !live)))
+    return set_error(PF_ERR_ARG, "pf_live_sites: bad arguments");
+  if (!diff && !adjacency) return set_error(PF_ERR_ARG, "pf_live_sites: needs the adjacency or its encoding");
+  if (diff && diff_kind != PF_FACES_RADIUS && diff_kind != PF_FACES_POWER)
+    return set_error(PF_ERR_ARG, "pf_live_sites: unknown face encoding");
+  if (cam->ray_maps) return set_error(PF_ERR_UNSUPPORTED, "pf_live_sites: pinhole cameras only");
+  if (n_sites == 0) return PF_OK;
+  live_sites_kernel<<<(n_sites + 4 * WAVE - 1) / (4 * WAVE), 256, 0, as_stream(stream)>>>(
+      reinterpret_cast<const float4*>(spheres), n_sites, adjacency, adjacency_offsets,
+      reinterpret_cast<const uint2*>(diff), diff_kind == PF_FACES_POWER ? 1 : 0,
+      f3{cam->eye[0], cam->eye[1], cam->eye[2]}, live);
   PF_HIP(hipGetLastError());
   return PF_OK;
 }

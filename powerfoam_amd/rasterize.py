@@ -93,6 +93,12 @@ class Rasterizer:
         # its total work, so by default it walks whole lists (split_backward).
         self.seg_len = 512
         self.split_backward = False
+        # Leave sites that no ray from the camera can shade (their sphere lies
+        # beyond one of their own faces: pf_live_sites) out of the tile lists
+        # of the forward, backward and benchmark passes. Results are the same
+        # up to float summation order; the visualisation pass, whose
+        # intersection counts follow list positions, always bins every site.
+        self.cull_hidden = True
 
     def _launch(self, name, fn, *a):
         if self.event_log is None:
@@ -121,9 +127,11 @@ class Rasterizer:
         rm = rm.detach().to(device=self.device, dtype=torch.float32).contiguous()
         return TorchCamera(camera.eye, camera.right, camera.up, camera.width, camera.height, rm).to_pf(), rm
 
-    def bin(self, camera, all_spheres):
+    def bin(self, camera, all_spheres, live=None):
         """count -> scan -> (one host sync for M) -> emit -> per-tile sort; ray-map
-        cameras count and emit through the cone tree (rasterize.py:1518-1760)."""
+        cameras count and emit through the cone tree (rasterize.py:1518-1760).
+        `live` (uint8 per site, from live_sites) leaves the sites marked 0 out
+        of the lists; the visibility mask still covers them."""
         lib = self.lib
         cam, keep = self._camera(camera)
         th, tw = _abi.tiles_hw(camera.height, camera.width)
@@ -140,7 +148,7 @@ class Rasterizer:
             check(lib.pf_bin_cones(_ref(cam), ptr(cones), st))
         counts = torch.empty(nt + 1, dtype=torch.int32, device=dev)
         if cones is None:
-            check(lib.pf_bin_count(_ref(cam), ptr(all_spheres), n, ptr(counts), st))
+            check(lib.pf_bin_count(_ref(cam), ptr(all_spheres), ptr(live), n, ptr(counts), st))
         else:
             check(lib.pf_bin_count_cones(_ref(cam), ptr(cones), ptr(all_spheres), n, ptr(counts), st))
         offsets = torch.empty(nt + 1, dtype=torch.int64, device=dev)
@@ -150,8 +158,8 @@ class Rasterizer:
         cursor = torch.empty(nt, dtype=torch.int32, device=dev)
         visible = torch.zeros(n, dtype=torch.uint8, device=dev)
         if cones is None:
-            check(lib.pf_bin_emit(_ref(cam), ptr(all_spheres), n, ptr(offsets), nt, ptr(records), ptr(cursor),
-                                  ptr(visible), st))
+            check(lib.pf_bin_emit(_ref(cam), ptr(all_spheres), ptr(live), n, ptr(offsets), nt, ptr(records),
+                                  ptr(cursor), ptr(visible), st))
         else:
             check(lib.pf_bin_emit_cones(_ref(cam), ptr(cones), ptr(all_spheres), n, ptr(offsets), nt, ptr(records),
                                         ptr(cursor), ptr(visible), st))
@@ -210,6 +218,23 @@ class Rasterizer:
         return _abi.PfTileLists(nt, offsets.data_ptr(), tile_sites.data_ptr() if tile_sites.numel() else None,
                                 order.data_ptr() if order is not None else None)
 
+    def live_sites(self, camera, all_spheres, adjacency, adjacency_offsets, diff, kind=_abi.PF_FACES_RADIUS):
+        """uint8 per site, 0 where the site's sphere lies beyond one of its own
+        faces (as the compositing kernels read them: `diff` in the encoding
+        `kind`, or the full-precision spheres when diff is None), so that no
+        ray from the camera shades it (pf_live_sites); None when culling is
+        off, for ray-map cameras, or without faces."""
+        if not self.cull_hidden or adjacency.shape[0] == 0:
+            return None
+        cam, ray_maps = self._camera(camera)
+        if ray_maps is not None:
+            return None
+        live = torch.empty(all_spheres.shape[0], dtype=torch.uint8, device=all_spheres.device)
+        check(self.lib.pf_live_sites(_ref(cam), ptr(all_spheres), all_spheres.shape[0], ptr(adjacency),
+                                     ptr(adjacency_offsets), ptr(diff) if diff is not None else None, int(kind),
+                                     ptr(live), stream_handle()))
+        return live
+
     def encode_adjacency(self, all_spheres, adjacency, adjacency_offsets):
         """prefetch_adjacency_kernel (rasterize.py:580-606): fp16 (c_adj - c, r_adj)."""
         diff = torch.empty((max(adjacency.shape[0], 1), 4), dtype=torch.float16, device=all_spheres.device)
@@ -241,10 +266,12 @@ class Rasterizer:
         ray_gt_c = _f32c(ray_gt) if ray_gt is not None else None
         dq = _f32c(depth_quantiles) if depth_quantiles is not None else None
 
-        offsets, tile_sites, prim_visible_mask, nt = self.bin(camera, all_spheres)
+        adjacency_diff = self.encode_adjacency(all_spheres, adjacency, adjacency_offsets)
+        live = self.live_sites(camera, all_spheres, adjacency, adjacency_offsets,
+                               None if self.exact_geometry else adjacency_diff)
+        offsets, tile_sites, prim_visible_mask, nt = self.bin(camera, all_spheres, live)
         sg = self.segments(offsets, tile_sites.shape[0])
         order = self.tile_order(offsets) if sg is None else None
-        adjacency_diff = self.encode_adjacency(all_spheres, adjacency, adjacency_offsets)
 
         f32 = dict(dtype=torch.float32, device=dev)
         color_out = torch.zeros((H, W, 3), **f32)
@@ -314,7 +341,9 @@ class Rasterizer:
             adjacency = adjacency.to(torch.int32).contiguous()
             adjacency_offsets = adjacency_offsets.to(torch.int32).contiguous()
             adjacency_diff = adjacency_diff.to(torch.float16).contiguous()
-            offsets, tile_sites, _, nt = self.bin(camera, all_spheres)
+            live = self.live_sites(camera, all_spheres, adjacency, adjacency_offsets, adjacency_diff,
+                                   _abi.PF_FACES_POWER)
+            offsets, tile_sites, _, nt = self.bin(camera, all_spheres, live)
             order = self.tile_order(offsets)
             color_out = torch.zeros((H, W, 3), dtype=torch.float32, device=points.device)
             a = _abi.PfRasterRgbArgs()

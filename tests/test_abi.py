@@ -73,3 +73,21 @@ def test_struct_layout_matches_c():
               ctypes.sizeof(SG), SG.count.offset, SG.seg_len.offset, SG.scratch.offset]
     assert n == len(expect)
     assert out[:n].tolist() == expect
+
+
+def test_live_sites_argument_checks_run_on_the_host():
+    """pf_live_sites refuses unknown face encodings and ray-map cameras before
+    any device work (the addresses below are never dereferenced)."""
+    if not os.path.exists(_abi.LIB_PATH):
+        pytest.skip("libpfhip.so not built")
+    lib = _abi.load()
+    cam = _abi.PfCamera()
+    cam.width, cam.height = 16, 16
+    fake = ctypes.c_void_p(0x1000)
+    st = lib.pf_live_sites(ctypes.byref(cam), fake, 4, fake, fake, fake, 5, fake, None)
+    assert st == _abi.PF_ERR_ARG and b"encoding" in lib.pf_last_error()
+    st = lib.pf_live_sites(ctypes.byref(cam), fake, 4, None, fake, None, 0, fake, None)
+    assert st == _abi.PF_ERR_ARG
+    cam.ray_maps = 0x2000
+    st = lib.pf_live_sites(ctypes.byref(cam), fake, 4, fake, fake, None, 0, fake, None)
+    assert st == _abi.PF_ERR_UNSUPPORTED

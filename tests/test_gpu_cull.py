@@ -1,0 +1,98 @@
+"""GPU side of the site culling: pf_live_sites against the oracle predicate,
+and binning with the mask bit-exact against the oracle's lists with the same
+sites left out (the visibility mask unchanged). Compositing parity with the
+culled lists is covered by the culled case of test_gpu_raster.py."""
+
+import ctypes
+import functools
+
+import numpy as np
+import pytest
+
+from oracle import orc
+from powerfoam_amd import synth
+from tests import harness
+
+pytestmark = pytest.mark.gpu
+
+
+@functools.lru_cache(maxsize=1)
+def _scenes():
+    c = harness.small_case(13, n=2000, width=64, height=48, r_range=(0.02, 0.2))
+    s = synth.scene_gmm(1, n=30_000)
+    adj, off = synth.power_adjacency(s["points"], s["radii"])
+    sp = np.concatenate([s["points"], s["radii"][:, None]], 1).astype(np.float32)
+    return [("cech2k", c.spheres(), c.adjacency, c.adjacency_offsets, c.cam),
+            ("gmm30k", sp, adj, off, synth.circle_cameras(n_views=32, width=331, img_h=217)[3])]
+
+
+def _gpu_live(sp, adj, off, cam_t, form):
+    import torch
+    from powerfoam_amd import _abi
+    from powerfoam_amd.camera import TorchCamera
+    from powerfoam_amd.rasterize import power_face_encoding
+    lib = _abi.load()
+    d_sp = torch.tensor(sp, device="cuda")
+    d_adj = torch.tensor(np.asarray(adj, np.int32), device="cuda")
+    d_off = torch.tensor(np.asarray(off, np.int32), device="cuda")
+    if form == "radius":
+        diff = torch.empty((max(len(adj), 1), 4), dtype=torch.float16, device="cuda")
+        _abi.check(lib.pf_adj_encode(_abi.ptr(d_sp), sp.shape[0], _abi.ptr(d_adj), _abi.ptr(d_off), _abi.ptr(diff),
+                                     None))
+    elif form == "power":
+        diff = power_face_encoding(d_sp[:, :3], d_sp[:, 3], d_adj, d_off)
+    else:
+        diff = None
+    cam = TorchCamera(cam_t.eye, cam_t.right, cam_t.up, cam_t.width, cam_t.height).to_pf()
+    live = torch.full((sp.shape[0],), 7, dtype=torch.uint8, device="cuda")
+    kind = _abi.PF_FACES_POWER if form == "power" else _abi.PF_FACES_RADIUS
+    _abi.check(lib.pf_live_sites(ctypes.byref(cam), _abi.ptr(d_sp), sp.shape[0], _abi.ptr(d_adj), _abi.ptr(d_off),
+                                 _abi.ptr(diff), kind, _abi.ptr(live), None))
+    torch.cuda.synchronize()
+    dh = None if diff is None else diff.cpu().numpy().view(np.uint16)[: len(adj)]
+    return live.cpu().numpy(), dh
+
+
+@pytest.mark.parametrize("form", ["exact", "radius", "power"])
+def test_live_sites_match_oracle(form):
+    for name, sp, adj, off, cam in _scenes():
+        print(f"live sites {name} {form}", flush=True)
+        live_g, diff = _gpu_live(sp, adj, off, cam, form)
+        assert set(np.unique(live_g)) <= {0, 1}
+        live_o, val, src = orc.live_sites(cam.eye.numpy(), sp, adj, off, diff, kind=int(form == "power"),
+                                          return_margin=True)
+        assert (live_o == 0).mean() > 0.15, name
+        # the float32 decision can differ only for a face within rounding of
+        # the margin (the GPU contracts products into fused multiply-adds)
+        for s in np.nonzero(live_g != live_o)[0]:
+            assert np.min(np.abs(val[src == s])) < 1e-6, (name, s)
+        assert np.count_nonzero(live_g != live_o) <= 2, name
+
+
+def test_binning_with_live_mask_bitexact():
+    import torch
+    from types import SimpleNamespace
+    from powerfoam_amd.camera import TorchCamera
+    from powerfoam_amd.rasterize import Rasterizer
+    for name, sp, adj, off, cam_t in _scenes():
+        print(f"binning with live mask {name}", flush=True)
+        r = Rasterizer(SimpleNamespace(num_texel_sites=8, render_objective="volume", is_pinhole=True), "cuda")
+        d_sp = torch.tensor(sp, device="cuda")
+        d_adj = torch.tensor(np.asarray(adj, np.int32), device="cuda")
+        d_off = torch.tensor(np.asarray(off, np.int32), device="cuda")
+        cam = TorchCamera(cam_t.eye, cam_t.right, cam_t.up, cam_t.width, cam_t.height)
+        diff = r.encode_adjacency(d_sp, d_adj, d_off)
+        live = r.live_sites(cam, d_sp, d_adj, d_off, diff)
+        assert live is not None
+        offsets, sites, visible, nt = r.bin(cam, d_sp, live)
+        live_h = live.cpu().numpy()
+        cam_o, _ = orc.camera(cam_t.eye.numpy(), cam_t.right.numpy(), cam_t.up.numpy(), cam_t.width, cam_t.height)
+        off_o, sites_o, _ = orc.bin_lists(cam_o, sp)
+        off_f, sites_f = orc.filter_lists(off_o, sites_o, live_h)
+        assert np.array_equal(offsets.cpu().numpy(), off_f)
+        assert np.array_equal(sites.cpu().numpy(), sites_f)
+        assert sites_f.size < 0.9 * sites_o.size, name
+        # the visibility mask is the footprint's, culled sites included
+        vis_o = np.zeros(sp.shape[0], bool)
+        vis_o[sites_o] = True
+        assert np.array_equal(visible.cpu().numpy(), vis_o)

@@ -32,6 +32,9 @@ CASES = [
     # ray-map cameras (equidistant fisheye), binned through the cone tree
     dict(seed=11, n=1500, width=56, height=56, eye=(0.0, 0.0, 1.3), fisheye_fov=150.0),
     dict(seed=12, n=1500, width=64, height=64, eye=(0.2, 0.1, 0.6), fisheye_fov=180.0, height_std=0.05),
+    # a fifth of the sites sit inside a larger neighbour: the tile lists leave
+    # them out (pf_live_sites), the oracle keeps them, as the reference does
+    dict(seed=13, n=2000, width=64, height=48, r_range=(0.02, 0.2), quantiles=[0.5, 0.1]),
 ]
 
 
@@ -70,8 +73,8 @@ def test_backward_matches_fp64_oracle(dev, cfg):
 
 
 @pytest.mark.parametrize("cfg,attr", [(CASES[0], "float"), (CASES[1], "float"), (CASES[2], "half"),
-                                      (CASES[4], "float")],
-                         ids=["case0", "case1", "case2-half", "case4-fisheye"])
+                                      (CASES[4], "float"), (CASES[6], "float")],
+                         ids=["case0", "case1", "case2-half", "case4-fisheye", "case6-culled"])
 def test_benchmark_matches_fp64_oracle(dev, cfg, attr):
     """Rasterizer.benchmark (colour only, per-ray stop, fp16 power faces)."""
     case = harness.small_case(**cfg)
@@ -127,6 +130,9 @@ SPLIT_CASES = [
     dict(seed=22, n=4000, width=48, height=40, r_range=(0.04, 0.15), quantiles=[0.9, 0.5, 0.1], fov=25.0,
          density_scale=5.0, height_std=0.05),
     dict(seed=12, n=1500, width=64, height=64, eye=(0.2, 0.1, 0.6), fisheye_fov=180.0, height_std=0.05),
+    # a fifth of the sites sit inside a larger neighbour: the tile lists leave
+    # them out (pf_live_sites), the oracle keeps them, as the reference does
+    dict(seed=13, n=2000, width=64, height=48, r_range=(0.02, 0.2), quantiles=[0.5, 0.1]),
 ]
 
 
