@@ -447,9 +447,13 @@ def main():
 
     # counts of every view for the roofline (kernel time averages over views)
     per_view = []
+    sp_all = torch.cat([w["params"]["points"], w["params"]["radii"][:, None]], 1).detach().float().contiguous()
     for v in range(len(cams)):
         frame(rast, head, w, cams[v])
         per_view.append(workload_counts(rast, w, args.width, args.height))
+        # the lists without the culling of sites that cannot shade (what the
+        # reference bins)
+        per_view[-1]["M_all"] = int(rast.bin(cams[v], sp_all)[0][-1])
     torch.cuda.synchronize()
     avg = {k: float(np.mean([c[k] for c in per_view])) for k in per_view[0]}
     order = [cams[i % len(cams)] for i in range(args.steps)]
@@ -478,7 +482,8 @@ def main():
                                      "32 pinhole views 1297x840",
                             sites=args.sites, width=args.width, height=args.height, views=args.views,
                             adjacency="power/alpha", adjacency_edges=int(w["stats"]["edges"]),
-                            pairs_per_frame=avg["M"], processed_pairs_per_frame=avg["pairs"]),
+                            pairs_per_frame=avg["M"], pairs_per_frame_all_sites=avg["M_all"],
+                            processed_pairs_per_frame=avg["pairs"]),
                 kernel_ms=kern_ms, roofline=roofline, cpu_baseline=cpu)
     print(json.dumps(line), flush=True)
 

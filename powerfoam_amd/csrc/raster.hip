@@ -182,7 +182,29 @@ __device__ __forceinline__ float4 face_of(const int32_t* adjacency, const uint16
   return make_float4(n.x, n.y, n.z, w);
 }
 
-// Texel records staged per site: (site - centre, height), (rgb, 0).
+// Stages faces e0 .. e0 + ec - 1 of a site (lane l < ec takes face e0 + l)
+// and pads the chunk to a multiple of four with a face no ray crosses: n = 0
+// makes d.n = 0 and the division's correction step 0 * inf, so t is NaN and
+// no bound comparison accepts it. The face loops then read four faces per
+// LDS round trip. Returns the padded count.
+template <bool PINHOLE>
+__device__ __forceinline__ int stage_faces(float4* s_face, const int32_t* adjacency, const uint16_t* diff,
+                                           const float4* spheres, int e0, int ec, float4 sph, f3 ce, int lane) {
+  const int ec4 = (ec + 3) & ~3;
+  if (lane < ec)
+    s_face[lane] = face_of<PINHOLE>(adjacency, diff, spheres, e0 + lane, sph, ce);
+  else if (lane < ec4)
+    s_face[lane] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+  return ec4;
+}
+
+// Texel records staged per site: (site - centre, height), (rgb, 0). Slots
+// T..7 hold a texel so far away that its weight exp(-10 |x - p|^2 / r^2)
+// is exactly 0 (the exponent overflows to -inf) while x - p stays finite:
+// it adds exact zeros to every sum, so the texel loops run all eight slots
+// without a branch per texel and their LDS reads issue together.
+constexpr float TEXEL_FAR = 1e18f;
+
 struct TexelLds {
   float4 pos_h;
   float4 rgb;
@@ -199,6 +221,9 @@ __device__ __forceinline__ bool stage_texels(TexelLds* s_tex, const float* texel
     s_tex[lane].pos_h = make_float4(ts[0] - c.x, ts[1] - c.y, ts[2] - c.z, v.w);
     s_tex[lane].rgb = make_float4(v.x, v.y, v.z, 0.0f);
     nz = v.w != 0.0f;
+  } else if (lane < PF_MAX_TEXELS) {
+    s_tex[lane].pos_h = make_float4(TEXEL_FAR, TEXEL_FAR, TEXEL_FAR, 0.0f);
+    s_tex[lane].rgb = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   }
   return ballot(nz) != 0;
 }
@@ -360,21 +385,27 @@ __global__ void __launch_bounds__(WAVE* WPB) raster_fwd_kernel(FwdParams p) {
       for (int e0 = sr.a0; e0 < sr.a1; e0 += WAVE) {
         const int ec = __builtin_amdgcn_readfirstlane(min(WAVE, sr.a1 - e0));
         wave_sync();
-        if (lane < ec) s_face[lane] = face_of<PINHOLE>(p.adjacency, p.adj_diff, p.spheres, e0 + lane, sr.sph, ce);
+        const int ec4 = stage_faces<PINHOLE>(s_face, p.adjacency, p.adj_diff, p.spheres, e0, ec, sr.sph, ce, lane);
         wave_sync();
         // every lane runs the faces (a lane that misses the sphere only
         // updates values nothing reads): the loop stays in scalar control
-        for (int e = 0; e < ec; e++) {
-          const float4 f = s_face[e];
-          const f3 n{f.x, f.y, f.z};
-          const float dp = dot(ray.d, n);
-          const float num = PINHOLE ? f.w : f.w + dot(co, n);
-          const float t = acc_div(num, dp);
-          // dp < 0 as the complement of dp >= 0: one compare (a NaN dp
-          // gives a NaN t, which no comparison below accepts)
-          const bool pos = dp >= 0.0f;
-          tf = (pos && t < tf) ? t : tf;
-          tn = (!pos && t > tn) ? t : tn;
+        for (int e = 0; e < ec4; e += 4) {
+          float4 fq[4];
+#pragma unroll
+          for (int j = 0; j < 4; j++) fq[j] = s_face[e + j];
+#pragma unroll
+          for (int j = 0; j < 4; j++) {
+            const float4 f = fq[j];
+            const f3 n{f.x, f.y, f.z};
+            const float dp = dot(ray.d, n);
+            const float num = PINHOLE ? f.w : f.w + dot(co, n);
+            const float t = acc_div(num, dp);
+            // dp < 0 as the complement of dp >= 0: one compare (a NaN dp
+            // gives a NaN t, which no comparison below accepts)
+            const bool pos = dp >= 0.0f;
+            tf = (pos && t < tf) ? t : tf;
+            tn = (!pos && t > tn) ? t : tn;
+          }
         }
       }
       const bool shade = hit && !ray.oob && tn < tf;
@@ -398,13 +429,11 @@ __global__ void __launch_bounds__(WAVE* WPB) raster_fwd_kernel(FwdParams p) {
           float hsum = 0.0f, ws0 = 0.0f;
 #pragma unroll
           for (int i = 0; i < PF_MAX_TEXELS; i++) {
-            if (i < T) {
-              const float4 ph = s_tex[i].pos_h;
-              const f3 df{x0.x - ph.x, x0.y - ph.y, x0.z - ph.z};
-              const float w = fast_exp(dot(df, df) * kexp);
-              hsum += w * ph.w;
-              ws0 += w;
-            }
+            const float4 ph = s_tex[i].pos_h;
+            const f3 df{x0.x - ph.x, x0.y - ph.y, x0.z - ph.z};
+            const float w = fast_exp(dot(df, df) * kexp);
+            hsum += w * ph.w;
+            ws0 += w;
           }
           height = hsum * fast_rcp(sel_max(ws0, 1e-20f));
         }
@@ -421,14 +450,12 @@ __global__ void __launch_bounds__(WAVE* WPB) raster_fwd_kernel(FwdParams p) {
         float ws = 0.0f;
 #pragma unroll
         for (int i = 0; i < PF_MAX_TEXELS; i++) {
-          if (i < T) {
-            const float4 ph = s_tex[i].pos_h;
-            const float4 cl = s_tex[i].rgb;
-            const f3 df{x1.x - ph.x, x1.y - ph.y, x1.z - ph.z};
-            const float w = fast_exp(dot(df, df) * kexp);
-            rs = rs + f3{cl.x, cl.y, cl.z} * w;
-            ws += w;
-          }
+          const float4 ph = s_tex[i].pos_h;
+          const float4 cl = s_tex[i].rgb;
+          const f3 df{x1.x - ph.x, x1.y - ph.y, x1.z - ph.z};
+          const float w = fast_exp(dot(df, df) * kexp);
+          rs = rs + f3{cl.x, cl.y, cl.z} * w;
+          ws += w;
         }
         const f3 col = rs * fast_rcp(sel_max(ws, 1e-20f));
         const float tf2 = (dpn >= 0.0f && ts1 < tf) ? ts1 : tf;
@@ -698,25 +725,31 @@ __global__ void __launch_bounds__(WAVE* WPB) PF_BWD_OCC raster_bwd_kernel(BwdPar
       for (int e0 = sr.a0; e0 < sr.a1; e0 += WAVE) {
         const int ec = __builtin_amdgcn_readfirstlane(min(WAVE, sr.a1 - e0));
         wave_sync();
-        if (lane < ec) s_face[lane] = face_of<PINHOLE>(p.adjacency, p.adj_diff, p.spheres, e0 + lane, sr.sph, ce);
+        const int ec4 = stage_faces<PINHOLE>(s_face, p.adjacency, p.adj_diff, p.spheres, e0, ec, sr.sph, ce, lane);
         wave_sync();
         // every lane runs the faces (a lane that misses the sphere only
         // updates values nothing reads): the loop stays in scalar control
-        for (int e = 0; e < ec; e++) {
-          const float4 f = s_face[e];
-          const f3 n{f.x, f.y, f.z};
-          const float dp = dot(ray.d, n);
-          const float num = PINHOLE ? f.w : f.w + dot(co, n);
-          const float t = acc_div(num, dp);
-          const int id = e0 - sr.a0 + e;
-          const bool pos = dp >= 0.0f;  // one compare, as in the forward
-          if (pos && t < tf) {
-            tf_id = id;
-            tf = t;
-          }
-          if (!pos && t > tn) {
-            tn_id = id;
-            tn = t;
+        for (int e = 0; e < ec4; e += 4) {
+          float4 fq[4];
+#pragma unroll
+          for (int j = 0; j < 4; j++) fq[j] = s_face[e + j];
+#pragma unroll
+          for (int j = 0; j < 4; j++) {
+            const float4 f = fq[j];
+            const f3 n{f.x, f.y, f.z};
+            const float dp = dot(ray.d, n);
+            const float num = PINHOLE ? f.w : f.w + dot(co, n);
+            const float t = acc_div(num, dp);
+            const int id = e0 - sr.a0 + e + j;
+            const bool pos = dp >= 0.0f;  // one compare, as in the forward
+            if (pos && t < tf) {
+              tf_id = id;
+              tf = t;
+            }
+            if (!pos && t > tn) {
+              tn_id = id;
+              tn = t;
+            }
           }
         }
       }
@@ -747,13 +780,11 @@ __global__ void __launch_bounds__(WAVE* WPB) PF_BWD_OCC raster_bwd_kernel(BwdPar
           float hsum = 0.0f, s0 = 0.0f;
 #pragma unroll
           for (int i = 0; i < PF_MAX_TEXELS; i++) {
-            if (i < T) {
-              const float4 ph = s_tex[i].pos_h;
-              const f3 df{x0.x - ph.x, x0.y - ph.y, x0.z - ph.z};
-              const float w = fast_exp(dot(df, df) * kexp);
-              hsum += w * ph.w;
-              s0 += w;
-            }
+            const float4 ph = s_tex[i].pos_h;
+            const f3 df{x0.x - ph.x, x0.y - ph.y, x0.z - ph.z};
+            const float w = fast_exp(dot(df, df) * kexp);
+            hsum += w * ph.w;
+            s0 += w;
           }
           ws0 = sel_max(s0, 1e-20f);
           height = hsum * fast_rcp(ws0);
@@ -767,15 +798,13 @@ __global__ void __launch_bounds__(WAVE* WPB) PF_BWD_OCC raster_bwd_kernel(BwdPar
         float s1 = 0.0f;
 #pragma unroll
         for (int i = 0; i < PF_MAX_TEXELS; i++) {
-          if (i < T) {
-            const float4 ph = s_tex[i].pos_h;
-            const float4 cl = s_tex[i].rgb;
-            const f3 df{x1.x - ph.x, x1.y - ph.y, x1.z - ph.z};
-            const float w = fast_exp(dot(df, df) * kexp);
-            w1[i] = w;
-            rs = rs + f3{cl.x, cl.y, cl.z} * w;
-            s1 += w;
-          }
+          const float4 ph = s_tex[i].pos_h;
+          const float4 cl = s_tex[i].rgb;
+          const f3 df{x1.x - ph.x, x1.y - ph.y, x1.z - ph.z};
+          const float w = fast_exp(dot(df, df) * kexp);
+          w1[i] = w;
+          rs = rs + f3{cl.x, cl.y, cl.z} * w;
+          s1 += w;
         }
         ws = sel_max(s1, 1e-20f);
         if (!heights) ws0 = ws;
@@ -876,16 +905,14 @@ __global__ void __launch_bounds__(WAVE* WPB) PF_BWD_OCC raster_bwd_kernel(BwdPar
         gdp_in = dot(dLdcol, col);
 #pragma unroll
         for (int i = 0; i < PF_MAX_TEXELS; i++) {
-          if (i < T) {
-            const float wn = w1[i] * inv_ws;
-            if (!(wn < cut)) {
-              const float4 ph = s_tex[i].pos_h;
-              const float4 cl = s_tex[i].rgb;
-              const f3 df{x1.x - ph.x, x1.y - ph.y, x1.z - ph.z};
-              const float gdp = dLdcol.x * cl.x + dLdcol.y * cl.y + dLdcol.z * cl.z;
-              dLdx = dLdx - df * (kw * wn * (gdp - gdp_in));
-            }
-          }
+          // a texel under the cut-off adds an exact zero (select, no branch)
+          const float wn = w1[i] * inv_ws;
+          const float4 ph = s_tex[i].pos_h;
+          const float4 cl = s_tex[i].rgb;
+          const f3 df{x1.x - ph.x, x1.y - ph.y, x1.z - ph.z};
+          const float gdp = dLdcol.x * cl.x + dLdcol.y * cl.y + dLdcol.z * cl.z;
+          const float sc = !(wn < cut) ? kw * wn * (gdp - gdp_in) : 0.0f;
+          dLdx = dLdx - df * sc;
         }
         const float dLdtq = dot(dLdx, ray.d);
         float dtn_q = (dpn >= 0.0f || tn0 >= ts1) ? dLdtq : 0.0f;
@@ -907,13 +934,12 @@ __global__ void __launch_bounds__(WAVE* WPB) PF_BWD_OCC raster_bwd_kernel(BwdPar
         if (heights) {
 #pragma unroll
           for (int i = 0; i < PF_MAX_TEXELS; i++) {
-            if (i < T) {
-              const float4 ph = s_tex[i].pos_h;
-              const f3 df{x0.x - ph.x, x0.y - ph.y, x0.z - ph.z};
-              const float w = fast_exp(dot(df, df) * kexp);
-              // the cut-off tests the colour-stage weight sum, as the reference
-              if (!(w * inv_ws < cut)) dLdx0 = dLdx0 - df * (kw * (w * inv_ws0) * (dLdheight * ph.w - gdp0_in));
-            }
+            const float4 ph = s_tex[i].pos_h;
+            const f3 df{x0.x - ph.x, x0.y - ph.y, x0.z - ph.z};
+            const float w = fast_exp(dot(df, df) * kexp);
+            // the cut-off tests the colour-stage weight sum, as the reference
+            const float sc = !(w * inv_ws < cut) ? kw * (w * inv_ws0) * (dLdheight * ph.w - gdp0_in) : 0.0f;
+            dLdx0 = dLdx0 - df * sc;
           }
         }
         const float dLdtq0 = dot(dLdx0, ray.d);
@@ -1198,6 +1224,7 @@ __global__ void __launch_bounds__(WAVE* WPB) raster_rgb_kernel(RgbParams p) {
       if (ballot(open) == 0) continue;
       for (int e0 = sr.a0; e0 < sr.a1 && ballot(open) != 0; e0 += WAVE) {
         const int ec = __builtin_amdgcn_readfirstlane(min(WAVE, sr.a1 - e0));
+        const int ec4 = (ec + 3) & ~3;
         wave_sync();
         if (lane < ec) {
           if (VIS) {
@@ -1210,17 +1237,27 @@ __global__ void __launch_bounds__(WAVE* WPB) raster_rgb_kernel(RgbParams p) {
             if (PINHOLE) w -= dot(p.cam.eye, n);
             s_face[lane] = make_float4(n.x, n.y, n.z, w);
           }
+        } else if (lane < ec4) {
+          s_face[lane] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);  // no ray crosses it (as stage_faces)
         }
         wave_sync();
         if (open) {
-          for (int e = 0; e < ec; e++) {
-            const float4 f = s_face[e];
-            const f3 n{f.x, f.y, f.z};
-            const float dp = dot(ray.d, n);
-            const float num = PINHOLE ? f.w : (VIS ? f.w + dot(co, n) : f.w - dot(ray.o, n));
-            const float t = acc_div(num, dp);
-            tf = (dp >= 0.0f && t < tf) ? t : tf;
-            tn = (dp < 0.0f && t > tn) ? t : tn;
+          // four faces per LDS round trip; a lane leaves once its interval is
+          // empty (later faces only narrow an empty interval further)
+          for (int e = 0; e < ec4; e += 4) {
+            float4 fq[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) fq[j] = s_face[e + j];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+              const float4 f = fq[j];
+              const f3 n{f.x, f.y, f.z};
+              const float dp = dot(ray.d, n);
+              const float num = PINHOLE ? f.w : (VIS ? f.w + dot(co, n) : f.w - dot(ray.o, n));
+              const float t = acc_div(num, dp);
+              tf = (dp >= 0.0f && t < tf) ? t : tf;
+              tn = (dp < 0.0f && t > tn) ? t : tn;
+            }
             if (tn > tf) break;
           }
           open = !(tn > tf);
@@ -1243,13 +1280,11 @@ __global__ void __launch_bounds__(WAVE* WPB) raster_rgb_kernel(RgbParams p) {
           float hsum = 0.0f, ws0 = 0.0f;
 #pragma unroll
           for (int i = 0; i < PF_MAX_TEXELS; i++) {
-            if (i < T) {
-              const float4 ph = s_tex[i].pos_h;
-              const f3 df{x0.x - ph.x, x0.y - ph.y, x0.z - ph.z};
-              const float w = fast_exp(dot(df, df) * kexp);
-              hsum += w * ph.w;
-              ws0 += w;
-            }
+            const float4 ph = s_tex[i].pos_h;
+            const f3 df{x0.x - ph.x, x0.y - ph.y, x0.z - ph.z};
+            const float w = fast_exp(dot(df, df) * kexp);
+            hsum += w * ph.w;
+            ws0 += w;
           }
           height = hsum * fast_rcp(sel_max(ws0, 1e-20f));
         }
@@ -1260,14 +1295,12 @@ __global__ void __launch_bounds__(WAVE* WPB) raster_rgb_kernel(RgbParams p) {
         float ws = 0.0f;
 #pragma unroll
         for (int i = 0; i < PF_MAX_TEXELS; i++) {
-          if (i < T) {
-            const float4 ph = s_tex[i].pos_h;
-            const float4 cl = s_tex[i].rgb;
-            const f3 df{x1.x - ph.x, x1.y - ph.y, x1.z - ph.z};
-            const float w = fast_exp(dot(df, df) * kexp);
-            rs = rs + f3{cl.x, cl.y, cl.z} * w;
-            ws += w;
-          }
+          const float4 ph = s_tex[i].pos_h;
+          const float4 cl = s_tex[i].rgb;
+          const f3 df{x1.x - ph.x, x1.y - ph.y, x1.z - ph.z};
+          const float w = fast_exp(dot(df, df) * kexp);
+          rs = rs + f3{cl.x, cl.y, cl.z} * w;
+          ws += w;
         }
         const f3 col = rs * fast_rcp(sel_max(ws, 1e-20f));
         const float tf2 = (dpn >= 0.0f && ts1 < tf) ? ts1 : tf;

@@ -89,10 +89,21 @@ class Rasterizer:
         self.longest_first = True
         # Split site lists longer than this many sites into segments that run
         # in parallel (None: one wave per list). A multiple of 64. The forward
-        # is bound by its longest lists and gains; the backward is bound by
-        # its total work, so by default it walks whole lists (split_backward).
+        # is bound by its longest lists and gains when they are long (see
+        # split_min); the backward is bound by its total work, so by default
+        # it walks whole lists (split_backward).
         self.seg_len = 512
         self.split_backward = False
+        # Split only when some list is longer than this many sites. Splitting
+        # adds a log-T pass over every segment but the last of each split
+        # list, a launch of its own before the main pass. On the 32 measured
+        # views of the 1M-site scene (longest lists 1,034-2,314 sites once the
+        # sites that cannot shade are left out; long lists mostly reach the
+        # transmittance stop before their end) whole lists are faster on every
+        # view; a lone wave walks about 0.6 sites per microsecond, so lists of
+        # several thousand walked sites would set the forward's duration.
+        self.split_min = 4096
+        self._longest = None
         # Leave sites that no ray from the camera can shade (their sphere lies
         # beyond one of their own faces: pf_live_sites) out of the tile lists
         # of the forward, backward and benchmark passes. Results are the same
@@ -153,7 +164,9 @@ class Rasterizer:
             check(lib.pf_bin_count_cones(_ref(cam), ptr(cones), ptr(all_spheres), n, ptr(counts), st))
         offsets = torch.empty(nt + 1, dtype=torch.int64, device=dev)
         check(lib.pf_bin_scan(ptr(counts), nt, ptr(offsets), st))
-        m = int(offsets[-1].item())
+        # the one host sync of the frame: the record count, and the longest
+        # list (which decides whether lists are split into segments)
+        m, self._longest = (int(v) for v in torch.stack([offsets[-1], counts.max().to(torch.int64)]).tolist())
         records = torch.empty(max(m, 1), dtype=torch.int64, device=dev)
         cursor = torch.empty(nt, dtype=torch.int32, device=dev)
         visible = torch.zeros(n, dtype=torch.uint8, device=dev)
@@ -180,11 +193,12 @@ class Rasterizer:
                                         stream_handle()))
         return order
 
-    def segments(self, offsets, n_records):
+    def segments(self, offsets, n_records, longest=None):
         """Segment table of the compositing kernels (pf_segments) for lists of
         n_records sites in all, with the longest-first tile order under
-        "tile_order", built on the device in one launch; or None."""
-        if not self.seg_len:
+        "tile_order", built on the device in one launch; or None (splitting
+        off, or no list longer than split_min)."""
+        if not self.seg_len or (longest is not None and longest <= self.split_min):
             return None
         L = int(self.seg_len)
         nt = offsets.shape[0] - 1
@@ -270,7 +284,7 @@ class Rasterizer:
         live = self.live_sites(camera, all_spheres, adjacency, adjacency_offsets,
                                None if self.exact_geometry else adjacency_diff)
         offsets, tile_sites, prim_visible_mask, nt = self.bin(camera, all_spheres, live)
-        sg = self.segments(offsets, tile_sites.shape[0])
+        sg = self.segments(offsets, tile_sites.shape[0], self._longest)
         order = self.tile_order(offsets) if sg is None else None
 
         f32 = dict(dtype=torch.float32, device=dev)

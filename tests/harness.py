@@ -108,6 +108,7 @@ def run_gpu(case, dev, grads_in=None, exact_gradients=False, exact_geometry=Fals
     r = Rasterizer(_args(T, case.cam.ray_maps is None), dev)
     if seg_len != "default":
         r.seg_len = seg_len
+        r.split_min = 0  # split every list longer than seg_len
     r.split_backward = split_backward
     r.exact_gradients = exact_gradients
     r.exact_geometry = exact_geometry

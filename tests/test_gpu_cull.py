@@ -63,7 +63,7 @@ def test_live_sites_match_oracle(form):
                                           return_margin=True)
         assert (live_o == 0).mean() > 0.15, name
         # the float32 decision can differ only for a face within rounding of
-        # the margin (the GPU contracts products into fused multiply-adds)
+        # the margin (the square roots and the order of the products)
         for s in np.nonzero(live_g != live_o)[0]:
             assert np.min(np.abs(val[src == s])) < 1e-6, (name, s)
         assert np.count_nonzero(live_g != live_o) <= 2, name
