@@ -121,6 +121,10 @@ typedef struct pf_raster_fwd_args {
   float* err;               /* [H,W] or NULL (loss_accumulation only) */
   float* contrib;           /* [N] accumulated */
   float* point_err;         /* [N] accumulated, or NULL */
+  /* [M] or NULL, zeroed by the caller: set to 1 for each list entry whose
+   * site shades some ray of its tile (an interval of positive length).
+   * Not in the reference: the backward skips the other entries. */
+  uint8_t* entry_shades;
 } pf_raster_fwd_args;
 
 typedef struct pf_raster_bwd_args {
@@ -150,6 +154,9 @@ typedef struct pf_raster_bwd_args {
   float* grad_texel_sites;  /* [N,T,3] */
   float* grad_texel_rgb;    /* [N,T,3] */
   float* grad_texel_height; /* [N,T] */
+  /* [M] from the forward (pf_raster_fwd_args.entry_shades), or NULL: the
+   * walk skips entries marked 0, which add nothing to any gradient */
+  const uint8_t* entry_shades;
 } pf_raster_bwd_args;
 
 /* Benchmark forward: colour only. scene.adjacency_diff holds the power-face

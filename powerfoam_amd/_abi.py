@@ -104,6 +104,7 @@ class PfRasterFwdArgs(ctypes.Structure):
         ("err", ctypes.c_void_p),
         ("contrib", ctypes.c_void_p),
         ("point_err", ctypes.c_void_p),
+        ("entry_shades", ctypes.c_void_p),
     ]
 
 
@@ -129,6 +130,7 @@ class PfRasterBwdArgs(ctypes.Structure):
         ("grad_texel_sites", ctypes.c_void_p),
         ("grad_texel_rgb", ctypes.c_void_p),
         ("grad_texel_height", ctypes.c_void_p),
+        ("entry_shades", ctypes.c_void_p),
     ]
 
 

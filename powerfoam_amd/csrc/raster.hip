@@ -135,7 +135,7 @@ __device__ __forceinline__ Ray lane_ray(const RasterCam& cam, int tile, int lane
 struct SiteRec {
   float4 sph;  // centre, radius
   float4 nsg;  // normal, sigma
-  int site, a0, a1, pad;
+  int site, a0, a1, pad;  // pad: the backward's copy of entry_shades
 };
 
 __device__ __forceinline__ float uniform_f(float x) {
@@ -152,7 +152,7 @@ __device__ __forceinline__ SiteRec uniform_site(const SiteRec& r) {
   u.site = __builtin_amdgcn_readfirstlane(r.site);
   u.a0 = __builtin_amdgcn_readfirstlane(r.a0);
   u.a1 = __builtin_amdgcn_readfirstlane(r.a1);
-  u.pad = 0;
+  u.pad = __builtin_amdgcn_readfirstlane(r.pad);
   return u;
 }
 
@@ -183,10 +183,10 @@ __device__ __forceinline__ float4 face_of(const int32_t* adjacency, const uint16
 }
 
 // Stages faces e0 .. e0 + ec - 1 of a site (lane l < ec takes face e0 + l)
-// and pads the chunk to a multiple of four with a face no ray crosses: n = 0
-// makes d.n = 0 and the division's correction step 0 * inf, so t is NaN and
-// no bound comparison accepts it. The face loops then read four faces per
-// LDS round trip. Returns the padded count.
+// and pads the chunk to a multiple of four with a face no ray crosses: its
+// offset is NaN, so every crossing t computed from it is NaN and no bound
+// comparison accepts it. The face loops then read four faces per LDS round
+// trip. Returns the padded count.
 template <bool PINHOLE>
 __device__ __forceinline__ int stage_faces(float4* s_face, const int32_t* adjacency, const uint16_t* diff,
                                            const float4* spheres, int e0, int ec, float4 sph, f3 ce, int lane) {
@@ -194,7 +194,7 @@ __device__ __forceinline__ int stage_faces(float4* s_face, const int32_t* adjace
   if (lane < ec)
     s_face[lane] = face_of<PINHOLE>(adjacency, diff, spheres, e0 + lane, sph, ce);
   else if (lane < ec4)
-    s_face[lane] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    s_face[lane] = make_float4(0.0f, 0.0f, 0.0f, __builtin_nanf(""));
   return ec4;
 }
 
@@ -260,6 +260,7 @@ struct FwdParams {
   const int32_t* seg_first;
   const int32_t* seg_tile;
   const int32_t* seg_order;
+  uint8_t* entry_shades;  // [M] or null: 1 for each walked entry that shades a ray
   float* seg_logt;   // [n_segs, 64] sum of -sigma dt over the segment
   float* seg_final;  // [n_segs, 64] log T where the segment's wave stopped
   uint8_t* seg_active;  // [n_segs, 64] the lane shaded a site of its walk
@@ -416,6 +417,7 @@ __global__ void __launch_bounds__(WAVE* WPB) raster_fwd_kernel(FwdParams p) {
       const bool heights = stage_texels(s_tex, p.texel_sites, p.texel_rgbh, sr.site, T, c, lane);
       wave_sync();
       float alpha_t = 0.0f, perr = 0.0f;
+      bool shaded = false;  // an interval of positive length (the backward's "active")
       if (shade) {
         // plane_intersection_fwd (texture.py:10-71)
         const float dpn = dot(pn, ray.d);
@@ -463,6 +465,7 @@ __global__ void __launch_bounds__(WAVE* WPB) raster_fwd_kernel(FwdParams p) {
         const float dt = tf2 - tn2;
         if (dt > 0.0f) {
           any_active = true;
+          shaded = true;
           const float dlt = -sigma * dt;
           const float alpha = 1.0f - fast_exp(dlt);
           const float at = alpha * trans;
@@ -489,6 +492,10 @@ __global__ void __launch_bounds__(WAVE* WPB) raster_fwd_kernel(FwdParams p) {
             }
           }
         }
+      }
+      if (!LOGT && p.entry_shades) {
+        const bool any = ballot(shaded) != 0;  // every lane votes
+        if (any && lane == 0) p.entry_shades[start + base + kk] = 1;
       }
       // contrib / point_err: one atomic per site per wave (rasterize.py:1101-1110)
       if (!LOGT && ballot(alpha_t > 0.0f) != 0) {
@@ -601,6 +608,7 @@ struct BwdParams {
   const float* seg_logt;  // [n_segs, 64] from the forward
   const uint8_t* seg_active;  // [n_segs, 64] from the forward
   float* seg_dsum;        // [n_segs, 64] per-segment sum of dL/dlog T increments
+  const uint8_t* entry_shades;  // [M] from the forward, or null: entries marked 0 are skipped
 };
 
 constexpr int RED_SLICE = 16;      // values reduced per LDS pass
@@ -616,11 +624,13 @@ __device__ __forceinline__ float* grad_slot(const BwdParams& p, int site, int T,
   return p.g_theight + (size_t)site * T + i;
 }
 
-#ifdef PF_BWD_WAVES
-#define PF_BWD_OCC __attribute__((amdgpu_waves_per_eu(PF_BWD_WAVES)))
-#else
-#define PF_BWD_OCC
+// The backward's register budget: three waves per SIMD (at most 168
+// VGPRs); left to itself the allocator lands just above and the kernel runs
+// two waves per SIMD.
+#ifndef PF_BWD_WAVES
+#define PF_BWD_WAVES 3
 #endif
+#define PF_BWD_OCC __attribute__((amdgpu_waves_per_eu(PF_BWD_WAVES)))
 // DSUM = true: the segment prepass of a split list. dL/dlog T, the adjoint
 // that carries information from the back of a list to its front, gains an
 // additive increment at every site; this pass sums the increments of one
@@ -704,10 +714,14 @@ __global__ void __launch_bounds__(WAVE* WPB) PF_BWD_OCC raster_bwd_kernel(BwdPar
       s_site[lane].site = s;
       s_site[lane].a0 = p.adj_off[s];
       s_site[lane].a1 = p.adj_off[s + 1];
+      s_site[lane].pad = p.entry_shades ? p.entry_shades[start + base + lane] : 1;
     }
     wave_sync();
     for (int kk = cnt - 1; kk >= 0; kk--) {
       const SiteRec sr = uniform_site(s_site[kk]);
+      // an entry that shades no ray of the tile (as the forward found)
+      // adds nothing: skip its geometry altogether
+      if (!sr.pad) continue;
       const f3 c{sr.sph.x, sr.sph.y, sr.sph.z};
       const float r = sr.sph.w;
       const f3 pn{sr.nsg.x, sr.nsg.y, sr.nsg.z};
@@ -1238,7 +1252,7 @@ __global__ void __launch_bounds__(WAVE* WPB) raster_rgb_kernel(RgbParams p) {
             s_face[lane] = make_float4(n.x, n.y, n.z, w);
           }
         } else if (lane < ec4) {
-          s_face[lane] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);  // no ray crosses it (as stage_faces)
+          s_face[lane] = make_float4(0.0f, 0.0f, 0.0f, __builtin_nanf(""));  // no ray crosses it (as stage_faces)
         }
         wave_sync();
         if (open) {
@@ -1406,6 +1420,7 @@ int pf_raster_fwd(const pf_raster_fwd_args* a, void* stream) {
   p.err = a->err;
   p.contrib = a->contrib;
   p.point_err = a->point_err;
+  p.entry_shades = a->entry_shades;
   const pf_segments& sg = a->segs;
   hipStream_t hs = as_stream(stream);
   if (sg.n_segs > 0) {
@@ -1491,6 +1506,7 @@ int pf_raster_bwd(const pf_raster_bwd_args* a, void* stream) {
   p.g_tsites = a->grad_texel_sites;
   p.g_trgb = a->grad_texel_rgb;
   p.g_theight = a->grad_texel_height;
+  p.entry_shades = a->entry_shades;
   const pf_segments& sg = a->segs;
   hipStream_t hs = as_stream(stream);
   if (sg.n_segs > 0) {

@@ -110,6 +110,10 @@ class Rasterizer:
         # up to float summation order; the visualisation pass, whose
         # intersection counts follow list positions, always bins every site.
         self.cull_hidden = True
+        # The forward marks the list entries that shade some ray of their tile
+        # and the backward walks only those (an entry that shades nothing adds
+        # nothing to any gradient). Not in the reference.
+        self.skip_unshaded = True
 
     def _launch(self, name, fn, *a):
         if self.event_log is None:
@@ -297,6 +301,8 @@ class Rasterizer:
         point_err_out = torch.zeros(N, **f32) if return_point_err else None
         err_out = torch.zeros((H, W), **f32) if self.loss_accumulation else None
         tile_early_stop = torch.zeros(nt, dtype=torch.int32, device=dev)
+        entry_shades = (torch.zeros(max(tile_sites.shape[0], 1), dtype=torch.uint8, device=dev)
+                        if self.skip_unshaded else None)
 
         a = _abi.PfRasterFwdArgs()
         a.cam, ray_maps = self._camera(camera)
@@ -320,6 +326,7 @@ class Rasterizer:
         a.err = err_out.data_ptr() if err_out is not None else None
         a.contrib = contrib_out.data_ptr()
         a.point_err = point_err_out.data_ptr() if point_err_out is not None else None
+        a.entry_shades = entry_shades.data_ptr() if entry_shades is not None else None
         self._launch("raster_fwd", self.lib.pf_raster_fwd, _ref(a), stream_handle())
         self.last_tiles = (offsets, tile_sites, tile_early_stop)
 
@@ -328,6 +335,7 @@ class Rasterizer:
         grad_ctx.ray_maps = ray_maps
         grad_ctx.order = sg["tile_order"] if sg is not None else order
         grad_ctx.segs = sg
+        grad_ctx.entry_shades = entry_shades
         grad_ctx.save_for_backward(dq, all_spheres, all_nsigmas, all_texel_sites, all_texel_rgbh, adjacency,
                                    adjacency_offsets, adjacency_diff, tile_sites, offsets, tile_early_stop,
                                    ray_gt_c, color_out, log_t_out)
@@ -468,6 +476,8 @@ class Rasterizer:
         a.grad_quantile_depths = grad_q.data_ptr() if grad_q is not None else None
         a.grad_contrib = grad_contrib.data_ptr()
         a.exact_gradients = int(bool(self.exact_gradients))
+        es = getattr(grad_ctx, "entry_shades", None)
+        a.entry_shades = es.data_ptr() if es is not None else None
         a.grad_spheres = spheres_grad.data_ptr()
         a.grad_nsigmas = nsigmas_grad.data_ptr()
         a.grad_texel_sites = texel_sites_grad.data_ptr()

@@ -64,7 +64,8 @@ def test_struct_layout_matches_c():
               ctypes.sizeof(S), S.adjacency_diff.offset,
               ctypes.sizeof(TL), TL.sites.offset,
               ctypes.sizeof(F), F.tiles.offset, F.transmittance_threshold.offset, F.point_err.offset,
-              ctypes.sizeof(B), B.exact_gradients.offset, B.grad_texel_height.offset,
+              F.entry_shades.offset,
+              ctypes.sizeof(B), B.exact_gradients.offset, B.grad_texel_height.offset, B.entry_shades.offset,
               ctypes.sizeof(RG), RG.color.offset,
               ctypes.sizeof(V), V.bkgd_color.offset, V.intersections.offset,
               ctypes.sizeof(TA), TA.max_steps.offset, TA.n_cells.offset,
