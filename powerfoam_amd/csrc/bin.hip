@@ -827,6 +827,23 @@ __global__ void __launch_bounds__(SCHED_THREADS)
   auto seg_bucket = [&](int64_t c, int k) {
     return (int)min(min(c - (int64_t)k * L, (int64_t)L), (int64_t)(SCHED_BUCKETS - 1));
   };
+  if (!segs) {
+    // the order alone: tiles interleaved over the threads (coalesced reads),
+    // histogram, descending bucket starts, scatter
+    if (!tile_order) return;
+    __syncthreads();  // the histogram is zeroed
+    for (int t = threadIdx.x; t < nt; t += SCHED_THREADS) atomicAdd(&ht[len_bucket(cnt(t))], 1);
+    __syncthreads();
+    {
+      const int bk = SCHED_BUCKETS - 1 - threadIdx.x;
+      int tot;
+      const int t_start = block_excl_scan(ht[bk], wsum, &tot);
+      ht[bk] = t_start;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < nt; t += SCHED_THREADS) tile_order[atomicAdd(&ht[len_bucket(cnt(t))], 1)] = t;
+    return;
+  }
   int local = 0;
   if (segs)
     for (int t = t0; t < t1; t++) local += nseg(cnt(t));

@@ -38,6 +38,8 @@
 // are summed by an LDS transpose in 16-value slices (two texels per slice,
 // recomputed in a final pass so no per-texel accumulators stay live), and
 // one lane per value issues the atomic.
+#include <algorithm>
+
 #include "pf_common.hpp"
 
 namespace pf {
@@ -1359,6 +1361,22 @@ __global__ void __launch_bounds__(256) scrub_kernel(float* x, int64_t n) {
   if (i < n && !isfinite(x[i])) x[i] = 0.0f;
 }
 
+// The same over float4 words (a 16-byte aligned buffer of n4 words): a read
+// of every value and a write only where one is not finite.
+__global__ void __launch_bounds__(256) scrub4_kernel(float4* x, int64_t n4) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
+    float4 v = x[i];
+    if (!(isfinite(v.x) && isfinite(v.y) && isfinite(v.z) && isfinite(v.w))) {
+      v.x = isfinite(v.x) ? v.x : 0.0f;
+      v.y = isfinite(v.y) ? v.y : 0.0f;
+      v.z = isfinite(v.z) ? v.z : 0.0f;
+      v.w = isfinite(v.w) ? v.w : 0.0f;
+      x[i] = v;
+    }
+  }
+}
+
 }  // namespace pf
 
 using namespace pf;
@@ -1628,7 +1646,15 @@ int pf_raster_vis(const pf_raster_vis_args* a, void* stream) {
 int pf_scrub_nonfinite(float* data, int64_t n, void* stream) {
   if (n < 0 || (n > 0 && !data)) return set_error(PF_ERR_ARG, "pf_scrub_nonfinite: bad arguments");
   if (n == 0) return PF_OK;
-  scrub_kernel<<<(unsigned)((n + 255) / 256), 256, 0, as_stream(stream)>>>(data, n);
+  hipStream_t st = as_stream(stream);
+  if (reinterpret_cast<uintptr_t>(data) % 16 == 0 && n >= 4) {
+    const int64_t n4 = n / 4;
+    const unsigned grid = (unsigned)std::min<int64_t>((n4 + 255) / 256, 4096);
+    scrub4_kernel<<<grid, 256, 0, st>>>(reinterpret_cast<float4*>(data), n4);
+    if (n % 4) scrub_kernel<<<1, 256, 0, st>>>(data + 4 * n4, n % 4);
+  } else {
+    scrub_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(data, n);
+  }
   PF_HIP(hipGetLastError());
   return PF_OK;
 }

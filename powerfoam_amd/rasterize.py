@@ -10,6 +10,7 @@ raises.
 
 from dataclasses import dataclass, field
 
+import numpy as np
 import torch
 
 from . import _abi
@@ -448,11 +449,12 @@ class Rasterizer:
         if dq is not None:
             grad_q = g(grad_quantile_depths_in, tuple(dq.shape))
 
-        spheres_grad = zf((N, 4))
-        nsigmas_grad = zf((N, 4))
-        texel_sites_grad = zf((N, T, 3))
-        texel_rgb_grad = zf((N, T, 3))
-        texel_height_grad = zf((N, T))
+        # the five gradients are views of one zeroed buffer: one fill, one scrub
+        shapes = ((N, 4), (N, 4), (N, T, 3), (N, T, 3), (N, T))
+        sizes = [int(np.prod(sh)) for sh in shapes]
+        grad_buf = zf((sum(sizes),))
+        spheres_grad, nsigmas_grad, texel_sites_grad, texel_rgb_grad, texel_height_grad = (
+            v.view(sh) for v, sh in zip(torch.split(grad_buf, sizes), shapes))
 
         a = _abi.PfRasterBwdArgs()
         a.cam, ray_maps = self._camera(camera, grad_ctx.ray_maps)
@@ -485,8 +487,7 @@ class Rasterizer:
         a.grad_texel_height = texel_height_grad.data_ptr()
         st = stream_handle()
         self._launch("raster_bwd", self.lib.pf_raster_bwd, _ref(a), st)
-        for t in (spheres_grad, nsigmas_grad, texel_sites_grad, texel_rgb_grad, texel_height_grad):
-            check(self.lib.pf_scrub_nonfinite(ptr(t), t.numel(), st))
+        check(self.lib.pf_scrub_nonfinite(ptr(grad_buf), grad_buf.numel(), st))
         del grad_ctx.rasterizer, grad_ctx.camera, grad_ctx.ray_maps, grad_ctx.order, grad_ctx.segs
         return (None, None, None, spheres_grad[:, :3], spheres_grad[:, 3], nsigmas_grad[:, 3],
                 nsigmas_grad[:, :3], texel_sites_grad, texel_rgb_grad, texel_height_grad, None, None, None, None)

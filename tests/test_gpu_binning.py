@@ -153,6 +153,32 @@ def test_tile_schedule_orders_and_segment_table(seg_len):
     assert np.all(np.diff(np.minimum(size[so], 1023)) <= 0)
 
 
+@pytest.mark.parametrize("nt", [1, 1000, 17115, 70000])
+def test_tile_schedule_order_only(nt):
+    """pf_tile_schedule without the segment outputs (the order the compositing
+    kernels take when no list is split): a permutation of the tiles sorted by
+    descending length bucket, also for tile counts that are not a multiple of
+    the 1024 threads and above 64 tiles per thread."""
+    import torch
+    from powerfoam_amd import _abi
+    from powerfoam_amd._abi import check, ptr
+    lib = _abi.load()
+    rng = np.random.default_rng(nt)
+    cnt = np.minimum(rng.lognormal(4.0, 1.6, nt).astype(np.int64), 40000)
+    cnt[rng.random(nt) < 0.05] = 0
+    off = np.zeros(nt + 1, np.int64)
+    off[1:] = np.cumsum(cnt)
+    offsets = torch.tensor(off, device="cuda")
+    order = torch.full((nt,), -1, dtype=torch.int32, device="cuda")
+    print(f"tile order nt={nt}", flush=True)
+    check(lib.pf_tile_schedule(ptr(offsets), nt, int(off[-1]), 0, 0, ptr(order), None, None, None, None, None))
+    torch.cuda.synchronize()
+    o = order.cpu().numpy()
+    assert np.array_equal(np.sort(o), np.arange(nt))
+    b = np.where(cnt < 512, cnt, np.minimum(512 + ((cnt - 512) >> 5), 1023))
+    assert np.all(np.diff(b[o]) <= 0)
+
+
 def test_sort_long_tiles_with_and_without_workspace():
     """pf_bin_sort on segments from 1 to 40,000 records (the 16k-record LDS path
     and its global-merge tail above that): with the workspace (long tiles in
