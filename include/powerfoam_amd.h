@@ -246,9 +246,11 @@ const char* pf_last_error(void);
  * projected footprint passes the tile test; tile t's count lands in
  * tile_counts[t+1] so an inclusive scan gives offsets directly. `live`
  * (uint8 [n_sites], from pf_live_sites) leaves out the sites marked 0; NULL
- * counts every site, as the reference does. */
+ * counts every site, as the reference does. `bounds` ([n_sites,4] f32 balls
+ * from pf_live_sites, 16-byte aligned, or NULL) keeps a (tile, site) pair
+ * only when the tile also meets the site's ball footprint. */
 int pf_bin_count(const pf_camera* cam, const float* spheres, const uint8_t* live,
-                 int32_t n_sites, int32_t* tile_counts, void* stream);
+                 const float* bounds, int32_t n_sites, int32_t* tile_counts, void* stream);
 
 /* offsets[k] = sum(tile_counts[0..k]) as int64 (torch.cumsum of the counts). */
 int pf_bin_scan(const int32_t* tile_counts, int32_t n_tiles, int64_t* offsets,
@@ -259,9 +261,10 @@ int pf_bin_scan(const int32_t* tile_counts, int32_t n_tiles, int64_t* offsets,
  * scratch array the call zeroes. Sets visible[site] = 1 for every site with
  * a tile (the reference's prim_visible_mask, rasterize.py:1761-1765), also
  * for sites that `live` leaves out of the lists; the caller zeroes `visible`
- * first (or passes NULL). `live` as for pf_bin_count, and the same mask. */
+ * first (or passes NULL). `live` and `bounds` as for pf_bin_count, and the
+ * same arrays. */
 int pf_bin_emit(const pf_camera* cam, const float* spheres, const uint8_t* live,
-                int32_t n_sites, const int64_t* offsets, int32_t n_tiles,
+                const float* bounds, int32_t n_sites, const int64_t* offsets, int32_t n_tiles,
                 uint64_t* records, int32_t* cursor, uint8_t* visible, void* stream);
 
 /* Face encodings pf_live_sites reads. */
@@ -278,11 +281,18 @@ enum {
  * same whether or not it is binned. The faces are read as the compositing
  * kernels read them: `diff` in the form `diff_kind` names, or, with diff
  * NULL, the full-precision spheres of `adjacency`. Pinhole cameras only.
+ * With `bounds` ([n_sites,4] f32, 16-byte aligned, or NULL) it also writes
+ * per site a ball (centre, radius) that holds every point the site can
+ * shade: the sphere (inflated for FP32 rounding), shrunk to the enclosing
+ * ball of its cap on the cell's side of each face that leaves the ball's
+ * centre outside, three sweeps over the faces; a site whose ball falls
+ * wholly outside a face gets live 0 as well.
  * Not in the reference: the reference bins every site in the footprint
  * (count_visible_pinhole_kernel rasterize.py:140-200 has no such test). */
 int pf_live_sites(const pf_camera* cam, const float* spheres, int32_t n_sites,
                   const int32_t* adjacency, const int32_t* adjacency_offsets,
-                  const uint16_t* diff, int32_t diff_kind, uint8_t* live, void* stream);
+                  const uint16_t* diff, int32_t diff_kind, uint8_t* live,
+                  float* bounds, void* stream);
 
 /* Ray-map cameras (cam->ray_maps set; rasterize.py:1681-1760 when
  * args.is_pinhole is false) bin through a quadtree of ray cones instead of

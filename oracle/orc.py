@@ -209,6 +209,89 @@ def live_sites(eye, spheres, adjacency, offsets, diff=None, kind=0, return_margi
     return live
 
 
+def site_bounds(eye, spheres, adjacency, offsets, diff=None, kind=0, live=None, passes=3):
+    """A ball (centre, radius) per site enclosing every point the compositing
+    kernels can shade for the site (pf_live_sites' bounds): the sphere,
+    inflated by 2e-3 sqrt(r |c - eye|) for the rounding of tangent rays, is
+    replaced, for each face whose plane (moved outward by the plane-rounding
+    margin) leaves the ball's centre outside, by the smallest ball enclosing
+    the cap on the cell's side; `passes` sweeps over the faces in list order.
+    Faces are read as live_sites reads them. Returns (bounds [N,4] float32,
+    empty [N] bool: a cap vanished, so nothing of the site can shade). Not a
+    reference function: the reference bins by the sphere alone."""
+    sp = np.asarray(spheres, np.float32)
+    adj = np.asarray(adjacency, np.int64)
+    off = np.asarray(offsets, np.int64)
+    n = sp.shape[0]
+    eye = np.asarray(eye, np.float32).reshape(3)
+    deg = np.diff(off)
+    src = np.repeat(np.arange(n), deg)
+    c = sp[src, :3]
+    r = sp[src, 3]
+    if diff is not None:
+        d = np.asarray(diff, np.uint16).view(np.float16).astype(np.float32).reshape(-1, 4)
+        nv, q = d[:, :3], d[:, 3]
+        h = q - (c * nv).sum(1, dtype=np.float32) if kind == 1 else \
+            np.float32(0.5) * ((nv * nv).sum(1, dtype=np.float32) + r * r - q * q)
+    else:
+        a = sp[adj]
+        nv = a[:, :3] - c
+        h = np.float32(0.5) * ((nv * nv).sum(1, dtype=np.float32) + r * r - a[:, 3] * a[:, 3])
+    nl = np.sqrt((nv * nv).sum(1, dtype=np.float32))
+    ok = nl > 0
+    inv = np.where(ok, np.float32(1) / np.where(ok, nl, 1), 0).astype(np.float32)
+    u = nv * inv[:, None]
+    dist = np.sqrt(((c - eye) ** 2).sum(1, dtype=np.float32)) + r
+    m = np.float32(1e-5) * (dist + np.sqrt((c * c).sum(1, dtype=np.float32)) + np.sqrt((eye * eye).sum())
+                            + np.abs(h) * inv)
+    hd = h * inv + m                      # plane offset from the site centre, moved outward
+    ctr = sp[:, :3].astype(np.float32).copy()
+    D = np.sqrt(((sp[:, :3] - eye) ** 2).sum(1, dtype=np.float32)) + sp[:, 3]
+    rad = (sp[:, 3] + np.float32(2e-3) * np.sqrt(sp[:, 3] * D)).astype(np.float32)
+    empty = np.zeros(n, bool)
+    pos = np.arange(adj.size) - off[src]
+    for _ in range(passes):
+        for k in range(int(deg.max()) if deg.size else 0):
+            e = np.nonzero((pos == k) & ok)[0]
+            s_ = src[e]
+            # signed distance of the ball centre inside the half-space y.u <= hd
+            delta = hd[e] - ((ctr[s_] - c[e]) * u[e]).sum(1, dtype=np.float32)
+            gone = delta <= -rad[s_]
+            empty[s_[gone]] = True
+            cut = (delta < 0) & ~gone
+            sc, ec, dc = s_[cut], e[cut], delta[cut]
+            ctr[sc] = ctr[sc] + u[ec] * dc[:, None]
+            rad[sc] = np.sqrt(np.maximum(rad[sc] * rad[sc] - dc * dc, 0)).astype(np.float32)
+    bounds = np.concatenate([ctr, rad[:, None]], 1).astype(np.float32)
+    if live is not None:
+        empty |= np.asarray(live) == 0
+    return bounds, empty
+
+
+def restrict_lists(cam, eye, right, up, offsets, sites, bounds, live):
+    """Tile lists as binning with pf_live_sites' mask and balls makes them:
+    the entries of live sites whose tile also meets the ball's footprint (a
+    ball that fails the sphere cull -- closer than 4 radii to the eye, or
+    behind it -- does not restrict its site), order kept."""
+    b = np.asarray(bounds, np.float32)
+    eye = np.asarray(eye, np.float32)
+    fwd = np.cross(np.asarray(up, np.float32), np.asarray(right, np.float32)).astype(np.float32)
+    v = b[:, :3] - eye
+    free = (np.sqrt((v * v).sum(1, dtype=np.float32)) < np.float32(4) * b[:, 3]) | \
+        ((v * fwd).sum(1, dtype=np.float32) < np.float32(0.1))
+    ob, sb, _ = bin_lists(cam, np.where(free[:, None], np.float32(1e6), b).astype(np.float32))
+    off = np.asarray(offsets, np.int64)
+    tile = np.repeat(np.arange(off.size - 1), np.diff(off))
+    tb = np.repeat(np.arange(ob.size - 1), np.diff(ob))
+    n = b.shape[0]
+    in_ball = np.isin(tile * np.int64(n) + sites, tb * np.int64(n) + sb)
+    keep = (np.asarray(live)[sites] != 0) & (in_ball | free[sites])
+    cnt = np.bincount(tile[keep], minlength=off.size - 1)
+    out = np.zeros_like(off)
+    out[1:] = np.cumsum(cnt)
+    return out, np.asarray(sites)[keep]
+
+
 def filter_lists(offsets, sites, live):
     """Tile lists with the sites live marks 0 left out (order kept)."""
     off = np.asarray(offsets, np.int64)

@@ -215,10 +215,10 @@ _VP = ctypes.c_void_p
 SIGNATURES = {
     "pf_version": ([], ctypes.c_int),
     "pf_last_error": ([], ctypes.c_char_p),
-    "pf_bin_count": ([_VP, _VP, _VP, ctypes.c_int32, _VP, _VP], ctypes.c_int),
+    "pf_bin_count": ([_VP, _VP, _VP, _VP, ctypes.c_int32, _VP, _VP], ctypes.c_int),
     "pf_bin_scan": ([_VP, ctypes.c_int32, _VP, _VP], ctypes.c_int),
-    "pf_bin_emit": ([_VP, _VP, _VP, ctypes.c_int32, _VP, ctypes.c_int32, _VP, _VP, _VP, _VP], ctypes.c_int),
-    "pf_live_sites": ([_VP, _VP, ctypes.c_int32, _VP, _VP, _VP, ctypes.c_int32, _VP, _VP], ctypes.c_int),
+    "pf_bin_emit": ([_VP, _VP, _VP, _VP, ctypes.c_int32, _VP, ctypes.c_int32, _VP, _VP, _VP, _VP], ctypes.c_int),
+    "pf_live_sites": ([_VP, _VP, ctypes.c_int32, _VP, _VP, _VP, ctypes.c_int32, _VP, _VP, _VP], ctypes.c_int),
     "pf_bin_sort_workspace_size": ([ctypes.c_int64, ctypes.c_int32], ctypes.c_size_t),
     "pf_bin_sort": ([_VP, ctypes.c_int32, _VP, _VP, _VP, ctypes.c_size_t, _VP], ctypes.c_int),
     "pf_tile_schedule": ([_VP, ctypes.c_int32, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, _VP, _VP, _VP, _VP, _VP,

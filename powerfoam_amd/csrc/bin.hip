@@ -174,14 +174,23 @@ __global__ void __launch_bounds__(256) fill_i32_kernel(int32_t* p, int64_t n, in
 // per-tile sort removes that order.
 struct WaveSite {
   float uc, vc, r1, r2, vx, vy;
+  Footprint ball;  // footprint of the site's bound (pf_live_sites), when has_ball
   int ti0, tj0, nj, start;
   uint32_t key;
-  int site;
+  int site, has_ball;
 };
 
+// A tile in a footprint: inside its tile rectangle and through the OBB test,
+// as binning decides for a sphere.
+__device__ __forceinline__ bool in_footprint(int ti, int tj, const BinCam& cam, const Footprint& f) {
+  return ti >= f.ti0 && ti <= f.ti1 && tj >= f.tj0 && tj <= f.tj1 && tile_obb_test(ti, tj, cam, f.obb);
+}
+
+// f(tile, site record, in_ball): in_ball is false when the tile misses the
+// site's bound ball (no ray of the tile can be shaded by the site).
 template <class F>
 __device__ __forceinline__ void for_each_overlap(const BinCam& cam, WaveSite* ws, int lane, bool valid,
-                                                 const Footprint& fp, int site, F&& f) {
+                                                 const Footprint& fp, int site, const float4* bounds, F&& f) {
   const int ni = valid ? max(fp.ti1 - fp.ti0 + 1, 0) : 0;
   const int nj = valid ? max(fp.tj1 - fp.tj0 + 1, 0) : 0;
   const int nc = ni * nj;
@@ -206,6 +215,8 @@ __device__ __forceinline__ void for_each_overlap(const BinCam& cam, WaveSite* ws
   me.start = incl - nc;
   me.key = fp.key;
   me.site = site;
+  // a ball that fails the sphere cull does not restrict its site
+  me.has_ball = (valid && bounds && site_footprint(cam, bounds[site], me.ball)) ? 1 : 0;
   wave_sync();
   for (int base = 0; base < total; base += WAVE) {
     const int k = base + lane;
@@ -219,27 +230,29 @@ __device__ __forceinline__ void for_each_overlap(const BinCam& cam, WaveSite* ws
       const int local = k - o.start;
       const int ti = o.ti0 + local / o.nj, tj = o.tj0 + local % o.nj;
       const Obb obb{o.uc, o.vc, o.r1, o.r2, o.vx, o.vy};
-      if (tile_obb_test(ti, tj, cam, obb)) f(ti + tj * cam.th, o);
+      if (tile_obb_test(ti, tj, cam, obb)) f(ti + tj * cam.th, o, !o.has_ball || in_footprint(ti, tj, cam, o.ball));
     }
   }
 }
 
 __global__ void __launch_bounds__(256)
-    bin_count_kernel(BinCam cam, const float4* __restrict__ spheres, const uint8_t* __restrict__ live, int n,
-                     int32_t* __restrict__ counts) {
+    bin_count_kernel(BinCam cam, const float4* __restrict__ spheres, const uint8_t* __restrict__ live,
+                     const float4* __restrict__ bounds, int n, int32_t* __restrict__ counts) {
   __shared__ WaveSite s_ws[256];
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & (WAVE - 1);
   Footprint fp{};
   const bool valid = s < n && (!live || live[s]) && site_footprint(cam, spheres[s], fp);
-  for_each_overlap(cam, s_ws + (threadIdx.x & ~(WAVE - 1)), lane, valid, fp, s,
-                   [&](int t, const WaveSite&) { atomicAdd(&counts[t + 1], 1); });
+  for_each_overlap(cam, s_ws + (threadIdx.x & ~(WAVE - 1)), lane, valid, fp, s, bounds,
+                   [&](int t, const WaveSite&, bool in_ball) {
+                     if (in_ball) atomicAdd(&counts[t + 1], 1);
+                   });
 }
 
 __global__ void __launch_bounds__(256)
-    bin_emit_kernel(BinCam cam, const float4* __restrict__ spheres, const uint8_t* __restrict__ live, int n,
-                    const int64_t* __restrict__ offsets, uint64_t* __restrict__ records, int32_t* __restrict__ cursor,
-                    uint8_t* __restrict__ visible) {
+    bin_emit_kernel(BinCam cam, const float4* __restrict__ spheres, const uint8_t* __restrict__ live,
+                    const float4* __restrict__ bounds, int n, const int64_t* __restrict__ offsets,
+                    uint64_t* __restrict__ records, int32_t* __restrict__ cursor, uint8_t* __restrict__ visible) {
   __shared__ WaveSite s_ws[256];
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & (WAVE - 1);
@@ -247,8 +260,9 @@ __global__ void __launch_bounds__(256)
   // a site left out of the lists still walks its tiles when the visibility
   // mask is asked for: the mask is the footprint's, as without the mask
   const bool valid = s < n && (visible || !live || live[s]) && site_footprint(cam, spheres[s], fp);
-  for_each_overlap(cam, s_ws + (threadIdx.x & ~(WAVE - 1)), lane, valid, fp, s, [&](int t, const WaveSite& o) {
-    if (!live || live[o.site]) {
+  for_each_overlap(cam, s_ws + (threadIdx.x & ~(WAVE - 1)), lane, valid, fp, s, bounds,
+                   [&](int t, const WaveSite& o, bool in_ball) {
+    if (in_ball && (!live || live[o.site])) {
       const int64_t idx = offsets[t] + (int64_t)atomicAdd(&cursor[t], 1);
       if (idx < offsets[t + 1]) records[idx] = ((uint64_t)o.key << 32) | (uint32_t)o.site;
     }
@@ -322,6 +336,69 @@ __global__ void __launch_bounds__(256)
   }
   wave_sync();
   if (lane < ns) live[s0 + lane] = sd[lane] ? 0 : 1;
+}
+
+// A ball enclosing every point a site can shade. It starts as the sphere,
+// inflated by 2e-3 sqrt(r |c - eye|) (four times the movement of a tangent
+// ray's interval ends under FP32 rounding); each face whose plane, moved
+// outward by the plane-rounding margin of live_sites_kernel, leaves the
+// ball's centre outside replaces the ball by the smallest ball enclosing the
+// cap on the cell's side (centre on the plane, radius sqrt(rho^2 - delta^2)).
+// Every point the site shades lies in each cap, so it stays inside; three
+// sweeps over the faces in list order. A cap that vanishes (the ball wholly
+// outside a plane) leaves nothing to shade: the site is marked not live.
+// Binning then keeps a (tile, site) pair only if the tile also meets the
+// ball's footprint. One thread per site (the sweeps are sequential).
+__global__ void __launch_bounds__(256)
+    site_bounds_kernel(const float4* __restrict__ spheres, int n, const int32_t* __restrict__ adj,
+                       const int32_t* __restrict__ off, const uint2* __restrict__ diff, int power, f3 eye,
+                       uint8_t* __restrict__ live, float4* __restrict__ bounds) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const float4 c = spheres[s];
+  const f3 ce{c.x - eye.x, c.y - eye.y, c.z - eye.z};
+  const float dist = __builtin_sqrtf(dot(ce, ce)) + c.w;
+  const float c_len = __builtin_sqrtf(c.x * c.x + c.y * c.y + c.z * c.z);
+  const float eye_len = __builtin_sqrtf(eye.x * eye.x + eye.y * eye.y + eye.z * eye.z);
+  f3 b{c.x, c.y, c.z};
+  float rad = c.w + 2e-3f * __builtin_sqrtf(c.w * dist);
+  bool empty = live[s] == 0;
+  const int e0 = off[s], e1 = off[s + 1];
+  for (int pass = 0; pass < 3 && !empty; pass++) {
+    for (int e = e0; e < e1; e++) {
+      f3 nv;
+      float h;
+      if (diff) {
+        const uint2 raw = diff[e];
+        nv = f3{half_bits_to_float((uint16_t)(raw.x & 0xffff)), half_bits_to_float((uint16_t)(raw.x >> 16)),
+                half_bits_to_float((uint16_t)(raw.y & 0xffff))};
+        const float q = half_bits_to_float((uint16_t)(raw.y >> 16));
+        h = power ? q - (c.x * nv.x + c.y * nv.y + c.z * nv.z) : 0.5f * (dot(nv, nv) + c.w * c.w - q * q);
+      } else {
+        const float4 a = spheres[adj[e]];
+        nv = f3{a.x - c.x, a.y - c.y, a.z - c.z};
+        h = 0.5f * (dot(nv, nv) + c.w * c.w - a.w * a.w);
+      }
+      const float nl = __builtin_sqrtf(dot(nv, nv));
+      if (!(nl > 0.0f)) continue;
+      const float inv = 1.0f / nl;
+      const f3 u{nv.x * inv, nv.y * inv, nv.z * inv};
+      const float m = 1e-5f * (dist + c_len + eye_len + fabsf(h) * inv);
+      const float hd = h * inv + m;
+      const f3 bc{b.x - c.x, b.y - c.y, b.z - c.z};
+      const float delta = hd - dot(bc, u);
+      if (delta <= -rad) {
+        empty = true;
+        break;
+      }
+      if (delta < 0.0f) {
+        b = f3{b.x + u.x * delta, b.y + u.y * delta, b.z + u.z * delta};
+        rad = __builtin_sqrtf(fmaxf(rad * rad - delta * delta, 0.0f));
+      }
+    }
+  }
+  if (empty) live[s] = 0;
+  bounds[s] = make_float4(b.x, b.y, b.z, rad);
 }
 
 // ------------------------------------------------ non-pinhole cameras
@@ -899,19 +976,21 @@ using namespace pf;
 
 extern "C" {
 
-int pf_bin_count(const pf_camera* cam, const float* spheres, const uint8_t* live, int32_t n_sites,
-                 int32_t* tile_counts, void* stream) {
+int pf_bin_count(const pf_camera* cam, const float* spheres, const uint8_t* live, const float* bounds,
+                 int32_t n_sites, int32_t* tile_counts, void* stream) {
   if (!cam || !tile_counts || n_sites < 0 || (n_sites > 0 && !spheres))
     return set_error(PF_ERR_ARG, "pf_bin_count: bad arguments");
   if (cam->width < 2 || cam->height < 2) return set_error(PF_ERR_ARG, "pf_bin_count: image must be >= 2x2");
   if (cam->ray_maps) return set_error(PF_ERR_UNSUPPORTED, "pf_bin_count: pinhole cameras only");
+  if (reinterpret_cast<uintptr_t>(bounds) % 16) return set_error(PF_ERR_ARG, "pf_bin_count: bounds not 16-byte aligned");
   hipStream_t st = as_stream(stream);
   const BinCam bc = make_bincam(*cam);
   const int64_t nt = (int64_t)tiles_h(*cam) * tiles_w(*cam);
   fill_i32_kernel<<<(unsigned)((nt + 1 + 255) / 256), 256, 0, st>>>(tile_counts, nt + 1, 0);
   if (n_sites > 0)
     bin_count_kernel<<<(n_sites + 255) / 256, 256, 0, st>>>(bc, reinterpret_cast<const float4*>(spheres), live,
-                                                           n_sites, tile_counts);
+                                                           reinterpret_cast<const float4*>(bounds), n_sites,
+                                                           tile_counts);
   PF_HIP(hipGetLastError());
   return PF_OK;
 }
@@ -923,36 +1002,45 @@ int pf_bin_scan(const int32_t* tile_counts, int32_t n_tiles, int64_t* offsets, v
   return PF_OK;
 }
 
-int pf_bin_emit(const pf_camera* cam, const float* spheres, const uint8_t* live, int32_t n_sites,
-                const int64_t* offsets, int32_t n_tiles, uint64_t* records, int32_t* cursor, uint8_t* visible,
-                void* stream) {
+int pf_bin_emit(const pf_camera* cam, const float* spheres, const uint8_t* live, const float* bounds,
+                int32_t n_sites, const int64_t* offsets, int32_t n_tiles, uint64_t* records, int32_t* cursor,
+                uint8_t* visible, void* stream) {
   if (!cam || !offsets || !cursor || n_sites < 0 || (n_sites > 0 && !spheres))
     return set_error(PF_ERR_ARG, "pf_bin_emit: bad arguments");
   if (cam->ray_maps) return set_error(PF_ERR_UNSUPPORTED, "pf_bin_emit: pinhole cameras only");
+  if (reinterpret_cast<uintptr_t>(bounds) % 16) return set_error(PF_ERR_ARG, "pf_bin_emit: bounds not 16-byte aligned");
   if (n_tiles != tiles_h(*cam) * tiles_w(*cam)) return set_error(PF_ERR_ARG, "pf_bin_emit: n_tiles mismatch");
   hipStream_t st = as_stream(stream);
   fill_i32_kernel<<<(unsigned)((n_tiles + 255) / 256), 256, 0, st>>>(cursor, n_tiles, 0);
   if (n_sites > 0)
     bin_emit_kernel<<<(n_sites + 255) / 256, 256, 0, st>>>(make_bincam(*cam), reinterpret_cast<const float4*>(spheres),
-                                                          live, n_sites, offsets, records, cursor, visible);
+                                                          live, reinterpret_cast<const float4*>(bounds), n_sites,
+                                                          offsets, records, cursor, visible);
   PF_HIP(hipGetLastError());
   return PF_OK;
 }
 
 int pf_live_sites(const pf_camera* cam, const float* spheres, int32_t n_sites, const int32_t* adjacency,
                   const int32_t* adjacency_offsets, const uint16_t* diff, int32_t diff_kind, uint8_t* live,
-                  void* stream) {
+                  float* bounds, void* stream) {
   if (!cam || n_sites < 0 || (n_sites > 0 && (!spheres || !adjacency_offsets || !live)))
     return set_error(PF_ERR_ARG, "pf_live_sites: bad arguments");
   if (!diff && !adjacency) return set_error(PF_ERR_ARG, "pf_live_sites: needs the adjacency or its encoding");
   if (diff && diff_kind != PF_FACES_RADIUS && diff_kind != PF_FACES_POWER)
     return set_error(PF_ERR_ARG, "pf_live_sites: unknown face encoding");
   if (cam->ray_maps) return set_error(PF_ERR_UNSUPPORTED, "pf_live_sites: pinhole cameras only");
+  if (reinterpret_cast<uintptr_t>(bounds) % 16)
+    return set_error(PF_ERR_ARG, "pf_live_sites: bounds not 16-byte aligned");
   if (n_sites == 0) return PF_OK;
   live_sites_kernel<<<(n_sites + 4 * WAVE - 1) / (4 * WAVE), 256, 0, as_stream(stream)>>>(
       reinterpret_cast<const float4*>(spheres), n_sites, adjacency, adjacency_offsets,
       reinterpret_cast<const uint2*>(diff), diff_kind == PF_FACES_POWER ? 1 : 0,
       f3{cam->eye[0], cam->eye[1], cam->eye[2]}, live);
+  if (bounds)
+    site_bounds_kernel<<<(n_sites + 255) / 256, 256, 0, as_stream(stream)>>>(
+        reinterpret_cast<const float4*>(spheres), n_sites, adjacency, adjacency_offsets,
+        reinterpret_cast<const uint2*>(diff), diff_kind == PF_FACES_POWER ? 1 : 0,
+        f3{cam->eye[0], cam->eye[1], cam->eye[2]}, live, reinterpret_cast<float4*>(bounds));
   PF_HIP(hipGetLastError());
   return PF_OK;
 }

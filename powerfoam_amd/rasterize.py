@@ -111,6 +111,9 @@ class Rasterizer:
         # up to float summation order; the visualisation pass, whose
         # intersection counts follow list positions, always bins every site.
         self.cull_hidden = True
+        # ... and bin each remaining site only to tiles that meet a ball
+        # holding everything it can shade (its sphere cut down by its faces).
+        self.bound_sites = True
         # The forward marks the list entries that shade some ray of their tile
         # and the backward walks only those (an entry that shades nothing adds
         # nothing to any gradient). Not in the reference.
@@ -143,11 +146,12 @@ class Rasterizer:
         rm = rm.detach().to(device=self.device, dtype=torch.float32).contiguous()
         return TorchCamera(camera.eye, camera.right, camera.up, camera.width, camera.height, rm).to_pf(), rm
 
-    def bin(self, camera, all_spheres, live=None):
+    def bin(self, camera, all_spheres, live=None, bounds=None):
         """count -> scan -> (one host sync for M) -> emit -> per-tile sort; ray-map
         cameras count and emit through the cone tree (rasterize.py:1518-1760).
         `live` (uint8 per site, from live_sites) leaves the sites marked 0 out
-        of the lists; the visibility mask still covers them."""
+        of the lists, `bounds` (their balls) the tiles that miss a site's
+        ball; the visibility mask still covers both."""
         lib = self.lib
         cam, keep = self._camera(camera)
         th, tw = _abi.tiles_hw(camera.height, camera.width)
@@ -164,7 +168,7 @@ class Rasterizer:
             check(lib.pf_bin_cones(_ref(cam), ptr(cones), st))
         counts = torch.empty(nt + 1, dtype=torch.int32, device=dev)
         if cones is None:
-            check(lib.pf_bin_count(_ref(cam), ptr(all_spheres), ptr(live), n, ptr(counts), st))
+            check(lib.pf_bin_count(_ref(cam), ptr(all_spheres), ptr(live), ptr(bounds), n, ptr(counts), st))
         else:
             check(lib.pf_bin_count_cones(_ref(cam), ptr(cones), ptr(all_spheres), n, ptr(counts), st))
         offsets = torch.empty(nt + 1, dtype=torch.int64, device=dev)
@@ -176,7 +180,7 @@ class Rasterizer:
         cursor = torch.empty(nt, dtype=torch.int32, device=dev)
         visible = torch.zeros(n, dtype=torch.uint8, device=dev)
         if cones is None:
-            check(lib.pf_bin_emit(_ref(cam), ptr(all_spheres), ptr(live), n, ptr(offsets), nt, ptr(records),
+            check(lib.pf_bin_emit(_ref(cam), ptr(all_spheres), ptr(live), ptr(bounds), n, ptr(offsets), nt, ptr(records),
                                   ptr(cursor), ptr(visible), st))
         else:
             check(lib.pf_bin_emit_cones(_ref(cam), ptr(cones), ptr(all_spheres), n, ptr(offsets), nt, ptr(records),
@@ -238,21 +242,25 @@ class Rasterizer:
                                 order.data_ptr() if order is not None else None)
 
     def live_sites(self, camera, all_spheres, adjacency, adjacency_offsets, diff, kind=_abi.PF_FACES_RADIUS):
-        """uint8 per site, 0 where the site's sphere lies beyond one of its own
-        faces (as the compositing kernels read them: `diff` in the encoding
-        `kind`, or the full-precision spheres when diff is None), so that no
-        ray from the camera shades it (pf_live_sites); None when culling is
-        off, for ray-map cameras, or without faces."""
+        """(live, bounds): uint8 per site, 0 where the site cannot shade any ray
+        from the camera (its sphere lies beyond one of its own faces, or a
+        bound ball's cap vanishes), and with `bound_sites` a [N,4] ball per
+        site holding everything it can shade (pf_live_sites); faces as the
+        compositing kernels read them (`diff` in the encoding `kind`, or the
+        full-precision spheres when diff is None). (None, None) when culling
+        is off, for ray-map cameras, or without faces."""
         if not self.cull_hidden or adjacency.shape[0] == 0:
-            return None
+            return None, None
         cam, ray_maps = self._camera(camera)
         if ray_maps is not None:
-            return None
-        live = torch.empty(all_spheres.shape[0], dtype=torch.uint8, device=all_spheres.device)
-        check(self.lib.pf_live_sites(_ref(cam), ptr(all_spheres), all_spheres.shape[0], ptr(adjacency),
-                                     ptr(adjacency_offsets), ptr(diff) if diff is not None else None, int(kind),
-                                     ptr(live), stream_handle()))
-        return live
+            return None, None
+        n = all_spheres.shape[0]
+        live = torch.empty(n, dtype=torch.uint8, device=all_spheres.device)
+        bounds = torch.empty((n, 4), dtype=torch.float32, device=all_spheres.device) if self.bound_sites else None
+        check(self.lib.pf_live_sites(_ref(cam), ptr(all_spheres), n, ptr(adjacency), ptr(adjacency_offsets),
+                                     ptr(diff) if diff is not None else None, int(kind), ptr(live), ptr(bounds),
+                                     stream_handle()))
+        return live, bounds
 
     def encode_adjacency(self, all_spheres, adjacency, adjacency_offsets):
         """prefetch_adjacency_kernel (rasterize.py:580-606): fp16 (c_adj - c, r_adj)."""
@@ -286,9 +294,9 @@ class Rasterizer:
         dq = _f32c(depth_quantiles) if depth_quantiles is not None else None
 
         adjacency_diff = self.encode_adjacency(all_spheres, adjacency, adjacency_offsets)
-        live = self.live_sites(camera, all_spheres, adjacency, adjacency_offsets,
-                               None if self.exact_geometry else adjacency_diff)
-        offsets, tile_sites, prim_visible_mask, nt = self.bin(camera, all_spheres, live)
+        live, bounds = self.live_sites(camera, all_spheres, adjacency, adjacency_offsets,
+                                       None if self.exact_geometry else adjacency_diff)
+        offsets, tile_sites, prim_visible_mask, nt = self.bin(camera, all_spheres, live, bounds)
         sg = self.segments(offsets, tile_sites.shape[0], self._longest)
         order = self.tile_order(offsets) if sg is None else None
 
@@ -364,9 +372,9 @@ class Rasterizer:
             adjacency = adjacency.to(torch.int32).contiguous()
             adjacency_offsets = adjacency_offsets.to(torch.int32).contiguous()
             adjacency_diff = adjacency_diff.to(torch.float16).contiguous()
-            live = self.live_sites(camera, all_spheres, adjacency, adjacency_offsets, adjacency_diff,
-                                   _abi.PF_FACES_POWER)
-            offsets, tile_sites, _, nt = self.bin(camera, all_spheres, live)
+            live, bounds = self.live_sites(camera, all_spheres, adjacency, adjacency_offsets, adjacency_diff,
+                                           _abi.PF_FACES_POWER)
+            offsets, tile_sites, _, nt = self.bin(camera, all_spheres, live, bounds)
             order = self.tile_order(offsets)
             color_out = torch.zeros((H, W, 3), dtype=torch.float32, device=points.device)
             a = _abi.PfRasterRgbArgs()

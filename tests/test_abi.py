@@ -85,10 +85,10 @@ def test_live_sites_argument_checks_run_on_the_host():
     cam = _abi.PfCamera()
     cam.width, cam.height = 16, 16
     fake = ctypes.c_void_p(0x1000)
-    st = lib.pf_live_sites(ctypes.byref(cam), fake, 4, fake, fake, fake, 5, fake, None)
+    st = lib.pf_live_sites(ctypes.byref(cam), fake, 4, fake, fake, fake, 5, fake, None, None)
     assert st == _abi.PF_ERR_ARG and b"encoding" in lib.pf_last_error()
-    st = lib.pf_live_sites(ctypes.byref(cam), fake, 4, None, fake, None, 0, fake, None)
+    st = lib.pf_live_sites(ctypes.byref(cam), fake, 4, None, fake, None, 0, fake, None, None)
     assert st == _abi.PF_ERR_ARG
     cam.ray_maps = 0x2000
-    st = lib.pf_live_sites(ctypes.byref(cam), fake, 4, fake, fake, None, 0, fake, None)
+    st = lib.pf_live_sites(ctypes.byref(cam), fake, 4, fake, fake, None, 0, fake, None, None)
     assert st == _abi.PF_ERR_UNSUPPORTED

@@ -130,3 +130,82 @@ def test_filter_lists_keeps_order():
     o2, s2 = orc.filter_lists(off, sites, live)
     assert o2.tolist() == [0, 2, 2, 3]
     assert s2.tolist() == [5, 2, 5]
+
+
+def _bounded(case, inp):
+    eye = case.cam.eye.numpy()
+    live = orc.live_sites(eye, case.spheres(), case.adjacency, case.adjacency_offsets, inp.adjacency_diff)
+    bounds, empty = orc.site_bounds(eye, case.spheres(), case.adjacency, case.adjacency_offsets,
+                                    inp.adjacency_diff, live=live)
+    return (~empty).astype(np.uint8), bounds
+
+
+@pytest.mark.parametrize("exact_geometry", [False, True])
+def test_oracle_outputs_unchanged_with_bound_balls(exact_geometry):
+    """Lists restricted to the tiles that meet each site's ball (and without
+    the sites whose ball vanished): the oracle's compositing forward and
+    backward are unchanged, per pixel bit for bit."""
+    case = _heavy(seed=4)
+    inp = oracle_inputs(case, exact_geometry)
+    live, bounds = _bounded(case, inp)
+    cam, full, _ = _lists(case, live)
+    c = case.cam
+    restricted = orc.restrict_lists(cam, c.eye.numpy(), c.right.numpy(), c.up.numpy(), *full, bounds, live)
+    assert restricted[1].size < 0.6 * full[1].size
+    g = random_grads(case, seed=2)
+    res = []
+    for off, sites in (full, restricted):
+        out = orc.raster_fwd(cam, inp, off, sites, depth_quantiles=case.depth_quantiles, ray_gt=case.gt,
+                             return_point_err=True, precision="f32")
+        glt = -g["opacity"].astype(np.float64) * np.exp(out["log_t"])
+        gr = orc.raster_bwd(cam, inp, off, sites, out["tile_early_stop"], out["log_t"], g["color"], glt,
+                            g["normal_distance"], g["normal"], g["contrib"], depth_quantiles=case.depth_quantiles,
+                            grad_quantile_depths=g["quantile_depths"], precision="f32")
+        res.append((out, gr))
+    (a, ga), (b, gb) = res
+    for k in ("color", "log_t", "normal_distance", "normal", "quantile_depths"):
+        assert np.array_equal(a[k], b[k]), k
+    for x, y, k in [(a[k], b[k], k) for k in ("contrib", "point_err")] + [(ga[k], gb[k], k) for k in GRAD_KEYS]:
+        np.testing.assert_allclose(y, x, rtol=4e-6, atol=4e-6 * float(np.abs(x).max()), err_msg=k)
+
+
+@pytest.mark.parametrize("form", ["exact", "radius"])
+def test_bound_balls_hold_the_cells(form):
+    """float64 brute force: points sampled uniformly in each sphere that lie on
+    the cell side of every face (as the kernels read the faces) are inside
+    the site's ball; a site whose ball vanished has no such point; balls of
+    live sites never exceed the inflated sphere."""
+    case = _heavy(seed=5)
+    sp32 = case.spheres()
+    sp = sp32.astype(np.float64)
+    adj, offs = case.adjacency, case.adjacency_offsets
+    diff = None if form == "exact" else orc.adj_encode(sp32, adj, offs)
+    eye = case.cam.eye.numpy()
+    live = orc.live_sites(eye, sp32, adj, offs, diff)
+    bounds, empty = orc.site_bounds(eye, sp32, adj, offs, diff, live=live)
+    assert (empty & (live == 1)).sum() > 0          # the sweeps find more empty cells
+    shrunk = (~empty) & (bounds[:, 3] < 0.9 * sp32[:, 3])
+    assert shrunk.sum() > 50
+    rng = np.random.default_rng(1)
+    checked = 0
+    for i in np.nonzero(live)[0][:600]:
+        a0, a1 = offs[i], offs[i + 1]
+        if diff is None:
+            nv = sp[adj[a0:a1], :3] - sp[i, :3]
+            h = 0.5 * ((nv * nv).sum(1) + sp[i, 3] ** 2 - sp[adj[a0:a1], 3] ** 2)
+        else:
+            d = diff[a0:a1].view(np.float16).astype(np.float64)
+            nv = d[:, :3]
+            h = 0.5 * ((nv * nv).sum(1) + sp[i, 3] ** 2 - d[:, 3] ** 2)
+        u = rng.normal(size=(4000, 3))
+        u /= np.linalg.norm(u, axis=1, keepdims=True)
+        y = u * sp[i, 3] * rng.uniform(0, 1, (4000, 1)) ** (1 / 3)
+        inside = np.all(y @ nv.T <= h, axis=1) if nv.size else np.ones(4000, bool)
+        if empty[i]:
+            assert not inside.any(), i
+            continue
+        p = y[inside] + sp[i, :3]
+        dist = np.linalg.norm(p - bounds[i, :3].astype(np.float64), axis=1)
+        assert np.all(dist <= bounds[i, 3] * (1 + 1e-6)), i
+        checked += int(inside.any())
+    assert checked > 100

@@ -26,7 +26,7 @@ def _scenes():
             ("gmm30k", sp, adj, off, synth.circle_cameras(n_views=32, width=331, img_h=217)[3])]
 
 
-def _gpu_live(sp, adj, off, cam_t, form):
+def _gpu_live(sp, adj, off, cam_t, form, with_bounds=False):
     import torch
     from powerfoam_amd import _abi
     from powerfoam_amd.camera import TorchCamera
@@ -45,11 +45,14 @@ def _gpu_live(sp, adj, off, cam_t, form):
         diff = None
     cam = TorchCamera(cam_t.eye, cam_t.right, cam_t.up, cam_t.width, cam_t.height).to_pf()
     live = torch.full((sp.shape[0],), 7, dtype=torch.uint8, device="cuda")
+    bounds = torch.full((sp.shape[0], 4), float("nan"), device="cuda") if with_bounds else None
     kind = _abi.PF_FACES_POWER if form == "power" else _abi.PF_FACES_RADIUS
     _abi.check(lib.pf_live_sites(ctypes.byref(cam), _abi.ptr(d_sp), sp.shape[0], _abi.ptr(d_adj), _abi.ptr(d_off),
-                                 _abi.ptr(diff), kind, _abi.ptr(live), None))
+                                 _abi.ptr(diff), kind, _abi.ptr(live), _abi.ptr(bounds), None))
     torch.cuda.synchronize()
     dh = None if diff is None else diff.cpu().numpy().view(np.uint16)[: len(adj)]
+    if with_bounds:
+        return live.cpu().numpy(), dh, bounds.cpu().numpy()
     return live.cpu().numpy(), dh
 
 
@@ -82,8 +85,9 @@ def test_binning_with_live_mask_bitexact():
         d_off = torch.tensor(np.asarray(off, np.int32), device="cuda")
         cam = TorchCamera(cam_t.eye, cam_t.right, cam_t.up, cam_t.width, cam_t.height)
         diff = r.encode_adjacency(d_sp, d_adj, d_off)
-        live = r.live_sites(cam, d_sp, d_adj, d_off, diff)
-        assert live is not None
+        r.bound_sites = False
+        live, bounds = r.live_sites(cam, d_sp, d_adj, d_off, diff)
+        assert live is not None and bounds is None
         offsets, sites, visible, nt = r.bin(cam, d_sp, live)
         live_h = live.cpu().numpy()
         cam_o, _ = orc.camera(cam_t.eye.numpy(), cam_t.right.numpy(), cam_t.up.numpy(), cam_t.width, cam_t.height)
@@ -93,6 +97,55 @@ def test_binning_with_live_mask_bitexact():
         assert np.array_equal(sites.cpu().numpy(), sites_f)
         assert sites_f.size < 0.9 * sites_o.size, name
         # the visibility mask is the footprint's, culled sites included
+        vis_o = np.zeros(sp.shape[0], bool)
+        vis_o[sites_o] = True
+        assert np.array_equal(visible.cpu().numpy(), vis_o)
+
+
+@pytest.mark.parametrize("form", ["exact", "radius", "power"])
+def test_site_bounds_match_oracle(form):
+    """pf_live_sites with balls: the mask (sites whose ball vanished included)
+    equals the oracle's up to faces within rounding of a margin, and each
+    live site's ball equals the oracle's to float rounding."""
+    for name, sp, adj, off, cam in _scenes():
+        print(f"site bounds {name} {form}", flush=True)
+        live_g, diff, b_g = _gpu_live(sp, adj, off, cam, form, with_bounds=True)
+        live0 = orc.live_sites(cam.eye.numpy(), sp, adj, off, diff, kind=int(form == "power"))
+        b_o, empty = orc.site_bounds(cam.eye.numpy(), sp, adj, off, diff, kind=int(form == "power"), live=live0)
+        live_o = (~empty).astype(np.uint8)
+        assert (live_o == 0).mean() > 0.15, name
+        assert np.count_nonzero(live_g != live_o) <= max(2, sp.shape[0] // 5000), name
+        both = (live_g == 1) & (live_o == 1)
+        scale = sp[both, 3] + 1e-6 * (1 + np.abs(sp[both, :3]).max(1))
+        err = np.abs(b_g[both] - b_o[both]).max(1) / scale
+        assert np.quantile(err, 0.999) < 1e-4 and err.max() < 1e-2, (name, err.max())
+
+
+def test_binning_with_bounds_bitexact():
+    """Lists binned with the mask and the balls equal the oracle's lists
+    restricted by the same mask and balls; the visibility mask is the
+    sphere footprint's, as without them."""
+    import torch
+    from types import SimpleNamespace
+    from powerfoam_amd.camera import TorchCamera
+    from powerfoam_amd.rasterize import Rasterizer
+    for name, sp, adj, off, cam_t in _scenes():
+        print(f"binning with bounds {name}", flush=True)
+        r = Rasterizer(SimpleNamespace(num_texel_sites=8, render_objective="volume", is_pinhole=True), "cuda")
+        d_sp = torch.tensor(sp, device="cuda")
+        d_adj = torch.tensor(np.asarray(adj, np.int32), device="cuda")
+        d_off = torch.tensor(np.asarray(off, np.int32), device="cuda")
+        cam = TorchCamera(cam_t.eye, cam_t.right, cam_t.up, cam_t.width, cam_t.height)
+        diff = r.encode_adjacency(d_sp, d_adj, d_off)
+        live, bounds = r.live_sites(cam, d_sp, d_adj, d_off, diff)
+        assert bounds is not None
+        offsets, sites, visible, nt = r.bin(cam, d_sp, live, bounds)
+        cam_o, _ = orc.camera(cam_t.eye.numpy(), cam_t.right.numpy(), cam_t.up.numpy(), cam_t.width, cam_t.height)
+        off_o, sites_o, _ = orc.bin_lists(cam_o, sp)
+        off_r, sites_r = orc.restrict_lists(cam_o, cam_t.eye.numpy(), cam_t.right.numpy(), cam_t.up.numpy(),
+                                            off_o, sites_o, bounds.cpu().numpy(), live.cpu().numpy())
+        assert np.array_equal(offsets.cpu().numpy(), off_r)
+        assert np.array_equal(sites.cpu().numpy(), sites_r)
         vis_o = np.zeros(sp.shape[0], bool)
         vis_o[sites_o] = True
         assert np.array_equal(visible.cpu().numpy(), vis_o)
