@@ -190,7 +190,8 @@ __device__ __forceinline__ bool in_footprint(int ti, int tj, const BinCam& cam, 
 // site's bound ball (no ray of the tile can be shaded by the site).
 template <class F>
 __device__ __forceinline__ void for_each_overlap(const BinCam& cam, WaveSite* ws, int lane, bool valid,
-                                                 const Footprint& fp, int site, const float4* bounds, F&& f) {
+                                                 const Footprint& fp, int site, const uint8_t* live,
+                                                 const float4* bounds, F&& f) {
   const int ni = valid ? max(fp.ti1 - fp.ti0 + 1, 0) : 0;
   const int nj = valid ? max(fp.tj1 - fp.tj0 + 1, 0) : 0;
   const int nc = ni * nj;
@@ -216,7 +217,7 @@ __device__ __forceinline__ void for_each_overlap(const BinCam& cam, WaveSite* ws
   me.key = fp.key;
   me.site = site;
   // a ball that fails the sphere cull does not restrict its site
-  me.has_ball = (valid && bounds && site_footprint(cam, bounds[site], me.ball)) ? 1 : 0;
+  me.has_ball = (valid && bounds && (!live || live[site]) && site_footprint(cam, bounds[site], me.ball)) ? 1 : 0;
   wave_sync();
   for (int base = 0; base < total; base += WAVE) {
     const int k = base + lane;
@@ -243,7 +244,7 @@ __global__ void __launch_bounds__(256)
   const int lane = threadIdx.x & (WAVE - 1);
   Footprint fp{};
   const bool valid = s < n && (!live || live[s]) && site_footprint(cam, spheres[s], fp);
-  for_each_overlap(cam, s_ws + (threadIdx.x & ~(WAVE - 1)), lane, valid, fp, s, bounds,
+  for_each_overlap(cam, s_ws + (threadIdx.x & ~(WAVE - 1)), lane, valid, fp, s, live, bounds,
                    [&](int t, const WaveSite&, bool in_ball) {
                      if (in_ball) atomicAdd(&counts[t + 1], 1);
                    });
@@ -260,7 +261,7 @@ __global__ void __launch_bounds__(256)
   // a site left out of the lists still walks its tiles when the visibility
   // mask is asked for: the mask is the footprint's, as without the mask
   const bool valid = s < n && (visible || !live || live[s]) && site_footprint(cam, spheres[s], fp);
-  for_each_overlap(cam, s_ws + (threadIdx.x & ~(WAVE - 1)), lane, valid, fp, s, bounds,
+  for_each_overlap(cam, s_ws + (threadIdx.x & ~(WAVE - 1)), lane, valid, fp, s, live, bounds,
                    [&](int t, const WaveSite& o, bool in_ball) {
     if (in_ball && (!live || live[o.site])) {
       const int64_t idx = offsets[t] + (int64_t)atomicAdd(&cursor[t], 1);
@@ -348,13 +349,15 @@ __global__ void __launch_bounds__(256)
 // sweeps over the faces in list order. A cap that vanishes (the ball wholly
 // outside a plane) leaves nothing to shade: the site is marked not live.
 // Binning then keeps a (tile, site) pair only if the tile also meets the
-// ball's footprint. One thread per site (the sweeps are sequential).
-__global__ void __launch_bounds__(256)
-    site_bounds_kernel(const float4* __restrict__ spheres, int n, const int32_t* __restrict__ adj,
-                       const int32_t* __restrict__ off, const uint2* __restrict__ diff, int power, f3 eye,
-                       uint8_t* __restrict__ live, float4* __restrict__ bounds) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
+// ball's footprint. The sweeps are sequential, so a lane takes a site; a
+// wave first gathers the live sites among its 256 (most sites are not) so
+// that its lanes stay busy. A site that is not live keeps its sphere.
+constexpr int BOUNDS_SPAN = 4 * WAVE;  // sites per wave
+
+__device__ __forceinline__ void site_bound(const float4* __restrict__ spheres, int s,
+                                           const int32_t* __restrict__ adj, const int32_t* __restrict__ off,
+                                           const uint2* __restrict__ diff, int power, f3 eye,
+                                           uint8_t* __restrict__ live, float4* __restrict__ bounds) {
   const float4 c = spheres[s];
   const f3 ce{c.x - eye.x, c.y - eye.y, c.z - eye.z};
   const float dist = __builtin_sqrtf(dot(ce, ce)) + c.w;
@@ -362,43 +365,87 @@ __global__ void __launch_bounds__(256)
   const float eye_len = __builtin_sqrtf(eye.x * eye.x + eye.y * eye.y + eye.z * eye.z);
   f3 b{c.x, c.y, c.z};
   float rad = c.w + 2e-3f * __builtin_sqrtf(c.w * dist);
-  bool empty = live[s] == 0;
+  bool empty = false;
   const int e0 = off[s], e1 = off[s + 1];
+  // faces are read four at a time (the sweeps are a chain of dependent
+  // steps; one load per step would leave every step waiting on memory)
   for (int pass = 0; pass < 3 && !empty; pass++) {
-    for (int e = e0; e < e1; e++) {
-      f3 nv;
-      float h;
-      if (diff) {
-        const uint2 raw = diff[e];
-        nv = f3{half_bits_to_float((uint16_t)(raw.x & 0xffff)), half_bits_to_float((uint16_t)(raw.x >> 16)),
-                half_bits_to_float((uint16_t)(raw.y & 0xffff))};
-        const float q = half_bits_to_float((uint16_t)(raw.y >> 16));
-        h = power ? q - (c.x * nv.x + c.y * nv.y + c.z * nv.z) : 0.5f * (dot(nv, nv) + c.w * c.w - q * q);
-      } else {
-        const float4 a = spheres[adj[e]];
-        nv = f3{a.x - c.x, a.y - c.y, a.z - c.z};
-        h = 0.5f * (dot(nv, nv) + c.w * c.w - a.w * a.w);
+    for (int e = e0; e < e1 && !empty; e += 4) {
+      float4 raw[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        raw[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (e + j < e1) {
+          if (diff) {
+            const uint2 r2 = diff[e + j];
+            raw[j] = make_float4(__builtin_bit_cast(float, r2.x), __builtin_bit_cast(float, r2.y), 0.0f, 0.0f);
+          } else {
+            raw[j] = spheres[adj[e + j]];
+          }
+        }
       }
-      const float nl = __builtin_sqrtf(dot(nv, nv));
-      if (!(nl > 0.0f)) continue;
-      const float inv = 1.0f / nl;
-      const f3 u{nv.x * inv, nv.y * inv, nv.z * inv};
-      const float m = 1e-5f * (dist + c_len + eye_len + fabsf(h) * inv);
-      const float hd = h * inv + m;
-      const f3 bc{b.x - c.x, b.y - c.y, b.z - c.z};
-      const float delta = hd - dot(bc, u);
-      if (delta <= -rad) {
-        empty = true;
-        break;
-      }
-      if (delta < 0.0f) {
-        b = f3{b.x + u.x * delta, b.y + u.y * delta, b.z + u.z * delta};
-        rad = __builtin_sqrtf(fmaxf(rad * rad - delta * delta, 0.0f));
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        if (e + j >= e1 || empty) break;
+        f3 nv;
+        float h;
+        if (diff) {
+          const uint32_t bx = __builtin_bit_cast(uint32_t, raw[j].x), by = __builtin_bit_cast(uint32_t, raw[j].y);
+          nv = f3{half_bits_to_float((uint16_t)(bx & 0xffff)), half_bits_to_float((uint16_t)(bx >> 16)),
+                  half_bits_to_float((uint16_t)(by & 0xffff))};
+          const float q = half_bits_to_float((uint16_t)(by >> 16));
+          h = power ? q - (c.x * nv.x + c.y * nv.y + c.z * nv.z) : 0.5f * (dot(nv, nv) + c.w * c.w - q * q);
+        } else {
+          const float4 a = raw[j];
+          nv = f3{a.x - c.x, a.y - c.y, a.z - c.z};
+          h = 0.5f * (dot(nv, nv) + c.w * c.w - a.w * a.w);
+        }
+        const float n2 = dot(nv, nv);
+        if (!(n2 > 0.0f)) continue;
+        // hardware 1/sqrt (about 1 ulp): the ball's margins are orders of
+        // magnitude wider than that rounding
+        const float inv = __builtin_amdgcn_rsqf(n2);
+        const f3 u{nv.x * inv, nv.y * inv, nv.z * inv};
+        const float m = 1e-5f * (dist + c_len + eye_len + fabsf(h) * inv);
+        const float hd = h * inv + m;
+        const f3 bc{b.x - c.x, b.y - c.y, b.z - c.z};
+        const float delta = hd - dot(bc, u);
+        if (delta <= -rad) {
+          empty = true;
+        } else if (delta < 0.0f) {
+          b = f3{b.x + u.x * delta, b.y + u.y * delta, b.z + u.z * delta};
+          rad = __builtin_sqrtf(fmaxf(rad * rad - delta * delta, 0.0f));
+        }
       }
     }
   }
   if (empty) live[s] = 0;
   bounds[s] = make_float4(b.x, b.y, b.z, rad);
+}
+
+__global__ void __launch_bounds__(256)
+    site_bounds_kernel(const float4* __restrict__ spheres, int n, const int32_t* __restrict__ adj,
+                       const int32_t* __restrict__ off, const uint2* __restrict__ diff, int power, f3 eye,
+                       uint8_t* __restrict__ live, float4* __restrict__ bounds) {
+  __shared__ int s_idx[4][BOUNDS_SPAN];
+  const int wv = threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
+  const int s0 = (blockIdx.x * 4 + wv) * BOUNDS_SPAN;
+  if (s0 >= n) return;
+  int* idx = s_idx[wv];
+  int cnt = 0;
+  for (int g = 0; g < BOUNDS_SPAN; g += WAVE) {
+    const int s = s0 + g + lane;
+    bool is_live = false;
+    if (s < n) {
+      is_live = live[s] != 0;
+      if (!is_live) bounds[s] = spheres[s];
+    }
+    const uint64_t m = __ballot(is_live);
+    if (is_live) idx[cnt + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))] = s;
+    cnt += __popcll(m);
+  }
+  wave_sync();
+  for (int i = lane; i < cnt; i += WAVE) site_bound(spheres, idx[i], adj, off, diff, power, eye, live, bounds);
 }
 
 // ------------------------------------------------ non-pinhole cameras
@@ -1037,7 +1084,7 @@ int pf_live_sites(const pf_camera* cam, const float* spheres, int32_t n_sites, c
       reinterpret_cast<const uint2*>(diff), diff_kind == PF_FACES_POWER ? 1 : 0,
       f3{cam->eye[0], cam->eye[1], cam->eye[2]}, live);
   if (bounds)
-    site_bounds_kernel<<<(n_sites + 255) / 256, 256, 0, as_stream(stream)>>>(
+    site_bounds_kernel<<<(n_sites + 4 * BOUNDS_SPAN - 1) / (4 * BOUNDS_SPAN), 256, 0, as_stream(stream)>>>(
         reinterpret_cast<const float4*>(spheres), n_sites, adjacency, adjacency_offsets,
         reinterpret_cast<const uint2*>(diff), diff_kind == PF_FACES_POWER ? 1 : 0,
         f3{cam->eye[0], cam->eye[1], cam->eye[2]}, live, reinterpret_cast<float4*>(bounds));
