@@ -144,18 +144,24 @@ __global__ void __launch_bounds__(SH_SITES)
     sh_bwd_kernel(const float* __restrict__ sh, const float* __restrict__ texel_sites,
                   const float* __restrict__ grad_rgb, int n, int T, f3 eye, float* __restrict__ grad_sh, bool vec,
                   bool vec_sh) {
-  __shared__ float s_ts[SH_SITES * SH_ROW];
-  __shared__ float s_gr[SH_SITES * SH_ROW];
+  // texel positions and colour gradients in, then (after a barrier) the
+  // sites' 48 coefficient gradients out through the same LDS (rows of 49
+  // words), so both directions move whole lines
+  static_assert(2 * SH_ROW >= 49, "gradient rows fit the staging space");
+  __shared__ float s_mem[2 * SH_SITES * SH_ROW];
+  float* s_ts = s_mem;
+  float* s_gr = s_mem + SH_SITES * SH_ROW;
   const int s0 = blockIdx.x * SH_SITES, nb = min(SH_SITES, n - s0);
   block_in(s_ts, texel_sites, s0, nb, 3 * T, vec);
   block_in(s_gr, grad_rgb, s0, nb, 3 * T, vec);
   __syncthreads();
+  float g[48];
+#pragma unroll
+  for (int k = 0; k < 48; k++) g[k] = 0.0f;
   if ((int)threadIdx.x < nb) {
     const int s = s0 + threadIdx.x;
-    float c[48], g[48];
+    float c[48];
     load_sh(sh, s, c, vec_sh);
-#pragma unroll
-    for (int k = 0; k < 48; k++) g[k] = 0.0f;
     const float* row = s_ts + threadIdx.x * (3 * T + 1);
     const float* gi = s_gr + threadIdx.x * (3 * T + 1);
     for (int t = 0; t < T; t++) {
@@ -180,15 +186,14 @@ __global__ void __launch_bounds__(SH_SITES)
         g[3 * (k + 1) + 2] += y[k] * gb;
       }
     }
-    if (vec_sh) {
-      float4* dst = reinterpret_cast<float4*>(grad_sh + (size_t)s * 48);
-#pragma unroll
-      for (int i = 0; i < 12; i++) dst[i] = make_float4(g[4 * i], g[4 * i + 1], g[4 * i + 2], g[4 * i + 3]);
-    } else {
-#pragma unroll
-      for (int i = 0; i < 48; i++) grad_sh[(size_t)s * 48 + i] = g[i];
-    }
   }
+  __syncthreads();  // the texel rows are read
+  if ((int)threadIdx.x < nb) {
+#pragma unroll
+    for (int k = 0; k < 48; k++) s_mem[threadIdx.x * 49 + k] = g[k];
+  }
+  __syncthreads();
+  block_out(grad_sh, s_mem, s0, nb, 48, vec_sh);
 }
 
 }  // namespace pf
