@@ -149,3 +149,47 @@ def test_binning_with_bounds_bitexact():
         vis_o = np.zeros(sp.shape[0], bool)
         vis_o[sites_o] = True
         assert np.array_equal(visible.cpu().numpy(), vis_o)
+
+
+def test_rasterizer_with_and_without_culling_on_power_scene():
+    """On the at-scale scene's geometry (GMM sites, power adjacency with hidden
+    sites, 331x217 view): the rasterizer with culled sites, ball binning and
+    backward entry skipping gives the outputs and gradients of the one that
+    bins and walks everything (up to float summation order), the same
+    visibility mask, and shorter lists."""
+    import torch
+    from types import SimpleNamespace
+    from powerfoam_amd.camera import TorchCamera
+    from powerfoam_amd.rasterize import Rasterizer
+    name, sp, adj, off, cam_t = _scenes()[1]
+    s = synth.scene_gmm(1, n=30_000)
+    rng = np.random.default_rng(0)
+    dev = torch.device("cuda")
+    tt = lambda a, dt=torch.float32: torch.tensor(np.ascontiguousarray(a), dtype=dt, device=dev)
+    rgb = rng.uniform(0, 1, (sp.shape[0], 8, 3)).astype(np.float32)
+    cam = TorchCamera(cam_t.eye, cam_t.right, cam_t.up, cam_t.width, cam_t.height)
+    gt = tt(rng.uniform(0, 1, (cam_t.height, cam_t.width, 3)))
+    res = []
+    for fast in (False, True):
+        r = Rasterizer(SimpleNamespace(num_texel_sites=8, render_objective="volume", is_pinhole=True), dev)
+        r.cull_hidden = r.bound_sites = r.skip_unshaded = fast
+        params = [tt(s["points"]).requires_grad_(), tt(s["radii"]).requires_grad_(),
+                  tt(s["density"]).requires_grad_(), tt(s["normals"]).requires_grad_(),
+                  tt(s["texel_sites"]).requires_grad_(), tt(rgb).requires_grad_(),
+                  tt(s["texel_height"]).requires_grad_()]
+        out = r.forward(cam, None, *params, tt(adj, torch.int32), tt(off, torch.int32), gt, True)
+        loss = ((out[0] - gt) ** 2).mean() + out[1].mean() + out[3].abs().mean()
+        loss.backward()
+        torch.cuda.synchronize()
+        res.append(([o.detach().cpu().numpy() if o is not None else None for o in out],
+                    [q.grad.cpu().numpy() for q in params], int(r.last_tiles[1].numel())))
+    (o0, g0, m0), (o1, g1, m1) = res
+    print(f"culling on {name}: {m0} -> {m1} list entries", flush=True)
+    assert m1 < 0.9 * m0
+    for k in (0, 1, 2, 3):  # colour, opacity, normal distance, normal
+        assert np.abs(o0[k] - o1[k]).max() < 1e-5, k
+    for k in (6, 7):  # contrib, point error (sums over tiles)
+        assert np.abs(o0[k] - o1[k]).max() <= 1e-5 * max(1.0, np.abs(o0[k]).max()), k
+    assert np.array_equal(o0[8], o1[8])  # prim_visible_mask
+    for a, b in zip(g0, g1):
+        assert np.linalg.norm(a - b) <= 1e-5 * max(np.linalg.norm(a), 1e-30)
