@@ -360,7 +360,22 @@ def cpu_baseline(w, width, height):
     dt = time.time() - t0
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return dict(value=width * height / dt / 1e6, unit="Mrays/s", cores=cores, kind="port",
-                sample=f"1 frame (view 0) of the workload: binning + fwd + bwd, FP32 oracle, {dt:.1f}s")
+                sample=f"1 frame (view 0) of the workload: binning + fwd + bwd, FP32 oracle, {dt:.1f}s",
+                host=host_cpu())
+
+
+def host_cpu():
+    """The measuring machine's CPU model and its online core count (nproc)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return f"{model}, nproc {len(os.sched_getaffinity(0))}, os.cpu_count {os.cpu_count()}"
 
 
 def main():
