@@ -49,10 +49,7 @@ extern "C" int pf_adj_encode(const float* spheres, int32_t n_sites, const int32_
                              const int32_t* adjacency_offsets, uint16_t* diff, void* stream) {
   if (n_sites < 0 || (n_sites > 0 && (!spheres || !adjacency_offsets)))
     return set_error(PF_ERR_ARG, "pf_adj_encode: bad arguments");
-  if (n_sites == 0) return PF_OK;
-  if (!adjacency || !diff) {
-    // an empty adjacency is allowed; the kernel never dereferences then
-  }
+  if (n_sites == 0) return PF_OK;  // an empty adjacency is allowed: no site then reads it
   adj_encode_kernel<<<(n_sites + 4 * WAVE - 1) / (4 * WAVE), 256, 0, as_stream(stream)>>>(
       reinterpret_cast<const float4*>(spheres), n_sites, adjacency, adjacency_offsets, reinterpret_cast<uint2*>(diff));
   PF_HIP(hipGetLastError());

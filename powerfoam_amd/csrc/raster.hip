@@ -1401,11 +1401,13 @@ int pf_raster_fwd(const pf_raster_fwd_args* a, void* stream) {
   const int nt = tiles_h(c) * tiles_w(c);
   if (a->tiles.n_tiles != nt || !a->tiles.offsets)
     return set_error(PF_ERR_ARG, "pf_raster_fwd: tile lists do not match the camera (%d vs %d)", a->tiles.n_tiles, nt);
-  if (!a->tile_early_stop || !a->color || !a->log_t || !a->normal_distance || !a->normal || !a->contrib)
+  // per-site arrays may be null for a scene without sites (no list entry reads them)
+  const bool sites = a->scene.n_sites > 0;
+  if (!a->tile_early_stop || !a->color || !a->log_t || !a->normal_distance || !a->normal || (sites && !a->contrib))
     return set_error(PF_ERR_ARG, "pf_raster_fwd: missing outputs");
   if ((a->return_point_err || a->loss_accumulation) && !a->ray_gt)
     return set_error(PF_ERR_ARG, "pf_raster_fwd: ray_gt is required for point error / surface loss");
-  if (a->return_point_err && !a->point_err) return set_error(PF_ERR_ARG, "pf_raster_fwd: missing point_err");
+  if (a->return_point_err && sites && !a->point_err) return set_error(PF_ERR_ARG, "pf_raster_fwd: missing point_err");
   if (a->depth_quantiles && (a->n_quantiles < 1 || !a->quantile_depths))
     return set_error(PF_ERR_ARG, "pf_raster_fwd: quantiles need n_quantiles >= 1 and an output");
   FwdParams p{};
@@ -1486,10 +1488,13 @@ int pf_raster_bwd(const pf_raster_bwd_args* a, void* stream) {
   const int nt = tiles_h(c) * tiles_w(c);
   if (a->tiles.n_tiles != nt || !a->tiles.offsets || !a->tile_early_stop)
     return set_error(PF_ERR_ARG, "pf_raster_bwd: tile lists do not match the camera");
+  // per-site arrays may be null for a scene without sites (no list entry reads them)
+  const bool sites = a->scene.n_sites > 0;
   if (!a->log_t || !a->grad_color || !a->grad_log_t || !a->grad_normal_distance || !a->grad_normal ||
-      !a->grad_contrib)
+      (sites && !a->grad_contrib))
     return set_error(PF_ERR_ARG, "pf_raster_bwd: missing incoming gradients");
-  if (!a->grad_spheres || !a->grad_nsigmas || !a->grad_texel_sites || !a->grad_texel_rgb || !a->grad_texel_height)
+  if (sites && (!a->grad_spheres || !a->grad_nsigmas || !a->grad_texel_sites || !a->grad_texel_rgb ||
+                !a->grad_texel_height))
     return set_error(PF_ERR_ARG, "pf_raster_bwd: missing gradient outputs");
   if (a->depth_quantiles && (a->n_quantiles < 1 || !a->grad_quantile_depths))
     return set_error(PF_ERR_ARG, "pf_raster_bwd: quantiles need their gradient");
