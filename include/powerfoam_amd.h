@@ -435,6 +435,15 @@ int pf_mlp_fwd(const pf_mlp_weights* w, const pf_mlp_rows* rows, float* rgb, voi
 int pf_mlp_bwd(const pf_mlp_weights* w, const pf_mlp_rows* rows, const float* grad_rgb, pf_mlp_grads* g,
                void* stream);
 
+/* The rasterizer's per-site inputs from the separate parameter arrays, as the
+ * concatenations of rasterize.py:1619-1624: spheres [n,4] = (points [n,3],
+ * radii [n]), nsigmas [n,4] = (normals [n,3], density [n]), texel_rgbh
+ * [n,n_texels,4] = (texel_rgb [n,n_texels,3], texel_height [n,n_texels]).
+ * Contiguous FP32 inputs; outputs 16-byte aligned. */
+int pf_pack_sites(const float* points, const float* radii, const float* normals, const float* density,
+                  const float* texel_rgb, const float* texel_height, int32_t n_sites, int32_t n_texels,
+                  float* spheres, float* nsigmas, float* texel_rgbh, void* stream);
+
 /* Replaces non-finite entries with 0 (rasterize.py:1954-1958). */
 int pf_scrub_nonfinite(float* data, int64_t n, void* stream);
 

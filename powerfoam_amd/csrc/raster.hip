@@ -1377,6 +1377,23 @@ __global__ void __launch_bounds__(256) scrub4_kernel(float4* x, int64_t n4) {
   }
 }
 
+// The per-site words the compositing kernels read, from the separate
+// parameter arrays: sphere (x, y, z, r), normal and density (nx, ny, nz, σ)
+// and per texel (r, g, b, height). One thread per texel row; the first
+// n_sites threads also write their site's two words.
+__global__ void __launch_bounds__(256)
+    pack_sites_kernel(const float* __restrict__ points, const float* __restrict__ radii,
+                      const float* __restrict__ normals, const float* __restrict__ density,
+                      const float* __restrict__ rgb, const float* __restrict__ height, int64_t n, int64_t rows,
+                      float4* __restrict__ spheres, float4* __restrict__ nsigmas, float4* __restrict__ rgbh) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < rows) rgbh[i] = make_float4(rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2], height[i]);
+  if (i < n) {
+    spheres[i] = make_float4(points[3 * i], points[3 * i + 1], points[3 * i + 2], radii[i]);
+    nsigmas[i] = make_float4(normals[3 * i], normals[3 * i + 1], normals[3 * i + 2], density[i]);
+  }
+}
+
 }  // namespace pf
 
 using namespace pf;
@@ -1644,6 +1661,25 @@ int pf_raster_vis(const pf_raster_vis_args* a, void* stream) {
     raster_rgb_kernel<false, true><<<grid, WAVE * WPB, 0, as_stream(stream)>>>(p);
   else
     raster_rgb_kernel<true, true><<<grid, WAVE * WPB, 0, as_stream(stream)>>>(p);
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+int pf_pack_sites(const float* points, const float* radii, const float* normals, const float* density,
+                  const float* texel_rgb, const float* texel_height, int32_t n_sites, int32_t n_texels,
+                  float* spheres, float* nsigmas, float* texel_rgbh, void* stream) {
+  if (n_sites < 0 || n_texels < 1 || n_texels > PF_MAX_TEXELS ||
+      (n_sites > 0 && (!points || !radii || !normals || !density || !texel_rgb || !texel_height || !spheres ||
+                       !nsigmas || !texel_rgbh)))
+    return set_error(PF_ERR_ARG, "pf_pack_sites: bad arguments");
+  if ((reinterpret_cast<uintptr_t>(spheres) | reinterpret_cast<uintptr_t>(nsigmas) |
+       reinterpret_cast<uintptr_t>(texel_rgbh)) % 16)
+    return set_error(PF_ERR_ARG, "pf_pack_sites: outputs must be 16-byte aligned");
+  if (n_sites == 0) return PF_OK;
+  const int64_t rows = (int64_t)n_sites * n_texels;
+  pack_sites_kernel<<<(unsigned)((rows + 255) / 256), 256, 0, as_stream(stream)>>>(
+      points, radii, normals, density, texel_rgb, texel_height, n_sites, rows, reinterpret_cast<float4*>(spheres),
+      reinterpret_cast<float4*>(nsigmas), reinterpret_cast<float4*>(texel_rgbh));
   PF_HIP(hipGetLastError());
   return PF_OK;
 }

@@ -262,6 +262,24 @@ class Rasterizer:
                                      stream_handle()))
         return live, bounds
 
+    def pack_sites(self, points, radii, normals, density, texel_rgb, texel_height):
+        """(spheres [N,4], nsigmas [N,4], texel_rgbh [N,T,4]) in FP32, the
+        concatenations of rasterize.py:1619-1624, in one launch (pf_pack_sites)."""
+        n = points.shape[0]
+        t = texel_rgb.shape[1] if texel_rgb.dim() == 3 else -1
+        if (points.shape != (n, 3) or radii.shape != (n,) or normals.shape != (n, 3) or density.shape != (n,)
+                or texel_rgb.shape != (n, t, 3) or texel_height.shape != (n, t)):
+            raise ValueError("points/normals [N,3], radii/density [N], texel_rgb [N,T,3] and texel_height [N,T] "
+                             "must agree")
+        dev = points.device
+        spheres = torch.empty((n, 4), dtype=torch.float32, device=dev)
+        nsigmas = torch.empty((n, 4), dtype=torch.float32, device=dev)
+        rgbh = torch.empty((n, t, 4), dtype=torch.float32, device=dev)
+        ins = [_f32c(x) for x in (points, radii, normals, density, texel_rgb, texel_height)]
+        check(self.lib.pf_pack_sites(*[ptr(x) for x in ins], n, t, ptr(spheres), ptr(nsigmas), ptr(rgbh),
+                                     stream_handle()))
+        return spheres, nsigmas, rgbh
+
     def encode_adjacency(self, all_spheres, adjacency, adjacency_offsets):
         """prefetch_adjacency_kernel (rasterize.py:580-606): fp16 (c_adj - c, r_adj)."""
         diff = torch.empty((max(adjacency.shape[0], 1), 4), dtype=torch.float16, device=all_spheres.device)
@@ -284,10 +302,9 @@ class Rasterizer:
         dev = points.device
         H, W = camera.height, camera.width
         N = points.shape[0]
-        all_spheres = _f32c(torch.cat([points, radii[:, None]], dim=-1))
-        all_nsigmas = _f32c(torch.cat([normals, density[:, None]], dim=-1))
+        all_spheres, all_nsigmas, all_texel_rgbh = self.pack_sites(points, radii, normals, density, texel_rgb,
+                                                                   texel_height)
         all_texel_sites = _f32c(texel_sites)
-        all_texel_rgbh = _f32c(torch.cat([texel_rgb, texel_height[..., None]], dim=-1))
         adjacency = adjacency.to(torch.int32).contiguous()
         adjacency_offsets = adjacency_offsets.to(torch.int32).contiguous()
         ray_gt_c = _f32c(ray_gt) if ray_gt is not None else None

@@ -161,3 +161,24 @@ def test_split_lists_match_unsplit_and_oracle(dev, cfg, threshold, split_backwar
     for k in harness.GRAD_KEYS:
         assert harness.rel_err(g_s[k], g_u[k]) < 1e-5, k
         assert harness.rel_err(g_s[k][keep], gref[k][keep]) < GRAD_REL_TOL, k
+
+
+@pytest.mark.parametrize("n,T", [(0, 8), (1, 1), (1000, 8), (300_001, 5)])
+def test_pack_sites_bit_exact(dev, n, T):
+    """pf_pack_sites equals the reference's concatenations (torch.cat) bit for
+    bit, for contiguous and strided inputs."""
+    import torch
+    from powerfoam_amd.rasterize import Rasterizer
+    print(f"pack sites n={n} T={T}", flush=True)
+    r = Rasterizer(harness._args(T), dev)
+    g = torch.Generator(device="cpu").manual_seed(n + T)
+    mk = lambda *shape: torch.randn(*shape, generator=g).to(dev)
+    pts, rad, nrm, den = mk(n, 3), mk(n), mk(n, 3), mk(n)
+    rgb, h = mk(n, T, 3), mk(T, n).t()  # the heights as a strided view
+    sp, ns, rgbh = r.pack_sites(pts, rad, nrm, den, rgb, h)
+    torch.cuda.synchronize()
+    assert torch.equal(sp, torch.cat([pts, rad[:, None]], -1))
+    assert torch.equal(ns, torch.cat([nrm, den[:, None]], -1))
+    assert torch.equal(rgbh, torch.cat([rgb, h[..., None]], -1))
+    with pytest.raises(ValueError):
+        r.pack_sites(pts, rad, nrm, den, rgb, h[:, :0])
