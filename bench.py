@@ -492,7 +492,7 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        log("cpu baseline (FP32 oracle, 1 frame)")
+        log("cpu baseline (FP32 oracle, frames until 10 s)")
         cpu = cpu_baseline(w, args.width, args.height)
         log(f"cpu baseline: {cpu}")
 
