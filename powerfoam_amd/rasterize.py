@@ -39,6 +39,9 @@ class RasterGradFn(torch.autograd.Function):
     @staticmethod
     def forward(ctx, rasterizer, *args):
         ctx.n_args = len(args)
+        # outputs the loss does not use come back as None (no zero tensors
+        # materialised per output); the backward reads one shared zero block
+        ctx.set_materialize_grads(False)
         return rasterizer._forward(ctx, *args)
 
     @staticmethod
@@ -462,24 +465,33 @@ class Rasterizer:
         N = all_spheres.shape[0]
         T = all_texel_sites.shape[1]
         dev = all_spheres.device
-        zf = lambda *s: torch.zeros(*s, dtype=torch.float32, device=dev)
-        g = lambda t, shape: _f32c(t) if t is not None else zf(shape)
+        # the five gradients are views of one zeroed buffer (one fill, one
+        # scrub); its tail, 256-byte aligned, stands in for every incoming
+        # gradient that is None (the kernel only reads those)
+        shapes = ((N, 4), (N, 4), (N, T, 3), (N, T, 3), (N, T))
+        sizes = [int(np.prod(sh)) for sh in shapes]
+        n_grads = sum(sizes)
+        zero_at = -(-n_grads // 64) * 64
+        missing = [(H * W * 3, grad_color_in), (H * W, grad_opacity_in), (H * W, grad_normal_distance_in),
+                   (H * W * 3, grad_normal_in), (N, grad_contrib_in)]
+        if dq is not None:
+            missing.append((int(np.prod(dq.shape)), grad_quantile_depths_in))
+        n_zero = max([n for n, t in missing if t is None], default=0)
+        grad_buf = torch.zeros(zero_at + n_zero, dtype=torch.float32, device=dev)
+        spheres_grad, nsigmas_grad, texel_sites_grad, texel_rgb_grad, texel_height_grad = (
+            v.view(sh) for v, sh in zip(torch.split(grad_buf[:n_grads], sizes), shapes))
+        g = lambda t, shape: _f32c(t) if t is not None else grad_buf[zero_at: zero_at + int(np.prod(shape))].view(shape)
         grad_color = g(grad_color_in, (H, W, 3))
-        grad_opacity = g(grad_opacity_in, (H, W))
-        grad_log_t = (-grad_opacity * torch.exp(log_t_out)).contiguous()
+        if grad_opacity_in is not None:
+            grad_log_t = (-_f32c(grad_opacity_in) * torch.exp(log_t_out)).contiguous()
+        else:
+            grad_log_t = g(None, (H, W))
         grad_nd = g(grad_normal_distance_in, (H, W))
         grad_n = g(grad_normal_in, (H, W, 3))
         grad_contrib = g(grad_contrib_in, (N,))
         grad_q = None
         if dq is not None:
             grad_q = g(grad_quantile_depths_in, tuple(dq.shape))
-
-        # the five gradients are views of one zeroed buffer: one fill, one scrub
-        shapes = ((N, 4), (N, 4), (N, T, 3), (N, T, 3), (N, T))
-        sizes = [int(np.prod(sh)) for sh in shapes]
-        grad_buf = zf((sum(sizes),))
-        spheres_grad, nsigmas_grad, texel_sites_grad, texel_rgb_grad, texel_height_grad = (
-            v.view(sh) for v, sh in zip(torch.split(grad_buf, sizes), shapes))
 
         a = _abi.PfRasterBwdArgs()
         a.cam, ray_maps = self._camera(camera, grad_ctx.ray_maps)
@@ -512,7 +524,7 @@ class Rasterizer:
         a.grad_texel_height = texel_height_grad.data_ptr()
         st = stream_handle()
         self._launch("raster_bwd", self.lib.pf_raster_bwd, _ref(a), st)
-        check(self.lib.pf_scrub_nonfinite(ptr(grad_buf), grad_buf.numel(), st))
+        check(self.lib.pf_scrub_nonfinite(ptr(grad_buf), n_grads, st))
         del grad_ctx.rasterizer, grad_ctx.camera, grad_ctx.ray_maps, grad_ctx.order, grad_ctx.segs
         return (None, None, None, spheres_grad[:, :3], spheres_grad[:, 3], nsigmas_grad[:, 3],
                 nsigmas_grad[:, :3], texel_sites_grad, texel_rgb_grad, texel_height_grad, None, None, None, None)

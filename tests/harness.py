@@ -97,10 +97,11 @@ def _args(T, is_pinhole=True):
 
 
 def run_gpu(case, dev, grads_in=None, exact_gradients=False, exact_geometry=False, return_point_err=True,
-            threshold=1e-3, seg_len="default", split_backward=False):
+            threshold=1e-3, seg_len="default", split_backward=False, loss_outputs=None):
     """Forward (+ backward with the given output gradients) through the drop-in
     Rasterizer. Returns numpy dicts. seg_len: the Rasterizer's list-splitting
-    length (None: no splitting)."""
+    length (None: no splitting). loss_outputs: the outputs the loss is built
+    from (default all of grads_in's); the others reach the backward as None."""
     import torch
     from powerfoam_amd.rasterize import Rasterizer
 
@@ -132,9 +133,11 @@ def run_gpu(case, dev, grads_in=None, exact_gradients=False, exact_geometry=Fals
     grads = None
     if grads_in is not None:
         terms = []
+        use = loss_outputs or ("color", "opacity", "normal_distance", "normal", "contrib", "quantile_depths")
         for k in ("color", "opacity", "normal_distance", "normal", "contrib"):
-            terms.append((res[names.index(k)] * tt(grads_in[k])).sum())
-        if "quantile_depths" in grads_in and res[4] is not None:
+            if k in use:
+                terms.append((res[names.index(k)] * tt(grads_in[k])).sum())
+        if "quantile_depths" in use and "quantile_depths" in grads_in and res[4] is not None:
             terms.append((res[4] * tt(grads_in["quantile_depths"])).sum())
         torch.autograd.backward(sum(terms))
         gs = {k: v.grad.detach().cpu().numpy() for k, v in leaves.items()}

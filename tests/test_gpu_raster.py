@@ -72,6 +72,29 @@ def test_backward_matches_fp64_oracle(dev, cfg):
         assert err < GRAD_REL_TOL, (k, err)
 
 
+@pytest.mark.parametrize("outputs", [("color",), ("opacity", "normal"), ("normal_distance", "contrib")])
+def test_backward_with_unused_outputs(dev, outputs):
+    """Outputs the loss leaves out reach the backward as None (gradients are
+    not materialised): the gradients equal the oracle's with zero incoming
+    gradients for those outputs."""
+    cfg = CASES[3]
+    case = harness.small_case(**cfg)
+    print(f"backward from {outputs} only", flush=True)
+    gin = harness.random_grads(case, seed=17)
+    for k in ("color", "opacity", "normal_distance", "normal", "contrib", "quantile_depths"):
+        if k not in outputs:
+            gin[k] = np.zeros_like(gin[k])
+    _, grads = harness.run_gpu(case, dev, gin, loss_outputs=outputs)
+    _, ref = harness.run_oracle(case, gin)
+    keep = harness.well_conditioned(ref)
+    for k in harness.GRAD_KEYS:
+        if not np.any(ref[k][keep]):
+            assert not np.any(grads[k][keep]), k
+            continue
+        err = harness.rel_err(grads[k][keep], ref[k][keep])
+        assert err < GRAD_REL_TOL, (k, err)
+
+
 @pytest.mark.parametrize("cfg,attr", [(CASES[0], "float"), (CASES[1], "float"), (CASES[2], "half"),
                                       (CASES[4], "float"), (CASES[6], "float")],
                          ids=["case0", "case1", "case2-half", "case4-fisheye", "case6-culled"])
