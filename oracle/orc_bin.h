@@ -5,8 +5,13 @@
  * order the HIP kernels use (compiled with -ffp-contract=off, correctly
  * rounded sqrt/div), so tile membership and sort keys compare bit-exact.
  *
- * Pinned by: brute-force (site, tile) enumeration and an exact FP64
- * ellipse/rectangle overlap test (tests/test_oracle_binning.py).
+ * Pinned by (tests/test_oracle_binning.py): coverage -- every pixel whose
+ * FP64 ray meets a non-culled ball lies in a listed tile; and a two-sided
+ * check of every (site, tile) pair against an ellipse fitted in FP64 to the
+ * sampled, projected silhouette of the ball (no shared conic algebra): listed
+ * tiles pass its pixel-range and two slab tests within 2e-4, tiles that pass
+ * them by more are listed. Planted mistakes (an always-true tile test,
+ * semi-axes scaled by 1.02, a shifted centre, swapped axes) fail that check.
  * Parity unpinned: the reference runs under Warp/NVRTC, whose FMA
  * contraction cannot be reproduced; bit-exactness is defined against this
  * mirror. Two Warp semantics are fixed here by choice: wp.cast(float,

@@ -6,9 +6,14 @@ bit-exact), by checks independent of its formulas:
   be conservative), for every site that is not culled;
 - culling: sites behind the camera or with |c - eye| < 4r are never binned;
 - order: each tile's list ascends by |c - eye|^2 - r^2 (then site id), and the
-  counts equal the list lengths.
+  counts equal the list lengths;
+- tile sets, two-sided: against an ellipse fitted in FP64 to the projected
+  silhouette of each ball, listed tiles pass the reference's pixel-range and
+  slab tests and tiles that pass them by a margin are listed; planted
+  mistakes (always-true tile test, inflated or shrunk semi-axes, shifted
+  centre, swapped axes) fail this check.
 A dropped term or wrong sign in the conic, the axis or the pixel mapping
-breaks coverage.
+breaks coverage or the two-sided check.
 """
 
 import numpy as np
@@ -213,3 +218,197 @@ def test_cone_binning_covers_hits_and_bounds_extras(seed, eye):
             assert ang[pix_tile == tile].min() - beta <= 2 * half + 1e-5, (k, tile)
             extra_checked += 1
     assert hit_checked > 50 and extra_checked > 100
+
+
+# ------------------------------------------------- pinhole tile sets, two-sided
+# The reference keeps a (tile, site) pair when the tile lies in the pixel range
+# of the box around the projected ellipse and passes the two-axis separating
+# test against the ellipse's bounding box (camera.py:79-178,
+# rasterize.py:162-200). Here the ellipse comes from FP64 geometry that shares
+# nothing with the oracle's conic algebra: the sphere's silhouette circle (the
+# ring of tangent points seen from the eye) is sampled, projected into the
+# camera's NDC plane and fitted by a conic; centre, axes and semi-axes follow
+# from that fit. Every listed tile must then pass the same two slab tests and
+# the pixel-range test within a small tolerance (bounds over-inclusion), and
+# every tile that passes them by more than the tolerance must be listed
+# (bounds under-inclusion). Mutated tile sets -- an always-true tile test,
+# inflated or shifted ellipses -- must fail the check.
+
+def _silhouette_ellipse(t, c, r, n_samples=720):
+    """(centre [2], axes [2,2] rows unit, semi [2], aabb (umin, umax, vmin, vmax))
+    of the NDC projection of the sphere (c, r) by FP64 sampling, or None when
+    part of the silhouette lies behind the eye plane (the conic is no ellipse)."""
+    e = t.eye.numpy().astype(np.float64)
+    X = t.right.numpy().astype(np.float64)
+    Y = t.up.numpy().astype(np.float64)
+    Z = np.cross(Y, X)
+    Z /= np.linalg.norm(Z)
+    v = c - e
+    D = np.linalg.norm(v)
+    w = v / D
+    m = e + v * (1.0 - r * r / (D * D))
+    rho = r * np.sqrt(D * D - r * r) / D
+    a = np.cross(w, [1.0, 0.0, 0.0] if abs(w[0]) < 0.9 else [0.0, 1.0, 0.0])
+    a /= np.linalg.norm(a)
+    b = np.cross(w, a)
+    th = np.linspace(0.0, 2.0 * np.pi, n_samples, endpoint=False)
+    p = m[None] + rho * (np.cos(th)[:, None] * a[None] + np.sin(th)[:, None] * b[None])
+    d = p - e[None]
+    dz = d @ Z
+    if dz.min() <= 1e-9 * D:
+        return None
+    u = (d @ X) / (X @ X) / dz
+    vv = (d @ Y) / (Y @ Y) / dz
+    # conic through the samples: null vector of [u^2, uv, v^2, u, v, 1]
+    s = max(np.abs(u).max(), np.abs(vv).max(), 1e-12)
+    us, vs = u / s, vv / s
+    M = np.stack([us * us, us * vs, vs * vs, us, vs, np.ones_like(us)], 1)
+    A, B, C, Dc, E, F = np.linalg.svd(M)[2][-1]
+    ctr = np.linalg.solve(np.array([[2 * A, B], [B, 2 * C]]), -np.array([Dc, E]))
+    F0 = A * ctr[0] ** 2 + B * ctr[0] * ctr[1] + C * ctr[1] ** 2 + Dc * ctr[0] + E * ctr[1] + F
+    lam, vec = np.linalg.eigh(np.array([[A, B / 2], [B / 2, C]]))
+    semi = np.sqrt(-F0 / lam) * s
+    return ctr * s, vec.T, semi, (u.min(), u.max(), vv.min(), vv.max())
+
+
+def _tile_rect(ti, tj, W, H):
+    """NDC rectangle of tile (ti, tj) as the reference's SAT forms it (camera.py:145-148)."""
+    return (2.0 * tj * 8 / (W - 1) - 1.0, 2.0 * (tj + 1) * 8 / (W - 1) - 1.0,
+            1.0 - 2.0 * (ti + 1) * 8 / (H - 1), 1.0 - 2.0 * ti * 8 / (H - 1))
+
+
+def _check_tile_sets(t, sp, offsets, sites, eps=2e-4):
+    """(extra, missing, checked): listed (tile, site) pairs that fail the FP64
+    tests by more than eps, and pairs that pass them by more than eps but are
+    not listed, over the non-culled sites whose projection is an ellipse."""
+    W, H = t.width, t.height
+    th, tw = 1 + (H - 1) // 8, 1 + (W - 1) // 8
+    listed = set()
+    for tile in range(offsets.shape[0] - 1):
+        for site in sites[offsets[tile]:offsets[tile + 1]]:
+            listed.add((int(tile), int(site)))
+    e = t.eye.numpy().astype(np.float64)
+    fwd_un = np.cross(t.up.numpy().astype(np.float64), t.right.numpy().astype(np.float64))
+    extra = missing = checked = 0
+    seen = set()
+    for k in range(sp.shape[0]):
+        c = sp[k, :3].astype(np.float64)
+        r = float(sp[k, 3])
+        v = c - e
+        if np.linalg.norm(v) < 4 * r * (1 + 1e-6) or v @ fwd_un < 0.1 * (1 + 1e-6):
+            continue
+        ell = _silhouette_ellipse(t, c, r)
+        if ell is None:
+            continue
+        ctr, axes, semi, _ = ell
+        if abs(semi[0] - semi[1]) < 1e-3 * semi.max():
+            continue  # a circle: the box orientation is arbitrary
+        # the reference's box is the ellipse's bounding box, and its pixel
+        # range the box's axis-aligned extent, floor/ceil to pixels
+        ext_u = abs(axes[0, 0]) * semi[0] + abs(axes[1, 0]) * semi[1]
+        ext_v = abs(axes[0, 1]) * semi[0] + abs(axes[1, 1]) * semi[1]
+        jf = ((ctr[0] - ext_u + 1) * 0.5 * (W - 1), (ctr[0] + ext_u + 1) * 0.5 * (W - 1))
+        i_f = ((1 - (ctr[1] + ext_v)) * 0.5 * (H - 1), (1 - (ctr[1] - ext_v)) * 0.5 * (H - 1))
+        tol_px = eps * (W + H)
+        for ti in range(th):
+            for tj in range(tw):
+                # pixel-range test: the tile's pixels meet [floor(min), ceil(max)]
+                lo_j, hi_j = 8 * tj, min(8 * tj + 7, W - 1)
+                lo_i, hi_i = 8 * ti, min(8 * ti + 7, H - 1)
+                gap = max(np.floor(jf[0]) - hi_j, lo_j - np.ceil(jf[1]),
+                          np.floor(i_f[0]) - hi_i, lo_i - np.ceil(i_f[1]))
+                # margin of the range test in pixels (distance of the float
+                # bounds to the nearest integer decides floor/ceil)
+                near_px = min(abs(x - round(x)) for x in (*jf, *i_f))
+                u0, u1, v0, v1 = _tile_rect(ti, tj, W, H)
+                tc = np.array([(u0 + u1) / 2, (v0 + v1) / 2]) - ctr
+                half = np.array([(u1 - u0) / 2, (v1 - v0) / 2])
+                slack = min(semi[q] + abs(axes[q, 0]) * half[0] + abs(axes[q, 1]) * half[1] - abs(tc @ axes[q])
+                            for q in range(2))
+                tol = eps * (1.0 + semi.max())
+                ok_in = gap <= 0 and slack > tol and (gap < 0 or near_px > tol_px)
+                ok_out = gap > 0 and near_px > tol_px or slack < -tol
+                is_listed = ((ti + tj * th), k) in listed
+                if is_listed:
+                    seen.add(((ti + tj * th), k))
+                if is_listed and ok_out:
+                    extra += 1
+                if ok_in and not is_listed:
+                    missing += 1
+                checked += ok_in or ok_out
+    return extra, missing, checked
+
+
+def _scene_wide(seed):
+    W, H = 96, 72
+    t, cam = _cam((0.4, 0.3, 2.0), (0.0, 0.0, 0.0), W, H, 110.0)
+    s = synth.scene_uniform(seed, n=400, r_range=(0.02, 0.2), extent=1.5)
+    sp = np.concatenate([s["points"], s["radii"][:, None]], 1).astype(np.float32)
+    return t, cam, sp
+
+
+@pytest.mark.parametrize("seed", [5, 6])
+def test_pinhole_tile_sets_two_sided(seed):
+    t, cam, sp = _scene_wide(seed)
+    offsets, sites, _ = orc.bin_lists(cam, sp)
+    extra, missing, checked = _check_tile_sets(t, sp, offsets, sites)
+    assert checked > 20000
+    assert extra == 0 and missing == 0, (extra, missing, checked)
+
+
+def _mutated_lists(cam, t, sp, mutate):
+    """Tile lists with a planted mistake in the tile test: mutate(obb) -> obb'
+    (or None for an always-true test), over the oracle's own tile ranges."""
+    th = 1 + (t.height - 1) // 8
+    nt = th * (1 + (t.width - 1) // 8)
+    per = [[] for _ in range(nt)]
+    for k in range(sp.shape[0]):
+        fp = orc.site_footprint(cam, sp[k])
+        if fp is None:
+            continue
+        (ti0, ti1, tj0, tj1), obb = fp
+        o2 = mutate(obb.copy())
+        for ti in range(ti0, ti1 + 1):
+            for tj in range(tj0, tj1 + 1):
+                if o2 is None or orc.tile_obb_test(cam, ti, tj, o2):
+                    per[ti + tj * th].append(k)
+    offsets = np.zeros(nt + 1, np.int64)
+    offsets[1:] = np.cumsum([len(p) for p in per])
+    return offsets, np.array([s for p in per for s in p], np.int32)
+
+
+def _inflate(o, f=1.2):
+    o[2] *= f
+    o[3] *= f
+    return o
+
+
+def _shift(o, du=0.02):
+    o[0] += du
+    return o
+
+
+def _swap_axes(o):
+    o[4], o[5] = -o[5], o[4]
+    return o
+
+
+@pytest.mark.parametrize("name,mutate,side", [("always-true tile test", lambda o: None, "extra"),
+                                              ("semi-axes x1.2", _inflate, "extra"),
+                                              ("semi-axes x0.8", lambda o: _inflate(o, 0.8), "missing"),
+                                              ("centre shifted", _shift, "both"),
+                                              ("axes rotated 90 deg", _swap_axes, "both")])
+def test_pinhole_tile_check_catches_planted_mistakes(name, mutate, side):
+    t, cam, sp = _scene_wide(5)
+    offsets, sites = _mutated_lists(cam, t, sp, mutate)
+    extra, missing, _ = _check_tile_sets(t, sp, offsets, sites)
+    if side in ("extra", "both"):
+        assert extra > 0, name
+    if side in ("missing", "both"):
+        assert missing > 0, name
+    # the unmutated test over the same ranges reproduces the oracle's lists
+    o0, s0 = _mutated_lists(cam, t, sp, lambda o: o)
+    ob, sb, _ = orc.bin_lists(cam, sp)
+    assert np.array_equal(o0, ob)
+    for tile in range(ob.shape[0] - 1):
+        assert sorted(s0[o0[tile]:o0[tile + 1]]) == sorted(sb[ob[tile]:ob[tile + 1]])
