@@ -333,6 +333,21 @@ int pf_bin_sort(const int64_t* offsets, int32_t n_tiles, uint64_t* records,
                 int32_t* sites, void* workspace, size_t workspace_bytes,
                 void* stream);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc): pf_bin_sort with its workspace (the list of long tiles) in a
+ * library-owned device array of PF_SORT_STATIC_MAX_TILES + 1 int32 per GPU
+ * (1 MiB + 4 B, held until the process ends), so the caller passes inputs
+ * and outputs only. Same results as pf_bin_sort with a workspace. The first
+ * call on a device looks up that device's array address (no allocation).
+ * Returns PF_ERR_OVERFLOW above PF_SORT_STATIC_MAX_TILES tiles (a 4096 x 4096
+ * image). Every call on one GPU shares its array: calls that overlap on one
+ * GPU (concurrent streams or threads) race; calls in one stream order are
+ * safe, as nothing in the array must start at zero. Allocates nothing. */
+#define PF_SORT_STATIC_MAX_TILES 262144
+int pf_bin_sort_static(const int64_t* offsets, int32_t n_tiles, uint64_t* records,
+                       int32_t* sites, void* stream);
+
 /* Dispatch order of the compositing kernels, built on the device from the
  * tile offsets (no reference counterpart: the reference starts tiles in index
  * order; this replaces host-side argsorts and a host sync). Writes

@@ -25,6 +25,7 @@ PF_FACES_POWER = 1  # (c_adj - c, pm_adj - pm), the benchmark's encoding
 TILE_WIDTH = 8
 TILE_SIZE = 64
 MAX_TEXELS = 8
+SORT_STATIC_MAX_TILES = 262144  # PF_SORT_STATIC_MAX_TILES
 
 c_float_p = ctypes.POINTER(ctypes.c_float)
 c_i32_p = ctypes.POINTER(ctypes.c_int32)
@@ -221,6 +222,7 @@ SIGNATURES = {
     "pf_live_sites": ([_VP, _VP, ctypes.c_int32, _VP, _VP, _VP, ctypes.c_int32, _VP, _VP, _VP], ctypes.c_int),
     "pf_bin_sort_workspace_size": ([ctypes.c_int64, ctypes.c_int32], ctypes.c_size_t),
     "pf_bin_sort": ([_VP, ctypes.c_int32, _VP, _VP, _VP, ctypes.c_size_t, _VP], ctypes.c_int),
+    "pf_bin_sort_static": ([_VP, ctypes.c_int32, _VP, _VP, _VP], ctypes.c_int),
     "pf_tile_schedule": ([_VP, ctypes.c_int32, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, _VP, _VP, _VP, _VP, _VP,
                           _VP], ctypes.c_int),
     "pf_cone_tree_nodes": ([ctypes.c_int32, ctypes.c_int32], ctypes.c_int32),

@@ -25,6 +25,8 @@
 #include <stdarg.h>
 #include <stdio.h>
 
+#include <mutex>
+
 #include "pf_common.hpp"
 
 namespace pf {
@@ -1017,6 +1019,51 @@ __global__ void __launch_bounds__(SCHED_THREADS)
   }
 }
 
+// The three sort kernels of one call; `list` ([n_tiles + 1] int32 or NULL)
+// holds the long tiles for bin_sort_big_kernel. big_tiles_kernel writes its
+// count word and entries before the sort reads them, so the list never needs
+// clearing between calls.
+static int launch_sort(const int64_t* offsets, int32_t n_tiles, uint64_t* records, int32_t* sites, int32_t* list,
+                       hipStream_t st) {
+  const int per = SORT_THREADS / WAVE;
+  bin_sort_small_kernel<<<(n_tiles + per - 1) / per, SORT_THREADS, 0, st>>>(offsets, n_tiles, records, sites);
+  bin_sort_kernel<<<n_tiles, SORT_THREADS, 0, st>>>(offsets, n_tiles, records, sites, list ? 1 : 0, 1);
+  if (list) {
+    big_tiles_kernel<<<1, 1024, 0, st>>>(offsets, n_tiles, list);
+    bin_sort_big_kernel<<<BIG_GRID, BIG_THREADS, 0, st>>>(offsets, list, records, sites);
+  }
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+// Long-tile list of pf_bin_sort_static, owned by the library: one copy per
+// GPU (1 MiB + 4 B), held until the process ends. Host code uses only the
+// device address that hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) int32_t scratch[PF_SORT_STATIC_MAX_TILES + 1];
+
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static int32_t* g_scratch_ptr[MAX_DEVICES] = {};
+
+// Address of the current device's `scratch`, looked up once per device (no
+// allocation; the array is part of the loaded code object).
+static int static_scratch(int32_t** out) {
+  int dev = -1;
+  PF_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= MAX_DEVICES) return set_error(PF_ERR_UNSUPPORTED, "device %d beyond %d devices", dev,
+                                                      MAX_DEVICES);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_check(e, "hipGetSymbolAddress(scratch)");
+    if (!p) return set_error(PF_ERR_HIP, "hipGetSymbolAddress(scratch) returned NULL");
+    g_scratch_ptr[dev] = static_cast<int32_t*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return PF_OK;
+}
+
 }  // namespace pf
 
 using namespace pf;
@@ -1161,18 +1208,20 @@ size_t pf_bin_sort_workspace_size(int64_t n_records, int32_t n_tiles) {
 int pf_bin_sort(const int64_t* offsets, int32_t n_tiles, uint64_t* records, int32_t* sites, void* workspace,
                 size_t workspace_bytes, void* stream) {
   if (!offsets || n_tiles < 1) return set_error(PF_ERR_ARG, "pf_bin_sort: bad arguments");
-  hipStream_t st = as_stream(stream);
   const bool big = workspace && workspace_bytes >= pf_bin_sort_workspace_size(0, n_tiles);
-  const int per = SORT_THREADS / WAVE;
-  bin_sort_small_kernel<<<(n_tiles + per - 1) / per, SORT_THREADS, 0, st>>>(offsets, n_tiles, records, sites);
-  bin_sort_kernel<<<n_tiles, SORT_THREADS, 0, st>>>(offsets, n_tiles, records, sites, big ? 1 : 0, 1);
-  if (big) {
-    int32_t* list = static_cast<int32_t*>(workspace);
-    big_tiles_kernel<<<1, 1024, 0, st>>>(offsets, n_tiles, list);
-    bin_sort_big_kernel<<<BIG_GRID, BIG_THREADS, 0, st>>>(offsets, list, records, sites);
-  }
-  PF_HIP(hipGetLastError());
-  return PF_OK;
+  return launch_sort(offsets, n_tiles, records, sites, big ? static_cast<int32_t*>(workspace) : nullptr,
+                     as_stream(stream));
+}
+
+int pf_bin_sort_static(const int64_t* offsets, int32_t n_tiles, uint64_t* records, int32_t* sites, void* stream) {
+  if (!offsets || n_tiles < 1) return set_error(PF_ERR_ARG, "pf_bin_sort_static: bad arguments");
+  if (n_tiles > PF_SORT_STATIC_MAX_TILES)
+    return set_error(PF_ERR_OVERFLOW, "pf_bin_sort_static: %d tiles exceed the static scratch of %d tiles", n_tiles,
+                     PF_SORT_STATIC_MAX_TILES);
+  int32_t* list = nullptr;
+  const int s = static_scratch(&list);
+  if (s != PF_OK) return s;
+  return launch_sort(offsets, n_tiles, records, sites, list, as_stream(stream));
 }
 
 int pf_tile_schedule(const int64_t* offsets, int32_t n_tiles, int64_t n_records, int32_t seg_len,
