@@ -63,7 +63,10 @@ typedef struct pf_scene {
   const float* texel_sites;         /* [N,T,3] world-space texel sites */
   const float* texel_rgbh;          /* [N,T,4] texel rgb, height */
   const int32_t* adjacency;         /* [E]   neighbour site ids */
-  const int32_t* adjacency_offsets; /* [N+1] CSR offsets into adjacency */
+  /* [N+1] CSR offsets into adjacency; the kernels trust them:
+   * adjacency_offsets[N] must equal E, the length of adjacency and
+   * adjacency_diff (the Python mirror checks it at the frame's host sync) */
+  const int32_t* adjacency_offsets;
   const uint16_t* adjacency_diff;   /* [E,4] IEEE binary16 (c_adj - c, r_adj) */
 } pf_scene;
 
@@ -262,10 +265,14 @@ int pf_bin_scan(const int32_t* tile_counts, int32_t n_tiles, int64_t* offsets,
  * a tile (the reference's prim_visible_mask, rasterize.py:1761-1765), also
  * for sites that `live` leaves out of the lists; the caller zeroes `visible`
  * first (or passes NULL). `live` and `bounds` as for pf_bin_count, and the
- * same arrays. */
+ * same arrays. A record whose slot lies past its tile's segment (offsets
+ * that do not come from the same count) is not written, as the reference's
+ * guard `idx < offsets[tile+1]` (rasterize.py:358) drops it; the call adds
+ * the number of such records to *dropped (a device int32 the caller zeroes,
+ * or NULL), so a host can report the overflow at its next synchronisation. */
 int pf_bin_emit(const pf_camera* cam, const float* spheres, const uint8_t* live,
                 const float* bounds, int32_t n_sites, const int64_t* offsets, int32_t n_tiles,
-                uint64_t* records, int32_t* cursor, uint8_t* visible, void* stream);
+                uint64_t* records, int32_t* cursor, uint8_t* visible, int32_t* dropped, void* stream);
 
 /* Face encodings pf_live_sites reads. */
 enum {
@@ -286,12 +293,13 @@ enum {
  * shade: the sphere (inflated for FP32 rounding), shrunk to the enclosing
  * ball of its cap on the cell's side of each face that leaves the ball's
  * centre outside, three sweeps over the faces; a site whose ball falls
- * wholly outside a face gets live 0 as well.
+ * wholly outside a face gets live 0 as well. n_edges is the length of the
+ * adjacency (and diff); offsets beyond it end a site's faces there.
  * Not in the reference: the reference bins every site in the footprint
  * (count_visible_pinhole_kernel rasterize.py:140-200 has no such test). */
 int pf_live_sites(const pf_camera* cam, const float* spheres, int32_t n_sites,
                   const int32_t* adjacency, const int32_t* adjacency_offsets,
-                  const uint16_t* diff, int32_t diff_kind, uint8_t* live,
+                  int64_t n_edges, const uint16_t* diff, int32_t diff_kind, uint8_t* live,
                   float* bounds, void* stream);
 
 /* Ray-map cameras (cam->ray_maps set; rasterize.py:1681-1760 when
@@ -315,11 +323,11 @@ int pf_bin_count_cones(const pf_camera* cam, const float* cones, const float* sp
                        int32_t* tile_counts, void* stream);
 
 /* pf_bin_emit for ray-map cameras: write_visible_generic_kernel
- * (rasterize.py:365-466). Records and their order after pf_bin_sort are as
- * for pf_bin_emit. */
+ * (rasterize.py:365-466, guard :429). Records and their order after
+ * pf_bin_sort, and *dropped, are as for pf_bin_emit. */
 int pf_bin_emit_cones(const pf_camera* cam, const float* cones, const float* spheres, int32_t n_sites,
                       const int64_t* offsets, int32_t n_tiles, uint64_t* records, int32_t* cursor,
-                      uint8_t* visible, void* stream);
+                      uint8_t* visible, int32_t* dropped, void* stream);
 
 /* Bytes of scratch pf_bin_sort can use for these tile segments. */
 size_t pf_bin_sort_workspace_size(int64_t n_records, int32_t n_tiles);
@@ -365,10 +373,13 @@ int pf_tile_schedule(const int64_t* offsets, int32_t n_tiles, int64_t n_records,
                      int32_t* seg_order, int32_t* count, void* stream);
 
 /* ---- adjacency encoding (prefetch_adjacency_kernel rasterize.py:580-606) */
-/* diff[e] = binary16(c_adj - c, r_adj) for every edge of every site. */
+/* diff[e] = binary16(c_adj - c, r_adj) for every edge of every site.
+ * n_edges is the length of adjacency and diff (adjacency_offsets[n_sites]);
+ * NULL adjacency or diff with n_edges > 0 is rejected, and offsets beyond
+ * n_edges end a site's edges there. */
 int pf_adj_encode(const float* spheres, int32_t n_sites,
                   const int32_t* adjacency, const int32_t* adjacency_offsets,
-                  uint16_t* diff, void* stream);
+                  int64_t n_edges, uint16_t* diff, void* stream);
 
 /* ---- compositing (forward_kernel rasterize.py:892-1146,
  *      backward_kernel rasterize.py:1150-1514) --------------------------- */

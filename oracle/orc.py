@@ -50,6 +50,8 @@ def load():
                                                            ctypes.c_float, ctypes.c_int, vp]
             getattr(lib, f"orc_trace_bwd_{p}").argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_float,
                                                            ctypes.c_float, ctypes.c_int, vp, vp]
+        lib.orc_set_fragile.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double]
+        lib.orc_set_fragile.restype = None
         lib.orc_power_face_encode.argtypes = [vp, ctypes.c_int32, vp, vp, vp]
         lib.orc_power_face_encode.restype = None
         for p in ("f32", "f64"):
@@ -60,6 +62,17 @@ def load():
             getattr(lib, f"orc_raster_bwd_{p}").argtypes = [vp, vp]
         _lib = lib
     return _lib
+
+
+FRAGILE_DEFAULT = (1e-6, 0.0, 1e-5)
+
+
+def set_fragile(h2_rel=1e-6, h2_abs=0.0, dt_rel=1e-5):
+    """Thresholds of raster_bwd's `fragile` sites: a ray with |h^2| <
+    h2_rel r^2 + h2_abs r |o - c| (near tangent to the sphere; h2_abs scales
+    the misplacement a float32 ray carries) or a clipped interval within
+    dt_rel (1 + t_far) of zero length."""
+    load().orc_set_fragile(float(h2_rel), float(h2_abs), float(dt_rel))
 
 
 def _p(a):

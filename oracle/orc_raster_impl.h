@@ -541,10 +541,12 @@ void FN(orc_raster_bwd)(const pf_raster_bwd_args* A, FN(bwd_out) * out) {
         tn = pf.dp < (R)0 ? FN(rmax)(pf.t_surf, tn) : tn;
         R dt = tf - tn;
         if (out->fragile && !oob[l]) {
-          V3 cr = FN(crs)(FN(sub)(ro[l], c), rd[l]);
+          V3 oc = FN(sub)(ro[l], c);
+          V3 cr = FN(crs)(oc, rd[l]);
           R h2 = r * r - FN(dot)(cr, cr);
-          int frag = fabs((double)h2) < 1e-6 * (double)(r * r);
-          if (hs[l].hit && fabs((double)dt) < 1e-5 * (1.0 + fabs((double)tf))) frag = 1;
+          int frag = fabs((double)h2) < g_frag_h2_rel * (double)(r * r) +
+                                            g_frag_h2_abs * (double)r * (double)FN(len)(oc);
+          if (hs[l].hit && fabs((double)dt) < g_frag_dt_rel * (1.0 + fabs((double)tf))) frag = 1;
           if (frag) {
             out->fragile[site] = 1;
             for (int e = a0; e < a1; e++) out->fragile[sc->adjacency[e]] = 1;
@@ -581,9 +583,11 @@ void FN(orc_raster_bwd)(const pf_raster_bwd_args* A, FN(bwd_out) * out) {
           dLdtf += dLddlt * -sigma;
           if (Q > 0) {
             const float* qs = A->depth_quantiles + px * Q;
-            const float* gq = A->grad_quantile_depths + px * Q;
+            /* no incoming quantile gradient (NULL): the crossings still
+             * advance, each adding zero */
+            const float* gq = A->grad_quantile_depths ? A->grad_quantile_depths + px * Q : NULL;
             while (qidx[l] >= 0 && trans > (R)qs[qidx[l]]) {
-              R g = (R)gq[qidx[l]];
+              R g = gq ? (R)gq[qidx[l]] : (R)0;
               dLdsig += -g * FN(rlog)(trans / (R)qs[qidx[l]]) / (sigma * sigma + qeps);
               dLdtn += g;
               dLdlog_t_i += g / (sigma + qeps);

@@ -218,6 +218,20 @@ void orc_f16_to_f32_array(const uint16_t* x, int64_t n, float* y) {
 
 /* ------------------------------------------------------------- compositing */
 
+/* Ill-conditioning test of the backward's `fragile` output (a site some ray
+ * meets where its gradient is not resolved): |h^2| < h2_rel r^2 +
+ * h2_abs r |o - c| (h^2 = r^2 - |(o - c) x d|^2, a tangent ray; the second
+ * term is the rounding a float32 ray carries: a direction correct to a few
+ * ulp misplaces the ray by ~eps |o - c| near the site), or a clipped interval
+ * within dt_rel (1 + t_far) of zero length. */
+static double g_frag_h2_rel = 1e-6, g_frag_h2_abs = 0.0, g_frag_dt_rel = 1e-5;
+
+void orc_set_fragile(double h2_rel, double h2_abs, double dt_rel) {
+  g_frag_h2_rel = h2_rel;
+  g_frag_h2_abs = h2_abs;
+  g_frag_dt_rel = dt_rel;
+}
+
 #define R float
 #define SFX _f32
 #include "orc_raster_impl.h"

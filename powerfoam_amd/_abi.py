@@ -218,8 +218,10 @@ SIGNATURES = {
     "pf_last_error": ([], ctypes.c_char_p),
     "pf_bin_count": ([_VP, _VP, _VP, _VP, ctypes.c_int32, _VP, _VP], ctypes.c_int),
     "pf_bin_scan": ([_VP, ctypes.c_int32, _VP, _VP], ctypes.c_int),
-    "pf_bin_emit": ([_VP, _VP, _VP, _VP, ctypes.c_int32, _VP, ctypes.c_int32, _VP, _VP, _VP, _VP], ctypes.c_int),
-    "pf_live_sites": ([_VP, _VP, ctypes.c_int32, _VP, _VP, _VP, ctypes.c_int32, _VP, _VP, _VP], ctypes.c_int),
+    "pf_bin_emit": ([_VP, _VP, _VP, _VP, ctypes.c_int32, _VP, ctypes.c_int32, _VP, _VP, _VP, _VP, _VP],
+                    ctypes.c_int),
+    "pf_live_sites": ([_VP, _VP, ctypes.c_int32, _VP, _VP, ctypes.c_int64, _VP, ctypes.c_int32, _VP, _VP, _VP],
+                      ctypes.c_int),
     "pf_bin_sort_workspace_size": ([ctypes.c_int64, ctypes.c_int32], ctypes.c_size_t),
     "pf_bin_sort": ([_VP, ctypes.c_int32, _VP, _VP, _VP, ctypes.c_size_t, _VP], ctypes.c_int),
     "pf_bin_sort_static": ([_VP, ctypes.c_int32, _VP, _VP, _VP], ctypes.c_int),
@@ -228,8 +230,9 @@ SIGNATURES = {
     "pf_cone_tree_nodes": ([ctypes.c_int32, ctypes.c_int32], ctypes.c_int32),
     "pf_bin_cones": ([_VP, _VP, _VP], ctypes.c_int),
     "pf_bin_count_cones": ([_VP, _VP, _VP, ctypes.c_int32, _VP, _VP], ctypes.c_int),
-    "pf_bin_emit_cones": ([_VP, _VP, _VP, ctypes.c_int32, _VP, ctypes.c_int32, _VP, _VP, _VP, _VP], ctypes.c_int),
-    "pf_adj_encode": ([_VP, ctypes.c_int32, _VP, _VP, _VP, _VP], ctypes.c_int),
+    "pf_bin_emit_cones": ([_VP, _VP, _VP, ctypes.c_int32, _VP, ctypes.c_int32, _VP, _VP, _VP, _VP, _VP],
+                          ctypes.c_int),
+    "pf_adj_encode": ([_VP, ctypes.c_int32, _VP, _VP, ctypes.c_int64, _VP, _VP], ctypes.c_int),
     "pf_raster_fwd": ([_VP, _VP], ctypes.c_int),
     "pf_raster_bwd": ([_VP, _VP], ctypes.c_int),
     "pf_raster_rgb": ([_VP, _VP], ctypes.c_int),

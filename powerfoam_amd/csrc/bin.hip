@@ -255,7 +255,8 @@ __global__ void __launch_bounds__(256)
 __global__ void __launch_bounds__(256)
     bin_emit_kernel(BinCam cam, const float4* __restrict__ spheres, const uint8_t* __restrict__ live,
                     const float4* __restrict__ bounds, int n, const int64_t* __restrict__ offsets,
-                    uint64_t* __restrict__ records, int32_t* __restrict__ cursor, uint8_t* __restrict__ visible) {
+                    uint64_t* __restrict__ records, int32_t* __restrict__ cursor, uint8_t* __restrict__ visible,
+                    int32_t* __restrict__ dropped) {
   __shared__ WaveSite s_ws[256];
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & (WAVE - 1);
@@ -267,7 +268,10 @@ __global__ void __launch_bounds__(256)
                    [&](int t, const WaveSite& o, bool in_ball) {
     if (in_ball && (!live || live[o.site])) {
       const int64_t idx = offsets[t] + (int64_t)atomicAdd(&cursor[t], 1);
-      if (idx < offsets[t + 1]) records[idx] = ((uint64_t)o.key << 32) | (uint32_t)o.site;
+      if (idx < offsets[t + 1])
+        records[idx] = ((uint64_t)o.key << 32) | (uint32_t)o.site;
+      else if (dropped)
+        atomicAdd(dropped, 1);  // count and emit disagree: reported, not silent
     }
     if (visible) visible[o.site] = 1;
   });
@@ -288,8 +292,8 @@ __global__ void __launch_bounds__(256)
 // the adjacency encoder does.
 __global__ void __launch_bounds__(256)
     live_sites_kernel(const float4* __restrict__ spheres, int n, const int32_t* __restrict__ adj,
-                      const int32_t* __restrict__ off, const uint2* __restrict__ diff, int power, f3 eye,
-                      uint8_t* __restrict__ live) {
+                      const int32_t* __restrict__ off, int64_t n_edges, const uint2* __restrict__ diff, int power,
+                      f3 eye, uint8_t* __restrict__ live) {
   __shared__ int s_off[4][WAVE + 1];
   __shared__ float4 s_sph[4][WAVE];
   __shared__ int s_dead[4][WAVE];
@@ -308,7 +312,7 @@ __global__ void __launch_bounds__(256)
   sd[lane] = 0;
   wave_sync();
   const float eye_len = __builtin_sqrtf(eye.x * eye.x + eye.y * eye.y + eye.z * eye.z);
-  const int e1 = so[ns];
+  const int e1 = (int)min((int64_t)so[ns], n_edges);  // an inconsistent CSR ends at the edge arrays
   for (int e = so[0] + lane; e < e1; e += WAVE) {
     int l = 0;
 #pragma unroll
@@ -358,7 +362,7 @@ constexpr int BOUNDS_SPAN = 4 * WAVE;  // sites per wave
 
 __device__ __forceinline__ void site_bound(const float4* __restrict__ spheres, int s,
                                            const int32_t* __restrict__ adj, const int32_t* __restrict__ off,
-                                           const uint2* __restrict__ diff, int power, f3 eye,
+                                           int64_t n_edges, const uint2* __restrict__ diff, int power, f3 eye,
                                            uint8_t* __restrict__ live, float4* __restrict__ bounds) {
   const float4 c = spheres[s];
   const f3 ce{c.x - eye.x, c.y - eye.y, c.z - eye.z};
@@ -368,7 +372,7 @@ __device__ __forceinline__ void site_bound(const float4* __restrict__ spheres, i
   f3 b{c.x, c.y, c.z};
   float rad = c.w + 2e-3f * __builtin_sqrtf(c.w * dist);
   bool empty = false;
-  const int e0 = off[s], e1 = off[s + 1];
+  const int e0 = (int)min((int64_t)off[s], n_edges), e1 = (int)min((int64_t)off[s + 1], n_edges);
   // faces are read four at a time (the sweeps are a chain of dependent
   // steps; one load per step would leave every step waiting on memory)
   for (int pass = 0; pass < 3 && !empty; pass++) {
@@ -427,8 +431,8 @@ __device__ __forceinline__ void site_bound(const float4* __restrict__ spheres, i
 
 __global__ void __launch_bounds__(256)
     site_bounds_kernel(const float4* __restrict__ spheres, int n, const int32_t* __restrict__ adj,
-                       const int32_t* __restrict__ off, const uint2* __restrict__ diff, int power, f3 eye,
-                       uint8_t* __restrict__ live, float4* __restrict__ bounds) {
+                       const int32_t* __restrict__ off, int64_t n_edges, const uint2* __restrict__ diff, int power,
+                       f3 eye, uint8_t* __restrict__ live, float4* __restrict__ bounds) {
   __shared__ int s_idx[4][BOUNDS_SPAN];
   const int wv = threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
   const int s0 = (blockIdx.x * 4 + wv) * BOUNDS_SPAN;
@@ -447,7 +451,7 @@ __global__ void __launch_bounds__(256)
     cnt += __popcll(m);
   }
   wave_sync();
-  for (int i = lane; i < cnt; i += WAVE) site_bound(spheres, idx[i], adj, off, diff, power, eye, live, bounds);
+  for (int i = lane; i < cnt; i += WAVE) site_bound(spheres, idx[i], adj, off, n_edges, diff, power, eye, live, bounds);
 }
 
 // ------------------------------------------------ non-pinhole cameras
@@ -667,7 +671,7 @@ __global__ void __launch_bounds__(256)
     bin_emit_cones_kernel(BinCam cam, ConeTree t, const float4* __restrict__ cones,
                           const float4* __restrict__ spheres, int n, const int64_t* __restrict__ offsets,
                           uint64_t* __restrict__ records, int32_t* __restrict__ cursor,
-                          uint8_t* __restrict__ visible) {
+                          uint8_t* __restrict__ visible, int32_t* __restrict__ dropped) {
   CONE_LDS_TABLES(t);
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
@@ -680,6 +684,8 @@ __global__ void __launch_bounds__(256)
     if (idx < offsets[tile + 1]) {
       records[idx] = rec;
       any = true;
+    } else if (dropped) {
+      atomicAdd(dropped, 1);
     }
   });
   if (any && visible) visible[s] = 1;
@@ -1098,7 +1104,7 @@ int pf_bin_scan(const int32_t* tile_counts, int32_t n_tiles, int64_t* offsets, v
 
 int pf_bin_emit(const pf_camera* cam, const float* spheres, const uint8_t* live, const float* bounds,
                 int32_t n_sites, const int64_t* offsets, int32_t n_tiles, uint64_t* records, int32_t* cursor,
-                uint8_t* visible, void* stream) {
+                uint8_t* visible, int32_t* dropped, void* stream) {
   if (!cam || !offsets || !cursor || n_sites < 0 || (n_sites > 0 && !spheres))
     return set_error(PF_ERR_ARG, "pf_bin_emit: bad arguments");
   if (cam->ray_maps) return set_error(PF_ERR_UNSUPPORTED, "pf_bin_emit: pinhole cameras only");
@@ -1109,17 +1115,18 @@ int pf_bin_emit(const pf_camera* cam, const float* spheres, const uint8_t* live,
   if (n_sites > 0)
     bin_emit_kernel<<<(n_sites + 255) / 256, 256, 0, st>>>(make_bincam(*cam), reinterpret_cast<const float4*>(spheres),
                                                           live, reinterpret_cast<const float4*>(bounds), n_sites,
-                                                          offsets, records, cursor, visible);
+                                                          offsets, records, cursor, visible, dropped);
   PF_HIP(hipGetLastError());
   return PF_OK;
 }
 
 int pf_live_sites(const pf_camera* cam, const float* spheres, int32_t n_sites, const int32_t* adjacency,
-                  const int32_t* adjacency_offsets, const uint16_t* diff, int32_t diff_kind, uint8_t* live,
-                  float* bounds, void* stream) {
-  if (!cam || n_sites < 0 || (n_sites > 0 && (!spheres || !adjacency_offsets || !live)))
+                  const int32_t* adjacency_offsets, int64_t n_edges, const uint16_t* diff, int32_t diff_kind,
+                  uint8_t* live, float* bounds, void* stream) {
+  if (!cam || n_sites < 0 || n_edges < 0 || (n_sites > 0 && (!spheres || !adjacency_offsets || !live)))
     return set_error(PF_ERR_ARG, "pf_live_sites: bad arguments");
-  if (!diff && !adjacency) return set_error(PF_ERR_ARG, "pf_live_sites: needs the adjacency or its encoding");
+  if (n_edges > 0 && !diff && !adjacency)
+    return set_error(PF_ERR_ARG, "pf_live_sites: needs the adjacency or its encoding");
   if (diff && diff_kind != PF_FACES_RADIUS && diff_kind != PF_FACES_POWER)
     return set_error(PF_ERR_ARG, "pf_live_sites: unknown face encoding");
   if (cam->ray_maps) return set_error(PF_ERR_UNSUPPORTED, "pf_live_sites: pinhole cameras only");
@@ -1127,12 +1134,12 @@ int pf_live_sites(const pf_camera* cam, const float* spheres, int32_t n_sites, c
     return set_error(PF_ERR_ARG, "pf_live_sites: bounds not 16-byte aligned");
   if (n_sites == 0) return PF_OK;
   live_sites_kernel<<<(n_sites + 4 * WAVE - 1) / (4 * WAVE), 256, 0, as_stream(stream)>>>(
-      reinterpret_cast<const float4*>(spheres), n_sites, adjacency, adjacency_offsets,
+      reinterpret_cast<const float4*>(spheres), n_sites, adjacency, adjacency_offsets, n_edges,
       reinterpret_cast<const uint2*>(diff), diff_kind == PF_FACES_POWER ? 1 : 0,
       f3{cam->eye[0], cam->eye[1], cam->eye[2]}, live);
   if (bounds)
     site_bounds_kernel<<<(n_sites + 4 * BOUNDS_SPAN - 1) / (4 * BOUNDS_SPAN), 256, 0, as_stream(stream)>>>(
-        reinterpret_cast<const float4*>(spheres), n_sites, adjacency, adjacency_offsets,
+        reinterpret_cast<const float4*>(spheres), n_sites, adjacency, adjacency_offsets, n_edges,
         reinterpret_cast<const uint2*>(diff), diff_kind == PF_FACES_POWER ? 1 : 0,
         f3{cam->eye[0], cam->eye[1], cam->eye[2]}, live, reinterpret_cast<float4*>(bounds));
   PF_HIP(hipGetLastError());
@@ -1183,7 +1190,7 @@ int pf_bin_count_cones(const pf_camera* cam, const float* cones, const float* sp
 
 int pf_bin_emit_cones(const pf_camera* cam, const float* cones, const float* spheres, int32_t n_sites,
                       const int64_t* offsets, int32_t n_tiles, uint64_t* records, int32_t* cursor, uint8_t* visible,
-                      void* stream) {
+                      int32_t* dropped, void* stream) {
   if (!cam || !cones || !offsets || !cursor || n_sites < 0 || (n_sites > 0 && !spheres))
     return set_error(PF_ERR_ARG, "pf_bin_emit_cones: bad arguments");
   ConeTree t;
@@ -1195,7 +1202,7 @@ int pf_bin_emit_cones(const pf_camera* cam, const float* cones, const float* sph
   if (n_sites > 0)
     bin_emit_cones_kernel<<<(n_sites + 255) / 256, 256, 0, st>>>(
         make_bincam(*cam), t, reinterpret_cast<const float4*>(cones), reinterpret_cast<const float4*>(spheres),
-        n_sites, offsets, records, cursor, visible);
+        n_sites, offsets, records, cursor, visible, dropped);
   PF_HIP(hipGetLastError());
   return PF_OK;
 }

@@ -108,6 +108,7 @@ class Rasterizer:
         # several thousand walked sites would set the forward's duration.
         self.split_min = 4096
         self._longest = None
+        self._dropped = None
         # Leave sites that no ray from the camera can shade (their sphere lies
         # beyond one of their own faces: pf_live_sites) out of the tile lists
         # of the forward, backward and benchmark passes. Results are the same
@@ -149,12 +150,17 @@ class Rasterizer:
         rm = rm.detach().to(device=self.device, dtype=torch.float32).contiguous()
         return TorchCamera(camera.eye, camera.right, camera.up, camera.width, camera.height, rm).to_pf(), rm
 
-    def bin(self, camera, all_spheres, live=None, bounds=None):
+    def bin(self, camera, all_spheres, live=None, bounds=None, checks=()):
         """count -> scan -> (one host sync for M) -> emit -> per-tile sort; ray-map
         cameras count and emit through the cone tree (rasterize.py:1518-1760).
         `live` (uint8 per site, from live_sites) leaves the sites marked 0 out
         of the lists, `bounds` (their balls) the tiles that miss a site's
-        ball; the visibility mask still covers both."""
+        ball; the visibility mask still covers both. `checks`: (device scalar,
+        expected int, message) triples read in the same host sync; a mismatch
+        raises ValueError. Records that an emit pass could not place (its
+        offsets disagreeing with the count, rasterize.py:358) are counted on
+        the device and raise PfError at the next sync (check_binning()
+        forces one)."""
         lib = self.lib
         cam, keep = self._camera(camera)
         th, tw = _abi.tiles_hw(camera.height, camera.width)
@@ -176,23 +182,49 @@ class Rasterizer:
             check(lib.pf_bin_count_cones(_ref(cam), ptr(cones), ptr(all_spheres), n, ptr(counts), st))
         offsets = torch.empty(nt + 1, dtype=torch.int64, device=dev)
         check(lib.pf_bin_scan(ptr(counts), nt, ptr(offsets), st))
-        # the one host sync of the frame: the record count, and the longest
-        # list (which decides whether lists are split into segments)
-        m, self._longest = (int(v) for v in torch.stack([offsets[-1], counts.max().to(torch.int64)]).tolist())
+        # the one host sync of the frame: the record count, the longest list
+        # (which decides whether lists are split into segments), the records
+        # earlier emits dropped, and the caller's consistency checks
+        dropped = self._dropped_word(dev)
+        vals = torch.stack([offsets[-1], counts.max().to(torch.int64), dropped[0].to(torch.int64)] +
+                           [t.reshape(()).to(torch.int64) for t, _, _ in checks]).tolist()
+        m, self._longest = int(vals[0]), int(vals[1])
+        self._raise_dropped(int(vals[2]))
+        for (_, want, msg), got in zip(checks, vals[3:]):
+            if int(got) != int(want):
+                raise ValueError(f"{msg} (got {int(got)}, expected {int(want)})")
         records = torch.empty(max(m, 1), dtype=torch.int64, device=dev)
         cursor = torch.empty(nt, dtype=torch.int32, device=dev)
         visible = torch.zeros(n, dtype=torch.uint8, device=dev)
         if cones is None:
             check(lib.pf_bin_emit(_ref(cam), ptr(all_spheres), ptr(live), ptr(bounds), n, ptr(offsets), nt, ptr(records),
-                                  ptr(cursor), ptr(visible), st))
+                                  ptr(cursor), ptr(visible), ptr(dropped), st))
         else:
             check(lib.pf_bin_emit_cones(_ref(cam), ptr(cones), ptr(all_spheres), n, ptr(offsets), nt, ptr(records),
-                                        ptr(cursor), ptr(visible), st))
+                                        ptr(cursor), ptr(visible), ptr(dropped), st))
         sites = torch.empty(max(m, 1), dtype=torch.int32, device=dev)
         ws_bytes = int(lib.pf_bin_sort_workspace_size(m, nt))
         ws = torch.empty(max(ws_bytes // 4, 1), dtype=torch.int32, device=dev)
         check(lib.pf_bin_sort(ptr(offsets), nt, ptr(records), ptr(sites), ptr(ws), ws_bytes, st))
         return offsets, sites[:m] if m > 0 else sites[:0], visible.bool(), nt
+
+    def _dropped_word(self, dev):
+        """Device int32 counting records the emit passes could not place."""
+        if self._dropped is None or self._dropped.device != dev:
+            self._dropped = torch.zeros(1, dtype=torch.int32, device=dev)
+        return self._dropped
+
+    def _raise_dropped(self, n):
+        if n:
+            self._dropped.zero_()
+            raise _abi.PfError(f"binning dropped {n} (tile, site) records: an emit pass found its tile's segment "
+                               "full (count and emit disagreed); that frame's lists were incomplete")
+
+    def check_binning(self):
+        """Synchronises and raises PfError if an emit pass since the last
+        check dropped records (the reference drops them silently, rasterize.py:358)."""
+        if self._dropped is not None:
+            self._raise_dropped(int(self._dropped.item()))
 
     def tile_order(self, offsets):
         """Tiles by descending list length (int32, pf_tile_schedule), or None for
@@ -261,8 +293,8 @@ class Rasterizer:
         live = torch.empty(n, dtype=torch.uint8, device=all_spheres.device)
         bounds = torch.empty((n, 4), dtype=torch.float32, device=all_spheres.device) if self.bound_sites else None
         check(self.lib.pf_live_sites(_ref(cam), ptr(all_spheres), n, ptr(adjacency), ptr(adjacency_offsets),
-                                     ptr(diff) if diff is not None else None, int(kind), ptr(live), ptr(bounds),
-                                     stream_handle()))
+                                     int(adjacency.shape[0]), ptr(diff) if diff is not None else None, int(kind),
+                                     ptr(live), ptr(bounds), stream_handle()))
         return live, bounds
 
     def pack_sites(self, points, radii, normals, density, texel_rgb, texel_height):
@@ -287,7 +319,7 @@ class Rasterizer:
         """prefetch_adjacency_kernel (rasterize.py:580-606): fp16 (c_adj - c, r_adj)."""
         diff = torch.empty((max(adjacency.shape[0], 1), 4), dtype=torch.float16, device=all_spheres.device)
         check(self.lib.pf_adj_encode(ptr(all_spheres), all_spheres.shape[0], ptr(adjacency),
-                                     ptr(adjacency_offsets), ptr(diff), stream_handle()))
+                                     ptr(adjacency_offsets), int(adjacency.shape[0]), ptr(diff), stream_handle()))
         return diff[: adjacency.shape[0]]
 
     # ------------------------------------------------------------------ forward
@@ -316,7 +348,11 @@ class Rasterizer:
         adjacency_diff = self.encode_adjacency(all_spheres, adjacency, adjacency_offsets)
         live, bounds = self.live_sites(camera, all_spheres, adjacency, adjacency_offsets,
                                        None if self.exact_geometry else adjacency_diff)
-        offsets, tile_sites, prim_visible_mask, nt = self.bin(camera, all_spheres, live, bounds)
+        if adjacency_offsets.shape != (N + 1,):
+            raise ValueError("adjacency_offsets must hold N+1 offsets")
+        offsets, tile_sites, prim_visible_mask, nt = self.bin(
+            camera, all_spheres, live, bounds,
+            checks=[(adjacency_offsets[-1], adjacency.shape[0], "adjacency_offsets[-1] must equal len(adjacency)")])
         sg = self.segments(offsets, tile_sites.shape[0], self._longest)
         order = self.tile_order(offsets) if sg is None else None
 

@@ -163,15 +163,36 @@ def cech_adjacency(points, radii):
 def power_adjacency(points, radii, alpha_complex=True, hidden_cap=32, return_stats=False):
     """Power (weighted Delaunay) adjacency via the lower hull of the lifted points
     (benchmark.py:25-83), alpha-filtered, plus clipping faces for hidden sites."""
-    from scipy.spatial import ConvexHull, cKDTree
     p = points.astype(np.float64)
     r = radii.astype(np.float64)
-    n = p.shape[0]
+    e, low = _power_edges(p, r)
+    return _power_csr(p, r, e, low, alpha_complex, hidden_cap, return_stats)
+
+
+def power_adjacency_pair(points, radii, hidden_cap=32, return_stats=False):
+    """(alpha-filtered, full) power adjacency from one lower hull: the
+    rasterizer's and the ray tracer's inputs for the same scene."""
+    p = points.astype(np.float64)
+    r = radii.astype(np.float64)
+    e, low = _power_edges(p, r)
+    return (_power_csr(p, r, e, low, True, hidden_cap, return_stats),
+            _power_csr(p, r, e, low, False, hidden_cap, return_stats))
+
+
+def _power_edges(p, r):
+    """Edges of the weighted Delaunay triangulation (lower hull of the points
+    lifted to |c|^2 - r^2, benchmark.py:25-50) and its tetrahedra."""
+    from scipy.spatial import ConvexHull
     w = (p * p).sum(1) - r * r
     hull = ConvexHull(np.concatenate([p, w[:, None]], 1))
     low = hull.simplices[hull.equations[:, 3] < 0]
     e = np.concatenate([low[:, [0, 1]], low[:, [1, 2]], low[:, [2, 0]], low[:, [0, 3]], low[:, [1, 3]], low[:, [2, 3]]])
-    e = np.unique(np.sort(e, 1), axis=0)
+    return np.unique(np.sort(e, 1), axis=0), low
+
+
+def _power_csr(p, r, e, low, alpha_complex, hidden_cap, return_stats):
+    from scipy.spatial import cKDTree
+    n = p.shape[0]
     if alpha_complex:
         d = np.linalg.norm(p[e[:, 0]] - p[e[:, 1]], axis=1)
         e = e[d < r[e[:, 0]] + r[e[:, 1]]]

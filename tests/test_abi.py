@@ -85,10 +85,24 @@ def test_live_sites_argument_checks_run_on_the_host():
     cam = _abi.PfCamera()
     cam.width, cam.height = 16, 16
     fake = ctypes.c_void_p(0x1000)
-    st = lib.pf_live_sites(ctypes.byref(cam), fake, 4, fake, fake, fake, 5, fake, None, None)
+    st = lib.pf_live_sites(ctypes.byref(cam), fake, 4, fake, fake, 10, fake, 5, fake, None, None)
     assert st == _abi.PF_ERR_ARG and b"encoding" in lib.pf_last_error()
-    st = lib.pf_live_sites(ctypes.byref(cam), fake, 4, None, fake, None, 0, fake, None, None)
+    st = lib.pf_live_sites(ctypes.byref(cam), fake, 4, None, fake, 10, None, 0, fake, None, None)
     assert st == _abi.PF_ERR_ARG
     cam.ray_maps = 0x2000
-    st = lib.pf_live_sites(ctypes.byref(cam), fake, 4, fake, fake, None, 0, fake, None, None)
+    st = lib.pf_live_sites(ctypes.byref(cam), fake, 4, fake, fake, 10, None, 0, fake, None, None)
     assert st == _abi.PF_ERR_UNSUPPORTED
+
+
+def test_adjacency_encode_rejects_null_edges_on_the_host():
+    """pf_adj_encode with edges but no adjacency or output is an argument
+    error before any device work (no GPU needed)."""
+    if not os.path.exists(_abi.LIB_PATH):
+        pytest.skip("libpfhip.so not built")
+    lib = _abi.load()
+    assert lib.pf_adj_encode(1, 10, None, 1, 5, 1, None) == _abi.PF_ERR_ARG
+    assert b"NULL adjacency" in lib.pf_last_error()
+    assert lib.pf_adj_encode(1, 10, 1, 1, 5, None, None) == _abi.PF_ERR_ARG
+    assert lib.pf_adj_encode(1, 10, 1, 1, -1, 1, None) == _abi.PF_ERR_ARG
+    # no edges: nothing is read, nothing to reject
+    assert lib.pf_adj_encode(1, 10, None, 1, 0, None, None) == _abi.PF_OK

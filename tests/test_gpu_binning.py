@@ -237,7 +237,7 @@ def test_adjacency_encoding_bit_exact(n):
     d_adj = torch.tensor(adj, device="cuda")
     d_off = torch.tensor(off.astype(np.int32), device="cuda")
     out = torch.zeros((max(adj.size, 1), 4), dtype=torch.int16, device="cuda")
-    check(lib.pf_adj_encode(ptr(d_sp), n, ptr(d_adj), ptr(d_off), ptr(out), None))
+    check(lib.pf_adj_encode(ptr(d_sp), n, ptr(d_adj), ptr(d_off), adj.size, ptr(out), None))
     torch.cuda.synchronize()
     got = out.cpu().numpy().view(np.uint16)[: adj.size]
     assert np.array_equal(got, want)

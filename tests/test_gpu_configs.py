@@ -1,0 +1,391 @@
+"""The benchmark workloads at their own sizes through the GPU path, against
+the oracle. Each test prints the max absolute error of every output and
+gradient tensor (RGB, opacity, normals, depth; per-entry gradients) against
+FP64, and the same figure for the reference's own formulas run in FP32 (the
+oracle's FP32 mirror, the arithmetic the reference's kernels do).
+
+Bars:
+  * integer work (tile lists, sort order, cell sequences): bit-exact against
+    the FP32 mirror;
+  * outputs: at least 99.9% of the values within 1e-4 of FP64 (or at most a
+    tenth of the FP32 mirror's share above 1e-4), and the largest deviation
+    at most a tenth of the FP32 mirror's (depths, which jump from site to
+    site where T meets the level at a site boundary, by the share only); the
+    ray tracer, whose walk must take the reference's FP32 formulas so that
+    its cell sequences are the reference's, half instead of a tenth;
+  * gradients: ||gpu - fp64|| / ||fp64|| below 1e-3 per tensor over the sites
+    the oracle does not flag as ill-conditioned, or, where FP32 arithmetic
+    itself cannot reach that on these scenes, no more than twice the FP32
+    mirror's deviation from FP64 over the same sites.
+
+Why the FP32 yardstick: at these sizes many sites are a pixel or two across
+and dense (density up to e^6 in the 1M-site scene), so a float32 ray's own
+rounding (~1e-7 |o - c| near a site) moves a grazing chord by 1e-5 and the
+alpha of a dense site by 1e-3; the reference's formulas in FP32 deviate
+from FP64 by 0.03 (50k sites) and 0.19 (1M sites) in colour, and its sphere
+gradients by 29% and 35%. The GPU path (cancellation-free intersection
+forms) is one to two orders of magnitude closer.
+
+The GPU side runs the timed path of the benchmark (sites that cannot shade
+left out, sites binned by their balls, entries that shade nothing skipped in
+the backward); the oracle runs the reference's full lists.
+"""
+
+import time
+
+import numpy as np
+import pytest
+
+from oracle import orc
+from tests import configs, harness
+
+pytestmark = pytest.mark.gpu
+
+ABS_TOL = 1e-4
+GRAD_REL_TOL = 1e-3
+
+
+def _log(msg):
+    print(f"[configs] {msg}", flush=True)
+
+
+def _max_err(a, b, mask=None):
+    d = np.abs(np.asarray(a, np.float64) - np.asarray(b, np.float64))
+    if mask is not None:
+        d = d[mask]
+    return float(d.max()) if d.size else 0.0
+
+
+def _out_bar(name, key, gpu, f64, f32, mask=None, continuous=True, ratio=0.1):
+    """Max abs error and fraction of values above 1e-4, for the GPU and for
+    the FP32 mirror, against FP64; asserts the output bar. A discontinuous
+    output (the depth where T reaches a level, which jumps to the next site
+    when T meets the level at a site boundary) is held to the fraction only.
+    `ratio`: the share of the FP32 mirror's deviation allowed."""
+    d = np.abs(np.asarray(gpu, np.float64) - np.asarray(f64, np.float64))
+    d32 = np.abs(np.asarray(f32, np.float64) - np.asarray(f64, np.float64))
+    if mask is not None:
+        d, d32 = d[mask], d32[mask]
+    if d.size == 0:
+        return 0.0
+    e, frac, e32, frac32 = float(d.max()), float((d > ABS_TOL).mean()), float(d32.max()), float((d32 > ABS_TOL).mean())
+    _log(f"{name}: {key} max abs {e:.2e} ({frac:.4%} above 1e-4); FP32 mirror {e32:.2e} ({frac32:.4%} above 1e-4)")
+    assert frac <= 1e-3 or frac <= ratio * frac32, (key, frac, frac32)
+    if continuous:
+        assert e < ABS_TOL or e <= ratio * e32, (key, e, e32)
+    return e
+
+
+def _grad_bar(name, key, gpu, f64, f32, keep):
+    f32 = np.where(np.isfinite(f32), f32, 0.0)  # the reference scrubs non-finite gradients
+    rel = harness.rel_err(gpu[keep], f64[keep])
+    rel32 = harness.rel_err(f32[keep], f64[keep])
+    _log(f"{name}: {key} rel {rel:.2e} (all sites {harness.rel_err(gpu, f64):.2e}), max abs "
+         f"{_max_err(gpu[keep], f64[keep]):.2e} of max {np.abs(f64).max():.2e}; FP32 mirror rel {rel32:.2e}")
+    assert rel < GRAD_REL_TOL or rel <= 2.0 * rel32, (key, rel, rel32)
+
+
+def _compare_frame(name, case):
+    """Full-frame forward and backward (L2 loss against the case's target)
+    through the drop-in Rasterizer against the FP64 oracle, with the FP32
+    mirror's deviation beside it."""
+    t0 = time.time()
+    cam, _keep = case.pf_camera()
+    sp = case.spheres()
+    offsets, sites, _ = orc.bin_lists(cam, sp)
+    inp = harness.oracle_inputs(case)
+    refs = {p: orc.raster_fwd(cam, inp, offsets, sites, depth_quantiles=case.depth_quantiles, ray_gt=case.gt,
+                              return_point_err=True, precision=p) for p in ("f64", "f32")}
+    for r in refs.values():
+        r["opacity"] = 1.0 - np.exp(r["log_t"])
+    ref = refs["f64"]
+    g_color = (2.0 * (ref["color"] - case.gt) / case.gt.size).astype(np.float32)
+    _log(f"{name}: oracle forwards over {sites.size} pairs in {time.time() - t0:.1f}s")
+    out, grads = harness.run_gpu(case, "cuda", dict(color=g_color), loss_outputs=("color",))
+    H, W = case.height, case.width
+    z = np.zeros((H, W), np.float32)
+    t0 = time.time()
+    rg = {p: orc.raster_bwd(cam, inp, offsets, sites, refs[p]["tile_early_stop"], refs[p]["log_t"], g_color, z, z,
+                            np.zeros((H, W, 3), np.float32), np.zeros(case.n, np.float32),
+                            depth_quantiles=case.depth_quantiles, precision=p) for p in ("f64", "f32")}
+    _log(f"{name}: oracle backwards in {time.time() - t0:.1f}s")
+    for k in ("color", "opacity", "normal", "normal_distance"):
+        _out_bar(name, k, out[k], ref[k], refs["f32"][k])
+    qd_g, qd_r = out["quantile_depths"], ref["quantile_depths"]
+    both = (qd_g != -1) & (qd_r != -1) & (refs["f32"]["quantile_depths"] != -1)
+    fill_mismatch = float(((qd_g == -1) != (qd_r == -1)).mean())
+    _log(f"{name}: quantile depth defined on {(qd_r != -1).mean():.1%} of rays, defined-or-not differs on "
+         f"{fill_mismatch:.4%}")
+    assert fill_mismatch <= 1e-3
+    _out_bar(name, "quantile_depth", qd_g, qd_r, refs["f32"]["quantile_depths"], both, continuous=False)
+    for k in ("contrib", "point_err"):
+        scale = max(np.abs(ref[k]).max(), 1e-30)
+        _out_bar(name, k + "/max", out[k] / scale, ref[k] / scale, refs["f32"][k] / scale)
+    frag = rg["f64"]["fragile"]
+    _log(f"{name}: sites flagged ill-conditioned {frag.mean():.2%} ({int(frag.sum())} of {frag.size})")
+    for k in harness.GRAD_KEYS:
+        _grad_bar(name, k, grads[k], rg["f64"][k], rg["f32"][k], ~frag)
+    return out, ref
+
+
+def _bin_gpu(case, live=None, bounds=None, diff=None):
+    import torch
+    from powerfoam_amd.rasterize import Rasterizer
+    r = Rasterizer(harness._args(case.texel_sites.shape[1]), "cuda")
+    sp = torch.tensor(case.spheres(), device="cuda")
+    offsets, sites, visible, _ = r.bin(case.cam, sp, live, bounds)
+    torch.cuda.synchronize()
+    return offsets.cpu().numpy(), sites.cpu().numpy(), visible.cpu().numpy()
+
+
+# ------------------------------------------------------------------ C0
+def test_c0_binning_bit_exact():
+    case = configs.c0()
+    cam, _ = case.pf_camera()
+    off_o, sites_o, _ = orc.bin_lists(cam, case.spheres())
+    off_g, sites_g, vis_g = _bin_gpu(case)
+    _log(f"c0 binning: {sites_o.size} (tile, site) pairs")
+    assert np.array_equal(off_g, off_o)
+    assert np.array_equal(sites_g, sites_o)
+    vis_o = np.zeros(case.n, bool)
+    vis_o[sites_o] = True
+    assert np.array_equal(vis_g, vis_o)
+
+
+def test_c0_frame_matches_oracle():
+    _compare_frame("c0 (50k sites, 640x480)", configs.c0())
+
+
+# ------------------------------------------------------------------ C1
+def test_c1_binning_bit_exact():
+    """Every site binned (culling off): the reference's lists."""
+    case = configs.c1(0)
+    cam, _ = case.pf_camera()
+    t0 = time.time()
+    off_o, sites_o, _ = orc.bin_lists(cam, case.spheres())
+    off_g, sites_g, vis_g = _bin_gpu(case)
+    _log(f"c1 binning: {sites_o.size} (tile, site) pairs, {time.time() - t0:.1f}s")
+    assert np.array_equal(off_g, off_o)
+    assert np.array_equal(sites_g, sites_o)
+    vis_o = np.zeros(case.n, bool)
+    vis_o[sites_o] = True
+    assert np.array_equal(vis_g, vis_o)
+
+
+def test_c1_culled_lists_against_oracle_culling():
+    """The timed path's lists (sites that cannot shade left out, sites binned by
+    their balls) against lists the oracle derives from its own restatement
+    of the same predicates (orc.live_sites, orc.site_bounds,
+    orc.restrict_lists) on the FP32 encoded faces: the fp16 faces bit-exact,
+    the live masks equal up to faces within rounding of the margin, the
+    lists equal up to pairs at the rim of a ball's footprint."""
+    import torch
+    from powerfoam_amd.rasterize import Rasterizer
+    case = configs.c1(0)
+    cam, _ = case.pf_camera()
+    sp = case.spheres()
+    eye = case.cam.eye.numpy()
+    diff_o = orc.adj_encode(sp, case.adjacency, case.adjacency_offsets)
+    live_o, margin, src = orc.live_sites(eye, sp, case.adjacency, case.adjacency_offsets, diff_o, 0,
+                                         return_margin=True)
+    bounds_o, empty_o = orc.site_bounds(eye, sp, case.adjacency, case.adjacency_offsets, diff_o, 0, live_o)
+    keep_o = (live_o != 0) & ~empty_o
+    off_full, sites_full, _ = orc.bin_lists(cam, sp)
+    off_o, sites_o = orc.restrict_lists(cam, eye, case.cam.right.numpy(), case.cam.up.numpy(), off_full,
+                                        sites_full, bounds_o, keep_o)
+
+    r = Rasterizer(harness._args(8), "cuda")
+    d_sp = torch.tensor(sp, device="cuda")
+    adj = torch.tensor(case.adjacency, device="cuda")
+    off = torch.tensor(case.adjacency_offsets, device="cuda")
+    diff_g = r.encode_adjacency(d_sp, adj, off)
+    live_g, bounds_g = r.live_sites(case.cam, d_sp, adj, off, diff_g)
+    offsets_g, sites_g, vis_g, _ = r.bin(case.cam, d_sp, live_g, bounds_g)
+    torch.cuda.synchronize()
+    assert np.array_equal(diff_g.cpu().numpy().view(np.uint16), diff_o)
+    lg = live_g.cpu().numpy()
+    bg = bounds_g.cpu().numpy()
+    off_g, st_g = offsets_g.cpu().numpy(), sites_g.cpu().numpy()
+    # live masks: a difference only where a face decides within rounding
+    # (the GPU mask also drops the sites whose ball vanished)
+    diff_live = np.nonzero((lg != 0) != keep_o)[0]
+    near = np.zeros(case.n, bool)
+    near[src[np.abs(margin) < 1e-5]] = True
+    _log(f"c1 culling: {int(keep_o.sum())} of {case.n} sites kept by the oracle ({int(live_o.sum())} before the "
+         f"balls), {diff_live.size} differ ({int(near[diff_live].sum())} within 1e-5 of a face margin)")
+    assert diff_live.size <= 1e-4 * case.n
+    # balls: same centres and radii to FP32 rounding of the sweeps
+    both = (lg != 0) & (live_o != 0) & ~empty_o
+    rad_err = np.abs(bg[both, 3] - bounds_o[both, 3]) / np.maximum(bounds_o[both, 3], 1e-12)
+    ctr_err = np.linalg.norm(bg[both, :3] - bounds_o[both, :3], axis=1) / np.maximum(bounds_o[both, 3], 1e-12)
+    _log(f"c1 balls: relative radius error median {np.median(rad_err):.1e} p99.9 {np.quantile(rad_err, 0.999):.1e} "
+         f"max {rad_err.max():.1e}; centre error / radius max {ctr_err.max():.1e}")
+    # lists: per tile, the same sites up to the rim of a ball's footprint
+    nt = off_o.size - 1
+    tile_o = np.repeat(np.arange(nt), np.diff(off_o))
+    tile_g = np.repeat(np.arange(nt), np.diff(off_g))
+    key_o = tile_o.astype(np.int64) * case.n + sites_o
+    key_g = tile_g.astype(np.int64) * case.n + st_g
+    only_g = np.setdiff1d(key_g, key_o)
+    only_o = np.setdiff1d(key_o, key_g)
+    _log(f"c1 lists: gpu {st_g.size}, oracle {sites_o.size}, full {sites_full.size}; only gpu {only_g.size}, "
+         f"only oracle {only_o.size}")
+    assert only_g.size + only_o.size <= 1e-4 * sites_o.size
+    # visibility mask: the footprint's, as without culling
+    vis_o = np.zeros(case.n, bool)
+    vis_o[sites_full] = True
+    assert np.array_equal(vis_g.cpu().numpy(), vis_o)
+    # common entries keep the depth order
+    common = np.isin(key_g, key_o)
+    sel = np.isin(key_o, key_g)
+    assert np.array_equal(key_g[common], key_o[sel])
+
+
+def test_c1_frame_matches_oracle():
+    _compare_frame("c1 (1M sites, 1297x840, view 0)", configs.c1(0))
+
+
+# ------------------------------------------------------------------ C2
+def _fisheye_crop(eye, size, i0, j0, h, w):
+    import torch
+    from powerfoam_amd.camera import TorchCamera, fisheye_ray_maps, look_at
+    rm = fisheye_ray_maps(eye, (0.0, 0.0, 0.0), size)[i0:i0 + h, j0:j0 + w]
+    t = look_at(eye, (0.0, 0.0, 0.0), size, size, 90.0)
+    return TorchCamera(t.eye, t.right, t.up, w, h, torch.tensor(np.ascontiguousarray(rm)))
+
+
+@pytest.mark.parametrize("crop", [(448, 448), (40, 500), (900, 120)], ids=["centre", "rim-top", "rim-corner"])
+def test_c2_fisheye_tracer_matches_oracle(crop):
+    """The ray tracer on the 1M-site scene with its full power adjacency
+    through the 1024x1024 180-degree equidistant fisheye of view 0, exact
+    neighbour geometry as the tracer workload uses: on a 96x96 block of its
+    rays (the tracer's rays are independent, so a block of the ray map is the
+    same rays), visited cells (walked to the end of the scene) bit-exact
+    against the FP32 walk; colour, opacity, normals, depth and the gradients
+    of a random loss on every ray against FP64, with the FP32 mirror's
+    deviation as the yardstick where FP32 cannot reach the plain bars."""
+    import torch
+    from types import SimpleNamespace
+    from powerfoam_amd.raytrace import RayTracer, start_cell
+    s = configs.c1_scene()
+    eye = configs.c1_cameras()[0].eye.numpy().astype(np.float64)
+    cam = _fisheye_crop(eye, 1024, crop[0], crop[1], 96, 96)
+    rgb = configs.texel_rgb(s, eye)
+    dev = "cuda"
+    tt = lambda a, dt=torch.float32: torch.tensor(np.ascontiguousarray(a), dtype=dt, device=dev)
+    leaves = [tt(s[k]) if k != "texel_rgb" else tt(rgb) for k in ("points", "radii", "density", "normals",
+                                                                  "texel_sites", "texel_rgb", "texel_height")]
+    adj, off = tt(s["adj_full"], torch.int32), tt(s["off_full"], torch.int32)
+    st = start_cell(leaves[0], leaves[1], eye)
+    gcam = type(cam)(cam.eye, cam.right, cam.up, cam.width, cam.height, cam.ray_maps.to(dev))
+    tracer = RayTracer(SimpleNamespace(num_texel_sites=8, is_pinhole=True), dev)
+    # the walk compared to the end of the scene (no transmittance stop, whose
+    # last step depends on the shading's exp rounding)
+    cells, n_cells = tracer.trace_cells(gcam, st, *leaves, adj, off, None, 0.0, max_cells=2048)
+    for v in leaves:
+        v.requires_grad_(True)
+    col, opac, depth, normal = tracer.forward(gcam, st, *leaves, adj, off, None, 1e-3, 0.5)
+    ocam, _k = orc.camera(cam.eye.numpy(), cam.right.numpy(), cam.up.numpy(), cam.width, cam.height,
+                          cam.ray_maps.numpy())
+    sp = np.concatenate([s["points"], s["radii"][:, None]], 1)
+    ns = np.concatenate([s["normals"], s["density"][:, None]], 1)
+    rgbh = np.concatenate([rgb, s["texel_height"][..., None]], -1)
+    inp = orc.Inputs(sp, ns, s["texel_sites"], rgbh, s["adj_full"], s["off_full"], None)
+    walk = orc.trace_fwd(ocam, inp, st, threshold=0.0, record_cells=2048, precision="f32")
+    ref = orc.trace_fwd(ocam, inp, st, threshold=1e-3, depth_q=0.5)
+    nc = n_cells.cpu().numpy()
+    _log(f"c2 {crop}: start cell {st}, cells per ray mean {nc.mean():.1f} max {nc.max()}")
+    assert np.array_equal(nc, walk["n_cells"])
+    assert np.array_equal(cells.cpu().numpy(), walk["cells"])
+    r32 = orc.trace_fwd(ocam, inp, st, threshold=1e-3, depth_q=0.5, precision="f32")
+    c, o_, nrm, d = (x.detach().cpu().numpy() for x in (col, opac, normal, depth))
+    name = f"c2 {crop}"
+    _log(f"{name}: rays crossing an ill-conditioned cell {ref['fragile'].mean():.1%}")
+    # the walk takes the reference's FP32 formulas (its cell sequence must be
+    # the reference's), so the tracer may keep half of their deviation
+    _out_bar(name, "color", c, ref["color"], r32["color"], ratio=0.5)
+    _out_bar(name, "opacity", o_, 1 - np.exp(ref["log_t"]), 1 - np.exp(r32["log_t"]), ratio=0.5)
+    _out_bar(name, "normal", nrm, ref["normal"], r32["normal"], ratio=0.5)
+    defined = (d != -1) & (ref["depth"] != -1) & (r32["depth"] != -1)
+    mismatch = float(((d == -1) != (ref["depth"] == -1)).mean())
+    _log(f"{name}: depth defined on {(ref['depth'] != -1).mean():.1%} of rays, defined-or-not differs on "
+         f"{mismatch:.4%}")
+    assert mismatch <= 1e-3
+    _out_bar(name, "depth", d, ref["depth"], r32["depth"], defined, continuous=False, ratio=0.5)
+    H, W = cam.height, cam.width
+    rng = np.random.default_rng(5)
+    w = dict(color=rng.normal(size=(H, W, 3)), normal=rng.normal(size=(H, W, 3)), opacity=rng.normal(size=(H, W)),
+             depth=rng.normal(size=(H, W)))
+    tw = {k: torch.tensor(v, dtype=torch.float32, device=dev) for k, v in w.items()}
+    valid = depth.detach() != -1
+    L = (col * tw["color"]).sum() + (normal * tw["normal"]).sum() + (opac * tw["opacity"]).sum() + \
+        (torch.where(valid, depth, torch.zeros_like(depth)) * tw["depth"]).sum()
+    L.backward()
+    g = {p: orc.trace_bwd(ocam, inp, st, r, w["color"], w["normal"], np.where(r["depth"] != -1, w["depth"], 0.0),
+                          w["opacity"], threshold=1e-3, depth_q=0.5, precision=p)
+         for p, r in (("f64", ref), ("f32", r32))}
+    got = dict(grad_spheres=np.concatenate([leaves[0].grad.cpu().numpy(), leaves[1].grad.cpu().numpy()[:, None]], 1),
+               grad_nsigmas=np.concatenate([leaves[3].grad.cpu().numpy(), leaves[2].grad.cpu().numpy()[:, None]], 1),
+               grad_texel_sites=leaves[4].grad.cpu().numpy(), grad_texel_rgb=leaves[5].grad.cpu().numpy(),
+               grad_texel_height=leaves[6].grad.cpu().numpy())
+    every = np.ones(s["points"].shape[0], bool)
+    for k in harness.GRAD_KEYS:
+        _grad_bar(name, k, got[k], g["f64"][k], g["f32"][k], every)
+
+
+# ------------------------------------------------------------------ C3
+def test_c3_learned_head_rows_and_render_psnr():
+    """The 3M-site scene (seed 2): the MLP head trained on the GPU with its own
+    kernels on SH-labelled (site, direction) pairs; its texel rows for one
+    view against the FP64 oracle network on a sample of sites; and the
+    render PSNR of the MLP head against the SH head on two views, asserted at
+    the level it reaches and above the view-independent DC colour (what the
+    head adds over the method without it)."""
+    import torch
+    from oracle import mlp as om
+    from powerfoam_amd.color_fn import SHColor
+    from powerfoam_amd.mlp_head import MLPColor, fit, latents_from_sh, psnr
+    from powerfoam_amd.rasterize import Rasterizer
+    s = configs.c3_scene()
+    dev = "cuda"
+    tt = lambda a, dt=torch.float32: torch.tensor(np.ascontiguousarray(a), dtype=dt, device=dev)
+    sh = tt(s["sh"])
+    head = MLPColor(sh.shape[0], dev, seed=0, latents=latents_from_sh(sh))
+    t0 = time.time()
+    hist = fit(head, sh, steps=2000, batch=65536, log_every=500)
+    torch.cuda.synchronize()
+    _log(f"c3: trained the MLP head in {time.time() - t0:.1f}s, loss {hist[0][1]:.4f} -> {hist[-1][1]:.4f}")
+    cams = configs.c1_cameras()
+    ts = tt(s["texel_sites"])
+    # rows against the oracle network on a sample of sites
+    rgb = head.forward(ts, cams[0]).detach()
+    rng = np.random.default_rng(3)
+    pick = np.sort(rng.choice(sh.shape[0], 4096, replace=False))
+    p = {k: v.detach().cpu().numpy().astype(np.float64) for k, v in head.params.items()}
+    sites, dirs = om.rows_texels(s["texel_sites"][pick], cams[0].eye.numpy(), 8)
+    ref, _ = om.forward(p, pick[sites], dirs)
+    got = rgb[torch.tensor(pick, device=dev)].cpu().numpy().reshape(-1, 3)
+    err = np.abs(got - ref)
+    _log(f"c3: MLP rows vs oracle on {pick.size} sites: max abs {err.max():.2e}, mean {err.mean():.2e}")
+    assert err.max() < 5e-3 and err.mean() < 5e-5
+    # render PSNR of each head against the SH render
+    r = Rasterizer(harness._args(8), dev)
+    adj, off = tt(s["adj"], torch.int32), tt(s["off"], torch.int32)
+    x = {k: tt(s[k]) for k in ("points", "radii", "density", "normals", "texel_height")}
+    sh_head = SHColor()
+    vals = dict(mlp=[], dc=[])
+    with torch.no_grad():
+        for cam in (cams[0], cams[16]):
+            imgs = {}
+            for name, fn in (("sh", lambda c: sh_head.forward(ts, c, sh)), ("mlp", lambda c: head.forward(ts, c)),
+                             ("dc", lambda c: torch.clamp(sh[:, 0, :], min=0.0)[:, None, :].expand(-1, 8, 3)
+                              .contiguous())):
+                out = r.forward(cam, None, x["points"], x["radii"], x["density"], x["normals"], ts, fn(cam).detach(),
+                                x["texel_height"], adj, off, None, False)
+                imgs[name] = out[0]
+            vals["mlp"].append(psnr(imgs["mlp"], imgs["sh"]))
+            vals["dc"].append(psnr(imgs["dc"], imgs["sh"]))
+    p_mlp, p_dc = float(np.mean(vals["mlp"])), float(np.mean(vals["dc"]))
+    _log(f"c3: render PSNR against the SH render: MLP {p_mlp:.2f} dB, DC colour {p_dc:.2f} dB")
+    assert p_mlp > 32.0
+    assert p_mlp > p_dc + 0.3

@@ -37,8 +37,8 @@ def _gpu_live(sp, adj, off, cam_t, form, with_bounds=False):
     d_off = torch.tensor(np.asarray(off, np.int32), device="cuda")
     if form == "radius":
         diff = torch.empty((max(len(adj), 1), 4), dtype=torch.float16, device="cuda")
-        _abi.check(lib.pf_adj_encode(_abi.ptr(d_sp), sp.shape[0], _abi.ptr(d_adj), _abi.ptr(d_off), _abi.ptr(diff),
-                                     None))
+        _abi.check(lib.pf_adj_encode(_abi.ptr(d_sp), sp.shape[0], _abi.ptr(d_adj), _abi.ptr(d_off), len(adj),
+                                     _abi.ptr(diff), None))
     elif form == "power":
         diff = power_face_encoding(d_sp[:, :3], d_sp[:, 3], d_adj, d_off)
     else:
@@ -48,7 +48,7 @@ def _gpu_live(sp, adj, off, cam_t, form, with_bounds=False):
     bounds = torch.full((sp.shape[0], 4), float("nan"), device="cuda") if with_bounds else None
     kind = _abi.PF_FACES_POWER if form == "power" else _abi.PF_FACES_RADIUS
     _abi.check(lib.pf_live_sites(ctypes.byref(cam), _abi.ptr(d_sp), sp.shape[0], _abi.ptr(d_adj), _abi.ptr(d_off),
-                                 _abi.ptr(diff), kind, _abi.ptr(live), _abi.ptr(bounds), None))
+                                 len(adj), _abi.ptr(diff), kind, _abi.ptr(live), _abi.ptr(bounds), None))
     torch.cuda.synchronize()
     dh = None if diff is None else diff.cpu().numpy().view(np.uint16)[: len(adj)]
     if with_bounds:
