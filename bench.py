@@ -11,12 +11,18 @@ adjoint). All scene tensors are resident in HBM before the timed region.
 
 Prints one JSON line (rank 0). `value` is rays per second (Mrays/s) over the
 timed frames; `ms_per_step` is ms per frame. The roofline entry is for the
-kernel that takes the most time, with algorithmic bytes per launch from
+kernel that takes the most time: algorithmic bytes per launch from
 `algorithmic_bytes` (per-pair, per-pixel and per-site byte counts applied to
-the frame's measured workload counts) divided by that kernel's average
-duration measured with HIP events on its launch stream.
-The CPU baseline is the FP32 CPU restatement of the same algorithm (the
-oracle, OpenMP) on one frame of the same workload.
+the measured workload counts of the views the timed frames rendered),
+divided by that kernel's average duration measured with HIP events on its
+launch stream; beside it the VALU issue fraction (wave-instructions per
+launch from a committed rocprofv3 counter run of the same kernels and views,
+over a VALU peak the run measures itself with an FMA kernel). The default
+line also carries the learned-head record (3M-site scene, MLP head trained
+on the GPU, fwd+bwd frames with each head, render PSNR against the SH
+render, MFMA utilisation of the head's kernels). The CPU baseline is the
+FP32 CPU restatement of the same algorithm (the oracle, OpenMP) on a
+bounded sample of the same workload.
 """
 
 import argparse
@@ -34,6 +40,8 @@ if ROOT not in sys.path:
 
 METRIC = "fwd+bwd ms/frame @1M sites 1297x840; Mrays/s; MLP vs SH head at PSNR≥45dB"
 HBM_PEAK_GBS = 8000.0
+MFMA_BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16
+LATENT, HIDDEN = 16, 64
 
 
 def log(msg):
@@ -119,14 +127,14 @@ def time_frames(rast, head, w, steps, heads_log=None):
     return elapsed, {k: float(np.mean(v)) for k, v in kern.items()}
 
 
-def render_psnr(rast, w, head_a, head_b):
+def render_psnr(rast, w, head_a, head_b, cams=None):
     """Mean over views of the PSNR of head_a's render against head_b's (forward only)."""
     import torch
     from powerfoam_amd.mlp_head import psnr
     p = w["params"]
     vals = []
     with torch.no_grad():
-        for cam in w["cams"]:
+        for cam in (cams if cams is not None else w["cams"]):
             imgs = []
             for h in (head_a, head_b):
                 rgb = h.forward(p["texel_sites"], cam, p["sh"]).detach()
@@ -137,14 +145,30 @@ def render_psnr(rast, w, head_a, head_b):
     return float(np.mean(vals))
 
 
-def run_mlp_head(args, rast, w, dev):
-    """Learned colour head workload: train on SH-labelled pairs, then time
-    fwd+bwd frames with the MLP head against the SH head on the same scene and
-    report the MLP render's PSNR against the SH render."""
+def mlp_flops_per_row():
+    """Algorithmic FLOPs per texel row of the learned head: forward
+    2 (64*32 + 64*64 + 3*64); backward the recomputed forward plus the input
+    gradients of layers 3 and 2 and of the latent slots of layer 1, and the
+    three weight gradients."""
+    fwd = 2 * (HIDDEN * 2 * LATENT + HIDDEN * HIDDEN + 3 * HIDDEN)
+    bwd = fwd + 2 * (3 * HIDDEN + HIDDEN * HIDDEN + HIDDEN * LATENT) + \
+        2 * (3 * HIDDEN + HIDDEN * HIDDEN + HIDDEN * 2 * LATENT)
+    return fwd, bwd
+
+
+def mlp_head_record(args, dev, psnr_views=4):
+    """Learned colour head on the 3M-site scene (seed 2): train on SH-labelled
+    (site, direction) pairs with the head's own kernels, render PSNR of the
+    MLP head (and of the view-independent DC colour) against the SH render,
+    then fwd+bwd frames with each head. Returns (record, ms per MLP frame)."""
     import torch
+    from types import SimpleNamespace
     from powerfoam_amd.color_fn import SHColor
     from powerfoam_amd.mlp_head import MLPColor, fit, latents_from_sh
+    from powerfoam_amd.rasterize import Rasterizer
 
+    w = build_workload(args.mlp_sites, args.views, args.width, args.height, dev, seed=2)
+    rast = Rasterizer(SimpleNamespace(num_texel_sites=8, render_objective="volume", is_pinhole=True), dev)
     sh = w["params"]["sh"].detach()
     mlp = MLPColor(sh.shape[0], dev, seed=0, latents=latents_from_sh(sh))
     t0 = time.time()
@@ -153,34 +177,45 @@ def run_mlp_head(args, rast, w, dev):
     train_s = time.time() - t0
     log(f"trained the MLP head: {args.mlp_steps} steps x {args.mlp_batch} pairs in {train_s:.1f}s, loss {hist}")
     sh_head, mlp_head = SHColor(), MlpSlot(mlp)
-    psnr_mlp = render_psnr(rast, w, mlp_head, sh_head)
 
     class _DC:
         def forward(self, texel_sites, camera, sh_):
             n, T = texel_sites.shape[:2]
             return torch.clamp(sh_[:, 0, :].detach(), min=0.0)[:, None, :].expand(n, T, 3).contiguous()
 
-    psnr_dc = render_psnr(rast, w, _DC(), sh_head)
-    log(f"render PSNR vs the SH render: MLP {psnr_mlp:.2f} dB, view-independent DC {psnr_dc:.2f} dB")
+    cams = w["cams"][:psnr_views]
+    psnr_mlp = render_psnr(rast, w, mlp_head, sh_head, cams)
+    psnr_dc = render_psnr(rast, w, _DC(), sh_head, cams)
+    log(f"render PSNR vs the SH render ({len(cams)} views): MLP {psnr_mlp:.2f} dB, view-independent DC "
+        f"{psnr_dc:.2f} dB")
+    steps = args.mlp_frames
     for i in range(args.warmup):
         frame(rast, sh_head, w, w["cams"][i % len(w["cams"])])
         frame(rast, mlp_head, w, w["cams"][i % len(w["cams"])])
-    sh_s, sh_k = time_frames(rast, sh_head, w, args.steps)
-    mlp_s, mlp_k = time_frames(rast, mlp_head, w, args.steps, heads_log=mlp)
-    P = args.width * args.height
-    ms = mlp_s * 1e3 / args.steps
-    line = dict(metric=METRIC, value=P * args.steps / mlp_s / 1e6, unit="Mrays/s", n_gpus=1, steps=args.steps,
-                warmup=args.warmup, ms_per_step=ms, higher_is_better=True, scaling="weak", vs_baseline=None,
-                dtype="bf16 head / f32 geometry", data="synthetic (seeded scene generator, random target image)",
-                config=dict(workload=f"rasterizer fwd+bwd with the learned MLP colour head vs the SH-3 head, "
-                                     f"{args.sites}-site GMM scene (seed 2), {args.views} pinhole views "
-                                     f"{args.width}x{args.height}",
-                            sites=args.sites, width=args.width, height=args.height, views=args.views,
-                            training_pairs=args.mlp_steps * args.mlp_batch),
-                sh_head_ms_per_step=sh_s * 1e3 / args.steps, speedup_vs_sh_head=sh_s / mlp_s,
-                psnr_vs_sh_render_db=psnr_mlp, psnr_dc_only_vs_sh_render_db=psnr_dc,
-                kernel_ms_mlp=mlp_k, kernel_ms_sh=sh_k, train_seconds=train_s, train_loss=hist)
-    print(json.dumps(line), flush=True)
+    sh_s, sh_k = time_frames(rast, sh_head, w, steps)
+    mlp_s, mlp_k = time_frames(rast, mlp_head, w, steps, heads_log=mlp)
+    rows = sh.shape[0] * 8
+    f_fwd, f_bwd = mlp_flops_per_row()
+    mfma = {}
+    for name, f in (("mlp_fwd", f_fwd), ("mlp_bwd", f_bwd)):
+        if name in mlp_k:
+            ach = f * rows / (mlp_k[name] * 1e-3) / 1e12
+            mfma[name] = dict(flops_per_launch=f * rows, ms=mlp_k[name], achieved=ach, peak=MFMA_BF16_PEAK_TFLOPS,
+                              unit="TFLOP/s", frac=ach / MFMA_BF16_PEAK_TFLOPS)
+    rec = dict(workload=f"{args.mlp_sites}-site GMM scene (seed 2), {args.views} pinhole views "
+                        f"{args.width}x{args.height}, fwd+bwd per frame with each colour head",
+               mlp_ms_per_frame=mlp_s * 1e3 / steps, sh_ms_per_frame=sh_s * 1e3 / steps,
+               speedup_vs_sh_head=sh_s / mlp_s, frames=steps,
+               psnr_vs_sh_render_db=psnr_mlp, psnr_dc_only_vs_sh_render_db=psnr_dc, psnr_views=len(cams),
+               training_pairs=args.mlp_steps * args.mlp_batch, train_seconds=train_s,
+               train_loss_first_last=[hist[0][1], hist[-1][1]] if hist else None,
+               kernel_ms_mlp=mlp_k, kernel_ms_sh=sh_k, mfma=mfma,
+               head="16-dim bf16 latent + SH-basis direction encoding -> 64 -> 64 -> 3, bf16 MFMA, fp32 accumulate")
+    log(f"learned head: MLP {rec['mlp_ms_per_frame']:.3f} ms vs SH {rec['sh_ms_per_frame']:.3f} ms per frame; "
+        f"MFMA {({k: round(v['frac'], 4) for k, v in mfma.items()})}")
+    del w, rast, mlp
+    torch.cuda.empty_cache()
+    return rec, mlp_s * 1e3 / steps
 
 
 def run_trace(args, dev):
@@ -384,6 +419,48 @@ def host_cpu():
     return f"{model}, nproc {len(os.sched_getaffinity(0))}, os.cpu_count {os.cpu_count()}"
 
 
+def valu_peak(dev, n_blocks=4096, iters=4096, reps=3):
+    """Wave64 VALU instructions per second the GPU issues on independent FP32
+    FMA chains (pf_probe_valu), best of `reps` timed launches."""
+    import torch
+    from powerfoam_amd import _abi
+    from powerfoam_amd._abi import ptr, stream_handle
+    lib = _abi.load()
+    out = torch.empty(n_blocks * 256, dtype=torch.float32, device=dev)
+    best = 0.0
+    for r in range(reps + 1):
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        n = lib.pf_probe_valu(n_blocks, iters, ptr(out), stream_handle())
+        e.record()
+        if n < 0:
+            _abi.check(int(n))
+        torch.cuda.synchronize()
+        if r > 0:  # the first launch warms up
+            best = max(best, n / (s.elapsed_time(e) * 1e-3))
+    return best
+
+
+def timed_views(steps, n_views):
+    """View index of each timed frame (frames cycle through the cameras)."""
+    return [i % n_views for i in range(steps)]
+
+
+def per_view_profile(name, kernel, key, views):
+    """Mean over the timed views of a per-view figure from a committed
+    rocprofv3 counter summary (profiles/<name>), with its source; None if
+    the summary is missing."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        d = json.load(f)
+    vals = d.get("kernels", {}).get(kernel, {}).get(key)
+    if not vals:
+        return None, None
+    return float(np.mean([vals[v % len(vals)] for v in views])), f"profiles/{name}: {d.get('command', '')}"
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -394,17 +471,19 @@ def main():
     ap.add_argument("--width", type=int, default=1297)
     ap.add_argument("--height", type=int, default=840)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mlp-head", action="store_true",
+                    help="leave the learned-head record (3M-site scene) out of the default line")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run 3 device copies of this many MiB (counter calibration)")
     ap.add_argument("--head", choices=("sh", "mlp"), default="sh",
-                    help="mlp: the learned-head workload (3M-site scene, MLP vs SH head, PSNR)")
+                    help="mlp: only the learned-head workload (3M-site scene, MLP vs SH head, PSNR)")
+    ap.add_argument("--mlp-sites", type=int, default=3_000_000)
     ap.add_argument("--mlp-steps", type=int, default=2000)
     ap.add_argument("--mlp-batch", type=int, default=65536)
+    ap.add_argument("--mlp-frames", type=int, default=16)
     ap.add_argument("--workload", choices=("raster", "trace"), default="raster",
                     help="raster: the rasterizer frame (default); trace: the ray tracer on fisheye views")
     args = ap.parse_args()
-    if args.head == "mlp" and args.sites == 1_000_000:
-        args.sites = 3_000_000
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: the renderer hot path is built and timed on a single MI355X; "
@@ -422,11 +501,20 @@ def main():
     dev = torch.device("cuda:0")
     if args.workload == "trace":
         return run_trace(args, dev)
-    w = build_workload(args.sites, args.views, args.width, args.height, dev, seed=2 if args.head == "mlp" else 1)
+    if args.head == "mlp":
+        rec, ms = mlp_head_record(args, dev)
+        P = args.width * args.height
+        line = dict(metric=METRIC, value=P / ms / 1e3, unit="Mrays/s", n_gpus=1, steps=args.mlp_frames,
+                    warmup=args.warmup, ms_per_step=ms, higher_is_better=True, scaling="weak", vs_baseline=None,
+                    dtype="bf16 head / f32 geometry", data="synthetic (seeded scene generator, random target image)",
+                    config=dict(workload="rasterizer fwd+bwd with the learned MLP colour head",
+                                sites=args.mlp_sites, width=args.width, height=args.height, views=args.views),
+                    mlp_head=rec)
+        print(json.dumps(line), flush=True)
+        return
+    w = build_workload(args.sites, args.views, args.width, args.height, dev, seed=1)
     rast = Rasterizer(SimpleNamespace(num_texel_sites=8, render_objective="volume", is_pinhole=True,
                                       disable_coop_prim_load=False, disable_coop_adj_load=False), dev)
-    if args.head == "mlp":
-        return run_mlp_head(args, rast, w, dev)
     head = SHColor()
     cams = w["cams"]
 
@@ -444,8 +532,6 @@ def main():
     for i in range(args.warmup):
         frame(rast, head, w, cams[i % len(cams)])
     torch.cuda.synchronize()
-    counts = workload_counts(rast, w, args.width, args.height)
-    log(f"workload counts (last warmup frame): {counts}")
 
     log(f"timing {args.steps} frames")
     rast.event_log = []
@@ -466,7 +552,8 @@ def main():
     value = P * args.steps / elapsed / 1e6
     log(f"{ms:.3f} ms/frame, {value:.1f} Mrays/s; kernels {kern_ms}")
 
-    # counts of every view for the roofline (kernel time averages over views)
+    # workload counts of every view; the roofline averages them over the
+    # views the timed frames rendered, as the kernel times do
     per_view = []
     sp_all = torch.cat([w["params"]["points"], w["params"]["radii"][:, None]], 1).detach().float().contiguous()
     for v in range(len(cams)):
@@ -476,19 +563,29 @@ def main():
         # reference bins)
         per_view[-1]["M_all"] = int(rast.bin(cams[v], sp_all)[0][-1])
     torch.cuda.synchronize()
-    avg = {k: float(np.mean([c[k] for c in per_view])) for k in per_view[0]}
-    order = [cams[i % len(cams)] for i in range(args.steps)]
-    del order
+    views = timed_views(args.steps, len(cams))
+    avg = {k: float(np.mean([per_view[v][k] for v in views])) for k in per_view[0]}
     abytes = algorithmic_bytes(avg)
     dom = max(kern_ms, key=kern_ms.get)
     achieved = abytes[dom] / (kern_ms[dom] * 1e-3) / 1e9
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "raster_c1_traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = json.load(f)["kernels"].get(dom, {}).get("bytes")
+    traffic, traffic_src = per_view_profile("raster_c1_traffic.json", dom, "per_view_bytes", views)
     roofline = dict(bound="hbm", kernel=dom, achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=achieved / HBM_PEAK_GBS, traffic=traffic, algorithmic_bytes=abytes[dom])
+                    frac=achieved / HBM_PEAK_GBS, traffic=traffic, algorithmic_bytes=abytes[dom],
+                    traffic_source=traffic_src)
+    peak_valu = valu_peak(dev)
+    log(f"VALU peak (FMA probe): {peak_valu:.4e} wave64 instructions/s")
+    valu = {}
+    for k in ("raster_fwd", "raster_bwd"):
+        insts, src = per_view_profile("raster_c1_valu.json", k, "per_view_valu_insts", views)
+        if insts is None or k not in kern_ms:
+            continue
+        ach = insts / (kern_ms[k] * 1e-3)
+        valu[k] = dict(achieved=ach, peak=peak_valu, unit="wave64 VALU instructions/s", frac=ach / peak_valu,
+                       insts_per_launch=insts, source=src)
+    roofline["valu"] = valu
+    roofline["bound_note"] = ("compositing does ~300-500 VALU wave-instructions per (tile, site) pair against "
+                              "~300 B of site data shared by 64 rays: VALU-bound; the HBM fraction is reported "
+                              "as asked, the VALU fraction beside it")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -506,6 +603,10 @@ def main():
                             pairs_per_frame=avg["M"], pairs_per_frame_all_sites=avg["M_all"],
                             processed_pairs_per_frame=avg["pairs"]),
                 kernel_ms=kern_ms, roofline=roofline, cpu_baseline=cpu)
+    if not args.no_mlp_head:
+        del w, rast
+        torch.cuda.empty_cache()
+        line["mlp_head"], _ = mlp_head_record(args, dev)
     print(json.dumps(line), flush=True)
 
 

@@ -473,6 +473,14 @@ int pf_pack_sites(const float* points, const float* radii, const float* normals,
 /* Replaces non-finite entries with 0 (rasterize.py:1954-1958). */
 int pf_scrub_nonfinite(float* data, int64_t n, void* stream);
 
+/* Measurement only, no reference counterpart: the VALU issue peak that the
+ * compositing kernels' instruction counts are set against. Launches n_blocks
+ * workgroups of 256 threads, each lane running eight independent FP32 FMA
+ * chains of `iters` steps; writes one float per thread to `out`
+ * [n_blocks*256]. Returns the number of v_fma_f32 wave64 instructions the
+ * launch issues (time it on `stream`), or a negative pf_status. */
+int64_t pf_probe_valu(int32_t n_blocks, int32_t iters, float* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
