@@ -470,6 +470,79 @@ int pf_pack_sites(const float* points, const float* radii, const float* normals,
                   const float* texel_rgb, const float* texel_height, int32_t n_sites, int32_t n_texels,
                   float* spheres, float* nsigmas, float* texel_rgbh, void* stream);
 
+/* ---- Spherical-Voronoi texel colour (the reference's colour head,
+ *      SphericalVoronoi color_fn.py:27-315) ------------------------------
+ * Per texel point p with d = normalize(p - eye): grey 0.5 when
+ * d . forward < fov_cos_cutoff, else
+ *   w_i = exp(-temp_i |d - axis_i|), rgb = max(0, sum_i w_i val_i / sum_i w_i + 0.5)
+ * over sv_dof lobes (color_fn.py:54-101). Lobe arrays are lobe-major as the
+ * reference's get_att_sv lays them out (scene.py:290-312): att_sites and
+ * att_values [sv_dof, P, 3], att_temps [sv_dof, P]. */
+typedef struct pf_sv_args {
+  int32_t n_points;         /* P (texels) */
+  int32_t sv_dof;           /* lobes */
+  const float* points;      /* [P,3] texel sites */
+  float eye[3];
+  float forward[3];         /* unit normalize(up x right) */
+  float fov_cos_cutoff;
+  const float* att_sites;   /* [D,P,3] unit lobe axes */
+  const float* att_values;  /* [D,P,3] */
+  const float* att_temps;   /* [D,P] */
+} pf_sv_args;
+
+/* values [P,3], weights_sum [P] (0 where cut off; the backward reads it). */
+int pf_sv_fwd(const pf_sv_args* args, float* values, float* weights_sum, void* stream);
+/* color_fn.py:105-184: gradients of the lobe axes, values and temperatures,
+ * written (not accumulated; zero for cut-off texels and for lobes whose
+ * normalised weight is below 1e-3); no gradient to the points. */
+int pf_sv_bwd(const pf_sv_args* args, const float* weights_sum, const float* values, const float* grad_values,
+              float* grad_att_sites, float* grad_att_values, float* grad_att_temps, void* stream);
+
+/* ---- scene glue (scene.py:239-337): the rasterizer's per-site inputs from
+ *      the trainable parameters, forward and adjoint, one launch each -------
+ *   radii   = softplus(radii_raw, beta=100), density likewise (torch's
+ *             softplus: x where beta x > 20)               scene.py:239-240, 287
+ *   normal, tangent, bitangent = columns of R(q / |q|), each normalised
+ *                                                            scene.py:242-285
+ *   texel_sites[n,k] = points[n] + radii[n] (off[n,k,0] t + off[n,k,1] b)
+ *                                                            scene.py:321-329
+ *   texel_height[n,k] = height_raw[n,k] radii[n]            scene.py:337 */
+typedef struct pf_glue_params {
+  const float* points;        /* [N,3] */
+  const float* radii;         /* [N]   raw (before softplus) */
+  const float* density;       /* [N]   raw */
+  const float* quaternions;   /* [N,4] (w, x, y, z), 16-byte aligned */
+  const float* texel_offsets; /* [N,T,2] in units of the radius, 8-byte aligned */
+  const float* texel_height;  /* [N,T] in units of the radius */
+} pf_glue_params;
+
+/* Forward outputs; also the incoming gradients of the adjoint (read only
+ * there, any may be NULL = zero). */
+typedef struct pf_glue_outputs {
+  float* radii;        /* [N] */
+  float* density;      /* [N] */
+  float* normals;      /* [N,3] */
+  float* texel_sites;  /* [N,T,3] */
+  float* texel_height; /* [N,T] */
+} pf_glue_outputs;
+
+/* Gradients of the parameters, written (not accumulated). `points` holds the
+ * texel sites' share only: a caller that also feeds the points to the
+ * rasterizer adds that gradient. */
+typedef struct pf_glue_grads {
+  float* points;        /* [N,3] */
+  float* radii;         /* [N] */
+  float* density;       /* [N] */
+  float* quaternions;   /* [N,4], 16-byte aligned */
+  float* texel_offsets; /* [N,T,2], 8-byte aligned */
+  float* texel_height;  /* [N,T] */
+} pf_glue_grads;
+
+int pf_glue_fwd(const pf_glue_params* params, int32_t n_sites, int32_t n_texels, const pf_glue_outputs* out,
+                void* stream);
+int pf_glue_bwd(const pf_glue_params* params, int32_t n_sites, int32_t n_texels, const pf_glue_outputs* grad_out,
+                const pf_glue_grads* grad_params, void* stream);
+
 /* Replaces non-finite entries with 0 (rasterize.py:1954-1958). */
 int pf_scrub_nonfinite(float* data, int64_t n, void* stream);
 

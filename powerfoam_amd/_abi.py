@@ -211,6 +211,34 @@ class PfTraceBwdArgs(ctypes.Structure):
         "grad_texel_height")]
 
 
+class PfSvArgs(ctypes.Structure):
+    _fields_ = [
+        ("n_points", ctypes.c_int32),
+        ("sv_dof", ctypes.c_int32),
+        ("points", ctypes.c_void_p),
+        ("eye", ctypes.c_float * 3),
+        ("forward", ctypes.c_float * 3),
+        ("fov_cos_cutoff", ctypes.c_float),
+        ("att_sites", ctypes.c_void_p),
+        ("att_values", ctypes.c_void_p),
+        ("att_temps", ctypes.c_void_p),
+    ]
+
+
+class PfGlueParams(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_void_p) for k in ("points", "radii", "density", "quaternions", "texel_offsets",
+                                               "texel_height")]
+
+
+class PfGlueOutputs(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_void_p) for k in ("radii", "density", "normals", "texel_sites", "texel_height")]
+
+
+class PfGlueGrads(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_void_p) for k in ("points", "radii", "density", "quaternions", "texel_offsets",
+                                               "texel_height")]
+
+
 # Exported C symbols and their ctypes signatures (argtypes, restype).
 _VP = ctypes.c_void_p
 SIGNATURES = {
@@ -244,6 +272,10 @@ SIGNATURES = {
     "pf_pack_sites": ([_VP] * 6 + [ctypes.c_int32, ctypes.c_int32] + [_VP] * 4, ctypes.c_int),
     "pf_scrub_nonfinite": ([_VP, ctypes.c_int64, _VP], ctypes.c_int),
     "pf_probe_valu": ([ctypes.c_int32, ctypes.c_int32, _VP, _VP], ctypes.c_int64),
+    "pf_glue_fwd": ([_VP, ctypes.c_int32, ctypes.c_int32, _VP, _VP], ctypes.c_int),
+    "pf_sv_fwd": ([_VP, _VP, _VP, _VP], ctypes.c_int),
+    "pf_sv_bwd": ([_VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP], ctypes.c_int),
+    "pf_glue_bwd": ([_VP, ctypes.c_int32, ctypes.c_int32, _VP, _VP, _VP], ctypes.c_int),
     "pf_sh_fwd": ([_VP, _VP, ctypes.c_int32, ctypes.c_int32, _VP, _VP, _VP], ctypes.c_int),
     "pf_sh_bwd": ([_VP, _VP, _VP, ctypes.c_int32, ctypes.c_int32, _VP, _VP, _VP], ctypes.c_int),
 }
