@@ -408,6 +408,35 @@ int pf_raster_vis(const pf_raster_vis_args* args, void* stream);
 int pf_trace_fwd(const pf_trace_args* args, void* stream);
 int pf_trace_bwd(const pf_trace_bwd_args* args, void* stream);
 
+/* FP64 recomputation of the walk on sampled rays (no reference counterpart:
+ * the precision check of the FP32 walk at full scene size). For each ray
+ * (pixel index into cam's image, the ray generated as pf_trace_fwd does) and
+ * each cell the FP32 walk visited (cells/n_cells as pf_trace_fwd records
+ * them, max_cells per ray), in FP64 from the FP32 scene values: the cell's
+ * entry and exit distances (sphere interval cut by the exact power faces of
+ * scene.adjacency; NaN where the sphere is missed or the step was not
+ * taken), the next cell the FP64 walk would choose (argmin exit-face time,
+ * first in list order on ties; 0x7FFFFFFF none, -1 past the walk) and the
+ * gap between the smallest and second-smallest exit-face times, with the
+ * sum of the FP32 rounding bounds of those two times (the unit roundoff
+ * times the magnitudes t = (pm - o.n) / (d.n) is formed from): where the
+ * gap is below it, the FP32 walk may order the two faces either way. */
+typedef struct pf_trace_check_args {
+  pf_camera cam;
+  pf_scene scene;            /* spheres, adjacency, adjacency_offsets */
+  int32_t n_rays;
+  const int32_t* pixels;     /* [R] row-major pixel index i*W + j */
+  const int32_t* cells;      /* [R,K] visited cells of those rays */
+  const int32_t* n_cells;    /* [R] */
+  int32_t max_cells;         /* K */
+  double* t_enter;           /* [R,K] */
+  double* t_exit;            /* [R,K] */
+  int32_t* next_cell;        /* [R,K] */
+  double* exit_gap;          /* [R,K] */
+  double* exit_tol;          /* [R,K] */
+} pf_trace_check_args;
+int pf_trace_check_f64(const pf_trace_check_args* args, void* stream);
+
 /* ---- learned colour head (no reference counterpart) -------------------
  * A shared MLP fills the texel-colour slot that the SH head (pf_sh_fwd) and
  * the reference's SphericalVoronoi pre-pass (color_fn.py:54-184) fill: for a

@@ -211,6 +211,13 @@ class PfTraceBwdArgs(ctypes.Structure):
         "grad_texel_height")]
 
 
+class PfTraceCheckArgs(ctypes.Structure):
+    _fields_ = [("cam", PfCamera), ("scene", PfScene), ("n_rays", ctypes.c_int32), ("pixels", ctypes.c_void_p),
+                ("cells", ctypes.c_void_p), ("n_cells", ctypes.c_void_p), ("max_cells", ctypes.c_int32),
+                ("t_enter", ctypes.c_void_p), ("t_exit", ctypes.c_void_p), ("next_cell", ctypes.c_void_p),
+                ("exit_gap", ctypes.c_void_p), ("exit_tol", ctypes.c_void_p)]
+
+
 class PfSvArgs(ctypes.Structure):
     _fields_ = [
         ("n_points", ctypes.c_int32),
@@ -269,6 +276,7 @@ SIGNATURES = {
     "pf_mlp_bwd": ([_VP, _VP, _VP, _VP, _VP], ctypes.c_int),
     "pf_trace_fwd": ([_VP, _VP], ctypes.c_int),
     "pf_trace_bwd": ([_VP, _VP], ctypes.c_int),
+    "pf_trace_check_f64": ([_VP, _VP], ctypes.c_int),
     "pf_pack_sites": ([_VP] * 6 + [ctypes.c_int32, ctypes.c_int32] + [_VP] * 4, ctypes.c_int),
     "pf_scrub_nonfinite": ([_VP, ctypes.c_int64, _VP], ctypes.c_int),
     "pf_probe_valu": ([ctypes.c_int32, ctypes.c_int32, _VP, _VP], ctypes.c_int64),

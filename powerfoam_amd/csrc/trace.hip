@@ -602,6 +602,94 @@ static TraceParams make_params(const pf_trace_args* a) {
   return p;
 }
 
+// ---- FP64 recomputation of the walk on sampled rays ----------------------
+// For ray r (pixel pixels[r], generated exactly as the walk generates it) and
+// each cell the FP32 walk visited, the cell's interval in FP64: the sphere
+// interval (no hit when |o - c| < 4r, as the walk) cut by every face plane,
+// entering faces (d.n < 0) raising the entry, leaving faces lowering the exit;
+// the FP64 choice of the next cell (argmin exit-face time, first in list order
+// on ties) and the gap to the second-smallest exit-face time.
+__global__ void __launch_bounds__(256)
+    trace_check_f64_kernel(TraceParams p, int n_rays, const int32_t* __restrict__ pixels,
+                           const int32_t* __restrict__ cells, const int32_t* __restrict__ n_cells, int K,
+                           double* __restrict__ t_enter, double* __restrict__ t_exit, int32_t* __restrict__ next64,
+                           double* __restrict__ gap, double* __restrict__ tol) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n_rays) return;
+  const int px = pixels[r];
+  f3 of, df;
+  walk_ray(p, px / p.W, px % p.W, of, df);
+  const double ox = of.x, oy = of.y, oz = of.z, dx = df.x, dy = df.y, dz = df.z;
+  const int steps = min(n_cells[r], K);
+  for (int k = 0; k < K; k++) {
+    const size_t q = (size_t)r * K + k;
+    if (k >= steps) {
+      t_enter[q] = t_exit[q] = __builtin_nan("");
+      next64[q] = -1;
+      gap[q] = __builtin_nan("");
+      tol[q] = __builtin_nan("");
+      continue;
+    }
+    const int prim = cells[q];
+    const float4 s4 = p.spheres[prim];
+    const double cx = s4.x, cy = s4.y, cz = s4.z, rr = s4.w;
+    const double ocx = ox - cx, ocy = oy - cy, ocz = oz - cz;
+    const double b = ocx * dx + ocy * dy + ocz * dz;
+    const double crx = ocy * dz - ocz * dy, cry = ocz * dx - ocx * dz, crz = ocx * dy - ocy * dx;
+    const double h2 = rr * rr - (crx * crx + cry * cry + crz * crz);
+    double tn = 0.0, tf = 0.0;
+    bool hit = h2 >= 0.0 && sqrt(ocx * ocx + ocy * ocy + ocz * ocz) >= 4.0 * rr;
+    if (hit) {
+      const double h = sqrt(h2);
+      const double nr = -b - h, fr = -b + h;
+      if (nr < 0.0 && fr < 0.0) hit = false;
+      tn = nr < 0.0 ? 0.0 : nr;
+      tf = fr;
+    }
+    const double pc = 0.5 * (cx * cx + cy * cy + cz * cz - rr * rr);
+    double best = 1e300, second = 1e300, err_best = 0.0, err_second = 0.0;
+    int nxt = SENTINEL;
+    // rounding bound of the FP32 walk's t = (pm - o.n) / (d.n): unit roundoff
+    // times the magnitudes its terms are formed from
+    const double eps = 5.960464477539063e-08;
+    const double olen = sqrt(ox * ox + oy * oy + oz * oz), clen = sqrt(cx * cx + cy * cy + cz * cz);
+    for (int e = p.adj_off[prim]; e < p.adj_off[prim + 1]; e++) {
+      const int a = p.adjacency[e];
+      const float4 a4 = p.spheres[a];
+      const double ax = a4.x, ay = a4.y, az = a4.z, ar = a4.w;
+      const double nx = ax - cx, ny = ay - cy, nz = az - cz;
+      const double pm = 0.5 * (ax * ax + ay * ay + az * az - ar * ar) - pc;
+      const double dp = dx * nx + dy * ny + dz * nz;
+      const double t = (pm - (ox * nx + oy * ny + oz * nz)) / dp;
+      if (dp >= 0.0) {
+        const double alen = sqrt(ax * ax + ay * ay + az * az);
+        const double num = fabs(pm) + 0.5 * (alen * alen + ar * ar + clen * clen + rr * rr) +
+                           3.0 * (fabs(ox * nx) + fabs(oy * ny) + fabs(oz * nz)) + olen * (alen + clen);
+        const double den = 3.0 * (fabs(dx * nx) + fabs(dy * ny) + fabs(dz * nz)) + alen + clen;
+        const double err = 4.0 * eps * (num + fabs(t) * den) / fabs(dp);
+        if (t < best) {
+          second = best;
+          err_second = err_best;
+          best = t;
+          err_best = err;
+          nxt = a;
+        } else if (t < second) {
+          second = t;
+          err_second = err;
+        }
+        tf = t < tf ? t : tf;
+      } else {
+        tn = t > tn ? t : tn;
+      }
+    }
+    t_enter[q] = hit ? tn : __builtin_nan("");
+    t_exit[q] = hit ? tf : __builtin_nan("");
+    next64[q] = nxt;
+    gap[q] = second - best;
+    tol[q] = err_best + err_second;
+  }
+}
+
 static int check_trace(const pf_trace_args* a, const char* who) {
   if (!a) return set_error(PF_ERR_ARG, "%s: null args", who);
   const pf_scene& s = a->scene;
@@ -636,6 +724,28 @@ int pf_trace_fwd(const pf_trace_args* a, void* stream) {
   p.K = a->max_cells;
   p.n_cells = a->n_cells;
   trace_fwd_kernel<<<(p.n_tiles + TWPB - 1) / TWPB, TWAVE * TWPB, 0, as_stream(stream)>>>(p);
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+int pf_trace_check_f64(const pf_trace_check_args* c, void* stream) {
+  if (!c) return set_error(PF_ERR_ARG, "pf_trace_check_f64: null args");
+  const pf_scene& s = c->scene;
+  if (s.n_sites < 1 || !s.spheres || !s.adjacency_offsets || !s.adjacency)
+    return set_error(PF_ERR_ARG, "pf_trace_check_f64: needs spheres and the full adjacency");
+  if (c->cam.width < 2 || c->cam.height < 2) return set_error(PF_ERR_ARG, "pf_trace_check_f64: image must be >= 2x2");
+  if (c->n_rays < 0 || c->max_cells < 1) return set_error(PF_ERR_ARG, "pf_trace_check_f64: bad sizes");
+  if (c->n_rays == 0) return PF_OK;
+  if (!c->pixels || !c->cells || !c->n_cells || !c->t_enter || !c->t_exit || !c->next_cell || !c->exit_gap ||
+      !c->exit_tol)
+    return set_error(PF_ERR_ARG, "pf_trace_check_f64: a NULL array");
+  pf_trace_args a{};
+  a.cam = c->cam;
+  a.scene = s;
+  TraceParams p = make_params(&a);
+  trace_check_f64_kernel<<<(c->n_rays + 255) / 256, 256, 0, as_stream(stream)>>>(
+      p, c->n_rays, c->pixels, c->cells, c->n_cells, c->max_cells, c->t_enter, c->t_exit, c->next_cell, c->exit_gap,
+      c->exit_tol);
   PF_HIP(hipGetLastError());
   return PF_OK;
 }

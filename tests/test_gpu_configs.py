@@ -389,3 +389,98 @@ def test_c3_learned_head_rows_and_render_psnr():
     _log(f"c3: render PSNR against the SH render: MLP {p_mlp:.2f} dB, DC colour {p_dc:.2f} dB")
     assert p_mlp > 32.0
     assert p_mlp > p_dc + 0.3
+
+
+def test_c2_walk_recomputed_in_fp64_on_the_gpu():
+    """The full 1024x1024 fisheye view of the 1M-site scene: the FP32 walk's
+    cell sequences of 65,536 sampled rays recomputed in FP64 on the GPU
+    (pf_trace_check_f64). Every step's next cell is the FP64 walk's choice
+    unless the two smallest exit-face times lie within FP32 reach of each
+    other (a tie the FP32 rounding may order either way; the kernel bounds
+    that rounding from the magnitudes each time is formed from); and the FP64
+    intervals are right: points sampled inside them lie in the cell's sphere
+    and are at least as near, in power distance, to the cell's site as to
+    every neighbour."""
+    import ctypes
+    import torch
+    from types import SimpleNamespace
+    from powerfoam_amd import _abi
+    from powerfoam_amd.camera import TorchCamera, fisheye_ray_maps, look_at
+    from powerfoam_amd.raytrace import RayTracer, start_cell
+    from powerfoam_amd.rasterize import _scene_struct
+    s = configs.c1_scene()
+    eye = configs.c1_cameras()[0].eye.numpy().astype(np.float64)
+    size = 1024
+    rm = torch.tensor(fisheye_ray_maps(eye, (0.0, 0.0, 0.0), size), device="cuda")
+    t = look_at(eye, (0.0, 0.0, 0.0), size, size, 90.0)
+    cam = TorchCamera(t.eye, t.right, t.up, size, size, rm)
+    dev = "cuda"
+    tt = lambda a, dt=torch.float32: torch.tensor(np.ascontiguousarray(a), dtype=dt, device=dev)
+    x = {k: tt(s[k]) for k in ("points", "radii", "density", "normals", "texel_sites", "texel_height")}
+    rgb = tt(configs.texel_rgb(s, eye))
+    adj, off = tt(s["adj_full"], torch.int32), tt(s["off_full"], torch.int32)
+    st = start_cell(x["points"], x["radii"], eye)
+    tracer = RayTracer(SimpleNamespace(num_texel_sites=8, is_pinhole=True), dev)
+    K = 256
+    cells, n_cells = tracer.trace_cells(cam, st, x["points"], x["radii"], x["density"], x["normals"],
+                                        x["texel_sites"], rgb, x["texel_height"], adj, off, None, 1e-3, max_cells=K)
+    R = 65536
+    pix = torch.tensor(np.sort(np.random.default_rng(9).choice(size * size, R, replace=False)).astype(np.int32),
+                       device=dev)
+    sc = cells.view(-1, K)[pix.long()].contiguous()
+    sn = n_cells.view(-1)[pix.long()].contiguous()
+    out = dict(t_enter=torch.empty((R, K), dtype=torch.float64, device=dev),
+               t_exit=torch.empty((R, K), dtype=torch.float64, device=dev),
+               next_cell=torch.empty((R, K), dtype=torch.int32, device=dev),
+               exit_gap=torch.empty((R, K), dtype=torch.float64, device=dev),
+               exit_tol=torch.empty((R, K), dtype=torch.float64, device=dev))
+    sp = torch.cat([x["points"], x["radii"][:, None]], 1).contiguous()
+    ns = torch.cat([x["normals"], x["density"][:, None]], 1).contiguous()
+    rgbh = torch.cat([rgb, x["texel_height"][..., None]], -1).contiguous()
+    a = _abi.PfTraceCheckArgs()
+    a.cam = cam.to_pf()
+    a.scene = _scene_struct(sp, ns, x["texel_sites"], rgbh, adj, off, None)
+    a.n_rays, a.pixels, a.cells, a.n_cells, a.max_cells = R, pix.data_ptr(), sc.data_ptr(), sn.data_ptr(), K
+    a.t_enter, a.t_exit = out["t_enter"].data_ptr(), out["t_exit"].data_ptr()
+    a.next_cell, a.exit_gap = out["next_cell"].data_ptr(), out["exit_gap"].data_ptr()
+    a.exit_tol = out["exit_tol"].data_ptr()
+    print(f"fp64 walk check: {R} rays of the 1024x1024 fisheye view", flush=True)
+    _abi.check(_abi.load().pf_trace_check_f64(ctypes.byref(a), _abi.stream_handle()))
+    torch.cuda.synchronize()
+    cl, n = sc.cpu().numpy(), np.minimum(sn.cpu().numpy(), K)  # walks longer than K: their first K cells
+    te, tx = out["t_enter"].cpu().numpy(), out["t_exit"].cpu().numpy()
+    nx, gap, tol = out["next_cell"].cpu().numpy(), out["exit_gap"].cpu().numpy(), out["exit_tol"].cpu().numpy()
+    k = np.arange(K)[None, :]
+    inner = k < (n[:, None] - 1)  # steps followed by another step of the FP32 walk
+    fp32_next = np.concatenate([cl[:, 1:], np.full((R, 1), -1)], 1)
+    differ = inner & (nx != fp32_next)
+    # within the FP32 rounding bounds of the two smallest exit-face times
+    tie = gap <= tol
+    _log(f"c2 fp64 walk: {int(inner.sum())} steps checked, {int(n.mean())} cells per ray; next cell differs on "
+         f"{int(differ.sum())} steps, {int((differ & ~tie).sum())} of them without a near-tie; steps within "
+         f"FP32 reach of a tie {int((inner & tie).sum())}")
+    assert int(inner.sum()) > 1_000_000
+    assert not np.any(differ & ~tie)
+    # FP64 intervals: sampled interior points lie in the sphere and the power cell
+    rng = np.random.default_rng(10)
+    ok = inner & (tx - te > 1e-4)  # clear of the interval ends by more than a direction ulp moves them
+    rr, kk = np.nonzero(ok)
+    sel = rng.choice(rr.size, 4000, replace=False)
+    rr, kk = rr[sel], kk[sel]
+    f = rng.uniform(0.05, 0.95, rr.size)
+    rmn = rm.view(-1, 6)[pix.long()].cpu().numpy().astype(np.float32)
+    o = rmn[:, :3].astype(np.float64)
+    d = rmn[:, 3:].astype(np.float32)
+    d = (d / np.sqrt((d * d).sum(1, dtype=np.float32))[:, None]).astype(np.float64)
+    P = o[rr] + d[rr] * (te[rr, kk] + f * (tx[rr, kk] - te[rr, kk]))[:, None]
+    site = cl[rr, kk]
+    pts, rad = s["points"].astype(np.float64), s["radii"].astype(np.float64)
+    own = ((P - pts[site]) ** 2).sum(1) - rad[site] ** 2
+    assert np.all(np.sqrt(((P - pts[site]) ** 2).sum(1)) <= rad[site] * (1 + 1e-9))
+    worst = 0.0
+    for i in range(rr.size):
+        nb = s["adj_full"][s["off_full"][site[i]]:s["off_full"][site[i] + 1]]
+        other = ((P[i] - pts[nb]) ** 2).sum(1) - rad[nb] ** 2
+        worst = max(worst, float(own[i] - other.min()) / (1.0 + abs(own[i])))
+    _log(f"c2 fp64 intervals: {rr.size} interior points, worst power-distance excess {worst:.2e}")
+    assert worst <= 1e-9
