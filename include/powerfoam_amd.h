@@ -214,6 +214,12 @@ typedef struct pf_trace_args {
   int32_t* cells;  /* [H,W,max_cells] or NULL: visited cells in order */
   int32_t max_cells;
   int32_t* n_cells;/* [H,W] or NULL: number of cells visited */
+  /* > 0: ballot compaction over groups of `compact` neighbouring tiles: a
+   * wave walks the group's rays and hands a lane whose ray has ended the
+   * group's next unstarted ray (its slot from a ballot of the lanes that
+   * ended in the same step); 0: one wave per tile, a lane idles once its ray
+   * ends. Same results either way. */
+  int32_t compact;
 } pf_trace_args;
 
 /* Exact adjoint of pf_trace_fwd (no reference counterpart: the reference ray

@@ -201,6 +201,7 @@ class PfTraceArgs(ctypes.Structure):
         ("cells", ctypes.c_void_p),
         ("max_cells", ctypes.c_int32),
         ("n_cells", ctypes.c_void_p),
+        ("compact", ctypes.c_int32),
     ]
 
 

@@ -194,12 +194,37 @@ __device__ __forceinline__ bool trace_lane(const TraceParams& p, int tile, int l
   return i < p.H && j < p.W;
 }
 
-__global__ void __launch_bounds__(TWAVE* TWPB) trace_fwd_kernel(TraceParams p) {
-  const int tile = blockIdx.x * TWPB + (threadIdx.x >> 6);
-  if (tile >= p.n_tiles) return;
+// COMPACT: a wave walks the rays of group_tiles neighbouring tiles; a lane
+// whose ray ends writes its outputs and takes the group's next unstarted ray
+// (slot = the wave's counter in LDS + the lane's rank among the lanes that
+// ended in the same step, from a ballot), so lanes stay busy until the group
+// runs out of rays instead of idling until the tile's longest ray ends.
+template <bool COMPACT>
+__global__ void __launch_bounds__(TWAVE* TWPB) trace_fwd_kernel(TraceParams p, int group_tiles) {
+  __shared__ int s_next[TWPB];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int group = blockIdx.x * TWPB + wv;
+  if (group * group_tiles >= p.n_tiles) return;
+  const int n_slots = group_tiles * TWAVE;
+  int slot = lane;
+  if (COMPACT) {
+    if (lane == 0) s_next[wv] = TWAVE;
+    wave_sync();
+  }
   int i, j;
-  if (!trace_lane(p, tile, threadIdx.x & 63, i, j)) return;
-  const size_t px = (size_t)i * p.W + j;
+  // a slot outside the image takes the next one (compaction) or idles
+  while (!(group * group_tiles + slot / TWAVE < p.n_tiles &&
+           trace_lane(p, group * group_tiles + slot / TWAVE, slot % TWAVE, i, j))) {
+    if (!COMPACT) return;
+    const uint64_t m = ballot(true);
+    const int base = s_next[wv];
+    slot = base + __popcll(m & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
+    wave_sync();
+    if (lane == __ffsll((long long)m) - 1) s_next[wv] = base + __popcll(m);
+    wave_sync();
+    if (slot >= n_slots) return;
+  }
+  size_t px = (size_t)i * p.W + j;
   f3 o, d;
   walk_ray(p, i, j, o, d);
   int prim = p.start_sites ? p.start_sites[px] : p.start_site;
@@ -208,7 +233,50 @@ __global__ void __launch_bounds__(TWAVE* TWPB) trace_fwd_kernel(TraceParams p) {
   f3 rgb{0.0f, 0.0f, 0.0f}, nacc{0.0f, 0.0f, 0.0f};
   while (true) {
     const float trans = fast_exp(log_t);
-    if (trans < p.thr || prim == SENTINEL || prim < 0 || step >= p.max_steps) break;
+    if (trans < p.thr || prim == SENTINEL || prim < 0 || step >= p.max_steps) {
+      if (!COMPACT) break;
+      p.color[3 * px + 0] = rgb.x;
+      p.color[3 * px + 1] = rgb.y;
+      p.color[3 * px + 2] = rgb.z;
+      if (p.log_t) p.log_t[px] = log_t;
+      if (p.normal) {
+        p.normal[3 * px + 0] = nacc.x;
+        p.normal[3 * px + 1] = nacc.y;
+        p.normal[3 * px + 2] = nacc.z;
+      }
+      if (p.depth) p.depth[px] = depth;
+      if (p.q_step) p.q_step[px] = qstep;
+      if (p.q_sigma) p.q_sigma[px] = qsig;
+      if (p.n_cells) p.n_cells[px] = step;
+      // the next unstarted ray of the group (skipping slots outside the image)
+      bool got = false;
+      while (true) {
+        const uint64_t m = ballot(true);
+        const int base = s_next[wv];
+        slot = base + __popcll(m & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
+        wave_sync();
+        if (lane == __ffsll((long long)m) - 1) s_next[wv] = base + __popcll(m);
+        wave_sync();
+        if (slot >= n_slots) break;
+        const int tl = group * group_tiles + slot / TWAVE;
+        if (tl < p.n_tiles && trace_lane(p, tl, slot % TWAVE, i, j)) {
+          got = true;
+          break;
+        }
+      }
+      if (!got) break;
+      px = (size_t)i * p.W + j;
+      walk_ray(p, i, j, o, d);
+      prim = p.start_sites ? p.start_sites[px] : p.start_site;
+      log_t = 0.0f;
+      depth = -1.0f;
+      qsig = 0.0f;
+      qstep = -1;
+      step = 0;
+      rgb = f3{0.0f, 0.0f, 0.0f};
+      nacc = f3{0.0f, 0.0f, 0.0f};
+      continue;
+    }
     if (p.cells && step < p.K) p.cells[px * p.K + step] = prim;
     const float4 sph = p.spheres[prim];
     const f3 c{sph.x, sph.y, sph.z};
@@ -723,7 +791,13 @@ int pf_trace_fwd(const pf_trace_args* a, void* stream) {
   p.cells = a->cells;
   p.K = a->max_cells;
   p.n_cells = a->n_cells;
-  trace_fwd_kernel<<<(p.n_tiles + TWPB - 1) / TWPB, TWAVE * TWPB, 0, as_stream(stream)>>>(p);
+  if (a->compact > 0) {
+    const int group_tiles = a->compact;
+    const int groups = (p.n_tiles + group_tiles - 1) / group_tiles;
+    trace_fwd_kernel<true><<<(groups + TWPB - 1) / TWPB, TWAVE * TWPB, 0, as_stream(stream)>>>(p, group_tiles);
+  } else {
+    trace_fwd_kernel<false><<<(p.n_tiles + TWPB - 1) / TWPB, TWAVE * TWPB, 0, as_stream(stream)>>>(p, 1);
+  }
   PF_HIP(hipGetLastError());
   return PF_OK;
 }
