@@ -68,6 +68,13 @@ typedef struct pf_scene {
    * adjacency_diff (the Python mirror checks it at the frame's host sync) */
   const int32_t* adjacency_offsets;
   const uint16_t* adjacency_diff;   /* [E,4] IEEE binary16 (c_adj - c, r_adj) */
+  /* [N,T,4] bfloat16 (rgb, height), 8-byte aligned, or NULL. When set, the
+   * rasterizer kernels (pf_raster_fwd/bwd/rgb/vis) read the texel features
+   * from it instead of texel_rgbh (which may then be NULL): the features
+   * stored in BF16, geometry and transmittance still FP32. Not in the
+   * reference (its fp16 attributes, rasterize.py:2008-2014, are the
+   * benchmark pass's); the ray tracer reads texel_rgbh. */
+  const uint16_t* texel_rgbh_bf16;
 } pf_scene;
 
 /* Depth-sorted per-tile site lists produced by binning. */

@@ -58,6 +58,7 @@ class PfScene(ctypes.Structure):
         ("adjacency", ctypes.c_void_p),
         ("adjacency_offsets", ctypes.c_void_p),
         ("adjacency_diff", ctypes.c_void_p),
+        ("texel_rgbh_bf16", ctypes.c_void_p),
     ]
 
 

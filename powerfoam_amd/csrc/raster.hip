@@ -214,12 +214,24 @@ struct TexelLds {
 
 // Loads the site's texels into LDS (lanes < T) and reports whether any
 // height is non-zero.
+// bfloat16 -> float: the 16 bits are the float's upper half
+__device__ __forceinline__ float bf16_to_float(uint32_t h) { return __builtin_bit_cast(float, h << 16); }
+
+// texel_rgbh_bf16 (4 x bfloat16 per texel), when set, replaces texel_rgbh:
+// the colour and height features stored in half the bytes.
 __device__ __forceinline__ bool stage_texels(TexelLds* s_tex, const float* texel_sites, const float4* texel_rgbh,
-                                             int site, int T, f3 c, int lane) {
+                                             const uint2* texel_rgbh_bf16, int site, int T, f3 c, int lane) {
   bool nz = false;
   if (lane < T) {
     const float* ts = texel_sites + ((size_t)site * T + lane) * 3;
-    const float4 v = texel_rgbh[(size_t)site * T + lane];
+    float4 v;
+    if (texel_rgbh_bf16) {
+      const uint2 h = texel_rgbh_bf16[(size_t)site * T + lane];
+      v = make_float4(bf16_to_float(h.x & 0xffffu), bf16_to_float(h.x >> 16), bf16_to_float(h.y & 0xffffu),
+                      bf16_to_float(h.y >> 16));
+    } else {
+      v = texel_rgbh[(size_t)site * T + lane];
+    }
     s_tex[lane].pos_h = make_float4(ts[0] - c.x, ts[1] - c.y, ts[2] - c.z, v.w);
     s_tex[lane].rgb = make_float4(v.x, v.y, v.z, 0.0f);
     nz = v.w != 0.0f;
@@ -237,6 +249,7 @@ struct FwdParams {
   const float4* nsigmas;
   const float* texel_sites;
   const float4* texel_rgbh;
+  const uint2* texel_rgbh_bf16;
   const int32_t* adjacency;
   const int32_t* adj_off;
   const uint16_t* adj_diff;
@@ -416,7 +429,7 @@ __global__ void __launch_bounds__(WAVE* WPB) raster_fwd_kernel(FwdParams p) {
 
       // texel records of this site (texture.py:23-24)
       wave_sync();
-      const bool heights = stage_texels(s_tex, p.texel_sites, p.texel_rgbh, sr.site, T, c, lane);
+      const bool heights = stage_texels(s_tex, p.texel_sites, p.texel_rgbh, p.texel_rgbh_bf16, sr.site, T, c, lane);
       wave_sync();
       float alpha_t = 0.0f, perr = 0.0f;
       bool shaded = false;  // an interval of positive length (the backward's "active")
@@ -580,6 +593,7 @@ struct BwdParams {
   const float4* nsigmas;
   const float* texel_sites;
   const float4* texel_rgbh;
+  const uint2* texel_rgbh_bf16;
   const int32_t* adjacency;
   const int32_t* adj_off;
   const uint16_t* adj_diff;
@@ -772,7 +786,7 @@ __global__ void __launch_bounds__(WAVE* WPB) PF_BWD_OCC raster_bwd_kernel(BwdPar
       const bool shade = hit && !ray.oob && tn < tf;
       if (ballot(shade) == 0) continue;
       wave_sync();
-      const bool heights = stage_texels(s_tex, p.texel_sites, p.texel_rgbh, sr.site, T, c, lane);
+      const bool heights = stage_texels(s_tex, p.texel_sites, p.texel_rgbh, p.texel_rgbh_bf16, sr.site, T, c, lane);
       wave_sync();
 
       // ---- recompute plane_intersection_fwd (texture.py:10-71), keeping the
@@ -1158,6 +1172,7 @@ struct RgbParams {
   const float4* nsigmas;
   const float* texel_sites;
   const float4* texel_rgbh;
+  const uint2* texel_rgbh_bf16;
   const int32_t* adjacency;
   const int32_t* adj_off;
   const uint16_t* adj_diff;
@@ -1281,7 +1296,7 @@ __global__ void __launch_bounds__(WAVE* WPB) raster_rgb_kernel(RgbParams p) {
       }
       if (ballot(open) == 0) continue;
       wave_sync();
-      const bool heights = stage_texels(s_tex, p.texel_sites, p.texel_rgbh, sr.site, T, c, lane);
+      const bool heights = stage_texels(s_tex, p.texel_sites, p.texel_rgbh, p.texel_rgbh_bf16, sr.site, T, c, lane);
       wave_sync();
       if (open) {
         // plane_intersection_fwd_local (rasterize.py:67-138)
@@ -1399,9 +1414,11 @@ __global__ void __launch_bounds__(256)
 using namespace pf;
 
 static int check_scene(const pf_scene& s, const char* who) {
-  if (s.n_sites < 0 || (s.n_sites > 0 && (!s.spheres || !s.nsigmas || !s.texel_sites || !s.texel_rgbh ||
-                                          !s.adjacency_offsets)))
+  if (s.n_sites < 0 || (s.n_sites > 0 && (!s.spheres || !s.nsigmas || !s.texel_sites ||
+                                          (!s.texel_rgbh && !s.texel_rgbh_bf16) || !s.adjacency_offsets)))
     return set_error(PF_ERR_ARG, "%s: missing scene arrays", who);
+  if (reinterpret_cast<uintptr_t>(s.texel_rgbh_bf16) % 8)
+    return set_error(PF_ERR_ARG, "%s: texel_rgbh_bf16 must be 8-byte aligned", who);
   if (s.n_texels < 1 || s.n_texels > PF_MAX_TEXELS)
     return set_error(PF_ERR_ARG, "%s: n_texels=%d outside 1..%d", who, s.n_texels, PF_MAX_TEXELS);
   return PF_OK;
@@ -1436,6 +1453,7 @@ int pf_raster_fwd(const pf_raster_fwd_args* a, void* stream) {
   p.nsigmas = reinterpret_cast<const float4*>(a->scene.nsigmas);
   p.texel_sites = a->scene.texel_sites;
   p.texel_rgbh = reinterpret_cast<const float4*>(a->scene.texel_rgbh);
+  p.texel_rgbh_bf16 = reinterpret_cast<const uint2*>(a->scene.texel_rgbh_bf16);
   p.adjacency = a->scene.adjacency;
   p.adj_off = a->scene.adjacency_offsets;
   p.adj_diff = a->scene.adjacency_diff;
@@ -1525,6 +1543,7 @@ int pf_raster_bwd(const pf_raster_bwd_args* a, void* stream) {
   p.nsigmas = reinterpret_cast<const float4*>(a->scene.nsigmas);
   p.texel_sites = a->scene.texel_sites;
   p.texel_rgbh = reinterpret_cast<const float4*>(a->scene.texel_rgbh);
+  p.texel_rgbh_bf16 = reinterpret_cast<const uint2*>(a->scene.texel_rgbh_bf16);
   p.adjacency = a->scene.adjacency;
   p.adj_off = a->scene.adjacency_offsets;
   p.adj_diff = a->scene.adjacency_diff;
@@ -1602,6 +1621,7 @@ int pf_raster_rgb(const pf_raster_rgb_args* a, void* stream) {
   p.nsigmas = reinterpret_cast<const float4*>(a->scene.nsigmas);
   p.texel_sites = a->scene.texel_sites;
   p.texel_rgbh = reinterpret_cast<const float4*>(a->scene.texel_rgbh);
+  p.texel_rgbh_bf16 = reinterpret_cast<const uint2*>(a->scene.texel_rgbh_bf16);
   p.adj_off = a->scene.adjacency_offsets;
   p.adj_diff = a->scene.adjacency_diff;
   p.offsets = a->tiles.offsets;
@@ -1640,6 +1660,7 @@ int pf_raster_vis(const pf_raster_vis_args* a, void* stream) {
   p.nsigmas = reinterpret_cast<const float4*>(a->scene.nsigmas);
   p.texel_sites = a->scene.texel_sites;
   p.texel_rgbh = reinterpret_cast<const float4*>(a->scene.texel_rgbh);
+  p.texel_rgbh_bf16 = reinterpret_cast<const uint2*>(a->scene.texel_rgbh_bf16);
   p.adjacency = a->scene.adjacency;
   p.adj_off = a->scene.adjacency_offsets;
   p.adj_diff = a->scene.adjacency_diff;

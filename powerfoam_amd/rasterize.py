@@ -122,6 +122,12 @@ class Rasterizer:
         # and the backward walks only those (an entry that shades nothing adds
         # nothing to any gradient). Not in the reference.
         self.skip_unshaded = True
+        # "bf16": the compositing kernels read the texel colour and height
+        # features as bfloat16 (8 B per texel instead of 16); geometry,
+        # transmittance and every gradient stay FP32. The backward treats the
+        # rounding as the identity (the gradient of the stored feature is the
+        # gradient of the FP32 input). Not in the reference.
+        self.feature_dtype = "float"
 
     def _launch(self, name, fn, *a):
         if self.event_log is None:
@@ -297,6 +303,14 @@ class Rasterizer:
                                      ptr(live), ptr(bounds), stream_handle()))
         return live, bounds
 
+    def _features(self, all_texel_rgbh):
+        """The bfloat16 copy of the texel features the kernels read (or None)."""
+        if self.feature_dtype == "float":
+            return None
+        if self.feature_dtype != "bf16":
+            raise ValueError(f"feature_dtype must be 'float' or 'bf16', not {self.feature_dtype!r}")
+        return all_texel_rgbh.to(torch.bfloat16).contiguous()
+
     def pack_sites(self, points, radii, normals, density, texel_rgb, texel_height):
         """(spheres [N,4], nsigmas [N,4], texel_rgbh [N,T,4]) in FP32, the
         concatenations of rasterize.py:1619-1624, in one launch (pf_pack_sites)."""
@@ -346,6 +360,7 @@ class Rasterizer:
         dq = _f32c(depth_quantiles) if depth_quantiles is not None else None
 
         adjacency_diff = self.encode_adjacency(all_spheres, adjacency, adjacency_offsets)
+        rgbh_bf16 = self._features(all_texel_rgbh)
         live, bounds = self.live_sites(camera, all_spheres, adjacency, adjacency_offsets,
                                        None if self.exact_geometry else adjacency_diff)
         if adjacency_offsets.shape != (N + 1,):
@@ -372,7 +387,7 @@ class Rasterizer:
         a = _abi.PfRasterFwdArgs()
         a.cam, ray_maps = self._camera(camera)
         a.scene = _scene_struct(all_spheres, all_nsigmas, all_texel_sites, all_texel_rgbh, adjacency,
-                                adjacency_offsets, None if self.exact_geometry else adjacency_diff)
+                                adjacency_offsets, None if self.exact_geometry else adjacency_diff, rgbh_bf16)
         a.tiles = self._tiles(nt, offsets, tile_sites, order)
         scratch = torch.empty(sg["n_segs"] * 64, **f32) if sg is not None else None
         a.segs = self._segs(sg, scratch)
@@ -401,6 +416,7 @@ class Rasterizer:
         grad_ctx.order = sg["tile_order"] if sg is not None else order
         grad_ctx.segs = sg
         grad_ctx.entry_shades = entry_shades
+        grad_ctx.rgbh_bf16 = rgbh_bf16
         grad_ctx.save_for_backward(dq, all_spheres, all_nsigmas, all_texel_sites, all_texel_rgbh, adjacency,
                                    adjacency_offsets, adjacency_diff, tile_sites, offsets, tile_early_stop,
                                    ray_gt_c, color_out, log_t_out)
@@ -532,7 +548,8 @@ class Rasterizer:
         a = _abi.PfRasterBwdArgs()
         a.cam, ray_maps = self._camera(camera, grad_ctx.ray_maps)
         a.scene = _scene_struct(all_spheres, all_nsigmas, all_texel_sites, all_texel_rgbh, adjacency,
-                                adjacency_offsets, None if self.exact_geometry else adjacency_diff)
+                                adjacency_offsets, None if self.exact_geometry else adjacency_diff,
+                                getattr(grad_ctx, "rgbh_bf16", None))
         sg = grad_ctx.segs if self.split_backward else None
         order = grad_ctx.order
         if sg is None and order is None:
@@ -561,7 +578,7 @@ class Rasterizer:
         st = stream_handle()
         self._launch("raster_bwd", self.lib.pf_raster_bwd, _ref(a), st)
         check(self.lib.pf_scrub_nonfinite(ptr(grad_buf), n_grads, st))
-        del grad_ctx.rasterizer, grad_ctx.camera, grad_ctx.ray_maps, grad_ctx.order, grad_ctx.segs
+        del grad_ctx.rasterizer, grad_ctx.camera, grad_ctx.ray_maps, grad_ctx.order, grad_ctx.segs, grad_ctx.rgbh_bf16
         return (None, None, None, spheres_grad[:, :3], spheres_grad[:, 3], nsigmas_grad[:, 3],
                 nsigmas_grad[:, :3], texel_sites_grad, texel_rgb_grad, texel_height_grad, None, None, None, None)
 
@@ -585,7 +602,7 @@ def _ref(struct):
 
 
 def _scene_struct(all_spheres, all_nsigmas, all_texel_sites, all_texel_rgbh, adjacency, adjacency_offsets,
-                  adjacency_diff):
+                  adjacency_diff, texel_rgbh_bf16=None):
     s = _abi.PfScene()
     s.n_sites = all_spheres.shape[0]
     s.n_texels = all_texel_sites.shape[1]
@@ -596,4 +613,5 @@ def _scene_struct(all_spheres, all_nsigmas, all_texel_sites, all_texel_rgbh, adj
     s.adjacency = adjacency.data_ptr() if adjacency.numel() else None
     s.adjacency_offsets = adjacency_offsets.data_ptr()
     s.adjacency_diff = adjacency_diff.data_ptr() if (adjacency_diff is not None and adjacency_diff.numel()) else None
+    s.texel_rgbh_bf16 = texel_rgbh_bf16.data_ptr() if texel_rgbh_bf16 is not None else None
     return s

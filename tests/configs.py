@@ -83,3 +83,34 @@ def c3_scene():
     s.update(adj=adj, off=off)
     log(f"c3 scene: {st}, {time.time() - t0:.1f}s")
     return s
+
+
+@functools.lru_cache(maxsize=None)
+def c3_head():
+    """The learned colour head on the 3M-site scene, trained on the GPU with its
+    own kernels: 2000 Adam steps of 65,536 SH-labelled (site, direction)
+    pairs. Returns (head, training loss history)."""
+    import torch
+    from powerfoam_amd.mlp_head import MLPColor, fit, latents_from_sh
+    s = c3_scene()
+    sh = torch.tensor(s["sh"], device="cuda")
+    head = MLPColor(sh.shape[0], "cuda", seed=0, latents=latents_from_sh(sh))
+    t0 = time.time()
+    hist = fit(head, sh, steps=2000, batch=65536, log_every=500)
+    torch.cuda.synchronize()
+    log(f"c3 head: trained in {time.time() - t0:.1f}s, loss {hist[0][1]:.4f} -> {hist[-1][1]:.4f}")
+    return head, hist
+
+
+def c4_cameras(scale=1):
+    """The mixed-resolution batch: 4 views of 1297x840, 2 of 979x546, 2 of
+    1920x1080 (each size divided by `scale`), the first 8 cameras of the
+    radius-4 circle, fov 70 deg."""
+    from powerfoam_amd.camera import look_at
+    sizes = [(1297, 840)] * 4 + [(979, 546)] * 2 + [(1920, 1080)] * 2
+    cams = []
+    for k, (w, h) in enumerate(sizes):
+        th = 2.0 * np.pi * k / 32
+        cams.append(look_at((4.0 * np.cos(th), 1.0, 4.0 * np.sin(th)), (0.0, 0.0, 0.0), max(w // scale, 8),
+                            max(h // scale, 8), 70.0))
+    return cams

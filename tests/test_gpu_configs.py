@@ -344,17 +344,14 @@ def test_c3_learned_head_rows_and_render_psnr():
     import torch
     from oracle import mlp as om
     from powerfoam_amd.color_fn import SHColor
-    from powerfoam_amd.mlp_head import MLPColor, fit, latents_from_sh, psnr
+    from powerfoam_amd.mlp_head import psnr
     from powerfoam_amd.rasterize import Rasterizer
     s = configs.c3_scene()
     dev = "cuda"
     tt = lambda a, dt=torch.float32: torch.tensor(np.ascontiguousarray(a), dtype=dt, device=dev)
     sh = tt(s["sh"])
-    head = MLPColor(sh.shape[0], dev, seed=0, latents=latents_from_sh(sh))
-    t0 = time.time()
-    hist = fit(head, sh, steps=2000, batch=65536, log_every=500)
-    torch.cuda.synchronize()
-    _log(f"c3: trained the MLP head in {time.time() - t0:.1f}s, loss {hist[0][1]:.4f} -> {hist[-1][1]:.4f}")
+    head, hist = configs.c3_head()
+    assert hist[-1][1] < 0.02 * hist[0][1]
     cams = configs.c1_cameras()
     ts = tt(s["texel_sites"])
     # rows against the oracle network on a sample of sites
