@@ -333,6 +333,54 @@ def run_trace(args, dev):
     print(json.dumps(line), flush=True)
 
 
+def run_cech(args, dev):
+    """Cech adjacency build (AABBTree.build_cech_complex, bvh.py:96-123) of the
+    GMM scene (seed 1, --sites sites): count pass, one host sync for the
+    total, emit pass, per-site ascending sort. Reports adjacency entries/s.
+    Algorithmic bytes per build: the spheres read once (16 B per site), the
+    offsets and lists written once (4 B per site, 4 B per entry). The CPU
+    baseline is the oracle's kd-tree query of a 4,000-site sample."""
+    import torch
+    from powerfoam_amd import synth
+    from powerfoam_amd.bvh import AABBTree
+
+    s = synth.scene_gmm(1, n=args.sites)
+    tree = AABBTree(dev)
+    tree.update(torch.tensor(s["points"], device=dev), torch.tensor(s["radii"], device=dev))
+    for _ in range(args.warmup):
+        adj, off = tree.build_cech_complex()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        adj, off = tree.build_cech_complex()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    m = adj.numel()
+    abytes = 20 * args.sites + 4 * m
+    achieved = abytes / (ms * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import cech as oc
+        sample = np.random.default_rng(0).choice(args.sites, 4000, replace=False)
+        t0 = time.perf_counter()
+        got = oc.neighbours(s["points"], s["radii"], sample)
+        dt = time.perf_counter() - t0
+        cpu = dict(value=sum(len(g) for g in got) / dt / 1e6, unit="M entries/s", cores=1, kind="port",
+                   sample="oracle kd-tree neighbour lists of 4,000 random sites of the same scene")
+    line = dict(metric="Cech adjacency build, M entries/s", value=m / (ms * 1e-3) / 1e6, unit="M entries/s",
+                n_gpus=1, steps=args.steps, warmup=args.warmup, ms_per_step=ms, higher_is_better=True,
+                scaling="weak", vs_baseline=None, dtype="f32", data="synthetic (seeded scene generator)",
+                config=dict(workload=f"Cech adjacency of the {args.sites}-site GMM scene", sites=args.sites,
+                            entries=m, mean_degree=m / args.sites),
+                roofline=dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
+                              frac=achieved / HBM_PEAK_GBS, traffic=None, algorithmic_bytes=abytes,
+                              bound_note="whole build; the neighbour search (candidate cells through L2) sets "
+                                         "the time, not HBM"),
+                cpu_baseline=cpu)
+    log(f"{ms:.2f} ms per build, {m} entries")
+    print(json.dumps(line), flush=True)
+
+
 def workload_counts(rast, w, width, height):
     """Per-frame counts for the algorithmic byte formulas, from the last frame."""
     import torch
@@ -481,8 +529,9 @@ def main():
     ap.add_argument("--mlp-steps", type=int, default=2000)
     ap.add_argument("--mlp-batch", type=int, default=65536)
     ap.add_argument("--mlp-frames", type=int, default=16)
-    ap.add_argument("--workload", choices=("raster", "trace"), default="raster",
-                    help="raster: the rasterizer frame (default); trace: the ray tracer on fisheye views")
+    ap.add_argument("--workload", choices=("raster", "trace", "cech"), default="raster",
+                    help="raster: the rasterizer frame (default); trace: the ray tracer on fisheye views; "
+                         "cech: the Cech adjacency build")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -501,6 +550,8 @@ def main():
     dev = torch.device("cuda:0")
     if args.workload == "trace":
         return run_trace(args, dev)
+    if args.workload == "cech":
+        return run_cech(args, dev)
     if args.head == "mlp":
         rec, ms = mlp_head_record(args, dev)
         P = args.width * args.height

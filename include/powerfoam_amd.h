@@ -385,6 +385,29 @@ int pf_tile_schedule(const int64_t* offsets, int32_t n_tiles, int64_t n_records,
                      int32_t max_segs, int32_t* tile_order, int32_t* tile_first, int32_t* seg_tile,
                      int32_t* seg_order, int32_t* count, void* stream);
 
+/* ---- Cech adjacency (AABBTree.update / build_cech_complex bvh.py:76-123,
+ *      count_adjacent / write_adjacent bvh.py:7-69) ----------------------
+ * j != i is a neighbour of site i when the float32 boxes c -+ r overlap and
+ * |c_i - c_j| < r_i + r_j, with c and r taken back from the box as the
+ * reference does. Sequence (one stream, the workspace untouched between):
+ *   pf_cech_count  -> counts [N+1] (counts[0] = 0, counts[i+1] = degree of i)
+ *   host: offsets = inclusive cumsum of counts (int64 [N+1]), M = offsets[N]
+ *   pf_cech_emit   -> neighbour ids as records [M] in each site's segment
+ *   pf_bin_sort(offsets, N, records, adjacency, ...) -> adjacency [M] int32,
+ *                     ids ascending within every site (the reference's BVH
+ *                     order is unspecified).
+ * spheres [N,4] = (x, y, z, r), 16-byte aligned, N <= PF_CECH_MAX_SITES; the
+ * workspace is pf_cech_workspace_size(N) bytes (about 48 N + 12 MiB at most
+ * for the hash table), 256-byte aligned. Entries beyond a segment are added
+ * to *dropped (cleared by the caller), never written. Non-finite centres or
+ * radii make no pairs of their own and never stop the build. */
+#define PF_CECH_MAX_SITES (1 << 29)
+size_t pf_cech_workspace_size(int32_t n_sites);
+int pf_cech_count(const float* spheres, int32_t n_sites, void* workspace, size_t workspace_bytes,
+                  int32_t* counts, void* stream);
+int pf_cech_emit(const float* spheres, int32_t n_sites, void* workspace, size_t workspace_bytes,
+                 const int64_t* offsets, uint64_t* records, int32_t* dropped, void* stream);
+
 /* ---- adjacency encoding (prefetch_adjacency_kernel rasterize.py:580-606) */
 /* diff[e] = binary16(c_adj - c, r_adj) for every edge of every site.
  * n_edges is the length of adjacency and diff (adjacency_offsets[n_sites]);

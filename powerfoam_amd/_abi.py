@@ -26,6 +26,7 @@ TILE_WIDTH = 8
 TILE_SIZE = 64
 MAX_TEXELS = 8
 SORT_STATIC_MAX_TILES = 262144  # PF_SORT_STATIC_MAX_TILES
+CECH_MAX_SITES = 1 << 29  # PF_CECH_MAX_SITES
 
 c_float_p = ctypes.POINTER(ctypes.c_float)
 c_i32_p = ctypes.POINTER(ctypes.c_int32)
@@ -270,6 +271,9 @@ SIGNATURES = {
     "pf_bin_emit_cones": ([_VP, _VP, _VP, ctypes.c_int32, _VP, ctypes.c_int32, _VP, _VP, _VP, _VP, _VP],
                           ctypes.c_int),
     "pf_adj_encode": ([_VP, ctypes.c_int32, _VP, _VP, ctypes.c_int64, _VP, _VP], ctypes.c_int),
+    "pf_cech_workspace_size": ([ctypes.c_int32], ctypes.c_size_t),
+    "pf_cech_count": ([_VP, ctypes.c_int32, _VP, ctypes.c_size_t, _VP, _VP], ctypes.c_int),
+    "pf_cech_emit": ([_VP, ctypes.c_int32, _VP, ctypes.c_size_t, _VP, _VP, _VP, _VP], ctypes.c_int),
     "pf_raster_fwd": ([_VP, _VP], ctypes.c_int),
     "pf_raster_bwd": ([_VP, _VP], ctypes.c_int),
     "pf_raster_rgb": ([_VP, _VP], ctypes.c_int),
