@@ -408,6 +408,22 @@ int pf_cech_count(const float* spheres, int32_t n_sites, void* workspace, size_t
 int pf_cech_emit(const float* spheres, int32_t n_sites, void* workspace, size_t workspace_bytes,
                  const int64_t* offsets, uint64_t* records, int32_t* dropped, void* stream);
 
+/* ---- restricted power adjacency ------------------------------------------
+ * No reference kernel: replaces the lifted-hull power adjacency with the
+ * overlap filter (build_power_adjacency, benchmark.py:25-83) for rendering.
+ * From symmetric, ascending Cech lists (pf_cech_*): keeps the entries (a, b)
+ * whose power face meets the two balls (FP64 disk-and-half-planes test on the
+ * radical plane against a's other neighbours; both directed entries take the
+ * answer of a < b), which is every face that bounds a site's cell inside its
+ * ball. A site left with none whose cell misses its ball keeps, one way, the
+ * neighbours whose radical planes cut into its ball, so it still renders
+ * empty. keep [n_edges] (0/1) and counts [N+1] (counts[0] = 0, kept entries
+ * per site) are overwritten; the lists are adjacency[keep] with offsets the
+ * cumulative counts. Neighbour ids outside [0, N) are ignored. */
+int pf_power_restrict(const float* spheres, int32_t n_sites, const int32_t* adjacency,
+                      const int32_t* adjacency_offsets, int64_t n_edges, uint8_t* keep, int32_t* counts,
+                      void* stream);
+
 /* ---- adjacency encoding (prefetch_adjacency_kernel rasterize.py:580-606) */
 /* diff[e] = binary16(c_adj - c, r_adj) for every edge of every site.
  * n_edges is the length of adjacency and diff (adjacency_offsets[n_sites]);

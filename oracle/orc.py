@@ -54,6 +54,10 @@ def load():
         lib.orc_set_fragile.restype = None
         lib.orc_power_face_encode.argtypes = [vp, ctypes.c_int32, vp, vp, vp]
         lib.orc_power_face_encode.restype = None
+        lib.orc_power_restrict.argtypes = [vp, ctypes.c_int32, vp, vp, vp, vp]
+        lib.orc_power_restrict.restype = None
+        lib.orc_power_face.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, vp, ctypes.c_int64]
+        lib.orc_power_face.restype = ctypes.c_int
         for p in ("f32", "f64"):
             getattr(lib, f"orc_raster_vis_{p}").argtypes = [vp, vp, vp, vp, ctypes.c_int, ctypes.c_float, ctypes.c_int,
                                                             ctypes.c_float, vp, vp]
@@ -176,6 +180,32 @@ def power_face_encode(spheres, adjacency, offsets):
     out = np.zeros((max(adj.shape[0], 1), 4), np.uint16)
     lib.orc_power_face_encode(_p(sp), sp.shape[0], _p(adj), _p(off), _p(out))
     return out[: adj.shape[0]]
+
+
+def power_restrict(spheres, adjacency, offsets):
+    """Restricted power adjacency (orc_power.h): (keep uint8 [M], counts int32
+    [N+1]) over the given symmetric, ascending (Cech) lists."""
+    sp = _f32(spheres)
+    adj = np.ascontiguousarray(adjacency, np.int32)
+    off = np.ascontiguousarray(offsets, np.int32)
+    n = sp.shape[0]
+    keep = np.zeros(max(adj.size, 1), np.uint8)
+    counts = np.zeros(n + 1, np.int32)
+    load().orc_power_restrict(_p(sp), n, _p(adj), _p(off), _p(keep), _p(counts))
+    return keep[:adj.size], counts
+
+
+def power_face(spheres, a, b, neighbours):
+    """The edge test of power_restrict for (a, b) against a's Cech list."""
+    sp = _f32(spheres)
+    nb = np.ascontiguousarray(neighbours, np.int32)
+    return bool(load().orc_power_face(_p(sp), int(a), int(b), _p(nb), nb.size))
+
+
+def restricted_adjacency(spheres, adjacency, offsets):
+    """(adjacency int32, offsets int32) of the kept entries."""
+    keep, counts = power_restrict(spheres, adjacency, offsets)
+    return np.asarray(adjacency)[keep.astype(bool)].astype(np.int32), np.cumsum(counts).astype(np.int32)
 
 
 def live_sites(eye, spheres, adjacency, offsets, diff=None, kind=0, return_margin=False):

@@ -244,6 +244,9 @@ void orc_set_fragile(double h2_rel, double h2_abs, double dt_rel) {
 #undef R
 #undef SFX
 
+/* ------------------------------------- restricted power adjacency (FP64) */
+#include "orc_power.h"
+
 /* ------------------------------------------------ ABI layout (for tests) */
 #include <stddef.h>
 /* sizeof and field offsets of the C-ABI structs as gcc lays them out. */

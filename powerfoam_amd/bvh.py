@@ -64,3 +64,31 @@ class AABBTree:
             if int(dropped) != 0:
                 raise PfError(f"Cech emit dropped {int(dropped)} entries that the count pass did not see")
         return adjacent, offsets.to(torch.int32)
+
+
+def restricted_power_adjacency(centers, radii, adjacency=None, offsets=None):
+    """The power faces that bound each site's cell inside its ball
+    (pf_power_restrict): from the Cech lists (built here when not given), the
+    entries whose radical-plane face meets the two balls, plus one-way faces
+    for sites whose cell misses their ball. Renders as the Cech adjacency does,
+    with the faces that lie outside the balls left out. Returns (adjacency
+    int32, offsets int32)."""
+    dev = centers.device
+    if adjacency is None:
+        tree = AABBTree(dev)
+        tree.update(centers, radii)
+        adjacency, offsets = tree.build_cech_complex()
+    lib = _abi.load()
+    n = centers.shape[0]
+    if offsets.shape != (n + 1,):
+        raise ValueError("offsets must be [N+1]")
+    sp = torch.cat([centers.detach().float(), radii.detach().float()[:, None]], 1).contiguous()
+    adjacency = adjacency.to(dev, torch.int32).contiguous()
+    offsets = offsets.to(dev, torch.int32).contiguous()
+    m = adjacency.numel()
+    keep = torch.empty(max(m, 1), dtype=torch.uint8, device=dev)
+    counts = torch.empty(n + 1, dtype=torch.int32, device=dev)
+    check(lib.pf_power_restrict(ptr(sp), n, ptr(adjacency), ptr(offsets), m, ptr(keep), ptr(counts),
+                                stream_handle()))
+    out = adjacency[keep[:m].bool()]
+    return out, torch.cumsum(counts, 0).to(torch.int32)
