@@ -424,6 +424,31 @@ int pf_power_restrict(const float* spheres, int32_t n_sites, const int32_t* adja
                       const int32_t* adjacency_offsets, int64_t n_edges, uint8_t* keep, int32_t* counts,
                       void* stream);
 
+/* ---- site ordering and the interpenetration regulariser (geometry.py) ----
+ * pf_morton_sort: morton_code_kernel / morton_sort (geometry.py:18-85).
+ * points [N,3]; order [N] receives the stable ascending order of the 63-bit
+ * Morton codes (21 bits per axis over the points' bounding box, extents
+ * floored at 1e-6, x-major interleave); codes [N] (or NULL) the unsorted code
+ * of every point. The workspace (pf_morton_workspace_size bytes, 256-byte
+ * aligned) holds the bounds, keys and the radix sort's storage. NaN
+ * coordinates are left out of the bounds and quantise to 0. */
+size_t pf_morton_workspace_size(int32_t n_points);
+int pf_morton_sort(const float* points, int32_t n_points, void* workspace, size_t workspace_bytes, int64_t* codes,
+                   int32_t* order, void* stream);
+/* interpenetration_kernel geometry.py:88-117 and its adjoint (:150-189):
+ * areas[i] = sum over i's neighbours j with d < r_i + r_j of (r_i + r_j - d)^2,
+ * d = |c_i - c_j|. spheres [N,4]; areas [N] overwritten; grad_spheres [N,4]
+ * accumulated (zero it first; xyz and r gradients, no direction term when
+ * d = 0). Neighbour ids outside [0, N) are skipped. pairing tells the
+ * backward what the lists guarantee: 0 nothing; 1 every list ascending without
+ * repeats (entries whose reverse exists are then summed without atomics);
+ * 2 symmetric (every entry's reverse exists once, as in Cech lists). */
+int pf_interpenetration_fwd(const float* spheres, int32_t n_sites, const int32_t* adjacency,
+                            const int32_t* adjacency_offsets, int64_t n_edges, float* areas, void* stream);
+int pf_interpenetration_bwd(const float* spheres, int32_t n_sites, const int32_t* adjacency,
+                            const int32_t* adjacency_offsets, int64_t n_edges, int32_t pairing,
+                            const float* grad_areas, float* grad_spheres, void* stream);
+
 /* ---- adjacency encoding (prefetch_adjacency_kernel rasterize.py:580-606) */
 /* diff[e] = binary16(c_adj - c, r_adj) for every edge of every site.
  * n_edges is the length of adjacency and diff (adjacency_offsets[n_sites]);

@@ -275,6 +275,11 @@ SIGNATURES = {
     "pf_cech_count": ([_VP, ctypes.c_int32, _VP, ctypes.c_size_t, _VP, _VP], ctypes.c_int),
     "pf_cech_emit": ([_VP, ctypes.c_int32, _VP, ctypes.c_size_t, _VP, _VP, _VP, _VP], ctypes.c_int),
     "pf_power_restrict": ([_VP, ctypes.c_int32, _VP, _VP, ctypes.c_int64, _VP, _VP, _VP], ctypes.c_int),
+    "pf_morton_workspace_size": ([ctypes.c_int32], ctypes.c_size_t),
+    "pf_morton_sort": ([_VP, ctypes.c_int32, _VP, ctypes.c_size_t, _VP, _VP, _VP], ctypes.c_int),
+    "pf_interpenetration_fwd": ([_VP, ctypes.c_int32, _VP, _VP, ctypes.c_int64, _VP, _VP], ctypes.c_int),
+    "pf_interpenetration_bwd": ([_VP, ctypes.c_int32, _VP, _VP, ctypes.c_int64, ctypes.c_int32, _VP, _VP, _VP],
+                                ctypes.c_int),
     "pf_raster_fwd": ([_VP, _VP], ctypes.c_int),
     "pf_raster_bwd": ([_VP, _VP], ctypes.c_int),
     "pf_raster_rgb": ([_VP, _VP], ctypes.c_int),

@@ -63,6 +63,7 @@ class AABBTree:
             check(lib.pf_bin_sort(ptr(offsets), n, ptr(records), ptr(adjacent), ptr(sort_ws), sort_bytes, st))
             if int(dropped) != 0:
                 raise PfError(f"Cech emit dropped {int(dropped)} entries that the count pass did not see")
+        adjacent._pf_pairing = 2  # symmetric lists (read by InterpenetrationFunction)
         return adjacent, offsets.to(torch.int32)
 
 
@@ -91,4 +92,5 @@ def restricted_power_adjacency(centers, radii, adjacency=None, offsets=None):
     check(lib.pf_power_restrict(ptr(sp), n, ptr(adjacency), ptr(offsets), m, ptr(keep), ptr(counts),
                                 stream_handle()))
     out = adjacency[keep[:m].bool()]
+    out._pf_pairing = 1  # ascending without repeats; one-way entries for sites outside their cells
     return out, torch.cumsum(counts, 0).to(torch.int32)

@@ -151,12 +151,23 @@ __global__ void __launch_bounds__(256) cech_bounds_kernel(const float4* __restri
     }
     rm = fmaxf(rm, __shfl_xor(rm, m, 64));
   }
+  // one atomic per value per workgroup: the waves meet in LDS first
+  __shared__ float s_b[4][7];
+  const int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
     for (int k = 0; k < 3; k++) {
-      atomicMin(&acc[k], ord_of(mn[k]));
-      atomicMax(&acc[3 + k], ord_of(mx[k]));
+      s_b[w][k] = mn[k];
+      s_b[w][3 + k] = mx[k];
     }
-    atomicMax(&acc[6], ord_of(rm));
+    s_b[w][6] = rm;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const int k = threadIdx.x;
+    atomicMin(&acc[k], ord_of(fminf(fminf(s_b[0][k], s_b[1][k]), fminf(s_b[2][k], s_b[3][k]))));
+  } else if (threadIdx.x < 7) {
+    const int k = threadIdx.x;
+    atomicMax(&acc[k], ord_of(fmaxf(fmaxf(s_b[0][k], s_b[1][k]), fmaxf(s_b[2][k], s_b[3][k]))));
   }
 }
 
@@ -478,7 +489,7 @@ int pf_cech_count(const float* spheres, int32_t n_sites, void* workspace, size_t
     const int64_t nb1 = w.buckets + 1;
     const unsigned tiles = (unsigned)cech_scan_tiles(w.buckets);
     cech_init_kernel<<<1, 64, 0, st>>>(w.acc);
-    cech_bounds_kernel<<<min(blocks, 1024u), 256, 0, st>>>(sp, n_sites, w.acc);
+    cech_bounds_kernel<<<min(blocks, 512u), 256, 0, st>>>(sp, n_sites, w.acc);
     cech_grid_kernel<<<1, 64, 0, st>>>(w.acc, w.grid, (uint32_t)(w.buckets - 1));
     cech_fill_kernel<<<cech_blocks(nb1, 256), 256, 0, st>>>(w.bucket_start, nb1);
     cech_fill_kernel<<<cech_blocks(w.buckets, 256), 256, 0, st>>>(w.bucket_fill, w.buckets);
