@@ -346,7 +346,9 @@ __global__ void __launch_bounds__(TWAVE* TWPB) trace_fwd_kernel(TraceParams p, i
 // slot of that site, and a slot goes out with one atomic per value when
 // another site claims it or the wave ends. Slot = site mod GC_SLOTS; a lane
 // whose slot another lane's site won this step adds to global memory
-// directly. Only the summation order of the gradients changes.
+// directly. The sphere gradients a visit sends to the neighbours across its
+// entry and exit faces join the neighbour's slot the same way when the slot
+// holds that neighbour. Only the summation order of the gradients changes.
 constexpr int GC_SLOTS = 32;
 constexpr int GC_VALS = 7 * PF_MAX_TEXELS + 8;
 
@@ -617,7 +619,12 @@ __global__ void __launch_bounds__(TWAVE* TWPB) trace_bwd_kernel(TraceParams p) {
             dLdc = dLdc + (oc * idp + qv) * g;
             dLdr += g * (r * idp);
             const f3 ga = ((a - o) * idp - qv) * g;
-            float* gadj = p.g_spheres + 4 * (size_t)adj;
+            // the neighbour is mostly a cell this wave walks too (the one
+            // before or after): its sphere gradient joins that site's cache
+            // slot when the slot holds it (tags only change at the top of a
+            // step), else goes to global memory
+            const int aslot = adj & (GC_SLOTS - 1);
+            float* gadj = gc.tag[aslot] == adj ? gc.v[aslot] + 7 * T : p.g_spheres + 4 * (size_t)adj;
             atomicAdd(gadj + 0, ga.x);
             atomicAdd(gadj + 1, ga.y);
             atomicAdd(gadj + 2, ga.z);
