@@ -348,7 +348,10 @@ __global__ void __launch_bounds__(TWAVE* TWPB) trace_fwd_kernel(TraceParams p, i
 // whose slot another lane's site won this step adds to global memory
 // directly. The sphere gradients a visit sends to the neighbours across its
 // entry and exit faces join the neighbour's slot the same way when the slot
-// holds that neighbour. Only the summation order of the gradients changes.
+// holds that neighbour. Lanes walk a site's texels (and the four neighbour
+// values) in an order rotated by lane, so lanes on one site add to different
+// words of a slot at a time (an LDS atomic serialises the lanes that hit one
+// word). Only the summation order of the gradients changes.
 constexpr int GC_SLOTS = 32;
 constexpr int GC_VALS = 7 * PF_MAX_TEXELS + 8;
 
@@ -446,13 +449,12 @@ __global__ void __launch_bounds__(TWAVE* TWPB) trace_bwd_kernel(TraceParams p) {
     const float sigma = ns.w;
     const f3 pn{ns.x, ns.y, ns.z};
     const f3 co = -oc;
-    float w1[PF_MAX_TEXELS];
     const float tn0 = tn;
     const int tn0_id = tn_id, tf0_id = tf_id;
     TexelEval te{};
     float dt = -1.0f;
     if (live && hit && !(tn > tf) && !(sigma < 1e-3f)) {
-      te = texel_eval(p, prim, c, r, pn, o, d, tn, w1);
+      te = texel_eval(p, prim, c, r, pn, o, d, tn, nullptr);
       if (te.dpn >= 0.0f && te.ts1 < tf) {
         tf_id = n_adj;
         tf = te.ts1;
@@ -532,11 +534,16 @@ __global__ void __launch_bounds__(TWAVE* TWPB) trace_bwd_kernel(TraceParams p) {
         const f3 x1 = d * tq - co;
         f3 dLdx{0.0f, 0.0f, 0.0f};
         const float gdp_in = dot(dLdcol, te.col);
-        for (int k = 0; k < T; k++) {
+        // lanes start at texel (lane mod T): lanes on one site then add to T
+        // different cache words at a time instead of all to the same one
+        const int rot = lane % T;
+        for (int kk = 0; kk < T; kk++) {
+          const int k = kk + rot < T ? kk + rot : kk + rot - T;
           const float* ts = p.texel_sites + ((size_t)prim * T + k) * 3;
           const float4 cl = p.texel_rgbh[(size_t)prim * T + k];
           const f3 df{x1.x - (ts[0] - c.x), x1.y - (ts[1] - c.y), x1.z - (ts[2] - c.z)};
-          const float wn = w1[k] * inv_ws;
+          // the colour-stage weight, recomputed as texel_eval forms it
+          const float wn = fast_exp(dot(df, df) * kexp) * inv_ws;
           acc(3 * k + 0, dLdcol.x * wn);
           acc(3 * k + 1, dLdcol.y * wn);
           acc(3 * k + 2, dLdcol.z * wn);
@@ -561,11 +568,14 @@ __global__ void __launch_bounds__(TWAVE* TWPB) trace_bwd_kernel(TraceParams p) {
         const f3 x0 = d * tq0 - co;
         f3 dLdx0{0.0f, 0.0f, 0.0f};
         const float gdp0_in = dLdheight * te.height;
-        for (int k = 0; k < T; k++) {
+        for (int kk = 0; kk < T; kk++) {
+          const int k = kk + rot < T ? kk + rot : kk + rot - T;
           const float* ts = p.texel_sites + ((size_t)prim * T + k) * 3;
           const float hk = p.texel_rgbh[(size_t)prim * T + k].w;
           const f3 df{x0.x - (ts[0] - c.x), x0.y - (ts[1] - c.y), x0.z - (ts[2] - c.z)};
-          const float w = te.heights ? fast_exp(dot(df, df) * kexp) : w1[k];
+          // with all heights zero x0 is the colour-stage point, and this is
+          // the colour-stage weight
+          const float w = fast_exp(dot(df, df) * kexp);
           const float wn0 = w * inv_ws0;
           acc(6 * T + k, wn0 * dLdheight);
           // all heights zero: hk = 0 and gdp0_in = 0, so sg is zero
@@ -625,10 +635,12 @@ __global__ void __launch_bounds__(TWAVE* TWPB) trace_bwd_kernel(TraceParams p) {
             // step), else goes to global memory
             const int aslot = adj & (GC_SLOTS - 1);
             float* gadj = gc.tag[aslot] == adj ? gc.v[aslot] + 7 * T : p.g_spheres + 4 * (size_t)adj;
-            atomicAdd(gadj + 0, ga.x);
-            atomicAdd(gadj + 1, ga.y);
-            atomicAdd(gadj + 2, ga.z);
-            atomicAdd(gadj + 3, g * (-as.w * idp));
+            const float gr = g * (-as.w * idp);
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+              const int k = (i + lane) & 3;
+              atomicAdd(gadj + k, k == 0 ? ga.x : k == 1 ? ga.y : k == 2 ? ga.z : gr);
+            }
           }
         }
         acc(7 * T + 0, dLdc.x);
