@@ -170,21 +170,31 @@ def mlp_head_record(args, dev, psnr_views=4):
     w = build_workload(args.mlp_sites, args.views, args.width, args.height, dev, seed=2)
     rast = Rasterizer(SimpleNamespace(num_texel_sites=8, render_objective="volume", is_pinhole=True), dev)
     sh = w["params"]["sh"].detach()
-    mlp = MLPColor(sh.shape[0], dev, seed=0, latents=latents_from_sh(sh))
-    t0 = time.time()
-    hist = fit(mlp, sh, steps=args.mlp_steps, batch=args.mlp_batch, log_every=max(1, args.mlp_steps // 8))
-    torch.cuda.synchronize()
-    train_s = time.time() - t0
-    log(f"trained the MLP head: {args.mlp_steps} steps x {args.mlp_batch} pairs in {train_s:.1f}s, loss {hist}")
-    sh_head, mlp_head = SHColor(), MlpSlot(mlp)
+    sh_head = SHColor()
+    cams = w["cams"][:psnr_views]
+    # each direction encoding trained the same way; the frames are timed with
+    # args.mlp_encoding, the other one's PSNR is reported beside it
+    psnr_by_enc, trained = {}, {}
+    for enc in [args.mlp_encoding] + [e for e in ("freq", "sh") if e != args.mlp_encoding]:
+        m = MLPColor(sh.shape[0], dev, seed=0, latents=latents_from_sh(sh), encoding=enc)
+        t0 = time.time()
+        h = fit(m, sh, steps=args.mlp_steps, batch=args.mlp_batch, log_every=max(1, args.mlp_steps // 8))
+        torch.cuda.synchronize()
+        trained[enc] = (m, h, time.time() - t0)
+        psnr_by_enc[enc] = render_psnr(rast, w, MlpSlot(m), sh_head, cams)
+        log(f"trained the MLP head ({enc} encoding): {args.mlp_steps} steps x {args.mlp_batch} pairs in "
+            f"{trained[enc][2]:.1f}s, loss {h}, render PSNR {psnr_by_enc[enc]:.2f} dB")
+        if enc != args.mlp_encoding:
+            del m
+    mlp, hist, train_s = trained[args.mlp_encoding]
+    mlp_head = MlpSlot(mlp)
 
     class _DC:
         def forward(self, texel_sites, camera, sh_):
             n, T = texel_sites.shape[:2]
             return torch.clamp(sh_[:, 0, :].detach(), min=0.0)[:, None, :].expand(n, T, 3).contiguous()
 
-    cams = w["cams"][:psnr_views]
-    psnr_mlp = render_psnr(rast, w, mlp_head, sh_head, cams)
+    psnr_mlp = psnr_by_enc[args.mlp_encoding]
     psnr_dc = render_psnr(rast, w, _DC(), sh_head, cams)
     log(f"render PSNR vs the SH render ({len(cams)} views): MLP {psnr_mlp:.2f} dB, view-independent DC "
         f"{psnr_dc:.2f} dB")
@@ -207,10 +217,12 @@ def mlp_head_record(args, dev, psnr_views=4):
                mlp_ms_per_frame=mlp_s * 1e3 / steps, sh_ms_per_frame=sh_s * 1e3 / steps,
                speedup_vs_sh_head=sh_s / mlp_s, frames=steps,
                psnr_vs_sh_render_db=psnr_mlp, psnr_dc_only_vs_sh_render_db=psnr_dc, psnr_views=len(cams),
+               encoding=args.mlp_encoding, psnr_vs_sh_render_db_by_encoding=psnr_by_enc,
                training_pairs=args.mlp_steps * args.mlp_batch, train_seconds=train_s,
                train_loss_first_last=[hist[0][1], hist[-1][1]] if hist else None,
                kernel_ms_mlp=mlp_k, kernel_ms_sh=sh_k, mfma=mfma,
-               head="16-dim bf16 latent + SH-basis direction encoding -> 64 -> 64 -> 3, bf16 MFMA, fp32 accumulate")
+               head="16-dim bf16 latent + 16-feature direction encoding (freq: 1, d, sin/cos(pi d), "
+                    "sin/cos(2 pi d); sh: 1, Y_1..Y_15) -> 64 -> 64 -> 3, bf16 MFMA, fp32 accumulate")
     log(f"learned head: MLP {rec['mlp_ms_per_frame']:.3f} ms vs SH {rec['sh_ms_per_frame']:.3f} ms per frame; "
         f"MFMA {({k: round(v['frac'], 4) for k, v in mfma.items()})}")
     del w, rast, mlp
@@ -529,6 +541,8 @@ def main():
     ap.add_argument("--mlp-steps", type=int, default=2000)
     ap.add_argument("--mlp-batch", type=int, default=65536)
     ap.add_argument("--mlp-frames", type=int, default=16)
+    ap.add_argument("--mlp-encoding", choices=("freq", "sh"), default="freq",
+                    help="direction encoding of the timed MLP head (the other is trained for its PSNR)")
     ap.add_argument("--workload", choices=("raster", "trace", "cech"), default="raster",
                     help="raster: the rasterizer frame (default); trace: the ray tracer on fisheye views; "
                          "cech: the Cech adjacency build")

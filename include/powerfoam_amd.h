@@ -544,7 +544,13 @@ typedef struct pf_mlp_rows {
   const int32_t* pair_sites; /* [n_rows] or NULL */
   const float* pair_dirs;    /* [n_rows,3] */
   const uint16_t* latents;   /* [N,16] bf16 */
+  /* direction encoding e(d), 16 features: PF_MLP_ENC_SH (1, Y_1..Y_15: the
+   * SH head's basis) or PF_MLP_ENC_FREQ (1, d, sin(pi d), cos(pi d),
+   * sin(2 pi d), cos(2 pi d), per component) */
+  int32_t encoding;
 } pf_mlp_rows;
+#define PF_MLP_ENC_SH 0
+#define PF_MLP_ENC_FREQ 1
 
 /* fp32 gradient buffers; the caller zeroes them first (weights and biases
  * are accumulated into; a texel-mode site whose rows one wave step covers has

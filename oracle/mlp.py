@@ -3,7 +3,8 @@ colour head (powerfoam_amd/csrc/mlp.hip) that tests/ use as the checker; the
 product never imports it. The head has no reference counterpart: it fills
 the texel-colour slot of the reference's SphericalVoronoi pre-pass
 (color_fn.py:54-184), with the encoding e(d) = (1, Y_1(d) .. Y_15(d)) of the
-SH basis (sh_utils.py:4-49).
+SH basis (sh_utils.py:4-49), or the 16-feature frequency encoding
+(1, d, sin(pi d), cos(pi d), sin(2 pi d), cos(2 pi d)).
 
 numpy, FP64 arithmetic. With rounding=True it rounds to bf16 (nearest-even)
 at the points the kernels do (inputs, weights, h1, h2; in the backward the
@@ -26,10 +27,15 @@ def bf16(x):
     return r.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
-def encode(d):
-    """e(d) = (1, Y_1..Y_15) at unit directions d [...,3]."""
-    y = sh_basis(np.asarray(d, np.float64))
-    return np.concatenate([np.ones(y.shape[:-1] + (1,)), y], -1)
+def encode(d, encoding="sh"):
+    """e(d) at unit directions d [...,3]: (1, Y_1..Y_15) for "sh"; for "freq"
+    (1, d, sin(pi d), cos(pi d), sin(2 pi d), cos(2 pi d)), per component."""
+    d = np.asarray(d, np.float64)
+    one = np.ones(d.shape[:-1] + (1,))
+    if encoding == "freq":
+        a = np.pi * d
+        return np.concatenate([one, d, np.sin(a), np.cos(a), np.sin(2 * a), np.cos(2 * a)], -1)
+    return np.concatenate([one, sh_basis(d)], -1)
 
 
 def rows_texels(texel_sites, eye, T):
@@ -41,12 +47,12 @@ def rows_texels(texel_sites, eye, T):
     return np.arange(ts.shape[0]) // T, d
 
 
-def forward(params, sites, dirs, rounding=True):
+def forward(params, sites, dirs, rounding=True, encoding="sh"):
     """rgb [R,3] and the saved activations. params: dict of latents [N,16],
     w1 [64,32], b1, w2 [64,64], b2, w3 [3,64], b3 (any float dtype)."""
     q = bf16 if rounding else (lambda a: np.asarray(a, np.float64))
     z = q(params["latents"])[sites]
-    e = q(encode(dirs))
+    e = q(encode(dirs, encoding))
     x = np.concatenate([z, e], -1)
     w1, w2, w3 = q(params["w1"]), q(params["w2"]), q(params["w3"])
     b1, b2, b3 = (np.asarray(params[k], np.float64) for k in ("b1", "b2", "b3"))
@@ -59,10 +65,10 @@ def forward(params, sites, dirs, rounding=True):
     return np.maximum(o, 0.0), saved
 
 
-def backward(params, sites, dirs, grad_rgb, rounding=True):
+def backward(params, sites, dirs, grad_rgb, rounding=True, encoding="sh"):
     """Gradients of sum(grad_rgb * rgb) w.r.t. latents and weights (dict)."""
     q = bf16 if rounding else (lambda a: np.asarray(a, np.float64))
-    _, s = forward(params, sites, dirs, rounding)
+    _, s = forward(params, sites, dirs, rounding, encoding)
     g = np.asarray(grad_rgb, np.float64)
     d_o = np.where(s["o"] > 0.0, g, 0.0)
     d_ob = q(d_o)

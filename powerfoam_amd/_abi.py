@@ -27,6 +27,8 @@ TILE_SIZE = 64
 MAX_TEXELS = 8
 SORT_STATIC_MAX_TILES = 262144  # PF_SORT_STATIC_MAX_TILES
 CECH_MAX_SITES = 1 << 29  # PF_CECH_MAX_SITES
+MLP_ENC_SH = 0  # PF_MLP_ENC_SH
+MLP_ENC_FREQ = 1  # PF_MLP_ENC_FREQ
 
 c_float_p = ctypes.POINTER(ctypes.c_float)
 c_i32_p = ctypes.POINTER(ctypes.c_int32)
@@ -178,6 +180,7 @@ class PfMlpRows(ctypes.Structure):
         ("pair_sites", ctypes.c_void_p),
         ("pair_dirs", ctypes.c_void_p),
         ("latents", ctypes.c_void_p),
+        ("encoding", ctypes.c_int32),
     ]
 
 

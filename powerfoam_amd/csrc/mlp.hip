@@ -3,12 +3,15 @@
 // Fills the same slot as the SH head (the reference's per-texel colour
 // pre-pass, color_fn.py:54-184); there is no reference counterpart. For a row
 // (a texel, or a training pair) of site s seen along unit direction d:
-//   x   = [ z_s (16, bf16) ; e(d) (16) ],  e(d) = (1, Y_1(d) .. Y_15(d)),
+//   x   = [ z_s (16, bf16) ; e(d) (16) ],  e(d) = (1, Y_1(d) .. Y_15(d)), or
+//         with the frequency encoding e(d) = (1, d, sin pi d, cos pi d,
+//         sin 2 pi d, cos 2 pi d) (per component; 16 features),
 //   h1  = relu(W1 x + b1)        W1 [64,32]
 //   h2  = relu(W2 h1 + b2)       W2 [64,64]
 //   rgb = max(0, W3 h2 + b3)     W3 [3,64]
 // with Y_k the real SH basis of the SH head, so e(d) is exactly the vector the
-// SH head contracts with its 16 coefficients. Weights and latents are bf16; x,
+// SH head contracts with its 16 coefficients (the frequency encoding carries
+// no such head start). Weights and latents are bf16; x,
 // h1 and h2 are rounded to bf16 (nearest-even) before each product; products
 // accumulate in FP32 (v_mfma_f32_16x16x32_bf16); biases are FP32.
 //
@@ -36,6 +39,7 @@ struct MlpParams {
   f3 eye;
   const int32_t* pair_sites;  // pair mode (when set): site and direction per row
   const float* pair_dirs;
+  int encoding;               // PF_MLP_ENC_SH or PF_MLP_ENC_FREQ
   const uint16_t* latents;  // [N,16] bf16
   const uint16_t* w1;
   const float* b1;
@@ -132,7 +136,26 @@ __device__ __forceinline__ void stage_row(const MlpParams& p, int row, uint16_t*
     d = texel_dir(p.texel_sites + 3 * (size_t)rc, p.eye);
   }
   float y[15];
-  sh_basis(d, y);
+  if (p.encoding == PF_MLP_ENC_FREQ) {
+    // (d, sin pi d, cos pi d, sin 2 pi d, cos 2 pi d) after the constant
+    float sn[3], cs[3], sn2[3], cs2[3];
+    const float dd[3] = {d.x, d.y, d.z};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      sincosf(3.14159265358979f * dd[k], &sn[k], &cs[k]);
+      sn2[k] = 2.0f * sn[k] * cs[k];        // sin 2x
+      cs2[k] = 1.0f - 2.0f * sn[k] * sn[k];  // cos 2x
+      y[k] = dd[k];
+      y[3 + k] = sn[k];
+      y[6 + k] = cs[k];
+      y[9 + k] = sn2[k];
+    }
+    y[12] = cs2[0];
+    y[13] = cs2[1];
+    y[14] = cs2[2];
+  } else {
+    sh_basis(d, y);
+  }
   const uint32_t m = ok ? 0xFFFFFFFFu : 0u;
   const uint4 q0 = lp[0], q1 = lp[1];
   uint4* o = reinterpret_cast<uint4*>(dst);
@@ -595,6 +618,9 @@ static int make_params(const pf_mlp_weights* w, const pf_mlp_rows* rows, MlpPara
   p.eye = f3{rows->eye[0], rows->eye[1], rows->eye[2]};
   p.pair_sites = rows->pair_sites;
   p.pair_dirs = rows->pair_dirs;
+  if (rows->encoding != PF_MLP_ENC_SH && rows->encoding != PF_MLP_ENC_FREQ)
+    return set_error(PF_ERR_ARG, "%s: unknown direction encoding %d", who, rows->encoding);
+  p.encoding = rows->encoding;
   p.latents = rows->latents;
   p.w1 = w->w1;
   p.b1 = w->b1;

@@ -50,9 +50,18 @@ class MLPFn(torch.autograd.Function):
         return (None, None, None) + grads
 
 
+ENCODINGS = {"sh": _abi.MLP_ENC_SH, "freq": _abi.MLP_ENC_FREQ}
+
+
 class MLPColor:
-    def __init__(self, n_sites, device, seed=0, latents=None):
-        """Weights drawn with He initialisation from `seed`; latents given or zero."""
+    def __init__(self, n_sites, device, seed=0, latents=None, encoding="sh"):
+        """Weights drawn with He initialisation from `seed`; latents given or
+        zero. encoding: the 16 direction features, "sh" (1, Y_1..Y_15, the SH
+        head's basis) or "freq" (1, d, sin(pi d), cos(pi d), sin(2 pi d),
+        cos(2 pi d) per component)."""
+        if encoding not in ENCODINGS:
+            raise ValueError(f"encoding must be one of {sorted(ENCODINGS)}")
+        self.encoding = encoding
         self.lib = _abi.load()
         self.device = torch.device(device)
         g = torch.Generator(device="cpu").manual_seed(seed)
@@ -93,6 +102,7 @@ class MLPColor:
         r.pair_sites = a["pair_sites"].data_ptr() if "pair_sites" in a else None
         r.pair_dirs = a["pair_dirs"].data_ptr() if "pair_dirs" in a else None
         r.latents = bf["latents"].data_ptr()
+        r.encoding = ENCODINGS[self.encoding]
         return r
 
     def _launch(self, name, fn, *a):
@@ -190,7 +200,7 @@ def fit(head, sh, steps=2000, batch=65536, lr=3e-3, lr_latent=3e-2, seed=0, log_
         opt.step()
         sched.step()
         if log_every and (it % log_every == 0 or it == steps - 1):
-            hist.append((it, float(loss)))
+            hist.append((it, float(loss.detach())))
     return hist
 
 

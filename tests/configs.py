@@ -86,19 +86,20 @@ def c3_scene():
 
 
 @functools.lru_cache(maxsize=None)
-def c3_head():
+def c3_head(encoding="sh"):
     """The learned colour head on the 3M-site scene, trained on the GPU with its
     own kernels: 2000 Adam steps of 65,536 SH-labelled (site, direction)
-    pairs. Returns (head, training loss history)."""
+    pairs, with the given direction encoding. Returns (head, training loss
+    history)."""
     import torch
     from powerfoam_amd.mlp_head import MLPColor, fit, latents_from_sh
     s = c3_scene()
     sh = torch.tensor(s["sh"], device="cuda")
-    head = MLPColor(sh.shape[0], "cuda", seed=0, latents=latents_from_sh(sh))
+    head = MLPColor(sh.shape[0], "cuda", seed=0, latents=latents_from_sh(sh), encoding=encoding)
     t0 = time.time()
     hist = fit(head, sh, steps=2000, batch=65536, log_every=500)
     torch.cuda.synchronize()
-    log(f"c3 head: trained in {time.time() - t0:.1f}s, loss {hist[0][1]:.4f} -> {hist[-1][1]:.4f}")
+    log(f"c3 head ({encoding}): trained in {time.time() - t0:.1f}s, loss {hist[0][1]:.4f} -> {hist[-1][1]:.4f}")
     return head, hist
 
 

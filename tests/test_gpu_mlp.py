@@ -18,11 +18,11 @@ from tests import harness
 pytestmark = pytest.mark.gpu
 
 
-def _head(n, seed):
+def _head(n, seed, encoding="sh"):
     import torch
     from powerfoam_amd.mlp_head import MLPColor
     rng = np.random.default_rng(seed)
-    head = MLPColor(n, "cuda", seed=seed, latents=rng.normal(0, 0.5, (n, 16)))
+    head = MLPColor(n, "cuda", seed=seed, latents=rng.normal(0, 0.5, (n, 16)), encoding=encoding)
     with torch.no_grad():
         head.params["b1"].copy_(torch.tensor(rng.normal(0, 0.1, 64)))
         head.params["b2"].copy_(torch.tensor(rng.normal(0, 0.1, 64)))
@@ -34,19 +34,20 @@ def _np_params(head):
     return {k: v.detach().cpu().numpy().astype(np.float64) for k, v in head.params.items()}
 
 
-def test_texel_forward_and_backward_match_oracle():
+@pytest.mark.parametrize("encoding", ["sh", "freq"])
+def test_texel_forward_and_backward_match_oracle(encoding):
     import torch
     from powerfoam_amd.camera import look_at
     n, T = 3001, 8  # ragged: the last 32-row step is partial
     s = synth.scene_uniform(11, n=n, n_texels=T)
-    head = _head(n, 1)
+    head = _head(n, 1, encoding)
     cam = look_at((0.3, 0.2, 2.5), (0.0, 0.0, 0.0), 64, 48, 60.0)
     ts = torch.tensor(s["texel_sites"], device="cuda")
-    print("mlp texel fwd/bwd n=3001 T=8", flush=True)
+    print(f"mlp texel fwd/bwd n=3001 T=8 encoding={encoding}", flush=True)
     rgb = head.forward(ts, cam)
     sites, dirs = om.rows_texels(s["texel_sites"], cam.eye.numpy(), T)
     p = _np_params(head)
-    ref, _ = om.forward(p, sites, dirs)
+    ref, _ = om.forward(p, sites, dirs, encoding=encoding)
     got = rgb.detach().cpu().numpy().reshape(-1, 3)
     err = np.abs(got - ref)
     print("fwd max", err.max(), "mean", err.mean(), flush=True)
@@ -54,16 +55,17 @@ def test_texel_forward_and_backward_match_oracle():
     assert (ref > 0).mean() > 0.5
     g = np.random.default_rng(2).normal(size=got.shape).astype(np.float32)
     (rgb.view(-1, 3) * torch.tensor(g, device="cuda")).sum().backward()
-    want = om.backward(p, sites, dirs, g)
+    want = om.backward(p, sites, dirs, g, encoding=encoding)
     errs = {k: harness.rel_err(v.grad.cpu().numpy(), want[k]) for k, v in head.params.items()}
     print(errs, flush=True)
     assert max(errs.values()) < 1e-3, errs
 
 
-def test_pair_forward_and_backward_match_oracle():
+@pytest.mark.parametrize("encoding", ["sh", "freq"])
+def test_pair_forward_and_backward_match_oracle(encoding):
     import torch
     n, B = 500, 5000
-    head = _head(n, 3)
+    head = _head(n, 3, encoding)
     rng = np.random.default_rng(4)
     sites = rng.integers(0, n, B).astype(np.int32)
     d = rng.normal(size=(B, 3))
@@ -71,12 +73,12 @@ def test_pair_forward_and_backward_match_oracle():
     print("mlp pair fwd/bwd", flush=True)
     rgb = head.forward_pairs(torch.tensor(sites, device="cuda"), torch.tensor(d, dtype=torch.float32, device="cuda"))
     p = _np_params(head)
-    ref, _ = om.forward(p, sites, d.astype(np.float32).astype(np.float64))
+    ref, _ = om.forward(p, sites, d.astype(np.float32).astype(np.float64), encoding=encoding)
     err = np.abs(rgb.detach().cpu().numpy() - ref)
     assert err.max() < 5e-3 and err.mean() < 5e-5, (err.max(), err.mean())
     g = rng.normal(size=ref.shape).astype(np.float32)
     (rgb * torch.tensor(g, device="cuda")).sum().backward()
-    want = om.backward(p, sites, d.astype(np.float32).astype(np.float64), g)
+    want = om.backward(p, sites, d.astype(np.float32).astype(np.float64), g, encoding=encoding)
     errs = {k: harness.rel_err(v.grad.cpu().numpy(), want[k]) for k, v in head.params.items()}
     print(errs, flush=True)
     assert max(errs.values()) < 1e-3, errs

@@ -93,3 +93,13 @@ def test_encoding_contracts_like_the_sh_head():
     d /= np.linalg.norm(d, axis=1, keepdims=True)
     lin = np.einsum("nk,nkc->nc", om.encode(d), sh)
     np.testing.assert_allclose(np.maximum(lin, 0), synth.sh_eval(sh, d), atol=1e-12)
+
+
+def test_frequency_encoding_known_values():
+    d = np.array([[1.0, 0.0, 0.0], [0.0, 0.5, -0.5]])
+    e = om.encode(d, "freq")
+    assert e.shape == (2, 16)
+    want0 = [1, 1, 0, 0, 0, 0, 0, -1, 1, 1, 0, 0, 0, 1, 1, 1]
+    want1 = [1, 0, 0.5, -0.5, 0, 1, -1, 1, 0, 0, 0, 0, 0, 1, -1, -1]
+    np.testing.assert_allclose(e[0], want0, atol=1e-12)
+    np.testing.assert_allclose(e[1], want1, atol=1e-12)
