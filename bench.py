@@ -81,9 +81,15 @@ def frame(rast, head, w, cam):
         v.grad = None
     if hasattr(head, "zero_grad"):
         head.zero_grad()
-    texel_rgb = head.forward(p["texel_sites"], cam, p["sh"])
-    out = rast.forward(cam, None, p["points"], p["radii"], p["density"], p["normals"], p["texel_sites"],
-                       texel_rgb, p["texel_height"], w["adjacency"], w["adjacency_offsets"], w["target"], True)
+    if getattr(head, "fused", False):
+        # the learned head evaluated inside the compositing forward
+        out = rast.forward(cam, None, p["points"], p["radii"], p["density"], p["normals"], p["texel_sites"],
+                           None, p["texel_height"], w["adjacency"], w["adjacency_offsets"], w["target"], True,
+                           color_head=head.mlp)
+    else:
+        texel_rgb = head.forward(p["texel_sites"], cam, p["sh"])
+        out = rast.forward(cam, None, p["points"], p["radii"], p["density"], p["normals"], p["texel_sites"],
+                           texel_rgb, p["texel_height"], w["adjacency"], w["adjacency_offsets"], w["target"], True)
     loss = torch.mean((out[0] - w["target"]) ** 2)
     loss.backward()
     return out
@@ -101,6 +107,35 @@ class MlpSlot:
     def zero_grad(self):
         for q in self.mlp.parameters():
             q.grad = None
+
+
+class MlpFused(MlpSlot):
+    """The learned head fused into the compositing forward (color_head=)."""
+    fused = True
+
+
+def mlp_bwd_rows(rast, mlp, w, cams):
+    """Mean over `cams` of the texel rows pf_mlp_bwd processes (32-row steps
+    holding a non-zero colour gradient; the others are skipped) and of the
+    rows with a non-zero gradient, from one pre-pass frame per view."""
+    import torch
+    p = w["params"]
+    done, nonzero = [], []
+    for cam in cams:
+        rgb = mlp.forward(p["texel_sites"], cam)
+        rgb.retain_grad()
+        out = rast.forward(cam, None, p["points"], p["radii"], p["density"], p["normals"], p["texel_sites"], rgb,
+                           p["texel_height"], w["adjacency"], w["adjacency_offsets"], w["target"], True)
+        torch.mean((out[0] - w["target"]) ** 2).backward()
+        g = rgb.grad.view(-1, 3) != 0
+        row = g.any(dim=1)
+        pad = (-row.numel()) % 32
+        steps = torch.nn.functional.pad(row, (0, pad)).view(-1, 32).any(dim=1)
+        done.append(int(steps.sum()) * 32)
+        nonzero.append(int(row.sum()))
+        for q in list(mlp.parameters()) + list(p.values()):
+            q.grad = None
+    return float(np.mean(done)), float(np.mean(nonzero))
 
 
 def time_frames(rast, head, w, steps, heads_log=None):
@@ -199,23 +234,32 @@ def mlp_head_record(args, dev, psnr_views=4):
     log(f"render PSNR vs the SH render ({len(cams)} views): MLP {psnr_mlp:.2f} dB, view-independent DC "
         f"{psnr_dc:.2f} dB")
     steps = args.mlp_frames
+    fused_head = MlpFused(mlp)
     for i in range(args.warmup):
         frame(rast, sh_head, w, w["cams"][i % len(w["cams"])])
         frame(rast, mlp_head, w, w["cams"][i % len(w["cams"])])
+        frame(rast, fused_head, w, w["cams"][i % len(w["cams"])])
     sh_s, sh_k = time_frames(rast, sh_head, w, steps)
     mlp_s, mlp_k = time_frames(rast, mlp_head, w, steps, heads_log=mlp)
+    fused_s, fused_k = time_frames(rast, fused_head, w, steps, heads_log=mlp)
+    # the backward skips 32-row steps without a colour gradient: its MFMA work
+    # is over the rows of the steps it processes (views of the timed frames)
+    bwd_rows, nz_rows = mlp_bwd_rows(rast, mlp, w, [w["cams"][i % len(w["cams"])] for i in range(min(steps, 8))])
     rows = sh.shape[0] * 8
     f_fwd, f_bwd = mlp_flops_per_row()
     mfma = {}
-    for name, f in (("mlp_fwd", f_fwd), ("mlp_bwd", f_bwd)):
+    for name, f, n_rows in (("mlp_fwd", f_fwd, rows), ("mlp_bwd", f_bwd, bwd_rows)):
         if name in mlp_k:
-            ach = f * rows / (mlp_k[name] * 1e-3) / 1e12
-            mfma[name] = dict(flops_per_launch=f * rows, ms=mlp_k[name], achieved=ach, peak=MFMA_BF16_PEAK_TFLOPS,
-                              unit="TFLOP/s", frac=ach / MFMA_BF16_PEAK_TFLOPS)
+            ach = f * n_rows / (mlp_k[name] * 1e-3) / 1e12
+            mfma[name] = dict(rows=n_rows, flops_per_launch=f * n_rows, ms=mlp_k[name], achieved=ach,
+                              peak=MFMA_BF16_PEAK_TFLOPS, unit="TFLOP/s", frac=ach / MFMA_BF16_PEAK_TFLOPS)
     rec = dict(workload=f"{args.mlp_sites}-site GMM scene (seed 2), {args.views} pinhole views "
                         f"{args.width}x{args.height}, fwd+bwd per frame with each colour head",
                mlp_ms_per_frame=mlp_s * 1e3 / steps, sh_ms_per_frame=sh_s * 1e3 / steps,
                speedup_vs_sh_head=sh_s / mlp_s, frames=steps,
+               mlp_fused_ms_per_frame=fused_s * 1e3 / steps, fused_speedup_vs_sh_head=sh_s / fused_s,
+               kernel_ms_mlp_fused=fused_k, mlp_bwd_rows_nonzero=nz_rows, mlp_bwd_rows_processed=bwd_rows,
+               texel_rows=rows,
                psnr_vs_sh_render_db=psnr_mlp, psnr_dc_only_vs_sh_render_db=psnr_dc, psnr_views=len(cams),
                encoding=args.mlp_encoding, psnr_vs_sh_render_db_by_encoding=psnr_by_enc,
                training_pairs=args.mlp_steps * args.mlp_batch, train_seconds=train_s,
@@ -223,8 +267,10 @@ def mlp_head_record(args, dev, psnr_views=4):
                kernel_ms_mlp=mlp_k, kernel_ms_sh=sh_k, mfma=mfma,
                head="16-dim bf16 latent + 16-feature direction encoding (freq: 1, d, sin/cos(pi d), "
                     "sin/cos(2 pi d); sh: 1, Y_1..Y_15) -> 64 -> 64 -> 3, bf16 MFMA, fp32 accumulate")
-    log(f"learned head: MLP {rec['mlp_ms_per_frame']:.3f} ms vs SH {rec['sh_ms_per_frame']:.3f} ms per frame; "
-        f"MFMA {({k: round(v['frac'], 4) for k, v in mfma.items()})}")
+    log(f"learned head: MLP {rec['mlp_ms_per_frame']:.3f} ms (fused into compositing "
+        f"{rec['mlp_fused_ms_per_frame']:.3f} ms) vs SH {rec['sh_ms_per_frame']:.3f} ms per frame; "
+        f"MFMA {({k: round(v['frac'], 4) for k, v in mfma.items()})}; kernels pre-pass {mlp_k}, fused {fused_k}, "
+        f"SH {sh_k}; backward rows {bwd_rows:.0f} processed, {nz_rows:.0f} non-zero of {rows}")
     del w, rast, mlp
     torch.cuda.empty_cache()
     return rec, mlp_s * 1e3 / steps

@@ -135,6 +135,10 @@ typedef struct pf_raster_fwd_args {
    * site shades some ray of its tile (an interval of positive length).
    * Not in the reference: the backward skips the other entries. */
   uint8_t* entry_shades;
+  /* The learned colour head evaluated inside the compositing (pf_mlp_head,
+   * below), or NULL: texel colours come from scene.texel_rgbh. Not in the
+   * reference. */
+  const struct pf_mlp_head* mlp_head;
 } pf_raster_fwd_args;
 
 typedef struct pf_raster_bwd_args {
@@ -566,10 +570,31 @@ typedef struct pf_mlp_grads {
   float* b3;
 } pf_mlp_grads;
 
+/* The head fused into the compositing forward (pf_raster_fwd_args.mlp_head).
+ * Each site that shades a tile gets its T texel colours from the head on the
+ * matrix cores inside the compositing kernel, with d = normalize(texel_site -
+ * cam.eye) as in texel mode; the results are bit-identical to pf_mlp_fwd's
+ * rows. The weights are staged once per workgroup in LDS; the colour
+ * channels of scene.texel_rgbh (or texel_rgbh_bf16) are not read, the
+ * heights are. The kernel writes (rgb, height) of every texel of every site
+ * that shades some tile into texel_rgbh_out [N,T,4] FP32 (16-byte aligned;
+ * it may be scene.texel_rgbh itself): the rows pf_raster_bwd then reads as
+ * its scene.texel_rgbh (texel_rgbh_bf16 NULL). Rows of other sites are left
+ * as they are; the backward reads only rows of sites the forward found
+ * shading. Its grad_texel_rgb, fed to pf_mlp_bwd in texel mode, gives the
+ * latent and weight gradients. */
+typedef struct pf_mlp_head {
+  pf_mlp_weights weights;
+  const uint16_t* latents;  /* [N,16] bf16, 16-byte aligned */
+  int32_t encoding;         /* PF_MLP_ENC_SH or PF_MLP_ENC_FREQ */
+  float* texel_rgbh_out;    /* [N,T,4] */
+} pf_mlp_head;
+
 /* rgb [n_rows,3]. */
 int pf_mlp_fwd(const pf_mlp_weights* w, const pf_mlp_rows* rows, float* rgb, void* stream);
 /* Adjoint of pf_mlp_fwd for dL/d rgb = grad_rgb [n_rows,3]; no gradient to
- * directions or texel positions (as color_fn.py:181-184). */
+ * directions or texel positions (as color_fn.py:181-184). Steps of 32 rows
+ * whose incoming gradients are all zero are skipped (they add nothing). */
 int pf_mlp_bwd(const pf_mlp_weights* w, const pf_mlp_rows* rows, const float* grad_rgb, pf_mlp_grads* g,
                void* stream);
 
@@ -577,7 +602,8 @@ int pf_mlp_bwd(const pf_mlp_weights* w, const pf_mlp_rows* rows, const float* gr
  * concatenations of rasterize.py:1619-1624: spheres [n,4] = (points [n,3],
  * radii [n]), nsigmas [n,4] = (normals [n,3], density [n]), texel_rgbh
  * [n,n_texels,4] = (texel_rgb [n,n_texels,3], texel_height [n,n_texels]).
- * Contiguous FP32 inputs; outputs 16-byte aligned. */
+ * Contiguous FP32 inputs; outputs 16-byte aligned. texel_rgb may be NULL
+ * (colours 0: the learned head fused into the compositing supplies them). */
 int pf_pack_sites(const float* points, const float* radii, const float* normals, const float* density,
                   const float* texel_rgb, const float* texel_height, int32_t n_sites, int32_t n_texels,
                   float* spheres, float* nsigmas, float* texel_rgbh, void* stream);

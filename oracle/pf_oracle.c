@@ -257,7 +257,9 @@ int orc_abi_layout(int64_t* out, int cap) {
       (int64_t)sizeof(pf_tile_lists), (int64_t)offsetof(pf_tile_lists, sites),
       (int64_t)sizeof(pf_raster_fwd_args), (int64_t)offsetof(pf_raster_fwd_args, tiles),
       (int64_t)offsetof(pf_raster_fwd_args, transmittance_threshold), (int64_t)offsetof(pf_raster_fwd_args, point_err),
-      (int64_t)offsetof(pf_raster_fwd_args, entry_shades),
+      (int64_t)offsetof(pf_raster_fwd_args, entry_shades), (int64_t)offsetof(pf_raster_fwd_args, mlp_head),
+      (int64_t)sizeof(pf_mlp_head), (int64_t)offsetof(pf_mlp_head, latents),
+      (int64_t)offsetof(pf_mlp_head, texel_rgbh_out),
       (int64_t)sizeof(pf_raster_bwd_args), (int64_t)offsetof(pf_raster_bwd_args, exact_gradients),
       (int64_t)offsetof(pf_raster_bwd_args, grad_texel_height), (int64_t)offsetof(pf_raster_bwd_args, entry_shades),
       (int64_t)sizeof(pf_raster_rgb_args), (int64_t)offsetof(pf_raster_rgb_args, color),

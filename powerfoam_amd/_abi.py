@@ -110,6 +110,7 @@ class PfRasterFwdArgs(ctypes.Structure):
         ("contrib", ctypes.c_void_p),
         ("point_err", ctypes.c_void_p),
         ("entry_shades", ctypes.c_void_p),
+        ("mlp_head", ctypes.c_void_p),
     ]
 
 
@@ -168,6 +169,15 @@ class PfRasterVisArgs(ctypes.Structure):
 
 class PfMlpWeights(ctypes.Structure):
     _fields_ = [(k, ctypes.c_void_p) for k in ("w1", "b1", "w2", "b2", "w3", "b3")]
+
+
+class PfMlpHead(ctypes.Structure):
+    _fields_ = [
+        ("weights", PfMlpWeights),
+        ("latents", ctypes.c_void_p),
+        ("encoding", ctypes.c_int32),
+        ("texel_rgbh_out", ctypes.c_void_p),
+    ]
 
 
 class PfMlpRows(ctypes.Structure):

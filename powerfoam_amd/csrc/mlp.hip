@@ -1,36 +1,16 @@
 // Learned colour head: a shared MLP on BF16 matrix cores.
 //
 // Fills the same slot as the SH head (the reference's per-texel colour
-// pre-pass, color_fn.py:54-184); there is no reference counterpart. For a row
-// (a texel, or a training pair) of site s seen along unit direction d:
-//   x   = [ z_s (16, bf16) ; e(d) (16) ],  e(d) = (1, Y_1(d) .. Y_15(d)), or
-//         with the frequency encoding e(d) = (1, d, sin pi d, cos pi d,
-//         sin 2 pi d, cos 2 pi d) (per component; 16 features),
-//   h1  = relu(W1 x + b1)        W1 [64,32]
-//   h2  = relu(W2 h1 + b2)       W2 [64,64]
-//   rgb = max(0, W3 h2 + b3)     W3 [3,64]
-// with Y_k the real SH basis of the SH head, so e(d) is exactly the vector the
-// SH head contracts with its 16 coefficients (the frequency encoding carries
-// no such head start). Weights and latents are bf16; x,
-// h1 and h2 are rounded to bf16 (nearest-even) before each product; products
-// accumulate in FP32 (v_mfma_f32_16x16x32_bf16); biases are FP32.
-//
-// MFMA use. The layers are computed transposed, H^T = W X^T, so each weight
-// matrix is the A operand (16 output units x 32 inputs) and the activations
-// are the B operand (32 inputs x 16 rows). For the 16x16x32 instruction lane
-// l = 16g + c16 holds A[c16][8g+j], B[8g+j][c16] and D[4g+j][c16], j = 0..7
-// (A, B) or 0..3 (D): a D tile gives each lane 4 consecutive units of one
-// row. (The 16x16x16 instruction, used for transposes, holds A[c16][4g+j],
-// B[4g+j][c16], D[4g+j][c16], j = 0..3.)
-#include "pf_common.hpp"
-#include "sh_basis.hpp"
+// pre-pass, color_fn.py:54-184); there is no reference counterpart. The row
+// arithmetic (layers, bf16 rounding points, MFMA operand layout and K-slot
+// order) is in mlp_core.hpp, shared with the compositing forward that
+// evaluates the head in place of a texel colour input. (The 16x16x16
+// instruction, used here for transposes, holds A[c16][4g+j], B[4g+j][c16],
+// D[4g+j][c16], j = 0..3.)
+#include "mlp_core.hpp"
 
 namespace pf {
 
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-
-constexpr int ML_LAT = 16, ML_IN = 32, ML_HID = 64;
 constexpr int ML_WPB = 4;  // forward waves per workgroup
 
 struct MlpParams {
@@ -58,18 +38,6 @@ struct MlpParams {
   float* g_b3;
 };
 
-__device__ __forceinline__ uint16_t to_bf(float x) { return __builtin_bit_cast(uint16_t, static_cast<__bf16>(x)); }
-
-__device__ __forceinline__ bf16x8 ld8(const uint16_t* p) {
-  return __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(p));
-}
-
-__device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
-  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
-}
-
-__device__ __forceinline__ f32x4 zero4() { return f32x4{0.0f, 0.0f, 0.0f, 0.0f}; }
-
 __device__ __forceinline__ bool row_ok(const MlpParams& p, int row) {
   if (row >= p.n_rows) return false;
   const int site = p.pair_sites ? p.pair_sites[row] : row / p.T;
@@ -77,46 +45,14 @@ __device__ __forceinline__ bool row_ok(const MlpParams& p, int row) {
 }
 
 // ---------------------------------------------------------------- forward
-// The forward chains the three products in registers. A D tile holds, per
-// lane, units 16t + 4g + j (j = 0..3) of row c16; two such tiles (t = 2q,
-// 2q + 1) packed to bf16 are exactly a B operand over the K-slot order
-//   slot 8g + j  <->  unit 32q + 4g + j         (j < 4)
-//                     unit 32q + 16 + 4g + j-4  (j >= 4),
-// so the next layer's A operand (its weights) is loaded with its columns in
-// that order and no activation goes through LDS. Weights stay in registers
-// for the whole persistent launch; biases ride in as the MFMA accumulator.
+// The forward chains the three products in registers (operand layout and
+// K-slot order in mlp_core.hpp): no activation goes through LDS. Weights stay
+// in registers for the whole persistent launch; biases ride in as the MFMA
+// accumulator.
 // The inputs of 64 rows are built one row per lane (latent copy, SH encoding)
 // and staged in LDS, from where each 16-row tile reads its B operand.
 constexpr int FW_ROWS = 64;  // rows per wave step
 constexpr int XS = 40;       // staged input row stride (bf16): 80 B, conflict-free 16-byte reads
-
-typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
-typedef float f32x2 __attribute__((ext_vector_type(2)));
-typedef short s16x2 __attribute__((ext_vector_type(2)));
-
-__device__ __forceinline__ uint32_t pk_bf(float a, float b) {
-  return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){a, b}, bf16x2));
-}
-
-// relu of two packed bf16: a bf16 is negative exactly when its bits are as int16
-__device__ __forceinline__ uint32_t relu_pk(uint32_t v) {
-  return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(s16x2, v), (s16x2){0, 0}));
-}
-
-// bf16(relu(z)) of two D tiles as one B operand (slot order above)
-__device__ __forceinline__ bf16x8 act_op(f32x4 lo, f32x4 hi) {
-  const uint4 u = {relu_pk(pk_bf(lo[0], lo[1])), relu_pk(pk_bf(lo[2], lo[3])), relu_pk(pk_bf(hi[0], hi[1])),
-                   relu_pk(pk_bf(hi[2], hi[3]))};
-  return __builtin_bit_cast(bf16x8, u);
-}
-
-// A operand of a 64-column weight row in slot order: columns 32q + 4g .. +3
-// and 32q + 16 + 4g .. +3 of row `row`.
-__device__ __forceinline__ bf16x8 ld_wrow(const uint16_t* w, int row, int q, int g) {
-  const uint2 a = *reinterpret_cast<const uint2*>(w + row * ML_HID + 32 * q + 4 * g);
-  const uint2 b = *reinterpret_cast<const uint2*>(w + row * ML_HID + 32 * q + 16 + 4 * g);
-  return __builtin_bit_cast(bf16x8, make_uint4(a.x, a.y, b.x, b.y));
-}
 
 // Input row x = [latent ; e(d)] of `row` as 32 bf16 at dst (16-byte aligned);
 // zero for rows past the end or naming no valid site.
@@ -136,33 +72,15 @@ __device__ __forceinline__ void stage_row(const MlpParams& p, int row, uint16_t*
     d = texel_dir(p.texel_sites + 3 * (size_t)rc, p.eye);
   }
   float y[15];
-  if (p.encoding == PF_MLP_ENC_FREQ) {
-    // (d, sin pi d, cos pi d, sin 2 pi d, cos 2 pi d) after the constant
-    float sn[3], cs[3], sn2[3], cs2[3];
-    const float dd[3] = {d.x, d.y, d.z};
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-      sincosf(3.14159265358979f * dd[k], &sn[k], &cs[k]);
-      sn2[k] = 2.0f * sn[k] * cs[k];        // sin 2x
-      cs2[k] = 1.0f - 2.0f * sn[k] * sn[k];  // cos 2x
-      y[k] = dd[k];
-      y[3 + k] = sn[k];
-      y[6 + k] = cs[k];
-      y[9 + k] = sn2[k];
-    }
-    y[12] = cs2[0];
-    y[13] = cs2[1];
-    y[14] = cs2[2];
-  } else {
-    sh_basis(d, y);
-  }
+  mlp_encode(d, p.encoding, y);
   const uint32_t m = ok ? 0xFFFFFFFFu : 0u;
   const uint4 q0 = lp[0], q1 = lp[1];
   uint4* o = reinterpret_cast<uint4*>(dst);
   o[0] = make_uint4(q0.x & m, q0.y & m, q0.z & m, q0.w & m);
   o[1] = make_uint4(q1.x & m, q1.y & m, q1.z & m, q1.w & m);
-  o[2] = make_uint4(pk_bf(1.0f, y[0]) & m, pk_bf(y[1], y[2]) & m, pk_bf(y[3], y[4]) & m, pk_bf(y[5], y[6]) & m);
-  o[3] = make_uint4(pk_bf(y[7], y[8]) & m, pk_bf(y[9], y[10]) & m, pk_bf(y[11], y[12]) & m, pk_bf(y[13], y[14]) & m);
+  const uint4 e2 = enc_half(y, 2), e3 = enc_half(y, 3);
+  o[2] = make_uint4(e2.x & m, e2.y & m, e2.z & m, e2.w & m);
+  o[3] = make_uint4(e3.x & m, e3.y & m, e3.z & m, e3.w & m);
 }
 
 __global__ void __launch_bounds__(WAVE* ML_WPB) mlp_fwd_kernel(MlpParams p) {
@@ -246,10 +164,6 @@ constexpr int BW_WPB = 8;  // waves per workgroup
 constexpr int BW_ROWS = 32;
 enum { IM_W1 = 0, IM_W2 = 4, IM_W3 = 12, IM_W3T = 14, IM_W2T = 18, IM_W1T = 26, IM_N = 28 };
 
-typedef short s16x4 __attribute__((ext_vector_type(4)));
-
-// K slot 8g + j of chunk q <-> unit 32q + 16(j / 4) + 4g + j % 4 (forward order)
-__device__ __forceinline__ int slot_unit(int q, int g, int j) { return 32 * q + 16 * (j >> 2) + 4 * g + (j & 3); }
 
 // entry i of the A-operand images: tile i / 512, lane (i / 8) % 64, element i % 8
 __device__ __forceinline__ uint16_t image_entry(const MlpParams& p, int i) {
@@ -392,6 +306,13 @@ __global__ void __launch_bounds__(WAVE* BW_WPB) mlp_bwd_kernel(MlpParams p) {
         }
       }
     }
+    // A step whose 32 incoming colour gradients are all zero adds exactly
+    // nothing (every product it would accumulate carries dOut) and its
+    // sites' latent gradients stay the caller's zeros: skip it. Texel rows
+    // of sites no ray shades arrive so from the rasterizer's backward.
+    if (ballot(gr[0][0] != 0.0f || gr[0][1] != 0.0f || gr[0][2] != 0.0f || gr[1][0] != 0.0f ||
+               gr[1][1] != 0.0f || gr[1][2] != 0.0f) == 0)
+      continue;
     if (lane < BW_ROWS) stage_row(p, base + lane, sx + lane * XS);
     wave_sync();
     uint2(*t0)[WAVE] = sm.t0[wv];

@@ -39,8 +39,9 @@
 // recomputed in a final pass so no per-texel accumulators stay live), and
 // one lane per value issues the atomic.
 #include <algorithm>
+#include <type_traits>
 
-#include "pf_common.hpp"
+#include "mlp_core.hpp"
 
 namespace pf {
 
@@ -89,18 +90,21 @@ __device__ __forceinline__ int xcd_group(int b, int n_groups, bool& valid) {
   return xcd < full ? xcd * per + k : full * per + (xcd - full) * (per - 1) + k;
 }
 
-// Tile of wave `wv` of workgroup `b`: with an order array, slot b*WPB + wv
-// takes order[slot] (workgroups start in slot order as CUs free up);
-// otherwise the XCD-aware natural order above. -1: no tile.
-__device__ __forceinline__ int wave_tile(const int32_t* order, int n_tiles, int b, int wv) {
+// Tile of wave `wv` of workgroup `b` (nw waves each): with an order array,
+// slot b*nw + wv takes order[slot] (workgroups start in slot order as CUs
+// free up); otherwise the XCD-aware natural order above. -1: no tile.
+__device__ __forceinline__ int wave_tile(const int32_t* order, int n_tiles, int b, int wv, int nw = WPB) {
   if (order) {
-    const int slot = b * WPB + wv;
+    const int slot = b * nw + wv;
     return slot < n_tiles ? order[slot] : -1;
   }
   bool valid;
-  const int t = xcd_group(b, (n_tiles + WPB - 1) / WPB, valid) * WPB + wv;
+  const int t = xcd_group(b, (n_tiles + nw - 1) / nw, valid) * nw + wv;
   return valid && t < n_tiles ? t : -1;
 }
+
+// Workgroups for n work items at nw waves each: a whole number of XCD rounds.
+static int xcd_grid(int n, int nw) { return (((n + nw - 1) / nw + 7) / 8) * 8; }
 
 struct Ray {
   f3 o, d;
@@ -218,9 +222,13 @@ struct TexelLds {
 __device__ __forceinline__ float bf16_to_float(uint32_t h) { return __builtin_bit_cast(float, h << 16); }
 
 // texel_rgbh_bf16 (4 x bfloat16 per texel), when set, replaces texel_rgbh:
-// the colour and height features stored in half the bytes.
+// the colour and height features stored in half the bytes. HEAD: the colour
+// of texel `lane` is `head_rgb` (the learned head's, evaluated by the wave),
+// and (rgb, height) also go to rgbh_out for the backward.
+template <bool HEAD = false>
 __device__ __forceinline__ bool stage_texels(TexelLds* s_tex, const float* texel_sites, const float4* texel_rgbh,
-                                             const uint2* texel_rgbh_bf16, int site, int T, f3 c, int lane) {
+                                             const uint2* texel_rgbh_bf16, int site, int T, f3 c, int lane,
+                                             f3 head_rgb = f3{0.0f, 0.0f, 0.0f}, float4* rgbh_out = nullptr) {
   bool nz = false;
   if (lane < T) {
     const float* ts = texel_sites + ((size_t)site * T + lane) * 3;
@@ -231,6 +239,10 @@ __device__ __forceinline__ bool stage_texels(TexelLds* s_tex, const float* texel
                       bf16_to_float(h.y >> 16));
     } else {
       v = texel_rgbh[(size_t)site * T + lane];
+    }
+    if (HEAD) {
+      v = make_float4(head_rgb.x, head_rgb.y, head_rgb.z, v.w);
+      rgbh_out[(size_t)site * T + lane] = v;
     }
     s_tex[lane].pos_h = make_float4(ts[0] - c.x, ts[1] - c.y, ts[2] - c.z, v.w);
     s_tex[lane].rgb = make_float4(v.x, v.y, v.z, 0.0f);
@@ -276,6 +288,11 @@ struct FwdParams {
   const int32_t* seg_tile;
   const int32_t* seg_order;
   uint8_t* entry_shades;  // [M] or null: 1 for each walked entry that shades a ray
+  // the learned colour head evaluated in place of the texel colours
+  const uint16_t *mlp_w1, *mlp_w2, *mlp_w3, *mlp_lat;
+  const float *mlp_b1, *mlp_b2, *mlp_b3;
+  int mlp_enc;
+  float4* rgbh_out;
   float* seg_logt;   // [n_segs, 64] sum of -sigma dt over the segment
   float* seg_final;  // [n_segs, 64] log T where the segment's wave stopped
   uint8_t* seg_active;  // [n_segs, 64] the lane shaded a site of its walk
@@ -290,9 +307,9 @@ struct WorkItem {
 
 __device__ __forceinline__ bool work_item(int n_segs, const int32_t* seg_count, const int32_t* seg_order,
                                           const int32_t* seg_tile, const int32_t* seg_first, const int32_t* order,
-                                          int n_tiles, int b, int wv, WorkItem& w) {
+                                          int n_tiles, int b, int wv, WorkItem& w, int nw = WPB) {
   if (n_segs > 0) {
-    const int slot = b * WPB + wv;
+    const int slot = b * nw + wv;
     if (slot >= n_segs || (seg_count && slot >= *seg_count)) return false;
     w.seg = seg_order[slot];
     w.tile = seg_tile[w.seg];
@@ -301,12 +318,87 @@ __device__ __forceinline__ bool work_item(int n_segs, const int32_t* seg_count, 
     w.k = w.seg - w.f0;
     return true;
   }
-  w.tile = wave_tile(order, n_tiles, b, wv);
+  w.tile = wave_tile(order, n_tiles, b, wv, nw);
   w.seg = -1;
   w.k = 0;
   w.nk = 1;
   w.f0 = 0;
   return w.tile >= 0;
+}
+
+// ------------------------------------------------ learned head, in place
+// HEAD = true (pf_raster_fwd_args.mlp_head): a site that shades its tile gets
+// its texel colours from the learned head, evaluated by the wave on the
+// matrix cores with the weights in LDS: the T texels are rows 0..T-1 of one
+// 16-row tile, so a site costs 14 v_mfma_f32_16x16x32_bf16 (4 + 8 + 2) and
+// the encoding of its T directions. The operands and instruction sequence
+// are those of pf_mlp_fwd (mlp_core.hpp), so the colours are bit-identical
+// to its rows. The weights are staged once per workgroup of HEAD_WPB waves:
+// W1 rows padded to 80 B and W2/W3 rows in K-slot order padded to 144 B, so
+// every A operand is one conflict-free 16-byte read.
+constexpr int HEAD_WPB = 4;
+constexpr int MW1S = 40, MW2S = 72;  // row strides (bf16)
+
+struct alignas(16) HeadLds {
+  uint16_t w1[ML_HID * MW1S];
+  uint16_t w2[ML_HID * MW2S];
+  uint16_t w3[3 * MW2S];
+  float b1[ML_HID], b2[ML_HID], b3[4];
+};
+
+template <class P>
+__device__ void stage_head(HeadLds& m, const P& p) {
+  for (int i = threadIdx.x; i < ML_HID * ML_IN; i += blockDim.x) m.w1[(i / ML_IN) * MW1S + i % ML_IN] = p.mlp_w1[i];
+  for (int i = threadIdx.x; i < ML_HID * ML_HID + 3 * ML_HID; i += blockDim.x) {
+    const int k = i < ML_HID * ML_HID ? i : i - ML_HID * ML_HID;
+    const int r = k / ML_HID, s = k % ML_HID;  // slot s = 32q + 8g + j
+    const int unit = slot_unit(s >> 5, (s >> 3) & 3, s & 7);
+    if (i < ML_HID * ML_HID)
+      m.w2[r * MW2S + s] = p.mlp_w2[r * ML_HID + unit];
+    else
+      m.w3[r * MW2S + s] = p.mlp_w3[r * ML_HID + unit];
+  }
+  for (int i = threadIdx.x; i < ML_HID; i += blockDim.x) {
+    m.b1[i] = p.mlp_b1[i];
+    m.b2[i] = p.mlp_b2[i];
+  }
+  if (threadIdx.x < 4) m.b3[threadIdx.x] = threadIdx.x < 3 ? p.mlp_b3[threadIdx.x] : 0.0f;
+}
+
+// Colours of the T texels of `site`: lane c16 < T (lane group 0) gets texel
+// c16's rgb. Every lane must take part (the matrix instructions read all 64).
+template <class P>
+__device__ __forceinline__ f3 head_texel_rgb(const HeadLds& m, const P& p, f3 eye, int site, int T, int lane) {
+  const int g = lane >> 4, c16 = lane & 15;
+  const bool row = c16 < T;
+  uint4 xv;
+  if (g < 2) {
+    xv = *reinterpret_cast<const uint4*>(p.mlp_lat + (size_t)site * ML_LAT + 8 * g);
+  } else {
+    const f3 d = texel_dir(p.texel_sites + ((size_t)site * T + (row ? c16 : 0)) * 3, eye);
+    float y[15];
+    mlp_encode(d, p.mlp_enc, y);
+    xv = enc_half(y, g);
+  }
+  const uint32_t mk = row ? 0xFFFFFFFFu : 0u;
+  const bf16x8 x = __builtin_bit_cast(bf16x8, make_uint4(xv.x & mk, xv.y & mk, xv.z & mk, xv.w & mk));
+  f32x4 z1[4], z2[4];
+#pragma unroll
+  for (int t = 0; t < 4; t++)
+    z1[t] = mfma(ld8(m.w1 + (16 * t + c16) * MW1S + 8 * g), x, *reinterpret_cast<const f32x4*>(m.b1 + 16 * t + 4 * g));
+  const bf16x8 h1[2] = {act_op(z1[0], z1[1]), act_op(z1[2], z1[3])};
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const uint16_t* w = m.w2 + (16 * u + c16) * MW2S + 8 * g;
+    z2[u] = mfma(ld8(w + 32), h1[1], mfma(ld8(w), h1[0], *reinterpret_cast<const f32x4*>(m.b2 + 16 * u + 4 * g)));
+  }
+  const bf16x8 h2[2] = {act_op(z2[0], z2[1]), act_op(z2[2], z2[3])};
+  const uint16_t* w3 = m.w3 + min(c16, 2) * MW2S + 8 * g;
+  const bf16x8 zero8 = __builtin_bit_cast(bf16x8, make_uint4(0u, 0u, 0u, 0u));
+  const bf16x8 w30 = c16 < 3 ? ld8(w3) : zero8, w31 = c16 < 3 ? ld8(w3 + 32) : zero8;
+  const f32x4 b3 = g == 0 ? *reinterpret_cast<const f32x4*>(m.b3) : zero4();
+  const f32x4 o = mfma(w31, h2[1], mfma(w30, h2[0], b3));
+  return f3{fmaxf(o[0], 0.0f), fmaxf(o[1], 0.0f), fmaxf(o[2], 0.0f)};
 }
 
 // LOGT = true: the segment prepass of a split list. It walks the segment's
@@ -317,16 +409,24 @@ __device__ __forceinline__ bool work_item(int n_segs, const int32_t* seg_count, 
 // site and the split reproduces the serial walk. (Ending the prepass once
 // every lane's own T is below the threshold was measured slower: long lists
 // on these views rarely get that opaque within one segment.)
-template <bool PINHOLE, bool LOGT>
-__global__ void __launch_bounds__(WAVE* WPB) raster_fwd_kernel(FwdParams p) {
-  __shared__ SiteRec s_site_all[WPB][WAVE];
-  __shared__ float4 s_face_all[WPB][WAVE];
-  __shared__ TexelLds s_tex_all[WPB][PF_MAX_TEXELS];
+template <bool PINHOLE, bool LOGT, bool HEAD = false>
+__global__ void __launch_bounds__(WAVE*(HEAD ? HEAD_WPB : WPB)) raster_fwd_kernel(FwdParams p) {
+  constexpr int NW = HEAD ? HEAD_WPB : WPB;
+  __shared__ SiteRec s_site_all[NW][WAVE];
+  __shared__ float4 s_face_all[NW][WAVE];
+  __shared__ TexelLds s_tex_all[NW][PF_MAX_TEXELS];
+  __shared__ typename std::conditional<HEAD, HeadLds, char>::type s_head;
 
+  if constexpr (HEAD) {
+    stage_head(s_head, p);
+    __syncthreads();
+  }
   const int wv = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   WorkItem wi;
-  if (!work_item(p.n_segs, p.seg_count, p.seg_order, p.seg_tile, p.seg_first, p.order, p.n_tiles, blockIdx.x, wv, wi)) return;
+  if (!work_item(p.n_segs, p.seg_count, p.seg_order, p.seg_tile, p.seg_first, p.order, p.n_tiles, blockIdx.x, wv, wi,
+                 NW))
+    return;
   if (LOGT && wi.k == wi.nk - 1) return;  // no later segment needs it
   const int tile = wi.tile;
   SiteRec* s_site = s_site_all[wv];
@@ -428,8 +528,11 @@ __global__ void __launch_bounds__(WAVE* WPB) raster_fwd_kernel(FwdParams p) {
       if (ballot(shade) == 0) continue;
 
       // texel records of this site (texture.py:23-24)
+      f3 head_rgb{0.0f, 0.0f, 0.0f};
+      if constexpr (HEAD) head_rgb = head_texel_rgb(s_head, p, p.cam.eye, sr.site, T, lane);
       wave_sync();
-      const bool heights = stage_texels(s_tex, p.texel_sites, p.texel_rgbh, p.texel_rgbh_bf16, sr.site, T, c, lane);
+      const bool heights = stage_texels<HEAD>(s_tex, p.texel_sites, p.texel_rgbh, p.texel_rgbh_bf16, sr.site, T, c,
+                                              lane, head_rgb, p.rgbh_out);
       wave_sync();
       float alpha_t = 0.0f, perr = 0.0f;
       bool shaded = false;  // an interval of positive length (the backward's "active")
@@ -1402,7 +1505,9 @@ __global__ void __launch_bounds__(256)
                       const float* __restrict__ rgb, const float* __restrict__ height, int64_t n, int64_t rows,
                       float4* __restrict__ spheres, float4* __restrict__ nsigmas, float4* __restrict__ rgbh) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < rows) rgbh[i] = make_float4(rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2], height[i]);
+  if (i < rows)
+    rgbh[i] = rgb ? make_float4(rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2], height[i])
+                  : make_float4(0.0f, 0.0f, 0.0f, height[i]);
   if (i < n) {
     spheres[i] = make_float4(points[3 * i], points[3 * i + 1], points[3 * i + 2], radii[i]);
     nsigmas[i] = make_float4(normals[3 * i], normals[3 * i + 1], normals[3 * i + 2], density[i]);
@@ -1476,6 +1581,25 @@ int pf_raster_fwd(const pf_raster_fwd_args* a, void* stream) {
   p.contrib = a->contrib;
   p.point_err = a->point_err;
   p.entry_shades = a->entry_shades;
+  const pf_mlp_head* hd = a->mlp_head;
+  if (hd) {
+    const pf_mlp_weights& w = hd->weights;
+    if (!w.w1 || !w.b1 || !w.w2 || !w.b2 || !w.w3 || !w.b3 || (sites && (!hd->latents || !hd->texel_rgbh_out)))
+      return set_error(PF_ERR_ARG, "pf_raster_fwd: the learned head needs its weights, latents and texel_rgbh_out");
+    if ((reinterpret_cast<uintptr_t>(hd->latents) | reinterpret_cast<uintptr_t>(hd->texel_rgbh_out)) % 16)
+      return set_error(PF_ERR_ARG, "pf_raster_fwd: latents and texel_rgbh_out must be 16-byte aligned");
+    if (hd->encoding != PF_MLP_ENC_SH && hd->encoding != PF_MLP_ENC_FREQ)
+      return set_error(PF_ERR_ARG, "pf_raster_fwd: unknown direction encoding %d", hd->encoding);
+    p.mlp_w1 = w.w1;
+    p.mlp_w2 = w.w2;
+    p.mlp_w3 = w.w3;
+    p.mlp_b1 = w.b1;
+    p.mlp_b2 = w.b2;
+    p.mlp_b3 = w.b3;
+    p.mlp_lat = hd->latents;
+    p.mlp_enc = hd->encoding;
+    p.rgbh_out = reinterpret_cast<float4*>(hd->texel_rgbh_out);
+  }
   const pf_segments& sg = a->segs;
   hipStream_t hs = as_stream(stream);
   if (sg.n_segs > 0) {
@@ -1493,24 +1617,39 @@ int pf_raster_fwd(const pf_raster_fwd_args* a, void* stream) {
     p.seg_active = sg.lane_active;
     fill_int_kernel<<<(nt + 255) / 256, 256, 0, hs>>>(a->tile_early_stop, nt, 0x7fffffff);
     const int grid = (sg.n_segs + WPB - 1) / WPB;
+    // the log-T prepass reads heights only: no head there
+    const int hgrid = (sg.n_segs + HEAD_WPB - 1) / HEAD_WPB;
     if (c.ray_maps) {
       raster_fwd_kernel<false, true><<<grid, WAVE * WPB, 0, hs>>>(p);
-      raster_fwd_kernel<false, false><<<grid, WAVE * WPB, 0, hs>>>(p);
+      if (hd)
+        raster_fwd_kernel<false, false, true><<<hgrid, WAVE * HEAD_WPB, 0, hs>>>(p);
+      else
+        raster_fwd_kernel<false, false><<<grid, WAVE * WPB, 0, hs>>>(p);
       raster_fwd_finish_kernel<false><<<nt, WAVE, 0, hs>>>(p);
     } else {
       raster_fwd_kernel<true, true><<<grid, WAVE * WPB, 0, hs>>>(p);
-      raster_fwd_kernel<true, false><<<grid, WAVE * WPB, 0, hs>>>(p);
+      if (hd)
+        raster_fwd_kernel<true, false, true><<<hgrid, WAVE * HEAD_WPB, 0, hs>>>(p);
+      else
+        raster_fwd_kernel<true, false><<<grid, WAVE * WPB, 0, hs>>>(p);
       raster_fwd_finish_kernel<true><<<nt, WAVE, 0, hs>>>(p);
     }
     PF_HIP(hipGetLastError());
     return PF_OK;
   }
-  const int groups = (nt + WPB - 1) / WPB;
-  const int grid = ((groups + 7) / 8) * 8;
-  if (c.ray_maps)
-    raster_fwd_kernel<false, false><<<grid, WAVE * WPB, 0, hs>>>(p);
-  else
-    raster_fwd_kernel<true, false><<<grid, WAVE * WPB, 0, hs>>>(p);
+  if (hd) {
+    const int grid = xcd_grid(nt, HEAD_WPB);
+    if (c.ray_maps)
+      raster_fwd_kernel<false, false, true><<<grid, WAVE * HEAD_WPB, 0, hs>>>(p);
+    else
+      raster_fwd_kernel<true, false, true><<<grid, WAVE * HEAD_WPB, 0, hs>>>(p);
+  } else {
+    const int grid = xcd_grid(nt, WPB);
+    if (c.ray_maps)
+      raster_fwd_kernel<false, false><<<grid, WAVE * WPB, 0, hs>>>(p);
+    else
+      raster_fwd_kernel<true, false><<<grid, WAVE * WPB, 0, hs>>>(p);
+  }
   PF_HIP(hipGetLastError());
   return PF_OK;
 }
@@ -1690,8 +1829,8 @@ int pf_pack_sites(const float* points, const float* radii, const float* normals,
                   const float* texel_rgb, const float* texel_height, int32_t n_sites, int32_t n_texels,
                   float* spheres, float* nsigmas, float* texel_rgbh, void* stream) {
   if (n_sites < 0 || n_texels < 1 || n_texels > PF_MAX_TEXELS ||
-      (n_sites > 0 && (!points || !radii || !normals || !density || !texel_rgb || !texel_height || !spheres ||
-                       !nsigmas || !texel_rgbh)))
+      (n_sites > 0 && (!points || !radii || !normals || !density || !texel_height || !spheres || !nsigmas ||
+                       !texel_rgbh)))
     return set_error(PF_ERR_ARG, "pf_pack_sites: bad arguments");
   if ((reinterpret_cast<uintptr_t>(spheres) | reinterpret_cast<uintptr_t>(nsigmas) |
        reinterpret_cast<uintptr_t>(texel_rgbh)) % 16)

@@ -8,6 +8,7 @@ current stream. There is no CPU path: without the HIP library every call
 raises.
 """
 
+import ctypes
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -128,6 +129,11 @@ class Rasterizer:
         # rounding as the identity (the gradient of the stored feature is the
         # gradient of the FP32 input). Not in the reference.
         self.feature_dtype = "float"
+        # With a color_head: keep the packed texel rows of the last forward
+        # (rgb written by the fused head for the sites that shade, height) in
+        # head_rows, for inspection. Off: the rows live only until backward.
+        self.keep_head_rows = False
+        self.head_rows = None
 
     def _launch(self, name, fn, *a):
         if self.event_log is None:
@@ -313,18 +319,20 @@ class Rasterizer:
 
     def pack_sites(self, points, radii, normals, density, texel_rgb, texel_height):
         """(spheres [N,4], nsigmas [N,4], texel_rgbh [N,T,4]) in FP32, the
-        concatenations of rasterize.py:1619-1624, in one launch (pf_pack_sites)."""
+        concatenations of rasterize.py:1619-1624, in one launch (pf_pack_sites).
+        texel_rgb None: colours 0 (a learned head fused into the compositing
+        supplies them)."""
         n = points.shape[0]
-        t = texel_rgb.shape[1] if texel_rgb.dim() == 3 else -1
+        t = texel_height.shape[1] if texel_height.dim() == 2 else -1
         if (points.shape != (n, 3) or radii.shape != (n,) or normals.shape != (n, 3) or density.shape != (n,)
-                or texel_rgb.shape != (n, t, 3) or texel_height.shape != (n, t)):
+                or (texel_rgb is not None and texel_rgb.shape != (n, t, 3)) or texel_height.shape != (n, t)):
             raise ValueError("points/normals [N,3], radii/density [N], texel_rgb [N,T,3] and texel_height [N,T] "
                              "must agree")
         dev = points.device
         spheres = torch.empty((n, 4), dtype=torch.float32, device=dev)
         nsigmas = torch.empty((n, 4), dtype=torch.float32, device=dev)
         rgbh = torch.empty((n, t, 4), dtype=torch.float32, device=dev)
-        ins = [_f32c(x) for x in (points, radii, normals, density, texel_rgb, texel_height)]
+        ins = [_f32c(x) if x is not None else None for x in (points, radii, normals, density, texel_rgb, texel_height)]
         check(self.lib.pf_pack_sites(*[ptr(x) for x in ins], n, t, ptr(spheres), ptr(nsigmas), ptr(rgbh),
                                      stream_handle()))
         return spheres, nsigmas, rgbh
@@ -339,7 +347,7 @@ class Rasterizer:
     # ------------------------------------------------------------------ forward
     def _forward(self, grad_ctx, camera, depth_quantiles, points, radii, density, normals, texel_sites,
                  texel_rgb, texel_height, adjacency, adjacency_offsets, ray_gt, return_point_err,
-                 transmittance_threshold=1e-3):
+                 transmittance_threshold=1e-3, color_head=None, *head_params):
         if self.attr_dtype != "float":
             raise ValueError("Forward pass only supports float precision")
         if ray_gt is None and self.loss_accumulation:
@@ -348,6 +356,10 @@ class Rasterizer:
             raise ValueError("Ground truth rays must be provided to return point error")
         if texel_sites.shape[1] != self.num_texel_sites:
             raise ValueError("texel_sites does not match args.num_texel_sites")
+        if color_head is None and texel_rgb is None:
+            raise ValueError("texel_rgb is required without a color_head")
+        if color_head is not None and color_head.params["latents"].shape[0] != points.shape[0]:
+            raise ValueError("color_head has a latent per site: its latents must match points")
         dev = points.device
         H, W = camera.height, camera.width
         N = points.shape[0]
@@ -407,7 +419,20 @@ class Rasterizer:
         a.contrib = contrib_out.data_ptr()
         a.point_err = point_err_out.data_ptr() if point_err_out is not None else None
         a.entry_shades = entry_shades.data_ptr() if entry_shades is not None else None
+        head_bf16 = None
+        if color_head is not None:
+            # the head's colours are computed in the compositing and written,
+            # with the heights, into the packed rows the backward reads
+            head_bf16 = color_head._bf16()
+            hs = _abi.PfMlpHead()
+            hs.weights = color_head._weights(head_bf16)
+            hs.latents = head_bf16["latents"].data_ptr()
+            hs.encoding = _head_encoding(color_head)
+            hs.texel_rgbh_out = all_texel_rgbh.data_ptr()
+            a.mlp_head = ctypes.addressof(hs)
         self._launch("raster_fwd", self.lib.pf_raster_fwd, _ref(a), stream_handle())
+        del head_bf16
+        self.head_rows = all_texel_rgbh if (color_head is not None and self.keep_head_rows) else None
         self.last_tiles = (offsets, tile_sites, tile_early_stop)
 
         grad_ctx.rasterizer = self
@@ -416,7 +441,9 @@ class Rasterizer:
         grad_ctx.order = sg["tile_order"] if sg is not None else order
         grad_ctx.segs = sg
         grad_ctx.entry_shades = entry_shades
-        grad_ctx.rgbh_bf16 = rgbh_bf16
+        # with the head, the backward reads the colours the forward wrote (FP32)
+        grad_ctx.rgbh_bf16 = rgbh_bf16 if color_head is None else None
+        grad_ctx.color_head = color_head
         grad_ctx.save_for_backward(dq, all_spheres, all_nsigmas, all_texel_sites, all_texel_rgbh, adjacency,
                                    adjacency_offsets, adjacency_diff, tile_sites, offsets, tile_early_stop,
                                    ray_gt_c, color_out, log_t_out)
@@ -424,8 +451,17 @@ class Rasterizer:
         return (color_out, opacity_out, normal_distance_out, normal_out, quantile_depths_out, err_out,
                 contrib_out, point_err_out, prim_visible_mask)
 
-    def forward(self, *args):
-        return RasterGradFn.apply(self, *args)
+    def forward(self, *args, color_head=None):
+        """rasterize.py:1858-1859. Not in the reference: color_head, a
+        `mlp_head.MLPColor` evaluated on the matrix cores inside the
+        compositing forward for the sites that shade (texel_rgb is then not
+        read and may be None); its latents and weights get gradients through
+        the head's backward over the texel colour gradients."""
+        if color_head is None:
+            return RasterGradFn.apply(self, *args)
+        if len(args) == 13:
+            args = args + (1e-3,)
+        return RasterGradFn.apply(self, *args, color_head, *color_head.parameters())
 
     # ---------------------------------------------------------------- benchmark
     def benchmark(self, camera, points, radii, density, normals, texel_sites, texel_rgb, texel_height, adjacency,
@@ -578,9 +614,20 @@ class Rasterizer:
         st = stream_handle()
         self._launch("raster_bwd", self.lib.pf_raster_bwd, _ref(a), st)
         check(self.lib.pf_scrub_nonfinite(ptr(grad_buf), n_grads, st))
+        head = grad_ctx.color_head
         del grad_ctx.rasterizer, grad_ctx.camera, grad_ctx.ray_maps, grad_ctx.order, grad_ctx.segs, grad_ctx.rgbh_bf16
-        return (None, None, None, spheres_grad[:, :3], spheres_grad[:, 3], nsigmas_grad[:, 3],
-                nsigmas_grad[:, :3], texel_sites_grad, texel_rgb_grad, texel_height_grad, None, None, None, None)
+        del grad_ctx.color_head
+        grads = (None, None, None, spheres_grad[:, :3], spheres_grad[:, 3], nsigmas_grad[:, 3],
+                 nsigmas_grad[:, :3], texel_sites_grad, texel_rgb_grad, texel_height_grad, None, None, None, None)
+        if head is None:
+            return grads
+        # the head's backward over the texel colour gradients (a texel's
+        # gradient is summed over every ray and tile first, then taken
+        # through the network once)
+        eye = torch.as_tensor(camera.eye).detach().to("cpu", torch.float32).reshape(-1)
+        rows = dict(n_rows=N * T, T=T, texel_sites=all_texel_sites, eye=tuple(float(v) for v in eye))
+        head_grads = head._run_bwd(rows, texel_rgb_grad.view(N * T, 3))
+        return grads[:8] + (None,) + grads[9:] + (None, None) + tuple(head_grads)
 
 
 def power_face_encoding(points, radii, adjacency, adjacency_offsets):
@@ -597,8 +644,12 @@ def power_face_encoding(points, radii, adjacency, adjacency_offsets):
 
 
 def _ref(struct):
-    import ctypes
     return ctypes.byref(struct)
+
+
+def _head_encoding(head):
+    from .mlp_head import ENCODINGS
+    return ENCODINGS[head.encoding]
 
 
 def _scene_struct(all_spheres, all_nsigmas, all_texel_sites, all_texel_rgbh, adjacency, adjacency_offsets,

@@ -64,7 +64,8 @@ def test_struct_layout_matches_c():
               ctypes.sizeof(S), S.adjacency_diff.offset,
               ctypes.sizeof(TL), TL.sites.offset,
               ctypes.sizeof(F), F.tiles.offset, F.transmittance_threshold.offset, F.point_err.offset,
-              F.entry_shades.offset,
+              F.entry_shades.offset, F.mlp_head.offset,
+              ctypes.sizeof(_abi.PfMlpHead), _abi.PfMlpHead.latents.offset, _abi.PfMlpHead.texel_rgbh_out.offset,
               ctypes.sizeof(B), B.exact_gradients.offset, B.grad_texel_height.offset, B.entry_shades.offset,
               ctypes.sizeof(RG), RG.color.offset,
               ctypes.sizeof(V), V.bkgd_color.offset, V.intersections.offset,

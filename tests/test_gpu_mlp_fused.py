@@ -1,0 +1,177 @@
+"""The learned colour head fused into the compositing forward
+(Rasterizer.forward(..., color_head=head)) against the same head run as a
+colour pre-pass (MLPColor.forward -> texel_rgb -> Rasterizer.forward), and
+against the oracle.
+
+Bars:
+  - the fused forward computes each shading site's texel colours with the
+    operands and instruction sequence of pf_mlp_fwd, so its rows are
+    bit-identical to the pre-pass rows and every per-pixel output is
+    bit-identical to the pre-pass path's (contrib and point_err, summed over
+    tiles through atomics, to summation order: 1e-6 relative);
+  - geometry gradients equal the pre-pass path's up to float summation
+    order (the same backward kernel; atomics commute up to rounding):
+    ||a - b|| / ||b|| < 1e-6 per tensor; latent and weight gradients < 1e-5;
+  - against the oracle: the oracle's head rows (FP64 with the kernels' bf16
+    rounding points, oracle/mlp.py) through the FP64 compositing oracle give
+    the colour within max abs 5e-3 and mean abs 1e-5 (an activation within an
+    FP32 rounding error of a bf16 tie may round the other way, 2^-8 relative,
+    as in the head's own test); the oracle's compositing adjoint of the texel
+    colours through the oracle's head backward gives the latent and weight
+    gradients within 1e-3 relative per tensor.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import mlp as om
+from tests import harness
+
+pytestmark = pytest.mark.gpu
+
+NAMES = ["color", "opacity", "normal_distance", "normal", "quantile_depths", "err", "contrib", "point_err", "visible"]
+
+
+def _head(n, seed, encoding):
+    import torch
+    from powerfoam_amd.mlp_head import MLPColor
+    rng = np.random.default_rng(seed)
+    head = MLPColor(n, "cuda", seed=seed, latents=rng.normal(0, 0.5, (n, 16)), encoding=encoding)
+    with torch.no_grad():
+        head.params["b1"].copy_(torch.tensor(rng.normal(0, 0.1, 64)))
+        head.params["b2"].copy_(torch.tensor(rng.normal(0, 0.1, 64)))
+        head.params["b3"].copy_(torch.tensor(rng.normal(0.4, 0.1, 3)))
+    return head
+
+
+def _run(case, head, fused, gin, feature_dtype="float", seg_len=None, heights=None):
+    """Forward + backward with the head fused or as a pre-pass; returns
+    (outputs, geometry gradients, head gradients, packed rows or None)."""
+    import torch
+    from powerfoam_amd.camera import TorchCamera
+    from powerfoam_amd.rasterize import Rasterizer
+    dev = torch.device("cuda")
+    T = case.texel_sites.shape[1]
+    r = Rasterizer(harness._args(T, case.cam.ray_maps is None), dev)
+    r.feature_dtype = feature_dtype
+    r.keep_head_rows = True
+    if seg_len is not None:
+        r.seg_len, r.split_min = seg_len, 0
+    tt = lambda a, dt=torch.float32: torch.tensor(np.ascontiguousarray(a), dtype=dt, device=dev)
+    cam = TorchCamera(case.cam.eye, case.cam.right, case.cam.up, case.width, case.height,
+                      None if case.cam.ray_maps is None else case.cam.ray_maps.to(dev))
+    th = case.texel_height if heights is None else heights
+    leaves = dict(points=tt(case.points), radii=tt(case.radii), density=tt(case.density), normals=tt(case.normals),
+                  texel_sites=tt(case.texel_sites), texel_height=tt(th))
+    for v in leaves.values():
+        v.requires_grad_(True)
+    for p in head.parameters():
+        p.grad = None
+    args = [cam, None, leaves["points"], leaves["radii"], leaves["density"], leaves["normals"], leaves["texel_sites"]]
+    rest = [leaves["texel_height"], tt(case.adjacency, torch.int32), tt(case.adjacency_offsets, torch.int32),
+            tt(case.gt), True, 1e-3]
+    if fused:
+        res = r.forward(*args, None, *rest, color_head=head)
+    else:
+        res = r.forward(*args, head.forward(leaves["texel_sites"], cam), *rest)
+    terms = [(res[NAMES.index(k)] * tt(gin[k])).sum() for k in ("color", "opacity", "normal_distance", "normal",
+                                                               "contrib")]
+    torch.autograd.backward(sum(terms))
+    torch.cuda.synchronize()
+    out = {k: v.detach().cpu().numpy() for k, v in zip(NAMES, res) if v is not None}
+    geo = {k: v.grad.cpu().numpy() for k, v in leaves.items()}
+    hg = {k: v.grad.cpu().numpy() for k, v in head.params.items()}
+    rows = r.head_rows.cpu().numpy() if r.head_rows is not None else None
+    return out, geo, hg, rows
+
+
+def _same_outputs(a, b):
+    """Per-pixel outputs bit for bit; the per-site sums over tiles (atomics)
+    to summation order."""
+    for k in b:
+        if k in ("contrib", "point_err"):
+            assert harness.rel_err(a[k], b[k]) < 1e-6, k
+        else:
+            assert np.array_equal(a[k], b[k]), k
+
+
+def _np_params(head):
+    return {k: v.detach().cpu().numpy().astype(np.float64) for k, v in head.params.items()}
+
+
+CASES = {
+    "pinhole": dict(seed=5, n=2000, width=97, height=61, height_std=0.02),
+    "fisheye": dict(seed=6, n=1500, width=64, height=64, height_std=0.02, fisheye_fov=180.0),
+}
+
+
+@pytest.mark.parametrize("encoding", ["freq", "sh"])
+@pytest.mark.parametrize("camera", sorted(CASES))
+def test_fused_head_equals_prepass_head(camera, encoding):
+    case = harness.small_case(**CASES[camera])
+    gin = harness.random_grads(case, seed=3)
+    print(f"fused head {camera} {encoding}: {case.n} sites {case.width}x{case.height}", flush=True)
+    head = _head(case.n, 7, encoding)
+    out_f, geo_f, hg_f, rows = _run(case, head, True, gin)
+    out_p, geo_p, hg_p, _ = _run(case, head, False, gin)
+    _same_outputs(out_f, out_p)
+    # rows of every site that shades: the pre-pass head's, bit for bit
+    import torch
+    from powerfoam_amd.camera import TorchCamera
+    cam = TorchCamera(case.cam.eye, case.cam.right, case.cam.up, case.width, case.height)
+    with torch.no_grad():
+        want = head.forward(torch.tensor(case.texel_sites, device="cuda"), cam).cpu().numpy()
+    shaded = out_p["contrib"] > 0
+    assert shaded.sum() > 100
+    assert np.array_equal(rows[shaded][..., :3], want[shaded])
+    assert np.array_equal(rows[shaded][..., 3], case.texel_height[shaded])
+    ge = {k: harness.rel_err(geo_f[k], geo_p[k]) for k in geo_p}
+    he = {k: harness.rel_err(hg_f[k], hg_p[k]) for k in hg_p}
+    print("geometry", ge, "head", he, flush=True)
+    assert max(ge.values()) < 1e-6, ge
+    assert max(he.values()) < 1e-5, he
+
+
+@pytest.mark.parametrize("encoding", ["freq", "sh"])
+def test_fused_head_matches_oracle(encoding):
+    case = harness.small_case(seed=8, n=1500, width=80, height=56, height_std=0.02)
+    gin = harness.random_grads(case, seed=4)
+    head = _head(case.n, 9, encoding)
+    print(f"fused head vs oracle {encoding}", flush=True)
+    out, _, hg, _ = _run(case, head, True, gin)
+    p = _np_params(head)
+    T = case.texel_sites.shape[1]
+    sites, dirs = om.rows_texels(case.texel_sites, case.cam.eye.numpy(), T)
+    rows, _ = om.forward(p, sites, dirs, encoding=encoding)
+    case.texel_rgb = rows.reshape(case.n, T, 3).astype(np.float32)
+    ref, rgrads = harness.run_oracle(case, gin)
+    err = np.abs(out["color"] - ref["color"])
+    print("colour max", err.max(), "mean", err.mean(), flush=True)
+    assert err.max() < 5e-3 and err.mean() < 1e-5
+    want = om.backward(p, sites, dirs, rgrads["grad_texel_rgb"].reshape(-1, 3), encoding=encoding)
+    errs = {k: harness.rel_err(hg[k], want[k]) for k in want}
+    print("head gradients", errs, flush=True)
+    assert max(errs.values()) < 1e-3, errs
+
+
+def test_fused_head_with_bf16_heights_and_split_lists():
+    """BF16 feature storage: the fused head reads the heights from the bf16
+    copy and keeps its colours in FP32, so it equals the pre-pass path run on
+    bf16-rounded heights in FP32 (bit for bit). Split lists: the segment
+    passes with the head, against the pre-pass path split the same way (the
+    segments' partial sums meet through atomics: 1e-6)."""
+    import torch
+    case = harness.small_case(seed=10, n=1200, width=72, height=48, height_std=0.02)
+    gin = harness.random_grads(case, seed=5)
+    head = _head(case.n, 11, "freq")
+    print("fused head bf16 heights", flush=True)
+    out_f, _, _, _ = _run(case, head, True, gin, feature_dtype="bf16")
+    hb = torch.tensor(case.texel_height).to(torch.bfloat16).to(torch.float32).numpy()
+    out_p, _, _, _ = _run(case, head, False, gin, heights=hb)
+    _same_outputs(out_f, out_p)
+    print("fused head split lists", flush=True)
+    out_f, geo_f, hg_f, _ = _run(case, head, True, gin, seg_len=64)
+    out_p, geo_p, hg_p, _ = _run(case, head, False, gin, seg_len=64)
+    for k in ("color", "opacity", "normal", "normal_distance", "contrib"):
+        assert harness.rel_err(out_f[k], out_p[k]) < 1e-6, k
+    assert max(harness.rel_err(hg_f[k], hg_p[k]) for k in hg_p) < 1e-5
