@@ -115,8 +115,8 @@ class MlpFused(MlpSlot):
 
 
 def mlp_bwd_rows(rast, mlp, w, cams):
-    """Mean over `cams` of the texel rows pf_mlp_bwd processes (32-row steps
-    holding a non-zero colour gradient; the others are skipped) and of the
+    """Mean over `cams` of the texel rows pf_mlp_bwd processes (every texel of
+    the sites with a non-zero colour gradient: pf_mlp_active_sites) and of the
     rows with a non-zero gradient, from one pre-pass frame per view."""
     import torch
     p = w["params"]
@@ -127,12 +127,10 @@ def mlp_bwd_rows(rast, mlp, w, cams):
         out = rast.forward(cam, None, p["points"], p["radii"], p["density"], p["normals"], p["texel_sites"], rgb,
                            p["texel_height"], w["adjacency"], w["adjacency_offsets"], w["target"], True)
         torch.mean((out[0] - w["target"]) ** 2).backward()
-        g = rgb.grad.view(-1, 3) != 0
-        row = g.any(dim=1)
-        pad = (-row.numel()) % 32
-        steps = torch.nn.functional.pad(row, (0, pad)).view(-1, 32).any(dim=1)
-        done.append(int(steps.sum()) * 32)
-        nonzero.append(int(row.sum()))
+        g = rgb.grad != 0
+        T = g.shape[1]
+        done.append(int(g.any(dim=2).any(dim=1).sum()) * T)
+        nonzero.append(int(g.any(dim=2).sum()))
         for q in list(mlp.parameters()) + list(p.values()):
             q.grad = None
     return float(np.mean(done)), float(np.mean(nonzero))

@@ -552,6 +552,13 @@ typedef struct pf_mlp_rows {
    * SH head's basis) or PF_MLP_ENC_FREQ (1, d, sin(pi d), cos(pi d),
    * sin(2 pi d), cos(2 pi d), per component) */
   int32_t encoding;
+  /* pf_mlp_bwd, texel mode with n_rows = n_sites * n_texels: when set, only
+   * the rows of the *site_count sites in site_list (device memory, e.g. from
+   * pf_mlp_active_sites) are processed; the other sites' rows must have a
+   * zero colour gradient (they add nothing) and keep the caller's zero
+   * latent gradient. NULL: every row. */
+  const int32_t* site_list;
+  const int32_t* site_count;
 } pf_mlp_rows;
 #define PF_MLP_ENC_SH 0
 #define PF_MLP_ENC_FREQ 1
@@ -597,6 +604,13 @@ int pf_mlp_fwd(const pf_mlp_weights* w, const pf_mlp_rows* rows, float* rgb, voi
  * whose incoming gradients are all zero are skipped (they add nothing). */
 int pf_mlp_bwd(const pf_mlp_weights* w, const pf_mlp_rows* rows, const float* grad_rgb, pf_mlp_grads* g,
                void* stream);
+/* Appends to site_list [n_sites] each site with a non-zero value among its
+ * n_texels x 3 colour gradients (grad_rgb [n_sites, n_texels, 3]) and adds
+ * their number to *site_count (zeroed by the caller); the list's order is
+ * arbitrary. The texel rows the rasterizer's backward leaves without a
+ * gradient (sites no ray shades) are most rows of a view. */
+int pf_mlp_active_sites(const float* grad_rgb, int32_t n_sites, int32_t n_texels, int32_t* site_list,
+                        int32_t* site_count, void* stream);
 
 /* The rasterizer's per-site inputs from the separate parameter arrays, as the
  * concatenations of rasterize.py:1619-1624: spheres [n,4] = (points [n,3],

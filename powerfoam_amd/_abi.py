@@ -191,6 +191,8 @@ class PfMlpRows(ctypes.Structure):
         ("pair_dirs", ctypes.c_void_p),
         ("latents", ctypes.c_void_p),
         ("encoding", ctypes.c_int32),
+        ("site_list", ctypes.c_void_p),
+        ("site_count", ctypes.c_void_p),
     ]
 
 
@@ -299,6 +301,7 @@ SIGNATURES = {
     "pf_raster_vis": ([_VP, _VP], ctypes.c_int),
     "pf_mlp_fwd": ([_VP, _VP, _VP, _VP], ctypes.c_int),
     "pf_mlp_bwd": ([_VP, _VP, _VP, _VP, _VP], ctypes.c_int),
+    "pf_mlp_active_sites": ([_VP, ctypes.c_int32, ctypes.c_int32, _VP, _VP, _VP], ctypes.c_int),
     "pf_trace_fwd": ([_VP, _VP], ctypes.c_int),
     "pf_trace_bwd": ([_VP, _VP], ctypes.c_int),
     "pf_trace_check_f64": ([_VP, _VP], ctypes.c_int),

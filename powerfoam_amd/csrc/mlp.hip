@@ -20,6 +20,8 @@ struct MlpParams {
   const int32_t* pair_sites;  // pair mode (when set): site and direction per row
   const float* pair_dirs;
   int encoding;               // PF_MLP_ENC_SH or PF_MLP_ENC_FREQ
+  const int32_t* site_list;   // texel mode over listed sites (when set): row r is
+  const int32_t* site_count;  // texel r % T of site site_list[r / T], r < count * T
   const uint16_t* latents;  // [N,16] bf16
   const uint16_t* w1;
   const float* b1;
@@ -38,9 +40,27 @@ struct MlpParams {
   float* g_b3;
 };
 
+// Site of row `row` (< n_rows) and its index into the per-row arrays
+// (texel_sites, pair_dirs, grad_rgb): the row itself, or for a site list the
+// texel row of the listed site.
+__device__ __forceinline__ int row_site(const MlpParams& p, int row, size_t& at) {
+  if (p.pair_sites) {
+    at = row;
+    return p.pair_sites[row];
+  }
+  if (p.site_list) {
+    const int s = p.site_list[row / p.T];
+    at = (size_t)s * p.T + row % p.T;
+    return s;
+  }
+  at = row;
+  return row / p.T;
+}
+
 __device__ __forceinline__ bool row_ok(const MlpParams& p, int row) {
   if (row >= p.n_rows) return false;
-  const int site = p.pair_sites ? p.pair_sites[row] : row / p.T;
+  size_t at;
+  const int site = row_site(p, row, at);
   return site >= 0 && site < p.n_sites;
 }
 
@@ -58,18 +78,18 @@ constexpr int XS = 40;       // staged input row stride (bf16): 80 B, conflict-f
 // zero for rows past the end or naming no valid site.
 __device__ __forceinline__ void stage_row(const MlpParams& p, int row, uint16_t* dst) {
   const bool in = row < p.n_rows;
-  const int rc = in ? row : 0;
-  int site = p.pair_sites ? p.pair_sites[rc] : rc / p.T;
+  size_t rc;
+  int site = row_site(p, in ? row : 0, rc);
   const bool ok = in && site >= 0 && site < p.n_sites;
   site = ok ? site : 0;
   const uint4* lp = reinterpret_cast<const uint4*>(p.latents + (size_t)site * ML_LAT);
   f3 d;
   if (p.pair_sites) {
-    d = ld3(p.pair_dirs + 3 * (size_t)rc);
+    d = ld3(p.pair_dirs + 3 * rc);
     const float l = sqrtf(dot(d, d));
     d = l > 0.0f ? d * (1.0f / l) : f3{0.0f, 0.0f, 0.0f};
   } else {
-    d = texel_dir(p.texel_sites + 3 * (size_t)rc, p.eye);
+    d = texel_dir(p.texel_sites + 3 * rc, p.eye);
   }
   float y[15];
   mlp_encode(d, p.encoding, y);
@@ -253,8 +273,10 @@ struct HalfT {
   uint2 h1[4], h2[4], z2[4], z1[4], x[2], o;
 };
 
-__global__ void __launch_bounds__(WAVE* BW_WPB) mlp_bwd_kernel(MlpParams p) {
+__global__ void __launch_bounds__(WAVE* BW_WPB) mlp_bwd_kernel(MlpParams pin) {
   __shared__ BwdShared sm;
+  MlpParams p = pin;
+  if (p.site_list) p.n_rows = *p.site_count * p.T;
   for (int i = threadIdx.x; i < IM_N * 512; i += blockDim.x) sm.im[i] = image_entry(p, i);
   for (int i = threadIdx.x; i < ML_HID; i += blockDim.x) {
     sm.b1[i] = p.b1[i];
@@ -301,8 +323,10 @@ __global__ void __launch_bounds__(WAVE* BW_WPB) mlp_bwd_kernel(MlpParams p) {
         const int row = base + 16 * h + c16;
         okr[h] = row_ok(p, row);
         if (okr[h]) {
+          size_t at;
+          row_site(p, row, at);
 #pragma unroll
-          for (int j = 0; j < 3; j++) gr[h][j] = p.grad_rgb[3 * (size_t)row + j];
+          for (int j = 0; j < 3; j++) gr[h][j] = p.grad_rgb[3 * at + j];
         }
       }
     }
@@ -440,15 +464,17 @@ __global__ void __launch_bounds__(WAVE* BW_WPB) mlp_bwd_kernel(MlpParams p) {
         if (site >= 0 && site < p.n_sites) atomicAdd(p.g_lat + (size_t)site * ML_LAT + u, sdl[rr * ML_LAT + u]);
       }
     } else {
+      // k: a site's position among the rows (its index, or its list slot)
       const int s0 = base / p.T, s1 = (row_end - 1) / p.T;
       for (int i = lane; i < (s1 - s0 + 1) * ML_LAT; i += WAVE) {
-        const int site = s0 + i / ML_LAT, u = i % ML_LAT;
-        const int ra = max(site * p.T, base), rb = min((site + 1) * p.T, row_end);
+        const int k = s0 + i / ML_LAT, u = i % ML_LAT;
+        const int ra = max(k * p.T, base), rb = min((k + 1) * p.T, row_end);
         float acc = 0.0f;
         for (int rr = ra; rr < rb; rr++) acc += sdl[(rr - base) * ML_LAT + u];
         // a site whose rows all fall in this step has no other writer
-        const bool whole = site * p.T >= base && ((site + 1) * p.T <= row_end || row_end == p.n_rows);
-        if (site < p.n_sites) {
+        const bool whole = k * p.T >= base && ((k + 1) * p.T <= row_end || row_end == p.n_rows);
+        const int site = p.site_list ? p.site_list[k] : k;
+        if (site >= 0 && site < p.n_sites) {
           if (whole)
             p.g_lat[(size_t)site * ML_LAT + u] = acc;
           else
@@ -517,6 +543,25 @@ __global__ void __launch_bounds__(WAVE* BW_WPB) mlp_bwd_kernel(MlpParams p) {
   }
 }
 
+// Sites with a non-zero colour gradient on some texel, appended to `list`
+// (one atomic per wave; the order within the list is arbitrary).
+__global__ void __launch_bounds__(256)
+    active_sites_kernel(const float* __restrict__ grad_rgb, int n_sites, int T, int32_t* list, int32_t* count) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  bool on = false;
+  if (s < n_sites) {
+    const float* g = grad_rgb + (size_t)s * T * 3;
+    for (int i = 0; i < 3 * T && !on; i++) on = g[i] != 0.0f;
+  }
+  const uint64_t m = ballot(on);
+  if (m == 0) return;
+  const int lane = threadIdx.x & 63;
+  int base = 0;
+  if (lane == 0) base = atomicAdd(count, __popcll(m));
+  base = __builtin_amdgcn_readfirstlane(base);
+  if (on) list[base + __popcll(m & ((1ull << lane) - 1))] = s;
+}
+
 static int make_params(const pf_mlp_weights* w, const pf_mlp_rows* rows, MlpParams& p, const char* who) {
   if (!w || !rows || !w->w1 || !w->b1 || !w->w2 || !w->b2 || !w->w3 || !w->b3)
     return set_error(PF_ERR_ARG, "%s: missing weights", who);
@@ -539,6 +584,10 @@ static int make_params(const pf_mlp_weights* w, const pf_mlp_rows* rows, MlpPara
   p.eye = f3{rows->eye[0], rows->eye[1], rows->eye[2]};
   p.pair_sites = rows->pair_sites;
   p.pair_dirs = rows->pair_dirs;
+  if (rows->site_list && (rows->pair_sites || !rows->site_count || (int64_t)rows->n_sites * rows->n_texels != rows->n_rows))
+    return set_error(PF_ERR_ARG, "%s: a site list needs texel mode, its count and n_rows = n_sites * n_texels", who);
+  p.site_list = rows->site_list;
+  p.site_count = rows->site_count;
   if (rows->encoding != PF_MLP_ENC_SH && rows->encoding != PF_MLP_ENC_FREQ)
     return set_error(PF_ERR_ARG, "%s: unknown direction encoding %d", who, rows->encoding);
   p.encoding = rows->encoding;
@@ -581,6 +630,7 @@ int pf_mlp_fwd(const pf_mlp_weights* w, const pf_mlp_rows* rows, float* rgb, voi
   if (p.n_rows == 0) return PF_OK;
   if (!rgb) return set_error(PF_ERR_ARG, "pf_mlp_fwd: missing output");
   p.rgb = rgb;
+  if (p.site_list) return set_error(PF_ERR_ARG, "pf_mlp_fwd: site lists are for the backward");
   mlp_fwd_kernel<<<persistent_grid(mlp_fwd_kernel, WAVE * ML_WPB, (p.n_rows + FW_ROWS - 1) / FW_ROWS), WAVE * ML_WPB, 0, as_stream(stream)>>>(p);
   PF_HIP(hipGetLastError());
   return PF_OK;
@@ -603,6 +653,17 @@ int pf_mlp_bwd(const pf_mlp_weights* w, const pf_mlp_rows* rows, const float* gr
   p.g_w3 = g->w3;
   p.g_b3 = g->b3;
   mlp_bwd_kernel<<<persistent_grid(mlp_bwd_kernel, WAVE * BW_WPB, (p.n_rows + BW_ROWS - 1) / BW_ROWS), WAVE * BW_WPB, 0, as_stream(stream)>>>(p);
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+int pf_mlp_active_sites(const float* grad_rgb, int32_t n_sites, int32_t n_texels, int32_t* site_list,
+                        int32_t* site_count, void* stream) {
+  if (n_sites < 0 || n_texels < 1 || (n_sites > 0 && (!grad_rgb || !site_list || !site_count)))
+    return set_error(PF_ERR_ARG, "pf_mlp_active_sites: bad arguments");
+  if (n_sites == 0) return PF_OK;
+  active_sites_kernel<<<(n_sites + 255) / 256, 256, 0, as_stream(stream)>>>(grad_rgb, n_sites, n_texels, site_list,
+                                                                             site_count);
   PF_HIP(hipGetLastError());
   return PF_OK;
 }

@@ -84,22 +84,20 @@ __device__ __forceinline__ bf16x8 ld_wrow(const uint16_t* w, int row, int q, int
 // The 15 non-constant direction features y[0..14] of e(d) = (1, y).
 __device__ __forceinline__ void mlp_encode(f3 d, int encoding, float* y) {
   if (encoding == PF_MLP_ENC_FREQ) {
-    // (d, sin pi d, cos pi d, sin 2 pi d, cos 2 pi d) after the constant
-    float sn[3], cs[3], sn2[3], cs2[3];
+    // (d, sin pi d, cos pi d, sin 2 pi d, cos 2 pi d) after the constant, from
+    // the hardware v_sin_f32 / v_cos_f32 (argument in turns: sin(2 pi x)),
+    // whose error (~1e-6 on |x| <= 1) is far below the bf16 rounding that
+    // follows; |d| <= 1 keeps the argument in their exact range
     const float dd[3] = {d.x, d.y, d.z};
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-      sincosf(3.14159265358979f * dd[k], &sn[k], &cs[k]);
-      sn2[k] = 2.0f * sn[k] * cs[k];        // sin 2x
-      cs2[k] = 1.0f - 2.0f * sn[k] * sn[k];  // cos 2x
+      const float half = 0.5f * dd[k];
       y[k] = dd[k];
-      y[3 + k] = sn[k];
-      y[6 + k] = cs[k];
-      y[9 + k] = sn2[k];
+      y[3 + k] = __builtin_amdgcn_sinf(half);
+      y[6 + k] = __builtin_amdgcn_cosf(half);
+      y[9 + k] = __builtin_amdgcn_sinf(dd[k]);
+      y[12 + k] = __builtin_amdgcn_cosf(dd[k]);
     }
-    y[12] = cs2[0];
-    y[13] = cs2[1];
-    y[14] = cs2[2];
   } else {
     sh_basis(d, y);
   }

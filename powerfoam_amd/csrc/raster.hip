@@ -329,11 +329,14 @@ __device__ __forceinline__ bool work_item(int n_segs, const int32_t* seg_count, 
 // ------------------------------------------------ learned head, in place
 // HEAD = true (pf_raster_fwd_args.mlp_head): a site that shades its tile gets
 // its texel colours from the learned head, evaluated by the wave on the
-// matrix cores with the weights in LDS: the T texels are rows 0..T-1 of one
-// 16-row tile, so a site costs 14 v_mfma_f32_16x16x32_bf16 (4 + 8 + 2) and
-// the encoding of its T directions. The operands and instruction sequence
-// are those of pf_mlp_fwd (mlp_core.hpp), so the colours are bit-identical
-// to its rows. The weights are staged once per workgroup of HEAD_WPB waves:
+// matrix cores with the weights in LDS. One 16-row tile holds two sites: the
+// T texels of the site that needs its colours (rows 0..T-1) and those of the
+// next site of the list (rows 8..8+T-1), whose colours wait in LDS for the
+// case that it shades too (most walked sites do), so an evaluation -- 14
+// v_mfma_f32_16x16x32_bf16 (4 + 8 + 2) and the encoding of 16 directions --
+// usually serves two sites. The operands and instruction sequence are those
+// of pf_mlp_fwd (mlp_core.hpp); a row's result depends on its own column
+// only, so the colours are bit-identical to its rows. The weights are staged once per workgroup of HEAD_WPB waves:
 // W1 rows padded to 80 B and W2/W3 rows in K-slot order padded to 144 B, so
 // every A operand is one conflict-free 16-byte read.
 constexpr int HEAD_WPB = 4;
@@ -365,17 +368,23 @@ __device__ void stage_head(HeadLds& m, const P& p) {
   if (threadIdx.x < 4) m.b3[threadIdx.x] = threadIdx.x < 3 ? p.mlp_b3[threadIdx.x] : 0.0f;
 }
 
-// Colours of the T texels of `site`: lane c16 < T (lane group 0) gets texel
-// c16's rgb. Every lane must take part (the matrix instructions read all 64).
+// Colours of the T texels of site_a and of site_b (-1: none): lane c16 of
+// lane group 0 gets the rgb of texel c16 of site_a (c16 < T) or of texel
+// c16 - 8 of site_b (8 <= c16 < 8 + T). Every lane must take part (the matrix
+// instructions read all 64).
 template <class P>
-__device__ __forceinline__ f3 head_texel_rgb(const HeadLds& m, const P& p, f3 eye, int site, int T, int lane) {
+__device__ __forceinline__ f3 head_texel_rgb(const HeadLds& m, const P& p, f3 eye, int site_a, int site_b, int T,
+                                             int lane) {
   const int g = lane >> 4, c16 = lane & 15;
-  const bool row = c16 < T;
+  const int texel = c16 & 7;
+  const int site = c16 < 8 ? site_a : site_b;
+  const bool row = texel < T && site >= 0;
+  const int rs = row ? site : site_a;
   uint4 xv;
   if (g < 2) {
-    xv = *reinterpret_cast<const uint4*>(p.mlp_lat + (size_t)site * ML_LAT + 8 * g);
+    xv = *reinterpret_cast<const uint4*>(p.mlp_lat + (size_t)rs * ML_LAT + 8 * g);
   } else {
-    const f3 d = texel_dir(p.texel_sites + ((size_t)site * T + (row ? c16 : 0)) * 3, eye);
+    const f3 d = texel_dir(p.texel_sites + ((size_t)rs * T + (row ? texel : 0)) * 3, eye);
     float y[15];
     mlp_encode(d, p.mlp_enc, y);
     xv = enc_half(y, g);
@@ -423,6 +432,9 @@ __global__ void __launch_bounds__(WAVE*(HEAD ? HEAD_WPB : WPB)) raster_fwd_kerne
   }
   const int wv = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
+  // the learned head's colours of the list entry after the last evaluated
+  // one (lanes 0..7), and that entry's position in the batch (-1: none)
+  __shared__ typename std::conditional<HEAD, float4[NW][PF_MAX_TEXELS], char>::type s_next_all;
   WorkItem wi;
   if (!work_item(p.n_segs, p.seg_count, p.seg_order, p.seg_tile, p.seg_first, p.order, p.n_tiles, blockIdx.x, wv, wi,
                  NW))
@@ -468,6 +480,7 @@ __global__ void __launch_bounds__(WAVE*(HEAD ? HEAD_WPB : WPB)) raster_fwd_kerne
 
   for (int base = lo; base < hi && !stop; base += WAVE) {
     const int cnt = min(WAVE, hi - base);
+    int next_pos = -1;  // HEAD: entry whose colours wait in s_next
     wave_sync();
     if (lane < cnt) {
       const int s = p.sites[start + base + lane];
@@ -529,7 +542,22 @@ __global__ void __launch_bounds__(WAVE*(HEAD ? HEAD_WPB : WPB)) raster_fwd_kerne
 
       // texel records of this site (texture.py:23-24)
       f3 head_rgb{0.0f, 0.0f, 0.0f};
-      if constexpr (HEAD) head_rgb = head_texel_rgb(s_head, p, p.cam.eye, sr.site, T, lane);
+      if constexpr (HEAD) {
+        float4* s_next = s_next_all[wv];
+        if (next_pos == kk) {
+          if (lane < PF_MAX_TEXELS) {
+            const float4 v = s_next[lane];
+            head_rgb = f3{v.x, v.y, v.z};
+          }
+        } else {
+          // this entry's texels and the next entry's (in this batch)
+          const int site_b = kk + 1 < cnt ? __builtin_amdgcn_readfirstlane(s_site[kk + 1].site) : -1;
+          head_rgb = head_texel_rgb(s_head, p, p.cam.eye, sr.site, site_b, T, lane);
+          wave_sync();
+          if (lane >= 8 && lane < 16) s_next[lane - 8] = make_float4(head_rgb.x, head_rgb.y, head_rgb.z, 0.0f);
+          next_pos = site_b >= 0 ? kk + 1 : -1;
+        }
+      }
       wave_sync();
       const bool heights = stage_texels<HEAD>(s_tex, p.texel_sites, p.texel_rgbh, p.texel_rgbh_bf16, sr.site, T, c,
                                               lane, head_rgb, p.rgbh_out);

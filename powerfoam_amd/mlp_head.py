@@ -5,18 +5,23 @@ SphericalVoronoi pre-pass fills (color_fn.py:54-184, called from
 scene.py:314-337) and that `color_fn.SHColor` fills with degree-3 SH. A row of
 site s seen along unit direction d gets
 
-    x = [z_s ; 1, Y_1(d) .. Y_15(d)]          z_s: per-site 16-dim latent
+    x = [z_s ; e(d)]                          z_s: per-site 16-dim latent
     rgb = max(0, W3 relu(W2 relu(W1 x + b1) + b2) + b3)
 
-with 32 -> 64 -> 64 -> 3 layers. Y is the SH head's basis, so e(d) is exactly
-what the SH head contracts with its 16 coefficients per channel. The kernels
+with 32 -> 64 -> 64 -> 3 layers and 16 direction features e(d): the SH head's
+basis (1, Y_1(d) .. Y_15(d)), exactly what the SH head contracts with its 16
+coefficients per channel, or the frequency encoding (1, d, sin(pi d),
+cos(pi d), sin(2 pi d), cos(2 pi d)), per component. The kernels
 read bf16 copies of the latents and weights; `MLPColor` keeps FP32 masters
 (what the optimiser updates) and refreshes the bf16 copies when they change.
 
 `MLPColor.forward(texel_sites, camera)` returns texel_rgb [N,T,3] for a
 camera and is differentiable w.r.t. the latents and weights (not the texel
-positions, as color_fn.py:181-184). `fit` trains the head on the GPU with the
-same kernels on (site, direction) pairs labelled by the SH head.
+positions, as color_fn.py:181-184); its texel-mode backward runs over the
+sites whose texels received a colour gradient only. The rasterizer can also
+evaluate the head inside its compositing forward
+(`Rasterizer.forward(..., color_head=head)`). `fit` trains the head on the GPU
+with the same kernels on (site, direction) pairs labelled by the SH head.
 """
 
 import math
@@ -79,6 +84,9 @@ class MLPColor:
         for p in self.params.values():
             p.requires_grad_(True)
         self.event_log = None
+        # texel-mode backward over the sites with a colour gradient only
+        # (pf_mlp_active_sites); False: every row
+        self.compact_backward = True
 
     # ------------------------------------------------------------- plumbing
     def _bf16(self):
@@ -128,6 +136,15 @@ class MLPColor:
         gs = _abi.PfMlpGrads(*[g[k].data_ptr() for k in PARAM_NAMES])
         w, r = self._weights(bf), self._rows(a, bf)
         gr = grad_rgb.detach().to(torch.float32).contiguous()
+        n_sites = self.params["latents"].shape[0]
+        if "texel_sites" in a and self.compact_backward and a["n_rows"] == n_sites * a.get("T", 1):
+            # only the sites whose texels got a colour gradient (the rows of
+            # the others add nothing): a device list, no host sync
+            lst = torch.empty(max(n_sites, 1), dtype=torch.int32, device=self.device)
+            cnt = torch.zeros(1, dtype=torch.int32, device=self.device)
+            check(self.lib.pf_mlp_active_sites(gr.data_ptr(), n_sites, a.get("T", 1), lst.data_ptr(), cnt.data_ptr(),
+                                               stream_handle()))
+            r.site_list, r.site_count = lst.data_ptr(), cnt.data_ptr()
         self._launch("mlp_bwd", self.lib.pf_mlp_bwd, _ref(w), _ref(r), gr.data_ptr(), _ref(gs), stream_handle())
         return tuple(g[k] for k in PARAM_NAMES)
 

@@ -61,6 +61,38 @@ def test_texel_forward_and_backward_match_oracle(encoding):
     assert max(errs.values()) < 1e-3, errs
 
 
+def test_texel_backward_over_listed_sites_equals_every_row():
+    """The texel-mode backward over the sites with a colour gradient
+    (pf_mlp_active_sites list) against the backward over every row, with the
+    gradient on a random fifth of the sites, and against the oracle."""
+    import torch
+    from powerfoam_amd.camera import look_at
+    n, T = 5003, 8
+    s = synth.scene_uniform(12, n=n, n_texels=T)
+    cam = look_at((0.3, 0.2, 2.5), (0.0, 0.0, 0.0), 64, 48, 60.0)
+    ts = torch.tensor(s["texel_sites"], device="cuda")
+    rng = np.random.default_rng(6)
+    g = rng.normal(size=(n, T, 3)).astype(np.float32)
+    g[rng.uniform(size=n) > 0.2] = 0.0
+    g[3, 5] = [0.0, 0.0, 1.0]  # a site with one non-zero value
+    res = {}
+    for compact in (True, False):
+        head = _head(n, 13, "freq")
+        head.compact_backward = compact
+        rgb = head.forward(ts, cam)
+        print(f"mlp texel backward compact={compact}", flush=True)
+        (rgb * torch.tensor(g, device="cuda")).sum().backward()
+        res[compact] = {k: v.grad.cpu().numpy() for k, v in head.params.items()}
+    errs = {k: harness.rel_err(res[True][k], res[False][k]) for k in res[False]}
+    print(errs, flush=True)
+    assert np.array_equal(res[True]["latents"], res[False]["latents"])
+    assert max(errs.values()) < 1e-5, errs
+    sites, dirs = om.rows_texels(s["texel_sites"], cam.eye.numpy(), T)
+    want = om.backward(_np_params(head), sites, dirs, g.reshape(-1, 3), encoding="freq")
+    errs = {k: harness.rel_err(res[True][k], want[k]) for k in want}
+    assert max(errs.values()) < 1e-3, errs
+
+
 @pytest.mark.parametrize("encoding", ["sh", "freq"])
 def test_pair_forward_and_backward_match_oracle(encoding):
     import torch
