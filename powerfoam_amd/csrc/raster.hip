@@ -254,6 +254,39 @@ __device__ __forceinline__ bool stage_texels(TexelLds* s_tex, const float* texel
   return ballot(nz) != 0;
 }
 
+// The backward's texel records for up to 8 list entries at once (the next
+// entries of its walk): lane L stages texel L % 8 of the entry at batch
+// position `pos` (its lane group's entry; -1: none) into s_win[L], padded as
+// stage_texels pads. One gather for 8 entries instead of a round trip per
+// entry. Returns the ballot of non-zero heights (bit 8e + i: texel i of
+// window entry e).
+__device__ __forceinline__ uint64_t stage_texel_window(TexelLds* s_win, const SiteRec* s_site, int pos,
+                                                       const float* texel_sites, const float4* texel_rgbh,
+                                                       const uint2* texel_rgbh_bf16, int T, int lane) {
+  const int i = lane & 7;
+  bool nz = false;
+  float4 ph = make_float4(TEXEL_FAR, TEXEL_FAR, TEXEL_FAR, 0.0f), cl = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (pos >= 0 && i < T) {
+    const int site = s_site[pos].site;
+    const float4 sph = s_site[pos].sph;
+    const float* ts = texel_sites + ((size_t)site * T + i) * 3;
+    float4 v;
+    if (texel_rgbh_bf16) {
+      const uint2 h = texel_rgbh_bf16[(size_t)site * T + i];
+      v = make_float4(bf16_to_float(h.x & 0xffffu), bf16_to_float(h.x >> 16), bf16_to_float(h.y & 0xffffu),
+                      bf16_to_float(h.y >> 16));
+    } else {
+      v = texel_rgbh[(size_t)site * T + i];
+    }
+    ph = make_float4(ts[0] - sph.x, ts[1] - sph.y, ts[2] - sph.z, v.w);
+    cl = make_float4(v.x, v.y, v.z, 0.0f);
+    nz = v.w != 0.0f;
+  }
+  s_win[lane].pos_h = ph;
+  s_win[lane].rgb = cl;
+  return ballot(nz);
+}
+
 struct FwdParams {
   RasterCam cam;
   int n_texels, n_tiles, Q;
@@ -758,8 +791,64 @@ struct BwdParams {
   const uint8_t* entry_shades;  // [M] from the forward, or null: entries marked 0 are skipped
 };
 
-constexpr int RED_SLICE = 16;      // values reduced per LDS pass
-constexpr int RED_W = RED_SLICE + 1;  // odd row stride: conflict-free
+constexpr int RED_SLICE = 16;  // values reduced per pass
+
+// x of the lane that DPP control CTRL names (within the lane's row)
+template <int CTRL>
+__device__ __forceinline__ float dpp(float x) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), CTRL, 0xf, 0xf, false));
+}
+
+// v_permlane32_swap_b32 a, b: the upper 32 lanes of a trade places with the
+// lower 32 lanes of b; v_permlane16_swap_b32 a, b: the odd rows (16 lanes) of
+// a with the even rows of b. Written as instructions that read and write both
+// registers: through the compiler builtins, hipcc (ROCm 7.2) adds the first
+// result to itself in place of the pair's sum. A swap must follow a VALU write
+// of its operands by two wait states (the s_nop 1 the compiler puts there for
+// the builtins; it does not look inside asm).
+__device__ __forceinline__ void permlane32_swap(float& a, float& b) {
+  asm("s_nop 1\n\tv_permlane32_swap_b32 %0, %1" : "+v"(a), "+v"(b));
+}
+__device__ __forceinline__ void permlane16_swap(float& a, float& b) {
+  asm("s_nop 1\n\tv_permlane16_swap_b32 %0, %1" : "+v"(a), "+v"(b));
+}
+
+// Sums 16 values over the wave without LDS: a butterfly reduce-scatter in
+// registers. permlane32_swap exchanges the half-waves and permlane16_swap the
+// odd and even rows of two values at once, so each exchange halves the values
+// a lane carries; DPP row_ror:8 and row_half_mirror pair lanes i, i^8 and
+// i, i^7 inside a row; two quad_perm adds finish the sum. Lanes 4u .. 4u+3
+// return the sum of v[u]. Every lane must be active.
+__device__ __forceinline__ float wave_sum16(const float* v, int lane) {
+  float r[8], s[4], t[2];
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    // lanes 0-31 keep v[q], lanes 32-63 keep v[q + 8]
+    float a = v[q], b = v[q + 8];
+    permlane32_swap(a, b);
+    r[q] = a + b;
+  }
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    // even rows keep r[q], odd rows r[q + 4]
+    float a = r[q], b = r[q + 4];
+    permlane16_swap(a, b);
+    s[q] = a + b;
+  }
+  const bool b3 = lane & 8, b2 = lane & 4;
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    // lanes with bit 3 clear keep s[q], the others s[q + 2]; partner i ^ 8
+    const float keep = b3 ? s[q + 2] : s[q], send = b3 ? s[q] : s[q + 2];
+    t[q] = keep + dpp<0x128>(send);  // row_ror:8
+  }
+  // bit 2 clear keeps t[0], set t[1]; partner i ^ 7 (row_half_mirror)
+  const float keep = b2 ? t[1] : t[0], send = b2 ? t[0] : t[1];
+  float u = keep + dpp<0x141>(send);
+  u += dpp<0xB1>(u);  // quad_perm [1,0,3,2]
+  u += dpp<0x4E>(u);  // quad_perm [2,3,0,1]
+  return u;
+}
 
 // Destination of reduced value v of a site (v < 8 + 7T).
 __device__ __forceinline__ float* grad_slot(const BwdParams& p, int site, int T, int v) {
@@ -788,9 +877,9 @@ template <bool PINHOLE, bool DSUM>
 __global__ void __launch_bounds__(WAVE* WPB) PF_BWD_OCC raster_bwd_kernel(BwdParams p) {
   __shared__ SiteRec s_site_all[WPB][WAVE];
   __shared__ float4 s_face_all[WPB][WAVE];
-  __shared__ TexelLds s_tex_all[WPB][PF_MAX_TEXELS];
+  __shared__ TexelLds s_win_all[WPB][WAVE];  // texels of 8 walk entries
+  __shared__ float s_gc_all[WPB][WAVE];      // dL/dcontrib of the batch's sites
   __shared__ float s_fgrad_all[WPB][WAVE * 4];
-  __shared__ float s_red_all[WPB][WAVE * RED_W];
 
   const int wv = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
@@ -806,9 +895,9 @@ __global__ void __launch_bounds__(WAVE* WPB) PF_BWD_OCC raster_bwd_kernel(BwdPar
   if (DSUM && wi.k == 0) return;  // no earlier segment needs it
   SiteRec* s_site = s_site_all[wv];
   float4* s_face = s_face_all[wv];
-  TexelLds* s_tex = s_tex_all[wv];
+  TexelLds* s_win = s_win_all[wv];
+  float* s_gc = s_gc_all[wv];
   float* s_fgrad = s_fgrad_all[wv];
-  float* s_red = s_red_all[wv];
 
   const Ray ray = lane_ray<PINHOLE>(p.cam, tile, lane);
   const size_t px = (size_t)ray.pi * p.cam.W + ray.pj;
@@ -862,13 +951,37 @@ __global__ void __launch_bounds__(WAVE* WPB) PF_BWD_OCC raster_bwd_kernel(BwdPar
       s_site[lane].a0 = p.adj_off[s];
       s_site[lane].a1 = p.adj_off[s + 1];
       s_site[lane].pad = p.entry_shades ? p.entry_shades[start + base + lane] : 1;
+      s_gc[lane] = p.g_contrib[s];
     }
+    // the entries to walk, back to front: an entry that shades no ray of the
+    // tile (as the forward found) adds nothing, so its geometry is skipped
+    uint64_t rest = ballot(lane < cnt && (!p.entry_shades || p.entry_shades[start + base + lane] != 0));
     wave_sync();
-    for (int kk = cnt - 1; kk >= 0; kk--) {
+    int walked = 0;
+    uint64_t win_heights = 0;
+    while (rest) {
+      const int kk = 63 - __builtin_clzll(rest);
+      const int slot = walked & 7;
+      if (slot == 0) {
+        // texels of this entry and the next 7 to walk: lane group e takes
+        // the e-th remaining entry
+        int pos = -1;
+        uint64_t m = rest;
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+          const int pe = m ? 63 - __builtin_clzll(m) : -1;
+          pos = (lane >> 3) == e ? pe : pos;
+          m = m ? (m & ~(1ull << pe)) : m;
+        }
+        wave_sync();
+        win_heights = stage_texel_window(s_win, s_site, pos, p.texel_sites, p.texel_rgbh, p.texel_rgbh_bf16, T, lane);
+        wave_sync();
+      }
+      TexelLds* s_tex = s_win + 8 * slot;
+      const bool heights = ((win_heights >> (8 * slot)) & 0xffull) != 0;
+      walked++;
+      rest &= ~(1ull << kk);
       const SiteRec sr = uniform_site(s_site[kk]);
-      // an entry that shades no ray of the tile (as the forward found)
-      // adds nothing: skip its geometry altogether
-      if (!sr.pad) continue;
       const f3 c{sr.sph.x, sr.sph.y, sr.sph.z};
       const float r = sr.sph.w;
       const f3 pn{sr.nsg.x, sr.nsg.y, sr.nsg.z};
@@ -916,9 +1029,6 @@ __global__ void __launch_bounds__(WAVE* WPB) PF_BWD_OCC raster_bwd_kernel(BwdPar
       }
       const bool shade = hit && !ray.oob && tn < tf;
       if (ballot(shade) == 0) continue;
-      wave_sync();
-      const bool heights = stage_texels(s_tex, p.texel_sites, p.texel_rgbh, p.texel_rgbh_bf16, sr.site, T, c, lane);
-      wave_sync();
 
       // ---- recompute plane_intersection_fwd (texture.py:10-71), keeping the
       // colour-stage weights
@@ -993,7 +1103,7 @@ __global__ void __launch_bounds__(WAVE* WPB) PF_BWD_OCC raster_bwd_kernel(BwdPar
           const float ndt = exact ? ndv * ndv : ndv;
           float dLdtrans = dot(g_rgb, col) * alpha;
           if (ndv > 0.0f) dLdtrans += g_nd * ndt * alpha;
-          dLdtrans += p.g_contrib[sr.site] * alpha * p.inv_hw;
+          dLdtrans += s_gc[kk] * alpha * p.inv_hw;
           dLdtrans += dot(g_n, pn) * alpha;
           dLdlog_t += dLdtrans * trans;
           while (qidx >= 0 && trans > qcur) {
@@ -1023,7 +1133,7 @@ __global__ void __launch_bounds__(WAVE* WPB) PF_BWD_OCC raster_bwd_kernel(BwdPar
         const float ndv = dpn;
         float dLdlog_t_i = dLdlog_t;
         float dLddlt = dLdlog_t;
-        const float gc = p.g_contrib[sr.site];
+        const float gc = s_gc[kk];
         const float cdot = dot(g_rgb, col);
         const float ndt = exact ? ndv * ndv : ndv;
         const float ndot = dot(g_n, pn);
@@ -1217,24 +1327,17 @@ __global__ void __launch_bounds__(WAVE* WPB) PF_BWD_OCC raster_bwd_kernel(BwdPar
         }
       }
 
-      // ---- texel and site gradients: summed over the wave by an LDS
-      // transpose in 16-value slices (lane L sums column L%16 over rows
-      // 16*(L/16)..+15, two shuffles finish the column, one atomic per value).
-      // Texel values are produced two texels (14 values) per slice, so no
-      // per-texel accumulators stay live across the adjoint.
+      // ---- texel and site gradients: summed over the wave in registers
+      // (wave_sum16) in 16-value slices, one atomic per value. Texel values
+      // are produced two texels (14 values) per slice, so no per-texel
+      // accumulators stay live across the adjoint.
       auto flush_slice = [&](const float* v, int count, int v0) {
-        wave_sync();
+        float w[RED_SLICE];
 #pragma unroll
-        for (int q = 0; q < RED_SLICE; q++) s_red[lane * RED_W + q] = q < count ? v[q] : 0.0f;
-        wave_sync();
-        const int col_ = lane & (RED_SLICE - 1);
-        const int row0 = (lane >> 4) * 16;
-        float acc = 0.0f;
-#pragma unroll
-        for (int rr = 0; rr < 16; rr++) acc += s_red[(row0 + rr) * RED_W + col_];
-        acc += __shfl_xor(acc, 16, 64);
-        acc += __shfl_xor(acc, 32, 64);
-        if (lane < count && acc != 0.0f) atomicAdd(grad_slot(p, sr.site, T, v0 + lane), acc);
+        for (int q = 0; q < RED_SLICE; q++) w[q] = q < count ? v[q] : 0.0f;
+        const float acc = wave_sum16(w, lane);
+        const int u = lane >> 2;
+        if ((lane & 3) == 0 && u < count && acc != 0.0f) atomicAdd(grad_slot(p, sr.site, T, v0 + u), acc);
       };
       // Branch-free: a skipped texel (w / sum w < cut, or an inactive lane)
       // selects zero, so no per-texel control flow forms in this pass.
