@@ -451,12 +451,18 @@ __device__ __forceinline__ f3 head_texel_rgb(const HeadLds& m, const P& p, f3 ey
 // site and the split reproduces the serial walk. (Ending the prepass once
 // every lane's own T is below the threshold was measured slower: long lists
 // on these views rarely get that opaque within one segment.)
+// Register budget: six waves per SIMD without the head (80 VGPRs; the
+// allocator alone lands at 84, five waves, and the kernel runs 3 % slower
+// there despite the 4 spilled registers at six), four with it.
 template <bool PINHOLE, bool LOGT, bool HEAD = false>
-__global__ void __launch_bounds__(WAVE*(HEAD ? HEAD_WPB : WPB)) raster_fwd_kernel(FwdParams p) {
+__global__ void __launch_bounds__(WAVE*(HEAD ? HEAD_WPB : WPB)) __attribute__((amdgpu_waves_per_eu(HEAD ? 4 : 6)))
+raster_fwd_kernel(FwdParams p) {
   constexpr int NW = HEAD ? HEAD_WPB : WPB;
   __shared__ SiteRec s_site_all[NW][WAVE];
   __shared__ float4 s_face_all[NW][WAVE];
-  __shared__ TexelLds s_tex_all[NW][PF_MAX_TEXELS];
+  // texel records: of the site being shaded (HEAD), or of the next 8 list
+  // entries, staged in one gather (stage_texel_window)
+  __shared__ TexelLds s_tex_all[NW][HEAD ? PF_MAX_TEXELS : WAVE];
   __shared__ typename std::conditional<HEAD, HeadLds, char>::type s_head;
 
   if constexpr (HEAD) {
@@ -524,12 +530,20 @@ __global__ void __launch_bounds__(WAVE*(HEAD ? HEAD_WPB : WPB)) raster_fwd_kerne
       s_site[lane].a1 = p.adj_off[s + 1];
     }
     wave_sync();
+    uint64_t win_heights = 0;
     for (int kk = 0; kk < cnt; kk++) {
       const float trans = LOGT ? 1.0f : fast_exp(log_t);
       if (!LOGT && ballot(trans >= p.thr) == 0) {
         early = base + kk;
         stop = true;
         break;
+      }
+      if (!HEAD && (kk & 7) == 0) {
+        // texels of entries kk .. kk + 7: lane group e takes entry kk + e
+        const int pos = kk + (lane >> 3) < cnt ? kk + (lane >> 3) : -1;
+        wave_sync();
+        win_heights = stage_texel_window(s_tex, s_site, pos, p.texel_sites, p.texel_rgbh, p.texel_rgbh_bf16, T, lane);
+        wave_sync();
       }
       const SiteRec sr = uniform_site(s_site[kk]);
       const f3 c{sr.sph.x, sr.sph.y, sr.sph.z};
@@ -591,10 +605,14 @@ __global__ void __launch_bounds__(WAVE*(HEAD ? HEAD_WPB : WPB)) raster_fwd_kerne
           next_pos = site_b >= 0 ? kk + 1 : -1;
         }
       }
-      wave_sync();
-      const bool heights = stage_texels<HEAD>(s_tex, p.texel_sites, p.texel_rgbh, p.texel_rgbh_bf16, sr.site, T, c,
-                                              lane, head_rgb, p.rgbh_out);
-      wave_sync();
+      const TexelLds* tex = HEAD ? s_tex : s_tex + 8 * (kk & 7);
+      bool heights = ((win_heights >> (8 * (kk & 7))) & 0xffull) != 0;
+      if constexpr (HEAD) {
+        wave_sync();
+        heights = stage_texels<HEAD>(s_tex, p.texel_sites, p.texel_rgbh, p.texel_rgbh_bf16, sr.site, T, c, lane,
+                                     head_rgb, p.rgbh_out);
+        wave_sync();
+      }
       float alpha_t = 0.0f, perr = 0.0f;
       bool shaded = false;  // an interval of positive length (the backward's "active")
       if (shade) {
@@ -610,7 +628,7 @@ __global__ void __launch_bounds__(WAVE*(HEAD ? HEAD_WPB : WPB)) raster_fwd_kerne
           float hsum = 0.0f, ws0 = 0.0f;
 #pragma unroll
           for (int i = 0; i < PF_MAX_TEXELS; i++) {
-            const float4 ph = s_tex[i].pos_h;
+            const float4 ph = tex[i].pos_h;
             const f3 df{x0.x - ph.x, x0.y - ph.y, x0.z - ph.z};
             const float w = fast_exp(dot(df, df) * kexp);
             hsum += w * ph.w;
@@ -631,8 +649,8 @@ __global__ void __launch_bounds__(WAVE*(HEAD ? HEAD_WPB : WPB)) raster_fwd_kerne
         float ws = 0.0f;
 #pragma unroll
         for (int i = 0; i < PF_MAX_TEXELS; i++) {
-          const float4 ph = s_tex[i].pos_h;
-          const float4 cl = s_tex[i].rgb;
+          const float4 ph = tex[i].pos_h;
+          const float4 cl = tex[i].rgb;
           const f3 df{x1.x - ph.x, x1.y - ph.y, x1.z - ph.z};
           const float w = fast_exp(dot(df, df) * kexp);
           rs = rs + f3{cl.x, cl.y, cl.z} * w;
