@@ -1357,6 +1357,11 @@ __global__ void __launch_bounds__(WAVE* WPB) PF_BWD_OCC raster_bwd_kernel(BwdPar
         const int u = lane >> 2;
         if ((lane & 3) == 0 && u < count && acc != 0.0f) atomicAdd(grad_slot(p, sr.site, T, v0 + u), acc);
       };
+      // the site's own values first: their registers are free for the texels
+      {
+        const float v[8] = {dLdc.x, dLdc.y, dLdc.z, dLdr, dLdnorm.x, dLdnorm.y, dLdnorm.z, dLdsig};
+        flush_slice(v, 8, 0);
+      }
       // Branch-free: a skipped texel (w / sum w < cut, or an inactive lane)
       // selects zero, so no per-texel control flow forms in this pass.
 #pragma unroll
@@ -1400,10 +1405,6 @@ __global__ void __launch_bounds__(WAVE* WPB) PF_BWD_OCC raster_bwd_kernel(BwdPar
           }
           flush_slice(v, i0 + 1 < T ? 14 : 7, 8 + 7 * i0);
         }
-      }
-      {
-        const float v[8] = {dLdc.x, dLdc.y, dLdc.z, dLdr, dLdnorm.x, dLdnorm.y, dLdnorm.z, dLdsig};
-        flush_slice(v, 8, 0);
       }
     }
   }
