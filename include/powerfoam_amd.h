@@ -75,6 +75,11 @@ typedef struct pf_scene {
    * reference (its fp16 attributes, rasterize.py:2008-2014, are the
    * benchmark pass's); the ray tracer reads texel_rgbh. */
   const uint16_t* texel_rgbh_bf16;
+  /* [N] or NULL: the compositing (pf_raster_fwd / pf_raster_bwd) reads only
+   * the first adjacency_count[s] faces of each site's range (the kept faces
+   * pf_prune_faces puts first); NULL: every face. Other entry points read the
+   * whole range. Not in the reference. */
+  const int32_t* adjacency_count;
 } pf_scene;
 
 /* Depth-sorted per-tile site lists produced by binning. */
@@ -424,6 +429,21 @@ int pf_cech_emit(const float* spheres, int32_t n_sites, void* workspace, size_t 
  * empty. keep [n_edges] (0/1) and counts [N+1] (counts[0] = 0, kept entries
  * per site) are overwritten; the lists are adjacency[keep] with offsets the
  * cumulative counts. Neighbour ids outside [0, N) are ignored. */
+/* Faces of each site's own list that can never bound a ray's interval inside
+ * the site's ball, moved to the end of its range (no reference counterpart):
+ * face f is dropped when the site's centre lies strictly on the cell side of
+ * f and either f's plane misses the ball by margin * r, or the disc f's plane
+ * cuts from the ball lies outside another face g (centre on g's cell side)
+ * by margin * r. Cell and ball then intersect exactly as with every face.
+ * Writes adjacency_out [E] (and diff_out [E,4] when adjacency_diff, the
+ * pf_adj_encode encoding, is given; NULL: faces from the full-precision
+ * spheres) with each site's kept faces first in list order, and their number
+ * into face_count [N] -- the pf_scene.adjacency_count of the compositing.
+ * FP32, eight lanes per site; spheres 16-byte aligned. */
+int pf_prune_faces(const float* spheres, int32_t n_sites, const int32_t* adjacency, const int32_t* adjacency_offsets,
+                   int64_t n_edges, const uint16_t* adjacency_diff, float margin, int32_t* adjacency_out,
+                   uint16_t* diff_out, int32_t* face_count, void* stream);
+
 int pf_power_restrict(const float* spheres, int32_t n_sites, const int32_t* adjacency,
                       const int32_t* adjacency_offsets, int64_t n_edges, uint8_t* keep, int32_t* counts,
                       void* stream);

@@ -254,6 +254,7 @@ int orc_abi_layout(int64_t* out, int cap) {
   int64_t v[] = {
       (int64_t)sizeof(pf_camera), (int64_t)offsetof(pf_camera, ray_maps), (int64_t)offsetof(pf_camera, fov_cos_cutoff),
       (int64_t)sizeof(pf_scene), (int64_t)offsetof(pf_scene, adjacency_diff),
+      (int64_t)offsetof(pf_scene, adjacency_count),
       (int64_t)sizeof(pf_tile_lists), (int64_t)offsetof(pf_tile_lists, sites),
       (int64_t)sizeof(pf_raster_fwd_args), (int64_t)offsetof(pf_raster_fwd_args, tiles),
       (int64_t)offsetof(pf_raster_fwd_args, transmittance_threshold), (int64_t)offsetof(pf_raster_fwd_args, point_err),

@@ -62,6 +62,7 @@ class PfScene(ctypes.Structure):
         ("adjacency_offsets", ctypes.c_void_p),
         ("adjacency_diff", ctypes.c_void_p),
         ("texel_rgbh_bf16", ctypes.c_void_p),
+        ("adjacency_count", ctypes.c_void_p),
     ]
 
 
@@ -289,6 +290,8 @@ SIGNATURES = {
     "pf_cech_workspace_size": ([ctypes.c_int32], ctypes.c_size_t),
     "pf_cech_count": ([_VP, ctypes.c_int32, _VP, ctypes.c_size_t, _VP, _VP], ctypes.c_int),
     "pf_cech_emit": ([_VP, ctypes.c_int32, _VP, ctypes.c_size_t, _VP, _VP, _VP, _VP], ctypes.c_int),
+    "pf_prune_faces": ([_VP, ctypes.c_int32, _VP, _VP, ctypes.c_int64, _VP, ctypes.c_float, _VP, _VP, _VP, _VP],
+                       ctypes.c_int),
     "pf_power_restrict": ([_VP, ctypes.c_int32, _VP, _VP, ctypes.c_int64, _VP, _VP, _VP], ctypes.c_int),
     "pf_morton_workspace_size": ([ctypes.c_int32], ctypes.c_size_t),
     "pf_morton_sort": ([_VP, ctypes.c_int32, _VP, ctypes.c_size_t, _VP, _VP, _VP], ctypes.c_int),

@@ -252,11 +252,126 @@ __global__ void __launch_bounds__(256) power_dead_kernel(const float4* __restric
   counts[a + 1] = c;
 }
 
+// ------------------------------------------------------- face pruning
+// Faces of a site's own list that can never bound a ray's interval inside
+// its ball (no reference counterpart; the compositing kernels then test
+// fewer faces per walked pair). With c the centre, face k is the plane
+// n_k.(x - c) = w_k, w_k = ((|n_k|^2 + r^2) - r_k^2)/2, the cell side
+// n_k.(x - c) < w_k. Face f is dropped when c is strictly on the cell side of
+// f (w_f > 0) and either the plane misses the ball by the margin
+// (w_f^2 >= r^2 (1+m)^2 |n_f|^2) or the disc it cuts from the ball lies
+// outside another face g with w_g > 0 by m r:
+//   n_g.q_f - rho_f |n_g - (n_g.n_f / |n_f|^2) n_f| - w_g > m r |n_g|,
+// q_f - c = n_f w_f / |n_f|^2 the disc centre, rho_f its radius. Then the
+// ball's part on g's cell side is connected, holds c and misses f's plane,
+// so it lies on f's cell side: removing f leaves cell and ball intersected
+// exactly as before (and a chain of such removals too, since a witness's
+// own witness bounds the same region). FP32 without contraction, in the
+// order of the CPU restatement; the margin m covers the compositing's
+// rounding. Eight lanes per site (a lane takes every eighth face and tests
+// it against all of the site's faces), kept faces first in list order.
+constexpr int PRUNE_LANES = 8;
+
+struct PruneFace {
+  float nx, ny, nz, w;
+};
+
+__device__ __forceinline__ PruneFace prune_face(const int32_t* adjacency, const uint16_t* diff, const float4* spheres,
+                                                int64_t e, float4 c) {
+  float nx, ny, nz, ra;
+  if (diff) {
+    const uint2 raw = *reinterpret_cast<const uint2*>(diff + 4 * e);
+    nx = half_bits_to_float((uint16_t)(raw.x & 0xffff));
+    ny = half_bits_to_float((uint16_t)(raw.x >> 16));
+    nz = half_bits_to_float((uint16_t)(raw.y & 0xffff));
+    ra = half_bits_to_float((uint16_t)(raw.y >> 16));
+  } else {
+    const float4 a = spheres[adjacency[e]];
+    nx = a.x - c.x;
+    ny = a.y - c.y;
+    nz = a.z - c.z;
+    ra = a.w;
+  }
+  const float n2 = (nx * nx + ny * ny) + nz * nz;
+  return PruneFace{nx, ny, nz, 0.5f * ((n2 + c.w * c.w) - ra * ra)};
+}
+
+__global__ void __launch_bounds__(256)
+    prune_faces_kernel(const float4* __restrict__ spheres, int n_sites, const int32_t* __restrict__ adjacency,
+                       const int32_t* __restrict__ offsets, int64_t n_edges, const uint16_t* __restrict__ diff, float m,
+                       int32_t* __restrict__ adj_out, uint16_t* __restrict__ diff_out, int32_t* __restrict__ count) {
+  const int lane = threadIdx.x & 63, j = lane & (PRUNE_LANES - 1);
+  const int s = (blockIdx.x * blockDim.x + threadIdx.x) / PRUNE_LANES;
+  if (s >= n_sites) return;
+  // offsets past the edge arrays (an inconsistent CSR) end the range there
+  const int a1 = (int)min((int64_t)offsets[s + 1], n_edges), a0 = min(offsets[s], a1);
+  const float4 c = spheres[s];
+  const float r2 = c.w * c.w, lim = (1.0f + m) * (1.0f + m), mr = m * c.w;
+  const int shift = lane & ~(PRUNE_LANES - 1);
+  int kept = 0, dropped = 0;
+  for (int f0 = a0; f0 < a1; f0 += PRUNE_LANES) {
+    const int f = f0 + j;
+    bool drop = false;
+    if (f < a1) {
+      const PruneFace F = prune_face(adjacency, diff, spheres, f, c);
+      if (F.w > 0.0f) {
+        const float nf2 = (F.nx * F.nx + F.ny * F.ny) + F.nz * F.nz;
+        if (F.w * F.w >= (r2 * lim) * nf2) {
+          drop = true;
+        } else {
+          const float q = F.w / nf2;
+          const float rho = sqrtf(fmaxf(r2 - F.w * q, 0.0f));
+          for (int g = a0; g < a1 && !drop; g++) {
+            if (g == f) continue;
+            const PruneFace G = prune_face(adjacency, diff, spheres, g, c);
+            if (!(G.w > 0.0f)) continue;
+            const float ng2 = (G.nx * G.nx + G.ny * G.ny) + G.nz * G.nz;
+            const float d = (G.nx * F.nx + G.ny * F.ny) + G.nz * F.nz;
+            const float perp2 = fmaxf(ng2 - d * (d / nf2), 0.0f);
+            drop = ((d * q - rho * sqrtf(perp2)) - G.w) > mr * sqrtf(ng2);
+          }
+        }
+      }
+    }
+    // this round's faces in list order: kept ones after the kept so far,
+    // dropped ones from the end of the range backwards
+    const uint32_t oct = (uint32_t)(ballot(f < a1 && !drop) >> shift) & 0xffu;
+    const uint32_t octd = (uint32_t)(ballot(f < a1 && drop) >> shift) & 0xffu;
+    const uint32_t below = (1u << j) - 1u;
+    if (f < a1) {
+      const int at = drop ? a1 - 1 - (dropped + __popc(octd & below)) : a0 + kept + __popc(oct & below);
+      adj_out[at] = adjacency[f];
+      if (diff) *reinterpret_cast<uint2*>(diff_out + 4 * (size_t)at) = *reinterpret_cast<const uint2*>(diff + 4 * (size_t)f);
+    }
+    kept += __popc(oct);
+    dropped += __popc(octd);
+  }
+  if (j == 0) count[s] = kept;
+}
+
 }  // namespace pf
 
 using namespace pf;
 
 extern "C" {
+
+int pf_prune_faces(const float* spheres, int32_t n_sites, const int32_t* adjacency, const int32_t* adjacency_offsets,
+                   int64_t n_edges, const uint16_t* adjacency_diff, float margin, int32_t* adjacency_out,
+                   uint16_t* diff_out, int32_t* face_count, void* stream) {
+  if (n_sites < 0 || n_edges < 0 || !(margin >= 0.0f) || (n_sites > 0 && (!spheres || !adjacency_offsets || !face_count)) ||
+      (n_edges > 0 && (!adjacency || !adjacency_out || (adjacency_diff && !diff_out))))
+    return set_error(PF_ERR_ARG, "pf_prune_faces: bad arguments");
+  if (reinterpret_cast<uintptr_t>(spheres) % 16 ||
+      (reinterpret_cast<uintptr_t>(adjacency_diff) | reinterpret_cast<uintptr_t>(diff_out)) % 8)
+    return set_error(PF_ERR_ARG, "pf_prune_faces: spheres must be 16-byte and the face encodings 8-byte aligned");
+  if (n_sites == 0) return PF_OK;
+  const int64_t threads = (int64_t)n_sites * PRUNE_LANES;
+  prune_faces_kernel<<<(unsigned)((threads + 255) / 256), 256, 0, as_stream(stream)>>>(
+      reinterpret_cast<const float4*>(spheres), n_sites, adjacency, adjacency_offsets, n_edges,
+      n_edges > 0 ? adjacency_diff : nullptr, margin, adjacency_out, diff_out, face_count);
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}
 
 int pf_power_restrict(const float* spheres, int32_t n_sites, const int32_t* adjacency,
                       const int32_t* adjacency_offsets, int64_t n_edges, uint8_t* keep, int32_t* counts,

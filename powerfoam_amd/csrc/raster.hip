@@ -297,6 +297,7 @@ struct FwdParams {
   const uint2* texel_rgbh_bf16;
   const int32_t* adjacency;
   const int32_t* adj_off;
+  const int32_t* adj_cnt;  // faces per site the walk reads (null: the whole range)
   const uint16_t* adj_diff;
   const int64_t* offsets;
   const int32_t* sites;
@@ -527,7 +528,7 @@ raster_fwd_kernel(FwdParams p) {
       s_site[lane].nsg = p.nsigmas[s];
       s_site[lane].site = s;
       s_site[lane].a0 = p.adj_off[s];
-      s_site[lane].a1 = p.adj_off[s + 1];
+      s_site[lane].a1 = p.adj_cnt ? p.adj_off[s] + p.adj_cnt[s] : p.adj_off[s + 1];
     }
     wave_sync();
     uint64_t win_heights = 0;
@@ -778,6 +779,7 @@ struct BwdParams {
   const uint2* texel_rgbh_bf16;
   const int32_t* adjacency;
   const int32_t* adj_off;
+  const int32_t* adj_cnt;  // faces per site the walk reads (null: the whole range)
   const uint16_t* adj_diff;
   const int64_t* offsets;
   const int32_t* sites;
@@ -967,7 +969,7 @@ __global__ void __launch_bounds__(WAVE* WPB) PF_BWD_OCC raster_bwd_kernel(BwdPar
       s_site[lane].nsg = p.nsigmas[s];
       s_site[lane].site = s;
       s_site[lane].a0 = p.adj_off[s];
-      s_site[lane].a1 = p.adj_off[s + 1];
+      s_site[lane].a1 = p.adj_cnt ? p.adj_off[s] + p.adj_cnt[s] : p.adj_off[s + 1];
       s_site[lane].pad = p.entry_shades ? p.entry_shades[start + base + lane] : 1;
       s_gc[lane] = p.g_contrib[s];
     }
@@ -1712,6 +1714,7 @@ int pf_raster_fwd(const pf_raster_fwd_args* a, void* stream) {
   p.adjacency = a->scene.adjacency;
   p.adj_off = a->scene.adjacency_offsets;
   p.adj_diff = a->scene.adjacency_diff;
+  p.adj_cnt = a->scene.adjacency_count;
   p.offsets = a->tiles.offsets;
   p.sites = a->tiles.sites;
   p.order = a->tiles.order;
@@ -1836,6 +1839,7 @@ int pf_raster_bwd(const pf_raster_bwd_args* a, void* stream) {
   p.adjacency = a->scene.adjacency;
   p.adj_off = a->scene.adjacency_offsets;
   p.adj_diff = a->scene.adjacency_diff;
+  p.adj_cnt = a->scene.adjacency_count;
   p.offsets = a->tiles.offsets;
   p.sites = a->tiles.sites;
   p.order = a->tiles.order;

@@ -123,6 +123,13 @@ class Rasterizer:
         # and the backward walks only those (an entry that shades nothing adds
         # nothing to any gradient). Not in the reference.
         self.skip_unshaded = True
+        # The compositing tests only the faces that can bound a ray's interval
+        # inside their site's ball (pf_prune_faces: a face whose disc in the
+        # ball lies outside another face, or whose plane misses the ball, is
+        # left out; margin * radius of slack covers float rounding). The
+        # cells inside the balls are unchanged. Not in the reference.
+        self.prune_faces = True
+        self.prune_margin = 1e-3
         # "bf16": the compositing kernels read the texel colour and height
         # features as bfloat16 (8 B per texel instead of 16); geometry,
         # transmittance and every gradient stay FP32. The backward treats the
@@ -337,6 +344,20 @@ class Rasterizer:
                                      stream_handle()))
         return spheres, nsigmas, rgbh
 
+    def prune(self, all_spheres, adjacency, adjacency_offsets, adjacency_diff):
+        """pf_prune_faces: (adjacency, fp16 faces or None, faces per site) with
+        each site's faces that can bound a ray inside its ball first."""
+        n, m = all_spheres.shape[0], adjacency.shape[0]
+        dev = all_spheres.device
+        adj_out = torch.empty(max(m, 1), dtype=torch.int32, device=dev)
+        diff_out = torch.empty((max(m, 1), 4), dtype=torch.float16, device=dev) if adjacency_diff is not None else None
+        count = torch.empty(max(n, 1), dtype=torch.int32, device=dev)
+        check(self.lib.pf_prune_faces(ptr(all_spheres), n, ptr(adjacency) if m else None, ptr(adjacency_offsets), m,
+                                      ptr(adjacency_diff) if (adjacency_diff is not None and m) else None,
+                                      float(self.prune_margin), ptr(adj_out),
+                                      ptr(diff_out) if diff_out is not None else None, ptr(count), stream_handle()))
+        return adj_out[:m], (diff_out[:m] if diff_out is not None else None), count[:n]
+
     def encode_adjacency(self, all_spheres, adjacency, adjacency_offsets):
         """prefetch_adjacency_kernel (rasterize.py:580-606): fp16 (c_adj - c, r_adj)."""
         diff = torch.empty((max(adjacency.shape[0], 1), 4), dtype=torch.float16, device=all_spheres.device)
@@ -382,6 +403,13 @@ class Rasterizer:
             checks=[(adjacency_offsets[-1], adjacency.shape[0], "adjacency_offsets[-1] must equal len(adjacency)")])
         sg = self.segments(offsets, tile_sites.shape[0], self._longest)
         order = self.tile_order(offsets) if sg is None else None
+        # the compositing's faces: each site's faces that can bound a ray first
+        face_count = None
+        if self.prune_faces:
+            adjacency, pdiff, face_count = self.prune(all_spheres, adjacency, adjacency_offsets,
+                                                      None if self.exact_geometry else adjacency_diff)
+            if pdiff is not None:
+                adjacency_diff = pdiff
 
         f32 = dict(dtype=torch.float32, device=dev)
         color_out = torch.zeros((H, W, 3), **f32)
@@ -399,7 +427,8 @@ class Rasterizer:
         a = _abi.PfRasterFwdArgs()
         a.cam, ray_maps = self._camera(camera)
         a.scene = _scene_struct(all_spheres, all_nsigmas, all_texel_sites, all_texel_rgbh, adjacency,
-                                adjacency_offsets, None if self.exact_geometry else adjacency_diff, rgbh_bf16)
+                                adjacency_offsets, None if self.exact_geometry else adjacency_diff, rgbh_bf16,
+                                face_count)
         a.tiles = self._tiles(nt, offsets, tile_sites, order)
         scratch = torch.empty(sg["n_segs"] * 64, **f32) if sg is not None else None
         a.segs = self._segs(sg, scratch)
@@ -444,6 +473,7 @@ class Rasterizer:
         # with the head, the backward reads the colours the forward wrote (FP32)
         grad_ctx.rgbh_bf16 = rgbh_bf16 if color_head is None else None
         grad_ctx.color_head = color_head
+        grad_ctx.face_count = face_count
         grad_ctx.save_for_backward(dq, all_spheres, all_nsigmas, all_texel_sites, all_texel_rgbh, adjacency,
                                    adjacency_offsets, adjacency_diff, tile_sites, offsets, tile_early_stop,
                                    ray_gt_c, color_out, log_t_out)
@@ -585,7 +615,7 @@ class Rasterizer:
         a.cam, ray_maps = self._camera(camera, grad_ctx.ray_maps)
         a.scene = _scene_struct(all_spheres, all_nsigmas, all_texel_sites, all_texel_rgbh, adjacency,
                                 adjacency_offsets, None if self.exact_geometry else adjacency_diff,
-                                getattr(grad_ctx, "rgbh_bf16", None))
+                                getattr(grad_ctx, "rgbh_bf16", None), grad_ctx.face_count)
         sg = grad_ctx.segs if self.split_backward else None
         order = grad_ctx.order
         if sg is None and order is None:
@@ -616,7 +646,7 @@ class Rasterizer:
         check(self.lib.pf_scrub_nonfinite(ptr(grad_buf), n_grads, st))
         head = grad_ctx.color_head
         del grad_ctx.rasterizer, grad_ctx.camera, grad_ctx.ray_maps, grad_ctx.order, grad_ctx.segs, grad_ctx.rgbh_bf16
-        del grad_ctx.color_head
+        del grad_ctx.color_head, grad_ctx.face_count
         grads = (None, None, None, spheres_grad[:, :3], spheres_grad[:, 3], nsigmas_grad[:, 3],
                  nsigmas_grad[:, :3], texel_sites_grad, texel_rgb_grad, texel_height_grad, None, None, None, None)
         if head is None:
@@ -653,7 +683,7 @@ def _head_encoding(head):
 
 
 def _scene_struct(all_spheres, all_nsigmas, all_texel_sites, all_texel_rgbh, adjacency, adjacency_offsets,
-                  adjacency_diff, texel_rgbh_bf16=None):
+                  adjacency_diff, texel_rgbh_bf16=None, adjacency_count=None):
     s = _abi.PfScene()
     s.n_sites = all_spheres.shape[0]
     s.n_texels = all_texel_sites.shape[1]
@@ -665,4 +695,5 @@ def _scene_struct(all_spheres, all_nsigmas, all_texel_sites, all_texel_rgbh, adj
     s.adjacency_offsets = adjacency_offsets.data_ptr()
     s.adjacency_diff = adjacency_diff.data_ptr() if (adjacency_diff is not None and adjacency_diff.numel()) else None
     s.texel_rgbh_bf16 = texel_rgbh_bf16.data_ptr() if texel_rgbh_bf16 is not None else None
+    s.adjacency_count = adjacency_count.data_ptr() if adjacency_count is not None else None
     return s

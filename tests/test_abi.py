@@ -61,7 +61,7 @@ def test_struct_layout_matches_c():
     RG, V, TA, TB = _abi.PfRasterRgbArgs, _abi.PfRasterVisArgs, _abi.PfTraceArgs, _abi.PfTraceBwdArgs
     MR, SG = _abi.PfMlpRows, _abi.PfSegments
     expect = [ctypes.sizeof(C), C.ray_maps.offset, C.fov_cos_cutoff.offset,
-              ctypes.sizeof(S), S.adjacency_diff.offset,
+              ctypes.sizeof(S), S.adjacency_diff.offset, S.adjacency_count.offset,
               ctypes.sizeof(TL), TL.sites.offset,
               ctypes.sizeof(F), F.tiles.offset, F.transmittance_threshold.offset, F.point_err.offset,
               F.entry_shades.offset, F.mlp_head.offset,
