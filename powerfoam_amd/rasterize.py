@@ -123,12 +123,15 @@ class Rasterizer:
         # and the backward walks only those (an entry that shades nothing adds
         # nothing to any gradient). Not in the reference.
         self.skip_unshaded = True
-        # The compositing tests only the faces that can bound a ray's interval
-        # inside their site's ball (pf_prune_faces: a face whose disc in the
-        # ball lies outside another face, or whose plane misses the ball, is
-        # left out; margin * radius of slack covers float rounding). The
-        # cells inside the balls are unchanged. Not in the reference.
-        self.prune_faces = True
+        # True: the compositing tests only the faces that can bound a ray's
+        # interval inside their site's ball (pf_prune_faces: a face whose disc
+        # in the ball lies outside another face, or whose plane misses the
+        # ball, is left out; margin * radius of slack covers float rounding).
+        # The cells inside the balls are unchanged. Off by default: on the
+        # 1M-site views the compositing kernels ran no faster with the pruned
+        # faces (their time is not in the face loop) and the pruning pass
+        # costs ~0.6 ms per frame. Not in the reference.
+        self.prune_faces = False
         self.prune_margin = 1e-3
         # "bf16": the compositing kernels read the texel colour and height
         # features as bfloat16 (8 B per texel instead of 16); geometry,

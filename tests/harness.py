@@ -97,11 +97,12 @@ def _args(T, is_pinhole=True):
 
 
 def run_gpu(case, dev, grads_in=None, exact_gradients=False, exact_geometry=False, return_point_err=True,
-            threshold=1e-3, seg_len="default", split_backward=False, loss_outputs=None):
+            threshold=1e-3, seg_len="default", split_backward=False, loss_outputs=None, prune_faces=None):
     """Forward (+ backward with the given output gradients) through the drop-in
     Rasterizer. Returns numpy dicts. seg_len: the Rasterizer's list-splitting
     length (None: no splitting). loss_outputs: the outputs the loss is built
-    from (default all of grads_in's); the others reach the backward as None."""
+    from (default all of grads_in's); the others reach the backward as None.
+    prune_faces: the Rasterizer's face pruning (None: its default)."""
     import torch
     from powerfoam_amd.rasterize import Rasterizer
 
@@ -111,6 +112,8 @@ def run_gpu(case, dev, grads_in=None, exact_gradients=False, exact_geometry=Fals
         r.seg_len = seg_len
         r.split_min = 0  # split every list longer than seg_len
     r.split_backward = split_backward
+    if prune_faces is not None:
+        r.prune_faces = prune_faces
     r.exact_gradients = exact_gradients
     r.exact_geometry = exact_geometry
     tt = lambda a, dt=torch.float32: torch.tensor(np.ascontiguousarray(a), dtype=dt, device=dev)

@@ -58,24 +58,11 @@ def test_prune_bitexact_against_restatement(kind, encoded):
 
 @pytest.mark.parametrize("case", [dict(seed=3, n=1500, width=97, height=61, height_std=0.02),
                                   dict(seed=6, n=1500, width=64, height=64, fisheye_fov=180.0)])
-def test_renders_with_pruned_faces_match_every_face(case, monkeypatch):
-    from powerfoam_amd import rasterize
+def test_renders_with_pruned_faces_match_every_face(case):
     c = harness.small_case(**case)
     gin = harness.random_grads(c, seed=2)
     print(f"render pruned vs every face: {case}", flush=True)
-    outs = {}
-    for prune in (True, False):
-        monkeypatch.setattr(rasterize.Rasterizer, "prune_faces", prune, raising=False)
-        orig = rasterize.Rasterizer.__init__
-
-        def init(self, *a, _p=prune, _o=orig, **k):
-            _o(self, *a, **k)
-            self.prune_faces = _p
-
-        monkeypatch.setattr(rasterize.Rasterizer, "__init__", init)
-        outs[prune] = harness.run_gpu(c, "cuda", gin)
-        monkeypatch.setattr(rasterize.Rasterizer, "__init__", orig)
-    (o1, g1), (o0, g0) = outs[True], outs[False]
+    (o1, g1), (o0, g0) = (harness.run_gpu(c, "cuda", gin, prune_faces=p) for p in (True, False))
     for k in ("color", "opacity", "normal", "normal_distance"):
         err = np.abs(o1[k] - o0[k]).max()
         print(k, err, flush=True)
