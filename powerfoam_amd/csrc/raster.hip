@@ -33,11 +33,11 @@
 //
 // The backward recomputes each segment back-to-front from the saved final
 // log-transmittance (rasterize.py:1219, 1372-1375) and reduces every per-site
-// gradient across the wave before touching global memory: neighbour-face
-// gradients are gathered in LDS per face slot, the site and texel gradients
-// are summed by an LDS transpose in 16-value slices (two texels per slice,
-// recomputed in a final pass so no per-texel accumulators stay live), and
-// one lane per value issues the atomic.
+// gradient across the wave in registers before touching global memory
+// (permlane swaps and DPP, wave_sum16): the neighbour-face gradients per
+// distinct face, the site and texel gradients in 16-value slices (two texels
+// per slice, recomputed in a final pass so no per-texel accumulators stay
+// live); one lane per value issues the atomic.
 #include <algorithm>
 #include <type_traits>
 
@@ -899,7 +899,6 @@ __global__ void __launch_bounds__(WAVE* WPB) PF_BWD_OCC raster_bwd_kernel(BwdPar
   __shared__ float4 s_face_all[WPB][WAVE];
   __shared__ TexelLds s_win_all[WPB][WAVE];  // texels of 8 walk entries
   __shared__ float s_gc_all[WPB][WAVE];      // dL/dcontrib of the batch's sites
-  __shared__ float s_fgrad_all[WPB][WAVE * 4];
 
   const int wv = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
@@ -917,7 +916,6 @@ __global__ void __launch_bounds__(WAVE* WPB) PF_BWD_OCC raster_bwd_kernel(BwdPar
   float4* s_face = s_face_all[wv];
   TexelLds* s_win = s_win_all[wv];
   float* s_gc = s_gc_all[wv];
-  float* s_fgrad = s_fgrad_all[wv];
 
   const Ray ray = lane_ray<PINHOLE>(p.cam, tile, lane);
   const size_t px = (size_t)ray.pi * p.cam.W + ray.pj;
@@ -1301,49 +1299,41 @@ __global__ void __launch_bounds__(WAVE* WPB) PF_BWD_OCC raster_bwd_kernel(BwdPar
         }
       }
 
-      // ---- neighbour gradients: gather per face slot in LDS, then flush
-      if (n_adj <= WAVE) {
-        wave_sync();
+      // ---- neighbour gradients, summed over the wave per face in
+      // registers: the rays of a tile mostly leave through the same few
+      // faces, so the distinct face ids of a side are taken four at a time
+      // (ballot), each lane's four values go to its face's slots of a
+      // 16-value slice and wave_sum16 sums them; one atomic per face value.
+      // (Per-lane LDS atomics into a slot per face serialise on those few
+      // addresses.)
 #pragma unroll
-        for (int q = 0; q < 4; q++) s_fgrad[q * WAVE + lane] = 0.0f;
-        wave_sync();
-        if (fid0 >= 0) {
-          atomicAdd(&s_fgrad[fid0 * 4 + 0], fg0.x);
-          atomicAdd(&s_fgrad[fid0 * 4 + 1], fg0.y);
-          atomicAdd(&s_fgrad[fid0 * 4 + 2], fg0.z);
-          atomicAdd(&s_fgrad[fid0 * 4 + 3], fg0.w);
-        }
-        if (fid1 >= 0) {
-          atomicAdd(&s_fgrad[fid1 * 4 + 0], fg1.x);
-          atomicAdd(&s_fgrad[fid1 * 4 + 1], fg1.y);
-          atomicAdd(&s_fgrad[fid1 * 4 + 2], fg1.z);
-          atomicAdd(&s_fgrad[fid1 * 4 + 3], fg1.w);
-        }
-        wave_sync();
-        if (lane < n_adj) {
-          const float4 v = *reinterpret_cast<const float4*>(&s_fgrad[lane * 4]);
-          if (v.x != 0.0f || v.y != 0.0f || v.z != 0.0f || v.w != 0.0f) {
-            float* g = p.g_spheres + 4 * (size_t)p.adjacency[sr.a0 + lane];
-            atomicAdd(g + 0, v.x);
-            atomicAdd(g + 1, v.y);
-            atomicAdd(g + 2, v.z);
-            atomicAdd(g + 3, v.w);
+      for (int side = 0; side < 2; side++) {
+        const int fid = side == 0 ? fid0 : fid1;
+        const float4 fg = side == 0 ? fg0 : fg1;
+        uint64_t pending = ballot(fid >= 0);
+        while (pending) {
+          int fk[4];
+          float v[RED_SLICE];
+#pragma unroll
+          for (int k = 0; k < 4; k++) {
+            int f = -1;
+            if (pending) {
+              f = __builtin_amdgcn_readlane(fid, __builtin_ctzll(pending));
+              pending &= ~ballot(fid == f);
+            }
+            fk[k] = f;
+            const bool in = f >= 0 && fid == f;
+            v[4 * k + 0] = in ? fg.x : 0.0f;
+            v[4 * k + 1] = in ? fg.y : 0.0f;
+            v[4 * k + 2] = in ? fg.z : 0.0f;
+            v[4 * k + 3] = in ? fg.w : 0.0f;
           }
-        }
-      } else {
-        if (fid0 >= 0) {
-          float* g = p.g_spheres + 4 * (size_t)p.adjacency[sr.a0 + fid0];
-          atomicAdd(g + 0, fg0.x);
-          atomicAdd(g + 1, fg0.y);
-          atomicAdd(g + 2, fg0.z);
-          atomicAdd(g + 3, fg0.w);
-        }
-        if (fid1 >= 0) {
-          float* g = p.g_spheres + 4 * (size_t)p.adjacency[sr.a0 + fid1];
-          atomicAdd(g + 0, fg1.x);
-          atomicAdd(g + 1, fg1.y);
-          atomicAdd(g + 2, fg1.z);
-          atomicAdd(g + 3, fg1.w);
+          const float acc = wave_sum16(v, lane);
+          // lane 4u holds value u: component u % 4 of face slot u / 4
+          const int u = lane >> 2, k = u >> 2;
+          const int f = k == 0 ? fk[0] : (k == 1 ? fk[1] : (k == 2 ? fk[2] : fk[3]));
+          if ((lane & 3) == 0 && f >= 0 && acc != 0.0f)
+            atomicAdd(p.g_spheres + 4 * (size_t)p.adjacency[sr.a0 + f] + (u & 3), acc);
         }
       }
 
