@@ -193,14 +193,18 @@ __device__ __forceinline__ float4 face_of(const int32_t* adjacency, const uint16
 // offset is NaN, so every crossing t computed from it is NaN and no bound
 // comparison accepts it. The face loops then read four faces per LDS round
 // trip. Returns the padded count.
+// s_adj (when set) takes the faces' neighbour ids, loaded beside the faces.
 template <bool PINHOLE>
 __device__ __forceinline__ int stage_faces(float4* s_face, const int32_t* adjacency, const uint16_t* diff,
-                                           const float4* spheres, int e0, int ec, float4 sph, f3 ce, int lane) {
+                                           const float4* spheres, int e0, int ec, float4 sph, f3 ce, int lane,
+                                           int* s_adj = nullptr) {
   const int ec4 = (ec + 3) & ~3;
-  if (lane < ec)
+  if (lane < ec) {
+    if (s_adj) s_adj[lane] = adjacency[e0 + lane];
     s_face[lane] = face_of<PINHOLE>(adjacency, diff, spheres, e0 + lane, sph, ce);
-  else if (lane < ec4)
+  } else if (lane < ec4) {
     s_face[lane] = make_float4(0.0f, 0.0f, 0.0f, __builtin_nanf(""));
+  }
   return ec4;
 }
 
@@ -899,6 +903,7 @@ __global__ void __launch_bounds__(WAVE* WPB) PF_BWD_OCC raster_bwd_kernel(BwdPar
   __shared__ float4 s_face_all[WPB][WAVE];
   __shared__ TexelLds s_win_all[WPB][WAVE];  // texels of 8 walk entries
   __shared__ float s_gc_all[WPB][WAVE];      // dL/dcontrib of the batch's sites
+  __shared__ int s_fadj_all[WPB][WAVE];      // neighbour ids of the staged faces
 
   const int wv = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
@@ -916,6 +921,7 @@ __global__ void __launch_bounds__(WAVE* WPB) PF_BWD_OCC raster_bwd_kernel(BwdPar
   float4* s_face = s_face_all[wv];
   TexelLds* s_win = s_win_all[wv];
   float* s_gc = s_gc_all[wv];
+  int* s_fadj = s_fadj_all[wv];
 
   const Ray ray = lane_ray<PINHOLE>(p.cam, tile, lane);
   const size_t px = (size_t)ray.pi * p.cam.W + ray.pj;
@@ -1017,7 +1023,8 @@ __global__ void __launch_bounds__(WAVE* WPB) PF_BWD_OCC raster_bwd_kernel(BwdPar
       for (int e0 = sr.a0; e0 < sr.a1; e0 += WAVE) {
         const int ec = __builtin_amdgcn_readfirstlane(min(WAVE, sr.a1 - e0));
         wave_sync();
-        const int ec4 = stage_faces<PINHOLE>(s_face, p.adjacency, p.adj_diff, p.spheres, e0, ec, sr.sph, ce, lane);
+        const int ec4 = stage_faces<PINHOLE>(s_face, p.adjacency, p.adj_diff, p.spheres, e0, ec, sr.sph, ce, lane,
+                                             s_fadj);
         wave_sync();
         // every lane runs the faces (a lane that misses the sphere only
         // updates values nothing reads): the loop stays in scalar control
@@ -1277,7 +1284,8 @@ __global__ void __launch_bounds__(WAVE* WPB) PF_BWD_OCC raster_bwd_kernel(BwdPar
           } else if (tid >= 0 && tid < n_adj) {
             // ray_pface_intersect_bwd (rendering_math.py:101-125) with the
             // neighbour's full-precision sphere
-            const int adj = p.adjacency[sr.a0 + tid];
+            // the id staged with the face when the list fits one chunk
+            const int adj = n_adj <= WAVE ? s_fadj[tid] : p.adjacency[sr.a0 + tid];
             const float4 as = p.spheres[adj];
             const f3 a{as.x, as.y, as.z};
             const f3 n = a - c;
@@ -1332,8 +1340,10 @@ __global__ void __launch_bounds__(WAVE* WPB) PF_BWD_OCC raster_bwd_kernel(BwdPar
           // lane 4u holds value u: component u % 4 of face slot u / 4
           const int u = lane >> 2, k = u >> 2;
           const int f = k == 0 ? fk[0] : (k == 1 ? fk[1] : (k == 2 ? fk[2] : fk[3]));
-          if ((lane & 3) == 0 && f >= 0 && acc != 0.0f)
-            atomicAdd(p.g_spheres + 4 * (size_t)p.adjacency[sr.a0 + f] + (u & 3), acc);
+          if ((lane & 3) == 0 && f >= 0 && acc != 0.0f) {
+            const int adj = n_adj <= WAVE ? s_fadj[f] : p.adjacency[sr.a0 + f];
+            atomicAdd(p.g_spheres + 4 * (size_t)adj + (u & 3), acc);
+          }
         }
       }
 
