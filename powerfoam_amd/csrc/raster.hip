@@ -1309,32 +1309,36 @@ __global__ void __launch_bounds__(WAVE* WPB) PF_BWD_OCC raster_bwd_kernel(BwdPar
 
       // ---- neighbour gradients, summed over the wave per face in
       // registers: the rays of a tile mostly leave through the same few
-      // faces, so the distinct face ids of a side are taken four at a time
-      // (ballot), each lane's four values go to its face's slots of a
-      // 16-value slice and wave_sum16 sums them; one atomic per face value.
-      // (Per-lane LDS atomics into a slot per face serialise on those few
-      // addresses.)
-#pragma unroll
-      for (int side = 0; side < 2; side++) {
-        const int fid = side == 0 ? fid0 : fid1;
-        const float4 fg = side == 0 ? fg0 : fg1;
-        uint64_t pending = ballot(fid >= 0);
-        while (pending) {
+      // faces, so the distinct (side, face id) pairs of the entry and exit
+      // faces are taken four at a time (ballot), each lane's four values go
+      // to its face's slots of a 16-value slice and wave_sum16 sums them;
+      // one atomic per face value. (Per-lane LDS atomics into a slot per
+      // face serialise on those few addresses.)
+      {
+        uint64_t pend0 = ballot(fid0 >= 0), pend1 = ballot(fid1 >= 0);
+        while (pend0 | pend1) {
           int fk[4];
           float v[RED_SLICE];
 #pragma unroll
           for (int k = 0; k < 4; k++) {
             int f = -1;
-            if (pending) {
-              f = __builtin_amdgcn_readlane(fid, __builtin_ctzll(pending));
-              pending &= ~ballot(fid == f);
+            bool in = false;
+            float4 g = fg0;
+            if (pend0) {
+              f = __builtin_amdgcn_readlane(fid0, __builtin_ctzll(pend0));
+              in = fid0 == f;
+              pend0 &= ~ballot(in);
+            } else if (pend1) {
+              f = __builtin_amdgcn_readlane(fid1, __builtin_ctzll(pend1));
+              in = fid1 == f;
+              pend1 &= ~ballot(in);
+              g = fg1;
             }
             fk[k] = f;
-            const bool in = f >= 0 && fid == f;
-            v[4 * k + 0] = in ? fg.x : 0.0f;
-            v[4 * k + 1] = in ? fg.y : 0.0f;
-            v[4 * k + 2] = in ? fg.z : 0.0f;
-            v[4 * k + 3] = in ? fg.w : 0.0f;
+            v[4 * k + 0] = in ? g.x : 0.0f;
+            v[4 * k + 1] = in ? g.y : 0.0f;
+            v[4 * k + 2] = in ? g.z : 0.0f;
+            v[4 * k + 3] = in ? g.w : 0.0f;
           }
           const float acc = wave_sum16(v, lane);
           // lane 4u holds value u: component u % 4 of face slot u / 4
@@ -1348,28 +1352,21 @@ __global__ void __launch_bounds__(WAVE* WPB) PF_BWD_OCC raster_bwd_kernel(BwdPar
       }
 
       // ---- texel and site gradients: summed over the wave in registers
-      // (wave_sum16) in 16-value slices, one atomic per value. Texel values
-      // are produced two texels (14 values) per slice, so no per-texel
-      // accumulators stay live across the adjoint.
-      auto flush_slice = [&](const float* v, int count, int v0) {
-        float w[RED_SLICE];
-#pragma unroll
-        for (int q = 0; q < RED_SLICE; q++) w[q] = q < count ? v[q] : 0.0f;
-        const float acc = wave_sum16(w, lane);
-        const int u = lane >> 2;
-        if ((lane & 3) == 0 && u < count && acc != 0.0f) atomicAdd(grad_slot(p, sr.site, T, v0 + u), acc);
-      };
-      // the site's own values first: their registers are free for the texels
-      {
-        const float v[8] = {dLdc.x, dLdc.y, dLdc.z, dLdr, dLdnorm.x, dLdnorm.y, dLdnorm.z, dLdsig};
-        flush_slice(v, 8, 0);
-      }
+      // (wave_sum16) in four 16-value slices, one atomic per value: slice j
+      // holds the 14 values of texels 2j and 2j + 1 (slots 0-13) and the
+      // site's own values 2j and 2j + 1 (slots 14-15), so the 8 + 7 * 8
+      // values of a site fill four reductions. Texel values are produced two
+      // texels per slice, so no per-texel accumulators stay live across the
+      // adjoint.
+      const float site_v[8] = {dLdc.x, dLdc.y, dLdc.z, dLdr, dLdnorm.x, dLdnorm.y, dLdnorm.z, dLdsig};
       // Branch-free: a skipped texel (w / sum w < cut, or an inactive lane)
       // selects zero, so no per-texel control flow forms in this pass.
 #pragma unroll
       for (int i0 = 0; i0 < PF_MAX_TEXELS; i0 += 2) {
-        if (i0 < T) {
-          float v[14];
+        {
+          float v[RED_SLICE];
+          v[14] = site_v[i0];
+          v[15] = site_v[i0 + 1];
 #pragma unroll
           for (int k = 0; k < 2; k++) {
             // i <= 7 stays inside s_tex; values past T are never flushed
@@ -1405,7 +1402,15 @@ __global__ void __launch_bounds__(WAVE* WPB) PF_BWD_OCC raster_bwd_kernel(BwdPar
             v[7 * k + 1] = sy;
             v[7 * k + 2] = sz;
           }
-          flush_slice(v, i0 + 1 < T ? 14 : 7, 8 + 7 * i0);
+          // texels past T hold zeros and are not flushed
+#pragma unroll
+          for (int q = 0; q < 14; q++)
+            if (i0 + q / 7 >= T) v[q] = 0.0f;
+          const float acc = wave_sum16(v, lane);
+          const int u = lane >> 2;  // this lane's value of the slice
+          const bool valid = u >= 14 || i0 + u / 7 < T;
+          const int slot = u >= 14 ? i0 + (u - 14) : 8 + 7 * i0 + u;
+          if ((lane & 3) == 0 && valid && acc != 0.0f) atomicAdd(grad_slot(p, sr.site, T, slot), acc);
         }
       }
     }
