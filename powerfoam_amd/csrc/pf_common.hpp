@@ -57,11 +57,43 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 // ------------------------------------------------------- wave64 reductions --
-// Full-wave sum, every lane must be active; the result is uniform.
+// x of the lane that DPP control CTRL names (within the lane's row)
+template <int CTRL>
+__device__ __forceinline__ float dpp(float x) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), CTRL, 0xf, 0xf, false));
+}
+
+// v_permlane32_swap_b32 a, b: the upper 32 lanes of a trade places with the
+// lower 32 lanes of b; v_permlane16_swap_b32 a, b: the odd rows (16 lanes) of
+// a with the even rows of b (gfx950). Written as instructions that read and
+// write both registers: through the compiler builtins, hipcc (ROCm 7.2) adds
+// the first result to itself in place of the pair's sum. A swap must follow a
+// VALU write of its operands by two wait states (the s_nop 1 the compiler
+// puts there for the builtins; it does not look inside asm).
+__device__ __forceinline__ void permlane32_swap(float& a, float& b) {
+  asm("s_nop 1\n\tv_permlane32_swap_b32 %0, %1" : "+v"(a), "+v"(b));
+}
+__device__ __forceinline__ void permlane16_swap(float& a, float& b) {
+  asm("s_nop 1\n\tv_permlane16_swap_b32 %0, %1" : "+v"(a), "+v"(b));
+}
+
+// Full-wave sum, every lane must be active; the result is uniform (each
+// step adds a lane's value to its partner's and the partner does the same
+// sum). All in registers: DPP pairs i, i^1 and i, i^2 (quad_perm), the two
+// quads of a half-row (row_half_mirror), the halves of a row (row_ror:8),
+// then the rows and half-waves swap.
 __device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
-  return v;
+  v += dpp<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp<0x141>(v);  // row_half_mirror
+  v += dpp<0x128>(v);  // row_ror:8
+  float a = v, b = v;
+  permlane16_swap(a, b);
+  v = a + b;
+  a = v;
+  b = v;
+  permlane32_swap(a, b);
+  return a + b;
 }
 
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }

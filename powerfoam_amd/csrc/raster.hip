@@ -817,26 +817,6 @@ struct BwdParams {
 
 constexpr int RED_SLICE = 16;  // values reduced per pass
 
-// x of the lane that DPP control CTRL names (within the lane's row)
-template <int CTRL>
-__device__ __forceinline__ float dpp(float x) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), CTRL, 0xf, 0xf, false));
-}
-
-// v_permlane32_swap_b32 a, b: the upper 32 lanes of a trade places with the
-// lower 32 lanes of b; v_permlane16_swap_b32 a, b: the odd rows (16 lanes) of
-// a with the even rows of b. Written as instructions that read and write both
-// registers: through the compiler builtins, hipcc (ROCm 7.2) adds the first
-// result to itself in place of the pair's sum. A swap must follow a VALU write
-// of its operands by two wait states (the s_nop 1 the compiler puts there for
-// the builtins; it does not look inside asm).
-__device__ __forceinline__ void permlane32_swap(float& a, float& b) {
-  asm("s_nop 1\n\tv_permlane32_swap_b32 %0, %1" : "+v"(a), "+v"(b));
-}
-__device__ __forceinline__ void permlane16_swap(float& a, float& b) {
-  asm("s_nop 1\n\tv_permlane16_swap_b32 %0, %1" : "+v"(a), "+v"(b));
-}
-
 // Sums 16 values over the wave without LDS: a butterfly reduce-scatter in
 // registers. permlane32_swap exchanges the half-waves and permlane16_swap the
 // odd and even rows of two values at once, so each exchange halves the values
