@@ -13,7 +13,8 @@ basis (1, Y_1(d) .. Y_15(d)), exactly what the SH head contracts with its 16
 coefficients per channel, or the frequency encoding (1, d, sin(pi d),
 cos(pi d), sin(2 pi d), cos(2 pi d)), per component. The kernels
 read bf16 copies of the latents and weights; `MLPColor` keeps FP32 masters
-(what the optimiser updates) and refreshes the bf16 copies when they change.
+(what the optimiser updates) and makes the bf16 copies again when a master's
+version counter moved (an optimiser step, an in-place write).
 
 `MLPColor.forward(texel_sites, camera)` returns texel_rgb [N,T,3] for a
 camera and is differentiable w.r.t. the latents and weights (not the texel
@@ -84,14 +85,27 @@ class MLPColor:
         for p in self.params.values():
             p.requires_grad_(True)
         self.event_log = None
+        self._bf16_cache, self._bf16_stamp = None, None
         # texel-mode backward over the sites with a colour gradient only
         # (pf_mlp_active_sites); False: every row
         self.compact_backward = True
 
     # ------------------------------------------------------------- plumbing
     def _bf16(self):
-        """bf16 copies the kernels read (16-byte aligned, contiguous)."""
-        return {k: self.params[k].detach().to(torch.bfloat16).contiguous() for k in ("latents", "w1", "w2", "w3")}
+        """bf16 copies the kernels read (16-byte aligned, contiguous), made
+        again only when an FP32 master changed since (its version counter,
+        which in-place optimiser steps and assignments advance)."""
+        keys = ("latents", "w1", "w2", "w3")
+        stamp = tuple((self.params[k].data_ptr(), self.params[k]._version) for k in keys)
+        if self._bf16_stamp != stamp:
+            self._bf16_cache = {k: self.params[k].detach().to(torch.bfloat16).contiguous() for k in keys}
+            self._bf16_stamp = stamp
+        return self._bf16_cache
+
+    def refresh(self):
+        """Drop the bf16 copies: for writes that bypass the version counter
+        (through `.data`); in-place writes and optimiser steps need no call."""
+        self._bf16_cache, self._bf16_stamp = None, None
 
     def _weights(self, bf):
         w = _abi.PfMlpWeights()
