@@ -92,19 +92,57 @@ __device__ __forceinline__ void walk_ray(const TraceParams& p, int i, int j, f3&
   d = f3{rd.x / len, rd.y / len, rd.z / len};
 }
 
-// Face e of a cell (centre c, radius r): normal n and offset pm, from the
-// binary16 (c_adj - c, pm_adj - pm) encoding or the exact spheres.
-__device__ __forceinline__ void walk_face(const TraceParams& p, int e, f3 c, float r, f3& n, float& pm) {
+// The faces of a cell (centre c, radius r) four at a time. A group's
+// neighbour ids, and their spheres (exact geometry) or binary16
+// (c_adj - c, pm_adj - pm) records, are all requested before any is used, and
+// the ids of the next group are requested while this group's spheres load, so
+// a lane waits about one memory latency per group instead of two per face.
+// Slots past the cell's last face repeat that face: a repeated face moves no
+// strict comparison of the walk.
+struct FaceQuad {
+  int id[4];
+  f3 n[4];
+  float pm[4];
+  float4 nb[4];  // neighbour spheres (exact geometry only)
+};
+
+__device__ __forceinline__ void quad_ids(const TraceParams& p, int e0, int a1, int id[4]) {
+#pragma unroll
+  for (int k = 0; k < 4; k++) id[k] = p.adjacency[min(e0 + k, a1 - 1)];
+}
+
+// group e0 .. e0 + 3 (e0 < a1) from its ids; ids become those of the group at
+// e0 + 4 when there is one. HALF: faces from the binary16 records, else from
+// the neighbour spheres.
+template <bool HALF>
+__device__ __forceinline__ void quad_faces(const TraceParams& p, int e0, int a1, f3 c, float r, int ids[4],
+                                           FaceQuad& q) {
 #pragma clang fp contract(off)
-  if (p.adj_diff) {
-    const uint2 raw = *reinterpret_cast<const uint2*>(p.adj_diff + 4 * (int64_t)e);
-    n = f3{half_bits_to_float((uint16_t)(raw.x & 0xffff)), half_bits_to_float((uint16_t)(raw.x >> 16)),
-           half_bits_to_float((uint16_t)(raw.y & 0xffff))};
-    pm = half_bits_to_float((uint16_t)(raw.y >> 16));
+#pragma unroll
+  for (int k = 0; k < 4; k++) q.id[k] = ids[k];
+  if (HALF) {
+    uint2 raw[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) raw[k] = *reinterpret_cast<const uint2*>(p.adj_diff + 4 * (int64_t)min(e0 + k, a1 - 1));
+    if (e0 + 4 < a1) quad_ids(p, e0 + 4, a1, ids);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      q.n[k] = f3{half_bits_to_float((uint16_t)(raw[k].x & 0xffff)), half_bits_to_float((uint16_t)(raw[k].x >> 16)),
+                  half_bits_to_float((uint16_t)(raw[k].y & 0xffff))};
+      q.pm[k] = half_bits_to_float((uint16_t)(raw[k].y >> 16));
+      q.nb[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
   } else {
-    const float4 a = p.spheres[p.adjacency[e]];
-    n = f3{a.x - c.x, a.y - c.y, a.z - c.z};
-    pm = 0.5f * ((a.x * a.x + a.y * a.y + a.z * a.z) - a.w * a.w) - 0.5f * ((c.x * c.x + c.y * c.y + c.z * c.z) - r * r);
+#pragma unroll
+    for (int k = 0; k < 4; k++) q.nb[k] = p.spheres[q.id[k]];
+    if (e0 + 4 < a1) quad_ids(p, e0 + 4, a1, ids);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const float4 a = q.nb[k];
+      q.n[k] = f3{a.x - c.x, a.y - c.y, a.z - c.z};
+      q.pm[k] = 0.5f * ((a.x * a.x + a.y * a.y + a.z * a.z) - a.w * a.w) -
+                0.5f * ((c.x * c.x + c.y * c.y + c.z * c.z) - r * r);
+    }
   }
 }
 
@@ -199,7 +237,7 @@ __device__ __forceinline__ bool trace_lane(const TraceParams& p, int tile, int l
 // (slot = the wave's counter in LDS + the lane's rank among the lanes that
 // ended in the same step, from a ballot), so lanes stay busy until the group
 // runs out of rays instead of idling until the tile's longest ray ends.
-template <bool COMPACT>
+template <bool COMPACT, bool HALF>
 __global__ void __launch_bounds__(TWAVE* TWPB) trace_fwd_kernel(TraceParams p, int group_tiles) {
   __shared__ int s_next[TWPB];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -228,6 +266,9 @@ __global__ void __launch_bounds__(TWAVE* TWPB) trace_fwd_kernel(TraceParams p, i
   f3 o, d;
   walk_ray(p, i, j, o, d);
   int prim = p.start_sites ? p.start_sites[px] : p.start_site;
+  // the sphere of the cell the walk enters, when the face loop read it
+  float4 csph = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  bool carried = false;
   float log_t = 0.0f, depth = -1.0f, qsig = 0.0f;
   int qstep = -1, step = 0;
   f3 rgb{0.0f, 0.0f, 0.0f}, nacc{0.0f, 0.0f, 0.0f};
@@ -268,6 +309,7 @@ __global__ void __launch_bounds__(TWAVE* TWPB) trace_fwd_kernel(TraceParams p, i
       px = (size_t)i * p.W + j;
       walk_ray(p, i, j, o, d);
       prim = p.start_sites ? p.start_sites[px] : p.start_site;
+      carried = false;
       log_t = 0.0f;
       depth = -1.0f;
       qsig = 0.0f;
@@ -278,7 +320,9 @@ __global__ void __launch_bounds__(TWAVE* TWPB) trace_fwd_kernel(TraceParams p, i
       continue;
     }
     if (p.cells && step < p.K) p.cells[px * p.K + step] = prim;
-    const float4 sph = p.spheres[prim];
+    const float4 sph = carried ? csph : p.spheres[prim];
+    const int a0 = p.adj_off[prim], a1 = p.adj_off[prim + 1];
+    const float4 ns = p.nsigmas[prim];
     const f3 c{sph.x, sph.y, sph.z};
     const float r = sph.w;
     float tn, tf;
@@ -287,20 +331,25 @@ __global__ void __launch_bounds__(TWAVE* TWPB) trace_fwd_kernel(TraceParams p, i
     if (__builtin_sqrtf(dot(oc, oc)) < 4.0f * r) hit = false;
     int next = SENTINEL;
     float pt_far = 1e10f;
-    const int a0 = p.adj_off[prim], a1 = p.adj_off[prim + 1];
-    for (int e = a0; e < a1; e++) {
-      f3 n;
-      float pm, dp;
-      walk_face(p, e, c, r, n, pm);
-      const float t = walk_cross(o, d, n, pm, dp);
-      if (dp >= 0.0f && t < pt_far) {
-        next = p.adjacency[e];
-        pt_far = t;
+    int ids[4];
+    if (a0 < a1) quad_ids(p, a0, a1, ids);
+    for (int e0 = a0; e0 < a1; e0 += 4) {
+      FaceQuad q;
+      quad_faces<HALF>(p, e0, a1, c, r, ids, q);
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        float dp;
+        const float t = walk_cross(o, d, q.n[k], q.pm[k], dp);
+        if (dp >= 0.0f && t < pt_far) {
+          next = q.id[k];
+          csph = q.nb[k];
+          pt_far = t;
+        }
+        tf = (dp >= 0.0f && t < tf) ? t : tf;
+        tn = (dp < 0.0f && t > tn) ? t : tn;
       }
-      tf = (dp >= 0.0f && t < tf) ? t : tf;
-      tn = (dp < 0.0f && t > tn) ? t : tn;
     }
-    const float4 ns = p.nsigmas[prim];
+    carried = !HALF;
     const float sigma = ns.w;
     if (hit && !(tn > tf) && !(sigma < 1e-3f)) {
       const f3 pn{ns.x, ns.y, ns.z};
@@ -381,7 +430,9 @@ __device__ __forceinline__ void gc_flush(const TraceParams& p, const GradCache& 
   }
 }
 
-__global__ void __launch_bounds__(TWAVE* TWPB) trace_bwd_kernel(TraceParams p) {
+template <bool HALF>
+__global__ void __launch_bounds__(TWAVE* TWPB) __attribute__((amdgpu_waves_per_eu(3)))
+trace_bwd_kernel(TraceParams p) {
   __shared__ GradCache s_gc[TWPB];
   const int tile = blockIdx.x * TWPB + (threadIdx.x >> 6);
   if (tile >= p.n_tiles) return;
@@ -405,6 +456,8 @@ __global__ void __launch_bounds__(TWAVE* TWPB) trace_bwd_kernel(TraceParams p) {
   const int qstep = live ? p.s_q_step[px] : -1;
   const float qsig = live ? p.s_q_sigma[px] : 1.0f;
   int prim = live ? (p.start_sites ? p.start_sites[px] : p.start_site) : SENTINEL;
+  float4 csph = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  bool carried = false;
   float log_t = 0.0f;
   int step = 0;
   f3 Cp{0.0f, 0.0f, 0.0f}, Np{0.0f, 0.0f, 0.0f};
@@ -414,7 +467,10 @@ __global__ void __launch_bounds__(TWAVE* TWPB) trace_bwd_kernel(TraceParams p) {
     if (live && (trans < p.thr || prim == SENTINEL || prim < 0 || step >= p.max_steps)) live = false;
     if (ballot(live) == 0) break;
     float4 sph = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
-    if (live) sph = p.spheres[prim];
+    if (live) sph = carried ? csph : p.spheres[prim];
+    const int a0 = live ? p.adj_off[prim] : 0, a1 = live ? p.adj_off[prim + 1] : 0;
+    float4 ns = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (live) ns = p.nsigmas[prim];
     const f3 c{sph.x, sph.y, sph.z};
     const float r = sph.w;
     float tn = 0.0f, tf = 0.0f;
@@ -424,28 +480,32 @@ __global__ void __launch_bounds__(TWAVE* TWPB) trace_bwd_kernel(TraceParams p) {
     int tn_id = tn < 1e-8f ? -2 : -1, tf_id = tf < 1e-8f ? -2 : -1;
     int next = SENTINEL;
     float pt_far = 1e10f;
-    const int a0 = live ? p.adj_off[prim] : 0, a1 = live ? p.adj_off[prim + 1] : 0;
     const int n_adj = a1 - a0;
-    for (int e = a0; e < a1; e++) {
-      f3 n;
-      float pm, dp;
-      walk_face(p, e, c, r, n, pm);
-      const float t = walk_cross(o, d, n, pm, dp);
-      if (dp >= 0.0f && t < pt_far) {
-        next = p.adjacency[e];
-        pt_far = t;
-      }
-      if (dp >= 0.0f && t < tf) {
-        tf_id = e - a0;
-        tf = t;
-      }
-      if (dp < 0.0f && t > tn) {
-        tn_id = e - a0;
-        tn = t;
+    int ids[4];
+    if (a0 < a1) quad_ids(p, a0, a1, ids);
+    for (int e0 = a0; e0 < a1; e0 += 4) {
+      FaceQuad q;
+      quad_faces<HALF>(p, e0, a1, c, r, ids, q);
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        float dp;
+        const float t = walk_cross(o, d, q.n[k], q.pm[k], dp);
+        if (dp >= 0.0f && t < pt_far) {
+          next = q.id[k];
+          csph = q.nb[k];
+          pt_far = t;
+        }
+        if (dp >= 0.0f && t < tf) {
+          tf_id = e0 + k - a0;
+          tf = t;
+        }
+        if (dp < 0.0f && t > tn) {
+          tn_id = e0 + k - a0;
+          tn = t;
+        }
       }
     }
-    float4 ns = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (live) ns = p.nsigmas[prim];
+    carried = !HALF;
     const float sigma = ns.w;
     const f3 pn{ns.x, ns.y, ns.z};
     const f3 co = -oc;
@@ -780,7 +840,8 @@ __global__ void __launch_bounds__(256)
 static int check_trace(const pf_trace_args* a, const char* who) {
   if (!a) return set_error(PF_ERR_ARG, "%s: null args", who);
   const pf_scene& s = a->scene;
-  if (s.n_sites < 1 || !s.spheres || !s.nsigmas || !s.texel_sites || !s.texel_rgbh || !s.adjacency_offsets)
+  if (s.n_sites < 1 || !s.spheres || !s.nsigmas || !s.texel_sites || !s.texel_rgbh || !s.adjacency_offsets ||
+      !s.adjacency)
     return set_error(PF_ERR_ARG, "%s: missing scene arrays", who);
   if (s.n_texels < 1 || s.n_texels > PF_MAX_TEXELS) return set_error(PF_ERR_ARG, "%s: n_texels out of range", who);
   if (a->cam.width < 2 || a->cam.height < 2) return set_error(PF_ERR_ARG, "%s: image must be >= 2x2", who);
@@ -810,12 +871,21 @@ int pf_trace_fwd(const pf_trace_args* a, void* stream) {
   p.cells = a->cells;
   p.K = a->max_cells;
   p.n_cells = a->n_cells;
+  const bool half = p.adj_diff != nullptr;
   if (a->compact > 0) {
     const int group_tiles = a->compact;
     const int groups = (p.n_tiles + group_tiles - 1) / group_tiles;
-    trace_fwd_kernel<true><<<(groups + TWPB - 1) / TWPB, TWAVE * TWPB, 0, as_stream(stream)>>>(p, group_tiles);
+    const dim3 grid((groups + TWPB - 1) / TWPB);
+    if (half)
+      trace_fwd_kernel<true, true><<<grid, TWAVE * TWPB, 0, as_stream(stream)>>>(p, group_tiles);
+    else
+      trace_fwd_kernel<true, false><<<grid, TWAVE * TWPB, 0, as_stream(stream)>>>(p, group_tiles);
   } else {
-    trace_fwd_kernel<false><<<(p.n_tiles + TWPB - 1) / TWPB, TWAVE * TWPB, 0, as_stream(stream)>>>(p, 1);
+    const dim3 grid((p.n_tiles + TWPB - 1) / TWPB);
+    if (half)
+      trace_fwd_kernel<false, true><<<grid, TWAVE * TWPB, 0, as_stream(stream)>>>(p, 1);
+    else
+      trace_fwd_kernel<false, false><<<grid, TWAVE * TWPB, 0, as_stream(stream)>>>(p, 1);
   }
   PF_HIP(hipGetLastError());
   return PF_OK;
@@ -867,7 +937,11 @@ int pf_trace_bwd(const pf_trace_bwd_args* b, void* stream) {
   p.g_tsites = b->grad_texel_sites;
   p.g_trgb = b->grad_texel_rgb;
   p.g_theight = b->grad_texel_height;
-  trace_bwd_kernel<<<(p.n_tiles + TWPB - 1) / TWPB, TWAVE * TWPB, 0, as_stream(stream)>>>(p);
+  const dim3 grid((p.n_tiles + TWPB - 1) / TWPB);
+  if (p.adj_diff)
+    trace_bwd_kernel<true><<<grid, TWAVE * TWPB, 0, as_stream(stream)>>>(p);
+  else
+    trace_bwd_kernel<false><<<grid, TWAVE * TWPB, 0, as_stream(stream)>>>(p);
   PF_HIP(hipGetLastError());
   return PF_OK;
 }
