@@ -98,6 +98,21 @@ __device__ __forceinline__ float wave_sum(float v) {
 
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 
+// Atomic adds with the address space spelled out: p must address LDS
+// (lds_add, a ds_add_f32) or global memory (global_add). Plain atomicAdd on
+// pointers that two branches select between can be merged into one generic
+// (flat) atomic, which goes through the vector memory path even for LDS.
+typedef __attribute__((address_space(3))) float lds_float;
+typedef __attribute__((address_space(1))) float global_float;
+__device__ __forceinline__ void lds_add(float* p, float v) {
+  __hip_atomic_fetch_add((lds_float*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void global_add(float* p, float v) {
+  __hip_atomic_fetch_add((global_float*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// a load that p must serve from LDS (ds_read), for the same reason
+__device__ __forceinline__ int lds_load(const int* p) { return *(const __attribute__((address_space(3))) int*)p; }
+
 // Uniform-broadcast of lane `l`'s 32-bit value.
 __device__ __forceinline__ int readlane(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
 __device__ __forceinline__ float readlane(float v, int l) {

@@ -1265,7 +1265,7 @@ __global__ void __launch_bounds__(WAVE* WPB) PF_BWD_OCC raster_bwd_kernel(BwdPar
             // ray_pface_intersect_bwd (rendering_math.py:101-125) with the
             // neighbour's full-precision sphere
             // the id staged with the face when the list fits one chunk
-            const int adj = n_adj <= WAVE ? s_fadj[tid] : p.adjacency[sr.a0 + tid];
+            const int adj = n_adj <= WAVE ? lds_load(s_fadj + tid) : p.adjacency[sr.a0 + tid];
             const float4 as = p.spheres[adj];
             const f3 a{as.x, as.y, as.z};
             const f3 n = a - c;
@@ -1325,7 +1325,7 @@ __global__ void __launch_bounds__(WAVE* WPB) PF_BWD_OCC raster_bwd_kernel(BwdPar
           const int u = lane >> 2, k = u >> 2;
           const int f = k == 0 ? fk[0] : (k == 1 ? fk[1] : (k == 2 ? fk[2] : fk[3]));
           if ((lane & 3) == 0 && f >= 0 && acc != 0.0f) {
-            const int adj = n_adj <= WAVE ? s_fadj[f] : p.adjacency[sr.a0 + f];
+            const int adj = n_adj <= WAVE ? lds_load(s_fadj + f) : p.adjacency[sr.a0 + f];
             atomicAdd(p.g_spheres + 4 * (size_t)adj + (u & 3), acc);
           }
         }

@@ -426,7 +426,7 @@ __device__ __forceinline__ void gc_flush(const TraceParams& p, const GradCache& 
   const int s = gc.tag[q];
   if (s >= 0 && lane < 7 * T + 8) {
     const float v = gc.v[q][lane];
-    if (v != 0.0f) atomicAdd(grad_addr(p, s, lane, T), v);
+    if (v != 0.0f) global_add(grad_addr(p, s, lane, T), v);
   }
 }
 
@@ -559,9 +559,9 @@ trace_bwd_kernel(TraceParams p) {
     // value k of this visit's site gradient: into the cache, or to global memory
     auto acc = [&](int k, float v) {
       if (in_lds)
-        atomicAdd(cache + k, v);
+        lds_add(cache + k, v);
       else
-        atomicAdd(grad_addr(p, prim, k, T), v);
+        global_add(grad_addr(p, prim, k, T), v);
     };
     if (act) {
         const float u = sigma * dt;
@@ -694,12 +694,16 @@ trace_bwd_kernel(TraceParams p) {
             // slot when the slot holds it (tags only change at the top of a
             // step), else goes to global memory
             const int aslot = adj & (GC_SLOTS - 1);
-            float* gadj = gc.tag[aslot] == adj ? gc.v[aslot] + 7 * T : p.g_spheres + 4 * (size_t)adj;
+            const bool adj_lds = gc.tag[aslot] == adj;
             const float gr = g * (-as.w * idp);
 #pragma unroll
             for (int i = 0; i < 4; i++) {
               const int k = (i + lane) & 3;
-              atomicAdd(gadj + k, k == 0 ? ga.x : k == 1 ? ga.y : k == 2 ? ga.z : gr);
+              const float v = k == 0 ? ga.x : k == 1 ? ga.y : k == 2 ? ga.z : gr;
+              if (adj_lds)
+                lds_add(gc.v[aslot] + 7 * T + k, v);
+              else
+                global_add(p.g_spheres + 4 * (size_t)adj + k, v);
             }
           }
         }
