@@ -236,6 +236,10 @@ typedef struct pf_trace_args {
    * ended in the same step); 0: one wave per tile, a lane idles once its ray
    * ends. Same results either way. */
   int32_t compact;
+  /* [n_tiles] or NULL (compact = 0 only): per 8x8 tile, the longest walk
+   * (cells visited) of its rays, for pf_trace_tile_order. n_tiles =
+   * ceil(H/8) * ceil(W/8), tile t = column-major (t % ceil(H/8) the tile row). */
+  int32_t* tile_steps;
 } pf_trace_args;
 
 /* Exact adjoint of pf_trace_fwd (no reference counterpart: the reference ray
@@ -256,6 +260,10 @@ typedef struct pf_trace_bwd_args {
   float* grad_texel_sites;     /* [N,T,3] */
   float* grad_texel_rgb;       /* [N,T,3] */
   float* grad_texel_height;    /* [N,T] */
+  /* [n_tiles] or NULL: the order in which the backward's waves take the tiles
+   * (a permutation of 0..n_tiles-1, e.g. from pf_trace_tile_order); NULL:
+   * tile index order. The summation order of the gradients follows it. */
+  const int32_t* tile_order;
 } pf_trace_bwd_args;
 
 /* ---- library ---------------------------------------------------------- */
@@ -508,6 +516,14 @@ int pf_raster_vis(const pf_raster_vis_args* args, void* stream);
 /* ---- ray tracer (RayTracer.benchmark_kernel raytrace.py:115-241) ------ */
 int pf_trace_fwd(const pf_trace_args* args, void* stream);
 int pf_trace_bwd(const pf_trace_bwd_args* args, void* stream);
+
+/* Longest-walk-first dispatch order for pf_trace_bwd (no reference
+ * counterpart: the reference tracer has no backward). tile_order [n_tiles]
+ * gets the tiles by descending tile_steps (as pf_trace_fwd writes it), walks
+ * of 1023 cells or more sharing the first bucket; the order among equal
+ * lengths is unspecified. Waves then start on the long tiles first instead of
+ * finishing on them. */
+int pf_trace_tile_order(const int32_t* tile_steps, int32_t n_tiles, int32_t* tile_order, void* stream);
 
 /* FP64 recomputation of the walk on sampled rays (no reference counterpart:
  * the precision check of the FP32 walk at full scene size). For each ray

@@ -220,6 +220,7 @@ class PfTraceArgs(ctypes.Structure):
         ("max_cells", ctypes.c_int32),
         ("n_cells", ctypes.c_void_p),
         ("compact", ctypes.c_int32),
+        ("tile_steps", ctypes.c_void_p),
     ]
 
 
@@ -227,7 +228,7 @@ class PfTraceBwdArgs(ctypes.Structure):
     _fields_ = [("fwd", PfTraceArgs)] + [(k, ctypes.c_void_p) for k in (
         "color", "normal", "log_t", "q_step", "q_sigma", "grad_color", "grad_normal", "grad_depth",
         "grad_opacity", "grad_spheres", "grad_nsigmas", "grad_texel_sites", "grad_texel_rgb",
-        "grad_texel_height")]
+        "grad_texel_height", "tile_order")]
 
 
 class PfTraceCheckArgs(ctypes.Structure):
@@ -307,6 +308,7 @@ SIGNATURES = {
     "pf_mlp_active_sites": ([_VP, ctypes.c_int32, ctypes.c_int32, _VP, _VP, _VP], ctypes.c_int),
     "pf_trace_fwd": ([_VP, _VP], ctypes.c_int),
     "pf_trace_bwd": ([_VP, _VP], ctypes.c_int),
+    "pf_trace_tile_order": ([_VP, ctypes.c_int32, _VP, _VP], ctypes.c_int),
     "pf_trace_check_f64": ([_VP, _VP], ctypes.c_int),
     "pf_pack_sites": ([_VP] * 6 + [ctypes.c_int32, ctypes.c_int32] + [_VP] * 4, ctypes.c_int),
     "pf_scrub_nonfinite": ([_VP, ctypes.c_int64, _VP], ctypes.c_int),

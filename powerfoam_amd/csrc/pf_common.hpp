@@ -98,6 +98,21 @@ __device__ __forceinline__ float wave_sum(float v) {
 
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 
+// Workgroups are dealt round-robin to the 8 XCDs; give each XCD a contiguous
+// run of tile groups so neighbouring tiles (which share sites) share an L2.
+// Returns the index of workgroup b's group among n_groups (valid: b maps to
+// one; launch a whole number of rounds of 8 workgroups).
+__device__ __forceinline__ int xcd_group(int b, int n_groups, bool& valid) {
+  const int per = (n_groups + 7) >> 3;
+  const int xcd = b & 7, k = b >> 3;
+  const int full = n_groups - (per - 1) * 8;  // XCDs that own `per` groups
+  valid = xcd < full ? k < per : k < per - 1;
+  return xcd < full ? xcd * per + k : full * per + (xcd - full) * (per - 1) + k;
+}
+
+// Workgroups for n work items at nw waves each: a whole number of XCD rounds.
+static inline int xcd_grid(int n, int nw) { return (((n + nw - 1) / nw + 7) / 8) * 8; }
+
 // Atomic adds with the address space spelled out: p must address LDS
 // (lds_add, a ds_add_f32) or global memory (global_add). Plain atomicAdd on
 // pointers that two branches select between can be merged into one generic

@@ -79,20 +79,9 @@ static RasterCam make_rcam(const pf_camera& c) {
   return r;
 }
 
-// Workgroups are dealt round-robin to the 8 XCDs; give each XCD a contiguous
-// run of tile groups so neighbouring tiles (which share sites) share an L2.
-// Returns the first tile of this workgroup's group of WPB tiles.
-__device__ __forceinline__ int xcd_group(int b, int n_groups, bool& valid) {
-  const int per = (n_groups + 7) >> 3;
-  const int xcd = b & 7, k = b >> 3;
-  const int full = n_groups - (per - 1) * 8;  // XCDs that own `per` groups
-  valid = xcd < full ? k < per : k < per - 1;
-  return xcd < full ? xcd * per + k : full * per + (xcd - full) * (per - 1) + k;
-}
-
 // Tile of wave `wv` of workgroup `b` (nw waves each): with an order array,
 // slot b*nw + wv takes order[slot] (workgroups start in slot order as CUs
-// free up); otherwise the XCD-aware natural order above. -1: no tile.
+// free up); otherwise the XCD-aware natural order (xcd_group). -1: no tile.
 __device__ __forceinline__ int wave_tile(const int32_t* order, int n_tiles, int b, int wv, int nw = WPB) {
   if (order) {
     const int slot = b * nw + wv;
@@ -103,8 +92,6 @@ __device__ __forceinline__ int wave_tile(const int32_t* order, int n_tiles, int 
   return valid && t < n_tiles ? t : -1;
 }
 
-// Workgroups for n work items at nw waves each: a whole number of XCD rounds.
-static int xcd_grid(int n, int nw) { return (((n + nw - 1) / nw + 7) / 8) * 8; }
 
 struct Ray {
   f3 o, d;

@@ -24,7 +24,11 @@
 namespace pf {
 
 constexpr int TWAVE = 64;
-constexpr int TWPB = 4;
+// One wave per workgroup: a workgroup's wave slots and LDS stay taken until
+// its last wave ends, and the walks of neighbouring tiles differ in length
+// (on the 1M-site fisheye views four-wave workgroups ran the forward in
+// 4.9 ms, one-wave ones in 3.4 ms)
+constexpr int TWPB = 1;
 constexpr int SENTINEL = 0x7FFFFFFF;
 
 struct TraceParams {
@@ -53,6 +57,8 @@ struct TraceParams {
   int32_t* cells;
   int K;
   int32_t* n_cells;
+  int32_t* tile_steps;
+  const int32_t* tile_order;
   // backward inputs / outputs
   const float* s_color;
   const float* s_normal;
@@ -237,6 +243,12 @@ __device__ __forceinline__ bool trace_lane(const TraceParams& p, int tile, int l
 // (slot = the wave's counter in LDS + the lane's rank among the lanes that
 // ended in the same step, from a ballot), so lanes stay busy until the group
 // runs out of rays instead of idling until the tile's longest ray ends.
+//
+// Without compaction a wave walks one tile, tiles in index order (workgroups
+// dealt round-robin to the XCDs, which spreads the costly tiles of the view's
+// centre over all eight; contiguous runs per XCD measured 20 % slower), and
+// the wave's longest walk is written to tile_steps[tile] when asked (the
+// backward's longest-first order).
 template <bool COMPACT, bool HALF>
 __global__ void __launch_bounds__(TWAVE* TWPB) trace_fwd_kernel(TraceParams p, int group_tiles) {
   __shared__ int s_next[TWPB];
@@ -245,8 +257,10 @@ __global__ void __launch_bounds__(TWAVE* TWPB) trace_fwd_kernel(TraceParams p, i
   if (group * group_tiles >= p.n_tiles) return;
   const int n_slots = group_tiles * TWAVE;
   int slot = lane;
-  if (COMPACT) {
-    if (lane == 0) s_next[wv] = TWAVE;
+  // the tile's first ray (lane 0) always lies in the image, so lane 0 reaches
+  // the end of the kernel
+  if (COMPACT || p.tile_steps) {
+    if (lane == 0) s_next[wv] = COMPACT ? TWAVE : 0;
     wave_sync();
   }
   int i, j;
@@ -386,6 +400,12 @@ __global__ void __launch_bounds__(TWAVE* TWPB) trace_fwd_kernel(TraceParams p, i
   if (p.q_step) p.q_step[px] = qstep;
   if (p.q_sigma) p.q_sigma[px] = qsig;
   if (p.n_cells) p.n_cells[px] = step;
+  if (!COMPACT && p.tile_steps) {
+    __hip_atomic_fetch_max((__attribute__((address_space(3))) int*)&s_next[wv], step, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WAVEFRONT);
+    wave_sync();
+    if (lane == 0) p.tile_steps[group] = s_next[wv];
+  }
 }
 
 // Gradient cache of a wave. The 64 rays of a wave walk mostly the same
@@ -434,8 +454,10 @@ template <bool HALF>
 __global__ void __launch_bounds__(TWAVE* TWPB) __attribute__((amdgpu_waves_per_eu(3)))
 trace_bwd_kernel(TraceParams p) {
   __shared__ GradCache s_gc[TWPB];
-  const int tile = blockIdx.x * TWPB + (threadIdx.x >> 6);
-  if (tile >= p.n_tiles) return;
+  // slot b * TWPB + wave takes tile tile_order[slot], or tile slot
+  const int slot = blockIdx.x * TWPB + (threadIdx.x >> 6);
+  if (slot >= p.n_tiles) return;
+  const int tile = p.tile_order ? p.tile_order[slot] : slot;
   const int lane = threadIdx.x & 63;
   GradCache& gc = s_gc[threadIdx.x >> 6];
   if (lane < GC_SLOTS) gc.tag[lane] = -1;
@@ -841,6 +863,36 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Tiles by descending walk length: a counting sort over ORDER_BUCKETS
+// lengths (longer walks share the last bucket) in one workgroup.
+constexpr int ORDER_BUCKETS = 1024;
+
+__global__ void __launch_bounds__(ORDER_BUCKETS) trace_tile_order_kernel(const int32_t* __restrict__ steps, int n_tiles,
+                                                                         int32_t* __restrict__ order) {
+  __shared__ int s_cnt[ORDER_BUCKETS];
+  __shared__ int s_scan[ORDER_BUCKETS];
+  const int t = threadIdx.x;
+  s_cnt[t] = 0;
+  __syncthreads();
+  // bucket 0 holds the longest walks
+  auto bucket = [&](int k) { return ORDER_BUCKETS - 1 - min(max(steps[k], 0), ORDER_BUCKETS - 1); };
+  for (int k = t; k < n_tiles; k += ORDER_BUCKETS) atomicAdd(&s_cnt[bucket(k)], 1);
+  __syncthreads();
+  // inclusive scan of the counts (Hillis-Steele over the workgroup)
+  int v = s_cnt[t];
+  s_scan[t] = v;
+  __syncthreads();
+  for (int off = 1; off < ORDER_BUCKETS; off <<= 1) {
+    const int add = t >= off ? s_scan[t - off] : 0;
+    __syncthreads();
+    s_scan[t] += add;
+    __syncthreads();
+  }
+  s_cnt[t] = s_scan[t] - v;  // first slot of bucket t
+  __syncthreads();
+  for (int k = t; k < n_tiles; k += ORDER_BUCKETS) order[atomicAdd(&s_cnt[bucket(k)], 1)] = k;
+}
+
 static int check_trace(const pf_trace_args* a, const char* who) {
   if (!a) return set_error(PF_ERR_ARG, "%s: null args", who);
   const pf_scene& s = a->scene;
@@ -875,6 +927,9 @@ int pf_trace_fwd(const pf_trace_args* a, void* stream) {
   p.cells = a->cells;
   p.K = a->max_cells;
   p.n_cells = a->n_cells;
+  p.tile_steps = a->tile_steps;
+  if (a->tile_steps && a->compact > 0)
+    return set_error(PF_ERR_ARG, "pf_trace_fwd: tile_steps needs one wave per tile (compact = 0)");
   const bool half = p.adj_diff != nullptr;
   if (a->compact > 0) {
     const int group_tiles = a->compact;
@@ -891,6 +946,15 @@ int pf_trace_fwd(const pf_trace_args* a, void* stream) {
     else
       trace_fwd_kernel<false, false><<<grid, TWAVE * TWPB, 0, as_stream(stream)>>>(p, 1);
   }
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+int pf_trace_tile_order(const int32_t* tile_steps, int32_t n_tiles, int32_t* tile_order, void* stream) {
+  if (n_tiles < 0) return set_error(PF_ERR_ARG, "pf_trace_tile_order: n_tiles < 0");
+  if (n_tiles == 0) return PF_OK;
+  if (!tile_steps || !tile_order) return set_error(PF_ERR_ARG, "pf_trace_tile_order: a NULL array");
+  trace_tile_order_kernel<<<1, ORDER_BUCKETS, 0, as_stream(stream)>>>(tile_steps, n_tiles, tile_order);
   PF_HIP(hipGetLastError());
   return PF_OK;
 }
@@ -941,6 +1005,7 @@ int pf_trace_bwd(const pf_trace_bwd_args* b, void* stream) {
   p.g_tsites = b->grad_texel_sites;
   p.g_trgb = b->grad_texel_rgb;
   p.g_theight = b->grad_texel_height;
+  p.tile_order = b->tile_order;
   const dim3 grid((p.n_tiles + TWPB - 1) / TWPB);
   if (p.adj_diff)
     trace_bwd_kernel<true><<<grid, TWAVE * TWPB, 0, as_stream(stream)>>>(p);

@@ -61,6 +61,11 @@ class RayTracer:
         # latency-bound walk: 5.75 ms per view without, 18.4 ms with groups
         # of 4 tiles, 6.27 ms for the refill bookkeeping alone (groups of 1).
         self.compact = 0
+        # The backward takes its tiles longest walk first (pf_trace_tile_order
+        # over the per-tile walk lengths the forward records), so the waves of
+        # the longest walks start early instead of running alone at the end.
+        # Needs compact = 0. The gradients' summation order follows the order.
+        self.longest_first = True
 
     def _launch(self, name, fn, *a):
         if self.event_log is None:
@@ -163,7 +168,13 @@ class RayTracer:
         q_sigma = torch.zeros((H, W), **f32)
         a.color, a.log_t, a.normal, a.depth = color.data_ptr(), log_t.data_ptr(), normal.data_ptr(), depth.data_ptr()
         a.q_step, a.q_sigma = q_step.data_ptr(), q_sigma.data_ptr()
+        tile_steps = None
+        if self.longest_first and self.compact == 0:
+            n_tiles = -(-H // TILE_WIDTH) * -(-W // TILE_WIDTH)
+            tile_steps = torch.empty(n_tiles, dtype=torch.int32, device=points.device)
+            a.tile_steps = tile_steps.data_ptr()
         self._launch("trace_fwd", self.lib.pf_trace_fwd, _ref(a), stream_handle())
+        ctx.tile_steps = tile_steps
         ctx.tracer = self
         ctx.camera = camera
         ctx.start = start_point_idx if keep is None else None
@@ -199,9 +210,14 @@ class RayTracer:
         b.grad_spheres, b.grad_nsigmas = gs.data_ptr(), gn.data_ptr()
         b.grad_texel_sites, b.grad_texel_rgb, b.grad_texel_height = gts.data_ptr(), gtr.data_ptr(), gth.data_ptr()
         st = stream_handle()
+        order = None
+        if ctx.tile_steps is not None:
+            order = torch.empty_like(ctx.tile_steps)
+            check(self.lib.pf_trace_tile_order(ctx.tile_steps.data_ptr(), ctx.tile_steps.numel(), order.data_ptr(), st))
+            b.tile_order = order.data_ptr()
         self._launch("trace_bwd", self.lib.pf_trace_bwd, _ref(b), st)
         for x in (gs, gn, gts, gtr, gth):
             check(self.lib.pf_scrub_nonfinite(_abi.ptr(x), x.numel(), st))
-        del ctx.tracer, ctx.camera
+        del ctx.tracer, ctx.camera, ctx.tile_steps
         return (None, None, None, gs[:, :3], gs[:, 3], gn[:, 3], gn[:, :3], gts, gtr, gth, None, None, None, None,
                 None)

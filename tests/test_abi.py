@@ -69,8 +69,8 @@ def test_struct_layout_matches_c():
               ctypes.sizeof(B), B.exact_gradients.offset, B.grad_texel_height.offset, B.entry_shades.offset,
               ctypes.sizeof(RG), RG.color.offset,
               ctypes.sizeof(V), V.bkgd_color.offset, V.intersections.offset,
-              ctypes.sizeof(TA), TA.max_steps.offset, TA.n_cells.offset,
-              ctypes.sizeof(TB), TB.grad_opacity.offset, TB.grad_texel_height.offset,
+              ctypes.sizeof(TA), TA.max_steps.offset, TA.n_cells.offset, TA.tile_steps.offset,
+              ctypes.sizeof(TB), TB.grad_opacity.offset, TB.grad_texel_height.offset, TB.tile_order.offset,
               ctypes.sizeof(MR), MR.eye.offset, MR.latents.offset, MR.site_count.offset,
               ctypes.sizeof(_abi.PfMlpGrads),
               ctypes.sizeof(SG), SG.count.offset, SG.seg_len.offset, SG.scratch.offset]
@@ -108,3 +108,14 @@ def test_adjacency_encode_rejects_null_edges_on_the_host():
     assert lib.pf_adj_encode(1, 10, 1, 1, -1, 1, None) == _abi.PF_ERR_ARG
     # no edges: nothing is read, nothing to reject
     assert lib.pf_adj_encode(1, 10, None, 1, 0, None, None) == _abi.PF_OK
+
+
+def test_trace_tile_order_argument_checks_run_on_the_host():
+    """pf_trace_tile_order: nothing to order is not an error, NULL arrays or a
+    negative count are (no device work either way)."""
+    if not os.path.exists(_abi.LIB_PATH):
+        pytest.skip("libpfhip.so not built")
+    lib = _abi.load()
+    assert lib.pf_trace_tile_order(None, 0, None, None) == _abi.PF_OK
+    assert lib.pf_trace_tile_order(None, 4, None, None) == _abi.PF_ERR_ARG
+    assert lib.pf_trace_tile_order(None, -1, None, None) == _abi.PF_ERR_ARG

@@ -188,3 +188,66 @@ def test_compacted_walk_equals_per_tile_walk(kind):
     for a, b in zip(res[4], res[0]):
         assert np.array_equal(a, b)
     assert res[4][-1].mean() > 3
+
+
+@pytest.mark.parametrize("kind", ["pinhole_ragged", "fisheye"])
+def test_tile_steps_and_longest_first_backward(kind):
+    """The forward's per-tile longest walk equals the maximum of the visited
+    cell counts over the tile's pixels (column-major 8x8 tiles, a ragged image
+    edge included); pf_trace_tile_order returns a permutation by descending
+    walk length; the backward taken in that order gives the gradients of the
+    index-order backward up to summation order, and the forward is unchanged."""
+    import torch
+    from powerfoam_amd import _abi
+    from powerfoam_amd.rasterize import _ref
+    s = big_scene(5)
+    if kind == "pinhole_ragged":
+        cam = look_at((0.1, 0.2, 2.2), (0.0, 0.0, 0.0), 70, 51, 60.0)
+    else:
+        cam = cameras("fisheye")
+    x = _tensors(s, "cuda")
+    st = start_cell(s, cam.eye.numpy().astype(np.float64))
+    gcam = _gpu_cam(cam)
+    H, W = cam.height, cam.width
+    th, tw = -(-H // 8), -(-W // 8)
+    tr = _tracer(8)
+    lib = _abi.load()
+    print(f"tile steps {kind}", flush=True)
+    t = tr._prep(x["points"], x["radii"], x["density"], x["normals"], x["texel_sites"], x["texel_rgb"],
+                 x["texel_height"], x["adj"], x["off"], None)
+    a, keep = tr._args(gcam, st, t, 1e-3, 0.5)
+    color = torch.zeros((H, W, 3), dtype=torch.float32, device="cuda")
+    n_cells = torch.zeros((H, W), dtype=torch.int32, device="cuda")
+    steps = torch.full((th * tw,), -7, dtype=torch.int32, device="cuda")
+    a.color, a.n_cells, a.tile_steps = color.data_ptr(), n_cells.data_ptr(), steps.data_ptr()
+    _abi.check(lib.pf_trace_fwd(_ref(a), _abi.stream_handle()))
+    n = n_cells.cpu().numpy()
+    pad = np.full((th * 8, tw * 8), -1, np.int64)
+    pad[:H, :W] = n
+    want = pad.reshape(th, 8, tw, 8).max(axis=(1, 3)).T.reshape(-1)  # tile t = row t % th, column t // th
+    got = steps.cpu().numpy()
+    assert np.array_equal(got, want)
+    order = torch.empty_like(steps)
+    _abi.check(lib.pf_trace_tile_order(steps.data_ptr(), steps.numel(), order.data_ptr(), _abi.stream_handle()))
+    o = order.cpu().numpy()
+    assert np.array_equal(np.sort(o), np.arange(th * tw))
+    assert np.all(np.diff(np.minimum(got[o], 1023)) <= 0)
+
+    # backward in that order vs index order
+    res = {}
+    for lf in (True, False):
+        print(f"backward {kind} longest_first={lf}", flush=True)
+        tr = _tracer(8)
+        tr.longest_first = lf
+        leaves = [x[k].detach().clone().requires_grad_(True) for k in ("points", "radii", "density", "normals",
+                                                                      "texel_sites", "texel_rgb", "texel_height")]
+        outs = tr.forward(gcam, st, *leaves, x["adj"], x["off"], None, 1e-3, 0.5)
+        rng = np.random.default_rng(11)
+        L = sum((o * torch.tensor(rng.normal(size=tuple(o.shape)), dtype=torch.float32, device="cuda")).sum()
+                for o in outs if o is not outs[2])
+        L.backward()
+        res[lf] = ([o.detach().cpu().numpy() for o in outs], [v.grad.cpu().numpy() for v in leaves])
+    for a_, b_ in zip(res[True][0], res[False][0]):
+        assert np.array_equal(a_, b_)
+    for ga, gb in zip(res[True][1], res[False][1]):
+        assert np.allclose(ga, gb, rtol=1e-4, atol=1e-5 * max(np.abs(gb).max(), 1e-30))
