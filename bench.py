@@ -20,9 +20,11 @@ launch from a committed rocprofv3 counter run of the same kernels and views,
 over a VALU peak the run measures itself with an FMA kernel). The default
 line also carries the learned-head record (3M-site scene, MLP head trained
 on the GPU, fwd+bwd frames with each head, render PSNR against the SH
-render, MFMA utilisation of the head's kernels). The CPU baseline is the
-FP32 CPU restatement of the same algorithm (the oracle, OpenMP) on a
-bounded sample of the same workload.
+render, MFMA utilisation of the head's kernels) and the ray-tracer record
+(the same 1M-site scene with its full power adjacency, 16 equidistant
+180-degree fisheye views of 1024x1024 rays, fwd+bwd; `trace_record`). The
+CPU baseline is the FP32 CPU restatement of the same algorithm (the oracle,
+OpenMP) on a bounded sample of the same workload.
 """
 
 import argparse
@@ -275,6 +277,12 @@ def mlp_head_record(args, dev, psnr_views=4):
 
 
 def run_trace(args, dev):
+    """`--workload trace`: the ray tracer workload below as the bench line."""
+    line = trace_record(args, dev, args.steps, args.warmup)
+    print(json.dumps(line), flush=True)
+
+
+def trace_record(args, dev, steps, warmup):
     """Ray tracer workload: the 1M-site GMM scene (seed 1) with its full power
     adjacency (the tracer walks power cells; alpha filtering would cut cells
     open), one equidistant 180-degree fisheye view of 1024x1024 rays per camera
@@ -328,12 +336,12 @@ def run_trace(args, dev):
                                                   offsets, None, 1e-3, 0.5)
         torch.mean((col - target) ** 2).backward()
 
-    for i in range(args.warmup):
+    for i in range(warmup):
         view(i)
     torch.cuda.synchronize()
     tracer.event_log = []
     t0 = time.perf_counter()
-    for i in range(args.steps):
+    for i in range(steps):
         view(i)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
@@ -349,7 +357,7 @@ def run_trace(args, dev):
     visits, uniq, uniq_faces = [], [], []
     K = 256
     with torch.no_grad():
-        for i in range(min(args.steps, len(cams))):
+        for i in range(min(steps, len(cams))):
             cam = cams[i]
             rgb = head.forward(p["texel_sites"], cam, p["sh"])
             cells, n_cells = tracer.trace_cells(cam, starts[i], p["points"], p["radii"], p["density"], p["normals"],
@@ -372,9 +380,9 @@ def run_trace(args, dev):
     abytes = dict(trace_fwd=fwd_bytes, trace_bwd=bwd_bytes)
     dom = max(kern_ms, key=kern_ms.get)
     achieved = abytes[dom] / (kern_ms[dom] * 1e-3) / 1e9
-    ms = elapsed * 1e3 / args.steps
-    line = dict(metric="ray tracer fwd+bwd, Mrays/s", value=rays * args.steps / elapsed / 1e6, unit="Mrays/s",
-                n_gpus=1, steps=args.steps, warmup=args.warmup, ms_per_step=ms, higher_is_better=True,
+    ms = elapsed * 1e3 / steps
+    line = dict(metric="ray tracer fwd+bwd, Mrays/s", value=rays * steps / elapsed / 1e6, unit="Mrays/s",
+                n_gpus=1, steps=steps, warmup=warmup, ms_per_step=ms, higher_is_better=True,
                 scaling="weak", vs_baseline=None, dtype="f32", data="synthetic (seeded scene generator)",
                 config=dict(workload=f"ray tracer fwd+bwd with SH-3 colour head, {args.sites}-site GMM scene, "
                                      f"{args.views} equidistant 180-degree fisheye views {size}x{size}",
@@ -386,7 +394,7 @@ def run_trace(args, dev):
                 roofline=dict(bound="hbm", kernel=dom, achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
                               frac=achieved / HBM_PEAK_GBS, traffic=None, algorithmic_bytes=abytes[dom]))
     log(f"{ms:.3f} ms/view, {line['value']:.1f} Mrays/s; kernels {kern_ms}; cells/ray {v_mean / rays:.1f}")
-    print(json.dumps(line), flush=True)
+    return line
 
 
 def run_cech(args, dev):
@@ -575,6 +583,9 @@ def main():
     ap.add_argument("--width", type=int, default=1297)
     ap.add_argument("--height", type=int, default=840)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-trace", action="store_true",
+                    help="skip the ray tracer record (1M-site fisheye views) of the default line")
+    ap.add_argument("--trace-steps", type=int, default=16)
     ap.add_argument("--no-mlp-head", action="store_true",
                     help="leave the learned-head record (3M-site scene) out of the default line")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
@@ -712,9 +723,12 @@ def main():
                             pairs_per_frame=avg["M"], pairs_per_frame_all_sites=avg["M_all"],
                             processed_pairs_per_frame=avg["pairs"]),
                 kernel_ms=kern_ms, roofline=roofline, cpu_baseline=cpu)
-    if not args.no_mlp_head:
-        del w, rast
+    del w, rast
+    torch.cuda.empty_cache()
+    if not args.no_trace:
+        line["trace"] = trace_record(args, dev, args.trace_steps, 2)
         torch.cuda.empty_cache()
+    if not args.no_mlp_head:
         line["mlp_head"], _ = mlp_head_record(args, dev)
     print(json.dumps(line), flush=True)
 

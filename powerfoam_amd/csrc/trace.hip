@@ -33,6 +33,7 @@ constexpr int SENTINEL = 0x7FFFFFFF;
 
 struct TraceParams {
   int W, H, tiles_h, n_tiles;
+  int n_sites;
   f3 eye, right, up;
   const float* ray_maps;
   int n_texels;
@@ -168,6 +169,30 @@ struct TexelEval {
   bool heights;
 };
 
+// The 3T texel-site floats of a site in 16-byte pieces (gfx950 reads
+// unaligned global data); a piece that would run past the array is read
+// float by float.
+__device__ __forceinline__ void load_texel_sites(const TraceParams& p, int site, int T, float* v) {
+  const int n = 3 * T;
+  const size_t base = (size_t)site * n, total = (size_t)p.n_sites * n;
+#pragma unroll
+  for (int j = 0; j < 3 * PF_MAX_TEXELS; j += 4) {
+    if (j < n) {
+      if (base + j + 4 <= total) {
+        float4 q;
+        __builtin_memcpy(&q, p.texel_sites + base + j, sizeof(q));
+        v[j] = q.x;
+        v[j + 1] = q.y;
+        v[j + 2] = q.z;
+        v[j + 3] = q.w;
+      } else {
+#pragma unroll
+        for (int k = 0; k < 4; k++) v[j + k] = j + k < n ? p.texel_sites[base + j + k] : 0.0f;
+      }
+    }
+  }
+}
+
 __device__ __forceinline__ TexelEval texel_eval(const TraceParams& p, int site, f3 c, float r, f3 pn, f3 o, f3 d,
                                                 float tn, float* w1) {
   TexelEval te;
@@ -177,15 +202,16 @@ __device__ __forceinline__ TexelEval texel_eval(const TraceParams& p, int site, 
   te.cn = dot(co, pn);
   te.ts0 = acc_div(te.cn, te.dpn);
   const float kexp = -10.0f * fast_rcp(r * r);
-  float4 ph[PF_MAX_TEXELS];
+  float4 ph[PF_MAX_TEXELS], cl[PF_MAX_TEXELS];
+  float tsv[3 * PF_MAX_TEXELS];
+  load_texel_sites(p, site, T, tsv);
   bool nz = false;
 #pragma unroll
   for (int i = 0; i < PF_MAX_TEXELS; i++) {
     if (i < T) {
-      const float* ts = p.texel_sites + ((size_t)site * T + i) * 3;
-      const float h = p.texel_rgbh[(size_t)site * T + i].w;
-      ph[i] = make_float4(ts[0] - c.x, ts[1] - c.y, ts[2] - c.z, h);
-      nz = nz || h != 0.0f;
+      cl[i] = p.texel_rgbh[(size_t)site * T + i];
+      ph[i] = make_float4(tsv[3 * i] - c.x, tsv[3 * i + 1] - c.y, tsv[3 * i + 2] - c.z, cl[i].w);
+      nz = nz || cl[i].w != 0.0f;
     }
   }
   te.heights = nz;
@@ -217,11 +243,10 @@ __device__ __forceinline__ TexelEval texel_eval(const TraceParams& p, int site, 
 #pragma unroll
   for (int i = 0; i < PF_MAX_TEXELS; i++) {
     if (i < T) {
-      const float4 cl = p.texel_rgbh[(size_t)site * T + i];
       const f3 df{x1.x - ph[i].x, x1.y - ph[i].y, x1.z - ph[i].z};
       const float w = fast_exp(dot(df, df) * kexp);
       if (w1) w1[i] = w;
-      rs = rs + f3{cl.x, cl.y, cl.z} * w;
+      rs = rs + f3{cl[i].x, cl[i].y, cl[i].z} * w;
       s1 += w;
     }
   }
@@ -760,6 +785,7 @@ static TraceParams make_params(const pf_trace_args* a) {
   p.up = f3{c.up[0], c.up[1], c.up[2]};
   p.ray_maps = c.ray_maps;
   p.n_texels = a->scene.n_texels;
+  p.n_sites = a->scene.n_sites;
   p.spheres = reinterpret_cast<const float4*>(a->scene.spheres);
   p.nsigmas = reinterpret_cast<const float4*>(a->scene.nsigmas);
   p.texel_sites = a->scene.texel_sites;
