@@ -240,6 +240,11 @@ typedef struct pf_trace_args {
    * (cells visited) of its rays, for pf_trace_tile_order. n_tiles =
    * ceil(H/8) * ceil(W/8), tile t = column-major (t % ceil(H/8) the tile row). */
   int32_t* tile_steps;
+  /* [E,4] or NULL, read when scene.adjacency_diff is NULL: the exact face
+   * planes from pf_trace_planes for the same spheres and adjacency. The walk
+   * then reads one contiguous record per face instead of the neighbour's id
+   * and sphere; same values, same results bit for bit. */
+  const float* face_planes;
 } pf_trace_args;
 
 /* Exact adjoint of pf_trace_fwd (no reference counterpart: the reference ray
@@ -516,6 +521,14 @@ int pf_raster_vis(const pf_raster_vis_args* args, void* stream);
 /* ---- ray tracer (RayTracer.benchmark_kernel raytrace.py:115-241) ------ */
 int pf_trace_fwd(const pf_trace_args* args, void* stream);
 int pf_trace_bwd(const pf_trace_bwd_args* args, void* stream);
+
+/* Exact face planes for pf_trace_args.face_planes (no reference
+ * counterpart: the reference walk forms them from the spheres per visit):
+ * face_planes[e] = (n, pm) with n = c_a - c_s and
+ * pm = (|c_a|^2 - r_a^2)/2 - (|c_s|^2 - r_s^2)/2 for the face e of site s
+ * towards a = adjacency[e], in FP32 as the walk forms them. face_planes holds
+ * E = adjacency_offsets[N] float4 (16-byte aligned). */
+int pf_trace_planes(const pf_scene* scene, float* face_planes, void* stream);
 
 /* Longest-walk-first dispatch order for pf_trace_bwd (no reference
  * counterpart: the reference tracer has no backward). tile_order [n_tiles]

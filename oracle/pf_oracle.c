@@ -268,6 +268,7 @@ int orc_abi_layout(int64_t* out, int cap) {
       (int64_t)offsetof(pf_raster_vis_args, intersections),
       (int64_t)sizeof(pf_trace_args), (int64_t)offsetof(pf_trace_args, max_steps),
       (int64_t)offsetof(pf_trace_args, n_cells), (int64_t)offsetof(pf_trace_args, tile_steps),
+      (int64_t)offsetof(pf_trace_args, face_planes),
       (int64_t)sizeof(pf_trace_bwd_args), (int64_t)offsetof(pf_trace_bwd_args, grad_opacity),
       (int64_t)offsetof(pf_trace_bwd_args, grad_texel_height), (int64_t)offsetof(pf_trace_bwd_args, tile_order),
       (int64_t)sizeof(pf_mlp_rows), (int64_t)offsetof(pf_mlp_rows, eye), (int64_t)offsetof(pf_mlp_rows, latents),

@@ -44,6 +44,7 @@ struct TraceParams {
   const int32_t* adjacency;
   const int32_t* adj_off;
   const uint16_t* adj_diff;
+  const float4* planes;
   int start_site;
   const int32_t* start_sites;
   float thr, depth_q;
@@ -99,18 +100,24 @@ __device__ __forceinline__ void walk_ray(const TraceParams& p, int i, int j, f3&
   d = f3{rd.x / len, rd.y / len, rd.z / len};
 }
 
-// The faces of a cell (centre c, radius r) four at a time. A group's
-// neighbour ids, and their spheres (exact geometry) or binary16
-// (c_adj - c, pm_adj - pm) records, are all requested before any is used, and
-// the ids of the next group are requested while this group's spheres load, so
-// a lane waits about one memory latency per group instead of two per face.
-// Slots past the cell's last face repeat that face: a repeated face moves no
-// strict comparison of the walk.
+// Where the walk takes a face's plane (n, pm) from: the neighbour's sphere
+// (FM_SPHERES, exact), the binary16 (c_adj - c, pm_adj - pm) record
+// (FM_HALF, the reference's benchmark encoding), or the exact plane of the
+// directed edge written beforehand by trace_planes_kernel (FM_PLANES: the
+// same FP32 values as FM_SPHERES, read as one contiguous record per face).
+constexpr int FM_SPHERES = 0, FM_HALF = 1, FM_PLANES = 2;
+
+// The faces of a cell (centre c, radius r) four at a time. A group's records
+// (neighbour ids and spheres, binary16 records or planes) are all requested
+// before any is used, and the ids of the next group are requested while this
+// group's spheres load, so a lane waits about one memory latency per group
+// instead of two per face. Slots past the cell's last face repeat that face:
+// a repeated face moves no strict comparison of the walk.
 struct FaceQuad {
-  int id[4];
+  int id[4];     // neighbour ids (FM_PLANES: face indices, the id read for the winner only)
   f3 n[4];
   float pm[4];
-  float4 nb[4];  // neighbour spheres (exact geometry only)
+  float4 nb[4];  // neighbour spheres (FM_SPHERES only)
 };
 
 __device__ __forceinline__ void quad_ids(const TraceParams& p, int e0, int a1, int id[4]) {
@@ -119,15 +126,26 @@ __device__ __forceinline__ void quad_ids(const TraceParams& p, int e0, int a1, i
 }
 
 // group e0 .. e0 + 3 (e0 < a1) from its ids; ids become those of the group at
-// e0 + 4 when there is one. HALF: faces from the binary16 records, else from
-// the neighbour spheres.
-template <bool HALF>
+// e0 + 4 when there is one (FM_PLANES reads no ids)
+template <int FM>
 __device__ __forceinline__ void quad_faces(const TraceParams& p, int e0, int a1, f3 c, float r, int ids[4],
                                            FaceQuad& q) {
 #pragma clang fp contract(off)
+  if (FM == FM_PLANES) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int e = min(e0 + k, a1 - 1);
+      const float4 pl = p.planes[e];
+      q.id[k] = e;
+      q.n[k] = f3{pl.x, pl.y, pl.z};
+      q.pm[k] = pl.w;
+      q.nb[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    return;
+  }
 #pragma unroll
   for (int k = 0; k < 4; k++) q.id[k] = ids[k];
-  if (HALF) {
+  if (FM == FM_HALF) {
     uint2 raw[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) raw[k] = *reinterpret_cast<const uint2*>(p.adj_diff + 4 * (int64_t)min(e0 + k, a1 - 1));
@@ -274,7 +292,7 @@ __device__ __forceinline__ bool trace_lane(const TraceParams& p, int tile, int l
 // centre over all eight; contiguous runs per XCD measured 20 % slower), and
 // the wave's longest walk is written to tile_steps[tile] when asked (the
 // backward's longest-first order).
-template <bool COMPACT, bool HALF>
+template <bool COMPACT, int FM>
 __global__ void __launch_bounds__(TWAVE* TWPB) trace_fwd_kernel(TraceParams p, int group_tiles) {
   __shared__ int s_next[TWPB];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -371,10 +389,10 @@ __global__ void __launch_bounds__(TWAVE* TWPB) trace_fwd_kernel(TraceParams p, i
     int next = SENTINEL;
     float pt_far = 1e10f;
     int ids[4];
-    if (a0 < a1) quad_ids(p, a0, a1, ids);
+    if (FM != FM_PLANES && a0 < a1) quad_ids(p, a0, a1, ids);
     for (int e0 = a0; e0 < a1; e0 += 4) {
       FaceQuad q;
-      quad_faces<HALF>(p, e0, a1, c, r, ids, q);
+      quad_faces<FM>(p, e0, a1, c, r, ids, q);
 #pragma unroll
       for (int k = 0; k < 4; k++) {
         float dp;
@@ -388,7 +406,8 @@ __global__ void __launch_bounds__(TWAVE* TWPB) trace_fwd_kernel(TraceParams p, i
         tn = (dp < 0.0f && t > tn) ? t : tn;
       }
     }
-    carried = !HALF;
+    if (FM == FM_PLANES && next != SENTINEL) next = p.adjacency[next];
+    carried = FM == FM_SPHERES;
     const float sigma = ns.w;
     if (hit && !(tn > tf) && !(sigma < 1e-3f)) {
       const f3 pn{ns.x, ns.y, ns.z};
@@ -475,7 +494,7 @@ __device__ __forceinline__ void gc_flush(const TraceParams& p, const GradCache& 
   }
 }
 
-template <bool HALF>
+template <int FM>
 __global__ void __launch_bounds__(TWAVE* TWPB) __attribute__((amdgpu_waves_per_eu(3)))
 trace_bwd_kernel(TraceParams p) {
   __shared__ GradCache s_gc[TWPB];
@@ -529,10 +548,10 @@ trace_bwd_kernel(TraceParams p) {
     float pt_far = 1e10f;
     const int n_adj = a1 - a0;
     int ids[4];
-    if (a0 < a1) quad_ids(p, a0, a1, ids);
+    if (FM != FM_PLANES && a0 < a1) quad_ids(p, a0, a1, ids);
     for (int e0 = a0; e0 < a1; e0 += 4) {
       FaceQuad q;
-      quad_faces<HALF>(p, e0, a1, c, r, ids, q);
+      quad_faces<FM>(p, e0, a1, c, r, ids, q);
 #pragma unroll
       for (int k = 0; k < 4; k++) {
         float dp;
@@ -552,7 +571,8 @@ trace_bwd_kernel(TraceParams p) {
         }
       }
     }
-    carried = !HALF;
+    if (FM == FM_PLANES && next != SENTINEL) next = p.adjacency[next];
+    carried = FM == FM_SPHERES;
     const float sigma = ns.w;
     const f3 pn{ns.x, ns.y, ns.z};
     const f3 co = -oc;
@@ -793,6 +813,7 @@ static TraceParams make_params(const pf_trace_args* a) {
   p.adjacency = a->scene.adjacency;
   p.adj_off = a->scene.adjacency_offsets;
   p.adj_diff = a->scene.adjacency_diff;
+  p.planes = a->scene.adjacency_diff ? nullptr : reinterpret_cast<const float4*>(a->face_planes);
   p.start_site = a->start_site;
   p.start_sites = a->start_sites;
   p.thr = a->transmittance_threshold;
@@ -889,6 +910,30 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Exact plane of every directed face: for the face between site s (centre
+// c, radius r) and neighbour a = adjacency[e], n = a - c and
+// pm = (|a|^2 - r_a^2) / 2 - (|c|^2 - r^2) / 2, formed exactly as the walk
+// forms them from the spheres (FP32, no contraction, same operation order),
+// so a walk that reads them makes the same choices bit for bit.
+__global__ void __launch_bounds__(256) trace_planes_kernel(const float4* __restrict__ spheres,
+                                                           const int32_t* __restrict__ adjacency,
+                                                           const int32_t* __restrict__ adj_off, int n_sites,
+                                                           float4* __restrict__ planes) {
+#pragma clang fp contract(off)
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  if (s >= n_sites) return;
+  const float4 cs = spheres[s];
+  const f3 c{cs.x, cs.y, cs.z};
+  const float r = cs.w;
+  const int e1 = adj_off[s + 1];
+  for (int e = adj_off[s]; e < e1; e++) {
+    const float4 a = spheres[adjacency[e]];
+    planes[e] = make_float4(a.x - c.x, a.y - c.y, a.z - c.z,
+                            0.5f * ((a.x * a.x + a.y * a.y + a.z * a.z) - a.w * a.w) -
+                                0.5f * ((c.x * c.x + c.y * c.y + c.z * c.z) - r * r));
+  }
+}
+
 // Tiles by descending walk length: a counting sort over ORDER_BUCKETS
 // lengths (longer walks share the last bucket) in one workgroup.
 constexpr int ORDER_BUCKETS = 1024;
@@ -956,22 +1001,40 @@ int pf_trace_fwd(const pf_trace_args* a, void* stream) {
   p.tile_steps = a->tile_steps;
   if (a->tile_steps && a->compact > 0)
     return set_error(PF_ERR_ARG, "pf_trace_fwd: tile_steps needs one wave per tile (compact = 0)");
-  const bool half = p.adj_diff != nullptr;
+  const int fm = p.adj_diff ? FM_HALF : (p.planes ? FM_PLANES : FM_SPHERES);
+  hipStream_t hs = as_stream(stream);
   if (a->compact > 0) {
     const int group_tiles = a->compact;
     const int groups = (p.n_tiles + group_tiles - 1) / group_tiles;
     const dim3 grid((groups + TWPB - 1) / TWPB);
-    if (half)
-      trace_fwd_kernel<true, true><<<grid, TWAVE * TWPB, 0, as_stream(stream)>>>(p, group_tiles);
+    if (fm == FM_HALF)
+      trace_fwd_kernel<true, FM_HALF><<<grid, TWAVE * TWPB, 0, hs>>>(p, group_tiles);
+    else if (fm == FM_PLANES)
+      trace_fwd_kernel<true, FM_PLANES><<<grid, TWAVE * TWPB, 0, hs>>>(p, group_tiles);
     else
-      trace_fwd_kernel<true, false><<<grid, TWAVE * TWPB, 0, as_stream(stream)>>>(p, group_tiles);
+      trace_fwd_kernel<true, FM_SPHERES><<<grid, TWAVE * TWPB, 0, hs>>>(p, group_tiles);
   } else {
     const dim3 grid((p.n_tiles + TWPB - 1) / TWPB);
-    if (half)
-      trace_fwd_kernel<false, true><<<grid, TWAVE * TWPB, 0, as_stream(stream)>>>(p, 1);
+    if (fm == FM_HALF)
+      trace_fwd_kernel<false, FM_HALF><<<grid, TWAVE * TWPB, 0, hs>>>(p, 1);
+    else if (fm == FM_PLANES)
+      trace_fwd_kernel<false, FM_PLANES><<<grid, TWAVE * TWPB, 0, hs>>>(p, 1);
     else
-      trace_fwd_kernel<false, false><<<grid, TWAVE * TWPB, 0, as_stream(stream)>>>(p, 1);
+      trace_fwd_kernel<false, FM_SPHERES><<<grid, TWAVE * TWPB, 0, hs>>>(p, 1);
   }
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+int pf_trace_planes(const pf_scene* scene, float* face_planes, void* stream) {
+  if (!scene) return set_error(PF_ERR_ARG, "pf_trace_planes: null scene");
+  if (scene->n_sites < 0) return set_error(PF_ERR_ARG, "pf_trace_planes: n_sites < 0");
+  if (scene->n_sites == 0) return PF_OK;
+  if (!scene->spheres || !scene->adjacency_offsets || !scene->adjacency || !face_planes)
+    return set_error(PF_ERR_ARG, "pf_trace_planes: needs spheres, the adjacency and the output");
+  trace_planes_kernel<<<(scene->n_sites + 255) / 256, 256, 0, as_stream(stream)>>>(
+      reinterpret_cast<const float4*>(scene->spheres), scene->adjacency, scene->adjacency_offsets, scene->n_sites,
+      reinterpret_cast<float4*>(face_planes));
   PF_HIP(hipGetLastError());
   return PF_OK;
 }
@@ -1033,10 +1096,13 @@ int pf_trace_bwd(const pf_trace_bwd_args* b, void* stream) {
   p.g_theight = b->grad_texel_height;
   p.tile_order = b->tile_order;
   const dim3 grid((p.n_tiles + TWPB - 1) / TWPB);
+  hipStream_t hs = as_stream(stream);
   if (p.adj_diff)
-    trace_bwd_kernel<true><<<grid, TWAVE * TWPB, 0, as_stream(stream)>>>(p);
+    trace_bwd_kernel<FM_HALF><<<grid, TWAVE * TWPB, 0, hs>>>(p);
+  else if (p.planes)
+    trace_bwd_kernel<FM_PLANES><<<grid, TWAVE * TWPB, 0, hs>>>(p);
   else
-    trace_bwd_kernel<false><<<grid, TWAVE * TWPB, 0, as_stream(stream)>>>(p);
+    trace_bwd_kernel<FM_SPHERES><<<grid, TWAVE * TWPB, 0, hs>>>(p);
   PF_HIP(hipGetLastError());
   return PF_OK;
 }

@@ -66,6 +66,11 @@ class RayTracer:
         # the longest walks start early instead of running alone at the end.
         # Needs compact = 0. The gradients' summation order follows the order.
         self.longest_first = True
+        # With exact geometry (no adjacency_diff) the walk reads each face's
+        # plane from one record written per call (pf_trace_planes: the same
+        # FP32 values it would form from the neighbour's sphere) instead of
+        # the neighbour's id and sphere. Results are identical either way.
+        self.face_planes = True
 
     def _launch(self, name, fn, *a):
         if self.event_log is None:
@@ -90,10 +95,16 @@ class RayTracer:
         diff = adjacency_diff.to(torch.float16).contiguous() if adjacency_diff is not None else None
         if texel_sites.shape[1] != self.num_texel_sites:
             raise ValueError("texel_sites does not match args.num_texel_sites")
-        return all_spheres, all_nsigmas, all_texel_sites, all_texel_rgbh, adjacency, adjacency_offsets, diff
+        planes = None
+        if diff is None and self.face_planes and adjacency.numel() > 0:
+            planes = torch.empty((adjacency.numel(), 4), dtype=torch.float32, device=all_spheres.device)
+            sc = _scene_struct(all_spheres, all_nsigmas, all_texel_sites, all_texel_rgbh, adjacency, adjacency_offsets,
+                               None)
+            check(self.lib.pf_trace_planes(_ref(sc), planes.data_ptr(), stream_handle()))
+        return all_spheres, all_nsigmas, all_texel_sites, all_texel_rgbh, adjacency, adjacency_offsets, diff, planes
 
     def _args(self, camera, start_point_idx, tensors, threshold, depth_quantile):
-        all_spheres, all_nsigmas, all_texel_sites, all_texel_rgbh, adjacency, adjacency_offsets, diff = tensors
+        all_spheres, all_nsigmas, all_texel_sites, all_texel_rgbh, adjacency, adjacency_offsets, diff, planes = tensors
         a = _abi.PfTraceArgs()
         a.cam = camera.to_pf()
         a.scene = _scene_struct(all_spheres, all_nsigmas, all_texel_sites, all_texel_rgbh, adjacency,
@@ -111,6 +122,7 @@ class RayTracer:
         a.depth_quantile = float(depth_quantile)
         a.max_steps = int(self.max_steps)
         a.compact = int(self.compact)
+        a.face_planes = planes.data_ptr() if planes is not None else None
         return a, keep
 
     def benchmark(self, camera, start_point_idx, points, radii, density, normals, texel_sites, texel_rgb,
@@ -179,17 +191,23 @@ class RayTracer:
         ctx.camera = camera
         ctx.start = start_point_idx if keep is None else None
         ctx.params = (transmittance_threshold, depth_quantile)
-        ctx.save_for_backward(*t[:6], *(x for x in (t[6],) if x is not None), color, normal, log_t, q_step, q_sigma,
+        ctx.save_for_backward(*t[:6], *(x for x in t[6:8] if x is not None), color, normal, log_t, q_step, q_sigma,
                               *(x for x in (keep,) if x is not None))
         ctx.has_diff = t[6] is not None
+        ctx.has_planes = t[7] is not None
         ctx.has_starts = keep is not None
         opacity = 1.0 - torch.exp(log_t)
         return color, opacity, depth, normal
 
     def _backward(self, ctx, g_color, g_opacity, g_depth, g_normal):
         saved = list(ctx.saved_tensors)
-        tensors = saved[:6] + ([saved[6]] if ctx.has_diff else [None])
-        rest = saved[7 if ctx.has_diff else 6:]
+        k = 6
+        diff = saved[k] if ctx.has_diff else None
+        k += ctx.has_diff
+        planes = saved[k] if ctx.has_planes else None
+        k += ctx.has_planes
+        tensors = saved[:6] + [diff, planes]
+        rest = saved[k:]
         color, normal, log_t, q_step, q_sigma = rest[:5]
         starts = rest[5] if ctx.has_starts else ctx.start
         camera = ctx.camera

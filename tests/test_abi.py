@@ -69,7 +69,7 @@ def test_struct_layout_matches_c():
               ctypes.sizeof(B), B.exact_gradients.offset, B.grad_texel_height.offset, B.entry_shades.offset,
               ctypes.sizeof(RG), RG.color.offset,
               ctypes.sizeof(V), V.bkgd_color.offset, V.intersections.offset,
-              ctypes.sizeof(TA), TA.max_steps.offset, TA.n_cells.offset, TA.tile_steps.offset,
+              ctypes.sizeof(TA), TA.max_steps.offset, TA.n_cells.offset, TA.tile_steps.offset, TA.face_planes.offset,
               ctypes.sizeof(TB), TB.grad_opacity.offset, TB.grad_texel_height.offset, TB.tile_order.offset,
               ctypes.sizeof(MR), MR.eye.offset, MR.latents.offset, MR.site_count.offset,
               ctypes.sizeof(_abi.PfMlpGrads),
@@ -119,3 +119,17 @@ def test_trace_tile_order_argument_checks_run_on_the_host():
     assert lib.pf_trace_tile_order(None, 0, None, None) == _abi.PF_OK
     assert lib.pf_trace_tile_order(None, 4, None, None) == _abi.PF_ERR_ARG
     assert lib.pf_trace_tile_order(None, -1, None, None) == _abi.PF_ERR_ARG
+
+
+def test_trace_planes_argument_checks_run_on_the_host():
+    """pf_trace_planes: no sites is not an error; a missing scene array or
+    output is, before any device work."""
+    if not os.path.exists(_abi.LIB_PATH):
+        pytest.skip("libpfhip.so not built")
+    lib = _abi.load()
+    sc = _abi.PfScene()
+    sc.n_sites = 0
+    assert lib.pf_trace_planes(ctypes.byref(sc), None, None) == _abi.PF_OK
+    sc.n_sites = 10
+    assert lib.pf_trace_planes(ctypes.byref(sc), None, None) == _abi.PF_ERR_ARG
+    assert lib.pf_trace_planes(None, None, None) == _abi.PF_ERR_ARG

@@ -251,3 +251,58 @@ def test_tile_steps_and_longest_first_backward(kind):
         assert np.array_equal(a_, b_)
     for ga, gb in zip(res[True][1], res[False][1]):
         assert np.allclose(ga, gb, rtol=1e-4, atol=1e-5 * max(np.abs(gb).max(), 1e-30))
+
+
+@pytest.mark.parametrize("kind", ["pinhole", "fisheye"])
+def test_face_planes_walk_equals_sphere_walk(kind):
+    """pf_trace_planes writes, per directed face, n = c_a - c_s and
+    pm = (|c_a|^2 - r_a^2)/2 - (|c_s|^2 - r_s^2)/2 bit-exactly as float32
+    numpy forms them in the same order; a walk that reads those planes
+    (RayTracer.face_planes) visits the same cells, bit for bit, as the walk
+    that forms them from the spheres, and returns the same outputs and
+    gradients to float rounding (the shading outside the walk may contract
+    multiply-adds differently in the two kernel variants)."""
+    import torch
+    from powerfoam_amd import _abi
+    from powerfoam_amd.rasterize import _ref, _scene_struct
+    s = big_scene(6)
+    cam = cameras(kind)
+    x = _tensors(s, "cuda")
+    st = start_cell(s, cam.eye.numpy().astype(np.float64))
+    gcam = _gpu_cam(cam)
+    tr = _tracer(8)
+    t = tr._prep(x["points"], x["radii"], x["density"], x["normals"], x["texel_sites"], x["texel_rgb"],
+                 x["texel_height"], x["adj"], x["off"], None)
+    planes = t[7].cpu().numpy()
+    sp = np.concatenate([s["points"], s["radii"][:, None]], 1).astype(np.float32)
+    owner = np.repeat(np.arange(len(sp)), np.diff(s["off"]))
+    c, a = sp[owner], sp[s["adj"]]
+    half = np.float32(0.5)
+    pm = half * ((a[:, 0] * a[:, 0] + a[:, 1] * a[:, 1] + a[:, 2] * a[:, 2]) - a[:, 3] * a[:, 3]) - \
+        half * ((c[:, 0] * c[:, 0] + c[:, 1] * c[:, 1] + c[:, 2] * c[:, 2]) - c[:, 3] * c[:, 3])
+    want = np.concatenate([a[:, :3] - c[:, :3], pm[:, None]], 1)
+    assert np.array_equal(planes.view(np.uint32), want.astype(np.float32).view(np.uint32))
+
+    res = {}
+    for fp in (True, False):
+        print(f"walk {kind} face_planes={fp}", flush=True)
+        tr = _tracer(8)
+        tr.face_planes = fp
+        leaves = [x[k].detach().clone().requires_grad_(True) for k in ("points", "radii", "density", "normals",
+                                                                      "texel_sites", "texel_rgb", "texel_height")]
+        outs = tr.forward(gcam, st, *leaves, x["adj"], x["off"], None, 1e-3, 0.5)
+        rng = np.random.default_rng(13)
+        L = sum((o * torch.tensor(rng.normal(size=tuple(o.shape)), dtype=torch.float32, device="cuda")).sum()
+                for o in (outs[0], outs[1], outs[3]))
+        L.backward()
+        cells, n = tr.trace_cells(gcam, st, *[v.detach() for v in leaves], x["adj"], x["off"], None, 1e-3,
+                                  max_cells=96)
+        res[fp] = ([o.detach().cpu().numpy() for o in outs] + [cells.cpu().numpy(), n.cpu().numpy()],
+                   [v.grad.cpu().numpy() for v in leaves])
+    for a_, b_ in zip(res[True][0][4:], res[False][0][4:]):  # visited cells and their count
+        assert np.array_equal(a_, b_)
+    for a_, b_ in zip(res[True][0][:4], res[False][0][:4]):  # colour, opacity, depth, normals
+        assert np.allclose(a_, b_, rtol=1e-5, atol=1e-6)
+    for ga, gb in zip(res[True][1], res[False][1]):
+        assert np.allclose(ga, gb, rtol=1e-4, atol=1e-5 * max(np.abs(gb).max(), 1e-30))
+    assert res[True][0][-1].mean() > 3
