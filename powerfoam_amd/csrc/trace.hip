@@ -498,6 +498,9 @@ template <int FM>
 __global__ void __launch_bounds__(TWAVE* TWPB) __attribute__((amdgpu_waves_per_eu(3)))
 trace_bwd_kernel(TraceParams p) {
   __shared__ GradCache s_gc[TWPB];
+  // per lane, the colour-stage texel weights of its visit ([texel][lane])
+  __shared__ float s_wn_all[TWPB][PF_MAX_TEXELS * TWAVE];
+  float* s_wn = s_wn_all[threadIdx.x >> 6];
   // slot b * TWPB + wave takes tile tile_order[slot], or tile slot
   const int slot = blockIdx.x * TWPB + (threadIdx.x >> 6);
   if (slot >= p.n_tiles) return;
@@ -669,8 +672,10 @@ trace_bwd_kernel(TraceParams p) {
           const float* ts = p.texel_sites + ((size_t)prim * T + k) * 3;
           const float4 cl = p.texel_rgbh[(size_t)prim * T + k];
           const f3 df{x1.x - (ts[0] - c.x), x1.y - (ts[1] - c.y), x1.z - (ts[2] - c.z)};
-          // the colour-stage weight, recomputed as texel_eval forms it
+          // the colour-stage weight, recomputed as texel_eval forms it, and
+          // kept for the height stage of a site whose heights are all zero
           const float wn = fast_exp(dot(df, df) * kexp) * inv_ws;
+          s_wn[kk * TWAVE + lane] = wn;
           acc(3 * k + 0, dLdcol.x * wn);
           acc(3 * k + 1, dLdcol.y * wn);
           acc(3 * k + 2, dLdcol.z * wn);
@@ -695,23 +700,27 @@ trace_bwd_kernel(TraceParams p) {
         const f3 x0 = d * tq0 - co;
         f3 dLdx0{0.0f, 0.0f, 0.0f};
         const float gdp0_in = dLdheight * te.height;
-        for (int kk = 0; kk < T; kk++) {
-          const int k = kk + rot < T ? kk + rot : kk + rot - T;
-          const float* ts = p.texel_sites + ((size_t)prim * T + k) * 3;
-          const float hk = p.texel_rgbh[(size_t)prim * T + k].w;
-          const f3 df{x0.x - (ts[0] - c.x), x0.y - (ts[1] - c.y), x0.z - (ts[2] - c.z)};
-          // with all heights zero x0 is the colour-stage point, and this is
-          // the colour-stage weight
-          const float w = fast_exp(dot(df, df) * kexp);
-          const float wn0 = w * inv_ws0;
-          acc(6 * T + k, wn0 * dLdheight);
-          // all heights zero: hk = 0 and gdp0_in = 0, so sg is zero
-          if (te.heights) {
+        if (te.heights) {
+          for (int kk = 0; kk < T; kk++) {
+            const int k = kk + rot < T ? kk + rot : kk + rot - T;
+            const float* ts = p.texel_sites + ((size_t)prim * T + k) * 3;
+            const float hk = p.texel_rgbh[(size_t)prim * T + k].w;
+            const f3 df{x0.x - (ts[0] - c.x), x0.y - (ts[1] - c.y), x0.z - (ts[2] - c.z)};
+            const float wn0 = fast_exp(dot(df, df) * kexp) * inv_ws0;
+            acc(6 * T + k, wn0 * dLdheight);
             const f3 sg = df * (kw * wn0 * (dLdheight * hk - gdp0_in));
             acc(3 * T + 3 * k + 0, sg.x);
             acc(3 * T + 3 * k + 1, sg.y);
             acc(3 * T + 3 * k + 2, sg.z);
             dLdx0 = dLdx0 - sg;
+          }
+        } else {
+          // all heights zero: x0 is the colour-stage point, ws0 = ws, the
+          // weights are the colour stage's, and hk = gdp0_in = 0 leaves no
+          // texel-site term
+          for (int kk = 0; kk < T; kk++) {
+            const int k = kk + rot < T ? kk + rot : kk + rot - T;
+            acc(6 * T + k, s_wn[kk * TWAVE + lane] * dLdheight);
           }
         }
         const float dLdtq0 = dot(dLdx0, d);
