@@ -211,27 +211,28 @@ __device__ __forceinline__ void load_texel_sites(const TraceParams& p, int site,
   }
 }
 
-__device__ __forceinline__ TexelEval texel_eval(const TraceParams& p, int site, f3 c, float r, f3 pn, f3 o, f3 d,
-                                                float tn, float* w1) {
+// A site's texel record in registers: site(i) = texel site i (from the 3T
+// site floats), rgbh(i) = its (rgb, height).
+struct RegRec {
+  const float* tsv;
+  const float4* cl;
+  __device__ __forceinline__ f3 site(int i) const { return f3{tsv[3 * i], tsv[3 * i + 1], tsv[3 * i + 2]}; }
+  __device__ __forceinline__ float4 rgbh(int i) const { return cl[i]; }
+};
+// The texel stage (plane_intersection_fwd_local) from a texel record.
+template <typename Rec>
+__device__ __forceinline__ TexelEval texel_eval_rec(int T, const Rec& rec, f3 c, float r, f3 pn, f3 o, f3 d, float tn,
+                                                    float* w1) {
   TexelEval te;
-  const int T = p.n_texels;
   const f3 co = c - o;
   te.dpn = dot(pn, d);
   te.cn = dot(co, pn);
   te.ts0 = acc_div(te.cn, te.dpn);
   const float kexp = -10.0f * fast_rcp(r * r);
-  float4 ph[PF_MAX_TEXELS], cl[PF_MAX_TEXELS];
-  float tsv[3 * PF_MAX_TEXELS];
-  load_texel_sites(p, site, T, tsv);
   bool nz = false;
 #pragma unroll
-  for (int i = 0; i < PF_MAX_TEXELS; i++) {
-    if (i < T) {
-      cl[i] = p.texel_rgbh[(size_t)site * T + i];
-      ph[i] = make_float4(tsv[3 * i] - c.x, tsv[3 * i + 1] - c.y, tsv[3 * i + 2] - c.z, cl[i].w);
-      nz = nz || cl[i].w != 0.0f;
-    }
-  }
+  for (int i = 0; i < PF_MAX_TEXELS; i++)
+    if (i < T) nz = nz || rec.rgbh(i).w != 0.0f;
   te.heights = nz;
   te.height = 0.0f;
   te.ws0 = 1.0f;
@@ -242,9 +243,10 @@ __device__ __forceinline__ TexelEval texel_eval(const TraceParams& p, int site, 
 #pragma unroll
     for (int i = 0; i < PF_MAX_TEXELS; i++) {
       if (i < T) {
-        const f3 df{x0.x - ph[i].x, x0.y - ph[i].y, x0.z - ph[i].z};
+        const f3 ts = rec.site(i);
+        const f3 df{x0.x - (ts.x - c.x), x0.y - (ts.y - c.y), x0.z - (ts.z - c.z)};
         const float w = fast_exp(dot(df, df) * kexp);
-        hsum += w * ph[i].w;
+        hsum += w * rec.rgbh(i).w;
         s0 += w;
       }
     }
@@ -261,10 +263,12 @@ __device__ __forceinline__ TexelEval texel_eval(const TraceParams& p, int site, 
 #pragma unroll
   for (int i = 0; i < PF_MAX_TEXELS; i++) {
     if (i < T) {
-      const f3 df{x1.x - ph[i].x, x1.y - ph[i].y, x1.z - ph[i].z};
+      const f3 ts = rec.site(i);
+      const f3 df{x1.x - (ts.x - c.x), x1.y - (ts.y - c.y), x1.z - (ts.z - c.z)};
       const float w = fast_exp(dot(df, df) * kexp);
       if (w1) w1[i] = w;
-      rs = rs + f3{cl[i].x, cl[i].y, cl[i].z} * w;
+      const float4 cl = rec.rgbh(i);
+      rs = rs + f3{cl.x, cl.y, cl.z} * w;
       s1 += w;
     }
   }
@@ -272,6 +276,18 @@ __device__ __forceinline__ TexelEval texel_eval(const TraceParams& p, int site, 
   if (!nz) te.ws0 = te.ws;
   te.col = rs * fast_rcp(te.ws);
   return te;
+}
+
+__device__ __forceinline__ TexelEval texel_eval(const TraceParams& p, int site, f3 c, float r, f3 pn, f3 o, f3 d,
+                                                float tn, float* w1) {
+  const int T = p.n_texels;
+  float4 cl[PF_MAX_TEXELS];
+  float tsv[3 * PF_MAX_TEXELS];
+  load_texel_sites(p, site, T, tsv);
+#pragma unroll
+  for (int i = 0; i < PF_MAX_TEXELS; i++)
+    if (i < T) cl[i] = p.texel_rgbh[(size_t)site * T + i];
+  return texel_eval_rec(T, RegRec{tsv, cl}, c, r, pn, o, d, tn, w1);
 }
 
 __device__ __forceinline__ bool trace_lane(const TraceParams& p, int tile, int lane, int& i, int& j) {
