@@ -31,6 +31,10 @@ constexpr int CECH_EDGE_LOG2 = 0;
 constexpr int CECH_COORD_BITS = 20;
 constexpr int CECH_COORD_MAX = (1 << CECH_COORD_BITS) - 1;
 constexpr int CECH_SCAN_TILE = 4096;  // entries per workgroup of the bucket scan
+// One wave per workgroup for the pair search: a site's search ranges from a
+// few to thousands of candidates, and a workgroup keeps its slots until its
+// slowest wave ends.
+constexpr int CECH_PAIRS_THREADS = 64;
 
 struct CechGrid {
   float lo[3];
@@ -329,7 +333,7 @@ __global__ void __launch_bounds__(256) cech_scatter_kernel(const float4* __restr
 // otherwise the ids go out as records, and entries beyond a segment are
 // counted into *dropped, never written.
 template <bool COUNT>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(CECH_PAIRS_THREADS)
     cech_pairs_kernel(int n, const CechGrid* __restrict__ gp, const int* __restrict__ level_count,
                       const int* __restrict__ bucket_start, const float4* __restrict__ ord_balls,
                       const int* __restrict__ ord_ids, const uint64_t* __restrict__ ord_keys,
@@ -499,7 +503,7 @@ int pf_cech_count(const float* spheres, int32_t n_sites, void* workspace, size_t
     cech_scan_add_kernel<<<tiles, 1024, 0, st>>>(w.bucket_start, nb1, w.part);
     cech_scatter_kernel<<<blocks, 256, 0, st>>>(sp, n_sites, w.grid, w.site_bucket, w.bucket_start, w.bucket_fill,
                                                 w.ord_balls, w.ord_ids, w.ord_keys);
-    cech_pairs_kernel<true><<<blocks, 256, 0, st>>>(n_sites, w.grid, w.acc + 8, w.bucket_start, w.ord_balls,
+    cech_pairs_kernel<true><<<cech_blocks(n_sites, CECH_PAIRS_THREADS), CECH_PAIRS_THREADS, 0, st>>>(n_sites, w.grid, w.acc + 8, w.bucket_start, w.ord_balls,
                                                     w.ord_ids, w.ord_keys, counts, w.own_deg, nullptr, nullptr,
                                                     nullptr, nullptr);
   }
@@ -517,7 +521,7 @@ int pf_cech_emit(const float* spheres, int32_t n_sites, void* workspace, size_t 
   const CechWorkspace w = cech_ws(workspace, n_sites);
   const unsigned blocks = cech_blocks(n_sites, 256);
   cech_fill_kernel<<<blocks, 256, 0, st>>>(w.fill, n_sites);
-  cech_pairs_kernel<false><<<blocks, 256, 0, st>>>(n_sites, w.grid, w.acc + 8, w.bucket_start, w.ord_balls, w.ord_ids,
+  cech_pairs_kernel<false><<<cech_blocks(n_sites, CECH_PAIRS_THREADS), CECH_PAIRS_THREADS, 0, st>>>(n_sites, w.grid, w.acc + 8, w.bucket_start, w.ord_balls, w.ord_ids,
                                                    w.ord_keys, nullptr, w.own_deg, offsets, w.fill, records, dropped);
   PF_HIP(hipGetLastError());
   return PF_OK;
