@@ -940,18 +940,37 @@ __global__ void __launch_bounds__(256)
 // pm = (|a|^2 - r_a^2) / 2 - (|c|^2 - r^2) / 2, formed exactly as the walk
 // forms them from the spheres (FP32, no contraction, same operation order),
 // so a walk that reads them makes the same choices bit for bit.
+// A wave takes 64 consecutive sites and deals their faces to its lanes in
+// order (coalesced id reads and plane writes); a lane finds the site of its
+// face by binary search over the wave's 65 offsets in LDS.
 __global__ void __launch_bounds__(256) trace_planes_kernel(const float4* __restrict__ spheres,
                                                            const int32_t* __restrict__ adjacency,
                                                            const int32_t* __restrict__ adj_off, int n_sites,
                                                            float4* __restrict__ planes) {
 #pragma clang fp contract(off)
-  const int s = blockIdx.x * 256 + threadIdx.x;
-  if (s >= n_sites) return;
-  const float4 cs = spheres[s];
-  const f3 c{cs.x, cs.y, cs.z};
-  const float r = cs.w;
-  const int e1 = adj_off[s + 1];
-  for (int e = adj_off[s]; e < e1; e++) {
+  __shared__ int s_off[4][WAVE + 1];
+  __shared__ float4 s_sph[4][WAVE];
+  const int wv = threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
+  const int s0 = (blockIdx.x * 4 + wv) * WAVE;
+  if (s0 >= n_sites) return;
+  const int ns = min(WAVE, n_sites - s0);
+  int* so = s_off[wv];
+  float4* sp = s_sph[wv];
+  if (lane < ns) {
+    sp[lane] = spheres[s0 + lane];
+    so[lane] = adj_off[s0 + lane];
+  }
+  if (lane == 0) so[ns] = adj_off[s0 + ns];  // ns may be 64: one past the last lane
+  wave_sync();
+  const int e0 = so[0], e1 = so[ns];
+  for (int e = e0 + lane; e < e1; e += WAVE) {
+    int l = 0;  // largest l with so[l] <= e: the site of face e
+#pragma unroll
+    for (int step = WAVE / 2; step >= 1; step >>= 1)
+      if (l + step < ns && so[l + step] <= e) l += step;
+    const float4 cs = sp[l];
+    const f3 c{cs.x, cs.y, cs.z};
+    const float r = cs.w;
     const float4 a = spheres[adjacency[e]];
     planes[e] = make_float4(a.x - c.x, a.y - c.y, a.z - c.z,
                             0.5f * ((a.x * a.x + a.y * a.y + a.z * a.z) - a.w * a.w) -
@@ -1057,7 +1076,7 @@ int pf_trace_planes(const pf_scene* scene, float* face_planes, void* stream) {
   if (scene->n_sites == 0) return PF_OK;
   if (!scene->spheres || !scene->adjacency_offsets || !scene->adjacency || !face_planes)
     return set_error(PF_ERR_ARG, "pf_trace_planes: needs spheres, the adjacency and the output");
-  trace_planes_kernel<<<(scene->n_sites + 255) / 256, 256, 0, as_stream(stream)>>>(
+  trace_planes_kernel<<<(scene->n_sites + 4 * WAVE - 1) / (4 * WAVE), 256, 0, as_stream(stream)>>>(
       reinterpret_cast<const float4*>(scene->spheres), scene->adjacency, scene->adjacency_offsets, scene->n_sites,
       reinterpret_cast<float4*>(face_planes));
   PF_HIP(hipGetLastError());
