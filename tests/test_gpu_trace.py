@@ -163,22 +163,24 @@ def test_trace_backward_matches_fp64_adjoint(kind):
 
 
 @pytest.mark.parametrize("kind", ["pinhole", "fisheye"])
-def test_compacted_walk_equals_per_tile_walk(kind):
+@pytest.mark.parametrize("encoded", [True, False], ids=["fp16faces", "planes"])
+def test_compacted_walk_equals_per_tile_walk(kind, encoded):
     """Ballot compaction (a wave walks 4 tiles' rays and refills ended lanes,
     skipping slots outside the image)
     changes which lane walks a ray, not the ray: every output and the visited
-    cells are bit-identical to one wave per tile."""
+    cells are bit-identical to one wave per tile, with the fp16 face records
+    and with the exact face planes."""
     from powerfoam_amd.rasterize import power_face_encoding
     s = big_scene(4)
     cam = cameras(kind)
     x = _tensors(s, "cuda")
-    diff = power_face_encoding(x["points"], x["radii"], x["adj"], x["off"])
+    diff = power_face_encoding(x["points"], x["radii"], x["adj"], x["off"]) if encoded else None
     st = start_cell(s, cam.eye.numpy().astype(np.float64))
     res = {}
     for compact in (4, 0):
         tr = _tracer(8)
         tr.compact = compact
-        print(f"walk {kind} compact={compact}", flush=True)
+        print(f"walk {kind} encoded={encoded} compact={compact}", flush=True)
         outs = tr.forward(_gpu_cam(cam), st, x["points"], x["radii"], x["density"], x["normals"], x["texel_sites"],
                           x["texel_rgb"], x["texel_height"], x["adj"], x["off"], diff, 1e-3, 0.5)
         cells, n = tr.trace_cells(_gpu_cam(cam), st, x["points"], x["radii"], x["density"], x["normals"],
