@@ -58,8 +58,9 @@ class RayTracer:
         # Off by default: on the 1M-site fisheye views a tile's lanes already
         # do 90 % useful steps (its 64 rays end within a few cells of each
         # other), while grouping tiles divides the waves in flight of a
-        # latency-bound walk: 5.75 ms per view without, 18.4 ms with groups
-        # of 4 tiles, 6.27 ms for the refill bookkeeping alone (groups of 1).
+        # latency-bound walk: forward 2.26 ms per view without, 3.92 ms with
+        # groups of 2 tiles, 6.36 ms with groups of 4. It also leaves the
+        # backward without the per-tile walk lengths of its longest-first order.
         self.compact = 0
         # The backward takes its tiles longest walk first (pf_trace_tile_order
         # over the per-tile walk lengths the forward records), so the waves of
