@@ -127,6 +127,9 @@ __device__ __forceinline__ void global_add(float* p, float v) {
 }
 // a load that p must serve from LDS (ds_read), for the same reason
 __device__ __forceinline__ int lds_load(const int* p) { return *(const __attribute__((address_space(3))) int*)p; }
+__device__ __forceinline__ float lds_load(const float* p) {
+  return *(const __attribute__((address_space(3))) float*)p;
+}
 
 // Uniform-broadcast of lane `l`'s 32-bit value.
 __device__ __forceinline__ int readlane(int v, int l) { return __builtin_amdgcn_readlane(v, l); }

@@ -266,7 +266,7 @@ __device__ __forceinline__ TexelEval texel_eval_rec(int T, const Rec& rec, f3 c,
       const f3 ts = rec.site(i);
       const f3 df{x1.x - (ts.x - c.x), x1.y - (ts.y - c.y), x1.z - (ts.z - c.z)};
       const float w = fast_exp(dot(df, df) * kexp);
-      if (w1) w1[i] = w;
+      if (w1) w1[i * TWAVE] = w;  // a lane's column of a [texel][lane] LDS array
       const float4 cl = rec.rgbh(i);
       rs = rs + f3{cl.x, cl.y, cl.z} * w;
       s1 += w;
@@ -514,7 +514,8 @@ template <int FM>
 __global__ void __launch_bounds__(TWAVE* TWPB) __attribute__((amdgpu_waves_per_eu(3)))
 trace_bwd_kernel(TraceParams p) {
   __shared__ GradCache s_gc[TWPB];
-  // per lane, the colour-stage texel weights of its visit ([texel][lane])
+  // per lane, the colour-stage texel weights of its visit ([texel][lane]),
+  // written by texel_eval
   __shared__ float s_wn_all[TWPB][PF_MAX_TEXELS * TWAVE];
   float* s_wn = s_wn_all[threadIdx.x >> 6];
   // slot b * TWPB + wave takes tile tile_order[slot], or tile slot
@@ -600,7 +601,7 @@ trace_bwd_kernel(TraceParams p) {
     TexelEval te{};
     float dt = -1.0f;
     if (live && hit && !(tn > tf) && !(sigma < 1e-3f)) {
-      te = texel_eval(p, prim, c, r, pn, o, d, tn, nullptr);
+      te = texel_eval(p, prim, c, r, pn, o, d, tn, s_wn + lane);
       if (te.dpn >= 0.0f && te.ts1 < tf) {
         tf_id = n_adj;
         tf = te.ts1;
@@ -688,10 +689,8 @@ trace_bwd_kernel(TraceParams p) {
           const float* ts = p.texel_sites + ((size_t)prim * T + k) * 3;
           const float4 cl = p.texel_rgbh[(size_t)prim * T + k];
           const f3 df{x1.x - (ts[0] - c.x), x1.y - (ts[1] - c.y), x1.z - (ts[2] - c.z)};
-          // the colour-stage weight, recomputed as texel_eval forms it, and
-          // kept for the height stage of a site whose heights are all zero
-          const float wn = fast_exp(dot(df, df) * kexp) * inv_ws;
-          s_wn[kk * TWAVE + lane] = wn;
+          // the colour-stage weight texel_eval formed (kept per lane in LDS)
+          const float wn = lds_load(s_wn + k * TWAVE + lane) * inv_ws;
           acc(3 * k + 0, dLdcol.x * wn);
           acc(3 * k + 1, dLdcol.y * wn);
           acc(3 * k + 2, dLdcol.z * wn);
@@ -736,7 +735,7 @@ trace_bwd_kernel(TraceParams p) {
           // texel-site term
           for (int kk = 0; kk < T; kk++) {
             const int k = kk + rot < T ? kk + rot : kk + rot - T;
-            acc(6 * T + k, s_wn[kk * TWAVE + lane] * dLdheight);
+            acc(6 * T + k, lds_load(s_wn + k * TWAVE + lane) * inv_ws0 * dLdheight);
           }
         }
         const float dLdtq0 = dot(dLdx0, d);
