@@ -526,9 +526,10 @@ int pf_trace_bwd(const pf_trace_bwd_args* args, void* stream);
  * counterpart: the reference walk forms them from the spheres per visit):
  * face_planes[e] = (n, pm) with n = c_a - c_s and
  * pm = (|c_a|^2 - r_a^2)/2 - (|c_s|^2 - r_s^2)/2 for the face e of site s
- * towards a = adjacency[e], in FP32 as the walk forms them. face_planes holds
- * E = adjacency_offsets[N] float4 (16-byte aligned). */
-int pf_trace_planes(const pf_scene* scene, float* face_planes, void* stream);
+ * towards a = adjacency[e], in FP32 as the walk forms them. n_faces = E, the
+ * length of scene.adjacency and the float4 capacity of face_planes (16-byte
+ * aligned); offsets beyond it (an inconsistent CSR) are not followed. */
+int pf_trace_planes(const pf_scene* scene, int64_t n_faces, float* face_planes, void* stream);
 
 /* Longest-walk-first dispatch order for pf_trace_bwd (no reference
  * counterpart: the reference tracer has no backward). tile_order [n_tiles]

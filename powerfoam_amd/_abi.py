@@ -310,7 +310,7 @@ SIGNATURES = {
     "pf_trace_fwd": ([_VP, _VP], ctypes.c_int),
     "pf_trace_bwd": ([_VP, _VP], ctypes.c_int),
     "pf_trace_tile_order": ([_VP, ctypes.c_int32, _VP, _VP], ctypes.c_int),
-    "pf_trace_planes": ([_VP, _VP, _VP], ctypes.c_int),
+    "pf_trace_planes": ([_VP, ctypes.c_int64, _VP, _VP], ctypes.c_int),
     "pf_trace_check_f64": ([_VP, _VP], ctypes.c_int),
     "pf_pack_sites": ([_VP] * 6 + [ctypes.c_int32, ctypes.c_int32] + [_VP] * 4, ctypes.c_int),
     "pf_scrub_nonfinite": ([_VP, ctypes.c_int64, _VP], ctypes.c_int),

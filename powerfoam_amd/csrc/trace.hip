@@ -945,7 +945,7 @@ __global__ void __launch_bounds__(256)
 __global__ void __launch_bounds__(256) trace_planes_kernel(const float4* __restrict__ spheres,
                                                            const int32_t* __restrict__ adjacency,
                                                            const int32_t* __restrict__ adj_off, int n_sites,
-                                                           float4* __restrict__ planes) {
+                                                           int64_t n_faces, float4* __restrict__ planes) {
 #pragma clang fp contract(off)
   __shared__ int s_off[4][WAVE + 1];
   __shared__ float4 s_sph[4][WAVE];
@@ -961,7 +961,8 @@ __global__ void __launch_bounds__(256) trace_planes_kernel(const float4* __restr
   }
   if (lane == 0) so[ns] = adj_off[s0 + ns];  // ns may be 64: one past the last lane
   wave_sync();
-  const int e0 = so[0], e1 = so[ns];
+  // offsets past the face arrays (an inconsistent CSR) end the pass there
+  const int e0 = so[0], e1 = (int)min((int64_t)so[ns], n_faces);
   for (int e = e0 + lane; e < e1; e += WAVE) {
     int l = 0;  // largest l with so[l] <= e: the site of face e
 #pragma unroll
@@ -1069,15 +1070,15 @@ int pf_trace_fwd(const pf_trace_args* a, void* stream) {
   return PF_OK;
 }
 
-int pf_trace_planes(const pf_scene* scene, float* face_planes, void* stream) {
+int pf_trace_planes(const pf_scene* scene, int64_t n_faces, float* face_planes, void* stream) {
   if (!scene) return set_error(PF_ERR_ARG, "pf_trace_planes: null scene");
-  if (scene->n_sites < 0) return set_error(PF_ERR_ARG, "pf_trace_planes: n_sites < 0");
-  if (scene->n_sites == 0) return PF_OK;
+  if (scene->n_sites < 0 || n_faces < 0) return set_error(PF_ERR_ARG, "pf_trace_planes: negative size");
+  if (scene->n_sites == 0 || n_faces == 0) return PF_OK;
   if (!scene->spheres || !scene->adjacency_offsets || !scene->adjacency || !face_planes)
     return set_error(PF_ERR_ARG, "pf_trace_planes: needs spheres, the adjacency and the output");
   trace_planes_kernel<<<(scene->n_sites + 4 * WAVE - 1) / (4 * WAVE), 256, 0, as_stream(stream)>>>(
       reinterpret_cast<const float4*>(scene->spheres), scene->adjacency, scene->adjacency_offsets, scene->n_sites,
-      reinterpret_cast<float4*>(face_planes));
+      n_faces, reinterpret_cast<float4*>(face_planes));
   PF_HIP(hipGetLastError());
   return PF_OK;
 }

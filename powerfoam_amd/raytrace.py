@@ -101,7 +101,7 @@ class RayTracer:
             planes = torch.empty((adjacency.numel(), 4), dtype=torch.float32, device=all_spheres.device)
             sc = _scene_struct(all_spheres, all_nsigmas, all_texel_sites, all_texel_rgbh, adjacency, adjacency_offsets,
                                None)
-            check(self.lib.pf_trace_planes(_ref(sc), planes.data_ptr(), stream_handle()))
+            check(self.lib.pf_trace_planes(_ref(sc), adjacency.numel(), planes.data_ptr(), stream_handle()))
         return all_spheres, all_nsigmas, all_texel_sites, all_texel_rgbh, adjacency, adjacency_offsets, diff, planes
 
     def _args(self, camera, start_point_idx, tensors, threshold, depth_quantile):

@@ -129,7 +129,9 @@ def test_trace_planes_argument_checks_run_on_the_host():
     lib = _abi.load()
     sc = _abi.PfScene()
     sc.n_sites = 0
-    assert lib.pf_trace_planes(ctypes.byref(sc), None, None) == _abi.PF_OK
+    assert lib.pf_trace_planes(ctypes.byref(sc), 5, None, None) == _abi.PF_OK
     sc.n_sites = 10
-    assert lib.pf_trace_planes(ctypes.byref(sc), None, None) == _abi.PF_ERR_ARG
-    assert lib.pf_trace_planes(None, None, None) == _abi.PF_ERR_ARG
+    assert lib.pf_trace_planes(ctypes.byref(sc), 0, None, None) == _abi.PF_OK
+    assert lib.pf_trace_planes(ctypes.byref(sc), 5, None, None) == _abi.PF_ERR_ARG
+    assert lib.pf_trace_planes(ctypes.byref(sc), -1, None, None) == _abi.PF_ERR_ARG
+    assert lib.pf_trace_planes(None, 5, None, None) == _abi.PF_ERR_ARG

@@ -308,3 +308,32 @@ def test_face_planes_walk_equals_sphere_walk(kind):
     for ga, gb in zip(res[True][1], res[False][1]):
         assert np.allclose(ga, gb, rtol=1e-4, atol=1e-5 * max(np.abs(gb).max(), 1e-30))
     assert res[True][0][-1].mean() > 3
+
+
+def test_face_planes_stop_at_the_face_count():
+    """pf_trace_planes follows no offset past n_faces: with CSR offsets that
+    claim more faces than the adjacency holds (the last site's range runs
+    past its end), the planes of the faces that exist are written as with a
+    consistent CSR and the guard words after them stay untouched. Every read
+    stays inside the arrays (ids below n_faces, offsets of real sites)."""
+    import torch
+    from powerfoam_amd import _abi
+    from powerfoam_amd.rasterize import _ref, _scene_struct
+    s = big_scene(7, n=500)
+    x = _tensors(s, "cuda")
+    tr = _tracer(8)
+    t = tr._prep(x["points"], x["radii"], x["density"], x["normals"], x["texel_sites"], x["texel_rgb"],
+                 x["texel_height"], x["adj"], x["off"], None)
+    want = t[7].clone()
+    E = int(x["adj"].numel())
+    bad_off = x["off"].clone()
+    bad_off[-1] = E + 100  # the last site claims 100 faces that do not exist
+    lib = _abi.load()
+    sc = _scene_struct(t[0], t[1], t[2], t[3], t[4], bad_off.contiguous(), None)
+    guard = 64
+    planes = torch.full((E + guard, 4), float("nan"), dtype=torch.float32, device="cuda")
+    _abi.check(lib.pf_trace_planes(_ref(sc), E, planes.data_ptr(), _abi.stream_handle()))
+    torch.cuda.synchronize()
+    got = planes.cpu().numpy()
+    assert np.array_equal(got[:E].view(np.uint32), want.cpu().numpy().view(np.uint32))
+    assert np.isnan(got[E:]).all()
