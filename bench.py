@@ -112,7 +112,8 @@ class MlpSlot:
 
 
 class MlpFused(MlpSlot):
-    """The learned head fused into the compositing forward (color_head=)."""
+    """The learned head inside the rasterizer (color_head=): in the
+    compositing forward or over the binned sites, per Rasterizer.head_mode."""
     fused = True
 
 
@@ -238,10 +239,17 @@ def mlp_head_record(args, dev, psnr_views=4):
     for i in range(args.warmup):
         frame(rast, sh_head, w, w["cams"][i % len(w["cams"])])
         frame(rast, mlp_head, w, w["cams"][i % len(w["cams"])])
-        frame(rast, fused_head, w, w["cams"][i % len(w["cams"])])
+        for mode in ("compositing", "binned"):
+            rast.head_mode = mode
+            frame(rast, fused_head, w, w["cams"][i % len(w["cams"])])
     sh_s, sh_k = time_frames(rast, sh_head, w, steps)
     mlp_s, mlp_k = time_frames(rast, mlp_head, w, steps, heads_log=mlp)
+    # the head inside the rasterizer: in the compositing forward, then once
+    # per site with a tile between binning and compositing
+    rast.head_mode = "compositing"
     fused_s, fused_k = time_frames(rast, fused_head, w, steps, heads_log=mlp)
+    rast.head_mode = "binned"
+    binned_s, binned_k = time_frames(rast, fused_head, w, steps, heads_log=mlp)
     # the backward skips 32-row steps without a colour gradient: its MFMA work
     # is over the rows of the steps it processes (views of the timed frames)
     bwd_rows, nz_rows = mlp_bwd_rows(rast, mlp, w, [w["cams"][i % len(w["cams"])] for i in range(min(steps, 8))])
@@ -258,7 +266,9 @@ def mlp_head_record(args, dev, psnr_views=4):
                mlp_ms_per_frame=mlp_s * 1e3 / steps, sh_ms_per_frame=sh_s * 1e3 / steps,
                speedup_vs_sh_head=sh_s / mlp_s, frames=steps,
                mlp_fused_ms_per_frame=fused_s * 1e3 / steps, fused_speedup_vs_sh_head=sh_s / fused_s,
-               kernel_ms_mlp_fused=fused_k, mlp_bwd_rows_nonzero=nz_rows, mlp_bwd_rows_processed=bwd_rows,
+               kernel_ms_mlp_fused=fused_k,
+               mlp_binned_ms_per_frame=binned_s * 1e3 / steps, binned_speedup_vs_sh_head=sh_s / binned_s,
+               kernel_ms_mlp_binned=binned_k, mlp_bwd_rows_nonzero=nz_rows, mlp_bwd_rows_processed=bwd_rows,
                texel_rows=rows,
                psnr_vs_sh_render_db=psnr_mlp, psnr_dc_only_vs_sh_render_db=psnr_dc, psnr_views=len(cams),
                encoding=args.mlp_encoding, psnr_vs_sh_render_db_by_encoding=psnr_by_enc,
@@ -268,7 +278,8 @@ def mlp_head_record(args, dev, psnr_views=4):
                head="16-dim bf16 latent + 16-feature direction encoding (freq: 1, d, sin/cos(pi d), "
                     "sin/cos(2 pi d); sh: 1, Y_1..Y_15) -> 64 -> 64 -> 3, bf16 MFMA, fp32 accumulate")
     log(f"learned head: MLP {rec['mlp_ms_per_frame']:.3f} ms (fused into compositing "
-        f"{rec['mlp_fused_ms_per_frame']:.3f} ms) vs SH {rec['sh_ms_per_frame']:.3f} ms per frame; "
+        f"{rec['mlp_fused_ms_per_frame']:.3f} ms, over the binned sites {rec['mlp_binned_ms_per_frame']:.3f} ms) "
+        f"vs SH {rec['sh_ms_per_frame']:.3f} ms per frame; "
         f"MFMA {({k: round(v['frac'], 4) for k, v in mfma.items()})}; kernels pre-pass {mlp_k}, fused {fused_k}, "
         f"SH {sh_k}; backward rows {bwd_rows:.0f} processed, {nz_rows:.0f} non-zero of {rows}")
     del w, rast, mlp

@@ -602,13 +602,18 @@ typedef struct pf_mlp_rows {
    * SH head's basis) or PF_MLP_ENC_FREQ (1, d, sin(pi d), cos(pi d),
    * sin(2 pi d), cos(2 pi d), per component) */
   int32_t encoding;
-  /* pf_mlp_bwd, texel mode with n_rows = n_sites * n_texels: when set, only
-   * the rows of the *site_count sites in site_list (device memory, e.g. from
-   * pf_mlp_active_sites) are processed; the other sites' rows must have a
-   * zero colour gradient (they add nothing) and keep the caller's zero
-   * latent gradient. NULL: every row. */
+  /* texel mode with n_rows = n_sites * n_texels: when set, only the rows of
+   * the *site_count sites in site_list (device memory, e.g. from
+   * pf_mlp_active_sites or pf_mask_sites) are processed. pf_mlp_fwd writes
+   * their rows only; for pf_mlp_bwd the other sites' rows must have a zero
+   * colour gradient (they add nothing) and keep the caller's zero latent
+   * gradient. NULL: every row. */
   const int32_t* site_list;
   const int32_t* site_count;
+  /* pf_mlp_fwd: floats between consecutive output rows, 0 or 3 for rgb
+   * [n_rows,3], 4 to write the colour into (rgb, height) records [n_rows,4]
+   * (the texel_rgbh layout; heights untouched). */
+  int32_t out_stride;
 } pf_mlp_rows;
 #define PF_MLP_ENC_SH 0
 #define PF_MLP_ENC_FREQ 1
@@ -649,6 +654,15 @@ typedef struct pf_mlp_head {
 
 /* rgb [n_rows,3]. */
 int pf_mlp_fwd(const pf_mlp_weights* w, const pf_mlp_rows* rows, float* rgb, void* stream);
+
+/* The sites whose mask byte is non-zero (and whose mask_and byte is, when
+ * mask_and is not NULL), appended to site_list (capacity n_sites) in an
+ * unspecified order; *site_count (device, zeroed by the caller) receives
+ * their number. No reference counterpart: it turns the binning's visibility
+ * mask (pf_bin_emit) and the live mask (pf_live_sites) into the site list of
+ * a texel-mode pf_mlp_fwd over the sites a frame's tile lists can hold. */
+int pf_mask_sites(const uint8_t* mask, const uint8_t* mask_and, int32_t n_sites, int32_t* site_list,
+                  int32_t* site_count, void* stream);
 /* Adjoint of pf_mlp_fwd for dL/d rgb = grad_rgb [n_rows,3]; no gradient to
  * directions or texel positions (as color_fn.py:181-184). Steps of 32 rows
  * whose incoming gradients are all zero are skipped (they add nothing). */

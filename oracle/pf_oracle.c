@@ -272,7 +272,7 @@ int orc_abi_layout(int64_t* out, int cap) {
       (int64_t)sizeof(pf_trace_bwd_args), (int64_t)offsetof(pf_trace_bwd_args, grad_opacity),
       (int64_t)offsetof(pf_trace_bwd_args, grad_texel_height), (int64_t)offsetof(pf_trace_bwd_args, tile_order),
       (int64_t)sizeof(pf_mlp_rows), (int64_t)offsetof(pf_mlp_rows, eye), (int64_t)offsetof(pf_mlp_rows, latents),
-      (int64_t)offsetof(pf_mlp_rows, site_count),
+      (int64_t)offsetof(pf_mlp_rows, site_count), (int64_t)offsetof(pf_mlp_rows, out_stride),
       (int64_t)sizeof(pf_mlp_grads),
       (int64_t)sizeof(pf_segments), (int64_t)offsetof(pf_segments, count), (int64_t)offsetof(pf_segments, seg_len),
       (int64_t)offsetof(pf_segments, scratch)};

@@ -194,6 +194,7 @@ class PfMlpRows(ctypes.Structure):
         ("encoding", ctypes.c_int32),
         ("site_list", ctypes.c_void_p),
         ("site_count", ctypes.c_void_p),
+        ("out_stride", ctypes.c_int32),
     ]
 
 
@@ -307,6 +308,7 @@ SIGNATURES = {
     "pf_mlp_fwd": ([_VP, _VP, _VP, _VP], ctypes.c_int),
     "pf_mlp_bwd": ([_VP, _VP, _VP, _VP, _VP], ctypes.c_int),
     "pf_mlp_active_sites": ([_VP, ctypes.c_int32, ctypes.c_int32, _VP, _VP, _VP], ctypes.c_int),
+    "pf_mask_sites": ([_VP, _VP, ctypes.c_int32, _VP, _VP, _VP], ctypes.c_int),
     "pf_trace_fwd": ([_VP, _VP], ctypes.c_int),
     "pf_trace_bwd": ([_VP, _VP], ctypes.c_int),
     "pf_trace_tile_order": ([_VP, ctypes.c_int32, _VP, _VP], ctypes.c_int),

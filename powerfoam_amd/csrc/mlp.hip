@@ -30,6 +30,7 @@ struct MlpParams {
   const uint16_t* w3;
   const float* b3;
   float* rgb;
+  int out_stride;  // floats between output rows (3, or 4 into (rgb, height) records)
   const float* grad_rgb;
   float* g_lat;
   float* g_w1;
@@ -103,8 +104,10 @@ __device__ __forceinline__ void stage_row(const MlpParams& p, int row, uint16_t*
   o[3] = make_uint4(e3.x & m, e3.y & m, e3.z & m, e3.w & m);
 }
 
-__global__ void __launch_bounds__(WAVE* ML_WPB) mlp_fwd_kernel(MlpParams p) {
+__global__ void __launch_bounds__(WAVE* ML_WPB) mlp_fwd_kernel(MlpParams pin) {
   __shared__ alignas(16) uint16_t s_x_all[ML_WPB][FW_ROWS * XS];
+  MlpParams p = pin;
+  if (p.site_list) p.n_rows = *p.site_count * p.T;
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int g = lane >> 4, c16 = lane & 15;
   uint16_t* sx = s_x_all[wv];
@@ -148,7 +151,9 @@ __global__ void __launch_bounds__(WAVE* ML_WPB) mlp_fwd_kernel(MlpParams p) {
       // lanes of group 0 hold outputs 0..2 of their row; rows naming no valid
       // site get the biases' colour of a zero input
       if (g == 0 && row < p.n_rows) {
-        float* out = p.rgb + 3 * (size_t)row;
+        size_t at = row;
+        if (p.site_list) row_site(p, row, at);  // the listed site's texel row
+        float* out = p.rgb + (size_t)p.out_stride * at;
         out[0] = fmaxf(o[0], 0.0f);
         out[1] = fmaxf(o[1], 0.0f);
         out[2] = fmaxf(o[2], 0.0f);
@@ -587,6 +592,33 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Sites with a non-zero mask byte, appended to `list` as active_sites_kernel
+// does (a wave takes ACT_CHUNKS chunks of 64 sites, one list reservation).
+__global__ void __launch_bounds__(256)
+    mask_sites_kernel(const uint8_t* __restrict__ mask, const uint8_t* __restrict__ mask_and, int n_sites,
+                      int32_t* list, int32_t* count) {
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int s_first = (blockIdx.x * 4 + wv) * WAVE * ACT_CHUNKS;
+  if (s_first >= n_sites) return;
+  uint64_t m[ACT_CHUNKS];
+  int total = 0;
+#pragma unroll
+  for (int c = 0; c < ACT_CHUNKS; c++) {
+    const int s = s_first + c * WAVE + lane;
+    m[c] = ballot(s < n_sites && mask[s] != 0 && (!mask_and || mask_and[s] != 0));
+    total += __popcll(m[c]);
+  }
+  if (total == 0) return;
+  int base = 0;
+  if (lane == 0) base = atomicAdd(count, total);
+  base = __builtin_amdgcn_readfirstlane(base);
+#pragma unroll
+  for (int c = 0; c < ACT_CHUNKS; c++) {
+    if ((m[c] >> lane) & 1) list[base + __popcll(m[c] & ((1ull << lane) - 1))] = s_first + c * WAVE + lane;
+    base += __popcll(m[c]);
+  }
+}
+
 static int make_params(const pf_mlp_weights* w, const pf_mlp_rows* rows, MlpParams& p, const char* who) {
   if (!w || !rows || !w->w1 || !w->b1 || !w->w2 || !w->b2 || !w->w3 || !w->b3)
     return set_error(PF_ERR_ARG, "%s: missing weights", who);
@@ -613,6 +645,11 @@ static int make_params(const pf_mlp_weights* w, const pf_mlp_rows* rows, MlpPara
     return set_error(PF_ERR_ARG, "%s: a site list needs texel mode, its count and n_rows = n_sites * n_texels", who);
   p.site_list = rows->site_list;
   p.site_count = rows->site_count;
+  if (rows->out_stride != 0 && rows->out_stride != 3 && rows->out_stride != 4)
+    return set_error(PF_ERR_ARG, "%s: out_stride must be 0, 3 or 4", who);
+  if (rows->out_stride == 4 && rows->pair_sites)
+    return set_error(PF_ERR_ARG, "%s: (rgb, height) output is for texel rows", who);
+  p.out_stride = rows->out_stride ? rows->out_stride : 3;
   if (rows->encoding != PF_MLP_ENC_SH && rows->encoding != PF_MLP_ENC_FREQ)
     return set_error(PF_ERR_ARG, "%s: unknown direction encoding %d", who, rows->encoding);
   p.encoding = rows->encoding;
@@ -655,7 +692,6 @@ int pf_mlp_fwd(const pf_mlp_weights* w, const pf_mlp_rows* rows, float* rgb, voi
   if (p.n_rows == 0) return PF_OK;
   if (!rgb) return set_error(PF_ERR_ARG, "pf_mlp_fwd: missing output");
   p.rgb = rgb;
-  if (p.site_list) return set_error(PF_ERR_ARG, "pf_mlp_fwd: site lists are for the backward");
   mlp_fwd_kernel<<<persistent_grid(mlp_fwd_kernel, WAVE * ML_WPB, (p.n_rows + FW_ROWS - 1) / FW_ROWS), WAVE * ML_WPB, 0, as_stream(stream)>>>(p);
   PF_HIP(hipGetLastError());
   return PF_OK;
@@ -678,6 +714,18 @@ int pf_mlp_bwd(const pf_mlp_weights* w, const pf_mlp_rows* rows, const float* gr
   p.g_w3 = g->w3;
   p.g_b3 = g->b3;
   mlp_bwd_kernel<<<persistent_grid(mlp_bwd_kernel, WAVE * BW_WPB, (p.n_rows + BW_ROWS - 1) / BW_ROWS), WAVE * BW_WPB, 0, as_stream(stream)>>>(p);
+  PF_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+int pf_mask_sites(const uint8_t* mask, const uint8_t* mask_and, int32_t n_sites, int32_t* site_list,
+                  int32_t* site_count, void* stream) {
+  if (n_sites < 0 || (n_sites > 0 && (!mask || !site_list || !site_count)))
+    return set_error(PF_ERR_ARG, "pf_mask_sites: bad arguments");
+  if (n_sites == 0) return PF_OK;
+  const int per_block = 4 * WAVE * ACT_CHUNKS;
+  mask_sites_kernel<<<(n_sites + per_block - 1) / per_block, 256, 0, as_stream(stream)>>>(mask, mask_and, n_sites,
+                                                                                         site_list, site_count);
   PF_HIP(hipGetLastError());
   return PF_OK;
 }

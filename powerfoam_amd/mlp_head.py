@@ -20,8 +20,10 @@ version counter moved (an optimiser step, an in-place write).
 camera and is differentiable w.r.t. the latents and weights (not the texel
 positions, as color_fn.py:181-184); its texel-mode backward runs over the
 sites whose texels received a colour gradient only. The rasterizer can also
-evaluate the head inside its compositing forward
-(`Rasterizer.forward(..., color_head=head)`). `fit` trains the head on the GPU
+evaluate the head itself (`Rasterizer.forward(..., color_head=head)`): once
+per site with a tile in the view, between binning and compositing
+(`Rasterizer.head_mode = "binned"`), or inside the compositing forward
+(`"compositing"`). `fit` trains the head on the GPU
 with the same kernels on (site, direction) pairs labelled by the SH head.
 """
 
@@ -144,14 +146,18 @@ class MLPColor:
         self._launch("mlp_fwd", self.lib.pf_mlp_fwd, _ref(w), _ref(r), out.data_ptr(), stream_handle())
         return out
 
-    def _run_bwd(self, a, grad_rgb):
+    def _run_bwd(self, a, grad_rgb, site_list=None):
+        """site_list: (list, count) device tensors naming every site whose
+        rows can carry a colour gradient (texel mode), or None."""
         bf = self._bf16()
         g = {k: torch.zeros_like(self.params[k]) for k in PARAM_NAMES}
         gs = _abi.PfMlpGrads(*[g[k].data_ptr() for k in PARAM_NAMES])
         w, r = self._weights(bf), self._rows(a, bf)
         gr = grad_rgb.detach().to(torch.float32).contiguous()
         n_sites = self.params["latents"].shape[0]
-        if "texel_sites" in a and self.compact_backward and a["n_rows"] == n_sites * a.get("T", 1):
+        if site_list is not None:
+            r.site_list, r.site_count = site_list[0].data_ptr(), site_list[1].data_ptr()
+        elif "texel_sites" in a and self.compact_backward and a["n_rows"] == n_sites * a.get("T", 1):
             # only the sites whose texels got a colour gradient (the rows of
             # the others add nothing): a device list, no host sync
             lst = torch.empty(max(n_sites, 1), dtype=torch.int32, device=self.device)

@@ -144,6 +144,14 @@ class Rasterizer:
         # head_rows, for inspection. Off: the rows live only until backward.
         self.keep_head_rows = False
         self.head_rows = None
+        # Where a color_head runs: "binned" (default) evaluates it once per
+        # site with a tile in this view (pf_mask_sites over the binning's
+        # visibility mask, then a texel-mode pf_mlp_fwd writing the colours
+        # into the packed rows) between binning and compositing;
+        # "compositing" evaluates it inside the compositing forward for each
+        # (tile, site) that shades (pf_raster_fwd_args.mlp_head). Same rows,
+        # same outputs.
+        self.head_mode = "binned"
 
     def _launch(self, name, fn, *a):
         if self.event_log is None:
@@ -327,6 +335,29 @@ class Rasterizer:
             raise ValueError(f"feature_dtype must be 'float' or 'bf16', not {self.feature_dtype!r}")
         return all_texel_rgbh.to(torch.bfloat16).contiguous()
 
+    def _head_binned(self, head, camera, all_texel_sites, all_texel_rgbh, visible, live):
+        """The learned head's texel colours for the sites the tile lists can
+        hold (a tile in this view and, with culling, live), written into their
+        packed (rgb, height) rows: pf_mask_sites over the visibility and live
+        masks, then pf_mlp_fwd over that site list (device-side count, no host
+        sync)."""
+        n, T = all_texel_rgbh.shape[0], all_texel_rgbh.shape[1]
+        dev = all_texel_rgbh.device
+        lst = torch.empty(max(n, 1), dtype=torch.int32, device=dev)
+        cnt = torch.zeros(1, dtype=torch.int32, device=dev)
+        st = stream_handle()
+        check(self.lib.pf_mask_sites(ptr(visible.view(torch.uint8)), ptr(live) if live is not None else None, n,
+                                     ptr(lst), ptr(cnt), st))
+        bf = head._bf16()
+        eye = torch.as_tensor(camera.eye).detach().to("cpu", torch.float32).reshape(-1)
+        r = head._rows(dict(n_rows=n * T, T=T, texel_sites=all_texel_sites, eye=tuple(float(v) for v in eye)), bf)
+        r.site_list, r.site_count, r.out_stride = ptr(lst), ptr(cnt), 4
+        w = head._weights(bf)
+        head._launch("mlp_fwd", self.lib.pf_mlp_fwd, _ref(w), _ref(r), ptr(all_texel_rgbh), st)
+        # every site that can get a colour gradient is on the list: the
+        # head's backward reuses it instead of scanning the gradient
+        return lst, cnt
+
     def pack_sites(self, points, radii, normals, density, texel_rgb, texel_height):
         """(spheres [N,4], nsigmas [N,4], texel_rgbh [N,T,4]) in FP32, the
         concatenations of rasterize.py:1619-1624, in one launch (pf_pack_sites).
@@ -406,6 +437,17 @@ class Rasterizer:
             checks=[(adjacency_offsets[-1], adjacency.shape[0], "adjacency_offsets[-1] must equal len(adjacency)")])
         sg = self.segments(offsets, tile_sites.shape[0], self._longest)
         order = self.tile_order(offsets) if sg is None else None
+        if color_head is not None and self.head_mode not in ("binned", "compositing"):
+            raise ValueError(f"head_mode must be 'binned' or 'compositing', not {self.head_mode!r}")
+        head_binned = color_head is not None and self.head_mode == "binned"
+        head_sites = None
+        if head_binned:
+            head_sites = self._head_binned(color_head, camera, all_texel_sites, all_texel_rgbh, prim_visible_mask,
+                                           live)
+            if rgbh_bf16 is not None:
+                # bf16 storage: the heights as stored, the head's colours in FP32
+                all_texel_rgbh[..., 3] = rgbh_bf16[..., 3].to(torch.float32)
+                rgbh_bf16 = None
         # the compositing's faces: each site's faces that can bound a ray first
         face_count = None
         if self.prune_faces:
@@ -452,7 +494,7 @@ class Rasterizer:
         a.point_err = point_err_out.data_ptr() if point_err_out is not None else None
         a.entry_shades = entry_shades.data_ptr() if entry_shades is not None else None
         head_bf16 = None
-        if color_head is not None:
+        if color_head is not None and not head_binned:
             # the head's colours are computed in the compositing and written,
             # with the heights, into the packed rows the backward reads
             head_bf16 = color_head._bf16()
@@ -476,6 +518,7 @@ class Rasterizer:
         # with the head, the backward reads the colours the forward wrote (FP32)
         grad_ctx.rgbh_bf16 = rgbh_bf16 if color_head is None else None
         grad_ctx.color_head = color_head
+        grad_ctx.head_sites = head_sites
         grad_ctx.face_count = face_count
         grad_ctx.save_for_backward(dq, all_spheres, all_nsigmas, all_texel_sites, all_texel_rgbh, adjacency,
                                    adjacency_offsets, adjacency_diff, tile_sites, offsets, tile_early_stop,
@@ -649,7 +692,8 @@ class Rasterizer:
         check(self.lib.pf_scrub_nonfinite(ptr(grad_buf), n_grads, st))
         head = grad_ctx.color_head
         del grad_ctx.rasterizer, grad_ctx.camera, grad_ctx.ray_maps, grad_ctx.order, grad_ctx.segs, grad_ctx.rgbh_bf16
-        del grad_ctx.color_head, grad_ctx.face_count
+        head_sites = grad_ctx.head_sites
+        del grad_ctx.color_head, grad_ctx.face_count, grad_ctx.head_sites
         grads = (None, None, None, spheres_grad[:, :3], spheres_grad[:, 3], nsigmas_grad[:, 3],
                  nsigmas_grad[:, :3], texel_sites_grad, texel_rgb_grad, texel_height_grad, None, None, None, None)
         if head is None:
@@ -659,7 +703,7 @@ class Rasterizer:
         # through the network once)
         eye = torch.as_tensor(camera.eye).detach().to("cpu", torch.float32).reshape(-1)
         rows = dict(n_rows=N * T, T=T, texel_sites=all_texel_sites, eye=tuple(float(v) for v in eye))
-        head_grads = head._run_bwd(rows, texel_rgb_grad.view(N * T, 3))
+        head_grads = head._run_bwd(rows, texel_rgb_grad.view(N * T, 3), site_list=head_sites)
         return grads[:8] + (None,) + grads[9:] + (None, None) + tuple(head_grads)
 
 

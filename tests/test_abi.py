@@ -71,7 +71,7 @@ def test_struct_layout_matches_c():
               ctypes.sizeof(V), V.bkgd_color.offset, V.intersections.offset,
               ctypes.sizeof(TA), TA.max_steps.offset, TA.n_cells.offset, TA.tile_steps.offset, TA.face_planes.offset,
               ctypes.sizeof(TB), TB.grad_opacity.offset, TB.grad_texel_height.offset, TB.tile_order.offset,
-              ctypes.sizeof(MR), MR.eye.offset, MR.latents.offset, MR.site_count.offset,
+              ctypes.sizeof(MR), MR.eye.offset, MR.latents.offset, MR.site_count.offset, MR.out_stride.offset,
               ctypes.sizeof(_abi.PfMlpGrads),
               ctypes.sizeof(SG), SG.count.offset, SG.seg_len.offset, SG.scratch.offset]
     assert n == len(expect)

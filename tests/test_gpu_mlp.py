@@ -93,6 +93,62 @@ def test_texel_backward_over_listed_sites_equals_every_row():
     assert max(errs.values()) < 1e-3, errs
 
 
+def test_forward_over_masked_sites_into_packed_rows():
+    """pf_mask_sites lists exactly the sites of non-zero mask bytes, alone and
+    with a second mask both must pass (an empty and a full mask included, the
+    last 64-site chunk partial), and a texel-mode
+    pf_mlp_fwd over that list with out_stride 4 writes, into (rgb, height)
+    records, the rows pf_mlp_fwd gives over every row, bit for bit, and
+    leaves the heights and the unlisted sites' records untouched."""
+    import torch
+    from powerfoam_amd import _abi
+    from powerfoam_amd.camera import look_at
+    from powerfoam_amd.rasterize import _ref
+    lib = _abi.load()
+    n, T = 3000, 8
+    s = synth.scene_uniform(14, n=n, n_texels=T)
+    cam = look_at((0.3, 0.2, 2.5), (0.0, 0.0, 0.0), 64, 48, 60.0)
+    ts = torch.tensor(s["texel_sites"], device="cuda")
+    rng = np.random.default_rng(8)
+    for frac in (0.0, 0.3, 1.0):
+        mask = (rng.uniform(size=n) < frac).astype(np.uint8)
+        print(f"mask sites frac={frac}", flush=True)
+        mask2 = (rng.uniform(size=n) < 0.5).astype(np.uint8)
+        for second in (None, mask2):
+            lst = torch.full((n,), -1, dtype=torch.int32, device="cuda")
+            cnt = torch.zeros(1, dtype=torch.int32, device="cuda")
+            m2 = torch.tensor(second, device="cuda") if second is not None else None
+            _abi.check(lib.pf_mask_sites(torch.tensor(mask, device="cuda").data_ptr(),
+                                         m2.data_ptr() if m2 is not None else None, n, lst.data_ptr(), cnt.data_ptr(),
+                                         _abi.stream_handle()))
+            want = np.nonzero(mask if second is None else mask & second)[0]
+            c = int(cnt.item())
+            assert c == len(want)
+            assert np.array_equal(np.sort(lst[:c].cpu().numpy()), want)
+    head = _head(n, 15, "freq")
+    with torch.no_grad():
+        full = head.forward(ts, cam).cpu().numpy()  # every row
+    rows = torch.full((n, T, 4), 7.0, dtype=torch.float32, device="cuda")
+    rows[..., 3] = torch.tensor(rng.normal(size=(n, T)), dtype=torch.float32, device="cuda")
+    h_before = rows[..., 3].cpu().numpy()
+    mask = (rng.uniform(size=n) < 0.4).astype(np.uint8)
+    lst = torch.empty(n, dtype=torch.int32, device="cuda")
+    cnt = torch.zeros(1, dtype=torch.int32, device="cuda")
+    _abi.check(lib.pf_mask_sites(torch.tensor(mask, device="cuda").data_ptr(), None, n, lst.data_ptr(), cnt.data_ptr(),
+                                 _abi.stream_handle()))
+    bf = head._bf16()
+    eye = tuple(float(v) for v in cam.eye.reshape(-1))
+    r = head._rows(dict(n_rows=n * T, T=T, texel_sites=ts, eye=eye), bf)
+    r.site_list, r.site_count, r.out_stride = lst.data_ptr(), cnt.data_ptr(), 4
+    print("mlp forward over listed sites", flush=True)
+    _abi.check(lib.pf_mlp_fwd(_ref(head._weights(bf)), _ref(r), rows.data_ptr(), _abi.stream_handle()))
+    got = rows.cpu().numpy()
+    on = mask.astype(bool)
+    assert np.array_equal(got[on][..., :3], full[on])
+    assert np.all(got[~on][..., :3] == 7.0)
+    assert np.array_equal(got[..., 3], h_before)
+
+
 @pytest.mark.parametrize("encoding", ["sh", "freq"])
 def test_pair_forward_and_backward_match_oracle(encoding):
     import torch

@@ -1,5 +1,7 @@
-"""The learned colour head fused into the compositing forward
-(Rasterizer.forward(..., color_head=head)) against the same head run as a
+"""The learned colour head inside the rasterizer
+(Rasterizer.forward(..., color_head=head)), evaluated in the compositing
+forward (head_mode "compositing") or once per site with a tile between
+binning and compositing (head_mode "binned"), against the same head run as a
 colour pre-pass (MLPColor.forward -> texel_rgb -> Rasterizer.forward), and
 against the oracle.
 
@@ -44,7 +46,7 @@ def _head(n, seed, encoding):
     return head
 
 
-def _run(case, head, fused, gin, feature_dtype="float", seg_len=None, heights=None):
+def _run(case, head, fused, gin, feature_dtype="float", seg_len=None, heights=None, mode="binned"):
     """Forward + backward with the head fused or as a pre-pass; returns
     (outputs, geometry gradients, head gradients, packed rows or None)."""
     import torch
@@ -55,6 +57,7 @@ def _run(case, head, fused, gin, feature_dtype="float", seg_len=None, heights=No
     r = Rasterizer(harness._args(T, case.cam.ray_maps is None), dev)
     r.feature_dtype = feature_dtype
     r.keep_head_rows = True
+    r.head_mode = mode
     if seg_len is not None:
         r.seg_len, r.split_min = seg_len, 0
     tt = lambda a, dt=torch.float32: torch.tensor(np.ascontiguousarray(a), dtype=dt, device=dev)
@@ -105,14 +108,18 @@ CASES = {
 }
 
 
+MODES = ["binned", "compositing"]
+
+
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("encoding", ["freq", "sh"])
 @pytest.mark.parametrize("camera", sorted(CASES))
-def test_fused_head_equals_prepass_head(camera, encoding):
+def test_fused_head_equals_prepass_head(camera, encoding, mode):
     case = harness.small_case(**CASES[camera])
     gin = harness.random_grads(case, seed=3)
-    print(f"fused head {camera} {encoding}: {case.n} sites {case.width}x{case.height}", flush=True)
+    print(f"fused head {camera} {encoding} {mode}: {case.n} sites {case.width}x{case.height}", flush=True)
     head = _head(case.n, 7, encoding)
-    out_f, geo_f, hg_f, rows = _run(case, head, True, gin)
+    out_f, geo_f, hg_f, rows = _run(case, head, True, gin, mode=mode)
     out_p, geo_p, hg_p, _ = _run(case, head, False, gin)
     _same_outputs(out_f, out_p)
     # rows of every site that shades: the pre-pass head's, bit for bit
@@ -132,13 +139,14 @@ def test_fused_head_equals_prepass_head(camera, encoding):
     assert max(he.values()) < 1e-5, he
 
 
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("encoding", ["freq", "sh"])
-def test_fused_head_matches_oracle(encoding):
+def test_fused_head_matches_oracle(encoding, mode):
     case = harness.small_case(seed=8, n=1500, width=80, height=56, height_std=0.02)
     gin = harness.random_grads(case, seed=4)
     head = _head(case.n, 9, encoding)
-    print(f"fused head vs oracle {encoding}", flush=True)
-    out, _, hg, _ = _run(case, head, True, gin)
+    print(f"fused head vs oracle {encoding} {mode}", flush=True)
+    out, _, hg, _ = _run(case, head, True, gin, mode=mode)
     p = _np_params(head)
     T = case.texel_sites.shape[1]
     sites, dirs = om.rows_texels(case.texel_sites, case.cam.eye.numpy(), T)
@@ -154,7 +162,8 @@ def test_fused_head_matches_oracle(encoding):
     assert max(errs.values()) < 1e-3, errs
 
 
-def test_fused_head_with_bf16_heights_and_split_lists():
+@pytest.mark.parametrize("mode", MODES)
+def test_fused_head_with_bf16_heights_and_split_lists(mode):
     """BF16 feature storage: the fused head reads the heights from the bf16
     copy and keeps its colours in FP32, so it equals the pre-pass path run on
     bf16-rounded heights in FP32 (bit for bit). Split lists: the segment
@@ -165,12 +174,12 @@ def test_fused_head_with_bf16_heights_and_split_lists():
     gin = harness.random_grads(case, seed=5)
     head = _head(case.n, 11, "freq")
     print("fused head bf16 heights", flush=True)
-    out_f, _, _, _ = _run(case, head, True, gin, feature_dtype="bf16")
+    out_f, _, _, _ = _run(case, head, True, gin, feature_dtype="bf16", mode=mode)
     hb = torch.tensor(case.texel_height).to(torch.bfloat16).to(torch.float32).numpy()
     out_p, _, _, _ = _run(case, head, False, gin, heights=hb)
     _same_outputs(out_f, out_p)
     print("fused head split lists", flush=True)
-    out_f, geo_f, hg_f, _ = _run(case, head, True, gin, seg_len=64)
+    out_f, geo_f, hg_f, _ = _run(case, head, True, gin, seg_len=64, mode=mode)
     out_p, geo_p, hg_p, _ = _run(case, head, False, gin, seg_len=64)
     for k in ("color", "opacity", "normal", "normal_distance", "contrib"):
         assert harness.rel_err(out_f[k], out_p[k]) < 1e-6, k
