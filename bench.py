@@ -736,11 +736,21 @@ def main():
                 kernel_ms=kern_ms, roofline=roofline, cpu_baseline=cpu)
     del w, rast
     torch.cuda.empty_cache()
+    # the extra records never cost the headline line: an error in one is
+    # recorded in its place
     if not args.no_trace:
-        line["trace"] = trace_record(args, dev, args.trace_steps, 2)
+        try:
+            line["trace"] = trace_record(args, dev, args.trace_steps, 2)
+        except Exception as e:  # noqa: BLE001
+            log(f"ray-tracer record failed: {e!r}")
+            line["trace"] = dict(error=repr(e))
         torch.cuda.empty_cache()
     if not args.no_mlp_head:
-        line["mlp_head"], _ = mlp_head_record(args, dev)
+        try:
+            line["mlp_head"], _ = mlp_head_record(args, dev)
+        except Exception as e:  # noqa: BLE001
+            log(f"learned-head record failed: {e!r}")
+            line["mlp_head"] = dict(error=repr(e))
     print(json.dumps(line), flush=True)
 
 
