@@ -354,8 +354,21 @@ class Rasterizer:
         r.site_list, r.site_count, r.out_stride = ptr(lst), ptr(cnt), 4
         w = head._weights(bf)
         head._launch("mlp_fwd", self.lib.pf_mlp_fwd, _ref(w), _ref(r), ptr(all_texel_rgbh), st)
-        # every site that can get a colour gradient is on the list: the
-        # head's backward reuses it instead of scanning the gradient
+        return lst, cnt
+
+    def _head_shading_sites(self, head_sites, contrib):
+        """The list for the head's backward, in the buffers of the forward's
+        list: the sites with contrib > 0. A texel row's colour gradient is a
+        sum over rays of alpha*T times the pixel gradient, and contrib[site]
+        sums alpha*T/(H*W) over the same rays, so a site off this list has a
+        zero row gradient -- unless every alpha*T it met was below 1.5e-39*H*W
+        (FP32 underflow of contrib), a gradient under 2e-33 times the pixel
+        gradient that the backward then leaves out. The sites binned but hidden
+        behind others (about half of those on the forward's list) drop out."""
+        lst, cnt = head_sites
+        cnt.zero_()
+        shades = (contrib != 0).view(torch.uint8)
+        check(self.lib.pf_mask_sites(ptr(shades), None, contrib.shape[0], ptr(lst), ptr(cnt), stream_handle()))
         return lst, cnt
 
     def pack_sites(self, points, radii, normals, density, texel_rgb, texel_height):
@@ -506,6 +519,8 @@ class Rasterizer:
             a.mlp_head = ctypes.addressof(hs)
         self._launch("raster_fwd", self.lib.pf_raster_fwd, _ref(a), stream_handle())
         del head_bf16
+        if head_binned:
+            head_sites = self._head_shading_sites(head_sites, contrib_out)
         self.head_rows = all_texel_rgbh if (color_head is not None and self.keep_head_rows) else None
         self.last_tiles = (offsets, tile_sites, tile_early_stop)
 
