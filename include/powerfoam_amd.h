@@ -669,7 +669,7 @@ int pf_mask_sites(const uint8_t* mask, const uint8_t* mask_and, int32_t n_sites,
 int pf_mlp_bwd(const pf_mlp_weights* w, const pf_mlp_rows* rows, const float* grad_rgb, pf_mlp_grads* g,
                void* stream);
 /* Appends to site_list [n_sites] each site with a non-zero value among its
- * n_texels x 3 colour gradients (grad_rgb [n_sites, n_texels, 3]) and adds
+ * n_texels (1..8) x 3 colour gradients (grad_rgb [n_sites, n_texels, 3]) and adds
  * their number to *site_count (zeroed by the caller); the list's order is
  * arbitrary. The texel rows the rasterizer's backward leaves without a
  * gradient (sites no ray shades) are most rows of a view. */

@@ -548,20 +548,42 @@ __global__ void __launch_bounds__(WAVE* BW_WPB) mlp_bwd_kernel(MlpParams pin) {
   }
 }
 
+// Slots in a site list for the four waves of a workgroup, `total` each, with
+// one atomic on the list's count per workgroup: returns the wave's first
+// slot. Every thread of the workgroup calls it. (An atomic per wave, ~5900
+// on one address for 3M sites, serialised to ~70 us.)
+__device__ __forceinline__ int reserve_slots(int total, int wv, int lane, int32_t* count) {
+  __shared__ int s_cnt[4];
+  __shared__ int s_base;
+  if (lane == 0) s_cnt[wv] = total;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int sum = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+    s_base = sum ? atomicAdd(count, sum) : 0;
+  }
+  __syncthreads();
+  int base = s_base;
+  for (int w = 0; w < wv; w++) base += s_cnt[w];
+  return base;
+}
+
 // Sites with a non-zero colour gradient on some texel, appended to `list`
 // (the order within the list is arbitrary). A wave takes ACT_CHUNKS chunks
 // of 64 consecutive sites: the chunk's 64 * 3T gradient words are read
-// coalesced, a non-zero word flags its site in LDS, and the wave reserves the
-// list slots of all its chunks with one atomic (a per-thread scan of a
-// site's 96 bytes and an atomic per wave had run at ~5 % of HBM bandwidth).
-constexpr int ACT_CHUNKS = 8;
+// coalesced (16-byte pieces, all issued before any is tested, when the
+// array is 16-byte aligned), a non-zero word flags its site in LDS, and the
+// workgroup reserves the list slots of its 4 * ACT_CHUNKS chunks with one
+// atomic (a per-thread scan of a site's 96 bytes had run at ~5 % of HBM
+// bandwidth, one word per lane and step at ~16 %).
+constexpr int ACT_CHUNKS = 16;
+constexpr int ACT_VEC = 3 * PF_MAX_TEXELS / 4;  // 16-byte pieces per lane of a full chunk (3T <= 24 words a site)
 
 __global__ void __launch_bounds__(256)
-    active_sites_kernel(const float* __restrict__ grad_rgb, int n_sites, int T, int32_t* list, int32_t* count) {
+    active_sites_kernel(const float* __restrict__ grad_rgb, int n_sites, int T, bool vec, int32_t* list,
+                        int32_t* count) {
   __shared__ int s_flag[4][WAVE];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int s_first = (blockIdx.x * 4 + wv) * WAVE * ACT_CHUNKS;
-  if (s_first >= n_sites) return;
   int* flag = s_flag[wv];
   const int n3 = 3 * T;
   uint64_t m[ACT_CHUNKS];
@@ -574,17 +596,42 @@ __global__ void __launch_bounds__(256)
     wave_sync();
     const float* g = grad_rgb + (size_t)s0 * n3;
     const int nd = ns * n3;
-    for (int q = lane; q < nd; q += WAVE)
-      if (g[q] != 0.0f) flag[q / n3] = 1;
+    if (vec) {
+      // piece q covers words 4q .. 4q + 3 of the chunk, of at most two sites
+      // (3T >= 3); a chunk starts 64 * 3T words (a multiple of 4) after the
+      // array's 16-byte aligned start
+      float4 v[ACT_VEC];
+#pragma unroll
+      for (int k = 0; k < ACT_VEC; k++) {
+        const int q = lane + WAVE * k;
+        v[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (4 * q + 4 <= nd) {
+          v[k] = reinterpret_cast<const float4*>(g)[q];
+        } else if (4 * q < nd) {  // the array's last, partial piece
+          v[k].x = g[4 * q];
+          if (4 * q + 1 < nd) v[k].y = g[4 * q + 1];
+          if (4 * q + 2 < nd) v[k].z = g[4 * q + 2];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < ACT_VEC; k++) {
+        const int w0 = 4 * (lane + WAVE * k);
+        const int site = w0 / n3, rem = w0 - site * n3;
+        const float c4[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+          if (c4[j] != 0.0f) flag[site + (rem + j >= n3 ? 1 : 0)] = 1;
+      }
+    } else {
+      for (int q = lane; q < nd; q += WAVE)
+        if (g[q] != 0.0f) flag[q / n3] = 1;
+    }
     wave_sync();
     m[c] = ballot(lane < ns && flag[lane] != 0);
     total += __popcll(m[c]);
     wave_sync();  // the flags are read before the next chunk clears them
   }
-  if (total == 0) return;
-  int base = 0;
-  if (lane == 0) base = atomicAdd(count, total);
-  base = __builtin_amdgcn_readfirstlane(base);
+  int base = reserve_slots(total, wv, lane, count);
 #pragma unroll
   for (int c = 0; c < ACT_CHUNKS; c++) {
     if ((m[c] >> lane) & 1) list[base + __popcll(m[c] & ((1ull << lane) - 1))] = s_first + c * WAVE + lane;
@@ -593,27 +640,26 @@ __global__ void __launch_bounds__(256)
 }
 
 // Sites with a non-zero mask byte, appended to `list` as active_sites_kernel
-// does (a wave takes ACT_CHUNKS chunks of 64 sites, one list reservation).
+// does (a wave takes MASK_CHUNKS chunks of 64 sites, one reservation per
+// workgroup).
+constexpr int MASK_CHUNKS = 32;
+
 __global__ void __launch_bounds__(256)
     mask_sites_kernel(const uint8_t* __restrict__ mask, const uint8_t* __restrict__ mask_and, int n_sites,
                       int32_t* list, int32_t* count) {
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int s_first = (blockIdx.x * 4 + wv) * WAVE * ACT_CHUNKS;
-  if (s_first >= n_sites) return;
-  uint64_t m[ACT_CHUNKS];
+  const int s_first = (blockIdx.x * 4 + wv) * WAVE * MASK_CHUNKS;
+  uint64_t m[MASK_CHUNKS];
   int total = 0;
 #pragma unroll
-  for (int c = 0; c < ACT_CHUNKS; c++) {
+  for (int c = 0; c < MASK_CHUNKS; c++) {
     const int s = s_first + c * WAVE + lane;
     m[c] = ballot(s < n_sites && mask[s] != 0 && (!mask_and || mask_and[s] != 0));
     total += __popcll(m[c]);
   }
-  if (total == 0) return;
-  int base = 0;
-  if (lane == 0) base = atomicAdd(count, total);
-  base = __builtin_amdgcn_readfirstlane(base);
+  int base = reserve_slots(total, wv, lane, count);
 #pragma unroll
-  for (int c = 0; c < ACT_CHUNKS; c++) {
+  for (int c = 0; c < MASK_CHUNKS; c++) {
     if ((m[c] >> lane) & 1) list[base + __popcll(m[c] & ((1ull << lane) - 1))] = s_first + c * WAVE + lane;
     base += __popcll(m[c]);
   }
@@ -723,8 +769,9 @@ int pf_mask_sites(const uint8_t* mask, const uint8_t* mask_and, int32_t n_sites,
   if (n_sites < 0 || (n_sites > 0 && (!mask || !site_list || !site_count)))
     return set_error(PF_ERR_ARG, "pf_mask_sites: bad arguments");
   if (n_sites == 0) return PF_OK;
-  const int per_block = 4 * WAVE * ACT_CHUNKS;
-  mask_sites_kernel<<<(n_sites + per_block - 1) / per_block, 256, 0, as_stream(stream)>>>(mask, mask_and, n_sites,
+  const int per_block = 4 * WAVE * MASK_CHUNKS;
+  const int grid = (int)(((int64_t)n_sites + per_block - 1) / per_block);
+  mask_sites_kernel<<<grid, 256, 0, as_stream(stream)>>>(mask, mask_and, n_sites,
                                                                                          site_list, site_count);
   PF_HIP(hipGetLastError());
   return PF_OK;
@@ -732,12 +779,14 @@ int pf_mask_sites(const uint8_t* mask, const uint8_t* mask_and, int32_t n_sites,
 
 int pf_mlp_active_sites(const float* grad_rgb, int32_t n_sites, int32_t n_texels, int32_t* site_list,
                         int32_t* site_count, void* stream) {
-  if (n_sites < 0 || n_texels < 1 || (n_sites > 0 && (!grad_rgb || !site_list || !site_count)))
+  if (n_sites < 0 || n_texels < 1 || n_texels > PF_MAX_TEXELS ||
+      (n_sites > 0 && (!grad_rgb || !site_list || !site_count)))
     return set_error(PF_ERR_ARG, "pf_mlp_active_sites: bad arguments");
   if (n_sites == 0) return PF_OK;
   const int per_block = 4 * WAVE * ACT_CHUNKS;
-  active_sites_kernel<<<(n_sites + per_block - 1) / per_block, 256, 0, as_stream(stream)>>>(
-      grad_rgb, n_sites, n_texels, site_list, site_count);
+  const int grid = (int)(((int64_t)n_sites + per_block - 1) / per_block);
+  active_sites_kernel<<<grid, 256, 0, as_stream(stream)>>>(
+      grad_rgb, n_sites, n_texels, (reinterpret_cast<uintptr_t>(grad_rgb) & 15) == 0, site_list, site_count);
   PF_HIP(hipGetLastError());
   return PF_OK;
 }

@@ -135,3 +135,19 @@ def test_trace_planes_argument_checks_run_on_the_host():
     assert lib.pf_trace_planes(ctypes.byref(sc), 5, None, None) == _abi.PF_ERR_ARG
     assert lib.pf_trace_planes(ctypes.byref(sc), -1, None, None) == _abi.PF_ERR_ARG
     assert lib.pf_trace_planes(None, 5, None, None) == _abi.PF_ERR_ARG
+
+
+def test_site_list_argument_checks_run_on_the_host():
+    """pf_mlp_active_sites takes 1..8 texels per site and pf_mask_sites needs
+    its mask and outputs; no sites is not an error (no device work either
+    way)."""
+    if not os.path.exists(_abi.LIB_PATH):
+        pytest.skip("libpfhip.so not built")
+    lib = _abi.load()
+    for t in (0, 9):
+        assert lib.pf_mlp_active_sites(1, 10, t, 1, 1, None) == _abi.PF_ERR_ARG
+    assert lib.pf_mlp_active_sites(None, 10, 8, 1, 1, None) == _abi.PF_ERR_ARG
+    assert lib.pf_mlp_active_sites(None, 0, 8, None, None, None) == _abi.PF_OK
+    assert lib.pf_mask_sites(None, None, 10, 1, 1, None) == _abi.PF_ERR_ARG
+    assert lib.pf_mask_sites(1, None, -1, 1, 1, None) == _abi.PF_ERR_ARG
+    assert lib.pf_mask_sites(None, None, 0, None, None, None) == _abi.PF_OK

@@ -93,10 +93,39 @@ def test_texel_backward_over_listed_sites_equals_every_row():
     assert max(errs.values()) < 1e-3, errs
 
 
+def test_active_sites_lists_exactly_the_sites_with_a_gradient():
+    """pf_mlp_active_sites against numpy: 1, 3 and 8 texels per site (the
+    last 16-byte piece of the array partial for 1 and 3), one and several
+    workgroups, an empty and a full gradient, and the array 16-byte aligned
+    or not (the word-by-word scan)."""
+    import torch
+    from powerfoam_amd import _abi
+    lib = _abi.load()
+    rng = np.random.default_rng(9)
+    for n, T, frac, offset in ((3001, 1, 0.3, 0), (70001, 3, 0.2, 0), (70001, 8, 0.05, 0), (5003, 8, 0.0, 0),
+                               (5003, 8, 1.0, 0), (70001, 3, 0.2, 1), (5003, 8, 0.3, 1)):
+        print(f"active sites n={n} T={T} frac={frac} offset={offset}", flush=True)
+        g = np.zeros((n, T, 3), np.float32)
+        on = rng.uniform(size=n) < frac
+        # one non-zero word at a random place among the site's 3T
+        k = rng.integers(0, 3 * T, size=n)
+        g.reshape(n, -1)[np.nonzero(on)[0], k[on]] = rng.normal(size=on.sum()).astype(np.float32) + 2.0
+        buf = torch.zeros(n * T * 3 + offset, dtype=torch.float32, device="cuda")
+        buf[offset:] = torch.tensor(g.reshape(-1), device="cuda")
+        lst = torch.full((n,), -1, dtype=torch.int32, device="cuda")
+        cnt = torch.zeros(1, dtype=torch.int32, device="cuda")
+        _abi.check(lib.pf_mlp_active_sites(buf[offset:].data_ptr(), n, T, lst.data_ptr(), cnt.data_ptr(),
+                                           _abi.stream_handle()))
+        c = int(cnt.item())
+        want = np.nonzero(on)[0]
+        assert c == len(want)
+        assert np.array_equal(np.sort(lst[:c].cpu().numpy()), want)
+
+
 def test_forward_over_masked_sites_into_packed_rows():
     """pf_mask_sites lists exactly the sites of non-zero mask bytes, alone and
     with a second mask both must pass (an empty and a full mask included, the
-    last 64-site chunk partial), and a texel-mode
+    last 64-site chunk partial, one and several workgroups), and a texel-mode
     pf_mlp_fwd over that list with out_stride 4 writes, into (rgb, height)
     records, the rows pf_mlp_fwd gives over every row, bit for bit, and
     leaves the heights and the unlisted sites' records untouched."""
@@ -110,17 +139,19 @@ def test_forward_over_masked_sites_into_packed_rows():
     cam = look_at((0.3, 0.2, 2.5), (0.0, 0.0, 0.0), 64, 48, 60.0)
     ts = torch.tensor(s["texel_sites"], device="cuda")
     rng = np.random.default_rng(8)
-    for frac in (0.0, 0.3, 1.0):
-        mask = (rng.uniform(size=n) < frac).astype(np.uint8)
-        print(f"mask sites frac={frac}", flush=True)
-        mask2 = (rng.uniform(size=n) < 0.5).astype(np.uint8)
+    # 3000 sites: one workgroup, waves past the end; 50001: 7 workgroups
+    # reserving their slots in turn, the last one partial
+    for n_m, frac in ((3000, 0.0), (3000, 0.3), (3000, 1.0), (50001, 0.3), (50001, 1.0)):
+        mask = (rng.uniform(size=n_m) < frac).astype(np.uint8)
+        print(f"mask sites n={n_m} frac={frac}", flush=True)
+        mask2 = (rng.uniform(size=n_m) < 0.5).astype(np.uint8)
         for second in (None, mask2):
-            lst = torch.full((n,), -1, dtype=torch.int32, device="cuda")
+            lst = torch.full((n_m,), -1, dtype=torch.int32, device="cuda")
             cnt = torch.zeros(1, dtype=torch.int32, device="cuda")
             m2 = torch.tensor(second, device="cuda") if second is not None else None
             _abi.check(lib.pf_mask_sites(torch.tensor(mask, device="cuda").data_ptr(),
-                                         m2.data_ptr() if m2 is not None else None, n, lst.data_ptr(), cnt.data_ptr(),
-                                         _abi.stream_handle()))
+                                         m2.data_ptr() if m2 is not None else None, n_m, lst.data_ptr(),
+                                         cnt.data_ptr(), _abi.stream_handle()))
             want = np.nonzero(mask if second is None else mask & second)[0]
             c = int(cnt.item())
             assert c == len(want)
